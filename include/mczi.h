@@ -1,0 +1,266 @@
+/*
+ * mczi.h — C ABI of the MI355X-native MCZI k-mer counting / subtraction path.
+ *
+ * Every entry point returns 0 on success and a nonzero code on error; the
+ * error text (an anyhow-style context chain, outermost context first) is
+ * available from mczi_last_error() on the calling thread. No C++ exception
+ * crosses this boundary. Memory returned through an out-pointer belongs to
+ * the caller and is released with mczi_free() (or the named *_free call).
+ *
+ * Reference citations are relative to the reference checkout
+ * (src/lib.rs, src/zi.rs, src/main.rs, vendor/simd-minimizers-2.3.1/src/...).
+ * The device work (scan, partitioning, counting, filters, ZI lookups) runs as
+ * HIP kernels on the current device; calls made from several host threads are
+ * serialized per device by the library.
+ */
+#ifndef MCZI_H
+#define MCZI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes -------------------------------------------------------- */
+enum {
+    MCZI_OK = 0,
+    MCZI_E_INVALID = 1,  /* configuration or argument rejected */
+    MCZI_E_IO = 2,       /* file could not be read or written */
+    MCZI_E_DEVICE = 3,   /* HIP runtime / kernel failure */
+    MCZI_E_NOMEM = 4,    /* host or device memory exhausted, or over budget */
+    MCZI_E_NOTFOUND = 5, /* unknown request / handle */
+    MCZI_E_PENDING = 6   /* request not finished yet (non-blocking poll) */
+};
+
+/* CounterConfig (src/lib.rs:48-53). k-mers are stored 2 bits per base,
+ * A0 C1 G2 T3, first base most significant, in (hi:lo) 128 bits. */
+typedef struct {
+    uint32_t k;
+    uint32_t minimizer;
+    uint64_t threshold;
+    uint32_t partition_count;
+} mczi_counter_config;
+
+/* CountedKmer (src/lib.rs:63-66): encoded = (hi << 64) | lo. */
+typedef struct {
+    uint64_t lo;
+    uint64_t hi;
+    uint64_t count;
+} mczi_counted_kmer;
+
+/* KmerFastaWriteStats (src/lib.rs:103-108), without the per-phase vector. */
+typedef struct {
+    uint64_t selected_kmers;
+    uint64_t minimizers_above_threshold;
+    double total_seconds;
+} mczi_kmer_fasta_stats;
+
+/* DEFAULT_PARTITION_COUNT (src/lib.rs:29). */
+#define MCZI_DEFAULT_PARTITION_COUNT 1024u
+
+const char *mczi_last_error(void);
+const char *mczi_version(void);
+void mczi_free(void *p);
+
+/* validate_config (src/lib.rs:110-117) / validate_dataset_presence_config
+ * (src/lib.rs:141-152). datasets != 0 selects the fof rules. */
+int mczi_validate_config(const mczi_counter_config *cfg, int datasets);
+
+/* expand_fofns (src/lib.rs:242-276). *out is an array of n_out C strings,
+ * released with mczi_free_strings. */
+int mczi_expand_fofns(const char *const *fofns, size_t n, char ***out, size_t *n_out);
+void mczi_free_strings(char **strs, size_t n);
+
+/* count_inputs (src/lib.rs:278-287): exact counts > threshold, sorted
+ * ascending by encoded k-mer. */
+int mczi_count_inputs(const char *const *paths, size_t n, const mczi_counter_config *cfg,
+                      mczi_counted_kmer **out, size_t *n_out);
+/* count_datasets (src/lib.rs:289-304): count = number of datasets (one input
+ * file each) holding the k-mer, emitted when > threshold, sorted. */
+int mczi_count_datasets(const char *const *paths, size_t n, const mczi_counter_config *cfg,
+                        mczi_counted_kmer **out, size_t *n_out);
+
+/* count_inputs_to_fasta_path (src/lib.rs:427-440): simplitig FASTA. */
+int mczi_count_inputs_to_fasta_path(const char *const *paths, size_t n,
+                                    const mczi_counter_config *cfg, const char *out_path);
+/* count_datasets_to_fasta_path (src/lib.rs:442-455). */
+int mczi_count_datasets_to_fasta_path(const char *const *paths, size_t n,
+                                      const mczi_counter_config *cfg, const char *out_path);
+/* count_inputs_to_kff_path (src/lib.rs:457-507): k <= 32 takes the KMC-style
+ * single-pass u8 path (src/lib.rs:599-635), k > 32 the 8-byte-count path. */
+int mczi_count_inputs_to_kff_path(const char *const *paths, size_t n,
+                                  const mczi_counter_config *cfg, const char *out_path);
+/* count_datasets_to_kff_path (src/lib.rs:509-522): 4-byte counts. */
+int mczi_count_datasets_to_kff_path(const char *const *paths, size_t n,
+                                    const mczi_counter_config *cfg, const char *out_path);
+/* count_inputs_to_kmer_fasta_path_with_stats (src/lib.rs:314-355). */
+int mczi_count_inputs_to_kmer_fasta_path(const char *const *paths, size_t n,
+                                         const mczi_counter_config *cfg, const char *out_path,
+                                         mczi_kmer_fasta_stats *stats);
+/* count_datasets_to_kmer_fasta_path_with_stats (src/lib.rs:365-371). */
+int mczi_count_datasets_to_kmer_fasta_path(const char *const *paths, size_t n,
+                                           const mczi_counter_config *cfg,
+                                           const char *out_path, mczi_kmer_fasta_stats *stats);
+
+/* write_fasta_path (src/lib.rs:1203-1207) and write_fasta_after_index
+ * (src/lib.rs:1194-1201, here writing to a path); last_index may be NULL. */
+int mczi_write_fasta_path(const char *path, const mczi_counted_kmer *kmers, size_t n, uint32_t k);
+int mczi_write_fasta_after_index(const char *path, const mczi_counted_kmer *kmers, size_t n,
+                                 uint32_t k, size_t last_written_index, size_t *last_index);
+/* write_kff (src/lib.rs:1278-1293): 8-byte counts, ordered=1. */
+int mczi_write_kff_path(const char *path, const mczi_counted_kmer *kmers, size_t n,
+                        const mczi_counter_config *cfg);
+/* decode_kmer (src/lib.rs:3056-3069); out must hold k bytes (no NUL added). */
+int mczi_decode_kmer(uint64_t lo, uint64_t hi, uint32_t k, char *out);
+
+/* ---- kernel-level boundary ------------------------------------------------
+ * Builder::run_hashes with super_kmers (vendor/simd-minimizers-2.3.1/src/
+ * lib.rs:569-616) as consumed by for_each_minimizer_run_on_seq
+ * (src/lib.rs:3694-3752): canonical minimizers (hash length m, window
+ * w = k - m + 1) of ONE packed sequence, computed on the device.
+ * packed: packed-seq bytes (A0 C1 T2 G3, 4 bases per byte, LSB first).
+ * hashes/starts must hold n_bases - k + 1 entries; *n_runs gets the number
+ * of runs (deduplicated selected hashes and the first window of each). */
+int mczi_minimizer_run_hashes(const uint8_t *packed, uint64_t n_bases, uint32_t k, uint32_t m,
+                              uint32_t *hashes, uint32_t *starts, size_t *n_runs);
+
+/* ---- batched device API ---------------------------------------------------
+ * A device-resident segmented buffer of reads (many files / datasets packed
+ * together). Fragments are ACGT-only runs; fragment i spans bases
+ * [frag_off[i], frag_off[i+1]) of the packed stream, which is 16 bases per
+ * little-endian uint32 word (the packed-seq byte layout read as words).
+ * dataset (optional) gives the fof dataset of each fragment. */
+typedef struct mczi_batch mczi_batch;
+
+int mczi_batch_from_files(const char *const *paths, size_t n, uint32_t min_len,
+                          int one_dataset_per_file, mczi_batch **out);
+int mczi_batch_from_packed(const uint32_t *words, const uint64_t *frag_off, size_t n_frag,
+                           const uint32_t *dataset, mczi_batch **out);
+int mczi_batch_info(const mczi_batch *b, uint64_t *n_bases, uint64_t *n_frag,
+                    uint32_t *n_datasets);
+void mczi_batch_free(mczi_batch *b);
+
+/* Counting modes. */
+enum {
+    MCZI_MODE_TWO_PASS = 0,  /* minimizer pre-count, then filtered super-k-mers (FASTA path) */
+    MCZI_MODE_SINGLE_PASS = 1, /* all super-k-mers (KMC-style KFF path) */
+    MCZI_MODE_DATASETS = 2   /* fof dataset presence */
+};
+/* Launch structure. */
+enum {
+    MCZI_LAUNCH_BATCHED = 0, /* one persistent launch per pass over the whole buffer (graph) */
+    MCZI_LAUNCH_PLAIN = 1    /* one launch per input chunk (per file / 128 Mi bases) */
+};
+/* Output order. */
+enum {
+    MCZI_ORDER_ENCODED = 0,  /* ascending encoded k-mer (count_inputs) */
+    MCZI_ORDER_KFF_U8 = 1,   /* reference KFF u8 order: partition, encoded & 255, encoded */
+    MCZI_ORDER_NONE = 2
+};
+
+typedef struct {
+    uint64_t n_bases;            /* bases in the batch */
+    uint64_t n_windows;          /* k-mer instances scanned */
+    uint64_t n_pieces;           /* minimizer runs (pieces) found */
+    uint64_t n_records;          /* super-k-mer records written to partitions */
+    uint64_t n_distinct;         /* distinct k-mers counted */
+    uint64_t n_overflow;         /* distinct k-mers that spilled from LDS to HBM */
+    uint64_t n_out;              /* k-mers emitted */
+    uint64_t minimizers_kept;    /* distinct minimizer keys (two-pass / fof) */
+    uint64_t partition_bytes;    /* bytes of super-k-mer records */
+    uint32_t internal_partitions;
+    uint32_t lds_slots;
+    double ms_scan1, ms_scan2, ms_count, ms_sort, ms_total; /* device time per phase */
+} mczi_run_stats;
+
+/* Count the k-mers of a batch on the device. Results stay on the device
+ * until mczi_batch_download; stats may be NULL. */
+int mczi_batch_count(mczi_batch *b, const mczi_counter_config *cfg, int mode, int launch,
+                     int order, mczi_run_stats *stats);
+int mczi_batch_download(mczi_batch *b, mczi_counted_kmer **out, size_t *n_out);
+
+/* ---- ZI (src/zi.rs) -------------------------------------------------------- */
+/* ZI main path (src/zi.rs:64-100): absent query k-mers as simplitig FASTA.
+ * m == 0 selects default_minimizer_size (src/zi.rs:109-112). */
+int mczi_zi_subtract(const char *index_path, const char *const *query_paths, size_t n,
+                     uint32_t k, uint32_t m, const char *out_path, uint32_t threads,
+                     uint32_t ram_limit_gib);
+/* collect_query_kmers_absent_from_index (src/zi.rs:152-175) as a library
+ * call: sorted, deduplicated absent canonical k-mers (count = 1). */
+int mczi_zi_absent(const char *index_path, const char *const *query_paths, size_t n,
+                   uint32_t k, mczi_counted_kmer **out, size_t *n_out);
+
+/* ---- ZI serving: a device-resident index and a continuous-batching loop ----
+ * No counterpart in the reference (which runs one query set per process).
+ * Requests are FASTA/FASTQ text; each result is the sorted absent set and its
+ * simplitig FASTA text. */
+typedef struct mczi_zi_server mczi_zi_server;
+typedef struct {
+    uint64_t request_id;
+    uint64_t n_windows;   /* query k-mers looked up */
+    uint64_t n_absent;    /* distinct absent k-mers */
+    mczi_counted_kmer *absent;
+    char *fasta;          /* simplitig FASTA text (NUL-terminated) */
+    size_t fasta_len;
+    double queue_ms;      /* submit -> admitted */
+    double latency_ms;    /* submit -> result ready */
+} mczi_zi_result;
+
+int mczi_zi_server_create(const char *index_path, uint32_t k, uint32_t m,
+                          uint64_t hbm_budget_bytes, uint32_t max_inflight,
+                          mczi_zi_server **out);
+/* Build the server from an in-memory index (FASTA text). */
+int mczi_zi_server_create_from_text(const char *index_fasta, size_t len, uint32_t k, uint32_t m,
+                                    uint64_t hbm_budget_bytes, uint32_t max_inflight,
+                                    mczi_zi_server **out);
+int mczi_zi_server_submit(mczi_zi_server *s, const char *fastx, size_t len, int make_fasta,
+                          uint64_t *request_id);
+/* wait != 0 blocks until the request is done; otherwise MCZI_E_PENDING. */
+int mczi_zi_server_poll(mczi_zi_server *s, uint64_t request_id, int wait,
+                        mczi_zi_result **result);
+void mczi_zi_result_free(mczi_zi_result *r);
+int mczi_zi_server_stats(mczi_zi_server *s, uint64_t *index_kmers, uint64_t *completed,
+                         uint64_t *lookups, double *busy_ms);
+void mczi_zi_server_destroy(mczi_zi_server *s);
+
+/* ---- synthetic inputs (no counterpart in the reference; test/bench data) --
+ * SplitMix64 genome and reads (bench/simulate_long_reads.rs:9-30, 166-180),
+ * extended with random strand, iid errors, indels and repeats. Produces
+ * packed words + fragment offsets + read-to-dataset ids in host memory. */
+typedef struct {
+    uint64_t genome_len;
+    uint64_t genome_seed;
+    uint64_t read_seed;
+    uint64_t n_reads;        /* short reads (or 0) */
+    uint32_t read_len;       /* short-read length */
+    double sub_rate, ins_rate, del_rate;
+    uint64_t long_bases;     /* total bases of lognormal long reads (or 0) */
+    double long_median, long_sigma;
+    uint32_t long_min, long_max;
+    double repeat_fraction;  /* genome fraction duplicated as repeats */
+    uint32_t repeat_len, repeat_copies;
+    uint32_t n_datasets;     /* > 0: dataset variants of the genome (fof) */
+    double snp_rate;         /* per-dataset SNP rate */
+} mczi_synth_spec;
+
+typedef struct {
+    uint32_t *words;
+    uint64_t n_words;
+    uint64_t *frag_off;
+    uint64_t n_frag;
+    uint32_t *dataset;
+    uint64_t n_bases;
+} mczi_synth_reads;
+
+int mczi_synth_generate(const mczi_synth_spec *spec, mczi_synth_reads *out);
+int mczi_synth_genome(uint64_t genome_len, uint64_t seed, uint32_t **words);
+/* Write reads as FASTA (one record per read); dataset_filter < 0 for all. */
+int mczi_synth_write_fasta(const mczi_synth_reads *r, const char *path, int64_t dataset_filter);
+void mczi_synth_free(mczi_synth_reads *r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,488 @@
+// C ABI (include/mczi.h): argument checks, error capture, and the drivers
+// that mirror the reference's lib.rs / zi.rs entry points.
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+#include "zi.hpp"
+
+using namespace mczi;
+
+namespace {
+
+thread_local std::string g_err;
+
+template <typename F>
+int guarded(const char *ctx, F &&f) {
+    try {
+        f();
+        g_err.clear();
+        return MCZI_OK;
+    } catch (const Error &e) {
+        g_err = std::string(ctx) + ": " + e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_err = std::string(ctx) + ": out of host memory";
+        return MCZI_E_NOMEM;
+    } catch (const std::exception &e) {
+        g_err = std::string(ctx) + ": " + e.what();
+        return MCZI_E_INVALID;
+    }
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// validate_shape_config / validate_config / validate_dataset_presence_config
+// (src/lib.rs:110-152), same messages.
+void validate_shape(const mczi_counter_config &c) {
+    require(c.k > 0, MCZI_E_INVALID, "k must be greater than 0");
+    require(c.k <= 64, MCZI_E_INVALID, "k must be <= 64 because MC stores k-mers in u128");
+    require(c.minimizer > 0, MCZI_E_INVALID, "minimizer size must be greater than 0");
+    require(c.minimizer <= c.k, MCZI_E_INVALID, "minimizer size must be <= k");
+    require(c.minimizer <= 64, MCZI_E_INVALID,
+            "minimizer size must be <= 64 because MC stores minimizers in u128");
+    require(c.partition_count > 0, MCZI_E_INVALID, "partition count must be greater than 0");
+}
+void validate_normal(const mczi_counter_config &c) {
+    validate_shape(c);
+    require(c.threshold < 255, MCZI_E_INVALID,
+            "threshold must be < 255 because compact minimizer counts saturate in u8");
+}
+void validate_datasets(const mczi_counter_config &c) {
+    validate_shape(c);
+    require(c.k <= 32, MCZI_E_INVALID, "FOFN dataset-presence mode currently requires k <= 32");
+    require(c.threshold < 0x7fffffffull, MCZI_E_INVALID, "FOFN dataset-presence threshold must be < 2^31");
+}
+
+std::vector<std::string> to_paths(const char *const *paths, size_t n) {
+    std::vector<std::string> v;
+    for (size_t i = 0; i < n; i++) {
+        require(paths && paths[i], MCZI_E_INVALID, "null input path");
+        v.emplace_back(paths[i]);
+    }
+    return v;
+}
+
+std::unique_ptr<Batch> load_inputs(const std::vector<std::string> &paths, uint32_t k, bool datasets) {
+    HostReads r;
+    r.n_datasets = datasets ? (uint32_t)paths.size() : 1;
+    for (size_t i = 0; i < paths.size(); i++) parse_fastx_file(paths[i], k, datasets ? (uint32_t)i : 0, r);
+    return make_batch(std::move(r));
+}
+
+std::vector<mczi_counted_kmer> run_count(const std::vector<std::string> &paths,
+                                         const mczi_counter_config &cfg, int mode, int order,
+                                         mczi_run_stats *st) {
+    auto b = load_inputs(paths, cfg.k, mode == MCZI_MODE_DATASETS);
+    std::vector<mczi_counted_kmer> out;
+    if (b->n_frag == 0) return out;
+    batch_count(*b, cfg, mode, MCZI_LAUNCH_BATCHED, order, st);
+    batch_download(*b, out);
+    return out;
+}
+
+void copy_out(const std::vector<mczi_counted_kmer> &v, mczi_counted_kmer **out, size_t *n_out) {
+    require(out && n_out, MCZI_E_INVALID, "null output pointer");
+    *n_out = v.size();
+    *out = (mczi_counted_kmer *)malloc(std::max<size_t>(v.size(), 1) * sizeof(mczi_counted_kmer));
+    require(*out != nullptr, MCZI_E_NOMEM, "out of host memory");
+    if (!v.empty()) memcpy(*out, v.data(), v.size() * sizeof(mczi_counted_kmer));
+}
+
+std::vector<mczi_counted_kmer> count_inputs_impl(const char *const *paths, size_t n,
+                                                 const mczi_counter_config *cfg, int order) {
+    require(cfg != nullptr, MCZI_E_INVALID, "null config");
+    validate_normal(*cfg);
+    require(n > 0, MCZI_E_INVALID, "at least one input file is required");
+    const double t0 = now_s();
+    mczi_run_stats st{};
+    auto v = run_count(to_paths(paths, n), *cfg, MCZI_MODE_TWO_PASS, order, &st);
+    log_phase("3_kmer_counting_and_output", now_s() - t0);
+    return v;
+}
+
+std::vector<mczi_counted_kmer> count_datasets_impl(const char *const *paths, size_t n,
+                                                   const mczi_counter_config *cfg) {
+    require(cfg != nullptr, MCZI_E_INVALID, "null config");
+    validate_datasets(*cfg);
+    require(n > 0, MCZI_E_INVALID, "at least one dataset file is required");
+    if (cfg->threshold >= n) {  // src/lib.rs:644-648
+        log_phase("1_dataset_minimizer_counting", 0);
+        log_phase("2_dataset_kmer_counting", 0);
+        return {};
+    }
+    const double t0 = now_s();
+    auto v = run_count(to_paths(paths, n), *cfg, MCZI_MODE_DATASETS, MCZI_ORDER_ENCODED, nullptr);
+    log_phase("2_dataset_kmer_counting", now_s() - t0);
+    return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mczi_last_error(void) { return g_err.c_str(); }
+const char *mczi_version(void) { return "mczi-hip 0.1.0 (gfx950)"; }
+void mczi_free(void *p) { free(p); }
+
+int mczi_validate_config(const mczi_counter_config *cfg, int datasets) {
+    return guarded("invalid configuration", [&] {
+        require(cfg != nullptr, MCZI_E_INVALID, "null config");
+        if (datasets) validate_datasets(*cfg);
+        else validate_normal(*cfg);
+    });
+}
+
+// expand_fofns (src/lib.rs:242-276)
+int mczi_expand_fofns(const char *const *fofns, size_t n, char ***out, size_t *n_out) {
+    return guarded("failed to expand FOFN", [&] {
+        require(out && n_out, MCZI_E_INVALID, "null output pointer");
+        std::vector<std::string> inputs;
+        for (size_t i = 0; i < n; i++) {
+            std::string f(fofns[i]);
+            std::ifstream in(f);
+            require((bool)in, MCZI_E_IO, "failed to read FOFN " + f);
+            std::string base = ".";
+            size_t sl = f.rfind('/');
+            if (sl != std::string::npos) base = sl == 0 ? "/" : f.substr(0, sl);
+            std::string line;
+            while (std::getline(in, line)) {
+                size_t a = line.find_first_not_of(" \t\r\n");
+                if (a == std::string::npos) continue;
+                size_t b = line.find_last_not_of(" \t\r\n");
+                std::string t = line.substr(a, b - a + 1);
+                if (t.empty() || t[0] == '#') continue;
+                if (t[0] != '/') t = (base == "/" ? "/" : base + "/") + t;
+                inputs.push_back(t);
+            }
+        }
+        require(!inputs.empty(), MCZI_E_INVALID, "no inputs found in FOFN(s)");
+        char **arr = (char **)calloc(inputs.size(), sizeof(char *));
+        require(arr != nullptr, MCZI_E_NOMEM, "out of host memory");
+        for (size_t i = 0; i < inputs.size(); i++) arr[i] = strdup(inputs[i].c_str());
+        *out = arr;
+        *n_out = inputs.size();
+    });
+}
+
+void mczi_free_strings(char **strs, size_t n) {
+    if (!strs) return;
+    for (size_t i = 0; i < n; i++) free(strs[i]);
+    free(strs);
+}
+
+int mczi_count_inputs(const char *const *paths, size_t n, const mczi_counter_config *cfg,
+                      mczi_counted_kmer **out, size_t *n_out) {
+    return guarded("failed to count inputs", [&] {
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        copy_out(count_inputs_impl(paths, n, cfg, MCZI_ORDER_ENCODED), out, n_out);
+    });
+}
+
+int mczi_count_datasets(const char *const *paths, size_t n, const mczi_counter_config *cfg,
+                        mczi_counted_kmer **out, size_t *n_out) {
+    return guarded("failed to count datasets", [&] {
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        copy_out(count_datasets_impl(paths, n, cfg), out, n_out);
+    });
+}
+
+int mczi_count_inputs_to_fasta_path(const char *const *paths, size_t n, const mczi_counter_config *cfg,
+                                    const char *out_path) {
+    return guarded("failed to write simplitig FASTA", [&] {
+        require(out_path != nullptr, MCZI_E_INVALID, "null output path");
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        const double t0 = now_s();
+        auto v = count_inputs_impl(paths, n, cfg, MCZI_ORDER_ENCODED);
+        const double t1 = now_s();
+        write_simplitig_fasta(out_path, v, cfg->k, 0, nullptr);
+        log_phase("4_simplitig_output", now_s() - t1);
+        log_phase("fasta_total", now_s() - t0);
+    });
+}
+
+int mczi_count_datasets_to_fasta_path(const char *const *paths, size_t n,
+                                      const mczi_counter_config *cfg, const char *out_path) {
+    return guarded("failed to write simplitig FASTA", [&] {
+        require(out_path != nullptr, MCZI_E_INVALID, "null output path");
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        const double t0 = now_s();
+        auto v = count_datasets_impl(paths, n, cfg);
+        const double t1 = now_s();
+        write_simplitig_fasta(out_path, v, cfg->k, 0, nullptr);
+        log_phase("3_output_streaming", now_s() - t1);
+        log_phase("total", now_s() - t0);
+    });
+}
+
+int mczi_count_inputs_to_kff_path(const char *const *paths, size_t n, const mczi_counter_config *cfg,
+                                  const char *out_path) {
+    return guarded("failed to write KFF", [&] {
+        require(cfg != nullptr && out_path != nullptr, MCZI_E_INVALID, "null argument");
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        validate_normal(*cfg);
+        require(n > 0, MCZI_E_INVALID, "at least one input file is required");
+        const double t0 = now_s();
+        if (cfg->k <= 32) {
+            // KMC-style single pass, u8 counts saturating at min(x+1, 255)
+            // (src/lib.rs:599-635, 2489-2506)
+            mczi_run_stats st{};
+            auto v = run_count(to_paths(paths, n), *cfg, MCZI_MODE_SINGLE_PASS, MCZI_ORDER_KFF_U8, &st);
+            fprintf(stderr, "MC_STAT\tpartition_bytes\t%llu\n", (unsigned long long)st.partition_bytes);
+            log_phase("1_superkmer_partitioning", st.ms_scan1 / 1e3);
+            const double t1 = now_s();
+            write_kff(out_path, v, *cfg, 1, false, std::min<uint64_t>(cfg->threshold + 1, 255));
+            log_phase("2b_output_streaming", now_s() - t1);
+        } else {
+            auto v = count_inputs_impl(paths, n, cfg, MCZI_ORDER_ENCODED);
+            write_kff(out_path, v, *cfg, 8, false, 0);
+        }
+        log_phase("total", now_s() - t0);
+    });
+}
+
+int mczi_count_datasets_to_kff_path(const char *const *paths, size_t n,
+                                    const mczi_counter_config *cfg, const char *out_path) {
+    return guarded("failed to write KFF", [&] {
+        require(out_path != nullptr, MCZI_E_INVALID, "null output path");
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        const double t0 = now_s();
+        auto v = count_datasets_impl(paths, n, cfg);
+        const double t1 = now_s();
+        write_kff(out_path, v, *cfg, 4, false, 0);
+        log_phase("3_output_streaming", now_s() - t1);
+        log_phase("total", now_s() - t0);
+    });
+}
+
+int mczi_count_inputs_to_kmer_fasta_path(const char *const *paths, size_t n,
+                                         const mczi_counter_config *cfg, const char *out_path,
+                                         mczi_kmer_fasta_stats *stats) {
+    return guarded("failed to write k-mer FASTA", [&] {
+        require(out_path != nullptr, MCZI_E_INVALID, "null output path");
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        const double t0 = now_s();
+        auto v = count_inputs_impl(paths, n, cfg, MCZI_ORDER_ENCODED);
+        write_kmer_fasta(out_path, v, cfg->k);
+        if (stats) {
+            stats->selected_kmers = v.size();
+            stats->minimizers_above_threshold = 0;
+            stats->total_seconds = now_s() - t0;
+        }
+    });
+}
+
+int mczi_count_datasets_to_kmer_fasta_path(const char *const *paths, size_t n,
+                                           const mczi_counter_config *cfg, const char *out_path,
+                                           mczi_kmer_fasta_stats *stats) {
+    return guarded("failed to write k-mer FASTA", [&] {
+        require(out_path != nullptr, MCZI_E_INVALID, "null output path");
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        const double t0 = now_s();
+        auto v = count_datasets_impl(paths, n, cfg);
+        write_kmer_fasta(out_path, v, cfg->k);
+        if (stats) {
+            stats->selected_kmers = v.size();
+            stats->minimizers_above_threshold = 0;
+            stats->total_seconds = now_s() - t0;
+        }
+    });
+}
+
+int mczi_write_fasta_path(const char *path, const mczi_counted_kmer *kmers, size_t n, uint32_t k) {
+    return guarded("failed to write FASTA", [&] {
+        std::vector<mczi_counted_kmer> v(kmers, kmers + n);
+        write_simplitig_fasta(path, v, k, 0, nullptr);
+    });
+}
+
+int mczi_write_fasta_after_index(const char *path, const mczi_counted_kmer *kmers, size_t n, uint32_t k,
+                                 size_t last_written_index, size_t *last_index) {
+    return guarded("failed to write FASTA", [&] {
+        std::vector<mczi_counted_kmer> v(kmers, kmers + n);
+        write_simplitig_fasta(path, v, k, last_written_index, last_index);
+    });
+}
+
+int mczi_write_kff_path(const char *path, const mczi_counted_kmer *kmers, size_t n,
+                        const mczi_counter_config *cfg) {
+    return guarded("failed to write KFF", [&] {
+        require(cfg != nullptr, MCZI_E_INVALID, "null config");
+        validate_normal(*cfg);
+        std::vector<mczi_counted_kmer> v(kmers, kmers + n);
+        write_kff(path, v, *cfg, 8, true, 0);
+    });
+}
+
+int mczi_decode_kmer(uint64_t lo, uint64_t hi, uint32_t k, char *out) {
+    return guarded("failed to decode k-mer", [&] {
+        require(k > 0 && k <= 64 && out != nullptr, MCZI_E_INVALID, "k must be in 1..=64");
+        static const char a[4] = {'A', 'C', 'G', 'T'};
+        for (uint32_t i = 0; i < k; i++) {
+            const uint32_t sh = 2 * (k - 1 - i);
+            const uint64_t v = sh >= 64 ? (hi >> (sh - 64)) : ((lo >> sh) | (sh ? (hi << (64 - sh)) : 0));
+            out[i] = a[v & 3u];
+        }
+    });
+}
+
+int mczi_minimizer_run_hashes(const uint8_t *packed, uint64_t n_bases, uint32_t k, uint32_t m,
+                              uint32_t *hashes, uint32_t *starts, size_t *n_runs) {
+    return guarded("failed to compute minimizer runs", [&] {
+        require(n_runs != nullptr, MCZI_E_INVALID, "null output pointer");
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        std::vector<uint32_t> h, s;
+        device_run_hashes(packed, n_bases, k, m, h, s);
+        if (!h.empty()) {
+            require(hashes && starts, MCZI_E_INVALID, "null output arrays");
+            memcpy(hashes, h.data(), h.size() * 4);
+            memcpy(starts, s.data(), s.size() * 4);
+        }
+        *n_runs = h.size();
+    });
+}
+
+// ---- batched device API -----------------------------------------------------
+struct mczi_batch {
+    std::unique_ptr<Batch> b;
+};
+
+int mczi_batch_from_files(const char *const *paths, size_t n, uint32_t min_len, int one_dataset_per_file,
+                          mczi_batch **out) {
+    return guarded("failed to load batch", [&] {
+        require(out != nullptr, MCZI_E_INVALID, "null output pointer");
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        auto h = new mczi_batch();
+        try {
+            h->b = load_inputs(to_paths(paths, n), std::max<uint32_t>(min_len, 1), one_dataset_per_file != 0);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int mczi_batch_from_packed(const uint32_t *words, const uint64_t *frag_off, size_t n_frag,
+                           const uint32_t *dataset, mczi_batch **out) {
+    return guarded("failed to load batch", [&] {
+        require(out != nullptr && frag_off != nullptr, MCZI_E_INVALID, "null argument");
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        HostReads r;
+        const uint64_t nb = frag_off[n_frag];
+        r.words.assign(words, words + (nb + 15) / 16);
+        r.cur_ = nb;
+        r.off.assign(frag_off, frag_off + n_frag + 1);
+        uint32_t nd = 1;
+        if (dataset) {
+            r.dataset.assign(dataset, dataset + n_frag);
+            for (size_t i = 0; i < n_frag; i++) nd = std::max(nd, dataset[i] + 1);
+        } else {
+            r.dataset.assign(n_frag, 0);
+        }
+        for (size_t i = 0; i < n_frag; i++)
+            require(frag_off[i + 1] >= frag_off[i], MCZI_E_INVALID, "fragment offsets must not decrease");
+        r.n_datasets = nd;
+        // input chunks of >= 128 Mi bases (BATCH_BASES, src/lib.rs:28) for the plain launch mode
+        uint64_t acc = 0;
+        for (size_t i = 0; i < n_frag; i++) {
+            acc += frag_off[i + 1] - frag_off[i];
+            if (acc >= (128ull << 20) && i + 1 < n_frag) {
+                r.chunk_frag.push_back(i + 1);
+                acc = 0;
+            }
+        }
+        auto h = new mczi_batch();
+        h->b = make_batch(std::move(r));
+        *out = h;
+    });
+}
+
+int mczi_batch_info(const mczi_batch *b, uint64_t *n_bases, uint64_t *n_frag, uint32_t *n_datasets) {
+    return guarded("failed to query batch", [&] {
+        require(b != nullptr, MCZI_E_INVALID, "null batch");
+        if (n_bases) *n_bases = b->b->n_bases;
+        if (n_frag) *n_frag = b->b->n_frag;
+        if (n_datasets) *n_datasets = b->b->n_datasets;
+    });
+}
+
+void mczi_batch_free(mczi_batch *b) {
+    if (!b) return;
+    try {
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        delete b;
+    } catch (...) {
+    }
+}
+
+int mczi_batch_count(mczi_batch *b, const mczi_counter_config *cfg, int mode, int launch, int order,
+                     mczi_run_stats *stats) {
+    return guarded("failed to count batch", [&] {
+        require(b != nullptr && cfg != nullptr, MCZI_E_INVALID, "null argument");
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        if (mode == MCZI_MODE_DATASETS) validate_datasets(*cfg);
+        else validate_normal(*cfg);
+        if (mode == MCZI_MODE_DATASETS && cfg->threshold >= b->b->n_datasets) {
+            b->b->n_out = 0;
+            if (stats) *stats = mczi_run_stats{};
+            return;
+        }
+        batch_count(*b->b, *cfg, mode, launch, order, stats);
+    });
+}
+
+int mczi_batch_download(mczi_batch *b, mczi_counted_kmer **out, size_t *n_out) {
+    return guarded("failed to download results", [&] {
+        require(b != nullptr, MCZI_E_INVALID, "null batch");
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        std::vector<mczi_counted_kmer> v;
+        batch_download(*b->b, v);
+        copy_out(v, out, n_out);
+    });
+}
+
+// ---- synthetic inputs -------------------------------------------------------
+int mczi_synth_generate(const mczi_synth_spec *spec, mczi_synth_reads *out) {
+    return guarded("failed to generate synthetic reads", [&] {
+        require(spec && out, MCZI_E_INVALID, "null argument");
+        *out = mczi_synth_reads{};
+        synth_generate(*spec, *out);
+    });
+}
+
+int mczi_synth_genome(uint64_t genome_len, uint64_t seed, uint32_t **words) {
+    return guarded("failed to generate genome", [&] { synth_genome_words(genome_len, seed, words); });
+}
+
+int mczi_synth_write_fasta(const mczi_synth_reads *r, const char *path, int64_t dataset_filter) {
+    return guarded("failed to write synthetic FASTA", [&] {
+        require(r && path, MCZI_E_INVALID, "null argument");
+        synth_write_fasta(*r, path, dataset_filter);
+    });
+}
+
+void mczi_synth_free(mczi_synth_reads *r) {
+    if (!r) return;
+    free(r->words);
+    free(r->frag_off);
+    free(r->dataset);
+    *r = mczi_synth_reads{};
+}
+
+}  // extern "C"
+
+// Errors raised inside the ZI translation unit are reported through the same
+// thread-local text as every other entry point.
+extern "C" __attribute__((visibility("hidden"))) void mczi_set_error_from_zi(const char *msg) {
+    g_err = msg ? msg : "";
+}

@@ -1,0 +1,266 @@
+// Device context, caching allocator, FASTA/FASTQ ingest and phase logs.
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <cctype>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "internal.hpp"
+
+namespace mczi {
+
+Device &Device::get() {
+    static Device d;
+    return d;
+}
+
+Device::Device() {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+        throw Error(MCZI_E_DEVICE,
+                    "no HIP device available: the MCZI device path needs an MI355X (gfx950) GPU");
+    MCZI_HIP(hipGetDevice(&dev_));
+    hipDeviceProp_t prop;
+    MCZI_HIP(hipGetDeviceProperties(&prop, dev_));
+    cus_ = prop.multiProcessorCount;
+    MCZI_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+}
+
+static size_t round_alloc(size_t b) {
+    const size_t g = 2u << 20;
+    return (b + g - 1) / g * g;
+}
+
+void *Device::alloc(size_t bytes) {
+    const size_t want = round_alloc(bytes);
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        auto it = free_.lower_bound(want);
+        if (it != free_.end() && it->first <= want * 2) {
+            void *p = it->second;
+            live_[p] = it->first;
+            free_.erase(it);
+            return p;
+        }
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto &kv : free_) (void)hipFree(kv.second);
+        free_.clear();
+        p = nullptr;
+        e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(MCZI_E_NOMEM, "device allocation of " + std::to_string(want) +
+                                          " bytes failed: " + hipGetErrorString(e));
+        }
+    }
+    std::lock_guard<std::mutex> lk(mu_);
+    live_[p] = want;
+    return p;
+}
+
+void Device::release(void *p) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = live_.find(p);
+    if (it == live_.end()) return;
+    free_.emplace(it->second, p);
+    live_.erase(it);
+}
+
+// ---------------------------------------------------------------------------
+// FASTA / FASTQ ingest. Base codes are packed-seq codes A0 C1 T2 G3; lower
+// case acgt count as bases (src/zi.rs:240-248 and src/lib.rs:3015-3026 map
+// both cases); any other character ends the current fragment.
+
+namespace {
+struct CodeTable {
+    uint8_t t[256];
+    CodeTable() {
+        memset(t, 5, sizeof t);
+        t[(int)'A'] = t[(int)'a'] = 0;
+        t[(int)'C'] = t[(int)'c'] = 1;
+        t[(int)'T'] = t[(int)'t'] = 2;
+        t[(int)'G'] = t[(int)'g'] = 3;
+        t[(int)'\n'] = t[(int)'\r'] = 4;
+    }
+};
+const CodeTable kCodes;
+
+struct Packer {
+    HostReads &r;
+    uint64_t min_len;
+    uint32_t ds;
+    std::vector<uint8_t> frag;
+    Packer(HostReads &rr, uint64_t ml, uint32_t d) : r(rr), min_len(ml), ds(d) { frag.reserve(1 << 16); }
+    void close() {
+        if (!frag.empty() && frag.size() >= min_len) {
+            uint64_t n = r.cur_;
+            const uint8_t *c = frag.data();
+            size_t len = frag.size();
+            r.words.resize((n + len + 15) / 16 + 1, 0);
+            size_t i = 0;
+            // head until word aligned
+            while (i < len && ((n + i) & 15u)) {
+                r.words[(n + i) >> 4] |= (uint32_t)c[i] << (((n + i) & 15u) * 2u);
+                i++;
+            }
+            for (; i + 16 <= len; i += 16) {
+                uint32_t w = 0;
+                for (int j = 0; j < 16; j++) w |= (uint32_t)c[i + j] << (2 * j);
+                r.words[(n + i) >> 4] = w;
+            }
+            for (; i < len; i++) r.words[(n + i) >> 4] |= (uint32_t)c[i] << (((n + i) & 15u) * 2u);
+            r.cur_ = n + len;
+            r.words.resize((r.cur_ + 15) / 16);
+            r.off.push_back(r.cur_);
+            r.dataset.push_back(ds);
+        }
+        frag.clear();
+    }
+};
+}  // namespace
+
+static void parse_buffer(const char *buf, size_t len, uint64_t min_len, uint32_t dataset,
+                         HostReads &out, const std::string &what) {
+    size_t i = 0;
+    while (i < len && (buf[i] == '\n' || buf[i] == '\r' || buf[i] == ' ' || buf[i] == '\t')) i++;
+    if (i >= len) return;
+    if (buf[i] != '>' && buf[i] != '@') {
+        fprintf(stderr, "MC_WARN\tskipped_invalid_fastx_input\tpath\t%s\terror\t%s\n", what.c_str(),
+                "Invalid record start");
+        return;
+    }
+    Packer pk(out, min_len, dataset);
+    const bool fastq = buf[i] == '@';
+    while (i < len) {
+        // header line
+        if (buf[i] != (fastq ? '@' : '>')) {
+            // stray line outside a record: skip it
+            const char *nl = (const char *)memchr(buf + i, '\n', len - i);
+            i = nl ? (size_t)(nl - buf) + 1 : len;
+            continue;
+        }
+        const char *nl = (const char *)memchr(buf + i, '\n', len - i);
+        i = nl ? (size_t)(nl - buf) + 1 : len;
+        size_t seq_len = 0;
+        // sequence lines
+        while (i < len && buf[i] != (fastq ? '+' : '>')) {
+            const uint8_t c = kCodes.t[(uint8_t)buf[i]];
+            if (c < 4) {
+                pk.frag.push_back(c);
+                seq_len++;
+            } else if (c == 5) {
+                pk.close();
+                seq_len++;
+            }
+            i++;
+        }
+        pk.close();
+        if (fastq && i < len) {
+            // '+' line, then quality characters matching the sequence length
+            const char *pl = (const char *)memchr(buf + i, '\n', len - i);
+            i = pl ? (size_t)(pl - buf) + 1 : len;
+            size_t q = 0;
+            while (i < len && q < seq_len) {
+                if (buf[i] != '\n' && buf[i] != '\r') q++;
+                i++;
+            }
+            while (i < len && (buf[i] == '\n' || buf[i] == '\r')) i++;
+        }
+    }
+    pk.close();
+}
+
+void parse_fastx_text(const char *text, size_t len, uint64_t min_len, uint32_t dataset,
+                      HostReads &out) {
+    parse_buffer(text, len, min_len, dataset, out, "<memory>");
+    out.end_chunk();
+}
+
+void parse_fastx_file(const std::string &path, uint64_t min_len, uint32_t dataset, HostReads &out) {
+    int fd = open(path.c_str(), O_RDONLY);
+    if (fd < 0)
+        throw Error(MCZI_E_IO, "failed to open FASTA/FASTQ input " + path + ": " + strerror(errno));
+    struct stat st;
+    if (fstat(fd, &st) != 0) {
+        close(fd);
+        throw Error(MCZI_E_IO, "failed to stat " + path);
+    }
+    size_t len = (size_t)st.st_size;
+    if (len == 0) {
+        close(fd);
+        out.end_chunk();
+        return;
+    }
+    void *map = mmap(nullptr, len, PROT_READ, MAP_PRIVATE, fd, 0);
+    close(fd);
+    if (map == MAP_FAILED) throw Error(MCZI_E_IO, "failed to map " + path);
+    madvise(map, len, MADV_SEQUENTIAL);
+    const char *buf = (const char *)map;
+    // Chunk boundaries every >= 128 Mi bases (BATCH_BASES, src/lib.rs:28)
+    // so the plain launch mode can issue one launch per chunk.
+    parse_buffer(buf, len, min_len, dataset, out, path);
+    munmap(map, len);
+    // split this file's fragments into chunks of >= 128 Mi bases
+    const uint64_t batch = 128ull << 20;
+    uint64_t first = out.chunk_frag.back();
+    uint64_t acc = 0;
+    for (uint64_t f = first; f < out.n_frag(); f++) {
+        acc += out.off[f + 1] - out.off[f];
+        if (acc >= batch && f + 1 < out.n_frag()) {
+            out.chunk_frag.push_back(f + 1);
+            acc = 0;
+        }
+    }
+    out.end_chunk();
+}
+
+// ---------------------------------------------------------------------------
+// "MC {phase} {s:.3}s" (src/lib.rs:994-1000, 1027-1062)
+static std::string compact_detail(const std::string &n) {
+    static const char *map[][2] = {
+        {"minimizer_counting", "minimizer count"},
+        {"index_minimizer_presence_counting", "minimizer count"},
+        {"dataset_minimizer_counting", "dataset minimizer count"},
+        {"kmer_counting", "kmer count"},
+        {"index_kmer_counting", "kmer count"},
+        {"dataset_kmer_counting", "dataset kmer count"},
+        {"filtered_kmer_counting", "filtered kmer count"},
+        {"index_kmer_partition_counting", "kmer partition count"},
+        {"superkmer_partitioning", "superkmer partition"},
+        {"output_streaming", "output"},
+        {"kmer_counting_and_output", "kmer count output"},
+        {"simplitig_output", "simplitig output"},
+        {"sshash_indexing", "sshash index"},
+        {"query_subtraction", "query subtraction"},
+        {"fasta_total", "fasta total"},
+        {"total", "total"},
+    };
+    for (auto &e : map)
+        if (n == e[0]) return e[1];
+    return n;
+}
+
+void log_phase(const char *name, double seconds) {
+    std::string n(name);
+    std::string shown;
+    size_t us = n.find('_');
+    if (us != std::string::npos && us > 0 && isdigit((unsigned char)n[0])) {
+        bool alnum = true;
+        for (size_t i = 0; i < us; i++) alnum = alnum && isalnum((unsigned char)n[i]);
+        if (alnum) shown = "phase" + n.substr(0, us) + " " + compact_detail(n.substr(us + 1));
+    }
+    if (shown.empty()) shown = compact_detail(n);
+    fprintf(stderr, "MC %s %.3fs\n", shown.c_str(), seconds);
+}
+
+}  // namespace mczi

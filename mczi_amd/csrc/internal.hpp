@@ -1,0 +1,187 @@
+// Host-side internals of libmczi: errors, device context, packed reads and
+// the MC / ZI drivers. Everything here stays behind the C ABI in mczi.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <atomic>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mczi.h"
+
+namespace mczi {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define MCZI_HIP(x)                                                                        \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            throw ::mczi::Error(MCZI_E_DEVICE, std::string("HIP error '") +                \
+                                                   hipGetErrorString(e_) + "' in " #x);   \
+    } while (0)
+
+inline void require(bool ok, int code, const std::string &msg) {
+    if (!ok) throw Error(code, msg);
+}
+
+// ---------------------------------------------------------------------------
+// Device context: one stream for the synchronous API, a caching allocator.
+class Device {
+  public:
+    static Device &get();
+    hipStream_t stream() const { return stream_; }
+    int cu_count() const { return cus_; }
+    int device() const { return dev_; }
+    // Cached device allocation (rounded to 2 MiB); contents undefined.
+    void *alloc(size_t bytes);
+    void release(void *p);
+    std::mutex &call_mutex() { return call_mu_; }
+
+  private:
+    Device();
+    hipStream_t stream_ = nullptr;
+    int cus_ = 0;
+    int dev_ = 0;
+    std::mutex mu_;
+    std::mutex call_mu_;
+    std::multimap<size_t, void *> free_;
+    std::map<void *, size_t> live_;
+};
+
+// RAII device buffer from the cache.
+class DevBuf {
+  public:
+    DevBuf() = default;
+    explicit DevBuf(size_t bytes) { reset(bytes); }
+    ~DevBuf() { reset(0); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { reset(0); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        return *this;
+    }
+    void reset(size_t bytes) {
+        if (p_) Device::get().release(p_);
+        p_ = nullptr;
+        n_ = 0;
+        if (bytes) { p_ = Device::get().alloc(bytes); n_ = bytes; }
+    }
+    // Grow-only reuse: keeps the buffer when it is already big enough.
+    void ensure(size_t bytes) { if (bytes > n_) reset(bytes); }
+    template <typename T> T *as() const { return static_cast<T *>(p_); }
+    size_t size() const { return n_; }
+
+  private:
+    void *p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Packed reads in host memory: 16 bases per u32 word, fragments back to back.
+struct HostReads {
+    std::vector<uint32_t> words;     // padded with zero words at the end
+    std::vector<uint64_t> off{0};    // fragment boundaries (n_frag + 1)
+    std::vector<uint32_t> dataset;   // dataset of each fragment (may be empty)
+    std::vector<uint64_t> chunk_frag{0}; // fragment index where each input chunk starts
+    uint32_t n_datasets = 1;
+    uint64_t n_bases() const { return off.back(); }
+    uint64_t n_frag() const { return off.size() - 1; }
+    void append_base(uint32_t code) {
+        uint64_t n = cur_;
+        if ((n >> 4) >= words.size()) words.push_back(0);
+        words[n >> 4] |= code << ((n & 15u) * 2u);
+        cur_++;
+    }
+    // Close the fragment being appended; drops it when shorter than min_len.
+    void close_fragment(uint64_t min_len, uint32_t ds) {
+        uint64_t start = off.back();
+        if (cur_ > start && cur_ - start >= min_len) {
+            off.push_back(cur_);
+            dataset.push_back(ds);
+        } else {
+            // drop it: clear the bits written for its bases
+            for (uint64_t i = start; i < cur_; i++) words[i >> 4] &= ~(3u << ((i & 15u) * 2u));
+            cur_ = start;
+            words.resize((cur_ + 15) / 16);
+        }
+    }
+    void end_chunk() {
+        if (chunk_frag.back() != n_frag()) chunk_frag.push_back(n_frag());
+    }
+    void finish() {
+        words.resize((cur_ + 15) / 16 + 32, 0);
+        end_chunk();
+    }
+    uint64_t cur_ = 0;
+};
+
+// Parse FASTA/FASTQ (headers and qualities ignored, split at non-ACGT,
+// fragments shorter than min_len dropped; src/lib.rs:3264-3308).
+void parse_fastx_file(const std::string &path, uint64_t min_len, uint32_t dataset, HostReads &out);
+void parse_fastx_text(const char *text, size_t len, uint64_t min_len, uint32_t dataset,
+                      HostReads &out);
+
+// ---------------------------------------------------------------------------
+// Device-resident segmented read buffer and the MC counting state.
+struct McPlan;
+
+struct Batch {
+    DevBuf words, off, dataset;
+    uint64_t n_bases = 0, n_frag = 0, n_words = 0;
+    uint32_t n_datasets = 1;
+    bool has_dataset = false;
+    std::vector<uint64_t> chunk_frag;
+    std::vector<uint64_t> host_off; // fragment offsets (host copy, for tiling)
+    // tile schedule (depends on k) and per-run state
+    std::unique_ptr<McPlan> plan;
+    // results of the last count (device)
+    DevBuf out_kmer, out_count, out_part;
+    uint64_t n_out = 0;
+    uint32_t k = 0;
+    mczi_run_stats last{};
+    ~Batch();
+};
+
+std::unique_ptr<Batch> make_batch(HostReads &&reads);
+void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch, int order,
+                 mczi_run_stats *stats);
+void batch_download(Batch &b, std::vector<mczi_counted_kmer> &out);
+void free_plan(McPlan *p);
+
+// Device scan boundary for one sequence (run_hashes).
+void device_run_hashes(const uint8_t *packed, uint64_t n_bases, uint32_t k, uint32_t m,
+                       std::vector<uint32_t> &hashes, std::vector<uint32_t> &starts);
+
+// ---------------------------------------------------------------------------
+// Output writers (host end of the path).
+void write_simplitig_fasta(const std::string &path, const std::vector<mczi_counted_kmer> &kmers,
+                           uint32_t k, size_t first_index, size_t *last_index);
+void write_kmer_fasta(const std::string &path, const std::vector<mczi_counted_kmer> &kmers,
+                      uint32_t k);
+void write_kff(const std::string &path, const std::vector<mczi_counted_kmer> &kmers,
+               const mczi_counter_config &cfg, uint64_t count_size, bool ordered,
+               uint64_t saturate_at);
+std::string simplitig_fasta_text(const std::vector<mczi_counted_kmer> &kmers, uint32_t k);
+
+// Phase log lines on stderr, in the reference's formats (src/lib.rs:994-1062).
+void log_phase(const char *name, double seconds);
+
+}  // namespace mczi
+
+namespace mczi {
+// Synthetic inputs (synth.cpp)
+void synth_generate(const mczi_synth_spec &spec, mczi_synth_reads &out);
+void synth_write_fasta(const mczi_synth_reads &r, const std::string &path, int64_t ds_filter);
+void synth_genome_words(uint64_t len, uint64_t seed, uint32_t **words);
+}  // namespace mczi

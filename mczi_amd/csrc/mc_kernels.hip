@@ -1,0 +1,1127 @@
+// MC counting path on gfx950: minimizer scan, pass-1 minimizer counts,
+// super-k-mer partitioning into HBM buckets, LDS-staged exact counting with
+// HBM spill, abundance / dataset-presence filters, output ordering.
+//
+// Reference path (src/lib.rs): for_each_minimizer_run_on_seq 3694-3752 ->
+// add_minimizer_counts_on_seq 3462-3481 (pass 1) -> add_filtered_superkmers_packed
+// 3483-3523 / add_all_superkmers_packed 3525-3560 (pass 2) ->
+// process_superkmer_partition 2400-2425 / ..._to_kff_u8_bucketed 2512-2570
+// (count + filter). fof: 637-703, 4253-4370, 2184-2247.
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace mczi {
+
+// ---------------------------------------------------------------------------
+// Scan kernel geometry
+constexpr int SCAN_THREADS = 256;
+constexpr int WPT = 8;                        // windows per thread
+constexpr int TILE = SCAN_THREADS * WPT;      // windows (base positions) per tile
+constexpr int MAXW = 64;                      // w = k - m + 1 <= 64
+constexpr int WORDS_SPAN = (15 + TILE + 128) / 16 + 8;
+constexpr int REC_MAX_BASES = 56;             // bases per 128-bit super-k-mer record
+
+// Count kernel geometry
+constexpr int CNT_THREADS = 256;
+constexpr int CNT_SLOTS = 4096;               // LDS table slots (u64 key + u32 value)
+constexpr int CNT_MAXPROBE = 32;
+constexpr int OVF_MAXPROBE = 1 << 16;
+
+enum ScanMode {
+    PASS1_TWO = 0,
+    PASS1_SINGLE = 1,
+    PASS1_FOF = 2,
+    PASS2_TWO = 3,
+    PASS2_SINGLE = 4,
+    PASS2_FOF = 5,
+    WINDOWS = 6
+};
+
+struct ScanArgs {
+    const uint32_t *words;
+    const uint64_t *off;
+    const uint32_t *dataset;
+    const uint64_t *tile_start;
+    const uint32_t *tile_frag;
+    const uint32_t *tile_len;
+    uint64_t tile_begin, tile_end;
+    uint64_t n_frag;
+    int k, m, w;
+    uint32_t x;
+    uint32_t sat;
+    uint32_t p_int;
+    uint32_t wrec;
+    uint32_t *mtab;
+    uint32_t *hist;
+    unsigned long long *cursor;
+    ulonglong2 *records;
+    uint32_t *win_out;
+    unsigned long long *stats;  // [0] pieces [1] records
+};
+
+__device__ __forceinline__ uint32_t lds_code(const uint32_t *sw, int i) {
+    return (sw[i >> 4] >> ((i & 15) * 2)) & 3u;
+}
+
+__device__ __forceinline__ uint64_t high_bit_mask(int nbases) {
+    // bit 2j+1 set for every base j < nbases (the T/G bit of a packed code)
+    const uint64_t all = 0xAAAAAAAAAAAAAAAAull;
+    return nbases >= 32 ? all : (all & ((1ull << (2 * nbases)) - 1ull));
+}
+
+// Exclusive block scan of one int per thread (SCAN_THREADS threads).
+__device__ __forceinline__ int block_exclusive_scan(int v, int *total, int *s_wsum) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wsum[wid] = x;
+    __syncthreads();
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_THREADS / 64; i++) {
+        int s = s_wsum[i];
+        pre += (i < wid) ? s : 0;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + x - v;
+}
+
+// One tile = up to TILE consecutive base positions of the segmented stream,
+// clipped to its input chunk. For every window (k-mer start) p of the tile:
+// canonical ntHash of each m-mer (rolling), #TG strand vote over the k bases,
+// leftmost / rightmost minimum of the upper 16 hash bits over the w m-mers
+// (sm/sliding_min.rs:279-340, sm/canonical.rs:11-29, sm/minimizers.rs:213-216),
+// then runs of equal selected hash (sm/collect.rs:38-75) cut at fragment and
+// tile boundaries ("pieces"). Pieces feed pass 1 or pass 2 by MODE.
+template <int MODE>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
+    __shared__ uint32_t s_words[WORDS_SPAN];
+    __shared__ uint32_t s_hash[TILE + MAXW];
+    __shared__ uint32_t s_v[TILE];
+    __shared__ uint8_t s_flag[TILE];
+    __shared__ int32_t s_fs[TILE + 2];
+    __shared__ uint32_t s_pst[TILE];
+    __shared__ int s_wsum[SCAN_THREADS / 64];
+
+    const int tid = threadIdx.x;
+    const int k = a.k, m = a.m, w = a.w;
+    const uint32_t R1 = (NT_ROT * (uint32_t)m) & 31u;
+    const uint32_t R2 = (NT_ROT * (uint32_t)(m - 1)) & 31u;
+    unsigned long long my_pieces = 0, my_records = 0;
+
+    for (uint64_t t = a.tile_begin + blockIdx.x; t < a.tile_end; t += gridDim.x) {
+        const uint64_t ts = a.tile_start[t];
+        const int tl = (int)a.tile_len[t];
+        const uint64_t f0 = a.tile_frag[t];
+        const int rb0 = (int)(ts & 15u);
+        const int nwords = ((rb0 + tl + 128) >> 4) + 3;
+        __syncthreads();  // previous tile fully consumed
+        for (int i = tid; i < nwords; i += SCAN_THREADS) s_words[i] = a.words[(ts >> 4) + i];
+        // Fragment boundaries relative to ts: s_fs[j] = off[f0 + j] - ts (clamped),
+        // for j = 0 .. nf, nf = number of fragments with a base in [ts, ts + tl).
+        int nf = 0;
+        for (int j0 = 0;; j0 += SCAN_THREADS) {
+            const int j = j0 + tid;
+            bool in = false;
+            if (j <= TILE + 1 && f0 + (uint64_t)j <= a.n_frag) {
+                const int64_t rel = (int64_t)a.off[f0 + j] - (int64_t)ts;
+                s_fs[j] = rel < 0 ? -1 : (rel > (1 << 30) ? (1 << 30) : (int32_t)rel);
+                in = rel < tl;
+            }
+            const int c = __syncthreads_count(in);
+            nf += c;
+            if (c < SCAN_THREADS) break;
+        }
+        __syncthreads();
+
+        // m-mer hashes for relative positions q in [0, tl + w - 1)
+        const int nh = tl + w - 1;
+        for (int q0 = tid * WPT; q0 < nh; q0 += TILE) {
+            const int rb = rb0 + q0;
+            uint32_t fw = 0, rc = 0, rot = 0;
+            for (int j = 0; j < m; j++) {
+                const uint32_t c = lds_code(s_words, rb + j);
+                fw = rotl32(fw, NT_ROT) ^ nt_seed(c);
+                rc ^= rotl32(nt_seed(c ^ 2u), rot);
+                rot = (rot + NT_ROT) & 31u;
+            }
+            s_hash[q0] = fw + rc;
+            for (int i = 1; i < WPT && q0 + i < nh; i++) {
+                const uint32_t co = lds_code(s_words, rb + i - 1);
+                const uint32_t ci = lds_code(s_words, rb + i - 1 + m);
+                fw = rotl32(fw, NT_ROT) ^ rotl32(nt_seed(co), R1) ^ nt_seed(ci);
+                rc = rotr32(rc ^ nt_seed(co ^ 2u), NT_ROT) ^ rotl32(nt_seed(ci ^ 2u), R2);
+                s_hash[q0 + i] = fw + rc;
+            }
+        }
+        __syncthreads();
+
+        // selected hash per window
+        const int r0 = tid * WPT;
+        if (r0 < tl) {
+            int lo = 0, hi = nf - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_fs[mid] <= r0) lo = mid;
+                else hi = mid - 1;
+            }
+            int j = lo;
+            for (int i = 0; i < WPT; i++) {
+                const int r = r0 + i;
+                if (r >= tl) break;
+                while (j + 1 < nf && s_fs[j + 1] <= r) j++;
+                const bool fstart = s_fs[j] == r;
+                const bool valid = r + k <= s_fs[j + 1];
+                uint32_t v = 0;
+                if (valid) {
+                    const uint64_t x0 = extract32_bases(s_words, (uint64_t)(rb0 + r));
+                    uint32_t tg = __popcll(x0 & high_bit_mask(k));
+                    if (k > 32) {
+                        const uint64_t x1 = extract32_bases(s_words, (uint64_t)(rb0 + r + 32));
+                        tg += __popcll(x1 & high_bit_mask(k - 32));
+                    }
+                    uint32_t best = s_hash[r] >> 16;
+                    int lp = r, rp = r;
+                    for (int ii = 1; ii < w; ii++) {
+                        const uint32_t hv = s_hash[r + ii] >> 16;
+                        if (hv < best) {
+                            best = hv;
+                            lp = rp = r + ii;
+                        } else if (hv == best) {
+                            rp = r + ii;
+                        }
+                    }
+                    v = (2u * tg > (uint32_t)k) ? s_hash[lp] : s_hash[rp];
+                }
+                s_v[r] = v;
+                s_flag[r] = (uint8_t)((valid ? 1 : 0) | (fstart ? 2 : 0));
+            }
+        }
+        __syncthreads();
+
+        if constexpr (MODE == WINDOWS) {
+            for (int r = tid; r < tl; r += SCAN_THREADS)
+                if (s_flag[r] & 1) a.win_out[ts + (uint64_t)r] = s_v[r];
+            continue;
+        }
+
+        // piece starts -> compacted list
+        uint32_t smask = 0;
+        int cnt = 0;
+        for (int i = 0; i < WPT; i++) {
+            const int r = r0 + i;
+            if (r >= tl) break;
+            const uint8_t f = s_flag[r];
+            if ((f & 1) && (r == 0 || (f & 2) || !(s_flag[r - 1] & 1) || s_v[r - 1] != s_v[r])) {
+                smask |= 1u << i;
+                cnt++;
+            }
+        }
+        int np = 0;
+        int o = block_exclusive_scan(cnt, &np, s_wsum);
+        for (int i = 0; i < WPT; i++)
+            if (smask & (1u << i)) s_pst[o++] = (uint32_t)(r0 + i);
+        __syncthreads();
+
+        for (int pi = tid; pi < np; pi += SCAN_THREADS) {
+            const int s = (int)s_pst[pi];
+            int e = (pi + 1 < np) ? (int)s_pst[pi + 1] : tl;
+            int lo = 0, hi = nf - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_fs[mid] <= s) lo = mid;
+                else hi = mid - 1;
+            }
+            const int vend = s_fs[lo + 1] - k + 1;
+            if (vend < e) e = vend;
+            const uint32_t v = s_v[s];
+            const uint32_t key = spread_hash_u32(v);          // visit_direct_hash_runs
+            const uint32_t q = spread_hash_u32(key) % a.p_int;  // minimizer_shard
+            const uint32_t len = (uint32_t)(e - s);
+            const uint32_t nrec = (len + a.wrec - 1) / a.wrec;
+            my_pieces++;
+            uint32_t ds = 0;
+            if constexpr (MODE == PASS1_FOF || MODE == PASS2_FOF) ds = a.dataset[f0 + lo];
+            if constexpr (MODE == PASS1_TWO) {
+                // M[key] = min(sat, M[key] + min(len, 255)) in a u8 lane
+                // (CompactMinimizerTable::add, src/lib.rs:4218-4236)
+                const uint32_t delta = len < 255u ? len : 255u;
+                uint32_t *wp = a.mtab + (key >> 2);
+                const uint32_t sh = (key & 3u) * 8u;
+                uint32_t old = __hip_atomic_load(wp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (;;) {
+                    const uint32_t b = (old >> sh) & 255u;
+                    if (b >= a.sat) break;
+                    const uint32_t nb = min(b + delta, a.sat);
+                    const uint32_t nw = (old & ~(255u << sh)) | (nb << sh);
+                    const uint32_t prev = atomicCAS(wp, old, nw);
+                    if (prev == old) break;
+                    old = prev;
+                }
+                atomicAdd(&a.hist[q], nrec);
+            } else if constexpr (MODE == PASS1_FOF) {
+                // per-key dataset mask (one bit per dataset: count at most once)
+                const uint32_t bit = 1u << ds;
+                const uint32_t cur = __hip_atomic_load(&a.mtab[key], __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+                if (!(cur & bit)) atomicOr(&a.mtab[key], bit);
+                atomicAdd(&a.hist[q], nrec);
+            } else if constexpr (MODE == PASS1_SINGLE) {
+                atomicAdd(&a.hist[q], nrec);
+            } else {
+                bool keep = true;
+                if constexpr (MODE == PASS2_TWO) {
+                    const uint32_t b = (a.mtab[key >> 2] >> ((key & 3u) * 8u)) & 255u;
+                    keep = b > a.x;  // MinimizerCounts::is_above_threshold
+                } else if constexpr (MODE == PASS2_FOF) {
+                    keep = (uint32_t)__popc(a.mtab[key]) > a.x;
+                }
+                if (keep) {
+                    for (int cs = s; cs < e; cs += (int)a.wrec) {
+                        const int ce = min(cs + (int)a.wrec, e);
+                        const uint32_t L = (uint32_t)(ce - cs + k - 1);
+                        uint64_t lo64 = extract32_bases(s_words, (uint64_t)(rb0 + cs));
+                        uint64_t hi64 = 0;
+                        if (L < 32) lo64 &= (1ull << (2 * L)) - 1ull;
+                        if (L > 32) {
+                            hi64 = extract32_bases(s_words, (uint64_t)(rb0 + cs + 32));
+                            hi64 &= (1ull << (2 * (L - 32))) - 1ull;
+                        }
+                        hi64 |= ((uint64_t)L << 48) | ((uint64_t)ds << 56);
+                        const unsigned long long pos = atomicAdd(&a.cursor[q], 1ull);
+                        a.records[pos] = make_ulonglong2(lo64, hi64);
+                        my_records++;
+                    }
+                }
+            }
+        }
+    }
+    if (my_pieces) atomicAdd(&a.stats[0], my_pieces);
+    if (my_records) atomicAdd(&a.stats[1], my_records);
+}
+
+// ---------------------------------------------------------------------------
+struct CountArgs {
+    const ulonglong2 *records;
+    const unsigned long long *base;
+    const unsigned long long *cursor;
+    uint32_t p_int;
+    int k;
+    uint32_t x;
+    unsigned long long *ovf_keys;
+    uint32_t *ovf_vals;
+    uint32_t *ovf_part;
+    uint64_t ovf_mask;
+    unsigned long long *out_kmer;
+    uint32_t *out_count;
+    uint32_t *out_part;
+    uint64_t out_cap;
+    // [0] work queue [1] out cursor [2] distinct (LDS) [3] distinct (HBM) [4] error bits
+    unsigned long long *ctr;
+};
+
+template <int MODE>
+__device__ __forceinline__ void val_update(uint32_t *p, uint32_t v) {
+    if constexpr (MODE == 0) atomicAdd(p, v);
+    else atomicOr(p, v);
+}
+
+template <int MODE>
+__device__ void overflow_insert(const CountArgs &a, unsigned long long key, uint32_t val,
+                                uint32_t q) {
+    uint64_t slot = spread_hash_u64(key ^ 0x9e3779b97f4a7c15ull) & a.ovf_mask;
+    for (int p = 0; p < OVF_MAXPROBE; p++) {
+        const unsigned long long cur =
+            __hip_atomic_load(&a.ovf_keys[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == key) {
+            val_update<MODE>(&a.ovf_vals[slot], val);
+            return;
+        }
+        if (cur == EMPTY_KEY) {
+            const unsigned long long prev = atomicCAS(&a.ovf_keys[slot], EMPTY_KEY, key);
+            if (prev == EMPTY_KEY) {
+                a.ovf_part[slot] = q;
+                atomicAdd(&a.ctr[3], 1ull);
+                val_update<MODE>(&a.ovf_vals[slot], val);
+                return;
+            }
+            if (prev == key) {
+                val_update<MODE>(&a.ovf_vals[slot], val);
+                return;
+            }
+        }
+        slot = (slot + 1) & a.ovf_mask;
+    }
+    atomicOr(&a.ctr[4], 1ull);  // spill table full: host retries with more partitions
+}
+
+// Wave-aggregated append: returns the output slot of this lane (if pred).
+__device__ __forceinline__ unsigned long long wave_append(bool pred, unsigned long long *cursor) {
+    const uint64_t mask = __ballot(pred);
+    const int lane = threadIdx.x & 63;
+    unsigned long long base = 0;
+    if (mask) {
+        const int leader = __ffsll((long long)mask) - 1;
+        if (lane == leader) base = atomicAdd(cursor, (unsigned long long)__popcll(mask));
+        base = __shfl(base, leader, 64);
+    }
+    return base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// Per partition: all k-mers of its super-k-mer records into an LDS open-
+// addressing table (CNT_SLOTS slots, linear probing, insert-only); a k-mer
+// whose probe window is full goes to the HBM spill table instead, so each
+// k-mer lives in exactly one of the two. Then count > x (MODE 0: counts,
+// src/lib.rs:2417 / 2502) or popcount(mask) > x (MODE 1: dataset presence,
+// src/lib.rs:2603) is emitted. add_packed_superkmer_kmer_counts
+// (src/lib.rs:3589-3614) gives the canonical k-mer roll.
+template <int MODE>
+__global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
+    __shared__ unsigned long long s_keys[CNT_SLOTS];
+    __shared__ uint32_t s_vals[CNT_SLOTS];
+    __shared__ uint32_t s_q;
+    const int tid = threadIdx.x;
+    const int k = a.k;
+    const uint64_t kmask = (k >= 32) ? ~0ull : ((1ull << (2 * k)) - 1ull);
+    const int hs = 2 * (k - 1);
+    unsigned long long my_distinct = 0;
+    for (;;) {
+        if (tid == 0) s_q = (uint32_t)atomicAdd(&a.ctr[0], 1ull);
+        for (int i = tid; i < CNT_SLOTS; i += CNT_THREADS) {
+            s_keys[i] = EMPTY_KEY;
+            s_vals[i] = 0;
+        }
+        __syncthreads();
+        const uint32_t q = s_q;
+        if (q >= a.p_int) break;
+        const uint64_t rb = a.base[q], re = a.cursor[q];
+        for (uint64_t ri = rb + tid; ri < re; ri += CNT_THREADS) {
+            const ulonglong2 r = a.records[ri];
+            const uint32_t L = (uint32_t)(r.y >> 48) & 255u;
+            const uint32_t val = (MODE == 0) ? 1u : (1u << (uint32_t)(r.y >> 56));
+            uint64_t fwd = 0, rev = 0;
+            for (uint32_t i = 0; i < L; i++) {
+                const uint32_t code = (i < 32) ? (uint32_t)(r.x >> (2 * i)) & 3u
+                                               : (uint32_t)(r.y >> (2 * (i - 32))) & 3u;
+                const uint64_t b = mc_bits(code);
+                fwd = ((fwd << 2) | b) & kmask;
+                rev = (rev >> 2) | ((b ^ 3ull) << hs);
+                if (i + 1 < (uint32_t)k) continue;
+                const unsigned long long key = fwd < rev ? fwd : rev;
+                uint32_t slot = (uint32_t)(spread_hash_u64(key) >> 32) & (CNT_SLOTS - 1);
+                bool done = false;
+                for (int p = 0; p < CNT_MAXPROBE; p++) {
+                    const unsigned long long cur = __hip_atomic_load(
+                        &s_keys[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (cur == key) {
+                        val_update<MODE>(&s_vals[slot], val);
+                        done = true;
+                        break;
+                    }
+                    if (cur == EMPTY_KEY) {
+                        const unsigned long long prev = atomicCAS(&s_keys[slot], EMPTY_KEY, key);
+                        if (prev == EMPTY_KEY || prev == key) {
+                            val_update<MODE>(&s_vals[slot], val);
+                            done = true;
+                            break;
+                        }
+                    }
+                    slot = (slot + 1) & (CNT_SLOTS - 1);
+                }
+                if (!done) overflow_insert<MODE>(a, key, val, q);
+            }
+        }
+        __syncthreads();
+        for (int i0 = 0; i0 < CNT_SLOTS; i0 += CNT_THREADS) {
+            const int i = i0 + tid;
+            const unsigned long long key = s_keys[i];
+            uint32_t c = 0;
+            if (key != EMPTY_KEY) {
+                my_distinct++;
+                c = (MODE == 0) ? s_vals[i] : (uint32_t)__popc(s_vals[i]);
+            }
+            const bool emit = key != EMPTY_KEY && c > a.x;
+            const unsigned long long pos = wave_append(emit, &a.ctr[1]);
+            if (emit && pos < a.out_cap) {
+                a.out_kmer[pos] = key;
+                a.out_count[pos] = c;
+                a.out_part[pos] = q;
+            }
+        }
+        __syncthreads();
+    }
+    if (my_distinct) atomicAdd(&a.ctr[2], my_distinct);
+}
+
+template <int MODE>
+__global__ void k_overflow_emit(CountArgs a) {
+    const uint64_t n = a.ovf_mask + 1;
+    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < n; i0 += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = i0 + threadIdx.x;
+        unsigned long long key = EMPTY_KEY;
+        uint32_t c = 0;
+        if (i < n) {
+            key = a.ovf_keys[i];
+            if (key != EMPTY_KEY) c = (MODE == 0) ? a.ovf_vals[i] : (uint32_t)__popc(a.ovf_vals[i]);
+        }
+        const bool emit = key != EMPTY_KEY && c > a.x;
+        const unsigned long long pos = wave_append(emit, &a.ctr[1]);
+        if (emit && pos < a.out_cap) {
+            a.out_kmer[pos] = key;
+            a.out_count[pos] = c;
+            a.out_part[pos] = a.ovf_part[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+// 16-byte stores for large clears (n16 = number of 16-byte units)
+__global__ void k_fill_zero16(uint4 *p, uint64_t n16) {
+    const uint4 z = make_uint4(0, 0, 0, 0);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = z;
+}
+
+// Exclusive scan of per-partition record counts -> bucket bases, cursors.
+constexpr int PSCAN_THREADS = 1024;
+__global__ __launch_bounds__(PSCAN_THREADS) void k_partition_scan(const uint32_t *hist, uint32_t n,
+                                                                  unsigned long long *base,
+                                                                  unsigned long long *cursor,
+                                                                  unsigned long long *total) {
+    __shared__ unsigned long long s_sum[PSCAN_THREADS];
+    const int tid = threadIdx.x;
+    const uint32_t per = (n + PSCAN_THREADS - 1) / PSCAN_THREADS;
+    const uint32_t b = min(n, tid * per), e = min(n, b + per);
+    unsigned long long s = 0;
+    for (uint32_t i = b; i < e; i++) s += hist[i];
+    s_sum[tid] = s;
+    __syncthreads();
+    for (int o = 1; o < PSCAN_THREADS; o <<= 1) {
+        unsigned long long y = (tid >= o) ? s_sum[tid - o] : 0ull;
+        __syncthreads();
+        s_sum[tid] += y;
+        __syncthreads();
+    }
+    unsigned long long run = s_sum[tid] - s;
+    for (uint32_t i = b; i < e; i++) {
+        base[i] = run;
+        cursor[i] = run;
+        run += hist[i];
+    }
+    if (tid == PSCAN_THREADS - 1) {
+        base[n] = s_sum[tid];
+        *total = s_sum[tid];
+    }
+}
+
+// KFF u8 order key: (encoded & 255, encoded >> 8) as one 64-bit key (k <= 32).
+__global__ void k_kff_keys(const unsigned long long *kmer, const uint32_t *part, uint64_t n,
+                           uint32_t p_user, unsigned long long *key2, uint32_t *pu,
+                           uint32_t *idx) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const unsigned long long K = kmer[i];
+        key2[i] = ((K & 255ull) << 56) | (K >> 8);
+        pu[i] = part[i] % p_user;
+        idx[i] = (uint32_t)i;
+    }
+}
+__global__ void k_gather_u32(const uint32_t *src, const uint32_t *idx, uint32_t *dst, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[idx[i]];
+}
+__global__ void k_gather_kmer(const unsigned long long *ks, const uint32_t *cs, const uint32_t *idx,
+                              unsigned long long *kd, uint32_t *cd, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t j = idx[i];
+        kd[i] = ks[j];
+        cd[i] = cs[j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+struct McPlan {
+    uint32_t k = 0;
+    // tile schedule
+    DevBuf tile_start, tile_frag, tile_len;
+    uint64_t n_tiles = 0;
+    std::vector<uint64_t> chunk_tile;  // tile index where each chunk starts
+    uint64_t n_windows = 0;
+    // state
+    DevBuf mtab;      // two-pass: 2^32 u8; fof: 2^32 u32
+    size_t mtab_words = 0;
+    DevBuf hist, base, cursor, records, ctr, ovf_keys, ovf_vals, ovf_part;
+    DevBuf out_kmer, out_count, out_part;
+    DevBuf sort_tmp, key2, pu, idx, idx2, pu2, k2out;
+    uint64_t ovf_slots = 0;
+    uint32_t p_int = 0;
+    uint64_t rec_cap = 0, out_cap = 0;
+    uint64_t rec_sig = 0;
+    double target_override = 0;
+    // graph cache
+    hipGraphExec_t graph = nullptr;
+    uint64_t graph_sig = 0;
+    ~McPlan() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+    }
+};
+
+void free_plan(McPlan *p) { delete p; }
+
+static unsigned grid_for(uint64_t n, unsigned threads, unsigned max_blocks) {
+    uint64_t b = (n + threads - 1) / threads;
+    if (b < 1) b = 1;
+    if (b > max_blocks) b = max_blocks;
+    return (unsigned)b;
+}
+
+static void fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, p, n, v);
+}
+static void fill_u64(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, p, n, v);
+}
+static void fill_zero_bytes(void *p, uint64_t bytes, hipStream_t s) {
+    // bytes is a multiple of 16 for every caller
+    uint64_t n16 = bytes / 16;
+    if (!n16) return;
+    hipLaunchKernelGGL(k_fill_zero16, dim3(grid_for(n16, 256, 16384)), dim3(256), 0, s,
+                       (uint4 *)p, n16);
+}
+
+std::unique_ptr<Batch> make_batch(HostReads &&r) {
+    r.finish();
+    auto b = std::make_unique<Batch>();
+    Device &d = Device::get();
+    b->n_bases = r.n_bases();
+    b->n_frag = r.n_frag();
+    b->n_words = r.words.size();
+    b->chunk_frag = r.chunk_frag;
+    b->host_off = r.off;
+    b->n_datasets = r.n_datasets;
+    b->words.reset(r.words.size() * 4);
+    b->off.reset(r.off.size() * 8);
+    MCZI_HIP(hipMemcpyAsync(b->words.as<void>(), r.words.data(), r.words.size() * 4,
+                            hipMemcpyHostToDevice, d.stream()));
+    MCZI_HIP(hipMemcpyAsync(b->off.as<void>(), r.off.data(), r.off.size() * 8,
+                            hipMemcpyHostToDevice, d.stream()));
+    if (!r.dataset.empty()) {
+        b->has_dataset = true;
+        b->dataset.reset(r.dataset.size() * 4);
+        MCZI_HIP(hipMemcpyAsync(b->dataset.as<void>(), r.dataset.data(), r.dataset.size() * 4,
+                                hipMemcpyHostToDevice, d.stream()));
+    }
+    MCZI_HIP(hipStreamSynchronize(d.stream()));
+    return b;
+}
+
+// Tile schedule: tiles of TILE positions, clipped at chunk ends, each with
+// the fragment holding its first position.
+static void build_tiles(Batch &b, McPlan &p, uint32_t k) {
+    std::vector<uint64_t> ts;
+    std::vector<uint32_t> tf, tl;
+    p.chunk_tile.clear();
+    const auto &off = b.host_off;
+    p.n_windows = 0;
+    for (size_t i = 0; i + 1 < off.size(); i++) {
+        uint64_t n = off[i + 1] - off[i];
+        if (n >= k) p.n_windows += n - k + 1;
+    }
+    for (size_t c = 0; c + 1 < b.chunk_frag.size(); c++) {
+        p.chunk_tile.push_back(ts.size());
+        uint64_t f = b.chunk_frag[c], fe = b.chunk_frag[c + 1];
+        uint64_t cs = off[f], ce = off[fe];
+        for (uint64_t s = cs; s < ce; s += TILE) {
+            while (f + 1 < fe && off[f + 1] <= s) f++;
+            ts.push_back(s);
+            tf.push_back((uint32_t)f);
+            tl.push_back((uint32_t)std::min<uint64_t>(TILE, ce - s));
+        }
+    }
+    p.chunk_tile.push_back(ts.size());
+    require(b.n_frag < (1ull << 32), MCZI_E_INVALID, "too many fragments in one batch");
+    p.n_tiles = ts.size();
+    Device &d = Device::get();
+    p.tile_start.reset(std::max<size_t>(ts.size(), 1) * 8);
+    p.tile_frag.reset(std::max<size_t>(tf.size(), 1) * 4);
+    p.tile_len.reset(std::max<size_t>(tl.size(), 1) * 4);
+    if (!ts.empty()) {
+        MCZI_HIP(hipMemcpyAsync(p.tile_start.as<void>(), ts.data(), ts.size() * 8,
+                                hipMemcpyHostToDevice, d.stream()));
+        MCZI_HIP(hipMemcpyAsync(p.tile_frag.as<void>(), tf.data(), tf.size() * 4,
+                                hipMemcpyHostToDevice, d.stream()));
+        MCZI_HIP(hipMemcpyAsync(p.tile_len.as<void>(), tl.data(), tl.size() * 4,
+                                hipMemcpyHostToDevice, d.stream()));
+    }
+    MCZI_HIP(hipStreamSynchronize(d.stream()));
+    p.k = k;
+}
+
+static uint64_t next_pow2(uint64_t v) {
+    uint64_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// Autotuned choice: internal partition count = P_user * 2^j so that the
+// expected k-mer instances per partition stay near `target` (the LDS table
+// holds CNT_SLOTS distinct k-mers); partition q maps to user partition
+// q % P_user, so the reference's output order is kept.
+static uint32_t choose_partitions(uint64_t n_windows, uint32_t p_user, double target) {
+    uint64_t p = p_user;
+    while ((double)n_windows / (double)p > target && p * 2 <= (1ull << 24)) p *= 2;
+    return (uint32_t)p;
+}
+
+static double tuned_target() {
+    const char *e = getenv("MCZI_PART_TARGET");
+    if (e && *e) return std::max(64.0, atof(e));
+    return 2.0 * CNT_SLOTS;
+}
+
+struct Timer {
+    hipEvent_t e[12];
+    int n = 0;
+    bool on;
+    explicit Timer(bool enabled) : on(enabled) {
+        if (on)
+            for (auto &x : e) MCZI_HIP(hipEventCreate(&x));
+    }
+    ~Timer() {
+        if (on)
+            for (auto &x : e) (void)hipEventDestroy(x);
+    }
+    void mark(hipStream_t s) {
+        if (on && n < 12) MCZI_HIP(hipEventRecord(e[n++], s));
+    }
+    double ms(int a, int b) {
+        if (!on || b >= n) return 0;
+        float v = 0;
+        MCZI_HIP(hipEventElapsedTime(&v, e[a], e[b]));
+        return v;
+    }
+};
+
+// Kernel arguments of one pipeline run (pointers depend on the plan's buffers).
+static ScanArgs scan_args(Batch &b, McPlan &p, const mczi_counter_config &cfg) {
+    const int k = (int)cfg.k, m = (int)cfg.minimizer;
+    ScanArgs sa{};
+    sa.words = b.words.as<uint32_t>();
+    sa.off = b.off.as<uint64_t>();
+    sa.dataset = b.has_dataset ? b.dataset.as<uint32_t>() : nullptr;
+    sa.tile_start = p.tile_start.as<uint64_t>();
+    sa.tile_frag = p.tile_frag.as<uint32_t>();
+    sa.tile_len = p.tile_len.as<uint32_t>();
+    sa.n_frag = b.n_frag;
+    sa.k = k;
+    sa.m = m;
+    sa.w = k - m + 1;
+    sa.x = (uint32_t)cfg.threshold;
+    sa.sat = (uint32_t)std::min<uint64_t>(cfg.threshold + 1, 255);
+    sa.p_int = p.p_int;
+    sa.wrec = (uint32_t)(REC_MAX_BASES - k + 1);
+    sa.mtab = p.mtab.as<uint32_t>();
+    sa.hist = p.hist.as<uint32_t>();
+    sa.cursor = p.cursor.as<unsigned long long>();
+    sa.records = p.records.as<ulonglong2>();
+    sa.stats = p.ctr.as<unsigned long long>() + 8;
+    return sa;
+}
+
+static void launch_scan(McPlan &p, const ScanArgs &sa, int mode_kernel, int launch, hipStream_t s) {
+    Device &d = Device::get();
+    const unsigned scan_grid_max = (unsigned)d.cu_count() * 8;
+    auto go = [&](uint64_t tb, uint64_t te) {
+        if (te <= tb) return;
+        ScanArgs ca = sa;
+        ca.tile_begin = tb;
+        ca.tile_end = te;
+        const unsigned g = (unsigned)std::min<uint64_t>(te - tb, scan_grid_max);
+        switch (mode_kernel) {
+        case PASS1_TWO: hipLaunchKernelGGL(k_scan<PASS1_TWO>, dim3(g), dim3(SCAN_THREADS), 0, s, ca); break;
+        case PASS1_SINGLE: hipLaunchKernelGGL(k_scan<PASS1_SINGLE>, dim3(g), dim3(SCAN_THREADS), 0, s, ca); break;
+        case PASS1_FOF: hipLaunchKernelGGL(k_scan<PASS1_FOF>, dim3(g), dim3(SCAN_THREADS), 0, s, ca); break;
+        case PASS2_TWO: hipLaunchKernelGGL(k_scan<PASS2_TWO>, dim3(g), dim3(SCAN_THREADS), 0, s, ca); break;
+        case PASS2_SINGLE: hipLaunchKernelGGL(k_scan<PASS2_SINGLE>, dim3(g), dim3(SCAN_THREADS), 0, s, ca); break;
+        default: hipLaunchKernelGGL(k_scan<PASS2_FOF>, dim3(g), dim3(SCAN_THREADS), 0, s, ca); break;
+        }
+    };
+    if (launch == MCZI_LAUNCH_PLAIN) {
+        for (size_t c = 0; c + 1 < p.chunk_tile.size(); c++) go(p.chunk_tile[c], p.chunk_tile[c + 1]);
+    } else {
+        go(0, p.n_tiles);
+    }
+}
+
+// Stage A: clears, pass 1 (minimizer counts / dataset masks + per-partition
+// record counts), bucket bases. No host synchronization inside.
+static void enqueue_stage_a(Batch &b, McPlan &p, const mczi_counter_config &cfg, int mode,
+                            int launch, hipStream_t s, Timer *tm) {
+    ScanArgs sa = scan_args(b, p, cfg);
+    unsigned long long *ctr = p.ctr.as<unsigned long long>();
+    fill_u64(ctr, 16, 0, s);
+    fill_u32(sa.hist, p.p_int, 0, s);
+    if (mode != MCZI_MODE_SINGLE_PASS) fill_zero_bytes(p.mtab.as<void>(), p.mtab_words * 4, s);
+    if (tm) tm->mark(s);
+    const int k1 = mode == MCZI_MODE_TWO_PASS ? PASS1_TWO
+                   : mode == MCZI_MODE_SINGLE_PASS ? PASS1_SINGLE : PASS1_FOF;
+    launch_scan(p, sa, k1, launch, s);
+    hipLaunchKernelGGL(k_partition_scan, dim3(1), dim3(PSCAN_THREADS), 0, s, sa.hist, p.p_int,
+                       p.base.as<unsigned long long>(), sa.cursor, ctr + 10);
+    if (tm) tm->mark(s);
+}
+
+static CountArgs count_args(McPlan &p, const mczi_counter_config &cfg) {
+    CountArgs ca{};
+    ca.records = p.records.as<ulonglong2>();
+    ca.base = p.base.as<unsigned long long>();
+    ca.cursor = p.cursor.as<unsigned long long>();
+    ca.p_int = p.p_int;
+    ca.k = (int)cfg.k;
+    ca.x = (uint32_t)cfg.threshold;
+    ca.ovf_keys = p.ovf_keys.as<unsigned long long>();
+    ca.ovf_vals = p.ovf_vals.as<uint32_t>();
+    ca.ovf_part = p.ovf_part.as<uint32_t>();
+    ca.ovf_mask = p.ovf_slots - 1;
+    ca.out_kmer = p.out_kmer.as<unsigned long long>();
+    ca.out_count = p.out_count.as<uint32_t>();
+    ca.out_part = p.out_part.as<uint32_t>();
+    ca.out_cap = p.out_cap;
+    ca.ctr = p.ctr.as<unsigned long long>();
+    return ca;
+}
+
+// Stage B: pass 2 (super-k-mer records into buckets), LDS counting with HBM
+// spill, filter + emit.
+static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg, int mode,
+                            int launch, hipStream_t s, Timer *tm) {
+    Device &d = Device::get();
+    ScanArgs sa = scan_args(b, p, cfg);
+    unsigned long long *ctr = p.ctr.as<unsigned long long>();
+    // pass-2 pieces are counted afresh
+    fill_u64(ctr + 8, 1, 0, s);
+    fill_u64(p.ovf_keys.as<unsigned long long>(), p.ovf_slots, EMPTY_KEY, s);
+    fill_u32(p.ovf_vals.as<uint32_t>(), p.ovf_slots, 0, s);
+    if (tm) tm->mark(s);
+    const int k2 = mode == MCZI_MODE_TWO_PASS ? PASS2_TWO
+                   : mode == MCZI_MODE_SINGLE_PASS ? PASS2_SINGLE : PASS2_FOF;
+    launch_scan(p, sa, k2, launch, s);
+    if (tm) tm->mark(s);
+    CountArgs ca = count_args(p, cfg);
+    const unsigned cgrid = (unsigned)std::min<uint64_t>(p.p_int, (uint64_t)d.cu_count() * 8);
+    const unsigned ogrid = grid_for(p.ovf_slots, 256, 8192);
+    if (mode == MCZI_MODE_DATASETS) {
+        hipLaunchKernelGGL(k_count<1>, dim3(cgrid), dim3(CNT_THREADS), 0, s, ca);
+        if (tm) tm->mark(s);
+        hipLaunchKernelGGL(k_overflow_emit<1>, dim3(ogrid), dim3(256), 0, s, ca);
+    } else {
+        hipLaunchKernelGGL(k_count<0>, dim3(cgrid), dim3(CNT_THREADS), 0, s, ca);
+        if (tm) tm->mark(s);
+        hipLaunchKernelGGL(k_overflow_emit<0>, dim3(ogrid), dim3(256), 0, s, ca);
+    }
+    if (tm) tm->mark(s);
+}
+
+static void sort_output(Batch &b, McPlan &p, uint64_t n, int order, uint32_t p_user, int k,
+                        hipStream_t s) {
+    if (order == MCZI_ORDER_NONE || n < 2) return;
+    require(n < (1ull << 32), MCZI_E_INVALID, "too many output k-mers for one sort");
+    unsigned long long *kin = p.out_kmer.as<unsigned long long>();
+    uint32_t *cin = p.out_count.as<uint32_t>();
+    b.out_kmer.ensure(n * 8);
+    b.out_count.ensure(n * 4);
+    unsigned long long *kout = b.out_kmer.as<unsigned long long>();
+    uint32_t *cout = b.out_count.as<uint32_t>();
+    size_t tmp = 0;
+    if (order == MCZI_ORDER_ENCODED) {
+        const unsigned end_bit = (unsigned)std::min(64, 2 * k);
+        MCZI_HIP(rocprim::radix_sort_pairs(nullptr, tmp, kin, kout, cin, cout, (size_t)n, 0, end_bit, s));
+        p.sort_tmp.ensure(tmp + 16);
+        MCZI_HIP(rocprim::radix_sort_pairs(p.sort_tmp.as<void>(), tmp, kin, kout, cin, cout, (size_t)n,
+                                           0, end_bit, s));
+        return;
+    }
+    // KFF u8 order: (partition % P_user, encoded & 255, encoded); two stable
+    // LSD sorts: by (encoded & 255, encoded >> 8), then by user partition.
+    p.key2.ensure(n * 8);
+    p.k2out.ensure(n * 8);
+    p.pu.ensure(n * 4);
+    p.pu2.ensure(n * 4);
+    p.idx.ensure(n * 4);
+    p.idx2.ensure(n * 4);
+    unsigned g = grid_for(n, 256, 8192);
+    hipLaunchKernelGGL(k_kff_keys, dim3(g), dim3(256), 0, s, kin, p.out_part.as<uint32_t>(), n, p_user,
+                       p.key2.as<unsigned long long>(), p.pu.as<uint32_t>(), p.idx.as<uint32_t>());
+    MCZI_HIP(rocprim::radix_sort_pairs(nullptr, tmp, p.key2.as<unsigned long long>(),
+                                       p.k2out.as<unsigned long long>(), p.idx.as<uint32_t>(),
+                                       p.idx2.as<uint32_t>(), (size_t)n, 0, 64, s));
+    size_t tmp2 = 0;
+    MCZI_HIP(rocprim::radix_sort_pairs(nullptr, tmp2, p.pu2.as<uint32_t>(), p.pu.as<uint32_t>(),
+                                       p.idx2.as<uint32_t>(), p.idx.as<uint32_t>(), (size_t)n, 0, 32, s));
+    p.sort_tmp.ensure(std::max(tmp, tmp2) + 16);
+    MCZI_HIP(rocprim::radix_sort_pairs(p.sort_tmp.as<void>(), tmp, p.key2.as<unsigned long long>(),
+                                       p.k2out.as<unsigned long long>(), p.idx.as<uint32_t>(),
+                                       p.idx2.as<uint32_t>(), (size_t)n, 0, 64, s));
+    hipLaunchKernelGGL(k_gather_u32, dim3(g), dim3(256), 0, s, p.pu.as<uint32_t>(), p.idx2.as<uint32_t>(),
+                       p.pu2.as<uint32_t>(), n);
+    unsigned pbits = 1;
+    while ((1ull << pbits) < p_user) pbits++;
+    MCZI_HIP(rocprim::radix_sort_pairs(p.sort_tmp.as<void>(), tmp2, p.pu2.as<uint32_t>(),
+                                       p.pu.as<uint32_t>(), p.idx2.as<uint32_t>(), p.idx.as<uint32_t>(),
+                                       (size_t)n, 0, pbits, s));
+    hipLaunchKernelGGL(k_gather_kmer, dim3(g), dim3(256), 0, s, kin, cin, p.idx.as<uint32_t>(), kout,
+                       cout, n);
+}
+
+static uint64_t mix_sig(uint64_t h, uint64_t v) {
+    return spread_hash_u64(h ^ (v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2)));
+}
+
+void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch, int order,
+                 mczi_run_stats *stats) {
+    Device &d = Device::get();
+    hipStream_t s = d.stream();
+    const uint32_t k = cfg.k;
+    require(k >= 1 && k <= 32, MCZI_E_INVALID,
+            "the device counting path supports k <= 32 (k-mers are 64-bit on the device)");
+    require(cfg.minimizer >= 1 && cfg.minimizer <= k, MCZI_E_INVALID, "minimizer size must be <= k");
+    require(k - cfg.minimizer + 1 <= MAXW, MCZI_E_INVALID, "window k - m + 1 must be <= 64");
+    require(k % 2 == 1, MCZI_E_INVALID,
+            "k must be odd: the canonical window length l = k must be odd (simd-minimizers)");
+    require(cfg.partition_count > 0, MCZI_E_INVALID, "partition count must be greater than 0");
+    if (mode == MCZI_MODE_DATASETS)
+        require(b.n_datasets <= 32, MCZI_E_INVALID,
+                "the device fof path supports at most 32 datasets per batch");
+    if (!b.plan) b.plan.reset(new McPlan());
+    McPlan &p = *b.plan;
+    if (p.k != k || p.n_tiles == 0) build_tiles(b, p, k);
+    const bool use_graph = launch == MCZI_LAUNCH_BATCHED && getenv("MCZI_NO_GRAPH") == nullptr;
+
+    double target = p.target_override > 0 ? p.target_override : tuned_target();
+    for (int attempt = 0;; attempt++) {
+        p.p_int = choose_partitions(p.n_windows, cfg.partition_count, target);
+        const size_t mtab_words = (mode == MCZI_MODE_DATASETS) ? (1ull << 32) : (1ull << 30);
+        if (mode != MCZI_MODE_SINGLE_PASS) {
+            p.mtab.ensure(mtab_words * 4);
+            p.mtab_words = mtab_words;
+        }
+        p.hist.ensure((size_t)p.p_int * 4 + 64);
+        p.base.ensure(((size_t)p.p_int + 1) * 8);
+        p.cursor.ensure(((size_t)p.p_int + 1) * 8);
+        p.ctr.ensure(16 * 8);
+        const uint64_t ovf = next_pow2(std::max<uint64_t>(p.n_windows / 16, 1 << 20)) << attempt;
+        p.ovf_slots = ovf;
+        p.ovf_keys.ensure(ovf * 8);
+        p.ovf_vals.ensure(ovf * 4);
+        p.ovf_part.ensure(ovf * 4);
+        p.out_cap = p.n_windows / (cfg.threshold + 1) + 16;
+        p.out_kmer.ensure(p.out_cap * 8);
+        p.out_count.ensure(p.out_cap * 4);
+        p.out_part.ensure(p.out_cap * 4);
+
+        // Record capacity = pass-1 total, measured once per (batch, config).
+        uint64_t cfg_sig = mix_sig(mix_sig(mix_sig(k, cfg.minimizer), cfg.threshold), mode);
+        cfg_sig = mix_sig(mix_sig(cfg_sig, p.p_int), launch);
+        if (p.rec_sig != cfg_sig) {
+            if (p.graph) {
+                MCZI_HIP(hipGraphExecDestroy(p.graph));
+                p.graph = nullptr;
+            }
+            enqueue_stage_a(b, p, cfg, mode, launch, s, nullptr);
+            unsigned long long tot = 0;
+            MCZI_HIP(hipMemcpyAsync(&tot, p.ctr.as<unsigned long long>() + 10, 8,
+                                    hipMemcpyDeviceToHost, s));
+            MCZI_HIP(hipStreamSynchronize(s));
+            p.records.ensure((tot + 16) * 16);
+            p.rec_cap = tot + 16;
+            p.rec_sig = cfg_sig;
+        }
+
+        const bool want_times = stats != nullptr && !use_graph;
+        Timer tm(stats != nullptr);
+        auto t0 = std::chrono::steady_clock::now();
+        tm.mark(s);
+        if (use_graph) {
+            uint64_t sig = cfg_sig;
+            for (const DevBuf *db : {&b.words, &b.off, &b.dataset, &p.mtab, &p.hist, &p.base, &p.cursor,
+                                     &p.records, &p.ctr, &p.ovf_keys, &p.ovf_vals, &p.ovf_part,
+                                     &p.out_kmer, &p.out_count, &p.out_part, &p.tile_start})
+                sig = mix_sig(sig, (uint64_t)(uintptr_t)db->as<void>());
+            sig = mix_sig(sig, p.ovf_slots);
+            if (!p.graph || p.graph_sig != sig) {
+                if (p.graph) {
+                    MCZI_HIP(hipGraphExecDestroy(p.graph));
+                    p.graph = nullptr;
+                }
+                hipGraph_t g;
+                MCZI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                enqueue_stage_a(b, p, cfg, mode, launch, s, nullptr);
+                enqueue_stage_b(b, p, cfg, mode, launch, s, nullptr);
+                MCZI_HIP(hipStreamEndCapture(s, &g));
+                MCZI_HIP(hipGraphInstantiate(&p.graph, g, nullptr, nullptr, 0));
+                MCZI_HIP(hipGraphDestroy(g));
+                p.graph_sig = sig;
+            }
+            MCZI_HIP(hipGraphLaunch(p.graph, s));
+        } else {
+            enqueue_stage_a(b, p, cfg, mode, launch, s, want_times ? &tm : nullptr);
+            enqueue_stage_b(b, p, cfg, mode, launch, s, want_times ? &tm : nullptr);
+        }
+        const int t_dev_end = tm.n;
+        tm.mark(s);
+        unsigned long long h[16];
+        MCZI_HIP(hipMemcpyAsync(h, p.ctr.as<void>(), sizeof h, hipMemcpyDeviceToHost, s));
+        MCZI_HIP(hipStreamSynchronize(s));
+        MCZI_HIP(hipGetLastError());
+        if (h[4] != 0 || h[1] > p.out_cap) {
+            require(attempt < 4, MCZI_E_NOMEM, "k-mer spill table overflowed repeatedly");
+            target /= 4;
+            continue;
+        }
+        const uint64_t n_out = h[1];
+        const int tsort0 = tm.n;
+        tm.mark(s);
+        sort_output(b, p, n_out, order, cfg.partition_count, (int)k, s);
+        if (order == MCZI_ORDER_NONE || n_out == 1) {
+            b.out_kmer.ensure(std::max<uint64_t>(n_out, 1) * 8);
+            b.out_count.ensure(std::max<uint64_t>(n_out, 1) * 4);
+            if (n_out) {
+                MCZI_HIP(hipMemcpyAsync(b.out_kmer.as<void>(), p.out_kmer.as<void>(), n_out * 8,
+                                        hipMemcpyDeviceToDevice, s));
+                MCZI_HIP(hipMemcpyAsync(b.out_count.as<void>(), p.out_count.as<void>(), n_out * 4,
+                                        hipMemcpyDeviceToDevice, s));
+            }
+        }
+        tm.mark(s);
+        MCZI_HIP(hipStreamSynchronize(s));
+        auto t1 = std::chrono::steady_clock::now();
+        b.n_out = n_out;
+        b.k = k;
+        mczi_run_stats st{};
+        st.n_bases = b.n_bases;
+        st.n_windows = p.n_windows;
+        st.n_pieces = h[8];
+        st.n_records = h[9];
+        st.n_distinct = h[2] + h[3];
+        st.n_overflow = h[3];
+        st.n_out = n_out;
+        st.partition_bytes = h[9] * 16;
+        st.internal_partitions = p.p_int;
+        st.lds_slots = CNT_SLOTS;
+        if (want_times && t_dev_end >= 7) {
+            // marks: 0 start, 1 clears done, 2 pass1 done, 3 spill clear, 4 pass2, 5 count, 6 emit
+            st.ms_scan1 = tm.ms(1, 2);
+            st.ms_scan2 = tm.ms(3, 4);
+            st.ms_count = tm.ms(4, 6);
+        }
+        st.ms_total = tm.ms(0, t_dev_end);
+        st.ms_sort = tm.ms(tsort0, tsort0 + 1);
+        (void)t0;
+        (void)t1;
+        b.last = st;
+        if (stats) *stats = st;
+        return;
+    }
+}
+
+void batch_download(Batch &b, std::vector<mczi_counted_kmer> &out) {
+    Device &d = Device::get();
+    out.resize(b.n_out);
+    if (!b.n_out) return;
+    std::vector<unsigned long long> ks(b.n_out);
+    std::vector<uint32_t> cs(b.n_out);
+    MCZI_HIP(hipMemcpyAsync(ks.data(), b.out_kmer.as<void>(), b.n_out * 8, hipMemcpyDeviceToHost,
+                            d.stream()));
+    MCZI_HIP(hipMemcpyAsync(cs.data(), b.out_count.as<void>(), b.n_out * 4, hipMemcpyDeviceToHost,
+                            d.stream()));
+    MCZI_HIP(hipStreamSynchronize(d.stream()));
+    for (uint64_t i = 0; i < b.n_out; i++) out[i] = mczi_counted_kmer{ks[i], 0, cs[i]};
+}
+
+Batch::~Batch() = default;
+
+// ---------------------------------------------------------------------------
+// run_hashes boundary: per-window selected hashes of one sequence on the
+// device, deduplicated into runs on the host.
+void device_run_hashes(const uint8_t *packed, uint64_t n_bases, uint32_t k, uint32_t m,
+                       std::vector<uint32_t> &hashes, std::vector<uint32_t> &starts) {
+    hashes.clear();
+    starts.clear();
+    require(k >= 1 && k <= 64 && m >= 1 && m <= k && k - m + 1 <= MAXW, MCZI_E_INVALID,
+            "invalid k / m for the minimizer scan");
+    require(k % 2 == 1, MCZI_E_INVALID, "window length l = k must be odd");
+    if (n_bases < k) return;
+    HostReads r;
+    for (uint64_t i = 0; i < n_bases; i++) r.append_base((packed[i / 4] >> (2 * (i % 4))) & 3u);
+    r.close_fragment(k, 0);
+    auto b = make_batch(std::move(r));
+    McPlan p;
+    build_tiles(*b, p, k);
+    Device &d = Device::get();
+    hipStream_t s = d.stream();
+    const uint64_t nw = n_bases - k + 1;
+    DevBuf win(n_bases * 4 + 64);
+    DevBuf ctr(16 * 8);
+    fill_u64(ctr.as<unsigned long long>(), 16, 0, s);
+    ScanArgs sa{};
+    sa.words = b->words.as<uint32_t>();
+    sa.off = b->off.as<uint64_t>();
+    sa.tile_start = p.tile_start.as<uint64_t>();
+    sa.tile_frag = p.tile_frag.as<uint32_t>();
+    sa.tile_len = p.tile_len.as<uint32_t>();
+    sa.tile_begin = 0;
+    sa.tile_end = p.n_tiles;
+    sa.n_frag = b->n_frag;
+    sa.k = (int)k;
+    sa.m = (int)m;
+    sa.w = (int)(k - m + 1);
+    sa.p_int = 1;
+    sa.wrec = 1;
+    sa.win_out = win.as<uint32_t>();
+    sa.stats = ctr.as<unsigned long long>();
+    hipLaunchKernelGGL(k_scan<WINDOWS>, dim3((unsigned)std::min<uint64_t>(p.n_tiles, 2048)),
+                       dim3(SCAN_THREADS), 0, s, sa);
+    MCZI_HIP(hipGetLastError());
+    std::vector<uint32_t> v(nw);
+    MCZI_HIP(hipMemcpyAsync(v.data(), win.as<void>(), nw * 4, hipMemcpyDeviceToHost, s));
+    MCZI_HIP(hipStreamSynchronize(s));
+    for (uint64_t j = 0; j < nw; j++) {
+        if (j == 0 || v[j] != v[j - 1]) {
+            hashes.push_back(v[j]);
+            starts.push_back((uint32_t)j);
+        }
+    }
+}
+
+}  // namespace mczi

@@ -1,0 +1,278 @@
+// Host end of the path: simplitig FASTA, per-k-mer FASTA and KFF writers.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "internal.hpp"
+
+namespace mczi {
+
+namespace {
+
+typedef unsigned __int128 u128;
+
+inline u128 enc_of(const mczi_counted_kmer &r) { return ((u128)r.hi << 64) | r.lo; }
+inline u128 kmask(uint32_t k) { return k >= 64 ? ~(u128)0 : (((u128)1 << (2 * k)) - 1); }
+
+inline uint64_t mix64(uint64_t v) {
+    v ^= v >> 33;
+    v *= 0xff51afd7ed558ccdull;
+    v ^= v >> 33;
+    v *= 0xc4ceb9fe1a85ec53ull;
+    return v ^ (v >> 33);
+}
+
+// Open-addressing set of 128-bit k-mers with removal (tombstones).
+class KmerSet {
+  public:
+    explicit KmerSet(size_t n) {
+        cap_ = 16;
+        while (cap_ < n * 2 + 16) cap_ <<= 1;
+        keys_.assign(cap_, 0);
+        st_.assign(cap_, 0);
+    }
+    void insert(u128 k) {
+        size_t i = slot(k);
+        size_t tomb = SIZE_MAX;
+        for (;;) {
+            if (st_[i] == 0) {
+                if (tomb != SIZE_MAX) i = tomb;
+                keys_[i] = k;
+                st_[i] = 1;
+                return;
+            }
+            if (st_[i] == 1 && keys_[i] == k) return;
+            if (st_[i] == 2 && tomb == SIZE_MAX) tomb = i;
+            i = (i + 1) & (cap_ - 1);
+        }
+    }
+    bool remove(u128 k) {
+        size_t i = slot(k);
+        for (;;) {
+            if (st_[i] == 0) return false;
+            if (st_[i] == 1 && keys_[i] == k) {
+                st_[i] = 2;
+                return true;
+            }
+            i = (i + 1) & (cap_ - 1);
+        }
+    }
+
+  private:
+    size_t slot(u128 k) const {
+        return (size_t)mix64((uint64_t)k ^ mix64((uint64_t)(k >> 64) + 0x9e3779b97f4a7c15ull)) &
+               (cap_ - 1);
+    }
+    size_t cap_;
+    std::vector<u128> keys_;
+    std::vector<uint8_t> st_;
+};
+
+const char kAscii[4] = {'A', 'C', 'G', 'T'};
+
+class Out {
+  public:
+    explicit Out(const std::string &path) : path_(path) {
+        fp_ = fopen(path.c_str(), "wb");
+        if (!fp_) throw Error(MCZI_E_IO, "failed to create " + path);
+        buf_.reserve(1 << 22);
+    }
+    explicit Out(std::string *sink) : sink_(sink) {}
+    ~Out() {
+        if (fp_) fclose(fp_);
+    }
+    void put(const char *s, size_t n) {
+        if (sink_) {
+            sink_->append(s, n);
+            return;
+        }
+        buf_.append(s, n);
+        if (buf_.size() >= (1u << 22)) flush();
+    }
+    void put(const std::string &s) { put(s.data(), s.size()); }
+    void flush() {
+        if (fp_ && !buf_.empty()) {
+            if (fwrite(buf_.data(), 1, buf_.size(), fp_) != buf_.size())
+                throw Error(MCZI_E_IO, "failed to write " + path_);
+            buf_.clear();
+        }
+    }
+    void finish() {
+        flush();
+        if (fp_) {
+            if (fclose(fp_) != 0) {
+                fp_ = nullptr;
+                throw Error(MCZI_E_IO, "failed to flush " + path_);
+            }
+            fp_ = nullptr;
+        }
+    }
+
+  private:
+    std::string path_;
+    FILE *fp_ = nullptr;
+    std::string *sink_ = nullptr;
+    std::string buf_;
+};
+
+void put_lines(Out &o, const std::vector<uint8_t> &seq) {
+    char line[81];
+    for (size_t i = 0; i < seq.size(); i += 80) {
+        size_t c = std::min<size_t>(80, seq.size() - i);
+        for (size_t j = 0; j < c; j++) line[j] = kAscii[seq[i + j]];
+        line[c] = '\n';
+        o.put(line, c + 1);
+    }
+}
+
+// for_each_simplitig_encoded (src/lib.rs:1478-1533): seeds in ascending order,
+// greedy extension with A, C, G, T (canonical_suffix_extension, 1580-1602),
+// then the same on the reverse complement.
+void simplitigs(Out &o, const std::vector<mczi_counted_kmer> &kmers, uint32_t k, size_t first_index,
+                size_t *last_index) {
+    require(k > 0 && k <= 64, MCZI_E_INVALID, "k must be greater than 0 and <= 64");
+    size_t idx = first_index;
+    if (kmers.empty()) {
+        if (last_index) *last_index = idx;
+        return;
+    }
+    std::vector<u128> seeds(kmers.size());
+    for (size_t i = 0; i < kmers.size(); i++) seeds[i] = enc_of(kmers[i]);
+    if (!std::is_sorted(seeds.begin(), seeds.end())) std::sort(seeds.begin(), seeds.end());
+    seeds.erase(std::unique(seeds.begin(), seeds.end()), seeds.end());
+    KmerSet rem(seeds.size());
+    for (u128 s : seeds) rem.insert(s);
+    const u128 mk = kmask(k), mk1 = kmask(k - 1);
+    const int hs = 2 * (int)(k - 1);
+    std::vector<uint8_t> seq;
+    seq.reserve(1024);
+    char head[96];
+    for (u128 seed : seeds) {
+        if (!rem.remove(seed)) continue;
+        seq.clear();
+        for (uint32_t i = 0; i < k; i++) seq.push_back((uint8_t)((seed >> (2 * (k - 1 - i))) & 3u));
+        for (int pass = 0; pass < 2; pass++) {
+            // rolling forward / reverse-complement encodings of the last k-1 bases
+            u128 f = 0, r = 0;
+            for (size_t i = seq.size() - (k - 1); i < seq.size(); i++) {
+                f = ((f << 2) | seq[i]) & mk1;
+            }
+            for (size_t i = seq.size(); i-- > seq.size() - (k - 1);) r = (r << 2) | (3u - seq[i]);
+            for (;;) {
+                bool ext = false;
+                for (uint32_t b = 0; b < 4; b++) {
+                    const u128 fk = ((f << 2) | b) & mk;
+                    const u128 rk = ((u128)(3u - b) << hs) | r;
+                    const u128 c = fk < rk ? fk : rk;
+                    if (rem.remove(c)) {
+                        seq.push_back((uint8_t)b);
+                        f = fk & mk1;
+                        r = rk >> 2;
+                        ext = true;
+                        break;
+                    }
+                }
+                if (!ext) break;
+            }
+            std::reverse(seq.begin(), seq.end());
+            for (auto &c : seq) c = (uint8_t)(3u - c);
+        }
+        idx++;
+        int hl = snprintf(head, sizeof head, ">MC_simplitig_%zu kmers=%zu\n", idx, seq.size() - k + 1);
+        o.put(head, (size_t)hl);
+        put_lines(o, seq);
+    }
+    if (last_index) *last_index = idx;
+}
+
+}  // namespace
+
+void write_simplitig_fasta(const std::string &path, const std::vector<mczi_counted_kmer> &kmers,
+                           uint32_t k, size_t first_index, size_t *last_index) {
+    Out o(path);
+    simplitigs(o, kmers, k, first_index, last_index);
+    o.finish();
+}
+
+std::string simplitig_fasta_text(const std::vector<mczi_counted_kmer> &kmers, uint32_t k) {
+    std::string s;
+    Out o(&s);
+    simplitigs(o, kmers, k, 0, nullptr);
+    return s;
+}
+
+// write_kmer_fasta_record_u128 (src/lib.rs:1535-1544)
+void write_kmer_fasta(const std::string &path, const std::vector<mczi_counted_kmer> &kmers,
+                      uint32_t k) {
+    Out o(path);
+    std::vector<uint8_t> seq(k);
+    char head[96];
+    for (size_t i = 0; i < kmers.size(); i++) {
+        const u128 e = enc_of(kmers[i]);
+        for (uint32_t j = 0; j < k; j++) seq[j] = (uint8_t)((e >> (2 * (k - 1 - j))) & 3u);
+        int hl = snprintf(head, sizeof head, ">MC_kmer_%zu count=%llu\n", i + 1,
+                          (unsigned long long)kmers[i].count);
+        o.put(head, (size_t)hl);
+        put_lines(o, seq);
+    }
+    o.finish();
+}
+
+// write_kff_header_with_count_size (src/lib.rs:1633-1664) + records
+// (write_packed_dna 3082-3087, write_kff_u8_record_u64 3089-3102,
+// write_kff_u32_record_u64 3104-3117) + "KFF" footer.
+void write_kff(const std::string &path, const std::vector<mczi_counted_kmer> &kmers,
+               const mczi_counter_config &cfg, uint64_t count_size, bool ordered,
+               uint64_t saturate_at) {
+    Out o(path);
+    std::string h = "KFF";
+    h.push_back(1);
+    h.push_back(0);
+    h.push_back(0x1b);
+    h.push_back(1);
+    h.push_back(1);
+    char fb[256];
+    int fl = snprintf(fb, sizeof fb,
+                      "producer=MC\nk=%u\nm=%u\nthreshold=%llu\nfilter=count>threshold\ncanonical=1\n",
+                      cfg.k, cfg.minimizer, (unsigned long long)cfg.threshold);
+    auto be32 = [&](uint32_t v) {
+        for (int i = 3; i >= 0; i--) h.push_back((char)(v >> (8 * i)));
+    };
+    auto be64 = [&](uint64_t v) {
+        for (int i = 7; i >= 0; i--) h.push_back((char)(v >> (8 * i)));
+    };
+    be32((uint32_t)fl);
+    h.append(fb, (size_t)fl);
+    h.push_back('v');
+    be64(6);
+    const char *names[6] = {"k", "max", "data_size", "ordered", "m", "threshold"};
+    const uint64_t vals[6] = {cfg.k, 1, count_size, ordered ? 1u : 0u, cfg.minimizer, cfg.threshold};
+    for (int i = 0; i < 6; i++) {
+        h.append(names[i]);
+        h.push_back('\0');
+        be64(vals[i]);
+    }
+    h.push_back('r');
+    be64(kmers.size());
+    o.put(h);
+    const size_t bytes = (cfg.k + 3) / 4;
+    std::string rec;
+    rec.reserve(1 << 20);
+    for (const auto &r : kmers) {
+        const u128 e = enc_of(r);
+        for (size_t i = 0; i < bytes; i++) rec.push_back((char)(uint8_t)(e >> (8 * (bytes - 1 - i))));
+        uint64_t c = r.count;
+        if (saturate_at && c > saturate_at) c = saturate_at;
+        for (int i = (int)count_size - 1; i >= 0; i--) rec.push_back((char)(uint8_t)(c >> (8 * i)));
+        if (rec.size() >= (1u << 20)) {
+            o.put(rec);
+            rec.clear();
+        }
+    }
+    rec.append("KFF");
+    o.put(rec);
+    o.finish();
+}
+
+}  // namespace mczi

@@ -1,0 +1,1007 @@
+// ZI on gfx950: device-resident canonical k-mer index of the index FASTA,
+// streaming lookup of every query window, absent set per request (sorted,
+// deduplicated), and the serving loop (continuous batching under an HBM
+// budget). Reference: src/zi.rs:114-189 (build_dictionary,
+// collect_query_kmers_absent_from_index, write_novel_simplitigs).
+#include <rocprim/device/device_segmented_radix_sort.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <set>
+
+#include "device_common.hpp"
+#include "zi.hpp"
+
+namespace mczi {
+
+constexpr int ZI_THREADS = 256;
+constexpr int ZI_WPT = 8;
+constexpr int ZI_TILE = ZI_THREADS * ZI_WPT;
+constexpr uint32_t NO_REQ = 0xffffffffu;
+
+struct Key2 {
+    uint64_t lo, hi;
+};
+
+// Canonical MC encoding of the k-mer at base position p (k <= 64).
+// Packed-seq codes -> MC bits (code ^ code>>1), lanes reversed for the
+// forward encoding (first base most significant); the reverse complement is
+// the complemented lanes in stored order (src/zi.rs:224-238).
+__device__ __forceinline__ uint64_t rev2_64(uint64_t x) {
+    x = __builtin_bitreverse64(x);
+    return ((x >> 1) & 0x5555555555555555ull) | ((x & 0x5555555555555555ull) << 1);
+}
+__device__ __forceinline__ uint64_t to_mc(uint64_t x) { return x ^ ((x >> 1) & 0x5555555555555555ull); }
+
+__device__ __forceinline__ uint64_t canon64(const uint32_t *words, uint64_t p, int k) {
+    const uint64_t mask = k >= 32 ? ~0ull : ((1ull << (2 * k)) - 1ull);
+    const uint64_t x = to_mc(extract32_bases(words, p)) & mask;
+    const uint64_t fwd = rev2_64(x) >> (64 - 2 * k);
+    const uint64_t rc = (~x) & mask;
+    return fwd < rc ? fwd : rc;
+}
+
+__device__ __forceinline__ Key2 canon128(const uint32_t *words, uint64_t p, int k) {
+    // bases 0..31 in a, 32..k-1 in b (k > 32)
+    const int kb = k - 32;
+    const uint64_t mb = kb >= 32 ? ~0ull : ((1ull << (2 * kb)) - 1ull);
+    const uint64_t a = to_mc(extract32_bases(words, p));
+    const uint64_t b = to_mc(extract32_bases(words, p + 32)) & mb;
+    // forward: 128-bit lane reversal of (b:a) then >> (128 - 2k)
+    uint64_t fhi = rev2_64(a), flo = rev2_64(b);
+    const int sh = 128 - 2 * k;  // 2..62
+    Key2 f;
+    f.lo = (flo >> sh) | (fhi << (64 - sh));
+    f.hi = fhi >> sh;
+    // reverse complement: complement of (b:a) in stored order
+    Key2 r;
+    r.lo = ~a;
+    r.hi = (~b) & mb;
+    const bool fl = f.hi < r.hi || (f.hi == r.hi && f.lo < r.lo);
+    return fl ? f : r;
+}
+
+__device__ __forceinline__ uint64_t slot_of(uint64_t lo, uint64_t hi) {
+    return spread_hash_u64(lo ^ spread_hash_u64(hi + 0x9e3779b97f4a7c15ull));
+}
+
+struct ZiTileArgs {
+    const uint32_t *words;
+    const uint64_t *off;
+    const uint32_t *frag_req;  // null for the index build
+    const uint64_t *tile_start;
+    const uint32_t *tile_frag;
+    const uint32_t *tile_len;
+    uint64_t n_tiles, n_frag;
+    int k;
+    unsigned long long *tlo;
+    unsigned long long *thi;
+    uint64_t mask;
+    // lookup outputs
+    unsigned long long *seg_cursor;
+    unsigned long long *out_lo;
+    unsigned long long *out_hi;
+    unsigned long long *ctr;  // [0] lookups
+};
+
+__device__ int load_fragments(const ZiTileArgs &a, uint64_t ts, int tl, uint64_t f0, int32_t *s_fs) {
+    int nf = 0;
+    for (int j0 = 0;; j0 += ZI_THREADS) {
+        const int j = j0 + (int)threadIdx.x;
+        bool in = false;
+        if (j <= ZI_TILE + 1 && f0 + (uint64_t)j <= a.n_frag) {
+            const int64_t rel = (int64_t)a.off[f0 + j] - (int64_t)ts;
+            s_fs[j] = rel < 0 ? -1 : (rel > (1 << 30) ? (1 << 30) : (int32_t)rel);
+            in = rel < tl;
+        }
+        const int c = __syncthreads_count(in);
+        nf += c;
+        if (c < ZI_THREADS) break;
+    }
+    __syncthreads();
+    return nf;
+}
+
+__device__ __forceinline__ int frag_of(const int32_t *s_fs, int nf, int r) {
+    int lo = 0, hi = nf - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_fs[mid] <= r) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Index build: insert the canonical k-mer of every window.
+template <bool WIDE>
+__global__ __launch_bounds__(ZI_THREADS) void k_zi_build(ZiTileArgs a) {
+    __shared__ int32_t s_fs[ZI_TILE + 2];
+    for (uint64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
+        const uint64_t ts = a.tile_start[t];
+        const int tl = (int)a.tile_len[t];
+        __syncthreads();
+        const int nf = load_fragments(a, ts, tl, a.tile_frag[t], s_fs);
+        for (int i = 0; i < ZI_WPT; i++) {
+            const int r = i * ZI_THREADS + (int)threadIdx.x;
+            if (r >= tl) continue;
+            const int j = frag_of(s_fs, nf, r);
+            if (r + a.k > s_fs[j + 1]) continue;
+            if constexpr (!WIDE) {
+                const unsigned long long key = canon64(a.words, ts + r, a.k);
+                uint64_t s = slot_of(key, 0) & a.mask;
+                for (;;) {
+                    const unsigned long long cur = a.tlo[s];
+                    if (cur == key) break;
+                    if (cur == EMPTY_KEY) {
+                        const unsigned long long prev = atomicCAS(&a.tlo[s], EMPTY_KEY, key);
+                        if (prev == EMPTY_KEY || prev == key) break;
+                    }
+                    s = (s + 1) & a.mask;
+                }
+            } else {
+                const Key2 key = canon128(a.words, ts + r, a.k);
+                uint64_t s = slot_of(key.lo, key.hi) & a.mask;
+                for (;;) {
+                    const unsigned long long ch = a.thi[s];
+                    if (ch == key.hi && a.tlo[s] == key.lo) break;
+                    if (ch == EMPTY_KEY) {
+                        const unsigned long long prev = atomicCAS(&a.thi[s], EMPTY_KEY, key.hi);
+                        if (prev == EMPTY_KEY) {
+                            a.tlo[s] = key.lo;
+                            break;
+                        }
+                    }
+                    s = (s + 1) & a.mask;
+                }
+            }
+        }
+    }
+}
+
+// Lookup: every window of every query fragment is looked up in the index;
+// absent canonical k-mers are appended to their request's segment.
+template <bool WIDE>
+__global__ __launch_bounds__(ZI_THREADS) void k_zi_lookup(ZiTileArgs a) {
+    __shared__ int32_t s_fs[ZI_TILE + 2];
+    unsigned long long my_lookups = 0;
+    const int lane = threadIdx.x & 63;
+    for (uint64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
+        const uint64_t ts = a.tile_start[t];
+        const int tl = (int)a.tile_len[t];
+        const uint64_t f0 = a.tile_frag[t];
+        __syncthreads();
+        const int nf = load_fragments(a, ts, tl, f0, s_fs);
+        for (int i = 0; i < ZI_WPT; i++) {
+            const int r = i * ZI_THREADS + (int)threadIdx.x;
+            bool absent = false;
+            uint32_t req = NO_REQ;
+            Key2 key{0, 0};
+            if (r < tl) {
+                const int j = frag_of(s_fs, nf, r);
+                req = a.frag_req[f0 + j];
+                if (r + a.k <= s_fs[j + 1] && req != NO_REQ) {
+                    my_lookups++;
+                    if constexpr (!WIDE) {
+                        key.lo = canon64(a.words, ts + r, a.k);
+                        uint64_t s = slot_of(key.lo, 0) & a.mask;
+                        for (;;) {
+                            const unsigned long long cur = a.tlo[s];
+                            if (cur == key.lo) break;
+                            if (cur == EMPTY_KEY) {
+                                absent = true;
+                                break;
+                            }
+                            s = (s + 1) & a.mask;
+                        }
+                    } else {
+                        key = canon128(a.words, ts + r, a.k);
+                        uint64_t s = slot_of(key.lo, key.hi) & a.mask;
+                        for (;;) {
+                            const unsigned long long ch = a.thi[s];
+                            if (ch == EMPTY_KEY) {
+                                absent = true;
+                                break;
+                            }
+                            if (ch == key.hi && a.tlo[s] == key.lo) break;
+                            s = (s + 1) & a.mask;
+                        }
+                    }
+                }
+            }
+            // append absent k-mers, aggregated per request within the wave
+            uint64_t pending = __ballot(absent);
+            while (pending) {
+                const int leader = __ffsll((long long)pending) - 1;
+                const uint32_t rl = __shfl(req, leader, 64);
+                const uint64_t same = __ballot(absent && req == rl);
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(&a.seg_cursor[rl], (unsigned long long)__popcll(same));
+                base = __shfl(base, leader, 64);
+                if (absent && req == rl) {
+                    const unsigned long long pos = base + __popcll(same & ((1ull << lane) - 1ull));
+                    a.out_lo[pos] = key.lo;
+                    if constexpr (WIDE) a.out_hi[pos] = key.hi;
+                }
+                pending &= ~same;
+            }
+        }
+    }
+    if (my_lookups) atomicAdd(&a.ctr[0], my_lookups);
+}
+
+__global__ void k_zi_fill(unsigned long long *p, uint64_t n, unsigned long long v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_zi_copy_u64(const unsigned long long *src, unsigned long long *dst, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+__global__ void k_zi_iota(uint32_t *p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = (uint32_t)i;
+}
+__global__ void k_zi_gather(const unsigned long long *src, const uint32_t *idx, unsigned long long *dst,
+                            const unsigned long long *seg_base, const unsigned long long *seg_end, uint32_t n_seg) {
+    // only positions inside segments hold data
+    for (uint32_t sgi = blockIdx.y; sgi < n_seg; sgi += gridDim.y) {
+        const uint64_t b = seg_base[sgi], e = seg_end[sgi];
+        for (uint64_t i = b + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e;
+             i += (uint64_t)gridDim.x * blockDim.x)
+            dst[i] = src[idx[i]];
+    }
+}
+
+// Ordered per-segment unique (one workgroup per segment).
+template <bool WIDE>
+__global__ __launch_bounds__(ZI_THREADS) void k_zi_seg_unique(const unsigned long long *lo, const unsigned long long *hi,
+                                                             const unsigned long long *seg_base,
+                                                             const unsigned long long *seg_end,
+                                                             unsigned long long *olo, unsigned long long *ohi,
+                                                             unsigned long long *seg_count) {
+    __shared__ int s_w[ZI_THREADS / 64];
+    const uint32_t sg = blockIdx.x;
+    const uint64_t b = seg_base[sg], e = seg_end[sg];
+    uint64_t written = 0;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (uint64_t c = b; c < e; c += ZI_THREADS) {
+        const uint64_t i = c + threadIdx.x;
+        bool f = false;
+        if (i < e) {
+            f = (i == b) || lo[i] != lo[i - 1];
+            if constexpr (WIDE) f = f || (i > b && hi[i] != hi[i - 1]);
+        }
+        const uint64_t m = __ballot(f);
+        const int pre = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) s_w[wid] = __popcll(m);
+        __syncthreads();
+        int off = 0, tot = 0;
+        for (int w = 0; w < ZI_THREADS / 64; w++) {
+            off += (w < wid) ? s_w[w] : 0;
+            tot += s_w[w];
+        }
+        if (f) {
+            olo[b + written + off + pre] = lo[i];
+            if constexpr (WIDE) ohi[b + written + off + pre] = hi[i];
+        }
+        written += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) seg_count[sg] = written;
+}
+
+// ---------------------------------------------------------------------------
+static unsigned zgrid(uint64_t n) {
+    uint64_t b = (n + 255) / 256;
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(b, 8192));
+}
+static void zfill(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_zi_fill, dim3(zgrid(n)), dim3(256), 0, s, p, n, v);
+}
+
+struct HostTiles {
+    std::vector<uint64_t> start;
+    std::vector<uint32_t> frag, len;
+};
+
+static void make_tiles(const std::vector<uint64_t> &off, HostTiles &t) {
+    t.start.clear();
+    t.frag.clear();
+    t.len.clear();
+    const uint64_t nb = off.back();
+    uint64_t f = 0;
+    const uint64_t nfr = off.size() - 1;
+    for (uint64_t s = 0; s < nb; s += ZI_TILE) {
+        while (f + 1 < nfr && off[f + 1] <= s) f++;
+        t.start.push_back(s);
+        t.frag.push_back((uint32_t)f);
+        t.len.push_back((uint32_t)std::min<uint64_t>(ZI_TILE, nb - s));
+    }
+}
+
+void zi_build_index(ZiIndex &idx, HostReads &&reads, uint32_t k, uint32_t m, hipStream_t s) {
+    reads.finish();
+    idx.k = k;
+    idx.m = m;
+    idx.wide = k > 32;
+    uint64_t windows = 0;
+    for (size_t i = 0; i + 1 < reads.off.size(); i++) {
+        const uint64_t n = reads.off[i + 1] - reads.off[i];
+        if (n >= k) windows += n - k + 1;
+    }
+    require(windows > 0, MCZI_E_INVALID, "index FASTA did not contain any sequence of length >= k");
+    require(reads.n_frag() < (1ull << 32), MCZI_E_INVALID, "too many index fragments");
+    uint64_t slots = 1024;
+    while (slots < windows * 2) slots <<= 1;
+    idx.mask = slots - 1;
+    idx.n_kmers = windows;
+    idx.lo.reset(slots * 8);
+    if (idx.wide) idx.hi.reset(slots * 8);
+    zfill(idx.lo.as<unsigned long long>(), slots, EMPTY_KEY, s);
+    if (idx.wide) zfill(idx.hi.as<unsigned long long>(), slots, EMPTY_KEY, s);
+    DevBuf words(reads.words.size() * 4), off(reads.off.size() * 8);
+    MCZI_HIP(hipMemcpyAsync(words.as<void>(), reads.words.data(), reads.words.size() * 4, hipMemcpyHostToDevice, s));
+    MCZI_HIP(hipMemcpyAsync(off.as<void>(), reads.off.data(), reads.off.size() * 8, hipMemcpyHostToDevice, s));
+    HostTiles t;
+    make_tiles(reads.off, t);
+    DevBuf ts(t.start.size() * 8 + 8), tf(t.frag.size() * 4 + 4), tl(t.len.size() * 4 + 4);
+    MCZI_HIP(hipMemcpyAsync(ts.as<void>(), t.start.data(), t.start.size() * 8, hipMemcpyHostToDevice, s));
+    MCZI_HIP(hipMemcpyAsync(tf.as<void>(), t.frag.data(), t.frag.size() * 4, hipMemcpyHostToDevice, s));
+    MCZI_HIP(hipMemcpyAsync(tl.as<void>(), t.len.data(), t.len.size() * 4, hipMemcpyHostToDevice, s));
+    ZiTileArgs a{};
+    a.words = words.as<uint32_t>();
+    a.off = off.as<uint64_t>();
+    a.tile_start = ts.as<uint64_t>();
+    a.tile_frag = tf.as<uint32_t>();
+    a.tile_len = tl.as<uint32_t>();
+    a.n_tiles = t.start.size();
+    a.n_frag = reads.n_frag();
+    a.k = (int)k;
+    a.tlo = idx.lo.as<unsigned long long>();
+    a.thi = idx.wide ? idx.hi.as<unsigned long long>() : nullptr;
+    a.mask = idx.mask;
+    const unsigned g = (unsigned)std::min<uint64_t>(a.n_tiles, (uint64_t)Device::get().cu_count() * 8);
+    if (idx.wide) hipLaunchKernelGGL(k_zi_build<true>, dim3(g), dim3(ZI_THREADS), 0, s, a);
+    else hipLaunchKernelGGL(k_zi_build<false>, dim3(g), dim3(ZI_THREADS), 0, s, a);
+    MCZI_HIP(hipStreamSynchronize(s));
+    MCZI_HIP(hipGetLastError());
+}
+
+struct ZiWork {
+    DevBuf words, off, req, ts, tf, tl, seg_base, seg_cursor, seg_count, olo, ohi, slo, shi, ulo, uhi, idx, idx2,
+        tmp, ctr;
+    uint64_t cap = 0;
+    std::vector<uint64_t> h_base, h_count;
+    uint64_t lookups = 0;
+    hipEvent_t done = nullptr;
+    ~ZiWork() {
+        if (done) (void)hipEventDestroy(done);
+    }
+};
+
+std::unique_ptr<ZiWork> zi_make_work() {
+    auto w = std::make_unique<ZiWork>();
+    MCZI_HIP(hipEventCreateWithFlags(&w->done, hipEventDisableTiming));
+    return w;
+}
+void zi_free_work(ZiWork *w) { delete w; }
+
+uint64_t zi_batch_bytes(uint64_t bases, uint64_t frags, bool wide) {
+    const uint64_t tiles = bases / ZI_TILE + 1;
+    const uint64_t per_key = wide ? 16 * 3 + 8 : 8 * 3;
+    return bases / 4 + 256 + frags * 12 + tiles * 16 + bases * per_key + bases * 8 + (64u << 10);
+}
+
+void zi_enqueue(ZiWork &w, const ZiIndex &idx, const HostReads &q, const std::vector<uint32_t> &frag_req,
+                uint32_t n_req, hipStream_t s) {
+    const bool wide = idx.wide;
+    const uint64_t nfr = q.n_frag();
+    require(nfr < (1ull << 32), MCZI_E_INVALID, "too many query fragments in one batch");
+    // per-request segment bases: capacity = the request's window count
+    std::vector<uint64_t> seg_cap(n_req, 0);
+    for (uint64_t f = 0; f < nfr; f++) {
+        const uint32_t r = frag_req[f];
+        const uint64_t n = q.off[f + 1] - q.off[f];
+        if (r != NO_REQ && n >= idx.k) seg_cap[r] += n - idx.k + 1;
+    }
+    w.h_base.assign(n_req + 1, 0);
+    for (uint32_t r = 0; r < n_req; r++) w.h_base[r + 1] = w.h_base[r] + seg_cap[r];
+    const uint64_t cap = std::max<uint64_t>(w.h_base[n_req], 1);
+    w.cap = cap;
+    w.words.ensure(q.words.size() * 4);
+    w.off.ensure(q.off.size() * 8);
+    w.req.ensure(std::max<uint64_t>(nfr, 1) * 4);
+    MCZI_HIP(hipMemcpyAsync(w.words.as<void>(), q.words.data(), q.words.size() * 4, hipMemcpyHostToDevice, s));
+    MCZI_HIP(hipMemcpyAsync(w.off.as<void>(), q.off.data(), q.off.size() * 8, hipMemcpyHostToDevice, s));
+    if (nfr)
+        MCZI_HIP(hipMemcpyAsync(w.req.as<void>(), frag_req.data(), nfr * 4, hipMemcpyHostToDevice, s));
+    HostTiles t;
+    make_tiles(q.off, t);
+    w.ts.ensure(t.start.size() * 8 + 8);
+    w.tf.ensure(t.frag.size() * 4 + 4);
+    w.tl.ensure(t.len.size() * 4 + 4);
+    if (!t.start.empty()) {
+        MCZI_HIP(hipMemcpyAsync(w.ts.as<void>(), t.start.data(), t.start.size() * 8, hipMemcpyHostToDevice, s));
+        MCZI_HIP(hipMemcpyAsync(w.tf.as<void>(), t.frag.data(), t.frag.size() * 4, hipMemcpyHostToDevice, s));
+        MCZI_HIP(hipMemcpyAsync(w.tl.as<void>(), t.len.data(), t.len.size() * 4, hipMemcpyHostToDevice, s));
+    }
+    w.seg_base.ensure((n_req + 1) * 8);
+    w.seg_cursor.ensure((n_req + 1) * 8);
+    w.seg_count.ensure((n_req + 1) * 8);
+    MCZI_HIP(hipMemcpyAsync(w.seg_base.as<void>(), w.h_base.data(), (n_req + 1) * 8, hipMemcpyHostToDevice, s));
+    MCZI_HIP(hipMemcpyAsync(w.seg_cursor.as<void>(), w.h_base.data(), (n_req + 1) * 8, hipMemcpyHostToDevice, s));
+    w.olo.ensure(cap * 8);
+    w.slo.ensure(cap * 8);
+    w.ulo.ensure(cap * 8);
+    if (wide) {
+        w.ohi.ensure(cap * 8);
+        w.shi.ensure(cap * 8);
+        w.uhi.ensure(cap * 8);
+        w.idx.ensure(cap * 4);
+        w.idx2.ensure(cap * 4);
+    }
+    w.ctr.ensure(64);
+    zfill(w.ctr.as<unsigned long long>(), 8, 0, s);
+
+    ZiTileArgs a{};
+    a.words = w.words.as<uint32_t>();
+    a.off = w.off.as<uint64_t>();
+    a.frag_req = w.req.as<uint32_t>();
+    a.tile_start = w.ts.as<uint64_t>();
+    a.tile_frag = w.tf.as<uint32_t>();
+    a.tile_len = w.tl.as<uint32_t>();
+    a.n_tiles = t.start.size();
+    a.n_frag = nfr;
+    a.k = (int)idx.k;
+    a.tlo = idx.lo.as<unsigned long long>();
+    a.thi = wide ? idx.hi.as<unsigned long long>() : nullptr;
+    a.mask = idx.mask;
+    a.seg_cursor = w.seg_cursor.as<unsigned long long>();
+    a.out_lo = w.olo.as<unsigned long long>();
+    a.out_hi = wide ? w.ohi.as<unsigned long long>() : nullptr;
+    a.ctr = w.ctr.as<unsigned long long>();
+    if (a.n_tiles) {
+        const unsigned g = (unsigned)std::min<uint64_t>(a.n_tiles, (uint64_t)Device::get().cu_count() * 8);
+        if (wide) hipLaunchKernelGGL(k_zi_lookup<true>, dim3(g), dim3(ZI_THREADS), 0, s, a);
+        else hipLaunchKernelGGL(k_zi_lookup<false>, dim3(g), dim3(ZI_THREADS), 0, s, a);
+    }
+    const unsigned long long *sb = w.seg_base.as<unsigned long long>();
+    const unsigned long long *se = w.seg_cursor.as<unsigned long long>();
+    const int kbits = (int)std::min<uint32_t>(64, 2 * idx.k);
+    size_t tmp = 0;
+    if (!wide) {
+        MCZI_HIP(rocprim::segmented_radix_sort_keys(nullptr, tmp, w.olo.as<unsigned long long>(),
+                                                    w.slo.as<unsigned long long>(), (unsigned)cap, n_req, sb, se,
+                                                    0, kbits, s));
+        w.tmp.ensure(tmp + 16);
+        MCZI_HIP(rocprim::segmented_radix_sort_keys(w.tmp.as<void>(), tmp, w.olo.as<unsigned long long>(),
+                                                    w.slo.as<unsigned long long>(), (unsigned)cap, n_req, sb, se,
+                                                    0, kbits, s));
+        hipLaunchKernelGGL(k_zi_seg_unique<false>, dim3(std::max<uint32_t>(n_req, 1)), dim3(ZI_THREADS), 0, s,
+                           w.slo.as<unsigned long long>(), nullptr, sb, se, w.ulo.as<unsigned long long>(), nullptr,
+                           w.seg_count.as<unsigned long long>());
+    } else {
+        // LSD over (lo, hi): stable segmented sorts carrying an index
+        hipLaunchKernelGGL(k_zi_iota, dim3(zgrid(cap)), dim3(256), 0, s, w.idx.as<uint32_t>(), cap);
+        size_t t1 = 0, t2 = 0;
+        MCZI_HIP(rocprim::segmented_radix_sort_pairs(nullptr, t1, w.olo.as<unsigned long long>(),
+                                                     w.slo.as<unsigned long long>(), w.idx.as<uint32_t>(),
+                                                     w.idx2.as<uint32_t>(), (unsigned)cap, n_req, sb, se, 0, 64, s));
+        const int hbits = 2 * (int)idx.k - 64;
+        MCZI_HIP(rocprim::segmented_radix_sort_pairs(nullptr, t2, w.shi.as<unsigned long long>(),
+                                                     w.uhi.as<unsigned long long>(), w.idx2.as<uint32_t>(),
+                                                     w.idx.as<uint32_t>(), (unsigned)cap, n_req, sb, se, 0, hbits, s));
+        w.tmp.ensure(std::max(t1, t2) + 16);
+        MCZI_HIP(rocprim::segmented_radix_sort_pairs(w.tmp.as<void>(), t1, w.olo.as<unsigned long long>(),
+                                                     w.slo.as<unsigned long long>(), w.idx.as<uint32_t>(),
+                                                     w.idx2.as<uint32_t>(), (unsigned)cap, n_req, sb, se, 0, 64, s));
+        dim3 gg(zgrid(cap), std::min<uint32_t>(std::max<uint32_t>(n_req, 1), 64));
+        hipLaunchKernelGGL(k_zi_gather, gg, dim3(256), 0, s, w.ohi.as<unsigned long long>(), w.idx2.as<uint32_t>(),
+                           w.shi.as<unsigned long long>(), sb, se, n_req);
+        MCZI_HIP(rocprim::segmented_radix_sort_pairs(w.tmp.as<void>(), t2, w.shi.as<unsigned long long>(),
+                                                     w.uhi.as<unsigned long long>(), w.idx2.as<uint32_t>(),
+                                                     w.idx.as<uint32_t>(), (unsigned)cap, n_req, sb, se, 0, hbits, s));
+        // original positions of the (hi, lo)-sorted order -> gather lo
+        hipLaunchKernelGGL(k_zi_gather, gg, dim3(256), 0, s, w.olo.as<unsigned long long>(), w.idx.as<uint32_t>(),
+                           w.slo.as<unsigned long long>(), sb, se, n_req);
+        hipLaunchKernelGGL(k_zi_seg_unique<true>, dim3(std::max<uint32_t>(n_req, 1)), dim3(ZI_THREADS), 0, s,
+                           w.slo.as<unsigned long long>(), w.uhi.as<unsigned long long>(), sb, se,
+                           w.ulo.as<unsigned long long>(), w.ohi.as<unsigned long long>(),
+                           w.seg_count.as<unsigned long long>());
+    }
+    MCZI_HIP(hipEventRecord(w.done, s));
+}
+
+void zi_collect(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, ZiQueryResult &out) {
+    MCZI_HIP(hipEventSynchronize(w.done));
+    MCZI_HIP(hipGetLastError());
+    w.h_count.assign(n_req + 1, 0);
+    unsigned long long lk = 0;
+    MCZI_HIP(hipMemcpyAsync(w.h_count.data(), w.seg_count.as<void>(), n_req * 8, hipMemcpyDeviceToHost, s));
+    MCZI_HIP(hipMemcpyAsync(&lk, w.ctr.as<void>(), 8, hipMemcpyDeviceToHost, s));
+    MCZI_HIP(hipStreamSynchronize(s));
+    w.lookups = lk;
+    out.absent.assign(n_req, {});
+    out.windows.assign(n_req, 0);
+    std::vector<std::vector<unsigned long long>> lo(n_req), hi(n_req);
+    for (uint32_t r = 0; r < n_req; r++) {
+        out.windows[r] = w.h_base[r + 1] - w.h_base[r];
+        const uint64_t n = w.h_count[r];
+        lo[r].resize(n);
+        if (n)
+            MCZI_HIP(hipMemcpyAsync(lo[r].data(), w.ulo.as<unsigned long long>() + w.h_base[r], n * 8,
+                                    hipMemcpyDeviceToHost, s));
+        if (idx.wide) {
+            hi[r].resize(n);
+            if (n)
+                MCZI_HIP(hipMemcpyAsync(hi[r].data(), w.ohi.as<unsigned long long>() + w.h_base[r], n * 8,
+                                        hipMemcpyDeviceToHost, s));
+        }
+    }
+    MCZI_HIP(hipStreamSynchronize(s));
+    for (uint32_t r = 0; r < n_req; r++) {
+        auto &v = out.absent[r];
+        v.resize(lo[r].size());
+        for (size_t i = 0; i < v.size(); i++) v[i] = mczi_counted_kmer{lo[r][i], idx.wide ? hi[r][i] : 0, 1};
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Serving loop
+struct ZiRequest {
+    uint64_t id = 0;
+    HostReads reads;
+    bool make_fasta = false;
+    uint64_t bytes = 0;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    bool done = false;
+    int err = 0;
+    std::string err_msg;
+    std::vector<mczi_counted_kmer> absent;
+    std::string fasta;
+    uint64_t windows = 0;
+};
+
+static double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace mczi
+
+using namespace mczi;
+
+struct mczi_zi_server {
+    ZiIndex idx;
+    uint64_t budget = 0, used = 0;
+    uint32_t max_inflight = 2;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<ZiRequest *> queue;
+    std::map<uint64_t, std::unique_ptr<ZiRequest>> reqs;
+    uint64_t next_id = 1;
+    bool stop = false;
+    struct Slot {
+        hipStream_t stream = nullptr;
+        std::unique_ptr<ZiWork> work;
+        std::vector<ZiRequest *> batch;
+        uint64_t bytes = 0;
+        bool busy = false;
+        uint64_t seq = 0;
+    };
+    std::vector<Slot> slots;
+    uint64_t seq_issue = 0, seq_done = 0;
+    std::thread admit_thread, done_thread;
+    uint64_t completed = 0, lookups = 0;
+    double busy_ms = 0;
+    std::string fatal;
+
+    void admit_loop();
+    void done_loop();
+};
+
+void mczi_zi_server::admit_loop() {
+    std::unique_lock<std::mutex> lk(mu);
+    const uint64_t max_batch_bases = 256ull << 20;
+    while (!stop) {
+        int free_slot = -1;
+        for (size_t i = 0; i < slots.size(); i++)
+            if (!slots[i].busy) {
+                free_slot = (int)i;
+                break;
+            }
+        if (queue.empty() || free_slot < 0) {
+            cv.wait(lk);
+            continue;
+        }
+        // admit from the head while memory and the batch size allow
+        std::vector<ZiRequest *> batch;
+        uint64_t bytes = 0, bases = 0, frags = 0;
+        while (!queue.empty()) {
+            ZiRequest *r = queue.front();
+            const uint64_t nb = r->reads.n_bases(), nf = r->reads.n_frag();
+            const uint64_t need = zi_batch_bytes(bases + nb + 16 * (batch.size() + 1), frags + nf + batch.size() + 1,
+                                                 idx.wide);
+            if (zi_batch_bytes(nb + 16, nf + 1, idx.wide) > budget) {
+                // can never fit
+                queue.pop_front();
+                r->err = MCZI_E_NOMEM;
+                r->err_msg = "request needs more device memory than the server's HBM budget";
+                r->done = true;
+                r->t_done = now_ms();
+                cv.notify_all();
+                continue;
+            }
+            if (!batch.empty() && (used + need > budget || bases + nb > max_batch_bases)) break;
+            if (batch.empty() && used + need > budget) break;
+            batch.push_back(r);
+            queue.pop_front();
+            bytes = need;
+            bases += nb;
+            frags += nf;
+        }
+        if (batch.empty()) {
+            cv.wait(lk);
+            continue;
+        }
+        Slot &sl = slots[free_slot];
+        sl.busy = true;
+        sl.batch = batch;
+        sl.bytes = bytes;
+        sl.seq = seq_issue++;
+        used += bytes;
+        const double t = now_ms();
+        for (auto *r : batch) r->t_admit = t;
+        lk.unlock();
+        // concatenate requests at word boundaries (padding = fragment of no request)
+        HostReads q;
+        std::vector<uint32_t> frag_req;
+        for (uint32_t ri = 0; ri < batch.size(); ri++) {
+            HostReads &src = batch[ri]->reads;
+            uint64_t base = q.cur_;
+            if (base % 16) {
+                const uint64_t pad = 16 - base % 16;
+                q.cur_ += pad;
+                q.words.resize(q.cur_ / 16, 0);
+                q.off.push_back(q.cur_);
+                frag_req.push_back(NO_REQ);
+                base = q.cur_;
+            }
+            const uint64_t nw = (src.n_bases() + 15) / 16;
+            q.words.resize(base / 16 + nw, 0);
+            std::copy(src.words.begin(), src.words.begin() + nw, q.words.begin() + base / 16);
+            for (uint64_t f = 0; f < src.n_frag(); f++) {
+                q.off.push_back(base + src.off[f + 1]);
+                frag_req.push_back(ri);
+            }
+            q.cur_ = base + src.n_bases();
+        }
+        q.words.resize(q.cur_ / 16 + 40, 0);
+        int err = 0;
+        std::string msg;
+        try {
+            if (!sl.work) sl.work = zi_make_work();
+            zi_enqueue(*sl.work, idx, q, frag_req, (uint32_t)batch.size(), sl.stream);
+        } catch (const Error &e) {
+            err = e.code;
+            msg = e.what();
+        }
+        lk.lock();
+        if (err) {
+            for (auto *r : batch) {
+                r->err = err;
+                r->err_msg = msg;
+                r->done = true;
+                r->t_done = now_ms();
+            }
+            used -= sl.bytes;
+            sl.busy = false;
+            sl.batch.clear();
+            seq_done = std::max(seq_done, sl.seq + 1);
+        }
+        cv.notify_all();
+    }
+}
+
+void mczi_zi_server::done_loop() {
+    std::unique_lock<std::mutex> lk(mu);
+    while (true) {
+        // next issued batch in order
+        Slot *sl = nullptr;
+        for (auto &s : slots)
+            if (s.busy && s.seq == seq_done && !s.batch.empty()) sl = &s;
+        if (!sl) {
+            if (stop) return;
+            cv.wait(lk);
+            continue;
+        }
+        std::vector<ZiRequest *> batch = sl->batch;
+        lk.unlock();
+        ZiQueryResult res;
+        int err = 0;
+        std::string msg;
+        const double t0 = now_ms();
+        try {
+            zi_collect(*sl->work, idx, (uint32_t)batch.size(), sl->stream, res);
+        } catch (const Error &e) {
+            err = e.code;
+            msg = e.what();
+        }
+        const double t1 = now_ms();
+        for (size_t i = 0; i < batch.size() && !err; i++) {
+            batch[i]->absent = std::move(res.absent[i]);
+            batch[i]->windows = res.windows[i];
+            if (batch[i]->make_fasta) batch[i]->fasta = simplitig_fasta_text(batch[i]->absent, idx.k);
+        }
+        lk.lock();
+        busy_ms += t1 - t0;
+        lookups += sl->work ? sl->work->lookups : 0;
+        const double t = now_ms();
+        for (auto *r : batch) {
+            if (err) {
+                r->err = err;
+                r->err_msg = msg;
+            }
+            r->done = true;
+            r->t_done = t;
+            completed++;
+        }
+        used -= sl->bytes;
+        sl->busy = false;
+        sl->batch.clear();
+        seq_done++;
+        cv.notify_all();
+    }
+}
+
+namespace {
+thread_local std::string z_err;
+
+template <typename F>
+int zguard(const char *ctx, F &&f) {
+    try {
+        f();
+        return MCZI_OK;
+    } catch (const Error &e) {
+        z_err = std::string(ctx) + ": " + e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        z_err = std::string(ctx) + ": " + e.what();
+        return MCZI_E_INVALID;
+    }
+}
+
+void check_k(uint32_t k, uint32_t m) {
+    // validate_k (src/zi.rs:103-107) and the minimizer check (src/zi.rs:70-74)
+    require(k >= 3 && k <= 63, MCZI_E_INVALID, "k must be in [3, 63] for sshash-rs");
+    require(k % 2 == 1, MCZI_E_INVALID, "k must be odd for sshash-rs");
+    require(m > 0 && m < k, MCZI_E_INVALID, "minimizer size must be > 0 and < k");
+}
+
+uint32_t default_m(uint32_t k) {
+    // default_minimizer_size (src/zi.rs:109-112)
+    uint32_t m = std::min<uint32_t>(19, k - 2);
+    return m % 2 == 1 ? m : m - 1;
+}
+
+std::vector<mczi_counted_kmer> zi_run(const char *index_path, const char *const *qp, size_t n, uint32_t k) {
+    Device &d = Device::get();
+    ZiIndex idx;
+    HostReads ir;
+    parse_fastx_file(index_path, 1, 0, ir);
+    zi_build_index(idx, std::move(ir), k, default_m(k), d.stream());
+    HostReads q;
+    for (size_t i = 0; i < n; i++) parse_fastx_file(qp[i], k, 0, q);
+    q.finish();
+    std::vector<uint32_t> fr(q.n_frag(), 0);
+    auto w = zi_make_work();
+    zi_enqueue(*w, idx, q, fr, 1, d.stream());
+    ZiQueryResult res;
+    zi_collect(*w, idx, 1, d.stream(), res);
+    return std::move(res.absent[0]);
+}
+}  // namespace
+
+// The generic error accessor lives in capi.cpp; ZI errors are mirrored there.
+extern "C" void mczi_set_error_from_zi(const char *msg);
+
+extern "C" {
+
+int mczi_zi_subtract(const char *index_path, const char *const *query_paths, size_t n, uint32_t k, uint32_t m,
+                     const char *out_path, uint32_t threads, uint32_t ram_limit_gib) {
+    (void)threads;
+    (void)ram_limit_gib;
+    int rc = zguard("ZI subtraction failed", [&] {
+        require(index_path && out_path && (n == 0 || query_paths), MCZI_E_INVALID, "null argument");
+        if (m == 0) m = default_m(k);
+        check_k(k, m);
+        require(n > 0, MCZI_E_INVALID, "at least one query FASTA is required");
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        auto t0 = std::chrono::steady_clock::now();
+        auto v = zi_run(index_path, query_paths, n, k);
+        auto t1 = std::chrono::steady_clock::now();
+        write_simplitig_fasta(out_path, v, k, 0, nullptr);
+        auto t2 = std::chrono::steady_clock::now();
+        fprintf(stderr, "ZI_PHASE\t2_query_subtraction\t%.6f\n", std::chrono::duration<double>(t1 - t0).count());
+        fprintf(stderr, "ZI_PHASE\t3_simplitig_output\t%.6f\n", std::chrono::duration<double>(t2 - t1).count());
+        fprintf(stderr, "ZI_PHASE\ttotal\t%.6f\n", std::chrono::duration<double>(t2 - t0).count());
+    });
+    if (rc) mczi_set_error_from_zi(z_err.c_str());
+    return rc;
+}
+
+int mczi_zi_absent(const char *index_path, const char *const *query_paths, size_t n, uint32_t k,
+                   mczi_counted_kmer **out, size_t *n_out) {
+    int rc = zguard("ZI subtraction failed", [&] {
+        require(index_path && out && n_out && (n == 0 || query_paths), MCZI_E_INVALID, "null argument");
+        check_k(k, default_m(k));
+        require(n > 0, MCZI_E_INVALID, "at least one query FASTA is required");
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        auto v = zi_run(index_path, query_paths, n, k);
+        *n_out = v.size();
+        *out = (mczi_counted_kmer *)malloc(std::max<size_t>(v.size(), 1) * sizeof(mczi_counted_kmer));
+        require(*out != nullptr, MCZI_E_NOMEM, "out of host memory");
+        if (!v.empty()) memcpy(*out, v.data(), v.size() * sizeof(mczi_counted_kmer));
+    });
+    if (rc) mczi_set_error_from_zi(z_err.c_str());
+    return rc;
+}
+
+static int server_create(HostReads &&ir, uint32_t k, uint32_t m, uint64_t budget, uint32_t max_inflight,
+                         mczi_zi_server **out) {
+    int rc = zguard("failed to create ZI server", [&] {
+        require(out != nullptr, MCZI_E_INVALID, "null output pointer");
+        if (m == 0) m = default_m(k);
+        check_k(k, m);
+        auto s = std::make_unique<mczi_zi_server>();
+        Device &d = Device::get();
+        {
+            std::lock_guard<std::mutex> lk(d.call_mutex());
+            zi_build_index(s->idx, std::move(ir), k, m, d.stream());
+        }
+        s->budget = budget ? budget : (8ull << 30);
+        s->max_inflight = std::max<uint32_t>(1, std::min<uint32_t>(max_inflight ? max_inflight : 2, 8));
+        s->slots.resize(s->max_inflight);
+        for (auto &sl : s->slots) MCZI_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+        mczi_zi_server *raw = s.get();
+        s->admit_thread = std::thread([raw] { raw->admit_loop(); });
+        s->done_thread = std::thread([raw] { raw->done_loop(); });
+        *out = s.release();
+    });
+    if (rc) mczi_set_error_from_zi(z_err.c_str());
+    return rc;
+}
+
+int mczi_zi_server_create(const char *index_path, uint32_t k, uint32_t m, uint64_t hbm_budget_bytes,
+                          uint32_t max_inflight, mczi_zi_server **out) {
+    HostReads ir;
+    int rc = zguard("failed to read ZI index", [&] {
+        require(index_path != nullptr, MCZI_E_INVALID, "null index path");
+        parse_fastx_file(index_path, 1, 0, ir);
+    });
+    if (rc) {
+        mczi_set_error_from_zi(z_err.c_str());
+        return rc;
+    }
+    return server_create(std::move(ir), k, m, hbm_budget_bytes, max_inflight, out);
+}
+
+int mczi_zi_server_create_from_text(const char *text, size_t len, uint32_t k, uint32_t m, uint64_t hbm_budget_bytes,
+                                    uint32_t max_inflight, mczi_zi_server **out) {
+    HostReads ir;
+    int rc = zguard("failed to read ZI index", [&] {
+        require(text != nullptr, MCZI_E_INVALID, "null index text");
+        parse_fastx_text(text, len, 1, 0, ir);
+    });
+    if (rc) {
+        mczi_set_error_from_zi(z_err.c_str());
+        return rc;
+    }
+    return server_create(std::move(ir), k, m, hbm_budget_bytes, max_inflight, out);
+}
+
+int mczi_zi_server_submit(mczi_zi_server *s, const char *fastx, size_t len, int make_fasta, uint64_t *request_id) {
+    int rc = zguard("failed to submit ZI request", [&] {
+        require(s && request_id && (fastx || len == 0), MCZI_E_INVALID, "null argument");
+        auto r = std::make_unique<ZiRequest>();
+        r->t_submit = now_ms();
+        parse_fastx_text(fastx, len, s->idx.k, 0, r->reads);
+        r->reads.finish();
+        r->make_fasta = make_fasta != 0;
+        std::lock_guard<std::mutex> lk(s->mu);
+        require(s->fatal.empty(), MCZI_E_DEVICE, s->fatal);
+        r->id = s->next_id++;
+        *request_id = r->id;
+        s->queue.push_back(r.get());
+        s->reqs[r->id] = std::move(r);
+        s->cv.notify_all();
+    });
+    if (rc) mczi_set_error_from_zi(z_err.c_str());
+    return rc;
+}
+
+int mczi_zi_server_poll(mczi_zi_server *s, uint64_t request_id, int wait, mczi_zi_result **result) {
+    int code = MCZI_OK;
+    int rc = zguard("failed to poll ZI request", [&] {
+        require(s && result, MCZI_E_INVALID, "null argument");
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(request_id);
+        require(it != s->reqs.end(), MCZI_E_NOTFOUND, "unknown request id " + std::to_string(request_id));
+        ZiRequest *r = it->second.get();
+        if (!r->done) {
+            if (!wait) {
+                code = MCZI_E_PENDING;
+                return;
+            }
+            s->cv.wait(lk, [&] { return r->done; });
+        }
+        std::unique_ptr<ZiRequest> own = std::move(it->second);
+        s->reqs.erase(it);
+        lk.unlock();
+        if (own->err) throw Error(own->err, own->err_msg);
+        auto *res = (mczi_zi_result *)calloc(1, sizeof(mczi_zi_result));
+        require(res != nullptr, MCZI_E_NOMEM, "out of host memory");
+        res->request_id = own->id;
+        res->n_windows = own->windows;
+        res->n_absent = own->absent.size();
+        res->absent = (mczi_counted_kmer *)malloc(std::max<size_t>(own->absent.size(), 1) * sizeof(mczi_counted_kmer));
+        if (!own->absent.empty())
+            memcpy(res->absent, own->absent.data(), own->absent.size() * sizeof(mczi_counted_kmer));
+        res->fasta = (char *)malloc(own->fasta.size() + 1);
+        memcpy(res->fasta, own->fasta.data(), own->fasta.size());
+        res->fasta[own->fasta.size()] = 0;
+        res->fasta_len = own->fasta.size();
+        res->queue_ms = own->t_admit - own->t_submit;
+        res->latency_ms = own->t_done - own->t_submit;
+        *result = res;
+    });
+    if (rc) {
+        mczi_set_error_from_zi(z_err.c_str());
+        return rc;
+    }
+    if (code == MCZI_E_PENDING) mczi_set_error_from_zi("request is still running");
+    return code;
+}
+
+void mczi_zi_result_free(mczi_zi_result *r) {
+    if (!r) return;
+    free(r->absent);
+    free(r->fasta);
+    free(r);
+}
+
+int mczi_zi_server_stats(mczi_zi_server *s, uint64_t *index_kmers, uint64_t *completed, uint64_t *lookups,
+                         double *busy_ms) {
+    if (!s) return MCZI_E_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (index_kmers) *index_kmers = s->idx.n_kmers;
+    if (completed) *completed = s->completed;
+    if (lookups) *lookups = s->lookups;
+    if (busy_ms) *busy_ms = s->busy_ms;
+    return MCZI_OK;
+}
+
+void mczi_zi_server_destroy(mczi_zi_server *s) {
+    if (!s) return;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        // let in-flight batches finish
+        s->cv.wait(lk, [&] {
+            for (auto &sl : s->slots)
+                if (sl.busy) return false;
+            return true;
+        });
+        s->stop = true;
+        s->cv.notify_all();
+    }
+    if (s->admit_thread.joinable()) s->admit_thread.join();
+    if (s->done_thread.joinable()) s->done_thread.join();
+    for (auto &sl : s->slots) {
+        sl.work.reset();
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    }
+    delete s;
+}
+
+}  // extern "C"

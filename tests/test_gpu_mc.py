@@ -1,0 +1,206 @@
+"""HIP path vs the oracle: minimizer scan, counting, KFF, fof (MI355X only).
+
+Every comparison is bit-exact: same seeded inputs, the oracle's restatement
+(or brute force) on the CPU, the library's device path through the C ABI.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import mczi_amd as M
+import pyoracle as O
+from helpers import as_pairs, random_seqs, revcomp, same_counted, write_fasta
+
+pytestmark = pytest.mark.gpu
+
+
+def _pack(codes: np.ndarray) -> bytes:
+    n = codes.size
+    b = np.zeros((n + 3) // 4, dtype=np.uint8)
+    for j in range(4):
+        b[: (n - j + 3) // 4] |= (codes[j::4] << (2 * j)).astype(np.uint8)
+    return b.tobytes()
+
+
+@pytest.mark.parametrize("k,m", [(31, 21), (5, 3), (3, 2), (21, 11), (15, 15), (63, 31), (31, 19), (1, 1)])
+def test_run_hashes_match_oracle(k, m):
+    print(f"run_hashes k={k} m={m}", flush=True)
+    rng = np.random.default_rng(k * 100 + m)
+    seqs = random_seqs(rng, 6, k, 700) + ["A" * 300, "AC" * 200, "ACGTTGCA" * 50]
+    seqs += ["".join(rng.choice(list("ACGT"), size=5000))]
+    for s in seqs:
+        if len(s) < k:
+            continue
+        codes = O.codes_of(s)
+        h, st = M.minimizer_run_hashes(_pack(codes), len(s), k, m)
+        oh, ost = O.minimizer_runs(codes, k, m)
+        assert np.array_equal(h, oh) and np.array_equal(st, ost), (k, m, len(s))
+
+
+def test_run_hashes_known_positions(golden):
+    # the reference's position vector, through the device scan: the selected
+    # hash of each window is the hash at the known position
+    case = golden["minimizer_positions"][0]
+    codes = O.codes_of(case["seq"])
+    k = case["k"] + case["w"] - 1
+    h, st = M.minimizer_run_hashes(_pack(codes), len(case["seq"]), k, case["k"])
+    pos_hashes = [O.nthash32(codes[p:p + case["k"]], case["k"], True) for p in case["positions"]]
+    assert list(h) == pos_hashes
+
+
+def _reads(seed, genome_len=20000, n=2000, L=150, err=0.01):
+    rng = np.random.default_rng(seed)
+    genome = "".join(rng.choice(list("ACGT"), size=genome_len))
+    reads = []
+    for _ in range(n):
+        p = int(rng.integers(0, genome_len - L))
+        r = list(genome[p:p + L])
+        for j in range(L):
+            if rng.random() < err:
+                r[j] = "ACGT"[("ACGT".index(r[j]) + int(rng.integers(1, 4))) % 4]
+        s = "".join(r)
+        reads.append(s if rng.random() < 0.5 else revcomp(s))
+    return reads
+
+
+@pytest.mark.parametrize("k,m,x,P", [(31, 21, 1, 64), (31, 21, 5, 1024), (21, 11, 2, 7), (31, 19, 0, 3),
+                                     (15, 9, 3, 1024), (9, 5, 1, 16)])
+def test_count_inputs_matches_oracle(tmp_path, k, m, x, P):
+    print(f"count_inputs k={k} m={m} x={x} P={P}", flush=True)
+    reads = _reads(k + x)
+    fa = tmp_path / "r.fa"
+    write_fasta(fa, reads)
+    got = M.count_inputs([fa], M.CounterConfig(k, m, x, P))
+    codes, off = O.parse_fastx(str(fa), k)
+    want = O.count_bruteforce(codes, off, k, x)
+    assert same_counted(got, want)
+    tp, _, _ = O.count_two_pass(codes, off, k, m, x, P)
+    assert same_counted(got, tp)
+
+
+def test_golden_count_cases(golden, tmp_path):
+    for i, case in enumerate(golden["count_inputs"]):
+        p = tmp_path / f"g{i}.fa"
+        p.write_text(case["fasta"])
+        got = as_pairs(M.count_inputs([p], M.CounterConfig(case["k"], case["m"], case["x"])), case["k"])
+        if "exact" in case:
+            assert got == [tuple(x) for x in case["exact"]], case["src"]
+        for x in case.get("contains", []):
+            assert tuple(x) in got, case["src"]
+    case = golden["kmer_fasta"][0]
+    p = tmp_path / "kf.fa"
+    p.write_text(case["fasta"])
+    n = M.count_inputs_to_kmer_fasta_path([p], M.CounterConfig(case["k"], case["m"], case["x"]), tmp_path / "o.fa")
+    assert n == case["selected"]
+    assert case["contains"] in (tmp_path / "o.fa").read_text()
+
+
+def test_edge_inputs(tmp_path):
+    cfg = M.CounterConfig(31, 21, 0, 64)
+    empty = tmp_path / "e.fa"
+    empty.write_text("")
+    assert len(M.count_inputs([empty], cfg)) == 0
+    short = tmp_path / "s.fa"
+    short.write_text(">a\nACGT\n>b\nNNNNNNNN\n>c\nACGTNACGTACGTACGT\n")
+    assert len(M.count_inputs([short], cfg)) == 0
+    # long fragment crossing many tiles, homopolymer run, lowercase, N splits
+    rng = np.random.default_rng(9)
+    long = "".join(rng.choice(list("ACGT"), size=100000))
+    text = ">l\n" + long + "\n>h\n" + "A" * 5000 + "\n>lc\n" + long[:3000].lower() + "N" + long[5000:9000] + "\n"
+    f = tmp_path / "l.fa"
+    f.write_text(text)
+    for x in (0, 1):
+        got = M.count_inputs([f], M.CounterConfig(31, 21, x, 64))
+        codes, off = O.parse_fastx(str(f), 31)
+        assert same_counted(got, O.count_bruteforce(codes, off, 31, x))
+    with pytest.raises(M.MczaError, match="odd"):
+        M.count_inputs([f], M.CounterConfig(30, 21, 1))
+
+
+def test_kff_bytes_match_oracle(tmp_path):
+    reads = _reads(77, n=3000)
+    fa = tmp_path / "r.fa"
+    write_fasta(fa, reads)
+    codes, off = O.parse_fastx(str(fa), 31)
+    for x, P in ((1, 64), (0, 1024), (3, 5)):
+        out = tmp_path / f"o{x}.kff"
+        M.count_inputs_to_kff_path([fa], M.CounterConfig(31, 21, x, P), out)
+        want, _ = O.kff_u8_kmc(codes, off, 31, 21, x, P)
+        assert out.read_bytes() == want, (x, P)
+
+
+def test_fasta_bytes_match_oracle(tmp_path):
+    reads = _reads(78, n=3000)
+    fa = tmp_path / "r.fa"
+    write_fasta(fa, reads)
+    codes, off = O.parse_fastx(str(fa), 31)
+    out = tmp_path / "o.fa"
+    M.count_inputs_to_fasta_path([fa], M.CounterConfig(31, 21, 1, 64), out)
+    kms = O.count_bruteforce(codes, off, 31, 1)
+    O.write_simplitigs(str(tmp_path / "w.fa"), kms, 31)
+    assert out.read_bytes() == (tmp_path / "w.fa").read_bytes()
+
+
+def test_datasets_match_oracle(tmp_path, golden):
+    for case in golden["datasets"]:
+        paths = []
+        for i, s in enumerate(case["datasets"]):
+            p = tmp_path / f"d{i}.fa"
+            p.write_text(f">d{i}\n{s}\n")
+            paths.append(p)
+        got = as_pairs(M.count_datasets(paths, M.CounterConfig(case["k"], case["m"], case["x"])), case["k"])
+        for x in case.get("contains", []):
+            assert tuple(x) in got, case["src"]
+        for a in case.get("absent", []):
+            assert a not in [g[0] for g in got], case["src"]
+        if "exact" in case:
+            assert got == [tuple(x) for x in case["exact"]], case["src"]
+    rng = np.random.default_rng(5)
+    base = "".join(rng.choice(list("ACGT"), size=5000))
+    paths, seqs, ds = [], [], []
+    for d in range(7):
+        g = list(base)
+        for _ in range(20):
+            q = int(rng.integers(0, len(g)))
+            g[q] = "ACGT"[("ACGT".index(g[q]) + 1) % 4]
+        g = "".join(g)
+        rs = [g[p:p + 150] for p in rng.integers(0, len(g) - 150, size=200)]
+        p = tmp_path / f"r{d}.fa"
+        write_fasta(p, rs)
+        paths.append(p)
+        seqs += rs
+        ds += [d] * len(rs)
+    codes, off, dsa = O.frags_of(seqs, datasets=ds)
+    for x in (0, 2, 5):
+        got = M.count_datasets(paths, M.CounterConfig(31, 21, x, 64))
+        want = O.count_datasets_bruteforce(codes, off, dsa, 7, 31, x)
+        assert same_counted(got, want), x
+
+
+def test_batch_plain_equals_batched_and_spill(monkeypatch):
+    spec = M.SynthSpec(genome_len=200_000, genome_seed=1, read_seed=101, n_reads=40_000, read_len=150,
+                       sub_rate=0.005)
+    r = M.SynthReads(spec)
+    b = M.Batch.from_packed(r.words, r.frag_off)
+    codes = r.codes()
+    cfg = M.CounterConfig(31, 21, 1, 64)
+    want = O.count_bruteforce(codes, r.frag_off, 31, 1)
+    s1 = b.count(cfg, M.MODE_TWO_PASS, M.LAUNCH_BATCHED)
+    a = b.download()
+    assert same_counted(a, want)
+    s2 = b.count(cfg, M.MODE_TWO_PASS, M.LAUNCH_PLAIN)
+    assert same_counted(b.download(), want)
+    s3 = b.count(cfg, M.MODE_SINGLE_PASS, M.LAUNCH_BATCHED)
+    assert same_counted(b.download(), want)
+    assert s1.n_out == s2.n_out == s3.n_out == len(want)
+    # repeated graph launches give identical results
+    for _ in range(3):
+        b.count(cfg, M.MODE_TWO_PASS, M.LAUNCH_BATCHED)
+        assert same_counted(b.download(), want)
+    # few, large partitions: most k-mers spill from LDS to the HBM table
+    monkeypatch.setenv("MCZI_PART_TARGET", str(10 ** 9))
+    b2 = M.Batch.from_packed(r.words, r.frag_off)
+    st = b2.count(cfg, M.MODE_TWO_PASS, M.LAUNCH_BATCHED)
+    assert st.n_overflow > 0
+    assert same_counted(b2.download(), want)
