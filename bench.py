@@ -70,6 +70,111 @@ def cpu_baseline(reads, k, m, x, parts, mode, seconds_target=15.0):
                       f"{'two-pass FASTA path' if mode == 0 else 'KMC-style KFF path'}, {dt:.1f} s"}
 
 
+def zi_serving(M, n_requests: int, k: int, seed: int = 5):
+    """ZI served subtraction: index = canonical k-mers of a 100 Mbp genome,
+    requests of 150 bp reads (half from the indexed genome, half from an
+    unrelated genome, 0.5% substitutions), read counts log-uniform in
+    [10, 1e6] rounded up to a multiple of 8. Measures the plain path (one
+    request at a time) and the served path (continuous batching) in lookups/s
+    and request latency."""
+    import ctypes as C
+    import threading
+
+    import numpy as np
+
+    G = 100_000_000
+    ga = M.synth_genome(G, seed)
+    gb = M.synth_genome(G, seed + 1000)
+    # index FASTA text of genome A
+    codes = np.unpackbits(ga.view(np.uint8)[:, None], axis=1, bitorder="little").reshape(-1, 4, 2)
+    codes = (codes[:, :, 0] | (codes[:, :, 1] << 1)).reshape(-1)[:G]
+    text = b">index\n" + np.frombuffer(b"ACTG", dtype=np.uint8)[codes].tobytes() + b"\n"
+    del codes
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(10), np.log(1e6), size=n_requests)).astype(np.int64)
+    counts = ((counts + 7) // 8) * 8
+    reads = M.synth_reads_from([(ga, G), (gb, G)], int(counts.sum()), 150, 0.005, seed + 1)
+    starts = np.concatenate([[0], np.cumsum(counts)])
+    reqs = []
+    for i in range(n_requests):
+        b0, b1 = int(starts[i]) * 150, int(starts[i + 1]) * 150
+        w = np.ascontiguousarray(reads.words[b0 // 16:(b1 + 15) // 16 + 1])
+        off = np.arange(0, (b1 - b0) + 1, 150, dtype=np.uint64)
+        reqs.append((w, off))
+
+    h = C.c_void_p()
+    t0 = time.perf_counter()
+    M._check(M._lib.mczi_zi_server_create_from_text(text, len(text), k, 0, 96 << 30, 3, C.byref(h)))
+    build_s = time.perf_counter() - t0
+
+    def submit(i):
+        w, off = reqs[i]
+        rid = C.c_uint64()
+        M._check(M._lib.mczi_zi_server_submit_packed(h, w.ctypes.data, off.ctypes.data, off.size - 1, 0,
+                                                      C.byref(rid)))
+        return rid.value
+
+    def poll(rid):
+        p = C.POINTER(M._ZiResult)()
+        M._check(M._lib.mczi_zi_server_poll(h, rid, 1, C.byref(p)))
+        r = p.contents
+        out = (r.n_windows, r.n_absent, r.latency_ms, r.queue_ms)
+        M._lib.mczi_zi_result_free(p)
+        return out
+
+    try:
+        poll(submit(0))  # warm-up
+        # plain: one request at a time
+        t0 = time.perf_counter()
+        plain = [poll(submit(i)) for i in range(n_requests)]
+        plain_s = time.perf_counter() - t0
+        lookups = sum(x[0] for x in plain)
+        # served, closed loop: every request submitted at once
+        t0 = time.perf_counter()
+        ids = [submit(i) for i in range(n_requests)]
+        burst = [poll(r) for r in ids]
+        burst_s = time.perf_counter() - t0
+        assert [x[1] for x in burst] == [x[1] for x in plain], "served results differ from plain"
+        # served, open loop: Poisson arrivals at 1.5x the plain request rate
+        rate = 1.5 * n_requests / plain_s
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
+        ids = [None] * n_requests
+
+        def feeder():
+            tb = time.perf_counter()
+            for i in range(n_requests):
+                d = arrivals[i] - (time.perf_counter() - tb)
+                if d > 0:
+                    time.sleep(d)
+                ids[i] = submit(i)
+
+        th = threading.Thread(target=feeder)
+        t0 = time.perf_counter()
+        th.start()
+        res = []
+        for i in range(n_requests):
+            while ids[i] is None:
+                time.sleep(0.0005)
+            res.append(poll(ids[i]))
+        th.join()
+        open_s = time.perf_counter() - t0
+        lat = np.array([x[2] for x in res])
+    finally:
+        M._lib.mczi_zi_server_destroy(h)
+    return {
+        "k": k, "requests": n_requests, "reads": int(counts.sum()), "lookups": int(lookups),
+        "index_build_s": build_s,
+        "plain_lookups_per_s": lookups / plain_s,
+        "served_lookups_per_s": lookups / burst_s,
+        "served_vs_plain": plain_s / burst_s,
+        "poisson_rate_per_s": rate,
+        "poisson_lookups_per_s": lookups / open_s,
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "plain_p50_ms": float(np.percentile([x[2] for x in plain], 50)),
+        "plain_p99_ms": float(np.percentile([x[2] for x in plain], 99)),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +183,8 @@ def main() -> int:
     ap.add_argument("--workload", default="c2_short_reads_100mbp", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--zi-requests", type=int, default=48)
+    ap.add_argument("--no-zi", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -122,6 +229,12 @@ def main() -> int:
         plain_value = reads.n_bases * 2 / (time.perf_counter() - tp)
     del os.environ["MCZI_NO_GRAPH"]
 
+    # plain device copy of 2 GiB: achievable HBM bandwidth on this box, and
+    # the reference dispatch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE
+    copy_bytes = 2 << 30
+    copy_ms = M.copy_probe(copy_bytes, 10)
+    copy_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
+
     nb = reads.n_bases
     rec_bytes = prof["n_records"] * 16
     out_bytes = prof["n_out"] * 16
@@ -157,11 +270,16 @@ def main() -> int:
         "phase_ms": {k2: prof[k2] for k2 in ("ms_scan1", "ms_scan2", "ms_count", "ms_sort", "ms_total")},
         "stats": {k2: last[k2] for k2 in ("n_windows", "n_pieces", "n_records", "n_distinct", "n_overflow",
                                            "n_out", "internal_partitions")},
+        "copy_probe": {"bytes": copy_bytes, "ms": copy_ms, "read_plus_write_GBs": copy_gbs},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes": alg, "kernel_ms": ms},
         "cpu_baseline": None,
     }
+    if not args.no_zi:
+        del batch
+        line["zi"] = zi_serving(M, args.zi_requests, 31)
+        print(f"[bench] zi: {line['zi']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(reads, k, m, x, parts, mode)
     print(json.dumps(line), flush=True)

@@ -181,6 +181,12 @@ int mczi_batch_count(mczi_batch *b, const mczi_counter_config *cfg, int mode, in
                      int order, mczi_run_stats *stats);
 int mczi_batch_download(mczi_batch *b, mczi_counted_kmer **out, size_t *n_out);
 
+/* Device copy of `bytes` bytes (16-byte vector loads and stores), `reps`
+ * times; *ms_per_copy gets the average duration. No counterpart in the
+ * reference: it calibrates HBM byte counters and measures the box's copy
+ * bandwidth for roofline reporting. */
+int mczi_copy_probe(uint64_t bytes, int reps, double *ms_per_copy);
+
 /* ---- ZI (src/zi.rs) -------------------------------------------------------- */
 /* ZI main path (src/zi.rs:64-100): absent query k-mers as simplitig FASTA.
  * m == 0 selects default_minimizer_size (src/zi.rs:109-112). */
@@ -217,6 +223,10 @@ int mczi_zi_server_create_from_text(const char *index_fasta, size_t len, uint32_
                                     mczi_zi_server **out);
 int mczi_zi_server_submit(mczi_zi_server *s, const char *fastx, size_t len, int make_fasta,
                           uint64_t *request_id);
+/* Same, for reads already packed (16 bases per u32 word, fragment offsets
+ * as in mczi_batch_from_packed); fragments shorter than k are skipped. */
+int mczi_zi_server_submit_packed(mczi_zi_server *s, const uint32_t *words, const uint64_t *frag_off,
+                                 size_t n_frag, int make_fasta, uint64_t *request_id);
 /* wait != 0 blocks until the request is done; otherwise MCZI_E_PENDING. */
 int mczi_zi_server_poll(mczi_zi_server *s, uint64_t request_id, int wait,
                         mczi_zi_result **result);
@@ -256,6 +266,11 @@ typedef struct {
 
 int mczi_synth_generate(const mczi_synth_spec *spec, mczi_synth_reads *out);
 int mczi_synth_genome(uint64_t genome_len, uint64_t seed, uint32_t **words);
+/* Reads sampled from given genomes (packed words): read i comes from genome
+ * i % n_genomes; same error / strand model as mczi_synth_generate. */
+int mczi_synth_reads_from(const uint32_t *const *genome_words, const uint64_t *genome_len,
+                          uint32_t n_genomes, uint64_t n_reads, uint32_t read_len, double sub_rate,
+                          uint64_t seed, mczi_synth_reads *out);
 /* Write reads as FASTA (one record per read); dataset_filter < 0 for all. */
 int mczi_synth_write_fasta(const mczi_synth_reads *r, const char *path, int64_t dataset_filter);
 void mczi_synth_free(mczi_synth_reads *r);

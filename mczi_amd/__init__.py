@@ -119,6 +119,7 @@ _sig("mczi_batch_info", C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint6
 _sig("mczi_batch_free", None, [_P])
 _sig("mczi_batch_count", C.c_int, [_P, C.POINTER(_Config), C.c_int, C.c_int, C.c_int, C.POINTER(RunStats)])
 _sig("mczi_batch_download", C.c_int, [_P, C.POINTER(C.POINTER(_Counted)), C.POINTER(C.c_size_t)])
+_sig("mczi_copy_probe", C.c_int, [C.c_uint64, C.c_int, C.POINTER(C.c_double)])
 _sig("mczi_zi_subtract", C.c_int, [C.c_char_p, _CP, C.c_size_t, C.c_uint32, C.c_uint32, C.c_char_p,
                                    C.c_uint32, C.c_uint32])
 _sig("mczi_zi_absent", C.c_int, [C.c_char_p, _CP, C.c_size_t, C.c_uint32, C.POINTER(C.POINTER(_Counted)),
@@ -127,6 +128,7 @@ _sig("mczi_zi_server_create", C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_
 _sig("mczi_zi_server_create_from_text", C.c_int, [C.c_char_p, C.c_size_t, C.c_uint32, C.c_uint32, C.c_uint64,
                                                   C.c_uint32, C.POINTER(_P)])
 _sig("mczi_zi_server_submit", C.c_int, [_P, C.c_char_p, C.c_size_t, C.c_int, C.POINTER(C.c_uint64)])
+_sig("mczi_zi_server_submit_packed", C.c_int, [_P, _P, _P, C.c_size_t, C.c_int, C.POINTER(C.c_uint64)])
 _sig("mczi_zi_server_poll", C.c_int, [_P, C.c_uint64, C.c_int, C.POINTER(C.POINTER(_ZiResult))])
 _sig("mczi_zi_result_free", None, [C.POINTER(_ZiResult)])
 _sig("mczi_zi_server_stats", C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
@@ -134,6 +136,8 @@ _sig("mczi_zi_server_stats", C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_
 _sig("mczi_zi_server_destroy", None, [_P])
 _sig("mczi_synth_generate", C.c_int, [C.POINTER(_SynthSpec), C.POINTER(_SynthReads)])
 _sig("mczi_synth_genome", C.c_int, [C.c_uint64, C.c_uint64, C.POINTER(C.POINTER(C.c_uint32))])
+_sig("mczi_synth_reads_from", C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.c_uint32, C.c_uint64,
+                                        C.c_uint32, C.c_double, C.c_uint64, C.POINTER(_SynthReads)])
 _sig("mczi_synth_write_fasta", C.c_int, [C.POINTER(_SynthReads), C.c_char_p, C.c_int64])
 _sig("mczi_synth_free", None, [C.POINTER(_SynthReads)])
 
@@ -145,10 +149,11 @@ EXPORTED = [
     "mczi_count_inputs_to_kmer_fasta_path", "mczi_count_datasets_to_kmer_fasta_path", "mczi_write_fasta_path",
     "mczi_write_fasta_after_index", "mczi_write_kff_path", "mczi_decode_kmer", "mczi_minimizer_run_hashes",
     "mczi_batch_from_files", "mczi_batch_from_packed", "mczi_batch_info", "mczi_batch_free", "mczi_batch_count",
-    "mczi_batch_download", "mczi_zi_subtract", "mczi_zi_absent", "mczi_zi_server_create",
-    "mczi_zi_server_create_from_text", "mczi_zi_server_submit", "mczi_zi_server_poll", "mczi_zi_result_free",
+    "mczi_batch_download", "mczi_copy_probe", "mczi_zi_subtract", "mczi_zi_absent", "mczi_zi_server_create",
+    "mczi_zi_server_create_from_text", "mczi_zi_server_submit", "mczi_zi_server_submit_packed",
+    "mczi_zi_server_poll", "mczi_zi_result_free",
     "mczi_zi_server_stats", "mczi_zi_server_destroy", "mczi_synth_generate", "mczi_synth_genome",
-    "mczi_synth_write_fasta", "mczi_synth_free",
+    "mczi_synth_write_fasta", "mczi_synth_free", "mczi_synth_reads_from",
 ]
 
 
@@ -169,6 +174,13 @@ def _take(ptr, n) -> np.ndarray:
         C.memmove(out.ctypes.data, ptr, n * COUNTED.itemsize)
     _lib.mczi_free(C.cast(ptr, C.c_void_p))
     return out
+
+
+def copy_probe(nbytes: int, reps: int = 10) -> float:
+    """Average ms of a plain device copy of nbytes (16-byte vectors)."""
+    ms = C.c_double()
+    _check(_lib.mczi_copy_probe(nbytes, reps, C.byref(ms)))
+    return ms.value
 
 
 def version() -> str:
@@ -378,9 +390,12 @@ class SynthSpec:
 class SynthReads:
     """Packed synthetic reads in host memory (numpy views owned by the library)."""
 
-    def __init__(self, spec: SynthSpec):
-        self._r = _SynthReads()
-        _check(_lib.mczi_synth_generate(C.byref(spec._c()), C.byref(self._r)))
+    def __init__(self, spec: SynthSpec | None = None, _raw: _SynthReads | None = None):
+        if _raw is None:
+            self._r = _SynthReads()
+            _check(_lib.mczi_synth_generate(C.byref(spec._c()), C.byref(self._r)))
+        else:
+            self._r = _raw
         r = self._r
         self.n_bases = r.n_bases
         self.words = np.ctypeslib.as_array(r.words, shape=(r.n_words,))
@@ -407,6 +422,16 @@ class SynthReads:
             self.close()
         except Exception:
             pass
+
+
+def synth_reads_from(genomes, n_reads: int, read_len: int, sub_rate: float, seed: int) -> SynthReads:
+    """Reads from packed genomes [(words, length), ...]; read i from genome i % len(genomes)."""
+    ws = [np.ascontiguousarray(w, dtype=np.uint32) for w, _ in genomes]
+    ptrs = (C.c_void_p * len(ws))(*[w.ctypes.data for w in ws])
+    lens = (C.c_uint64 * len(ws))(*[int(n) for _, n in genomes])
+    raw = _SynthReads()
+    _check(_lib.mczi_synth_reads_from(ptrs, lens, len(ws), n_reads, read_len, sub_rate, seed, C.byref(raw)))
+    return SynthReads(_raw=raw)
 
 
 def synth_genome(genome_len: int, seed: int) -> np.ndarray:

@@ -351,6 +351,14 @@ int mczi_minimizer_run_hashes(const uint8_t *packed, uint64_t n_bases, uint32_t 
     });
 }
 
+int mczi_copy_probe(uint64_t bytes, int reps, double *ms_per_copy) {
+    return guarded("copy probe failed", [&] {
+        require(ms_per_copy != nullptr, MCZI_E_INVALID, "null output pointer");
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        *ms_per_copy = device_copy_probe(bytes, reps);
+    });
+}
+
 // ---- batched device API -----------------------------------------------------
 struct mczi_batch {
     std::unique_ptr<Batch> b;
@@ -462,6 +470,16 @@ int mczi_synth_generate(const mczi_synth_spec *spec, mczi_synth_reads *out) {
 
 int mczi_synth_genome(uint64_t genome_len, uint64_t seed, uint32_t **words) {
     return guarded("failed to generate genome", [&] { synth_genome_words(genome_len, seed, words); });
+}
+
+int mczi_synth_reads_from(const uint32_t *const *genome_words, const uint64_t *genome_len, uint32_t n_genomes,
+                          uint64_t n_reads, uint32_t read_len, double sub_rate, uint64_t seed,
+                          mczi_synth_reads *out) {
+    return guarded("failed to generate synthetic reads", [&] {
+        require(genome_words && genome_len && out, MCZI_E_INVALID, "null argument");
+        *out = mczi_synth_reads{};
+        synth_reads_from(genome_words, genome_len, n_genomes, n_reads, read_len, sub_rate, seed, *out);
+    });
 }
 
 int mczi_synth_write_fasta(const mczi_synth_reads *r, const char *path, int64_t dataset_filter) {

@@ -159,6 +159,9 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
 void batch_download(Batch &b, std::vector<mczi_counted_kmer> &out);
 void free_plan(McPlan *p);
 
+// Plain device copy of `bytes` (average ms per launch over `reps`).
+double device_copy_probe(uint64_t bytes, int reps);
+
 // Device scan boundary for one sequence (run_hashes).
 void device_run_hashes(const uint8_t *packed, uint64_t n_bases, uint32_t k, uint32_t m,
                        std::vector<uint32_t> &hashes, std::vector<uint32_t> &starts);
@@ -184,4 +187,6 @@ namespace mczi {
 void synth_generate(const mczi_synth_spec &spec, mczi_synth_reads &out);
 void synth_write_fasta(const mczi_synth_reads &r, const std::string &path, int64_t ds_filter);
 void synth_genome_words(uint64_t len, uint64_t seed, uint32_t **words);
+void synth_reads_from(const uint32_t *const *gw, const uint64_t *gl, uint32_t ng, uint64_t n_reads,
+                      uint32_t read_len, double sub, uint64_t seed, mczi_synth_reads &out);
 }  // namespace mczi

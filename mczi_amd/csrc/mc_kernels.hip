@@ -25,7 +25,7 @@ constexpr int SCAN_THREADS = 256;
 constexpr int WPT = 8;                        // windows per thread
 constexpr int TILE = SCAN_THREADS * WPT;      // windows (base positions) per tile
 constexpr int MAXW = 64;                      // w = k - m + 1 <= 64
-constexpr int WORDS_SPAN = (15 + TILE + 128) / 16 + 8;
+constexpr int WORDS_SPAN = (15 + TILE + 160) / 16 + 8;
 constexpr int REC_MAX_BASES = 56;             // bases per 128-bit super-k-mer record
 
 // Count kernel geometry
@@ -99,6 +99,22 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *total, int *s_ws
     return pre + x - v;
 }
 
+// Block-wide sum of one value per thread; thread 0 adds it to *dst.
+__device__ __forceinline__ void block_sum_atomic(unsigned long long v, unsigned long long *dst,
+                                                 unsigned long long *s_red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) s_red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) t += s_red[i];
+        if (t) atomicAdd(dst, t);
+    }
+    __syncthreads();
+}
+
 // One tile = up to TILE consecutive base positions of the segmented stream,
 // clipped to its input chunk. For every window (k-mer start) p of the tile:
 // canonical ntHash of each m-mer (rolling), #TG strand vote over the k bases,
@@ -127,7 +143,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
         const int tl = (int)a.tile_len[t];
         const uint64_t f0 = a.tile_frag[t];
         const int rb0 = (int)(ts & 15u);
-        const int nwords = ((rb0 + tl + 128) >> 4) + 3;
+        const int nwords = ((rb0 + tl + 160) >> 4) + 3;
         __syncthreads();  // previous tile fully consumed
         for (int i = tid; i < nwords; i += SCAN_THREADS) s_words[i] = a.words[(ts >> 4) + i];
         // Fragment boundaries relative to ts: s_fs[j] = off[f0 + j] - ts (clamped),
@@ -147,21 +163,32 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
         }
         __syncthreads();
 
-        // m-mer hashes for relative positions q in [0, tl + w - 1)
+        // m-mer hashes for relative positions q in [0, tl + w - 1). Each thread
+        // owns WPT consecutive positions; their bases (<= WPT + 63) are held in
+        // three 32-base registers.
         const int nh = tl + w - 1;
         for (int q0 = tid * WPT; q0 < nh; q0 += TILE) {
             const int rb = rb0 + q0;
+            const uint64_t B0 = extract32_bases(s_words, (uint64_t)rb);
+            const uint64_t B1 = extract32_bases(s_words, (uint64_t)rb + 32);
+            const uint64_t B2 = extract32_bases(s_words, (uint64_t)rb + 64);
+            auto code_at = [&](int i) -> uint32_t {
+                const uint64_t x = i < 32 ? B0 : (i < 64 ? B1 : B2);
+                return (uint32_t)(x >> (2 * (i & 31))) & 3u;
+            };
             uint32_t fw = 0, rc = 0, rot = 0;
             for (int j = 0; j < m; j++) {
-                const uint32_t c = lds_code(s_words, rb + j);
+                const uint32_t c = code_at(j);
                 fw = rotl32(fw, NT_ROT) ^ nt_seed(c);
                 rc ^= rotl32(nt_seed(c ^ 2u), rot);
                 rot = (rot + NT_ROT) & 31u;
             }
             s_hash[q0] = fw + rc;
-            for (int i = 1; i < WPT && q0 + i < nh; i++) {
-                const uint32_t co = lds_code(s_words, rb + i - 1);
-                const uint32_t ci = lds_code(s_words, rb + i - 1 + m);
+#pragma unroll
+            for (int i = 1; i < WPT; i++) {
+                if (q0 + i >= nh) break;
+                const uint32_t co = code_at(i - 1);
+                const uint32_t ci = code_at(i - 1 + m);
                 fw = rotl32(fw, NT_ROT) ^ rotl32(nt_seed(co), R1) ^ nt_seed(ci);
                 rc = rotr32(rc ^ nt_seed(co ^ 2u), NT_ROT) ^ rotl32(nt_seed(ci ^ 2u), R2);
                 s_hash[q0 + i] = fw + rc;
@@ -169,7 +196,12 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
         }
         __syncthreads();
 
-        // selected hash per window
+        // selected hash per window. Keys compare the upper 16 hash bits, then
+        // the position: (h & 0xffff0000) | t gives the leftmost minimum,
+        // (h & 0xffff0000) | (0xffff - t) the rightmost (sm/sliding_min.rs:
+        // 279-340). With w >= WPT the thread's WPT windows share the middle
+        // hashes [WPT-1, w-1]; suffix minima on the left and prefix minima on
+        // the right complete each window (van Herk / Gil-Werman).
         const int r0 = tid * WPT;
         if (r0 < tl) {
             int lo = 0, hi = nf - 1;
@@ -179,32 +211,68 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
                 else hi = mid - 1;
             }
             int j = lo;
+            const int rb = rb0 + r0;
+            const uint64_t B0 = extract32_bases(s_words, (uint64_t)rb);
+            const uint64_t B1 = extract32_bases(s_words, (uint64_t)rb + 32);
+            const uint64_t B2 = extract32_bases(s_words, (uint64_t)rb + 64);
+            auto tgbit = [&](int i) -> uint32_t {
+                const uint64_t x = i < 32 ? B0 : (i < 64 ? B1 : B2);
+                return (uint32_t)(x >> (2 * (i & 31) + 1)) & 1u;
+            };
+            uint32_t tg = __popcll(B0 & high_bit_mask(k));
+            if (k > 32) tg += __popcll(B1 & high_bit_mask(k - 32));
+            uint32_t sufL[WPT], sufR[WPT], preL[WPT], preR[WPT], midL = ~0u, midR = ~0u;
+            const bool gw = w >= WPT;
+            if (gw) {
+                for (int t = WPT - 1; t <= w - 1; t++) {
+                    const uint32_t h = s_hash[r0 + t] & 0xffff0000u;
+                    midL = min(midL, h | (uint32_t)t);
+                    midR = min(midR, h | (0xffffu - (uint32_t)t));
+                }
+                sufL[WPT - 1] = sufR[WPT - 1] = ~0u;
+#pragma unroll
+                for (int i = WPT - 2; i >= 0; i--) {
+                    const uint32_t h = s_hash[r0 + i] & 0xffff0000u;
+                    sufL[i] = min(sufL[i + 1], h | (uint32_t)i);
+                    sufR[i] = min(sufR[i + 1], h | (0xffffu - (uint32_t)i));
+                }
+                preL[0] = preR[0] = ~0u;
+#pragma unroll
+                for (int i = 1; i < WPT; i++) {
+                    const uint32_t t = (uint32_t)(w + i - 1);
+                    const uint32_t h = s_hash[r0 + t] & 0xffff0000u;
+                    preL[i] = min(preL[i - 1], h | t);
+                    preR[i] = min(preR[i - 1], h | (0xffffu - t));
+                }
+            }
+#pragma unroll
             for (int i = 0; i < WPT; i++) {
                 const int r = r0 + i;
+                if (i > 0) tg = tg - tgbit(i - 1) + tgbit(i - 1 + k);
                 if (r >= tl) break;
                 while (j + 1 < nf && s_fs[j + 1] <= r) j++;
                 const bool fstart = s_fs[j] == r;
                 const bool valid = r + k <= s_fs[j + 1];
                 uint32_t v = 0;
                 if (valid) {
-                    const uint64_t x0 = extract32_bases(s_words, (uint64_t)(rb0 + r));
-                    uint32_t tg = __popcll(x0 & high_bit_mask(k));
-                    if (k > 32) {
-                        const uint64_t x1 = extract32_bases(s_words, (uint64_t)(rb0 + r + 32));
-                        tg += __popcll(x1 & high_bit_mask(k - 32));
-                    }
-                    uint32_t best = s_hash[r] >> 16;
-                    int lp = r, rp = r;
-                    for (int ii = 1; ii < w; ii++) {
-                        const uint32_t hv = s_hash[r + ii] >> 16;
-                        if (hv < best) {
-                            best = hv;
-                            lp = rp = r + ii;
-                        } else if (hv == best) {
-                            rp = r + ii;
+                    int lp, rp;
+                    if (gw) {
+                        lp = (int)(min(min(sufL[i], midL), preL[i]) & 0xffffu);
+                        rp = (int)(0xffffu - (min(min(sufR[i], midR), preR[i]) & 0xffffu));
+                    } else {
+                        uint32_t best = s_hash[r] >> 16;
+                        lp = rp = i;
+                        for (int ii = 1; ii < w; ii++) {
+                            const uint32_t hv = s_hash[r + ii] >> 16;
+                            if (hv < best) {
+                                best = hv;
+                                lp = rp = i + ii;
+                            } else if (hv == best) {
+                                rp = i + ii;
+                            }
                         }
                     }
-                    v = (2u * tg > (uint32_t)k) ? s_hash[lp] : s_hash[rp];
+                    v = (2u * tg > (uint32_t)k) ? s_hash[r0 + lp] : s_hash[r0 + rp];
                 }
                 s_v[r] = v;
                 s_flag[r] = (uint8_t)((valid ? 1 : 0) | (fstart ? 2 : 0));
@@ -309,8 +377,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
             }
         }
     }
-    if (my_pieces) atomicAdd(&a.stats[0], my_pieces);
-    if (my_records) atomicAdd(&a.stats[1], my_records);
+    __shared__ unsigned long long s_red[SCAN_THREADS / 64];
+    block_sum_atomic(my_pieces, &a.stats[0], s_red);
+    block_sum_atomic(my_records, &a.stats[1], s_red);
 }
 
 // ---------------------------------------------------------------------------
@@ -381,46 +450,83 @@ __device__ __forceinline__ unsigned long long wave_append(bool pred, unsigned lo
     return base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
-// Per partition: all k-mers of its super-k-mer records into an LDS open-
-// addressing table (CNT_SLOTS slots, linear probing, insert-only); a k-mer
-// whose probe window is full goes to the HBM spill table instead, so each
-// k-mer lives in exactly one of the two. Then count > x (MODE 0: counts,
+// Canonical MC encoding of k <= 32 bases held packed-seq style in x
+// (base j at bits 2j): forward = lanes reversed (first base most
+// significant), reverse complement = complemented lanes in stored order.
+__device__ __forceinline__ uint64_t canon_from_packed(uint64_t x, int k) {
+    const uint64_t mask = k >= 32 ? ~0ull : ((1ull << (2 * k)) - 1ull);
+    x = (x ^ ((x >> 1) & 0x5555555555555555ull)) & mask;  // packed-seq code -> MC bits
+    uint64_t r = __builtin_bitreverse64(x);
+    r = ((r >> 1) & 0x5555555555555555ull) | ((r & 0x5555555555555555ull) << 1);
+    const uint64_t fwd = r >> (64 - 2 * k);
+    const uint64_t rc = (~x) & mask;
+    return fwd < rc ? fwd : rc;
+}
+
+// Per partition: all k-mers of its super-k-mer records go into an LDS
+// open-addressing table (CNT_SLOTS slots, linear probing, insert-only); a
+// k-mer whose probe window is full goes to the HBM spill table instead, so
+// each k-mer lives in exactly one of the two. Then count > x (MODE 0: counts,
 // src/lib.rs:2417 / 2502) or popcount(mask) > x (MODE 1: dataset presence,
-// src/lib.rs:2603) is emitted. add_packed_superkmer_kmer_counts
-// (src/lib.rs:3589-3614) gives the canonical k-mer roll.
+// src/lib.rs:2603) is emitted with one output reservation per partition.
+// Work is balanced per wave: 64 records are loaded, their k-mers numbered by
+// a wave prefix sum, and lane j takes k-mer j (owner record found by a
+// shuffle binary search), extracting it directly from the 128-bit record
+// (the same canonical k-mers add_packed_superkmer_kmer_counts rolls,
+// src/lib.rs:3589-3614).
 template <int MODE>
 __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
     __shared__ unsigned long long s_keys[CNT_SLOTS];
     __shared__ uint32_t s_vals[CNT_SLOTS];
-    __shared__ uint32_t s_q;
+    __shared__ int s_wsum[CNT_THREADS / 64];
+    __shared__ unsigned long long s_red[CNT_THREADS / 64];
+    __shared__ unsigned long long s_obase;
     const int tid = threadIdx.x;
+    const int lane = tid & 63, wid = tid >> 6;
     const int k = a.k;
-    const uint64_t kmask = (k >= 32) ? ~0ull : ((1ull << (2 * k)) - 1ull);
-    const int hs = 2 * (k - 1);
     unsigned long long my_distinct = 0;
-    for (;;) {
-        if (tid == 0) s_q = (uint32_t)atomicAdd(&a.ctr[0], 1ull);
+    for (uint32_t q = blockIdx.x; q < a.p_int; q += gridDim.x) {
         for (int i = tid; i < CNT_SLOTS; i += CNT_THREADS) {
             s_keys[i] = EMPTY_KEY;
             s_vals[i] = 0;
         }
         __syncthreads();
-        const uint32_t q = s_q;
-        if (q >= a.p_int) break;
         const uint64_t rb = a.base[q], re = a.cursor[q];
-        for (uint64_t ri = rb + tid; ri < re; ri += CNT_THREADS) {
-            const ulonglong2 r = a.records[ri];
-            const uint32_t L = (uint32_t)(r.y >> 48) & 255u;
-            const uint32_t val = (MODE == 0) ? 1u : (1u << (uint32_t)(r.y >> 56));
-            uint64_t fwd = 0, rev = 0;
-            for (uint32_t i = 0; i < L; i++) {
-                const uint32_t code = (i < 32) ? (uint32_t)(r.x >> (2 * i)) & 3u
-                                               : (uint32_t)(r.y >> (2 * (i - 32))) & 3u;
-                const uint64_t b = mc_bits(code);
-                fwd = ((fwd << 2) | b) & kmask;
-                rev = (rev >> 2) | ((b ^ 3ull) << hs);
-                if (i + 1 < (uint32_t)k) continue;
-                const unsigned long long key = fwd < rev ? fwd : rev;
+        for (uint64_t wb = rb + (uint64_t)wid * 64; wb < re; wb += CNT_THREADS) {
+            const uint64_t ri = wb + lane;
+            ulonglong2 r = make_ulonglong2(0, 0);
+            int n = 0;
+            if (ri < re) {
+                r = a.records[ri];
+                n = (int)((r.y >> 48) & 255u) - k + 1;
+                if (n < 0) n = 0;
+            }
+            int incl = n;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += y;
+            }
+            const int total = __shfl(incl, 63, 64);
+            for (int j0 = 0; j0 < total; j0 += 64) {
+                const int j = j0 + lane;
+                // owner lane: first lane whose inclusive count exceeds j
+                int o = 0;
+#pragma unroll
+                for (int st = 32; st > 0; st >>= 1) {
+                    const int v = __shfl(incl, o + st - 1, 64);
+                    if (v <= j) o += st;
+                }
+                const unsigned long long rx = __shfl(r.x, o, 64);
+                const unsigned long long ry = __shfl(r.y, o, 64);
+                const int excl = __shfl(incl, o, 64) - __shfl(n, o, 64);
+                if (j >= total) continue;
+                const int off = j - excl;  // k-mer start inside the record
+                const uint64_t rd = ry & 0xFFFFFFFFFFFFull;  // bases 32..55
+                const int sh = 2 * off;
+                const uint64_t lo = sh == 0 ? rx : (sh < 64 ? ((rx >> sh) | (rd << (64 - sh))) : (rd >> (sh - 64)));
+                const unsigned long long key = canon_from_packed(lo, k);
+                const uint32_t val = (MODE == 0) ? 1u : (1u << (uint32_t)(ry >> 56));
                 uint32_t slot = (uint32_t)(spread_hash_u64(key) >> 32) & (CNT_SLOTS - 1);
                 bool done = false;
                 for (int p = 0; p < CNT_MAXPROBE; p++) {
@@ -445,25 +551,53 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
             }
         }
         __syncthreads();
-        for (int i0 = 0; i0 < CNT_SLOTS; i0 += CNT_THREADS) {
-            const int i = i0 + tid;
-            const unsigned long long key = s_keys[i];
-            uint32_t c = 0;
-            if (key != EMPTY_KEY) {
-                my_distinct++;
-                c = (MODE == 0) ? s_vals[i] : (uint32_t)__popc(s_vals[i]);
+        // emit: count survivors, one reservation per partition, then write
+        constexpr int PER = CNT_SLOTS / CNT_THREADS;
+        uint32_t emask = 0;
+        int cnt = 0;
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int sl = i * CNT_THREADS + tid;
+            const unsigned long long key = s_keys[sl];
+            if (key == EMPTY_KEY) continue;
+            my_distinct++;
+            const uint32_t c = (MODE == 0) ? s_vals[sl] : (uint32_t)__popc(s_vals[sl]);
+            if (c > a.x) {
+                emask |= 1u << i;
+                cnt++;
             }
-            const bool emit = key != EMPTY_KEY && c > a.x;
-            const unsigned long long pos = wave_append(emit, &a.ctr[1]);
-            if (emit && pos < a.out_cap) {
-                a.out_kmer[pos] = key;
-                a.out_count[pos] = c;
+        }
+        int x = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) s_wsum[wid] = x;
+        __syncthreads();
+        int pre = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < CNT_THREADS / 64; w++) {
+            pre += (w < wid) ? s_wsum[w] : 0;
+            tot += s_wsum[w];
+        }
+        if (tid == 0) s_obase = tot ? atomicAdd(&a.ctr[1], (unsigned long long)tot) : 0ull;
+        __syncthreads();
+        unsigned long long pos = s_obase + (unsigned long long)(pre + x - cnt);
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            if (!(emask & (1u << i))) continue;
+            const int sl = i * CNT_THREADS + tid;
+            if (pos < a.out_cap) {
+                a.out_kmer[pos] = s_keys[sl];
+                a.out_count[pos] = (MODE == 0) ? s_vals[sl] : (uint32_t)__popc(s_vals[sl]);
                 a.out_part[pos] = q;
             }
+            pos++;
         }
         __syncthreads();
     }
-    if (my_distinct) atomicAdd(&a.ctr[2], my_distinct);
+    block_sum_atomic(my_distinct, &a.ctr[2], s_red);
 }
 
 template <int MODE>
@@ -1070,6 +1204,38 @@ void batch_download(Batch &b, std::vector<mczi_counted_kmer> &out) {
 }
 
 Batch::~Batch() = default;
+
+// ---------------------------------------------------------------------------
+// Plain device copy (16-byte vectors) used to calibrate HBM byte counters and
+// to measure achievable bandwidth on the box.
+__global__ void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n16) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+double device_copy_probe(uint64_t bytes, int reps) {
+    Device &d = Device::get();
+    hipStream_t s = d.stream();
+    const uint64_t n16 = std::max<uint64_t>(bytes / 16, 1);
+    DevBuf a(n16 * 16), b(n16 * 16);
+    fill_zero_bytes(a.as<void>(), n16 * 16, s);
+    const unsigned g = (unsigned)std::min<uint64_t>((n16 + 255) / 256, (uint64_t)d.cu_count() * 16);
+    hipLaunchKernelGGL(k_copy16, dim3(g), dim3(256), 0, s, a.as<uint4>(), b.as<uint4>(), n16);
+    hipEvent_t e0, e1;
+    MCZI_HIP(hipEventCreate(&e0));
+    MCZI_HIP(hipEventCreate(&e1));
+    MCZI_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; r++)
+        hipLaunchKernelGGL(k_copy16, dim3(g), dim3(256), 0, s, a.as<uint4>(), b.as<uint4>(), n16);
+    MCZI_HIP(hipEventRecord(e1, s));
+    MCZI_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    MCZI_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms / std::max(reps, 1);
+}
 
 // ---------------------------------------------------------------------------
 // run_hashes boundary: per-window selected hashes of one sequence on the

@@ -186,6 +186,47 @@ void synth_generate(const mczi_synth_spec &spec, mczi_synth_reads &out) {
     }
 }
 
+void synth_reads_from(const uint32_t *const *gw, const uint64_t *gl, uint32_t ng, uint64_t n_reads,
+                      uint32_t read_len, double sub, uint64_t seed, mczi_synth_reads &out) {
+    require(ng > 0, MCZI_E_INVALID, "at least one genome is required");
+    std::vector<std::vector<uint8_t>> genomes(ng);
+    for (uint32_t g = 0; g < ng; g++) {
+        genomes[g].resize(gl[g]);
+        for (uint64_t p = 0; p < gl[g]; p++) genomes[g][p] = (uint8_t)((gw[g][p >> 4] >> ((p & 15u) * 2u)) & 3u);
+        require(gl[g] >= read_len, MCZI_E_INVALID, "genome shorter than the read length");
+    }
+    const uint64_t total = n_reads * read_len;
+    out.n_bases = total;
+    out.n_frag = n_reads;
+    out.n_words = (total + 15) / 16 + 32;
+    out.words = (uint32_t *)calloc(out.n_words, 4);
+    out.frag_off = (uint64_t *)malloc((n_reads + 1) * 8);
+    out.dataset = (uint32_t *)malloc(std::max<uint64_t>(n_reads, 1) * 4);
+    require(out.words && out.frag_off && out.dataset, MCZI_E_NOMEM, "synthetic reads: out of memory");
+    for (uint64_t i = 0; i <= n_reads; i++) out.frag_off[i] = i * read_len;
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; t++) {
+        th.emplace_back([&, t]() {
+            std::vector<uint8_t> tmp;
+            // threads own disjoint word ranges: read_len * reads-per-thread aligned to 16 bases
+            const uint64_t per = ((n_reads + nt - 1) / nt + 15) / 16 * 16;
+            const uint64_t r0 = std::min<uint64_t>(n_reads, per * t), r1 = std::min<uint64_t>(n_reads, r0 + per);
+            for (uint64_t i = r0; i < r1; i++) {
+                SplitMix64 rng(read_seed(seed, i));
+                sample_read(genomes[i % ng], read_len, sub, 0.0, 0.0, rng, tmp);
+                uint64_t p = i * read_len;
+                for (uint8_t c : tmp) {
+                    out.words[p >> 4] |= (uint32_t)c << ((p & 15u) * 2u);
+                    p++;
+                }
+                out.dataset[i] = (uint32_t)(i % ng);
+            }
+        });
+    }
+    for (auto &x : th) x.join();
+}
+
 void synth_write_fasta(const mczi_synth_reads &r, const std::string &path, int64_t ds_filter) {
     FILE *fp = fopen(path.c_str(), "wb");
     if (!fp) throw Error(MCZI_E_IO, "failed to create " + path);

@@ -35,8 +35,8 @@ struct ZiWork;  // device buffers of one in-flight batch
 std::unique_ptr<ZiWork> zi_make_work();
 void zi_free_work(ZiWork *w);
 // Enqueue upload + lookup + segmented sort + unique on stream s (async).
-void zi_enqueue(ZiWork &w, const ZiIndex &idx, const HostReads &q, const std::vector<uint32_t> &frag_req,
-                uint32_t n_req, hipStream_t s);
+void zi_enqueue(ZiWork &w, const ZiIndex &idx, HostReads &&q, std::vector<uint32_t> &&frag_req, uint32_t n_req,
+                hipStream_t s);
 // Wait for the batch and collect per-request results.
 void zi_collect(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, ZiQueryResult &out);
 

@@ -3,7 +3,8 @@
 // deduplicated), and the serving loop (continuous batching under an HBM
 // budget). Reference: src/zi.rs:114-189 (build_dictionary,
 // collect_query_kmers_absent_from_index, write_novel_simplitigs).
-#include <rocprim/device/device_segmented_radix_sort.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -243,53 +244,50 @@ __global__ void k_zi_iota(uint32_t *p, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = (uint32_t)i;
 }
-__global__ void k_zi_gather(const unsigned long long *src, const uint32_t *idx, unsigned long long *dst,
-                            const unsigned long long *seg_base, const unsigned long long *seg_end, uint32_t n_seg) {
-    // only positions inside segments hold data
-    for (uint32_t sgi = blockIdx.y; sgi < n_seg; sgi += gridDim.y) {
-        const uint64_t b = seg_base[sgi], e = seg_end[sgi];
+__global__ void k_zi_gather_range(const unsigned long long *src, const uint32_t *idx, unsigned long long *dst,
+                                  uint64_t b, uint64_t e) {
+    for (uint64_t i = b + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[idx[i]];
+}
+
+// Unique within each request's sorted segment: flag the first of every run
+// of equal keys among the segment's filled entries [base, cursor).
+template <bool WIDE>
+__global__ void k_zi_flags(const unsigned long long *lo, const unsigned long long *hi,
+                           const unsigned long long *seg_base, const unsigned long long *seg_end, uint32_t n_seg,
+                           uint32_t *flags, uint64_t cap) {
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) flags[cap] = 0;
+    for (uint32_t sg = blockIdx.y; sg < n_seg; sg += gridDim.y) {
+        const uint64_t b = seg_base[sg], e = seg_base[sg + 1], f = seg_end[sg];
         for (uint64_t i = b + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e;
-             i += (uint64_t)gridDim.x * blockDim.x)
-            dst[i] = src[idx[i]];
+             i += (uint64_t)gridDim.x * blockDim.x) {
+            bool fl = false;
+            if (i < f) {
+                fl = i == b || lo[i] != lo[i - 1];
+                if constexpr (WIDE) fl = fl || hi[i] != hi[i - 1];
+            }
+            flags[i] = fl ? 1u : 0u;
+        }
     }
 }
 
-// Ordered per-segment unique (one workgroup per segment).
 template <bool WIDE>
-__global__ __launch_bounds__(ZI_THREADS) void k_zi_seg_unique(const unsigned long long *lo, const unsigned long long *hi,
-                                                             const unsigned long long *seg_base,
-                                                             const unsigned long long *seg_end,
-                                                             unsigned long long *olo, unsigned long long *ohi,
-                                                             unsigned long long *seg_count) {
-    __shared__ int s_w[ZI_THREADS / 64];
-    const uint32_t sg = blockIdx.x;
-    const uint64_t b = seg_base[sg], e = seg_end[sg];
-    uint64_t written = 0;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (uint64_t c = b; c < e; c += ZI_THREADS) {
-        const uint64_t i = c + threadIdx.x;
-        bool f = false;
-        if (i < e) {
-            f = (i == b) || lo[i] != lo[i - 1];
-            if constexpr (WIDE) f = f || (i > b && hi[i] != hi[i - 1]);
-        }
-        const uint64_t m = __ballot(f);
-        const int pre = __popcll(m & ((1ull << lane) - 1ull));
-        if (lane == 0) s_w[wid] = __popcll(m);
-        __syncthreads();
-        int off = 0, tot = 0;
-        for (int w = 0; w < ZI_THREADS / 64; w++) {
-            off += (w < wid) ? s_w[w] : 0;
-            tot += s_w[w];
-        }
-        if (f) {
-            olo[b + written + off + pre] = lo[i];
-            if constexpr (WIDE) ohi[b + written + off + pre] = hi[i];
-        }
-        written += tot;
-        __syncthreads();
+__global__ void k_zi_scatter(const unsigned long long *lo, const unsigned long long *hi, const uint32_t *flags,
+                             const uint32_t *pos, uint64_t cap, unsigned long long *olo, unsigned long long *ohi) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (!flags[i]) continue;
+        olo[pos[i]] = lo[i];
+        if constexpr (WIDE) ohi[pos[i]] = hi[i];
     }
-    if (threadIdx.x == 0) seg_count[sg] = written;
+}
+
+__global__ void k_zi_seg_counts(const uint32_t *pos, const unsigned long long *seg_base, uint32_t n_seg,
+                                unsigned long long *out_start, unsigned long long *out_count) {
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n_seg; r += gridDim.x * blockDim.x) {
+        const uint64_t a = pos[seg_base[r]], b = pos[seg_base[r + 1]];
+        out_start[r] = a;
+        out_count[r] = b - a;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -370,8 +368,13 @@ void zi_build_index(ZiIndex &idx, HostReads &&reads, uint32_t k, uint32_t m, hip
 }
 
 struct ZiWork {
-    DevBuf words, off, req, ts, tf, tl, seg_base, seg_cursor, seg_count, olo, ohi, slo, shi, ulo, uhi, idx, idx2,
-        tmp, ctr;
+    // host copies kept alive until the batch's asynchronous copies are done
+    HostReads q;
+    std::vector<uint32_t> frag_req;
+    std::vector<uint64_t> t_start;
+    std::vector<uint32_t> t_frag, t_len;
+    DevBuf words, off, req, ts, tf, tl, seg_base, seg_cursor, seg_count, seg_start, olo, ohi, slo, shi, ulo, uhi, idx,
+        idx2, flags, pos, tmp, ctr;
     uint64_t cap = 0;
     std::vector<uint64_t> h_base, h_count;
     uint64_t lookups = 0;
@@ -394,9 +397,13 @@ uint64_t zi_batch_bytes(uint64_t bases, uint64_t frags, bool wide) {
     return bases / 4 + 256 + frags * 12 + tiles * 16 + bases * per_key + bases * 8 + (64u << 10);
 }
 
-void zi_enqueue(ZiWork &w, const ZiIndex &idx, const HostReads &q, const std::vector<uint32_t> &frag_req,
+void zi_enqueue(ZiWork &w, const ZiIndex &idx, HostReads &&q_in, std::vector<uint32_t> &&frag_req_in,
                 uint32_t n_req, hipStream_t s) {
     const bool wide = idx.wide;
+    w.q = std::move(q_in);
+    w.frag_req = std::move(frag_req_in);
+    const HostReads &q = w.q;
+    const std::vector<uint32_t> &frag_req = w.frag_req;
     const uint64_t nfr = q.n_frag();
     require(nfr < (1ull << 32), MCZI_E_INVALID, "too many query fragments in one batch");
     // per-request segment bases: capacity = the request's window count
@@ -409,6 +416,7 @@ void zi_enqueue(ZiWork &w, const ZiIndex &idx, const HostReads &q, const std::ve
     w.h_base.assign(n_req + 1, 0);
     for (uint32_t r = 0; r < n_req; r++) w.h_base[r + 1] = w.h_base[r] + seg_cap[r];
     const uint64_t cap = std::max<uint64_t>(w.h_base[n_req], 1);
+    require(cap < (1ull << 32) - 1, MCZI_E_INVALID, "too many query k-mers in one batch (limit 2^32 - 2)");
     w.cap = cap;
     w.words.ensure(q.words.size() * 4);
     w.off.ensure(q.off.size() * 8);
@@ -419,17 +427,23 @@ void zi_enqueue(ZiWork &w, const ZiIndex &idx, const HostReads &q, const std::ve
         MCZI_HIP(hipMemcpyAsync(w.req.as<void>(), frag_req.data(), nfr * 4, hipMemcpyHostToDevice, s));
     HostTiles t;
     make_tiles(q.off, t);
+    w.t_start = t.start;
+    w.t_frag = t.frag;
+    w.t_len = t.len;
     w.ts.ensure(t.start.size() * 8 + 8);
     w.tf.ensure(t.frag.size() * 4 + 4);
     w.tl.ensure(t.len.size() * 4 + 4);
     if (!t.start.empty()) {
-        MCZI_HIP(hipMemcpyAsync(w.ts.as<void>(), t.start.data(), t.start.size() * 8, hipMemcpyHostToDevice, s));
-        MCZI_HIP(hipMemcpyAsync(w.tf.as<void>(), t.frag.data(), t.frag.size() * 4, hipMemcpyHostToDevice, s));
-        MCZI_HIP(hipMemcpyAsync(w.tl.as<void>(), t.len.data(), t.len.size() * 4, hipMemcpyHostToDevice, s));
+        MCZI_HIP(hipMemcpyAsync(w.ts.as<void>(), w.t_start.data(), t.start.size() * 8, hipMemcpyHostToDevice, s));
+        MCZI_HIP(hipMemcpyAsync(w.tf.as<void>(), w.t_frag.data(), t.frag.size() * 4, hipMemcpyHostToDevice, s));
+        MCZI_HIP(hipMemcpyAsync(w.tl.as<void>(), w.t_len.data(), t.len.size() * 4, hipMemcpyHostToDevice, s));
     }
     w.seg_base.ensure((n_req + 1) * 8);
     w.seg_cursor.ensure((n_req + 1) * 8);
     w.seg_count.ensure((n_req + 1) * 8);
+    w.seg_start.ensure((n_req + 1) * 8);
+    w.flags.ensure((cap + 1) * 4);
+    w.pos.ensure((cap + 1) * 4);
     MCZI_HIP(hipMemcpyAsync(w.seg_base.as<void>(), w.h_base.data(), (n_req + 1) * 8, hipMemcpyHostToDevice, s));
     MCZI_HIP(hipMemcpyAsync(w.seg_cursor.as<void>(), w.h_base.data(), (n_req + 1) * 8, hipMemcpyHostToDevice, s));
     w.olo.ensure(cap * 8);
@@ -444,6 +458,8 @@ void zi_enqueue(ZiWork &w, const ZiIndex &idx, const HostReads &q, const std::ve
     }
     w.ctr.ensure(64);
     zfill(w.ctr.as<unsigned long long>(), 8, 0, s);
+    zfill(w.olo.as<unsigned long long>(), cap, EMPTY_KEY, s);
+    if (wide) zfill(w.ohi.as<unsigned long long>(), cap, EMPTY_KEY, s);
 
     ZiTileArgs a{};
     a.words = w.words.as<uint32_t>();
@@ -470,47 +486,69 @@ void zi_enqueue(ZiWork &w, const ZiIndex &idx, const HostReads &q, const std::ve
     const unsigned long long *sb = w.seg_base.as<unsigned long long>();
     const unsigned long long *se = w.seg_cursor.as<unsigned long long>();
     const int kbits = (int)std::min<uint32_t>(64, 2 * idx.k);
-    size_t tmp = 0;
+    const int hbits = 2 * (int)idx.k - 64;
+    uint64_t max_seg = 1;
+    for (uint32_t r = 0; r < n_req; r++) max_seg = std::max<uint64_t>(max_seg, seg_cap[r]);
+    // Each request's segment is sorted with a device-wide radix sort; unused
+    // entries hold the all-ones sentinel, which sorts last (no canonical
+    // k-mer of k <= 63 equals it).
+    size_t tmp = 0, t2 = 0, t3 = 0;
     if (!wide) {
-        MCZI_HIP(rocprim::segmented_radix_sort_keys(nullptr, tmp, w.olo.as<unsigned long long>(),
-                                                    w.slo.as<unsigned long long>(), (unsigned)cap, n_req, sb, se,
-                                                    0, kbits, s));
-        w.tmp.ensure(tmp + 16);
-        MCZI_HIP(rocprim::segmented_radix_sort_keys(w.tmp.as<void>(), tmp, w.olo.as<unsigned long long>(),
-                                                    w.slo.as<unsigned long long>(), (unsigned)cap, n_req, sb, se,
-                                                    0, kbits, s));
-        hipLaunchKernelGGL(k_zi_seg_unique<false>, dim3(std::max<uint32_t>(n_req, 1)), dim3(ZI_THREADS), 0, s,
-                           w.slo.as<unsigned long long>(), nullptr, sb, se, w.ulo.as<unsigned long long>(), nullptr,
-                           w.seg_count.as<unsigned long long>());
+        MCZI_HIP(rocprim::radix_sort_keys(nullptr, tmp, w.olo.as<unsigned long long>(), w.slo.as<unsigned long long>(),
+                                          (size_t)max_seg, 0, kbits, s));
     } else {
-        // LSD over (lo, hi): stable segmented sorts carrying an index
-        hipLaunchKernelGGL(k_zi_iota, dim3(zgrid(cap)), dim3(256), 0, s, w.idx.as<uint32_t>(), cap);
-        size_t t1 = 0, t2 = 0;
-        MCZI_HIP(rocprim::segmented_radix_sort_pairs(nullptr, t1, w.olo.as<unsigned long long>(),
-                                                     w.slo.as<unsigned long long>(), w.idx.as<uint32_t>(),
-                                                     w.idx2.as<uint32_t>(), (unsigned)cap, n_req, sb, se, 0, 64, s));
-        const int hbits = 2 * (int)idx.k - 64;
-        MCZI_HIP(rocprim::segmented_radix_sort_pairs(nullptr, t2, w.shi.as<unsigned long long>(),
-                                                     w.uhi.as<unsigned long long>(), w.idx2.as<uint32_t>(),
-                                                     w.idx.as<uint32_t>(), (unsigned)cap, n_req, sb, se, 0, hbits, s));
-        w.tmp.ensure(std::max(t1, t2) + 16);
-        MCZI_HIP(rocprim::segmented_radix_sort_pairs(w.tmp.as<void>(), t1, w.olo.as<unsigned long long>(),
-                                                     w.slo.as<unsigned long long>(), w.idx.as<uint32_t>(),
-                                                     w.idx2.as<uint32_t>(), (unsigned)cap, n_req, sb, se, 0, 64, s));
-        dim3 gg(zgrid(cap), std::min<uint32_t>(std::max<uint32_t>(n_req, 1), 64));
-        hipLaunchKernelGGL(k_zi_gather, gg, dim3(256), 0, s, w.ohi.as<unsigned long long>(), w.idx2.as<uint32_t>(),
-                           w.shi.as<unsigned long long>(), sb, se, n_req);
-        MCZI_HIP(rocprim::segmented_radix_sort_pairs(w.tmp.as<void>(), t2, w.shi.as<unsigned long long>(),
-                                                     w.uhi.as<unsigned long long>(), w.idx2.as<uint32_t>(),
-                                                     w.idx.as<uint32_t>(), (unsigned)cap, n_req, sb, se, 0, hbits, s));
-        // original positions of the (hi, lo)-sorted order -> gather lo
-        hipLaunchKernelGGL(k_zi_gather, gg, dim3(256), 0, s, w.olo.as<unsigned long long>(), w.idx.as<uint32_t>(),
-                           w.slo.as<unsigned long long>(), sb, se, n_req);
-        hipLaunchKernelGGL(k_zi_seg_unique<true>, dim3(std::max<uint32_t>(n_req, 1)), dim3(ZI_THREADS), 0, s,
-                           w.slo.as<unsigned long long>(), w.uhi.as<unsigned long long>(), sb, se,
-                           w.ulo.as<unsigned long long>(), w.ohi.as<unsigned long long>(),
-                           w.seg_count.as<unsigned long long>());
+        MCZI_HIP(rocprim::radix_sort_pairs(nullptr, tmp, w.olo.as<unsigned long long>(), w.slo.as<unsigned long long>(),
+                                           w.idx.as<uint32_t>(), w.idx2.as<uint32_t>(), (size_t)max_seg, 0, 64, s));
+        MCZI_HIP(rocprim::radix_sort_pairs(nullptr, t2, w.shi.as<unsigned long long>(), w.uhi.as<unsigned long long>(),
+                                           w.idx2.as<uint32_t>(), w.idx.as<uint32_t>(), (size_t)max_seg, 0, hbits, s));
     }
+    MCZI_HIP(rocprim::exclusive_scan(nullptr, t3, w.flags.as<uint32_t>(), w.pos.as<uint32_t>(), 0u, (size_t)cap + 1,
+                                     rocprim::plus<uint32_t>(), s));
+    w.tmp.ensure(std::max(std::max(tmp, t2), t3) + 16);
+    if (wide) hipLaunchKernelGGL(k_zi_iota, dim3(zgrid(cap)), dim3(256), 0, s, w.idx.as<uint32_t>(), cap);
+    for (uint32_t r = 0; r < n_req; r++) {
+        const uint64_t b0 = w.h_base[r], n = seg_cap[r];
+        if (n == 0) continue;
+        size_t ts1 = tmp, ts2 = t2;
+        if (!wide) {
+            MCZI_HIP(rocprim::radix_sort_keys(w.tmp.as<void>(), ts1, w.olo.as<unsigned long long>() + b0,
+                                              w.slo.as<unsigned long long>() + b0, (size_t)n, 0, kbits, s));
+        } else {
+            // LSD over (lo, hi) carrying original positions
+            MCZI_HIP(rocprim::radix_sort_pairs(w.tmp.as<void>(), ts1, w.olo.as<unsigned long long>() + b0,
+                                               w.slo.as<unsigned long long>() + b0, w.idx.as<uint32_t>() + b0,
+                                               w.idx2.as<uint32_t>() + b0, (size_t)n, 0, 64, s));
+            hipLaunchKernelGGL(k_zi_gather_range, dim3(zgrid(n)), dim3(256), 0, s, w.ohi.as<unsigned long long>(),
+                               w.idx2.as<uint32_t>(), w.shi.as<unsigned long long>(), b0, b0 + n);
+            MCZI_HIP(rocprim::radix_sort_pairs(w.tmp.as<void>(), ts2, w.shi.as<unsigned long long>() + b0,
+                                               w.uhi.as<unsigned long long>() + b0, w.idx2.as<uint32_t>() + b0,
+                                               w.idx.as<uint32_t>() + b0, (size_t)n, 0, hbits, s));
+            hipLaunchKernelGGL(k_zi_gather_range, dim3(zgrid(n)), dim3(256), 0, s, w.olo.as<unsigned long long>(),
+                               w.idx.as<uint32_t>(), w.slo.as<unsigned long long>(), b0, b0 + n);
+        }
+    }
+    // unique per request: flags -> exclusive scan -> scatter
+    dim3 gg(zgrid(cap), std::min<uint32_t>(std::max<uint32_t>(n_req, 1), 64));
+    const unsigned long long *khi = wide ? w.uhi.as<unsigned long long>() : nullptr;
+    if (wide)
+        hipLaunchKernelGGL(k_zi_flags<true>, gg, dim3(256), 0, s, w.slo.as<unsigned long long>(), khi, sb, se, n_req,
+                           w.flags.as<uint32_t>(), cap);
+    else
+        hipLaunchKernelGGL(k_zi_flags<false>, gg, dim3(256), 0, s, w.slo.as<unsigned long long>(), khi, sb, se, n_req,
+                           w.flags.as<uint32_t>(), cap);
+    size_t ts3 = t3;
+    MCZI_HIP(rocprim::exclusive_scan(w.tmp.as<void>(), ts3, w.flags.as<uint32_t>(), w.pos.as<uint32_t>(), 0u,
+                                     (size_t)cap + 1, rocprim::plus<uint32_t>(), s));
+    if (wide)
+        hipLaunchKernelGGL(k_zi_scatter<true>, dim3(zgrid(cap)), dim3(256), 0, s, w.slo.as<unsigned long long>(), khi,
+                           w.flags.as<uint32_t>(), w.pos.as<uint32_t>(), cap, w.ulo.as<unsigned long long>(),
+                           w.ohi.as<unsigned long long>());
+    else
+        hipLaunchKernelGGL(k_zi_scatter<false>, dim3(zgrid(cap)), dim3(256), 0, s, w.slo.as<unsigned long long>(), khi,
+                           w.flags.as<uint32_t>(), w.pos.as<uint32_t>(), cap, w.ulo.as<unsigned long long>(),
+                           (unsigned long long *)nullptr);
+    hipLaunchKernelGGL(k_zi_seg_counts, dim3(1), dim3(256), 0, s, w.pos.as<uint32_t>(), sb, n_req,
+                       w.seg_start.as<unsigned long long>(), w.seg_count.as<unsigned long long>());
     MCZI_HIP(hipEventRecord(w.done, s));
 }
 
@@ -518,8 +556,10 @@ void zi_collect(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, Zi
     MCZI_HIP(hipEventSynchronize(w.done));
     MCZI_HIP(hipGetLastError());
     w.h_count.assign(n_req + 1, 0);
+    std::vector<uint64_t> h_start(n_req + 1, 0);
     unsigned long long lk = 0;
     MCZI_HIP(hipMemcpyAsync(w.h_count.data(), w.seg_count.as<void>(), n_req * 8, hipMemcpyDeviceToHost, s));
+    MCZI_HIP(hipMemcpyAsync(h_start.data(), w.seg_start.as<void>(), n_req * 8, hipMemcpyDeviceToHost, s));
     MCZI_HIP(hipMemcpyAsync(&lk, w.ctr.as<void>(), 8, hipMemcpyDeviceToHost, s));
     MCZI_HIP(hipStreamSynchronize(s));
     w.lookups = lk;
@@ -531,12 +571,12 @@ void zi_collect(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, Zi
         const uint64_t n = w.h_count[r];
         lo[r].resize(n);
         if (n)
-            MCZI_HIP(hipMemcpyAsync(lo[r].data(), w.ulo.as<unsigned long long>() + w.h_base[r], n * 8,
+            MCZI_HIP(hipMemcpyAsync(lo[r].data(), w.ulo.as<unsigned long long>() + h_start[r], n * 8,
                                     hipMemcpyDeviceToHost, s));
         if (idx.wide) {
             hi[r].resize(n);
             if (n)
-                MCZI_HIP(hipMemcpyAsync(hi[r].data(), w.ohi.as<unsigned long long>() + w.h_base[r], n * 8,
+                MCZI_HIP(hipMemcpyAsync(hi[r].data(), w.ohi.as<unsigned long long>() + h_start[r], n * 8,
                                         hipMemcpyDeviceToHost, s));
         }
     }
@@ -588,6 +628,7 @@ struct mczi_zi_server {
         std::vector<ZiRequest *> batch;
         uint64_t bytes = 0;
         bool busy = false;
+        bool enqueued = false;
         uint64_t seq = 0;
     };
     std::vector<Slot> slots;
@@ -647,6 +688,7 @@ void mczi_zi_server::admit_loop() {
         }
         Slot &sl = slots[free_slot];
         sl.busy = true;
+        sl.enqueued = false;
         sl.batch = batch;
         sl.bytes = bytes;
         sl.seq = seq_issue++;
@@ -682,7 +724,7 @@ void mczi_zi_server::admit_loop() {
         std::string msg;
         try {
             if (!sl.work) sl.work = zi_make_work();
-            zi_enqueue(*sl.work, idx, q, frag_req, (uint32_t)batch.size(), sl.stream);
+            zi_enqueue(*sl.work, idx, std::move(q), std::move(frag_req), (uint32_t)batch.size(), sl.stream);
         } catch (const Error &e) {
             err = e.code;
             msg = e.what();
@@ -699,6 +741,8 @@ void mczi_zi_server::admit_loop() {
             sl.busy = false;
             sl.batch.clear();
             seq_done = std::max(seq_done, sl.seq + 1);
+        } else {
+            sl.enqueued = true;
         }
         cv.notify_all();
     }
@@ -710,7 +754,7 @@ void mczi_zi_server::done_loop() {
         // next issued batch in order
         Slot *sl = nullptr;
         for (auto &s : slots)
-            if (s.busy && s.seq == seq_done && !s.batch.empty()) sl = &s;
+            if (s.busy && s.enqueued && s.seq == seq_done && !s.batch.empty()) sl = &s;
         if (!sl) {
             if (stop) return;
             cv.wait(lk);
@@ -749,6 +793,7 @@ void mczi_zi_server::done_loop() {
         }
         used -= sl->bytes;
         sl->busy = false;
+        sl->enqueued = false;
         sl->batch.clear();
         seq_done++;
         cv.notify_all();
@@ -796,7 +841,7 @@ std::vector<mczi_counted_kmer> zi_run(const char *index_path, const char *const 
     q.finish();
     std::vector<uint32_t> fr(q.n_frag(), 0);
     auto w = zi_make_work();
-    zi_enqueue(*w, idx, q, fr, 1, d.stream());
+    zi_enqueue(*w, idx, std::move(q), std::move(fr), 1, d.stream());
     ZiQueryResult res;
     zi_collect(*w, idx, 1, d.stream(), res);
     return std::move(res.absent[0]);
@@ -908,6 +953,33 @@ int mczi_zi_server_submit(mczi_zi_server *s, const char *fastx, size_t len, int 
         r->t_submit = now_ms();
         parse_fastx_text(fastx, len, s->idx.k, 0, r->reads);
         r->reads.finish();
+        r->make_fasta = make_fasta != 0;
+        std::lock_guard<std::mutex> lk(s->mu);
+        require(s->fatal.empty(), MCZI_E_DEVICE, s->fatal);
+        r->id = s->next_id++;
+        *request_id = r->id;
+        s->queue.push_back(r.get());
+        s->reqs[r->id] = std::move(r);
+        s->cv.notify_all();
+    });
+    if (rc) mczi_set_error_from_zi(z_err.c_str());
+    return rc;
+}
+
+int mczi_zi_server_submit_packed(mczi_zi_server *s, const uint32_t *words, const uint64_t *frag_off, size_t n_frag,
+                                 int make_fasta, uint64_t *request_id) {
+    int rc = zguard("failed to submit ZI request", [&] {
+        require(s && request_id && frag_off && (words || n_frag == 0), MCZI_E_INVALID, "null argument");
+        auto r = std::make_unique<ZiRequest>();
+        r->t_submit = now_ms();
+        HostReads &h = r->reads;
+        const uint64_t nb = frag_off[n_frag];
+        h.words.assign(words, words + (nb + 15) / 16);
+        h.cur_ = nb;
+        h.off.assign(frag_off, frag_off + n_frag + 1);
+        for (size_t i = 0; i < n_frag; i++)
+            require(frag_off[i + 1] >= frag_off[i], MCZI_E_INVALID, "fragment offsets must not decrease");
+        h.finish();
         r->make_fasta = make_fasta != 0;
         std::lock_guard<std::mutex> lk(s->mu);
         require(s->fatal.empty(), MCZI_E_DEVICE, s->fatal);
