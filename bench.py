@@ -236,12 +236,16 @@ def main() -> int:
     copy_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
 
     nb = reads.n_bases
-    rec_bytes = prof["n_records"] * 16
+    R = prof["n_records"]
     out_bytes = prof["n_out"] * 16
+    # algorithmic (compulsory) bytes per kernel group, one pass each:
+    #  pass 1 scan: packed reads in, staged records (16 B) + minimizer keys (4 B) out
+    #  partitioning: staged records + keys in, partition-ordered records + keys out
+    #  counting: partitioned records + keys in, (k-mer, count, partition) out
     kernels = {
-        "k_scan_pass1": (nb / 4, prof["ms_scan1"]),
-        "k_scan_pass2": (nb / 4 + rec_bytes, prof["ms_scan2"]),
-        "k_count": (rec_bytes + out_bytes, prof["ms_count"]),
+        "k_scan": (nb / 4 + 20 * R, prof["ms_scan1"]),
+        "partition_sort_gather": (40 * R, prof["ms_scan2"]),
+        "k_count": (20 * R + out_bytes, prof["ms_count"]),
     }
     dom = max(kernels, key=lambda n: kernels[n][1])
     alg, ms = kernels[dom]
@@ -270,6 +274,7 @@ def main() -> int:
         "phase_ms": {k2: prof[k2] for k2 in ("ms_scan1", "ms_scan2", "ms_count", "ms_sort", "ms_total")},
         "stats": {k2: last[k2] for k2 in ("n_windows", "n_pieces", "n_records", "n_distinct", "n_overflow",
                                            "n_out", "internal_partitions")},
+        "device": M.device_info(),
         "copy_probe": {"bytes": copy_bytes, "ms": copy_ms, "read_plus_write_GBs": copy_gbs},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

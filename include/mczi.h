@@ -180,12 +180,21 @@ typedef struct {
 int mczi_batch_count(mczi_batch *b, const mczi_counter_config *cfg, int mode, int launch,
                      int order, mczi_run_stats *stats);
 int mczi_batch_download(mczi_batch *b, mczi_counted_kmer **out, size_t *n_out);
+/* Autotuner (no counterpart in the reference): times the count at several
+ * internal partition sizes (expected k-mer instances per LDS table) and keeps
+ * the fastest for later mczi_batch_count calls on this batch; the result
+ * does not change the output. *instances_per_partition gets the choice. */
+int mczi_batch_autotune(mczi_batch *b, const mczi_counter_config *cfg, int mode, int launch, int order,
+                        double *instances_per_partition);
 
 /* Device copy of `bytes` bytes (16-byte vector loads and stores), `reps`
  * times; *ms_per_copy gets the average duration. No counterpart in the
  * reference: it calibrates HBM byte counters and measures the box's copy
  * bandwidth for roofline reporting. */
 int mczi_copy_probe(uint64_t bytes, int reps, double *ms_per_copy);
+/* Device description as JSON text (name, gcnArchName, CUs, clocks, memory,
+ * LDS limits); no counterpart in the reference. */
+int mczi_device_info(char *buf, size_t len);
 
 /* ---- ZI (src/zi.rs) -------------------------------------------------------- */
 /* ZI main path (src/zi.rs:64-100): absent query k-mers as simplitig FASTA.
@@ -200,14 +209,15 @@ int mczi_zi_absent(const char *index_path, const char *const *query_paths, size_
 
 /* ---- ZI serving: a device-resident index and a continuous-batching loop ----
  * No counterpart in the reference (which runs one query set per process).
- * Requests are FASTA/FASTQ text; each result is the sorted absent set and its
- * simplitig FASTA text. */
+ * Requests are FASTA/FASTQ text (or packed reads); each result is the sorted
+ * absent set and, on request, its simplitig FASTA text. */
 typedef struct mczi_zi_server mczi_zi_server;
 typedef struct {
     uint64_t request_id;
     uint64_t n_windows;   /* query k-mers looked up */
     uint64_t n_absent;    /* distinct absent k-mers */
-    mczi_counted_kmer *absent;
+    uint64_t *absent_lo;  /* sorted absent k-mers, low 64 bits */
+    uint64_t *absent_hi;  /* high 64 bits (k > 32), else NULL */
     char *fasta;          /* simplitig FASTA text (NUL-terminated) */
     size_t fasta_len;
     double queue_ms;      /* submit -> admitted */

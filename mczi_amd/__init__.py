@@ -79,7 +79,8 @@ class _SynthReads(C.Structure):
 
 class _ZiResult(C.Structure):
     _fields_ = [("request_id", C.c_uint64), ("n_windows", C.c_uint64), ("n_absent", C.c_uint64),
-                ("absent", C.POINTER(_Counted)), ("fasta", C.c_char_p), ("fasta_len", C.c_size_t),
+                ("absent_lo", C.POINTER(C.c_uint64)), ("absent_hi", C.POINTER(C.c_uint64)),
+                ("fasta", C.c_char_p), ("fasta_len", C.c_size_t),
                 ("queue_ms", C.c_double), ("latency_ms", C.c_double)]
 
 
@@ -118,8 +119,10 @@ _sig("mczi_batch_from_packed", C.c_int, [_P, _P, C.c_size_t, _P, C.POINTER(_P)])
 _sig("mczi_batch_info", C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)])
 _sig("mczi_batch_free", None, [_P])
 _sig("mczi_batch_count", C.c_int, [_P, C.POINTER(_Config), C.c_int, C.c_int, C.c_int, C.POINTER(RunStats)])
+_sig("mczi_batch_autotune", C.c_int, [_P, C.POINTER(_Config), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)])
 _sig("mczi_batch_download", C.c_int, [_P, C.POINTER(C.POINTER(_Counted)), C.POINTER(C.c_size_t)])
 _sig("mczi_copy_probe", C.c_int, [C.c_uint64, C.c_int, C.POINTER(C.c_double)])
+_sig("mczi_device_info", C.c_int, [C.c_char_p, C.c_size_t])
 _sig("mczi_zi_subtract", C.c_int, [C.c_char_p, _CP, C.c_size_t, C.c_uint32, C.c_uint32, C.c_char_p,
                                    C.c_uint32, C.c_uint32])
 _sig("mczi_zi_absent", C.c_int, [C.c_char_p, _CP, C.c_size_t, C.c_uint32, C.POINTER(C.POINTER(_Counted)),
@@ -149,7 +152,7 @@ EXPORTED = [
     "mczi_count_inputs_to_kmer_fasta_path", "mczi_count_datasets_to_kmer_fasta_path", "mczi_write_fasta_path",
     "mczi_write_fasta_after_index", "mczi_write_kff_path", "mczi_decode_kmer", "mczi_minimizer_run_hashes",
     "mczi_batch_from_files", "mczi_batch_from_packed", "mczi_batch_info", "mczi_batch_free", "mczi_batch_count",
-    "mczi_batch_download", "mczi_copy_probe", "mczi_zi_subtract", "mczi_zi_absent", "mczi_zi_server_create",
+    "mczi_batch_download", "mczi_batch_autotune", "mczi_copy_probe", "mczi_device_info", "mczi_zi_subtract", "mczi_zi_absent", "mczi_zi_server_create",
     "mczi_zi_server_create_from_text", "mczi_zi_server_submit", "mczi_zi_server_submit_packed",
     "mczi_zi_server_poll", "mczi_zi_result_free",
     "mczi_zi_server_stats", "mczi_zi_server_destroy", "mczi_synth_generate", "mczi_synth_genome",
@@ -181,6 +184,13 @@ def copy_probe(nbytes: int, reps: int = 10) -> float:
     ms = C.c_double()
     _check(_lib.mczi_copy_probe(nbytes, reps, C.byref(ms)))
     return ms.value
+
+
+def device_info() -> dict:
+    import json
+    buf = C.create_string_buffer(4096)
+    _check(_lib.mczi_device_info(buf, 4096))
+    return json.loads(buf.value.decode())
 
 
 def version() -> str:
@@ -342,6 +352,12 @@ class Batch:
         st = RunStats()
         _check(_lib.mczi_batch_count(self._h, C.byref(cfg._c()), mode, launch, order, C.byref(st)))
         return st
+
+    def autotune(self, cfg: CounterConfig, mode: int = MODE_TWO_PASS, launch: int = LAUNCH_BATCHED,
+                 order: int = ORDER_ENCODED) -> float:
+        t = C.c_double()
+        _check(_lib.mczi_batch_autotune(self._h, C.byref(cfg._c()), mode, launch, order, C.byref(t)))
+        return t.value
 
     def download(self) -> np.ndarray:
         p = C.POINTER(_Counted)()

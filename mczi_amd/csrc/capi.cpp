@@ -359,6 +359,24 @@ int mczi_copy_probe(uint64_t bytes, int reps, double *ms_per_copy) {
     });
 }
 
+int mczi_device_info(char *buf, size_t len) {
+    return guarded("device query failed", [&] {
+        require(buf != nullptr && len > 0, MCZI_E_INVALID, "null buffer");
+        Device &d = Device::get();
+        hipDeviceProp_t p;
+        MCZI_HIP(hipGetDeviceProperties(&p, d.device()));
+        int optin = 0;
+        (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, d.device());
+        snprintf(buf, len,
+                 "{\"name\": \"%s\", \"arch\": \"%s\", \"cus\": %d, \"clock_khz\": %d, "
+                 "\"mem_clock_khz\": %d, \"bus_width\": %d, \"total_mem\": %zu, \"lds_per_block\": %zu, "
+                 "\"lds_per_cu\": %zu, \"lds_per_block_optin\": %d, \"l2_bytes\": %d}",
+                 p.name, p.gcnArchName, p.multiProcessorCount, p.clockRate, p.memoryClockRate,
+                 p.memoryBusWidth, (size_t)p.totalGlobalMem, (size_t)p.sharedMemPerBlock,
+                 (size_t)p.maxSharedMemoryPerMultiProcessor, optin, p.l2CacheSize);
+    });
+}
+
 // ---- batched device API -----------------------------------------------------
 struct mczi_batch {
     std::unique_ptr<Batch> b;
@@ -446,6 +464,18 @@ int mczi_batch_count(mczi_batch *b, const mczi_counter_config *cfg, int mode, in
             return;
         }
         batch_count(*b->b, *cfg, mode, launch, order, stats);
+    });
+}
+
+int mczi_batch_autotune(mczi_batch *b, const mczi_counter_config *cfg, int mode, int launch, int order,
+                        double *instances_per_partition) {
+    return guarded("failed to autotune batch", [&] {
+        require(b != nullptr && cfg != nullptr, MCZI_E_INVALID, "null argument");
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        if (mode == MCZI_MODE_DATASETS) validate_datasets(*cfg);
+        else validate_normal(*cfg);
+        const double t = batch_autotune(*b->b, *cfg, mode, launch, order);
+        if (instances_per_partition) *instances_per_partition = t;
     });
 }
 

@@ -157,6 +157,7 @@ std::unique_ptr<Batch> make_batch(HostReads &&reads);
 void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch, int order,
                  mczi_run_stats *stats);
 void batch_download(Batch &b, std::vector<mczi_counted_kmer> &out);
+double batch_autotune(Batch &b, const mczi_counter_config &cfg, int mode, int launch, int order);
 void free_plan(McPlan *p);
 
 // Plain device copy of `bytes` (average ms per launch over `reps`).

@@ -58,12 +58,14 @@ struct ScanArgs {
     uint32_t sat;
     uint32_t p_int;
     uint32_t wrec;
-    uint32_t *mtab;
-    uint32_t *hist;
-    unsigned long long *cursor;
-    ulonglong2 *records;
     uint32_t *win_out;
     unsigned long long *stats;  // [0] pieces [1] records
+    // pass-1 staging of super-k-mer records (null: count only)
+    ulonglong2 *stage_rec;
+    uint32_t *stage_key;
+    uint64_t stage_cap;
+    unsigned long long *stage_ctr;
+    unsigned long long *err;
 };
 
 __device__ __forceinline__ uint32_t lds_code(const uint32_t *sw, int i) {
@@ -131,12 +133,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
     __shared__ int32_t s_fs[TILE + 2];
     __shared__ uint32_t s_pst[TILE];
     __shared__ int s_wsum[SCAN_THREADS / 64];
+    __shared__ unsigned long long s_sbase;
 
     const int tid = threadIdx.x;
     const int k = a.k, m = a.m, w = a.w;
     const uint32_t R1 = (NT_ROT * (uint32_t)m) & 31u;
     const uint32_t R2 = (NT_ROT * (uint32_t)(m - 1)) & 31u;
-    unsigned long long my_pieces = 0, my_records = 0;
+    unsigned long long my_pieces = 0;
 
     for (uint64_t t = a.tile_begin + blockIdx.x; t < a.tile_end; t += gridDim.x) {
         const uint64_t ts = a.tile_start[t];
@@ -304,6 +307,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
             if (smask & (1u << i)) s_pst[o++] = (uint32_t)(r0 + i);
         __syncthreads();
 
+        // Pieces -> super-k-mer records of at most wrec windows (bases
+        // [start, end + k - 1), src/lib.rs:3483-3560), staged contiguously per
+        // tile (one reservation per tile; with no staging buffer only the
+        // count is taken) together with the minimizer key of their run.
+        int my_nrec = 0;
         for (int pi = tid; pi < np; pi += SCAN_THREADS) {
             const int s = (int)s_pst[pi];
             int e = (pi + 1 < np) ? (int)s_pst[pi + 1] : tl;
@@ -315,78 +323,85 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
             }
             const int vend = s_fs[lo + 1] - k + 1;
             if (vend < e) e = vend;
-            const uint32_t v = s_v[s];
-            const uint32_t key = spread_hash_u32(v);          // visit_direct_hash_runs
-            const uint32_t q = spread_hash_u32(key) % a.p_int;  // minimizer_shard
-            const uint32_t len = (uint32_t)(e - s);
-            const uint32_t nrec = (len + a.wrec - 1) / a.wrec;
             my_pieces++;
-            uint32_t ds = 0;
-            if constexpr (MODE == PASS1_FOF || MODE == PASS2_FOF) ds = a.dataset[f0 + lo];
-            if constexpr (MODE == PASS1_TWO) {
-                // M[key] = min(sat, M[key] + min(len, 255)) in a u8 lane
-                // (CompactMinimizerTable::add, src/lib.rs:4218-4236)
-                const uint32_t delta = len < 255u ? len : 255u;
-                uint32_t *wp = a.mtab + (key >> 2);
-                const uint32_t sh = (key & 3u) * 8u;
-                uint32_t old = __hip_atomic_load(wp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                for (;;) {
-                    const uint32_t b = (old >> sh) & 255u;
-                    if (b >= a.sat) break;
-                    const uint32_t nb = min(b + delta, a.sat);
-                    const uint32_t nw = (old & ~(255u << sh)) | (nb << sh);
-                    const uint32_t prev = atomicCAS(wp, old, nw);
-                    if (prev == old) break;
-                    old = prev;
+            my_nrec += (e - s + (int)a.wrec - 1) / (int)a.wrec;
+        }
+        int tot = 0;
+        const int off = block_exclusive_scan(my_nrec, &tot, s_wsum);
+        if (tid == 0) s_sbase = tot ? atomicAdd(a.stage_ctr, (unsigned long long)tot) : 0ull;
+        __syncthreads();
+        if (a.stage_rec != nullptr) {
+            uint64_t pos = s_sbase + (uint64_t)off;
+            for (int pi = tid; pi < np; pi += SCAN_THREADS) {
+                const int s = (int)s_pst[pi];
+                int e = (pi + 1 < np) ? (int)s_pst[pi + 1] : tl;
+                int lo = 0, hi = nf - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_fs[mid] <= s) lo = mid;
+                    else hi = mid - 1;
                 }
-                atomicAdd(&a.hist[q], nrec);
-            } else if constexpr (MODE == PASS1_FOF) {
-                // per-key dataset mask (one bit per dataset: count at most once)
-                const uint32_t bit = 1u << ds;
-                const uint32_t cur = __hip_atomic_load(&a.mtab[key], __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_AGENT);
-                if (!(cur & bit)) atomicOr(&a.mtab[key], bit);
-                atomicAdd(&a.hist[q], nrec);
-            } else if constexpr (MODE == PASS1_SINGLE) {
-                atomicAdd(&a.hist[q], nrec);
-            } else {
-                bool keep = true;
-                if constexpr (MODE == PASS2_TWO) {
-                    const uint32_t b = (a.mtab[key >> 2] >> ((key & 3u) * 8u)) & 255u;
-                    keep = b > a.x;  // MinimizerCounts::is_above_threshold
-                } else if constexpr (MODE == PASS2_FOF) {
-                    keep = (uint32_t)__popc(a.mtab[key]) > a.x;
-                }
-                if (keep) {
-                    for (int cs = s; cs < e; cs += (int)a.wrec) {
-                        const int ce = min(cs + (int)a.wrec, e);
-                        const uint32_t L = (uint32_t)(ce - cs + k - 1);
-                        uint64_t lo64 = extract32_bases(s_words, (uint64_t)(rb0 + cs));
-                        uint64_t hi64 = 0;
-                        if (L < 32) lo64 &= (1ull << (2 * L)) - 1ull;
-                        if (L > 32) {
-                            hi64 = extract32_bases(s_words, (uint64_t)(rb0 + cs + 32));
-                            hi64 &= (1ull << (2 * (L - 32))) - 1ull;
-                        }
-                        hi64 |= ((uint64_t)L << 48) | ((uint64_t)ds << 56);
-                        const unsigned long long pos = atomicAdd(&a.cursor[q], 1ull);
-                        a.records[pos] = make_ulonglong2(lo64, hi64);
-                        my_records++;
+                const int vend = s_fs[lo + 1] - k + 1;
+                if (vend < e) e = vend;
+                const uint32_t key = spread_hash_u32(s_v[s]);  // visit_direct_hash_runs
+                const uint64_t ds = a.dataset != nullptr ? (uint64_t)a.dataset[f0 + lo] : 0ull;
+                for (int cs = s; cs < e; cs += (int)a.wrec) {
+                    const int ce = min(cs + (int)a.wrec, e);
+                    const uint32_t L = (uint32_t)(ce - cs + k - 1);
+                    uint64_t lo64 = extract32_bases(s_words, (uint64_t)(rb0 + cs));
+                    uint64_t hi64 = 0;
+                    if (L < 32) lo64 &= (1ull << (2 * L)) - 1ull;
+                    if (L > 32) {
+                        hi64 = extract32_bases(s_words, (uint64_t)(rb0 + cs + 32));
+                        hi64 &= (1ull << (2 * (L - 32))) - 1ull;
                     }
+                    hi64 |= ((uint64_t)L << 48) | (ds << 56);
+                    if (pos < a.stage_cap) {
+                        a.stage_rec[pos] = make_ulonglong2(lo64, hi64);
+                        a.stage_key[pos] = key;
+                    } else {
+                        atomicOr(a.err, 2ull);
+                    }
+                    pos++;
                 }
             }
         }
     }
     __shared__ unsigned long long s_red[SCAN_THREADS / 64];
     block_sum_atomic(my_pieces, &a.stats[0], s_red);
-    block_sum_atomic(my_records, &a.stats[1], s_red);
+}
+
+// Pass 2, partitioning (src/lib.rs:3483-3560): every staged record goes to
+// partition spread_hash_u32(key) % P (minimizer_shard, 2927-2929). The
+// partition ids are radix-sorted with the record positions (rocPRIM), then
+// records and keys are gathered into partition order and each partition's
+// [start, end) is marked; no device-scope atomics per record.
+__global__ void k_part_keys(const uint32_t *stage_key, uint64_t n, uint32_t p_int, uint32_t *q, uint32_t *idx) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        q[i] = spread_hash_u32(stage_key[i]) % p_int;
+        idx[i] = (uint32_t)i;
+    }
+}
+
+__global__ void k_gather_records(const ulonglong2 *stage_rec, const uint32_t *stage_key, const uint32_t *perm,
+                                 const uint32_t *qs, uint64_t n, ulonglong2 *rec, uint32_t *rkey,
+                                 unsigned long long *pstart, unsigned long long *pend) {
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t src = perm[j];
+        rec[j] = stage_rec[src];
+        rkey[j] = stage_key[src];
+        const uint32_t q = qs[j];
+        if (j == 0 || qs[j - 1] != q) pstart[q] = j;
+        if (j + 1 == n || qs[j + 1] != q) pend[q] = j + 1;
+    }
 }
 
 // ---------------------------------------------------------------------------
 struct CountArgs {
-    const ulonglong2 *records;
-    const unsigned long long *base;
-    const unsigned long long *cursor;
+    ulonglong2 *records;
+    const uint32_t *rkey;
+    const unsigned long long *base;    // partition start
+    const unsigned long long *cursor;  // partition end
     uint32_t p_int;
     int k;
     uint32_t x;
@@ -474,7 +489,7 @@ __device__ __forceinline__ uint64_t canon_from_packed(uint64_t x, int k) {
 // shuffle binary search), extracting it directly from the 128-bit record
 // (the same canonical k-mers add_packed_superkmer_kmer_counts rolls,
 // src/lib.rs:3589-3614).
-template <int MODE>
+template <int MODE, int FILTER>
 __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
     __shared__ unsigned long long s_keys[CNT_SLOTS];
     __shared__ uint32_t s_vals[CNT_SLOTS];
@@ -484,7 +499,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
     const int tid = threadIdx.x;
     const int lane = tid & 63, wid = tid >> 6;
     const int k = a.k;
-    unsigned long long my_distinct = 0;
+    unsigned long long my_distinct = 0, my_minimizers = 0;
     for (uint32_t q = blockIdx.x; q < a.p_int; q += gridDim.x) {
         for (int i = tid; i < CNT_SLOTS; i += CNT_THREADS) {
             s_keys[i] = EMPTY_KEY;
@@ -492,13 +507,74 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
         }
         __syncthreads();
         const uint64_t rb = a.base[q], re = a.cursor[q];
+        if constexpr (FILTER != 0) {
+            // Pass 1 on the partition's own records (all runs of a key share
+            // its partition): two-pass sums the windows of every record of a
+            // key (= sum of run lengths; the reference's sum of
+            // min(len, 255) saturating at x + 1 gives the same test
+            // "> x" because x < 255, src/lib.rs:3462-3481, 4218-4236, 4625);
+            // fof ORs one bit per dataset (4291-4341). Records of failing
+            // keys are marked. A key that finds no LDS slot is kept: the
+            // filter only drops k-mers whose count cannot exceed x, so
+            // keeping extra records never changes the output.
+            for (uint64_t ri = rb + tid; ri < re; ri += CNT_THREADS) {
+                const unsigned long long key = a.rkey[ri];
+                uint32_t val;
+                if constexpr (FILTER == 1) {
+                    const int n = (int)((a.records[ri].y >> 48) & 255u) - k + 1;
+                    val = n > 0 ? (uint32_t)n : 0u;
+                } else {
+                    val = 1u << (uint32_t)(a.records[ri].y >> 56);
+                }
+                uint32_t slot = spread_hash_u32((uint32_t)key) & (CNT_SLOTS - 1);
+                for (int p = 0; p < CNT_MAXPROBE; p++) {
+                    const unsigned long long cur = __hip_atomic_load(&s_keys[slot], __ATOMIC_RELAXED,
+                                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+                    bool hit = cur == key;
+                    if (cur == EMPTY_KEY) {
+                        const unsigned long long prev = atomicCAS(&s_keys[slot], EMPTY_KEY, key);
+                        hit = prev == EMPTY_KEY || prev == key;
+                    }
+                    if (hit) {
+                        if constexpr (FILTER == 1) atomicAdd(&s_vals[slot], val);
+                        else atomicOr(&s_vals[slot], val);
+                        break;
+                    }
+                    slot = (slot + 1) & (CNT_SLOTS - 1);
+                }
+            }
+            __syncthreads();
+            for (uint64_t ri = rb + tid; ri < re; ri += CNT_THREADS) {
+                const unsigned long long key = a.rkey[ri];
+                uint32_t slot = spread_hash_u32((uint32_t)key) & (CNT_SLOTS - 1);
+                for (int p = 0; p < CNT_MAXPROBE; p++) {
+                    const unsigned long long cur = s_keys[slot];
+                    if (cur == EMPTY_KEY) break;
+                    if (cur == key) {
+                        const uint32_t v = s_vals[slot];
+                        const uint32_t c = (FILTER == 1) ? v : (uint32_t)__popc(v);
+                        if (!(c > a.x)) a.records[ri].y |= (1ull << 63);  // failing minimizer
+                        break;
+                    }
+                    slot = (slot + 1) & (CNT_SLOTS - 1);
+                }
+            }
+            __syncthreads();
+            for (int i = tid; i < CNT_SLOTS; i += CNT_THREADS) {
+                if (s_keys[i] != EMPTY_KEY) my_minimizers++;
+                s_keys[i] = EMPTY_KEY;
+                s_vals[i] = 0;
+            }
+            __threadfence_block();
+            __syncthreads();
+        }
         for (uint64_t wb = rb + (uint64_t)wid * 64; wb < re; wb += CNT_THREADS) {
             const uint64_t ri = wb + lane;
             ulonglong2 r = make_ulonglong2(0, 0);
             int n = 0;
             if (ri < re) {
                 r = a.records[ri];
-                n = (int)((r.y >> 48) & 255u) - k + 1;
+                n = (r.y >> 63) ? 0 : (int)((r.y >> 48) & 255u) - k + 1;
                 if (n < 0) n = 0;
             }
             int incl = n;
@@ -526,7 +602,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
                 const int sh = 2 * off;
                 const uint64_t lo = sh == 0 ? rx : (sh < 64 ? ((rx >> sh) | (rd << (64 - sh))) : (rd >> (sh - 64)));
                 const unsigned long long key = canon_from_packed(lo, k);
-                const uint32_t val = (MODE == 0) ? 1u : (1u << (uint32_t)(ry >> 56));
+                const uint32_t val = (MODE == 0) ? 1u : (1u << (uint32_t)((ry >> 56) & 31u));
                 uint32_t slot = (uint32_t)(spread_hash_u64(key) >> 32) & (CNT_SLOTS - 1);
                 bool done = false;
                 for (int p = 0; p < CNT_MAXPROBE; p++) {
@@ -598,6 +674,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
         __syncthreads();
     }
     block_sum_atomic(my_distinct, &a.ctr[2], s_red);
+    block_sum_atomic(my_minimizers, &a.ctr[5], s_red);
 }
 
 template <int MODE>
@@ -711,9 +788,10 @@ struct McPlan {
     std::vector<uint64_t> chunk_tile;  // tile index where each chunk starts
     uint64_t n_windows = 0;
     // state
-    DevBuf mtab;      // two-pass: 2^32 u8; fof: 2^32 u32
-    size_t mtab_words = 0;
-    DevBuf hist, base, cursor, records, ctr, ovf_keys, ovf_vals, ovf_part;
+    DevBuf base, cursor, records, rkey, ctr, ovf_keys, ovf_vals, ovf_part, stage_rec, stage_key, pq, pqs, pidx,
+        pperm, part_tmp;
+    uint64_t stage_cap = 0, n_staged = 0;
+    size_t part_tmp_bytes = 0;
     DevBuf out_kmer, out_count, out_part;
     DevBuf sort_tmp, key2, pu, idx, idx2, pu2, k2out;
     uint64_t ovf_slots = 0;
@@ -885,11 +963,12 @@ static ScanArgs scan_args(Batch &b, McPlan &p, const mczi_counter_config &cfg) {
     sa.sat = (uint32_t)std::min<uint64_t>(cfg.threshold + 1, 255);
     sa.p_int = p.p_int;
     sa.wrec = (uint32_t)(REC_MAX_BASES - k + 1);
-    sa.mtab = p.mtab.as<uint32_t>();
-    sa.hist = p.hist.as<uint32_t>();
-    sa.cursor = p.cursor.as<unsigned long long>();
-    sa.records = p.records.as<ulonglong2>();
     sa.stats = p.ctr.as<unsigned long long>() + 8;
+    sa.stage_rec = p.stage_rec.as<ulonglong2>();
+    sa.stage_key = p.stage_key.as<uint32_t>();
+    sa.stage_cap = p.stage_cap;
+    sa.stage_ctr = p.ctr.as<unsigned long long>() + 11;
+    sa.err = p.ctr.as<unsigned long long>() + 4;
     return sa;
 }
 
@@ -905,10 +984,7 @@ static void launch_scan(McPlan &p, const ScanArgs &sa, int mode_kernel, int laun
         switch (mode_kernel) {
         case PASS1_TWO: hipLaunchKernelGGL(k_scan<PASS1_TWO>, dim3(g), dim3(SCAN_THREADS), 0, s, ca); break;
         case PASS1_SINGLE: hipLaunchKernelGGL(k_scan<PASS1_SINGLE>, dim3(g), dim3(SCAN_THREADS), 0, s, ca); break;
-        case PASS1_FOF: hipLaunchKernelGGL(k_scan<PASS1_FOF>, dim3(g), dim3(SCAN_THREADS), 0, s, ca); break;
-        case PASS2_TWO: hipLaunchKernelGGL(k_scan<PASS2_TWO>, dim3(g), dim3(SCAN_THREADS), 0, s, ca); break;
-        case PASS2_SINGLE: hipLaunchKernelGGL(k_scan<PASS2_SINGLE>, dim3(g), dim3(SCAN_THREADS), 0, s, ca); break;
-        default: hipLaunchKernelGGL(k_scan<PASS2_FOF>, dim3(g), dim3(SCAN_THREADS), 0, s, ca); break;
+        default: hipLaunchKernelGGL(k_scan<PASS1_FOF>, dim3(g), dim3(SCAN_THREADS), 0, s, ca); break;
         }
     };
     if (launch == MCZI_LAUNCH_PLAIN) {
@@ -921,24 +997,21 @@ static void launch_scan(McPlan &p, const ScanArgs &sa, int mode_kernel, int laun
 // Stage A: clears, pass 1 (minimizer counts / dataset masks + per-partition
 // record counts), bucket bases. No host synchronization inside.
 static void enqueue_stage_a(Batch &b, McPlan &p, const mczi_counter_config &cfg, int mode,
-                            int launch, hipStream_t s, Timer *tm) {
+                            int launch, hipStream_t s, Timer *tm, bool stage = true) {
     ScanArgs sa = scan_args(b, p, cfg);
+    if (!stage) sa.stage_rec = nullptr;
+    if (mode != MCZI_MODE_DATASETS) sa.dataset = nullptr;
     unsigned long long *ctr = p.ctr.as<unsigned long long>();
     fill_u64(ctr, 16, 0, s);
-    fill_u32(sa.hist, p.p_int, 0, s);
-    if (mode != MCZI_MODE_SINGLE_PASS) fill_zero_bytes(p.mtab.as<void>(), p.mtab_words * 4, s);
     if (tm) tm->mark(s);
-    const int k1 = mode == MCZI_MODE_TWO_PASS ? PASS1_TWO
-                   : mode == MCZI_MODE_SINGLE_PASS ? PASS1_SINGLE : PASS1_FOF;
-    launch_scan(p, sa, k1, launch, s);
-    hipLaunchKernelGGL(k_partition_scan, dim3(1), dim3(PSCAN_THREADS), 0, s, sa.hist, p.p_int,
-                       p.base.as<unsigned long long>(), sa.cursor, ctr + 10);
+    launch_scan(p, sa, PASS1_TWO, launch, s);
     if (tm) tm->mark(s);
 }
 
 static CountArgs count_args(McPlan &p, const mczi_counter_config &cfg) {
     CountArgs ca{};
     ca.records = p.records.as<ulonglong2>();
+    ca.rkey = p.rkey.as<uint32_t>();
     ca.base = p.base.as<unsigned long long>();
     ca.cursor = p.cursor.as<unsigned long long>();
     ca.p_int = p.p_int;
@@ -956,31 +1029,49 @@ static CountArgs count_args(McPlan &p, const mczi_counter_config &cfg) {
     return ca;
 }
 
-// Stage B: pass 2 (super-k-mer records into buckets), LDS counting with HBM
-// spill, filter + emit.
+static unsigned bits_for(uint64_t n) {
+    unsigned b = 1;
+    while ((1ull << b) < n) b++;
+    return b;
+}
+
+// Stage B: records into partition order (radix sort of partition ids),
+// per-partition pass-1 filter + LDS counting with HBM spill, emit.
 static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg, int mode,
                             int launch, hipStream_t s, Timer *tm) {
     Device &d = Device::get();
-    ScanArgs sa = scan_args(b, p, cfg);
-    unsigned long long *ctr = p.ctr.as<unsigned long long>();
-    // pass-2 pieces are counted afresh
-    fill_u64(ctr + 8, 1, 0, s);
+    (void)b;
+    (void)launch;
+    const uint64_t n = p.n_staged;
     fill_u64(p.ovf_keys.as<unsigned long long>(), p.ovf_slots, EMPTY_KEY, s);
     fill_u32(p.ovf_vals.as<uint32_t>(), p.ovf_slots, 0, s);
+    fill_u64(p.base.as<unsigned long long>(), p.p_int, 0, s);
+    fill_u64(p.cursor.as<unsigned long long>(), p.p_int, 0, s);
     if (tm) tm->mark(s);
-    const int k2 = mode == MCZI_MODE_TWO_PASS ? PASS2_TWO
-                   : mode == MCZI_MODE_SINGLE_PASS ? PASS2_SINGLE : PASS2_FOF;
-    launch_scan(p, sa, k2, launch, s);
+    if (n) {
+        const unsigned g = grid_for(n, 256, 16384);
+        hipLaunchKernelGGL(k_part_keys, dim3(g), dim3(256), 0, s, p.stage_key.as<uint32_t>(), n, p.p_int,
+                           p.pq.as<uint32_t>(), p.pidx.as<uint32_t>());
+        size_t tmp = p.part_tmp_bytes;
+        MCZI_HIP(rocprim::radix_sort_pairs(p.part_tmp.as<void>(), tmp, p.pq.as<uint32_t>(), p.pqs.as<uint32_t>(),
+                                           p.pidx.as<uint32_t>(), p.pperm.as<uint32_t>(), (size_t)n, 0,
+                                           bits_for(p.p_int), s));
+        hipLaunchKernelGGL(k_gather_records, dim3(g), dim3(256), 0, s, p.stage_rec.as<ulonglong2>(),
+                           p.stage_key.as<uint32_t>(), p.pperm.as<uint32_t>(), p.pqs.as<uint32_t>(), n,
+                           p.records.as<ulonglong2>(), p.rkey.as<uint32_t>(), p.base.as<unsigned long long>(),
+                           p.cursor.as<unsigned long long>());
+    }
     if (tm) tm->mark(s);
     CountArgs ca = count_args(p, cfg);
     const unsigned cgrid = (unsigned)std::min<uint64_t>(p.p_int, (uint64_t)d.cu_count() * 8);
     const unsigned ogrid = grid_for(p.ovf_slots, 256, 8192);
     if (mode == MCZI_MODE_DATASETS) {
-        hipLaunchKernelGGL(k_count<1>, dim3(cgrid), dim3(CNT_THREADS), 0, s, ca);
+        hipLaunchKernelGGL((k_count<1, 2>), dim3(cgrid), dim3(CNT_THREADS), 0, s, ca);
         if (tm) tm->mark(s);
         hipLaunchKernelGGL(k_overflow_emit<1>, dim3(ogrid), dim3(256), 0, s, ca);
     } else {
-        hipLaunchKernelGGL(k_count<0>, dim3(cgrid), dim3(CNT_THREADS), 0, s, ca);
+        if (mode == MCZI_MODE_TWO_PASS) hipLaunchKernelGGL((k_count<0, 1>), dim3(cgrid), dim3(CNT_THREADS), 0, s, ca);
+        else hipLaunchKernelGGL((k_count<0, 0>), dim3(cgrid), dim3(CNT_THREADS), 0, s, ca);
         if (tm) tm->mark(s);
         hipLaunchKernelGGL(k_overflow_emit<0>, dim3(ogrid), dim3(256), 0, s, ca);
     }
@@ -1065,12 +1156,6 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
     double target = p.target_override > 0 ? p.target_override : tuned_target();
     for (int attempt = 0;; attempt++) {
         p.p_int = choose_partitions(p.n_windows, cfg.partition_count, target);
-        const size_t mtab_words = (mode == MCZI_MODE_DATASETS) ? (1ull << 32) : (1ull << 30);
-        if (mode != MCZI_MODE_SINGLE_PASS) {
-            p.mtab.ensure(mtab_words * 4);
-            p.mtab_words = mtab_words;
-        }
-        p.hist.ensure((size_t)p.p_int * 4 + 64);
         p.base.ensure(((size_t)p.p_int + 1) * 8);
         p.cursor.ensure(((size_t)p.p_int + 1) * 8);
         p.ctr.ensure(16 * 8);
@@ -1084,22 +1169,39 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
         p.out_count.ensure(p.out_cap * 4);
         p.out_part.ensure(p.out_cap * 4);
 
-        // Record capacity = pass-1 total, measured once per (batch, config).
-        uint64_t cfg_sig = mix_sig(mix_sig(mix_sig(k, cfg.minimizer), cfg.threshold), mode);
-        cfg_sig = mix_sig(mix_sig(cfg_sig, p.p_int), launch);
+        // Staged record count: measured once per (batch, k, m) by a count-only
+        // pass 1 (the records of a batch do not depend on x, mode or P).
+        const uint64_t cfg_sig = mix_sig(mix_sig(k, cfg.minimizer), b.n_bases);
         if (p.rec_sig != cfg_sig) {
             if (p.graph) {
                 MCZI_HIP(hipGraphExecDestroy(p.graph));
                 p.graph = nullptr;
             }
-            enqueue_stage_a(b, p, cfg, mode, launch, s, nullptr);
+            enqueue_stage_a(b, p, cfg, mode, launch, s, nullptr, false);
             unsigned long long tot = 0;
-            MCZI_HIP(hipMemcpyAsync(&tot, p.ctr.as<unsigned long long>() + 10, 8,
-                                    hipMemcpyDeviceToHost, s));
+            MCZI_HIP(hipMemcpyAsync(&tot, p.ctr.as<unsigned long long>() + 11, 8, hipMemcpyDeviceToHost, s));
             MCZI_HIP(hipStreamSynchronize(s));
-            p.records.ensure((tot + 16) * 16);
+            require(tot < (1ull << 32), MCZI_E_INVALID, "too many super-k-mer records for one batch (limit 2^32)");
+            p.n_staged = tot;
+            p.stage_cap = tot + 16;
             p.rec_cap = tot + 16;
+            p.records.ensure(p.rec_cap * 16);
+            p.rkey.ensure(p.rec_cap * 4);
+            p.stage_rec.ensure(p.stage_cap * 16);
+            p.stage_key.ensure(p.stage_cap * 4);
+            p.pq.ensure(p.stage_cap * 4);
+            p.pqs.ensure(p.stage_cap * 4);
+            p.pidx.ensure(p.stage_cap * 4);
+            p.pperm.ensure(p.stage_cap * 4);
             p.rec_sig = cfg_sig;
+        }
+        {
+            size_t tmp = 0;
+            MCZI_HIP(rocprim::radix_sort_pairs(nullptr, tmp, p.pq.as<uint32_t>(), p.pqs.as<uint32_t>(),
+                                               p.pidx.as<uint32_t>(), p.pperm.as<uint32_t>(),
+                                               (size_t)std::max<uint64_t>(p.n_staged, 1), 0, bits_for(p.p_int), s));
+            p.part_tmp.ensure(tmp + 256);
+            p.part_tmp_bytes = tmp;
         }
 
         const bool want_times = stats != nullptr && !use_graph;
@@ -1108,11 +1210,13 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
         tm.mark(s);
         if (use_graph) {
             uint64_t sig = cfg_sig;
-            for (const DevBuf *db : {&b.words, &b.off, &b.dataset, &p.mtab, &p.hist, &p.base, &p.cursor,
-                                     &p.records, &p.ctr, &p.ovf_keys, &p.ovf_vals, &p.ovf_part,
-                                     &p.out_kmer, &p.out_count, &p.out_part, &p.tile_start})
+            sig = mix_sig(mix_sig(mix_sig(sig, cfg.threshold), (uint64_t)mode), p.p_int);
+            for (const DevBuf *db : {&b.words, &b.off, &b.dataset, &p.base, &p.cursor, &p.records, &p.rkey,
+                                     &p.ctr, &p.ovf_keys, &p.ovf_vals, &p.ovf_part, &p.out_kmer, &p.out_count,
+                                     &p.out_part, &p.tile_start, &p.stage_rec, &p.stage_key, &p.pq, &p.pqs,
+                                     &p.pidx, &p.pperm, &p.part_tmp})
                 sig = mix_sig(sig, (uint64_t)(uintptr_t)db->as<void>());
-            sig = mix_sig(sig, p.ovf_slots);
+            sig = mix_sig(mix_sig(sig, p.ovf_slots), p.stage_cap);
             if (!p.graph || p.graph_sig != sig) {
                 if (p.graph) {
                     MCZI_HIP(hipGraphExecDestroy(p.graph));
@@ -1138,6 +1242,8 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
         MCZI_HIP(hipMemcpyAsync(h, p.ctr.as<void>(), sizeof h, hipMemcpyDeviceToHost, s));
         MCZI_HIP(hipStreamSynchronize(s));
         MCZI_HIP(hipGetLastError());
+        require((h[4] & 2ull) == 0 && h[11] == p.n_staged, MCZI_E_DEVICE,
+                "super-k-mer staging differs from its measured size");
         if (h[4] != 0 || h[1] > p.out_cap) {
             require(attempt < 4, MCZI_E_NOMEM, "k-mer spill table overflowed repeatedly");
             target /= 4;
@@ -1166,15 +1272,17 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
         st.n_bases = b.n_bases;
         st.n_windows = p.n_windows;
         st.n_pieces = h[8];
-        st.n_records = h[9];
+        st.n_records = p.n_staged;
+        st.minimizers_kept = h[5];
         st.n_distinct = h[2] + h[3];
         st.n_overflow = h[3];
         st.n_out = n_out;
-        st.partition_bytes = h[9] * 16;
+        st.partition_bytes = p.n_staged * 16;
         st.internal_partitions = p.p_int;
         st.lds_slots = CNT_SLOTS;
         if (want_times && t_dev_end >= 7) {
-            // marks: 0 start, 1 clears done, 2 pass1 done, 3 spill clear, 4 pass2, 5 count, 6 emit
+            // marks: 0 start, 1 clears, 2 pass 1 (scan + staging), 3 clears,
+            // 4 partition sort + gather, 5 count, 6 spill emit
             st.ms_scan1 = tm.ms(1, 2);
             st.ms_scan2 = tm.ms(3, 4);
             st.ms_count = tm.ms(4, 6);
@@ -1187,6 +1295,28 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
         if (stats) *stats = st;
         return;
     }
+}
+
+// Autotuner: sweep the expected k-mer instances per internal partition (the
+// LDS table fill) and keep the fastest; results do not depend on it.
+double batch_autotune(Batch &b, const mczi_counter_config &cfg, int mode, int launch, int order) {
+    if (!b.plan) b.plan.reset(new McPlan());
+    McPlan &p = *b.plan;
+    const double cands[] = {0.5, 1.0, 2.0, 4.0, 8.0};
+    double best = 0, best_ms = 1e300;
+    for (double c : cands) {
+        p.target_override = c * CNT_SLOTS;
+        mczi_run_stats st{};
+        batch_count(b, cfg, mode, launch, order, &st);  // calibrates / captures
+        batch_count(b, cfg, mode, launch, order, &st);
+        const double ms = st.ms_total + st.ms_sort;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = p.target_override;
+        }
+    }
+    p.target_override = best;
+    return best;
 }
 
 void batch_download(Batch &b, std::vector<mczi_counted_kmer> &out) {

@@ -23,9 +23,41 @@ struct ZiIndex {
 void zi_build_index(ZiIndex &idx, HostReads &&reads, uint32_t k, uint32_t m, hipStream_t s);
 
 // One batch of query requests (fragments tagged with their request slot).
+// Sorted, unique absent k-mers of one request, in malloc'd arrays (ownership
+// passes to the caller's result).
+struct ZiKeys {
+    uint64_t *lo = nullptr, *hi = nullptr;
+    size_t n = 0;
+    ZiKeys() = default;
+    ZiKeys(const ZiKeys &) = delete;
+    ZiKeys &operator=(const ZiKeys &) = delete;
+    ZiKeys(ZiKeys &&o) noexcept : lo(o.lo), hi(o.hi), n(o.n) { o.lo = o.hi = nullptr; o.n = 0; }
+    ZiKeys &operator=(ZiKeys &&o) noexcept {
+        if (this != &o) {
+            free(lo);
+            free(hi);
+            lo = o.lo;
+            hi = o.hi;
+            n = o.n;
+            o.lo = o.hi = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~ZiKeys() {
+        free(lo);
+        free(hi);
+    }
+    std::vector<mczi_counted_kmer> counted() const {
+        std::vector<mczi_counted_kmer> v(n);
+        for (size_t i = 0; i < n; i++) v[i] = mczi_counted_kmer{lo[i], hi ? hi[i] : 0, 1};
+        return v;
+    }
+};
+
 struct ZiQueryResult {
-    std::vector<std::vector<mczi_counted_kmer>> absent;  // per request, sorted, unique
-    std::vector<uint64_t> windows;                       // per request
+    std::vector<ZiKeys> absent;     // per request
+    std::vector<uint64_t> windows;  // per request
 };
 
 // Device memory a batch of `bases` query bases in `frags` fragments needs.

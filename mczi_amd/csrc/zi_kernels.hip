@@ -563,29 +563,25 @@ void zi_collect(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, Zi
     MCZI_HIP(hipMemcpyAsync(&lk, w.ctr.as<void>(), 8, hipMemcpyDeviceToHost, s));
     MCZI_HIP(hipStreamSynchronize(s));
     w.lookups = lk;
-    out.absent.assign(n_req, {});
+    out.absent.clear();
+    out.absent.resize(n_req);
     out.windows.assign(n_req, 0);
-    std::vector<std::vector<unsigned long long>> lo(n_req), hi(n_req);
     for (uint32_t r = 0; r < n_req; r++) {
         out.windows[r] = w.h_base[r + 1] - w.h_base[r];
         const uint64_t n = w.h_count[r];
-        lo[r].resize(n);
+        ZiKeys &kk = out.absent[r];
+        kk.n = n;
+        kk.lo = (uint64_t *)malloc(std::max<uint64_t>(n, 1) * 8);
+        if (idx.wide) kk.hi = (uint64_t *)malloc(std::max<uint64_t>(n, 1) * 8);
+        require(kk.lo && (!idx.wide || kk.hi), MCZI_E_NOMEM, "out of host memory for ZI results");
         if (n)
-            MCZI_HIP(hipMemcpyAsync(lo[r].data(), w.ulo.as<unsigned long long>() + h_start[r], n * 8,
+            MCZI_HIP(hipMemcpyAsync(kk.lo, w.ulo.as<unsigned long long>() + h_start[r], n * 8,
                                     hipMemcpyDeviceToHost, s));
-        if (idx.wide) {
-            hi[r].resize(n);
-            if (n)
-                MCZI_HIP(hipMemcpyAsync(hi[r].data(), w.ohi.as<unsigned long long>() + h_start[r], n * 8,
-                                        hipMemcpyDeviceToHost, s));
-        }
+        if (idx.wide && n)
+            MCZI_HIP(hipMemcpyAsync(kk.hi, w.ohi.as<unsigned long long>() + h_start[r], n * 8,
+                                    hipMemcpyDeviceToHost, s));
     }
     MCZI_HIP(hipStreamSynchronize(s));
-    for (uint32_t r = 0; r < n_req; r++) {
-        auto &v = out.absent[r];
-        v.resize(lo[r].size());
-        for (size_t i = 0; i < v.size(); i++) v[i] = mczi_counted_kmer{lo[r][i], idx.wide ? hi[r][i] : 0, 1};
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -599,7 +595,7 @@ struct ZiRequest {
     bool done = false;
     int err = 0;
     std::string err_msg;
-    std::vector<mczi_counted_kmer> absent;
+    ZiKeys absent;
     std::string fasta;
     uint64_t windows = 0;
 };
@@ -776,7 +772,7 @@ void mczi_zi_server::done_loop() {
         for (size_t i = 0; i < batch.size() && !err; i++) {
             batch[i]->absent = std::move(res.absent[i]);
             batch[i]->windows = res.windows[i];
-            if (batch[i]->make_fasta) batch[i]->fasta = simplitig_fasta_text(batch[i]->absent, idx.k);
+            if (batch[i]->make_fasta) batch[i]->fasta = simplitig_fasta_text(batch[i]->absent.counted(), idx.k);
         }
         lk.lock();
         busy_ms += t1 - t0;
@@ -844,7 +840,7 @@ std::vector<mczi_counted_kmer> zi_run(const char *index_path, const char *const 
     zi_enqueue(*w, idx, std::move(q), std::move(fr), 1, d.stream());
     ZiQueryResult res;
     zi_collect(*w, idx, 1, d.stream(), res);
-    return std::move(res.absent[0]);
+    return res.absent[0].counted();
 }
 }  // namespace
 
@@ -1016,10 +1012,11 @@ int mczi_zi_server_poll(mczi_zi_server *s, uint64_t request_id, int wait, mczi_z
         require(res != nullptr, MCZI_E_NOMEM, "out of host memory");
         res->request_id = own->id;
         res->n_windows = own->windows;
-        res->n_absent = own->absent.size();
-        res->absent = (mczi_counted_kmer *)malloc(std::max<size_t>(own->absent.size(), 1) * sizeof(mczi_counted_kmer));
-        if (!own->absent.empty())
-            memcpy(res->absent, own->absent.data(), own->absent.size() * sizeof(mczi_counted_kmer));
+        res->n_absent = own->absent.n;
+        res->absent_lo = own->absent.lo;
+        res->absent_hi = own->absent.hi;
+        own->absent.lo = own->absent.hi = nullptr;
+        own->absent.n = 0;
         res->fasta = (char *)malloc(own->fasta.size() + 1);
         memcpy(res->fasta, own->fasta.data(), own->fasta.size());
         res->fasta[own->fasta.size()] = 0;
@@ -1038,7 +1035,8 @@ int mczi_zi_server_poll(mczi_zi_server *s, uint64_t request_id, int wait, mczi_z
 
 void mczi_zi_result_free(mczi_zi_result *r) {
     if (!r) return;
-    free(r->absent);
+    free(r->absent_lo);
+    free(r->absent_hi);
     free(r->fasta);
     free(r);
 }

@@ -204,3 +204,86 @@ def test_batch_plain_equals_batched_and_spill(monkeypatch):
     st = b2.count(cfg, M.MODE_TWO_PASS, M.LAUNCH_BATCHED)
     assert st.n_overflow > 0
     assert same_counted(b2.download(), want)
+
+
+def _concat(a, b):
+    """Two packed read sets in one segmented buffer (word-aligned; the gap
+    between them is a short fragment that has no k-mer)."""
+    na = int(a.frag_off[-1])
+    wa = (na + 15) // 16
+    pad = wa * 16
+    words = np.concatenate([a.words[:wa], b.words])
+    off = np.concatenate([a.frag_off, [pad] if pad != na else [], b.frag_off[1:] + pad]).astype(np.uint64)
+    return words, off
+
+
+def test_long_and_short_reads_one_launch():
+    long = M.SynthReads(M.SynthSpec(genome_len=300_000, genome_seed=3, read_seed=103, long_bases=3_000_000,
+                                    long_median=8000, long_sigma=0.8, long_min=500, long_max=100_000,
+                                    sub_rate=0.04, ins_rate=0.02, del_rate=0.02))
+    short = M.SynthReads(M.SynthSpec(genome_len=300_000, genome_seed=3, read_seed=203, n_reads=20_000,
+                                     read_len=150, sub_rate=0.005))
+    words, off = _concat(long, short)
+    b = M.Batch.from_packed(words, off)
+    gap = int(off[len(long.frag_off)]) - int(long.frag_off[-1]) if len(off) > len(long.frag_off) + len(
+        short.frag_off) - 1 else 0
+    codes = np.concatenate([long.codes(), np.zeros(gap, dtype=np.uint8), short.codes()])
+    cfg = M.CounterConfig(31, 21, 3, 1024)
+    want = O.count_bruteforce(codes, off, 31, 3)
+    t = b.autotune(cfg)
+    assert t > 0
+    b.count(cfg)
+    assert same_counted(b.download(), want)
+    b.count(cfg, M.MODE_TWO_PASS, M.LAUNCH_PLAIN)
+    assert same_counted(b.download(), want)
+
+
+def test_fof_32_datasets_batch():
+    r = M.SynthReads(M.SynthSpec(genome_len=60_000, genome_seed=4, read_seed=104, n_reads=1200, read_len=150,
+                                 sub_rate=0.005, n_datasets=32, snp_rate=0.001))
+    b = M.Batch.from_packed(r.words, r.frag_off, r.dataset)
+    assert b.info()[2] == 32
+    codes = r.codes()
+    for x in (5, 20, 31):
+        st = b.count(M.CounterConfig(31, 21, x, 1024), M.MODE_DATASETS)
+        got = b.download()
+        want = O.count_datasets_bruteforce(codes, r.frag_off, r.dataset, 32, 31, x)
+        assert same_counted(got, want), x
+        assert st.n_out == len(want)
+
+
+def test_c1_full_config_kff_bytes(tmp_path):
+    """MC abundance mode, plain method, small: 1 Mbp genome (seed 1), 200k
+    150 bp reads, 0.5% substitutions, k=31, m=21, x=1, 64 partitions, KFF."""
+    r = M.SynthReads(M.SynthSpec(genome_len=1_000_000, genome_seed=1, read_seed=101, n_reads=200_000,
+                                 read_len=150, sub_rate=0.005))
+    fa = tmp_path / "c1.fa"
+    r.write_fasta(fa)
+    out = tmp_path / "c1.kff"
+    M.count_inputs_to_kff_path([fa], M.CounterConfig(31, 21, 1, 64), out)
+    want, nrec = O.kff_u8_kmc(r.codes(), r.frag_off, 31, 21, 1, 64)
+    assert out.read_bytes() == want
+    assert nrec == 1053631
+
+
+def test_c2_full_size_properties():
+    """Short reads at scale (100 Mbp + repeats, 20M reads): batched and plain
+    launches agree exactly; output sorted, unique, all counts > x; the total
+    count of emitted k-mers cannot exceed the windows."""
+    r = M.SynthReads(M.SynthSpec(genome_len=100_000_000, genome_seed=2, read_seed=102, n_reads=20_000_000,
+                                 read_len=150, sub_rate=0.01, repeat_fraction=0.02, repeat_len=5000,
+                                 repeat_copies=10))
+    b = M.Batch.from_packed(r.words, r.frag_off)
+    cfg = M.CounterConfig(31, 21, 5, 1024)
+    st = b.count(cfg)
+    a = b.download()
+    b.count(cfg, M.MODE_TWO_PASS, M.LAUNCH_PLAIN)
+    c = b.download()
+    assert same_counted(a, c)
+    assert len(a) == st.n_out > 90_000_000
+    assert np.all(a["hi"] == 0) and np.all(np.diff(a["lo"].astype(np.uint64)) > 0)
+    assert np.all(a["count"] > 5)
+    assert int(a["count"].sum()) <= st.n_windows
+    # single pass (KMC-style) gives the same set and counts
+    b.count(cfg, M.MODE_SINGLE_PASS)
+    assert same_counted(b.download(), a)

@@ -112,7 +112,10 @@ def _poll(h, rid):
     r = p.contents
     arr = np.zeros(r.n_absent, dtype=M.COUNTED)
     if r.n_absent:
-        C.memmove(arr.ctypes.data, r.absent, r.n_absent * M.COUNTED.itemsize)
+        arr["lo"] = np.ctypeslib.as_array(r.absent_lo, shape=(r.n_absent,))
+        if r.absent_hi:
+            arr["hi"] = np.ctypeslib.as_array(r.absent_hi, shape=(r.n_absent,))
+        arr["count"] = 1
     fasta = C.string_at(r.fasta, r.fasta_len).decode()
     M._lib.mczi_zi_result_free(p)
     return arr, fasta
