@@ -30,18 +30,14 @@ constexpr int REC_MAX_BASES = 56;             // bases per 128-bit super-k-mer r
 
 // Count kernel geometry
 constexpr int CNT_THREADS = 256;
-constexpr int CNT_SLOTS = 4096;               // LDS table slots (u64 key + u32 value)
+constexpr int CNT_LOG2_SLOTS = 12;
+constexpr int CNT_SLOTS = 1 << CNT_LOG2_SLOTS;  // LDS table slots (u64 key + u32 value)
 constexpr int CNT_MAXPROBE = 32;
 constexpr int OVF_MAXPROBE = 1 << 16;
 
 enum ScanMode {
-    PASS1_TWO = 0,
-    PASS1_SINGLE = 1,
-    PASS1_FOF = 2,
-    PASS2_TWO = 3,
-    PASS2_SINGLE = 4,
-    PASS2_FOF = 5,
-    WINDOWS = 6
+    STAGE = 0,    // super-k-mer records + partition ids (or only their count)
+    WINDOWS = 1   // selected hash of every window
 };
 
 struct ScanArgs {
@@ -60,9 +56,9 @@ struct ScanArgs {
     uint32_t wrec;
     uint32_t *win_out;
     unsigned long long *stats;  // [0] pieces [1] records
-    // pass-1 staging of super-k-mer records (null: count only)
+    // staging of super-k-mer records and their internal partition (null: count only)
     ulonglong2 *stage_rec;
-    uint32_t *stage_key;
+    uint32_t *stage_q;
     uint64_t stage_cap;
     unsigned long long *stage_ctr;
     unsigned long long *err;
@@ -310,7 +306,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
         // Pieces -> super-k-mer records of at most wrec windows (bases
         // [start, end + k - 1), src/lib.rs:3483-3560), staged contiguously per
         // tile (one reservation per tile; with no staging buffer only the
-        // count is taken) together with the minimizer key of their run.
+        // count is taken) together with their partition: key =
+        // spread_hash_u32(minimizer hash) (visit_direct_hash_runs), partition
+        // = spread_hash_u32(key) % P (minimizer_shard, src/lib.rs:2927-2929).
+        // Bases are stored in MC's 2-bit code (A0 C1 G2 T3, src/lib.rs:
+        // 3192-3197), base j of the record at bits 2j.
         int my_nrec = 0;
         for (int pi = tid; pi < np; pi += SCAN_THREADS) {
             const int s = (int)s_pst[pi];
@@ -343,7 +343,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
                 }
                 const int vend = s_fs[lo + 1] - k + 1;
                 if (vend < e) e = vend;
-                const uint32_t key = spread_hash_u32(s_v[s]);  // visit_direct_hash_runs
+                const uint32_t q = spread_hash_u32(spread_hash_u32(s_v[s])) % a.p_int;
                 const uint64_t ds = a.dataset != nullptr ? (uint64_t)a.dataset[f0 + lo] : 0ull;
                 for (int cs = s; cs < e; cs += (int)a.wrec) {
                     const int ce = min(cs + (int)a.wrec, e);
@@ -355,10 +355,12 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
                         hi64 = extract32_bases(s_words, (uint64_t)(rb0 + cs + 32));
                         hi64 &= (1ull << (2 * (L - 32))) - 1ull;
                     }
+                    lo64 ^= (lo64 >> 1) & 0x5555555555555555ull;  // packed code -> MC code
+                    hi64 ^= (hi64 >> 1) & 0x5555555555555555ull;
                     hi64 |= ((uint64_t)L << 48) | (ds << 56);
                     if (pos < a.stage_cap) {
                         a.stage_rec[pos] = make_ulonglong2(lo64, hi64);
-                        a.stage_key[pos] = key;
+                        a.stage_q[pos] = q;
                     } else {
                         atomicOr(a.err, 2ull);
                     }
@@ -371,25 +373,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
     block_sum_atomic(my_pieces, &a.stats[0], s_red);
 }
 
-// Pass 2, partitioning (src/lib.rs:3483-3560): every staged record goes to
-// partition spread_hash_u32(key) % P (minimizer_shard, 2927-2929). The
-// partition ids are radix-sorted with the record positions (rocPRIM), then
-// records and keys are gathered into partition order and each partition's
-// [start, end) is marked; no device-scope atomics per record.
-__global__ void k_part_keys(const uint32_t *stage_key, uint64_t n, uint32_t p_int, uint32_t *q, uint32_t *idx) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        q[i] = spread_hash_u32(stage_key[i]) % p_int;
-        idx[i] = (uint32_t)i;
-    }
-}
-
-__global__ void k_gather_records(const ulonglong2 *stage_rec, const uint32_t *stage_key, const uint32_t *perm,
-                                 const uint32_t *qs, uint64_t n, ulonglong2 *rec, uint32_t *rkey,
-                                 unsigned long long *pstart, unsigned long long *pend) {
+// Partitioning (src/lib.rs:3483-3560 writes each super-k-mer to its
+// partition's bucket): the staged (partition, record) pairs are radix-sorted
+// by partition (rocPRIM onesweep, records moved as 16-byte values), then each
+// partition's [start, end) is marked; no device-scope atomics per record.
+__global__ void k_part_bounds(const uint32_t *qs, uint64_t n, unsigned long long *pstart,
+                              unsigned long long *pend) {
     for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t src = perm[j];
-        rec[j] = stage_rec[src];
-        rkey[j] = stage_key[src];
         const uint32_t q = qs[j];
         if (j == 0 || qs[j - 1] != q) pstart[q] = j;
         if (j + 1 == n || qs[j + 1] != q) pend[q] = j + 1;
@@ -398,8 +388,7 @@ __global__ void k_gather_records(const ulonglong2 *stage_rec, const uint32_t *st
 
 // ---------------------------------------------------------------------------
 struct CountArgs {
-    ulonglong2 *records;
-    const uint32_t *rkey;
+    const ulonglong2 *records;
     const unsigned long long *base;    // partition start
     const unsigned long long *cursor;  // partition end
     uint32_t p_int;
@@ -465,17 +454,25 @@ __device__ __forceinline__ unsigned long long wave_append(bool pred, unsigned lo
     return base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
-// Canonical MC encoding of k <= 32 bases held packed-seq style in x
-// (base j at bits 2j): forward = lanes reversed (first base most
-// significant), reverse complement = complemented lanes in stored order.
-__device__ __forceinline__ uint64_t canon_from_packed(uint64_t x, int k) {
-    const uint64_t mask = k >= 32 ? ~0ull : ((1ull << (2 * k)) - 1ull);
-    x = (x ^ ((x >> 1) & 0x5555555555555555ull)) & mask;  // packed-seq code -> MC bits
+// Canonical MC encoding (src/lib.rs:3589-3614) of the k <= 32 MC-coded bases
+// in x (base j at bits 2j, x already masked to 2k bits): forward = 2-bit
+// groups reversed (first base most significant), reverse complement =
+// complemented groups in stored order.
+__device__ __forceinline__ uint64_t canon_from_mc(uint64_t x, uint64_t mask, int rshift) {
     uint64_t r = __builtin_bitreverse64(x);
     r = ((r >> 1) & 0x5555555555555555ull) | ((r & 0x5555555555555555ull) << 1);
-    const uint64_t fwd = r >> (64 - 2 * k);
+    const uint64_t fwd = r >> rshift;
     const uint64_t rc = (~x) & mask;
     return fwd < rc ? fwd : rc;
+}
+
+// LDS table slot of a k-mer: multiplicative mix of both 32-bit halves (the
+// spill table uses spread_hash_u64, an independent function).
+__device__ __forceinline__ uint32_t lds_slot(uint64_t key) {
+    uint32_t h = ((uint32_t)key * 0x9E3779B1u) ^ ((uint32_t)(key >> 32) * 0x85EBCA77u);
+    h ^= h >> 16;
+    h *= 0x2C1B3C6Du;
+    return h >> (32 - CNT_LOG2_SLOTS);
 }
 
 // Per partition: all k-mers of its super-k-mer records go into an LDS
@@ -484,12 +481,17 @@ __device__ __forceinline__ uint64_t canon_from_packed(uint64_t x, int k) {
 // each k-mer lives in exactly one of the two. Then count > x (MODE 0: counts,
 // src/lib.rs:2417 / 2502) or popcount(mask) > x (MODE 1: dataset presence,
 // src/lib.rs:2603) is emitted with one output reservation per partition.
+// The reference's pass-1 minimizer filter (two-pass 3462-3523, fof
+// 4291-4341) is not run: every occurrence of a k-mer K has the same
+// minimizer (the scheme is canonical), so that minimizer's window sum
+// (dataset set) is >= c(K) (covers K's datasets), and the filter can only
+// drop k-mers that fail "> x" anyway; the emitted set and counts are equal.
 // Work is balanced per wave: 64 records are loaded, their k-mers numbered by
 // a wave prefix sum, and lane j takes k-mer j (owner record found by a
 // shuffle binary search), extracting it directly from the 128-bit record
 // (the same canonical k-mers add_packed_superkmer_kmer_counts rolls,
 // src/lib.rs:3589-3614).
-template <int MODE, int FILTER>
+template <int MODE>
 __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
     __shared__ unsigned long long s_keys[CNT_SLOTS];
     __shared__ uint32_t s_vals[CNT_SLOTS];
@@ -499,7 +501,9 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
     const int tid = threadIdx.x;
     const int lane = tid & 63, wid = tid >> 6;
     const int k = a.k;
-    unsigned long long my_distinct = 0, my_minimizers = 0;
+    const uint64_t kmask = k >= 32 ? ~0ull : ((1ull << (2 * k)) - 1ull);
+    const int rshift = 64 - 2 * k;
+    unsigned long long my_distinct = 0;
     for (uint32_t q = blockIdx.x; q < a.p_int; q += gridDim.x) {
         for (int i = tid; i < CNT_SLOTS; i += CNT_THREADS) {
             s_keys[i] = EMPTY_KEY;
@@ -507,74 +511,13 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
         }
         __syncthreads();
         const uint64_t rb = a.base[q], re = a.cursor[q];
-        if constexpr (FILTER != 0) {
-            // Pass 1 on the partition's own records (all runs of a key share
-            // its partition): two-pass sums the windows of every record of a
-            // key (= sum of run lengths; the reference's sum of
-            // min(len, 255) saturating at x + 1 gives the same test
-            // "> x" because x < 255, src/lib.rs:3462-3481, 4218-4236, 4625);
-            // fof ORs one bit per dataset (4291-4341). Records of failing
-            // keys are marked. A key that finds no LDS slot is kept: the
-            // filter only drops k-mers whose count cannot exceed x, so
-            // keeping extra records never changes the output.
-            for (uint64_t ri = rb + tid; ri < re; ri += CNT_THREADS) {
-                const unsigned long long key = a.rkey[ri];
-                uint32_t val;
-                if constexpr (FILTER == 1) {
-                    const int n = (int)((a.records[ri].y >> 48) & 255u) - k + 1;
-                    val = n > 0 ? (uint32_t)n : 0u;
-                } else {
-                    val = 1u << (uint32_t)(a.records[ri].y >> 56);
-                }
-                uint32_t slot = spread_hash_u32((uint32_t)key) & (CNT_SLOTS - 1);
-                for (int p = 0; p < CNT_MAXPROBE; p++) {
-                    const unsigned long long cur = __hip_atomic_load(&s_keys[slot], __ATOMIC_RELAXED,
-                                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
-                    bool hit = cur == key;
-                    if (cur == EMPTY_KEY) {
-                        const unsigned long long prev = atomicCAS(&s_keys[slot], EMPTY_KEY, key);
-                        hit = prev == EMPTY_KEY || prev == key;
-                    }
-                    if (hit) {
-                        if constexpr (FILTER == 1) atomicAdd(&s_vals[slot], val);
-                        else atomicOr(&s_vals[slot], val);
-                        break;
-                    }
-                    slot = (slot + 1) & (CNT_SLOTS - 1);
-                }
-            }
-            __syncthreads();
-            for (uint64_t ri = rb + tid; ri < re; ri += CNT_THREADS) {
-                const unsigned long long key = a.rkey[ri];
-                uint32_t slot = spread_hash_u32((uint32_t)key) & (CNT_SLOTS - 1);
-                for (int p = 0; p < CNT_MAXPROBE; p++) {
-                    const unsigned long long cur = s_keys[slot];
-                    if (cur == EMPTY_KEY) break;
-                    if (cur == key) {
-                        const uint32_t v = s_vals[slot];
-                        const uint32_t c = (FILTER == 1) ? v : (uint32_t)__popc(v);
-                        if (!(c > a.x)) a.records[ri].y |= (1ull << 63);  // failing minimizer
-                        break;
-                    }
-                    slot = (slot + 1) & (CNT_SLOTS - 1);
-                }
-            }
-            __syncthreads();
-            for (int i = tid; i < CNT_SLOTS; i += CNT_THREADS) {
-                if (s_keys[i] != EMPTY_KEY) my_minimizers++;
-                s_keys[i] = EMPTY_KEY;
-                s_vals[i] = 0;
-            }
-            __threadfence_block();
-            __syncthreads();
-        }
         for (uint64_t wb = rb + (uint64_t)wid * 64; wb < re; wb += CNT_THREADS) {
             const uint64_t ri = wb + lane;
             ulonglong2 r = make_ulonglong2(0, 0);
             int n = 0;
             if (ri < re) {
                 r = a.records[ri];
-                n = (r.y >> 63) ? 0 : (int)((r.y >> 48) & 255u) - k + 1;
+                n = (int)((r.y >> 48) & 255u) - k + 1;
                 if (n < 0) n = 0;
             }
             int incl = n;
@@ -584,6 +527,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
                 if (lane >= o) incl += y;
             }
             const int total = __shfl(incl, 63, 64);
+            const int my_excl = incl - n;
             for (int j0 = 0; j0 < total; j0 += 64) {
                 const int j = j0 + lane;
                 // owner lane: first lane whose inclusive count exceeds j
@@ -595,15 +539,15 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
                 }
                 const unsigned long long rx = __shfl(r.x, o, 64);
                 const unsigned long long ry = __shfl(r.y, o, 64);
-                const int excl = __shfl(incl, o, 64) - __shfl(n, o, 64);
+                const int excl = __shfl(my_excl, o, 64);
                 if (j >= total) continue;
                 const int off = j - excl;  // k-mer start inside the record
                 const uint64_t rd = ry & 0xFFFFFFFFFFFFull;  // bases 32..55
                 const int sh = 2 * off;
                 const uint64_t lo = sh == 0 ? rx : (sh < 64 ? ((rx >> sh) | (rd << (64 - sh))) : (rd >> (sh - 64)));
-                const unsigned long long key = canon_from_packed(lo, k);
+                const unsigned long long key = canon_from_mc(lo & kmask, kmask, rshift);
                 const uint32_t val = (MODE == 0) ? 1u : (1u << (uint32_t)((ry >> 56) & 31u));
-                uint32_t slot = (uint32_t)(spread_hash_u64(key) >> 32) & (CNT_SLOTS - 1);
+                uint32_t slot = lds_slot(key);
                 bool done = false;
                 for (int p = 0; p < CNT_MAXPROBE; p++) {
                     const unsigned long long cur = __hip_atomic_load(
@@ -674,7 +618,6 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
         __syncthreads();
     }
     block_sum_atomic(my_distinct, &a.ctr[2], s_red);
-    block_sum_atomic(my_minimizers, &a.ctr[5], s_red);
 }
 
 template <int MODE>
@@ -788,8 +731,7 @@ struct McPlan {
     std::vector<uint64_t> chunk_tile;  // tile index where each chunk starts
     uint64_t n_windows = 0;
     // state
-    DevBuf base, cursor, records, rkey, ctr, ovf_keys, ovf_vals, ovf_part, stage_rec, stage_key, pq, pqs, pidx,
-        pperm, part_tmp;
+    DevBuf base, cursor, records, ctr, ovf_keys, ovf_vals, ovf_part, stage_rec, stage_q, pqs, part_tmp;
     uint64_t stage_cap = 0, n_staged = 0;
     size_t part_tmp_bytes = 0;
     DevBuf out_kmer, out_count, out_part;
@@ -965,14 +907,14 @@ static ScanArgs scan_args(Batch &b, McPlan &p, const mczi_counter_config &cfg) {
     sa.wrec = (uint32_t)(REC_MAX_BASES - k + 1);
     sa.stats = p.ctr.as<unsigned long long>() + 8;
     sa.stage_rec = p.stage_rec.as<ulonglong2>();
-    sa.stage_key = p.stage_key.as<uint32_t>();
+    sa.stage_q = p.stage_q.as<uint32_t>();
     sa.stage_cap = p.stage_cap;
     sa.stage_ctr = p.ctr.as<unsigned long long>() + 11;
     sa.err = p.ctr.as<unsigned long long>() + 4;
     return sa;
 }
 
-static void launch_scan(McPlan &p, const ScanArgs &sa, int mode_kernel, int launch, hipStream_t s) {
+static void launch_scan(McPlan &p, const ScanArgs &sa, int launch, hipStream_t s) {
     Device &d = Device::get();
     const unsigned scan_grid_max = (unsigned)d.cu_count() * 8;
     auto go = [&](uint64_t tb, uint64_t te) {
@@ -981,11 +923,7 @@ static void launch_scan(McPlan &p, const ScanArgs &sa, int mode_kernel, int laun
         ca.tile_begin = tb;
         ca.tile_end = te;
         const unsigned g = (unsigned)std::min<uint64_t>(te - tb, scan_grid_max);
-        switch (mode_kernel) {
-        case PASS1_TWO: hipLaunchKernelGGL(k_scan<PASS1_TWO>, dim3(g), dim3(SCAN_THREADS), 0, s, ca); break;
-        case PASS1_SINGLE: hipLaunchKernelGGL(k_scan<PASS1_SINGLE>, dim3(g), dim3(SCAN_THREADS), 0, s, ca); break;
-        default: hipLaunchKernelGGL(k_scan<PASS1_FOF>, dim3(g), dim3(SCAN_THREADS), 0, s, ca); break;
-        }
+        hipLaunchKernelGGL(k_scan<STAGE>, dim3(g), dim3(SCAN_THREADS), 0, s, ca);
     };
     if (launch == MCZI_LAUNCH_PLAIN) {
         for (size_t c = 0; c + 1 < p.chunk_tile.size(); c++) go(p.chunk_tile[c], p.chunk_tile[c + 1]);
@@ -994,8 +932,8 @@ static void launch_scan(McPlan &p, const ScanArgs &sa, int mode_kernel, int laun
     }
 }
 
-// Stage A: clears, pass 1 (minimizer counts / dataset masks + per-partition
-// record counts), bucket bases. No host synchronization inside.
+// Stage A: clears, scan + record staging (with stage = false only the record
+// count). No host synchronization inside.
 static void enqueue_stage_a(Batch &b, McPlan &p, const mczi_counter_config &cfg, int mode,
                             int launch, hipStream_t s, Timer *tm, bool stage = true) {
     ScanArgs sa = scan_args(b, p, cfg);
@@ -1004,14 +942,13 @@ static void enqueue_stage_a(Batch &b, McPlan &p, const mczi_counter_config &cfg,
     unsigned long long *ctr = p.ctr.as<unsigned long long>();
     fill_u64(ctr, 16, 0, s);
     if (tm) tm->mark(s);
-    launch_scan(p, sa, PASS1_TWO, launch, s);
+    launch_scan(p, sa, launch, s);
     if (tm) tm->mark(s);
 }
 
 static CountArgs count_args(McPlan &p, const mczi_counter_config &cfg) {
     CountArgs ca{};
     ca.records = p.records.as<ulonglong2>();
-    ca.rkey = p.rkey.as<uint32_t>();
     ca.base = p.base.as<unsigned long long>();
     ca.cursor = p.cursor.as<unsigned long long>();
     ca.p_int = p.p_int;
@@ -1035,8 +972,8 @@ static unsigned bits_for(uint64_t n) {
     return b;
 }
 
-// Stage B: records into partition order (radix sort of partition ids),
-// per-partition pass-1 filter + LDS counting with HBM spill, emit.
+// Stage B: records into partition order (radix sort of (partition, record)
+// pairs), per-partition LDS counting with HBM spill, emit.
 static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg, int mode,
                             int launch, hipStream_t s, Timer *tm) {
     Device &d = Device::get();
@@ -1050,28 +987,23 @@ static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg,
     if (tm) tm->mark(s);
     if (n) {
         const unsigned g = grid_for(n, 256, 16384);
-        hipLaunchKernelGGL(k_part_keys, dim3(g), dim3(256), 0, s, p.stage_key.as<uint32_t>(), n, p.p_int,
-                           p.pq.as<uint32_t>(), p.pidx.as<uint32_t>());
         size_t tmp = p.part_tmp_bytes;
-        MCZI_HIP(rocprim::radix_sort_pairs(p.part_tmp.as<void>(), tmp, p.pq.as<uint32_t>(), p.pqs.as<uint32_t>(),
-                                           p.pidx.as<uint32_t>(), p.pperm.as<uint32_t>(), (size_t)n, 0,
-                                           bits_for(p.p_int), s));
-        hipLaunchKernelGGL(k_gather_records, dim3(g), dim3(256), 0, s, p.stage_rec.as<ulonglong2>(),
-                           p.stage_key.as<uint32_t>(), p.pperm.as<uint32_t>(), p.pqs.as<uint32_t>(), n,
-                           p.records.as<ulonglong2>(), p.rkey.as<uint32_t>(), p.base.as<unsigned long long>(),
-                           p.cursor.as<unsigned long long>());
+        MCZI_HIP(rocprim::radix_sort_pairs(p.part_tmp.as<void>(), tmp, p.stage_q.as<uint32_t>(),
+                                           p.pqs.as<uint32_t>(), p.stage_rec.as<ulonglong2>(),
+                                           p.records.as<ulonglong2>(), (size_t)n, 0, bits_for(p.p_int), s));
+        hipLaunchKernelGGL(k_part_bounds, dim3(g), dim3(256), 0, s, p.pqs.as<uint32_t>(), n,
+                           p.base.as<unsigned long long>(), p.cursor.as<unsigned long long>());
     }
     if (tm) tm->mark(s);
     CountArgs ca = count_args(p, cfg);
     const unsigned cgrid = (unsigned)std::min<uint64_t>(p.p_int, (uint64_t)d.cu_count() * 8);
     const unsigned ogrid = grid_for(p.ovf_slots, 256, 8192);
     if (mode == MCZI_MODE_DATASETS) {
-        hipLaunchKernelGGL((k_count<1, 2>), dim3(cgrid), dim3(CNT_THREADS), 0, s, ca);
+        hipLaunchKernelGGL(k_count<1>, dim3(cgrid), dim3(CNT_THREADS), 0, s, ca);
         if (tm) tm->mark(s);
         hipLaunchKernelGGL(k_overflow_emit<1>, dim3(ogrid), dim3(256), 0, s, ca);
     } else {
-        if (mode == MCZI_MODE_TWO_PASS) hipLaunchKernelGGL((k_count<0, 1>), dim3(cgrid), dim3(CNT_THREADS), 0, s, ca);
-        else hipLaunchKernelGGL((k_count<0, 0>), dim3(cgrid), dim3(CNT_THREADS), 0, s, ca);
+        hipLaunchKernelGGL(k_count<0>, dim3(cgrid), dim3(CNT_THREADS), 0, s, ca);
         if (tm) tm->mark(s);
         hipLaunchKernelGGL(k_overflow_emit<0>, dim3(ogrid), dim3(256), 0, s, ca);
     }
@@ -1186,19 +1118,15 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
             p.stage_cap = tot + 16;
             p.rec_cap = tot + 16;
             p.records.ensure(p.rec_cap * 16);
-            p.rkey.ensure(p.rec_cap * 4);
             p.stage_rec.ensure(p.stage_cap * 16);
-            p.stage_key.ensure(p.stage_cap * 4);
-            p.pq.ensure(p.stage_cap * 4);
+            p.stage_q.ensure(p.stage_cap * 4);
             p.pqs.ensure(p.stage_cap * 4);
-            p.pidx.ensure(p.stage_cap * 4);
-            p.pperm.ensure(p.stage_cap * 4);
             p.rec_sig = cfg_sig;
         }
         {
             size_t tmp = 0;
-            MCZI_HIP(rocprim::radix_sort_pairs(nullptr, tmp, p.pq.as<uint32_t>(), p.pqs.as<uint32_t>(),
-                                               p.pidx.as<uint32_t>(), p.pperm.as<uint32_t>(),
+            MCZI_HIP(rocprim::radix_sort_pairs(nullptr, tmp, p.stage_q.as<uint32_t>(), p.pqs.as<uint32_t>(),
+                                               p.stage_rec.as<ulonglong2>(), p.records.as<ulonglong2>(),
                                                (size_t)std::max<uint64_t>(p.n_staged, 1), 0, bits_for(p.p_int), s));
             p.part_tmp.ensure(tmp + 256);
             p.part_tmp_bytes = tmp;
@@ -1211,10 +1139,10 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
         if (use_graph) {
             uint64_t sig = cfg_sig;
             sig = mix_sig(mix_sig(mix_sig(sig, cfg.threshold), (uint64_t)mode), p.p_int);
-            for (const DevBuf *db : {&b.words, &b.off, &b.dataset, &p.base, &p.cursor, &p.records, &p.rkey,
-                                     &p.ctr, &p.ovf_keys, &p.ovf_vals, &p.ovf_part, &p.out_kmer, &p.out_count,
-                                     &p.out_part, &p.tile_start, &p.stage_rec, &p.stage_key, &p.pq, &p.pqs,
-                                     &p.pidx, &p.pperm, &p.part_tmp})
+            for (const DevBuf *db : {&b.words, &b.off, &b.dataset, &p.base, &p.cursor, &p.records, &p.ctr,
+                                     &p.ovf_keys, &p.ovf_vals, &p.ovf_part, &p.out_kmer, &p.out_count,
+                                     &p.out_part, &p.tile_start, &p.stage_rec, &p.stage_q, &p.pqs,
+                                     &p.part_tmp})
                 sig = mix_sig(sig, (uint64_t)(uintptr_t)db->as<void>());
             sig = mix_sig(mix_sig(sig, p.ovf_slots), p.stage_cap);
             if (!p.graph || p.graph_sig != sig) {
@@ -1273,7 +1201,7 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
         st.n_windows = p.n_windows;
         st.n_pieces = h[8];
         st.n_records = p.n_staged;
-        st.minimizers_kept = h[5];
+        st.minimizers_kept = 0;  // no pass-1 filter on the device
         st.n_distinct = h[2] + h[3];
         st.n_overflow = h[3];
         st.n_out = n_out;
@@ -1281,8 +1209,8 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
         st.internal_partitions = p.p_int;
         st.lds_slots = CNT_SLOTS;
         if (want_times && t_dev_end >= 7) {
-            // marks: 0 start, 1 clears, 2 pass 1 (scan + staging), 3 clears,
-            // 4 partition sort + gather, 5 count, 6 spill emit
+            // marks: 0 start, 1 clears, 2 scan + staging, 3 clears,
+            // 4 partition sort + bounds, 5 count, 6 spill emit
             st.ms_scan1 = tm.ms(1, 2);
             st.ms_scan2 = tm.ms(3, 4);
             st.ms_count = tm.ms(4, 6);
