@@ -168,7 +168,7 @@ typedef struct {
     uint64_t n_distinct;         /* distinct k-mers counted */
     uint64_t n_overflow;         /* distinct k-mers that spilled from LDS to HBM */
     uint64_t n_out;              /* k-mers emitted */
-    uint64_t minimizers_kept;    /* distinct minimizer keys (two-pass / fof) */
+    uint64_t tile_windows;       /* windows per scan tile (pieces are also cut at tile starts) */
     uint64_t partition_bytes;    /* bytes of super-k-mer records */
     uint32_t internal_partitions;
     uint32_t lds_slots;

@@ -54,7 +54,7 @@ class _FastaStats(C.Structure):
 
 class RunStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("n_bases", "n_windows", "n_pieces", "n_records", "n_distinct",
-                                         "n_overflow", "n_out", "minimizers_kept", "partition_bytes")] + [
+                                         "n_overflow", "n_out", "tile_windows", "partition_bytes")] + [
         ("internal_partitions", C.c_uint32), ("lds_slots", C.c_uint32)] + [
         (n, C.c_double) for n in ("ms_scan1", "ms_scan2", "ms_count", "ms_sort", "ms_total")]
 

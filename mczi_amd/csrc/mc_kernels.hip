@@ -7,6 +7,7 @@
 // 3483-3523 / add_all_superkmers_packed 3525-3560 (pass 2) ->
 // process_superkmer_partition 2400-2425 / ..._to_kff_u8_bucketed 2512-2570
 // (count + filter). fof: 637-703, 4253-4370, 2184-2247.
+#include <rocprim/block/block_scan.hpp>
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
@@ -22,10 +23,16 @@ namespace mczi {
 // ---------------------------------------------------------------------------
 // Scan kernel geometry
 constexpr int SCAN_THREADS = 256;
-constexpr int WPT = 8;                        // windows per thread
-constexpr int TILE = SCAN_THREADS * WPT;      // windows (base positions) per tile
+constexpr int WPT = 8;                        // windows (and hash positions) per thread
+constexpr int SPAN = SCAN_THREADS * WPT;      // hash / prefix positions per tile
+constexpr int MAXK = 64;
+constexpr int TILE = SPAN - 2 * MAXK;         // windows per tile: tl + k - 1 < SPAN
 constexpr int MAXW = 64;                      // w = k - m + 1 <= 64
-constexpr int WORDS_SPAN = (15 + TILE + 160) / 16 + 8;
+constexpr int WORDS_SPAN = (15 + SPAN + 160) / 16 + 4;
+// LDS arrays read and written with a stride of WPT elements per thread get
+// one pad element every 8, so the 64 lanes of a wave hit distinct banks.
+#define PADDED(n) ((n) + (n) / 8 + 1)
+__device__ __forceinline__ int P8(int i) { return i + (i >> 3); }
 constexpr int REC_MAX_BASES = 56;             // bases per 128-bit super-k-mer record
 
 // Count kernel geometry
@@ -113,28 +120,54 @@ __device__ __forceinline__ void block_sum_atomic(unsigned long long v, unsigned 
     __syncthreads();
 }
 
+struct XorOp {
+    __device__ __forceinline__ unsigned long long operator()(unsigned long long x, unsigned long long y) const {
+        return x ^ y;
+    }
+};
+using ScanXor = rocprim::block_scan<unsigned long long, SCAN_THREADS>;
+
+// 8 bases starting at base index `pos` of the LDS word stream (base j at
+// bits 2j of the result).
+__device__ __forceinline__ uint32_t extract16_bases(const uint32_t *sw, int pos) {
+    const int w = pos >> 4;
+    const uint32_t sh = (uint32_t)(pos & 15) * 2u;
+    const uint64_t v = (uint64_t)sw[w] | ((uint64_t)sw[w + 1] << 32);
+    return (uint32_t)(v >> sh) & 0xffffu;
+}
+
 // One tile = up to TILE consecutive base positions of the segmented stream,
 // clipped to its input chunk. For every window (k-mer start) p of the tile:
-// canonical ntHash of each m-mer (rolling), #TG strand vote over the k bases,
+// canonical ntHash of each m-mer, #TG strand vote over the k bases,
 // leftmost / rightmost minimum of the upper 16 hash bits over the w m-mers
 // (sm/sliding_min.rs:279-340, sm/canonical.rs:11-29, sm/minimizers.rs:213-216),
 // then runs of equal selected hash (sm/collect.rs:38-75) cut at fragment and
-// tile boundaries ("pieces"). Pieces feed pass 1 or pass 2 by MODE.
+// tile boundaries ("pieces"). STAGE writes the pieces as super-k-mer records,
+// WINDOWS the selected hash of every window.
 template <int MODE>
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
+__global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_scan(ScanArgs a) {
     __shared__ uint32_t s_words[WORDS_SPAN];
-    __shared__ uint32_t s_hash[TILE + MAXW];
-    __shared__ uint32_t s_v[TILE];
-    __shared__ uint8_t s_flag[TILE];
+    __shared__ uint32_t s_hash[PADDED(TILE + MAXW)];
+    // prefix XORs live only until the hashes are written; the selected
+    // hashes and piece starts reuse their space
+    __shared__ union {
+        struct {
+            uint32_t sa[PADDED(SPAN)];
+            uint32_t sb[PADDED(SPAN)];
+        } pre;
+        struct {
+            uint32_t v[PADDED(TILE)];
+            uint32_t pst[TILE];
+        } run;
+    } s_u;
+    __shared__ typename ScanXor::storage_type s_scan;
+    __shared__ uint8_t s_flag[PADDED(TILE)];
     __shared__ int32_t s_fs[TILE + 2];
-    __shared__ uint32_t s_pst[TILE];
     __shared__ int s_wsum[SCAN_THREADS / 64];
     __shared__ unsigned long long s_sbase;
 
     const int tid = threadIdx.x;
     const int k = a.k, m = a.m, w = a.w;
-    const uint32_t R1 = (NT_ROT * (uint32_t)m) & 31u;
-    const uint32_t R2 = (NT_ROT * (uint32_t)(m - 1)) & 31u;
     unsigned long long my_pieces = 0;
 
     for (uint64_t t = a.tile_begin + blockIdx.x; t < a.tile_end; t += gridDim.x) {
@@ -144,7 +177,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
         const int rb0 = (int)(ts & 15u);
         const int nwords = ((rb0 + tl + 160) >> 4) + 3;
         __syncthreads();  // previous tile fully consumed
-        for (int i = tid; i < nwords; i += SCAN_THREADS) s_words[i] = a.words[(ts >> 4) + i];
+        for (int i = tid; i < WORDS_SPAN; i += SCAN_THREADS) s_words[i] = i < nwords ? a.words[(ts >> 4) + i] : 0u;
         // Fragment boundaries relative to ts: s_fs[j] = off[f0 + j] - ts (clamped),
         // for j = 0 .. nf, nf = number of fragments with a base in [ts, ts + tl).
         int nf = 0;
@@ -162,35 +195,54 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
         }
         __syncthreads();
 
-        // m-mer hashes for relative positions q in [0, tl + w - 1). Each thread
-        // owns WPT consecutive positions; their bases (<= WPT + 63) are held in
-        // three 32-base registers.
-        const int nh = tl + w - 1;
-        for (int q0 = tid * WPT; q0 < nh; q0 += TILE) {
-            const int rb = rb0 + q0;
-            const uint64_t B0 = extract32_bases(s_words, (uint64_t)rb);
-            const uint64_t B1 = extract32_bases(s_words, (uint64_t)rb + 32);
-            const uint64_t B2 = extract32_bases(s_words, (uint64_t)rb + 64);
-            auto code_at = [&](int i) -> uint32_t {
-                const uint64_t x = i < 32 ? B0 : (i < 64 ? B1 : B2);
-                return (uint32_t)(x >> (2 * (i & 31))) & 3u;
-            };
-            uint32_t fw = 0, rc = 0, rot = 0;
-            for (int j = 0; j < m; j++) {
-                const uint32_t c = code_at(j);
-                fw = rotl32(fw, NT_ROT) ^ nt_seed(c);
-                rc ^= rotl32(nt_seed(c ^ 2u), rot);
-                rot = (rot + NT_ROT) & 31u;
-            }
-            s_hash[q0] = fw + rc;
+        // m-mer hashes for relative positions i in [0, tl + w - 1). ntHash is
+        // XOR-linear, so with the exclusive prefix XORs over the tile's bases
+        //   SA_p = XOR_{p' < p} rotr(F[c_p'], 7p'),  SB_p = XOR_{p' < p} rotl(F[c_p' ^ 2], 7p')
+        // (rotations mod 32) the canonical hash NtHasher<true> rolls is
+        //   fw(i) = rotl(SA_{i+m} ^ SA_i, 7(i+m-1)),  rc(i) = rotr(SB_{i+m} ^ SB_i, 7i),
+        //   h(i) = fw(i) + rc(i)
+        // (vendor/simd-minimizers-2.3.1/src/minimizers.rs:258-291): one term
+        // per base and one combination per position, no per-thread m-base
+        // warm-up.
+        {
+            const int p0 = tid * WPT;
+            const uint32_t bw = extract16_bases(s_words, rb0 + p0);
+            uint32_t amt = (uint32_t)(7 * p0) & 31u;
+            uint32_t ta[WPT], tb[WPT], xa = 0, xb = 0;
 #pragma unroll
-            for (int i = 1; i < WPT; i++) {
-                if (q0 + i >= nh) break;
-                const uint32_t co = code_at(i - 1);
-                const uint32_t ci = code_at(i - 1 + m);
-                fw = rotl32(fw, NT_ROT) ^ rotl32(nt_seed(co), R1) ^ nt_seed(ci);
-                rc = rotr32(rc ^ nt_seed(co ^ 2u), NT_ROT) ^ rotl32(nt_seed(ci ^ 2u), R2);
-                s_hash[q0 + i] = fw + rc;
+            for (int t = 0; t < WPT; t++) {
+                ta[t] = xa;
+                tb[t] = xb;
+                const uint32_t c = (bw >> (2 * t)) & 3u;
+                const uint32_t s_lo = (c & 1u) ? NT_SEED1 : NT_SEED0;
+                const uint32_t s_hi = (c & 1u) ? NT_SEED3 : NT_SEED2;
+                const uint32_t f = (c & 2u) ? s_hi : s_lo;  // F[c]
+                const uint32_t g = (c & 2u) ? s_lo : s_hi;  // F[c ^ 2]
+                xa ^= __builtin_rotateright32(f, amt);
+                xb ^= __builtin_rotateleft32(g, amt);
+                amt = (amt + 7u) & 31u;
+            }
+            unsigned long long pre;
+            ScanXor().exclusive_scan(((unsigned long long)xb << 32) | xa, pre, 0ull, s_scan, XorOp());
+            const uint32_t pa = (uint32_t)pre, pb = (uint32_t)(pre >> 32);
+#pragma unroll
+            for (int t = 0; t < WPT; t++) {
+                s_u.pre.sa[P8(p0 + t)] = pa ^ ta[t];
+                s_u.pre.sb[P8(p0 + t)] = pb ^ tb[t];
+            }
+            __syncthreads();
+            const int nh = tl + w - 1;
+            uint32_t ai = (uint32_t)(7 * p0) & 31u;
+            const uint32_t am = (uint32_t)(7 * (m - 1)) & 31u;
+#pragma unroll
+            for (int t = 0; t < WPT; t++) {
+                const int i = p0 + t;
+                if (i < nh) {
+                    const uint32_t A = s_u.pre.sa[P8(i + m)] ^ pa ^ ta[t];
+                    const uint32_t Bv = s_u.pre.sb[P8(i + m)] ^ pb ^ tb[t];
+                    s_hash[P8(i)] = __builtin_rotateleft32(A, (ai + am) & 31u) + __builtin_rotateright32(Bv, ai);
+                }
+                ai = (ai + 7u) & 31u;
             }
         }
         __syncthreads();
@@ -220,34 +272,38 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
             };
             uint32_t tg = __popcll(B0 & high_bit_mask(k));
             if (k > 32) tg += __popcll(B1 & high_bit_mask(k - 32));
-            uint32_t sufL[WPT], sufR[WPT], preL[WPT], preR[WPT], midL = ~0u, midR = ~0u;
+            // suffix minima over [i, w-1] (the shared middle folded in); prefix
+            // minima over [w, w+i-1] are kept running in the window loop
+            uint32_t sufL[WPT], sufR[WPT], preL = ~0u, preR = ~0u;
             const bool gw = w >= WPT;
             if (gw) {
+                uint32_t midL = ~0u, midR = ~0u;
                 for (int t = WPT - 1; t <= w - 1; t++) {
-                    const uint32_t h = s_hash[r0 + t] & 0xffff0000u;
+                    const uint32_t h = s_hash[P8(r0 + t)] & 0xffff0000u;
                     midL = min(midL, h | (uint32_t)t);
                     midR = min(midR, h | (0xffffu - (uint32_t)t));
                 }
-                sufL[WPT - 1] = sufR[WPT - 1] = ~0u;
+                sufL[WPT - 1] = midL;
+                sufR[WPT - 1] = midR;
 #pragma unroll
                 for (int i = WPT - 2; i >= 0; i--) {
-                    const uint32_t h = s_hash[r0 + i] & 0xffff0000u;
+                    const uint32_t h = s_hash[P8(r0 + i)] & 0xffff0000u;
                     sufL[i] = min(sufL[i + 1], h | (uint32_t)i);
                     sufR[i] = min(sufR[i + 1], h | (0xffffu - (uint32_t)i));
-                }
-                preL[0] = preR[0] = ~0u;
-#pragma unroll
-                for (int i = 1; i < WPT; i++) {
-                    const uint32_t t = (uint32_t)(w + i - 1);
-                    const uint32_t h = s_hash[r0 + t] & 0xffff0000u;
-                    preL[i] = min(preL[i - 1], h | t);
-                    preR[i] = min(preR[i - 1], h | (0xffffu - t));
                 }
             }
 #pragma unroll
             for (int i = 0; i < WPT; i++) {
                 const int r = r0 + i;
-                if (i > 0) tg = tg - tgbit(i - 1) + tgbit(i - 1 + k);
+                if (i > 0) {
+                    tg = tg - tgbit(i - 1) + tgbit(i - 1 + k);
+                    if (gw) {
+                        const uint32_t t = (uint32_t)(w + i - 1);
+                        const uint32_t h = s_hash[P8(r0 + (int)t)] & 0xffff0000u;
+                        preL = min(preL, h | t);
+                        preR = min(preR, h | (0xffffu - t));
+                    }
+                }
                 if (r >= tl) break;
                 while (j + 1 < nf && s_fs[j + 1] <= r) j++;
                 const bool fstart = s_fs[j] == r;
@@ -256,13 +312,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
                 if (valid) {
                     int lp, rp;
                     if (gw) {
-                        lp = (int)(min(min(sufL[i], midL), preL[i]) & 0xffffu);
-                        rp = (int)(0xffffu - (min(min(sufR[i], midR), preR[i]) & 0xffffu));
+                        lp = (int)(min(sufL[i], preL) & 0xffffu);
+                        rp = (int)(0xffffu - (min(sufR[i], preR) & 0xffffu));
                     } else {
-                        uint32_t best = s_hash[r] >> 16;
+                        uint32_t best = s_hash[P8(r)] >> 16;
                         lp = rp = i;
                         for (int ii = 1; ii < w; ii++) {
-                            const uint32_t hv = s_hash[r + ii] >> 16;
+                            const uint32_t hv = s_hash[P8(r + ii)] >> 16;
                             if (hv < best) {
                                 best = hv;
                                 lp = rp = i + ii;
@@ -271,17 +327,17 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
                             }
                         }
                     }
-                    v = (2u * tg > (uint32_t)k) ? s_hash[r0 + lp] : s_hash[r0 + rp];
+                    v = s_hash[P8(r0 + ((2u * tg > (uint32_t)k) ? lp : rp))];
                 }
-                s_v[r] = v;
-                s_flag[r] = (uint8_t)((valid ? 1 : 0) | (fstart ? 2 : 0));
+                s_u.run.v[P8(r)] = v;
+                s_flag[P8(r)] = (uint8_t)((valid ? 1 : 0) | (fstart ? 2 : 0));
             }
         }
         __syncthreads();
 
         if constexpr (MODE == WINDOWS) {
             for (int r = tid; r < tl; r += SCAN_THREADS)
-                if (s_flag[r] & 1) a.win_out[ts + (uint64_t)r] = s_v[r];
+                if (s_flag[P8(r)] & 1) a.win_out[ts + (uint64_t)r] = s_u.run.v[P8(r)];
             continue;
         }
 
@@ -291,8 +347,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
         for (int i = 0; i < WPT; i++) {
             const int r = r0 + i;
             if (r >= tl) break;
-            const uint8_t f = s_flag[r];
-            if ((f & 1) && (r == 0 || (f & 2) || !(s_flag[r - 1] & 1) || s_v[r - 1] != s_v[r])) {
+            const uint8_t f = s_flag[P8(r)];
+            if ((f & 1) && (r == 0 || (f & 2) || !(s_flag[P8(r - 1)] & 1) || s_u.run.v[P8(r - 1)] != s_u.run.v[P8(r)])) {
                 smask |= 1u << i;
                 cnt++;
             }
@@ -300,7 +356,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
         int np = 0;
         int o = block_exclusive_scan(cnt, &np, s_wsum);
         for (int i = 0; i < WPT; i++)
-            if (smask & (1u << i)) s_pst[o++] = (uint32_t)(r0 + i);
+            if (smask & (1u << i)) s_u.run.pst[o++] = (uint32_t)(r0 + i);
         __syncthreads();
 
         // Pieces -> super-k-mer records of at most wrec windows (bases
@@ -313,8 +369,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
         // 3192-3197), base j of the record at bits 2j.
         int my_nrec = 0;
         for (int pi = tid; pi < np; pi += SCAN_THREADS) {
-            const int s = (int)s_pst[pi];
-            int e = (pi + 1 < np) ? (int)s_pst[pi + 1] : tl;
+            const int s = (int)s_u.run.pst[pi];
+            int e = (pi + 1 < np) ? (int)s_u.run.pst[pi + 1] : tl;
             int lo = 0, hi = nf - 1;
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
@@ -333,8 +389,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
         if (a.stage_rec != nullptr) {
             uint64_t pos = s_sbase + (uint64_t)off;
             for (int pi = tid; pi < np; pi += SCAN_THREADS) {
-                const int s = (int)s_pst[pi];
-                int e = (pi + 1 < np) ? (int)s_pst[pi + 1] : tl;
+                const int s = (int)s_u.run.pst[pi];
+                int e = (pi + 1 < np) ? (int)s_u.run.pst[pi + 1] : tl;
                 int lo = 0, hi = nf - 1;
                 while (lo < hi) {
                     const int mid = (lo + hi + 1) >> 1;
@@ -343,7 +399,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
                 }
                 const int vend = s_fs[lo + 1] - k + 1;
                 if (vend < e) e = vend;
-                const uint32_t q = spread_hash_u32(spread_hash_u32(s_v[s])) % a.p_int;
+                const uint32_t q = spread_hash_u32(spread_hash_u32(s_u.run.v[P8(s)])) % a.p_int;
                 const uint64_t ds = a.dataset != nullptr ? (uint64_t)a.dataset[f0 + lo] : 0ull;
                 for (int cs = s; cs < e; cs += (int)a.wrec) {
                     const int ce = min(cs + (int)a.wrec, e);
@@ -454,6 +510,28 @@ __device__ __forceinline__ unsigned long long wave_append(bool pred, unsigned lo
     return base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
+// Inclusive scans over the 64 lanes of a wave with DPP row shifts (within
+// 16-lane rows) and row broadcasts (lane 15 -> next row, lane 31 -> rows 2-3).
+__device__ __forceinline__ int wave_incl_add(int x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return x;
+}
+// (x >= 0 for every lane)
+__device__ __forceinline__ int wave_incl_max(int x) {
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));
+    return x;
+}
+
 // Canonical MC encoding (src/lib.rs:3589-3614) of the k <= 32 MC-coded bases
 // in x (base j at bits 2j, x already masked to 2k bits): forward = 2-bit
 // groups reversed (first base most significant), reverse complement =
@@ -486,15 +564,18 @@ __device__ __forceinline__ uint32_t lds_slot(uint64_t key) {
 // minimizer (the scheme is canonical), so that minimizer's window sum
 // (dataset set) is >= c(K) (covers K's datasets), and the filter can only
 // drop k-mers that fail "> x" anyway; the emitted set and counts are equal.
-// Work is balanced per wave: 64 records are loaded, their k-mers numbered by
-// a wave prefix sum, and lane j takes k-mer j (owner record found by a
-// shuffle binary search), extracting it directly from the 128-bit record
+// Work is balanced per wave: 64 records are loaded (and kept in LDS), their
+// k-mers numbered by a wave prefix sum, and lane j takes k-mer j (its owner
+// record found by a max-scan of record-start markers), extracting it directly
+// from the 128-bit record
 // (the same canonical k-mers add_packed_superkmer_kmer_counts rolls,
 // src/lib.rs:3589-3614).
 template <int MODE>
 __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
     __shared__ unsigned long long s_keys[CNT_SLOTS];
     __shared__ uint32_t s_vals[CNT_SLOTS];
+    __shared__ ulonglong2 s_rec[CNT_THREADS / 64][64];  // each wave's current 64 records
+    __shared__ uint8_t s_mark[CNT_THREADS / 64][64];    // record starts in the current 64 k-mers
     __shared__ int s_wsum[CNT_THREADS / 64];
     __shared__ unsigned long long s_red[CNT_THREADS / 64];
     __shared__ unsigned long long s_obase;
@@ -520,28 +601,29 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
                 n = (int)((r.y >> 48) & 255u) - k + 1;
                 if (n < 0) n = 0;
             }
-            int incl = n;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int y = __shfl_up(incl, o, 64);
-                if (lane >= o) incl += y;
-            }
-            const int total = __shfl(incl, 63, 64);
+            s_rec[wid][lane] = r;
+            const int incl = wave_incl_add(n);
+            const int total = __builtin_amdgcn_readlane(incl, 63);
             const int my_excl = incl - n;
+            int carry = 0;
             for (int j0 = 0; j0 < total; j0 += 64) {
                 const int j = j0 + lane;
-                // owner lane: first lane whose inclusive count exceeds j
-                int o = 0;
-#pragma unroll
-                for (int st = 32; st > 0; st >>= 1) {
-                    const int v = __shfl(incl, o + st - 1, 64);
-                    if (v <= j) o += st;
-                }
-                const unsigned long long rx = __shfl(r.x, o, 64);
-                const unsigned long long ry = __shfl(r.y, o, 64);
+                // owner of k-mer j: the last record starting at or before j.
+                // Each record starting in this chunk leaves lane + 1 at its
+                // start; an inclusive max-scan spreads it forward.
+                s_mark[wid][lane] = 0;
+                if (n > 0 && my_excl >= j0 && my_excl < j0 + 64) s_mark[wid][my_excl - j0] = (uint8_t)(lane + 1);
+                __builtin_amdgcn_wave_barrier();
+                int mk = wave_incl_max((int)s_mark[wid][lane]);
+                mk = max(mk, carry);
+                carry = __builtin_amdgcn_readlane(mk, 63);
+                __builtin_amdgcn_wave_barrier();
+                const int o = (mk - 1) & 63;
                 const int excl = __shfl(my_excl, o, 64);
                 if (j >= total) continue;
                 const int off = j - excl;  // k-mer start inside the record
+                const ulonglong2 ro = s_rec[wid][o];
+                const unsigned long long rx = ro.x, ry = ro.y;
                 const uint64_t rd = ry & 0xFFFFFFFFFFFFull;  // bases 32..55
                 const int sh = 2 * off;
                 const uint64_t lo = sh == 0 ? rx : (sh < 64 ? ((rx >> sh) | (rd << (64 - sh))) : (rd >> (sh - 64)));
@@ -751,6 +833,13 @@ struct McPlan {
 
 void free_plan(McPlan *p) { delete p; }
 
+// Blocks of `threads` threads of kernel f that one CU holds at once.
+static unsigned resident_blocks(const void *f, unsigned threads) {
+    int nb = 0;
+    MCZI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, (int)threads, 0));
+    return (unsigned)std::max(nb, 1);
+}
+
 static unsigned grid_for(uint64_t n, unsigned threads, unsigned max_blocks) {
     uint64_t b = (n + threads - 1) / threads;
     if (b < 1) b = 1;
@@ -916,7 +1005,8 @@ static ScanArgs scan_args(Batch &b, McPlan &p, const mczi_counter_config &cfg) {
 
 static void launch_scan(McPlan &p, const ScanArgs &sa, int launch, hipStream_t s) {
     Device &d = Device::get();
-    const unsigned scan_grid_max = (unsigned)d.cu_count() * 8;
+    static const unsigned scan_res = resident_blocks((const void *)k_scan<STAGE>, SCAN_THREADS);
+    const unsigned scan_grid_max = (unsigned)d.cu_count() * scan_res;
     auto go = [&](uint64_t tb, uint64_t te) {
         if (te <= tb) return;
         ScanArgs ca = sa;
@@ -996,7 +1086,9 @@ static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg,
     }
     if (tm) tm->mark(s);
     CountArgs ca = count_args(p, cfg);
-    const unsigned cgrid = (unsigned)std::min<uint64_t>(p.p_int, (uint64_t)d.cu_count() * 8);
+    // persistent grid: exactly the blocks that are resident at once
+    static const unsigned count_res = resident_blocks((const void *)k_count<0>, CNT_THREADS);
+    const unsigned cgrid = (unsigned)std::min<uint64_t>(p.p_int, (uint64_t)d.cu_count() * count_res);
     const unsigned ogrid = grid_for(p.ovf_slots, 256, 8192);
     if (mode == MCZI_MODE_DATASETS) {
         hipLaunchKernelGGL(k_count<1>, dim3(cgrid), dim3(CNT_THREADS), 0, s, ca);
@@ -1201,7 +1293,7 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
         st.n_windows = p.n_windows;
         st.n_pieces = h[8];
         st.n_records = p.n_staged;
-        st.minimizers_kept = 0;  // no pass-1 filter on the device
+        st.tile_windows = TILE;
         st.n_distinct = h[2] + h[3];
         st.n_overflow = h[3];
         st.n_out = n_out;

@@ -287,3 +287,40 @@ def test_c2_full_size_properties():
     # single pass (KMC-style) gives the same set and counts
     b.count(cfg, M.MODE_SINGLE_PASS)
     assert same_counted(b.download(), a)
+
+
+@pytest.mark.parametrize("k,m", [(31, 21), (21, 11), (15, 9)])
+def test_pieces_and_records_match_oracle_runs(k, m):
+    """The scan's super-k-mer split: pieces = the oracle's minimizer runs of
+    each fragment, also cut where a scan tile starts inside a run; each piece
+    becomes ceil(windows / (57 - k)) records (src/lib.rs:3483-3560)."""
+    print(f"pieces k={k} m={m}", flush=True)
+    long = M.SynthReads(M.SynthSpec(genome_len=50_000, genome_seed=7, read_seed=17, long_bases=60_000,
+                                    long_median=6000, long_sigma=0.5, long_min=500, long_max=20_000,
+                                    sub_rate=0.01))
+    short = M.SynthReads(M.SynthSpec(genome_len=50_000, genome_seed=7, read_seed=27, n_reads=600,
+                                     read_len=150, sub_rate=0.01))
+    words, off = _concat(long, short)
+    b = M.Batch.from_packed(words, off)
+    st = b.count(M.CounterConfig(k, m, 0, 64))
+    tile = int(st.tile_windows)
+    assert 0 < tile
+    n_bases = int(off[-1])
+    codes = np.zeros(n_bases, dtype=np.uint8)
+    codes[: int(long.frag_off[-1])] = long.codes()
+    codes[n_bases - int(short.frag_off[-1]):] = short.codes()
+    wrec = 57 - k
+    pieces = records = 0
+    for f in range(len(off) - 1):
+        s0, s1 = int(off[f]), int(off[f + 1])
+        nwin = s1 - s0 - k + 1
+        if nwin <= 0:
+            continue
+        _, starts = O.minimizer_runs(np.ascontiguousarray(codes[s0:s1]), k, m)
+        cuts = set(int(x) for x in starts)
+        first_tile = (s0 // tile + 1) * tile
+        cuts.update(t - s0 for t in range(first_tile, s0 + nwin, tile))
+        bounds = sorted(cuts) + [nwin]
+        pieces += len(bounds) - 1
+        records += sum((bounds[i + 1] - bounds[i] + wrec - 1) // wrec for i in range(len(bounds) - 1))
+    assert (int(st.n_pieces), int(st.n_records)) == (pieces, records)

@@ -127,6 +127,36 @@ def test_window_hash_reverse_complement_symmetry():
             assert np.array_equal(a, b[::-1]), (k, m, s)
 
 
+def test_prefix_xor_form_of_nthash():
+    """The scan kernel computes every m-mer hash from two exclusive prefix
+    XORs of per-base terms rotated by their position (ntHash is XOR-linear):
+    fw(i) = rotl(SA[i+m] ^ SA[i], 7(i+m-1)), rc(i) = rotr(SB[i+m] ^ SB[i], 7i).
+    Any origin of the positions gives the same hashes as the oracle's rolling
+    NtHasher<true>."""
+    seeds = [0x95C60474, 0x62A02B4C, 0x82572324, 0x4BE24456]
+    M = 0xFFFFFFFF
+
+    def rotl(x, r):
+        r &= 31
+        return ((x << r) | (x >> (32 - r))) & M if r else x
+
+    rng = np.random.default_rng(11)
+    for m in (1, 2, 3, 11, 19, 21, 31):
+        codes = rng.integers(0, 4, size=160).astype(np.uint8)
+        for origin in (0, 5, 37):
+            sa, sb = [0], [0]
+            for j, c in enumerate(codes):
+                p = origin + j
+                sa.append(sa[-1] ^ rotl(seeds[c], -7 * p))
+                sb.append(sb[-1] ^ rotl(seeds[c ^ 2], 7 * p))
+            for i in range(len(codes) - m + 1):
+                p = origin + i
+                fw = rotl(sa[i + m] ^ sa[i], 7 * (p + m - 1))
+                rc = rotl(sb[i + m] ^ sb[i], -7 * p)
+                want = O.nthash32(np.ascontiguousarray(codes[i:i + m]), m, True)
+                assert (fw + rc) & M == want, (m, origin, i)
+
+
 # ---------------------------------------------------------------------------
 # Counting
 def test_exact_counts(golden):
