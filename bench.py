@@ -185,6 +185,7 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--zi-requests", type=int, default=48)
     ap.add_argument("--no-zi", action="store_true")
+    ap.add_argument("--zi-only", action="store_true", help="run only the ZI serving measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -192,6 +193,10 @@ def main() -> int:
         return 2
 
     import mczi_amd as M
+
+    if args.zi_only:
+        print(json.dumps({"zi": zi_serving(M, args.zi_requests, 31)}), flush=True)
+        return 0
 
     spec_kw, k, m, x, parts, mode = WORKLOADS[args.workload]
     t0 = time.perf_counter()

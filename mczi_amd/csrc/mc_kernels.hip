@@ -25,6 +25,7 @@ namespace mczi {
 constexpr int SCAN_THREADS = 256;
 constexpr int WPT = 8;                        // windows (and hash positions) per thread
 constexpr int SPAN = SCAN_THREADS * WPT;      // hash / prefix positions per tile
+static_assert(WPT == 8, "the padded LDS indexing assumes 8 positions per thread");
 constexpr int MAXK = 64;
 constexpr int TILE = SPAN - 2 * MAXK;         // windows per tile: tl + k - 1 < SPAN
 constexpr int MAXW = 64;                      // w = k - m + 1 <= 64
@@ -169,6 +170,10 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
     const int tid = threadIdx.x;
     const int k = a.k, m = a.m, w = a.w;
     unsigned long long my_pieces = 0;
+    // each thread owns positions r0 .. r0 + WPT - 1 of a tile; P8(r0 + c),
+    // the padded LDS index, is 9 * tid + c + floor(c / 8) since r0 = 8 * tid
+    const int r0 = tid * WPT;
+    auto px = [&](int c) { return 9 * tid + c + (c >> 3); };
 
     for (uint64_t t = a.tile_begin + blockIdx.x; t < a.tile_end; t += gridDim.x) {
         const uint64_t ts = a.tile_start[t];
@@ -205,9 +210,8 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
         // per base and one combination per position, no per-thread m-base
         // warm-up.
         {
-            const int p0 = tid * WPT;
-            const uint32_t bw = extract16_bases(s_words, rb0 + p0);
-            uint32_t amt = (uint32_t)(7 * p0) & 31u;
+            const uint32_t bw = extract16_bases(s_words, rb0 + r0);
+            uint32_t amt = (uint32_t)(7 * r0) & 31u;
             uint32_t ta[WPT], tb[WPT], xa = 0, xb = 0;
 #pragma unroll
             for (int t = 0; t < WPT; t++) {
@@ -227,20 +231,19 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             const uint32_t pa = (uint32_t)pre, pb = (uint32_t)(pre >> 32);
 #pragma unroll
             for (int t = 0; t < WPT; t++) {
-                s_u.pre.sa[P8(p0 + t)] = pa ^ ta[t];
-                s_u.pre.sb[P8(p0 + t)] = pb ^ tb[t];
+                s_u.pre.sa[px(t)] = pa ^ ta[t];
+                s_u.pre.sb[px(t)] = pb ^ tb[t];
             }
             __syncthreads();
             const int nh = tl + w - 1;
-            uint32_t ai = (uint32_t)(7 * p0) & 31u;
+            uint32_t ai = (uint32_t)(7 * r0) & 31u;
             const uint32_t am = (uint32_t)(7 * (m - 1)) & 31u;
 #pragma unroll
             for (int t = 0; t < WPT; t++) {
-                const int i = p0 + t;
-                if (i < nh) {
-                    const uint32_t A = s_u.pre.sa[P8(i + m)] ^ pa ^ ta[t];
-                    const uint32_t Bv = s_u.pre.sb[P8(i + m)] ^ pb ^ tb[t];
-                    s_hash[P8(i)] = __builtin_rotateleft32(A, (ai + am) & 31u) + __builtin_rotateright32(Bv, ai);
+                if (r0 + t < nh) {
+                    const uint32_t A = s_u.pre.sa[px(t + m)] ^ pa ^ ta[t];
+                    const uint32_t Bv = s_u.pre.sb[px(t + m)] ^ pb ^ tb[t];
+                    s_hash[px(t)] = __builtin_rotateleft32(A, (ai + am) & 31u) + __builtin_rotateright32(Bv, ai);
                 }
                 ai = (ai + 7u) & 31u;
             }
@@ -253,7 +256,6 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
         // 279-340). With w >= WPT the thread's WPT windows share the middle
         // hashes [WPT-1, w-1]; suffix minima on the left and prefix minima on
         // the right complete each window (van Herk / Gil-Werman).
-        const int r0 = tid * WPT;
         if (r0 < tl) {
             int lo = 0, hi = nf - 1;
             while (lo < hi) {
@@ -262,16 +264,15 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
                 else hi = mid - 1;
             }
             int j = lo;
+            int fcur = s_fs[j], fnext = s_fs[j + 1];  // bounds of the fragment holding r
             const int rb = rb0 + r0;
-            const uint64_t B0 = extract32_bases(s_words, (uint64_t)rb);
-            const uint64_t B1 = extract32_bases(s_words, (uint64_t)rb + 32);
-            const uint64_t B2 = extract32_bases(s_words, (uint64_t)rb + 64);
-            auto tgbit = [&](int i) -> uint32_t {
-                const uint64_t x = i < 32 ? B0 : (i < 64 ? B1 : B2);
-                return (uint32_t)(x >> (2 * (i & 31) + 1)) & 1u;
-            };
-            uint32_t tg = __popcll(B0 & high_bit_mask(k));
-            if (k > 32) tg += __popcll(B1 & high_bit_mask(k - 32));
+            // T/G bits (bit 2b+1 of each code) of bases r0 .. r0 + 95; the
+            // strand vote of window i counts them over bases [i, i + k)
+            const uint64_t H0 = extract32_bases(s_words, (uint64_t)rb) & 0xAAAAAAAAAAAAAAAAull;
+            const uint64_t H1 = extract32_bases(s_words, (uint64_t)rb + 32) & 0xAAAAAAAAAAAAAAAAull;
+            const uint64_t H2 = k > 32 ? extract32_bases(s_words, (uint64_t)rb + 64) & 0xAAAAAAAAAAAAAAAAull : 0ull;
+            const uint64_t mlo = high_bit_mask(k < 32 ? k : 32);
+            const uint64_t mhi = k > 32 ? high_bit_mask(k - 32) : 0ull;
             // suffix minima over [i, w-1] (the shared middle folded in); prefix
             // minima over [w, w+i-1] are kept running in the window loop
             uint32_t sufL[WPT], sufR[WPT], preL = ~0u, preR = ~0u;
@@ -279,7 +280,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             if (gw) {
                 uint32_t midL = ~0u, midR = ~0u;
                 for (int t = WPT - 1; t <= w - 1; t++) {
-                    const uint32_t h = s_hash[P8(r0 + t)] & 0xffff0000u;
+                    const uint32_t h = s_hash[px(t)] & 0xffff0000u;
                     midL = min(midL, h | (uint32_t)t);
                     midR = min(midR, h | (0xffffu - (uint32_t)t));
                 }
@@ -287,7 +288,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
                 sufR[WPT - 1] = midR;
 #pragma unroll
                 for (int i = WPT - 2; i >= 0; i--) {
-                    const uint32_t h = s_hash[P8(r0 + i)] & 0xffff0000u;
+                    const uint32_t h = s_hash[px(i)] & 0xffff0000u;
                     sufL[i] = min(sufL[i + 1], h | (uint32_t)i);
                     sufR[i] = min(sufR[i + 1], h | (0xffffu - (uint32_t)i));
                 }
@@ -295,30 +296,37 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
 #pragma unroll
             for (int i = 0; i < WPT; i++) {
                 const int r = r0 + i;
-                if (i > 0) {
-                    tg = tg - tgbit(i - 1) + tgbit(i - 1 + k);
-                    if (gw) {
-                        const uint32_t t = (uint32_t)(w + i - 1);
-                        const uint32_t h = s_hash[P8(r0 + (int)t)] & 0xffff0000u;
-                        preL = min(preL, h | t);
-                        preR = min(preR, h | (0xffffu - t));
-                    }
+                if (i > 0 && gw) {
+                    const int t = w + i - 1;
+                    const uint32_t h = s_hash[px(t)] & 0xffff0000u;
+                    preL = min(preL, h | (uint32_t)t);
+                    preR = min(preR, h | (0xffffu - (uint32_t)t));
                 }
                 if (r >= tl) break;
-                while (j + 1 < nf && s_fs[j + 1] <= r) j++;
-                const bool fstart = s_fs[j] == r;
-                const bool valid = r + k <= s_fs[j + 1];
+                while (j + 1 < nf && fnext <= r) {
+                    j++;
+                    fcur = fnext;
+                    fnext = s_fs[j + 1];
+                }
+                const bool fstart = fcur == r;
+                const bool valid = r + k <= fnext;
                 uint32_t v = 0;
                 if (valid) {
+                    const uint64_t lo64 = i ? ((H0 >> (2 * i)) | (H1 << (64 - 2 * i))) : H0;
+                    uint32_t tg = (uint32_t)__popcll(lo64 & mlo);
+                    if (k > 32) {
+                        const uint64_t hi64 = i ? ((H1 >> (2 * i)) | (H2 << (64 - 2 * i))) : H1;
+                        tg += (uint32_t)__popcll(hi64 & mhi);
+                    }
                     int lp, rp;
                     if (gw) {
                         lp = (int)(min(sufL[i], preL) & 0xffffu);
                         rp = (int)(0xffffu - (min(sufR[i], preR) & 0xffffu));
                     } else {
-                        uint32_t best = s_hash[P8(r)] >> 16;
+                        uint32_t best = s_hash[px(i)] >> 16;
                         lp = rp = i;
                         for (int ii = 1; ii < w; ii++) {
-                            const uint32_t hv = s_hash[P8(r + ii)] >> 16;
+                            const uint32_t hv = s_hash[px(i + ii)] >> 16;
                             if (hv < best) {
                                 best = hv;
                                 lp = rp = i + ii;
@@ -327,10 +335,10 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
                             }
                         }
                     }
-                    v = s_hash[P8(r0 + ((2u * tg > (uint32_t)k) ? lp : rp))];
+                    v = s_hash[px((2u * tg > (uint32_t)k) ? lp : rp)];
                 }
-                s_u.run.v[P8(r)] = v;
-                s_flag[P8(r)] = (uint8_t)((valid ? 1 : 0) | (fstart ? 2 : 0));
+                s_u.run.v[px(i)] = v;
+                s_flag[px(i)] = (uint8_t)((valid ? 1 : 0) | (fstart ? 2 : 0));
             }
         }
         __syncthreads();
@@ -347,8 +355,8 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
         for (int i = 0; i < WPT; i++) {
             const int r = r0 + i;
             if (r >= tl) break;
-            const uint8_t f = s_flag[P8(r)];
-            if ((f & 1) && (r == 0 || (f & 2) || !(s_flag[P8(r - 1)] & 1) || s_u.run.v[P8(r - 1)] != s_u.run.v[P8(r)])) {
+            const uint8_t f = s_flag[px(i)];
+            if ((f & 1) && (r == 0 || (f & 2) || !(s_flag[px(i - 1)] & 1) || s_u.run.v[px(i - 1)] != s_u.run.v[px(i)])) {
                 smask |= 1u << i;
                 cnt++;
             }
