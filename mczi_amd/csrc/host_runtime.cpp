@@ -76,6 +76,58 @@ void Device::release(void *p) {
     live_.erase(it);
 }
 
+void *Device::host_alloc(size_t bytes) {
+    size_t want = 64u << 10;
+    while (want < bytes) want <<= 1;
+    {
+        std::lock_guard<std::mutex> lk(hmu_);
+        auto it = hfree_.find(want);
+        if (it != hfree_.end()) {
+            void *p = it->second;
+            hfree_.erase(it);
+            hcached_ -= want;
+            hlive_[p] = want;
+            return p;
+        }
+    }
+    void *p = nullptr;
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        {
+            std::lock_guard<std::mutex> lk(hmu_);
+            for (auto &kv : hfree_) (void)hipHostFree(kv.second);
+            hfree_.clear();
+            hcached_ = 0;
+        }
+        p = nullptr;
+        e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(MCZI_E_NOMEM, "pinned host allocation of " + std::to_string(want) +
+                                          " bytes failed: " + hipGetErrorString(e));
+        }
+    }
+    std::lock_guard<std::mutex> lk(hmu_);
+    hlive_[p] = want;
+    return p;
+}
+
+void Device::host_release(void *p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(hmu_);
+    auto it = hlive_.find(p);
+    if (it == hlive_.end()) return;
+    const size_t n = it->second;
+    hlive_.erase(it);
+    if (hcached_ + n > kPinnedCacheBytes) {
+        (void)hipHostFree(p);
+        return;
+    }
+    hfree_.emplace(n, p);
+    hcached_ += n;
+}
+
 // ---------------------------------------------------------------------------
 // FASTA / FASTQ ingest. Base codes are packed-seq codes A0 C1 T2 G3; lower
 // case acgt count as bases (src/zi.rs:240-248 and src/lib.rs:3015-3026 map

@@ -45,6 +45,11 @@ class Device {
     // Cached device allocation (rounded to 2 MiB); contents undefined.
     void *alloc(size_t bytes);
     void release(void *p);
+    // Cached pinned host allocation (hipHostMalloc, power-of-two size
+    // classes): device <-> host copies into it run as plain DMA. Released
+    // blocks stay cached up to kPinnedCacheBytes.
+    void *host_alloc(size_t bytes);
+    void host_release(void *p);
     std::mutex &call_mutex() { return call_mu_; }
 
   private:
@@ -56,6 +61,11 @@ class Device {
     std::mutex call_mu_;
     std::multimap<size_t, void *> free_;
     std::map<void *, size_t> live_;
+    static constexpr size_t kPinnedCacheBytes = 16ull << 30;
+    std::mutex hmu_;
+    std::multimap<size_t, void *> hfree_;
+    std::map<void *, size_t> hlive_;
+    size_t hcached_ = 0;
 };
 
 // RAII device buffer from the cache.

@@ -23,8 +23,9 @@ struct ZiIndex {
 void zi_build_index(ZiIndex &idx, HostReads &&reads, uint32_t k, uint32_t m, hipStream_t s);
 
 // One batch of query requests (fragments tagged with their request slot).
-// Sorted, unique absent k-mers of one request, in malloc'd arrays (ownership
-// passes to the caller's result).
+// Sorted, unique absent k-mers of one request, in pinned host arrays from
+// the device's host pool (ownership passes to the caller's result, which
+// returns them with Device::host_release).
 struct ZiKeys {
     uint64_t *lo = nullptr, *hi = nullptr;
     size_t n = 0;
@@ -34,8 +35,7 @@ struct ZiKeys {
     ZiKeys(ZiKeys &&o) noexcept : lo(o.lo), hi(o.hi), n(o.n) { o.lo = o.hi = nullptr; o.n = 0; }
     ZiKeys &operator=(ZiKeys &&o) noexcept {
         if (this != &o) {
-            free(lo);
-            free(hi);
+            release();
             lo = o.lo;
             hi = o.hi;
             n = o.n;
@@ -44,9 +44,15 @@ struct ZiKeys {
         }
         return *this;
     }
-    ~ZiKeys() {
-        free(lo);
-        free(hi);
+    ~ZiKeys() { release(); }
+    void release() {
+        if (lo || hi) {
+            Device &d = Device::get();
+            d.host_release(lo);
+            d.host_release(hi);
+        }
+        lo = hi = nullptr;
+        n = 0;
     }
     std::vector<mczi_counted_kmer> counted() const {
         std::vector<mczi_counted_kmer> v(n);
@@ -58,6 +64,7 @@ struct ZiKeys {
 struct ZiQueryResult {
     std::vector<ZiKeys> absent;     // per request
     std::vector<uint64_t> windows;  // per request
+    double wait_ms = 0;             // host wait for the batch's device work
 };
 
 // Device memory a batch of `bases` query bases in `frags` fragments needs.
@@ -66,9 +73,24 @@ uint64_t zi_batch_bytes(uint64_t bases, uint64_t frags, bool wide);
 struct ZiWork;  // device buffers of one in-flight batch
 std::unique_ptr<ZiWork> zi_make_work();
 void zi_free_work(ZiWork *w);
-// Enqueue upload + lookup + segmented sort + unique on stream s (async).
-void zi_enqueue(ZiWork &w, const ZiIndex &idx, HostReads &&q, std::vector<uint32_t> &&frag_req, uint32_t n_req,
-                hipStream_t s);
+// A batch of query requests laid out in one device word buffer: fragment
+// offsets in bases (requests start on word boundaries; a gap before a
+// request is a fragment of no request), the request slot of each fragment,
+// and the host word ranges copied to their place (pinned sources copy as
+// plain DMA). Sources must stay valid until the batch is collected.
+struct ZiWordSpan {
+    const uint32_t *src;
+    uint64_t n_words;
+    uint64_t dst_word;
+};
+struct ZiBatchInput {
+    std::vector<uint64_t> off{0};
+    std::vector<uint32_t> frag_req;
+    std::vector<ZiWordSpan> spans;
+    uint64_t n_words = 0;
+};
+// Enqueue upload + lookup + per-request sort + unique on stream s (async).
+void zi_enqueue(ZiWork &w, const ZiIndex &idx, ZiBatchInput &&in, uint32_t n_req, hipStream_t s);
 // Wait for the batch and collect per-request results.
 void zi_collect(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, ZiQueryResult &out);
 

@@ -20,6 +20,7 @@ namespace mczi {
 constexpr int ZI_THREADS = 256;
 constexpr int ZI_WPT = 8;
 constexpr int ZI_TILE = ZI_THREADS * ZI_WPT;
+constexpr int ZI_WORDS = (15 + ZI_TILE + 64 + 32) / 16 + 4;  // tile bases + k + extraction slack
 constexpr uint32_t NO_REQ = 0xffffffffu;
 
 struct Key2 {
@@ -162,74 +163,101 @@ __global__ __launch_bounds__(ZI_THREADS) void k_zi_build(ZiTileArgs a) {
 }
 
 // Lookup: every window of every query fragment is looked up in the index;
-// absent canonical k-mers are appended to their request's segment.
+// absent canonical k-mers are appended to their request's segment. The
+// tile's bases are staged in LDS; each thread first forms the keys of its
+// ZI_WPT windows, then probes all of them in rounds, so up to ZI_WPT
+// independent table reads per thread are in flight.
 template <bool WIDE>
 __global__ __launch_bounds__(ZI_THREADS) void k_zi_lookup(ZiTileArgs a) {
     __shared__ int32_t s_fs[ZI_TILE + 2];
+    __shared__ uint32_t s_words[ZI_WORDS];
     unsigned long long my_lookups = 0;
     const int lane = threadIdx.x & 63;
     for (uint64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
         const uint64_t ts = a.tile_start[t];
         const int tl = (int)a.tile_len[t];
         const uint64_t f0 = a.tile_frag[t];
+        const uint64_t w0 = ts >> 4;
+        const int nw = (int)((((ts & 15u) + (uint64_t)tl + 64 + 32) >> 4) + 2);
         __syncthreads();
+        for (int i = threadIdx.x; i < ZI_WORDS; i += ZI_THREADS) s_words[i] = i < nw ? a.words[w0 + i] : 0u;
         const int nf = load_fragments(a, ts, tl, f0, s_fs);
+        const uint64_t rb = ts & 15u;  // tile start inside s_words
+        Key2 key[ZI_WPT];
+        uint64_t slot[ZI_WPT];
+        uint32_t req[ZI_WPT];
+        uint32_t pending = 0, absent = 0;
+#pragma unroll
         for (int i = 0; i < ZI_WPT; i++) {
             const int r = i * ZI_THREADS + (int)threadIdx.x;
-            bool absent = false;
-            uint32_t req = NO_REQ;
-            Key2 key{0, 0};
+            req[i] = NO_REQ;
+            key[i] = Key2{0, 0};
+            slot[i] = 0;
             if (r < tl) {
                 const int j = frag_of(s_fs, nf, r);
-                req = a.frag_req[f0 + j];
-                if (r + a.k <= s_fs[j + 1] && req != NO_REQ) {
+                req[i] = a.frag_req[f0 + j];
+                if (r + a.k <= s_fs[j + 1] && req[i] != NO_REQ) {
                     my_lookups++;
                     if constexpr (!WIDE) {
-                        key.lo = canon64(a.words, ts + r, a.k);
-                        uint64_t s = slot_of(key.lo, 0) & a.mask;
-                        for (;;) {
-                            const unsigned long long cur = a.tlo[s];
-                            if (cur == key.lo) break;
-                            if (cur == EMPTY_KEY) {
-                                absent = true;
-                                break;
-                            }
-                            s = (s + 1) & a.mask;
-                        }
+                        key[i].lo = canon64(s_words, rb + (uint64_t)r, a.k);
+                        slot[i] = slot_of(key[i].lo, 0) & a.mask;
                     } else {
-                        key = canon128(a.words, ts + r, a.k);
-                        uint64_t s = slot_of(key.lo, key.hi) & a.mask;
-                        for (;;) {
-                            const unsigned long long ch = a.thi[s];
-                            if (ch == EMPTY_KEY) {
-                                absent = true;
-                                break;
-                            }
-                            if (ch == key.hi && a.tlo[s] == key.lo) break;
-                            s = (s + 1) & a.mask;
-                        }
+                        key[i] = canon128(s_words, rb + (uint64_t)r, a.k);
+                        slot[i] = slot_of(key[i].lo, key[i].hi) & a.mask;
                     }
+                    pending |= 1u << i;
                 }
             }
-            // append absent k-mers, aggregated per request within the wave
-            uint64_t pending = __ballot(absent);
-            while (pending) {
-                const int leader = __ffsll((long long)pending) - 1;
-                const uint32_t rl = __shfl(req, leader, 64);
-                const uint64_t same = __ballot(absent && req == rl);
+        }
+        while (pending) {
+            unsigned long long cur[ZI_WPT];
+#pragma unroll
+            for (int i = 0; i < ZI_WPT; i++)
+                if (pending & (1u << i)) cur[i] = WIDE ? a.thi[slot[i]] : a.tlo[slot[i]];
+#pragma unroll
+            for (int i = 0; i < ZI_WPT; i++) {
+                if (!(pending & (1u << i))) continue;
+                if (cur[i] == EMPTY_KEY) {
+                    absent |= 1u << i;
+                    pending &= ~(1u << i);
+                } else if (cur[i] == (WIDE ? key[i].hi : key[i].lo) && (!WIDE || a.tlo[slot[i]] == key[i].lo)) {
+                    pending &= ~(1u << i);
+                } else {
+                    slot[i] = (slot[i] + 1) & a.mask;
+                }
+            }
+        }
+        // append absent k-mers, aggregated per request within the wave
+#pragma unroll
+        for (int i = 0; i < ZI_WPT; i++) {
+            const bool ab = (absent >> i) & 1u;
+            uint64_t todo = __ballot(ab);
+            while (todo) {
+                const int leader = __ffsll((long long)todo) - 1;
+                const uint32_t rl = __shfl(req[i], leader, 64);
+                const uint64_t same = __ballot(ab && req[i] == rl);
                 unsigned long long base = 0;
                 if (lane == leader) base = atomicAdd(&a.seg_cursor[rl], (unsigned long long)__popcll(same));
                 base = __shfl(base, leader, 64);
-                if (absent && req == rl) {
+                if (ab && req[i] == rl) {
                     const unsigned long long pos = base + __popcll(same & ((1ull << lane) - 1ull));
-                    a.out_lo[pos] = key.lo;
-                    if constexpr (WIDE) a.out_hi[pos] = key.hi;
+                    a.out_lo[pos] = key[i].lo;
+                    if constexpr (WIDE) a.out_hi[pos] = key[i].hi;
                 }
-                pending &= ~same;
+                todo &= ~same;
             }
         }
     }
-    if (my_lookups) atomicAdd(&a.ctr[0], my_lookups);
+    __shared__ unsigned long long s_red[ZI_THREADS / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) my_lookups += __shfl_xor(my_lookups, o, 64);
+    if (lane == 0) s_red[threadIdx.x >> 6] = my_lookups;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        for (int i = 0; i < ZI_THREADS / 64; i++) tot += s_red[i];
+        if (tot) atomicAdd(&a.ctr[0], tot);
+    }
 }
 
 __global__ void k_zi_fill(unsigned long long *p, uint64_t n, unsigned long long v) {
@@ -369,7 +397,7 @@ void zi_build_index(ZiIndex &idx, HostReads &&reads, uint32_t k, uint32_t m, hip
 
 struct ZiWork {
     // host copies kept alive until the batch's asynchronous copies are done
-    HostReads q;
+    std::vector<uint64_t> h_off;
     std::vector<uint32_t> frag_req;
     std::vector<uint64_t> t_start;
     std::vector<uint32_t> t_frag, t_len;
@@ -397,20 +425,20 @@ uint64_t zi_batch_bytes(uint64_t bases, uint64_t frags, bool wide) {
     return bases / 4 + 256 + frags * 12 + tiles * 16 + bases * per_key + bases * 8 + (64u << 10);
 }
 
-void zi_enqueue(ZiWork &w, const ZiIndex &idx, HostReads &&q_in, std::vector<uint32_t> &&frag_req_in,
-                uint32_t n_req, hipStream_t s) {
+void zi_enqueue(ZiWork &w, const ZiIndex &idx, ZiBatchInput &&in, uint32_t n_req, hipStream_t s) {
     const bool wide = idx.wide;
-    w.q = std::move(q_in);
-    w.frag_req = std::move(frag_req_in);
-    const HostReads &q = w.q;
+    w.h_off = std::move(in.off);
+    w.frag_req = std::move(in.frag_req);
+    const std::vector<uint64_t> &qoff = w.h_off;
     const std::vector<uint32_t> &frag_req = w.frag_req;
-    const uint64_t nfr = q.n_frag();
+    const uint64_t nfr = qoff.size() - 1;
+    require(frag_req.size() == nfr, MCZI_E_INVALID, "fragment / request tags differ in length");
     require(nfr < (1ull << 32), MCZI_E_INVALID, "too many query fragments in one batch");
     // per-request segment bases: capacity = the request's window count
     std::vector<uint64_t> seg_cap(n_req, 0);
     for (uint64_t f = 0; f < nfr; f++) {
         const uint32_t r = frag_req[f];
-        const uint64_t n = q.off[f + 1] - q.off[f];
+        const uint64_t n = qoff[f + 1] - qoff[f];
         if (r != NO_REQ && n >= idx.k) seg_cap[r] += n - idx.k + 1;
     }
     w.h_base.assign(n_req + 1, 0);
@@ -418,15 +446,23 @@ void zi_enqueue(ZiWork &w, const ZiIndex &idx, HostReads &&q_in, std::vector<uin
     const uint64_t cap = std::max<uint64_t>(w.h_base[n_req], 1);
     require(cap < (1ull << 32) - 1, MCZI_E_INVALID, "too many query k-mers in one batch (limit 2^32 - 2)");
     w.cap = cap;
-    w.words.ensure(q.words.size() * 4);
-    w.off.ensure(q.off.size() * 8);
+    // words of the batch + slack read past the last base (extraction and
+    // tile staging); gap and slack contents never reach a key
+    require(in.n_words * 16 >= qoff.back(), MCZI_E_INVALID, "query batch words shorter than its fragments");
+    w.words.ensure((in.n_words + 48) * 4);
+    w.off.ensure(qoff.size() * 8);
     w.req.ensure(std::max<uint64_t>(nfr, 1) * 4);
-    MCZI_HIP(hipMemcpyAsync(w.words.as<void>(), q.words.data(), q.words.size() * 4, hipMemcpyHostToDevice, s));
-    MCZI_HIP(hipMemcpyAsync(w.off.as<void>(), q.off.data(), q.off.size() * 8, hipMemcpyHostToDevice, s));
+    for (const ZiWordSpan &sp : in.spans) {
+        require(sp.dst_word + sp.n_words <= in.n_words, MCZI_E_INVALID, "query word span outside the batch");
+        if (sp.n_words)
+            MCZI_HIP(hipMemcpyAsync(w.words.as<uint32_t>() + sp.dst_word, sp.src, sp.n_words * 4,
+                                    hipMemcpyHostToDevice, s));
+    }
+    MCZI_HIP(hipMemcpyAsync(w.off.as<void>(), qoff.data(), qoff.size() * 8, hipMemcpyHostToDevice, s));
     if (nfr)
         MCZI_HIP(hipMemcpyAsync(w.req.as<void>(), frag_req.data(), nfr * 4, hipMemcpyHostToDevice, s));
     HostTiles t;
-    make_tiles(q.off, t);
+    make_tiles(qoff, t);
     w.t_start = t.start;
     w.t_frag = t.frag;
     w.t_len = t.len;
@@ -553,7 +589,9 @@ void zi_enqueue(ZiWork &w, const ZiIndex &idx, HostReads &&q_in, std::vector<uin
 }
 
 void zi_collect(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, ZiQueryResult &out) {
+    const auto tw = std::chrono::steady_clock::now();
     MCZI_HIP(hipEventSynchronize(w.done));
+    out.wait_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tw).count();
     MCZI_HIP(hipGetLastError());
     w.h_count.assign(n_req + 1, 0);
     std::vector<uint64_t> h_start(n_req + 1, 0);
@@ -571,8 +609,8 @@ void zi_collect(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, Zi
         const uint64_t n = w.h_count[r];
         ZiKeys &kk = out.absent[r];
         kk.n = n;
-        kk.lo = (uint64_t *)malloc(std::max<uint64_t>(n, 1) * 8);
-        if (idx.wide) kk.hi = (uint64_t *)malloc(std::max<uint64_t>(n, 1) * 8);
+        kk.lo = (uint64_t *)Device::get().host_alloc(std::max<uint64_t>(n, 1) * 8);
+        if (idx.wide) kk.hi = (uint64_t *)Device::get().host_alloc(std::max<uint64_t>(n, 1) * 8);
         require(kk.lo && (!idx.wide || kk.hi), MCZI_E_NOMEM, "out of host memory for ZI results");
         if (n)
             MCZI_HIP(hipMemcpyAsync(kk.lo, w.ulo.as<unsigned long long>() + h_start[r], n * 8,
@@ -588,7 +626,15 @@ void zi_collect(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, Zi
 // Serving loop
 struct ZiRequest {
     uint64_t id = 0;
-    HostReads reads;
+    HostReads reads;        // fragment offsets (the words live in `pin`)
+    uint32_t *pin = nullptr;  // packed words in pinned host memory
+    ~ZiRequest() {
+        if (pin) Device::get().host_release(pin);
+    }
+    void pin_words(const uint32_t *src, uint64_t n_words) {
+        pin = (uint32_t *)Device::get().host_alloc(std::max<uint64_t>(n_words, 1) * 4);
+        if (n_words) memcpy(pin, src, n_words * 4);
+    }
     bool make_fasta = false;
     uint64_t bytes = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
@@ -632,6 +678,7 @@ struct mczi_zi_server {
     std::thread admit_thread, done_thread;
     uint64_t completed = 0, lookups = 0;
     double busy_ms = 0;
+    const bool trace = getenv("MCZI_ZI_TRACE") != nullptr;  // per-batch host timings on stderr
     std::string fatal;
 
     void admit_loop();
@@ -692,39 +739,44 @@ void mczi_zi_server::admit_loop() {
         const double t = now_ms();
         for (auto *r : batch) r->t_admit = t;
         lk.unlock();
-        // concatenate requests at word boundaries (padding = fragment of no request)
-        HostReads q;
-        std::vector<uint32_t> frag_req;
+        const double ta0 = now_ms();
+        // lay the requests out at word boundaries (a gap before a request is a
+        // fragment of no request); each request's pinned words are copied
+        // straight to its place in the device buffer
+        ZiBatchInput in;
         for (uint32_t ri = 0; ri < batch.size(); ri++) {
-            HostReads &src = batch[ri]->reads;
-            uint64_t base = q.cur_;
+            const ZiRequest &rq = *batch[ri];
+            uint64_t base = in.off.back();
             if (base % 16) {
-                const uint64_t pad = 16 - base % 16;
-                q.cur_ += pad;
-                q.words.resize(q.cur_ / 16, 0);
-                q.off.push_back(q.cur_);
-                frag_req.push_back(NO_REQ);
-                base = q.cur_;
+                base += 16 - base % 16;
+                in.off.push_back(base);
+                in.frag_req.push_back(NO_REQ);
             }
-            const uint64_t nw = (src.n_bases() + 15) / 16;
-            q.words.resize(base / 16 + nw, 0);
-            std::copy(src.words.begin(), src.words.begin() + nw, q.words.begin() + base / 16);
-            for (uint64_t f = 0; f < src.n_frag(); f++) {
-                q.off.push_back(base + src.off[f + 1]);
-                frag_req.push_back(ri);
+            if (rq.reads.off[0] > 0) {  // bases before the first fragment belong to none
+                in.off.push_back(base + rq.reads.off[0]);
+                in.frag_req.push_back(NO_REQ);
             }
-            q.cur_ = base + src.n_bases();
+            for (uint64_t f = 0; f < rq.reads.n_frag(); f++) {
+                in.off.push_back(base + rq.reads.off[f + 1]);
+                in.frag_req.push_back(ri);
+            }
+            const uint64_t nw = (rq.reads.n_bases() + 15) / 16;
+            in.spans.push_back(ZiWordSpan{rq.pin, nw, base / 16});
+            in.n_words = base / 16 + nw;
         }
-        q.words.resize(q.cur_ / 16 + 40, 0);
         int err = 0;
         std::string msg;
+        const double ta1 = now_ms();
         try {
             if (!sl.work) sl.work = zi_make_work();
-            zi_enqueue(*sl.work, idx, std::move(q), std::move(frag_req), (uint32_t)batch.size(), sl.stream);
+            zi_enqueue(*sl.work, idx, std::move(in), (uint32_t)batch.size(), sl.stream);
         } catch (const Error &e) {
             err = e.code;
             msg = e.what();
         }
+        if (trace)
+            fprintf(stderr, "ZI_TRACE admit slot=%d reqs=%zu bases=%llu concat_ms=%.2f enqueue_ms=%.2f\n", free_slot,
+                    batch.size(), (unsigned long long)bases, ta1 - ta0, now_ms() - ta1);
         lk.lock();
         if (err) {
             for (auto *r : batch) {
@@ -774,6 +826,9 @@ void mczi_zi_server::done_loop() {
             batch[i]->windows = res.windows[i];
             if (batch[i]->make_fasta) batch[i]->fasta = simplitig_fasta_text(batch[i]->absent.counted(), idx.k);
         }
+        if (trace)
+            fprintf(stderr, "ZI_TRACE done reqs=%zu wait_ms=%.2f copy_ms=%.2f post_ms=%.2f\n", batch.size(),
+                    res.wait_ms, t1 - t0 - res.wait_ms, now_ms() - t1);
         lk.lock();
         busy_ms += t1 - t0;
         lookups += sl->work ? sl->work->lookups : 0;
@@ -835,9 +890,13 @@ std::vector<mczi_counted_kmer> zi_run(const char *index_path, const char *const 
     HostReads q;
     for (size_t i = 0; i < n; i++) parse_fastx_file(qp[i], k, 0, q);
     q.finish();
-    std::vector<uint32_t> fr(q.n_frag(), 0);
+    ZiBatchInput in;
+    in.off = q.off;
+    in.frag_req.assign(q.n_frag(), 0);
+    in.n_words = (q.n_bases() + 15) / 16;
+    in.spans.push_back(ZiWordSpan{q.words.data(), in.n_words, 0});
     auto w = zi_make_work();
-    zi_enqueue(*w, idx, std::move(q), std::move(fr), 1, d.stream());
+    zi_enqueue(*w, idx, std::move(in), 1, d.stream());
     ZiQueryResult res;
     zi_collect(*w, idx, 1, d.stream(), res);
     return res.absent[0].counted();
@@ -949,6 +1008,8 @@ int mczi_zi_server_submit(mczi_zi_server *s, const char *fastx, size_t len, int 
         r->t_submit = now_ms();
         parse_fastx_text(fastx, len, s->idx.k, 0, r->reads);
         r->reads.finish();
+        r->pin_words(r->reads.words.data(), (r->reads.n_bases() + 15) / 16);
+        std::vector<uint32_t>().swap(r->reads.words);
         r->make_fasta = make_fasta != 0;
         std::lock_guard<std::mutex> lk(s->mu);
         require(s->fatal.empty(), MCZI_E_DEVICE, s->fatal);
@@ -969,13 +1030,12 @@ int mczi_zi_server_submit_packed(mczi_zi_server *s, const uint32_t *words, const
         auto r = std::make_unique<ZiRequest>();
         r->t_submit = now_ms();
         HostReads &h = r->reads;
-        const uint64_t nb = frag_off[n_frag];
-        h.words.assign(words, words + (nb + 15) / 16);
-        h.cur_ = nb;
-        h.off.assign(frag_off, frag_off + n_frag + 1);
         for (size_t i = 0; i < n_frag; i++)
             require(frag_off[i + 1] >= frag_off[i], MCZI_E_INVALID, "fragment offsets must not decrease");
-        h.finish();
+        const uint64_t nb = frag_off[n_frag];
+        r->pin_words(words, (nb + 15) / 16);
+        h.cur_ = nb;
+        h.off.assign(frag_off, frag_off + n_frag + 1);
         r->make_fasta = make_fasta != 0;
         std::lock_guard<std::mutex> lk(s->mu);
         require(s->fatal.empty(), MCZI_E_DEVICE, s->fatal);
@@ -1035,8 +1095,11 @@ int mczi_zi_server_poll(mczi_zi_server *s, uint64_t request_id, int wait, mczi_z
 
 void mczi_zi_result_free(mczi_zi_result *r) {
     if (!r) return;
-    free(r->absent_lo);
-    free(r->absent_hi);
+    if (r->absent_lo || r->absent_hi) {
+        Device &d = Device::get();
+        d.host_release(r->absent_lo);
+        d.host_release(r->absent_hi);
+    }
     free(r->fasta);
     free(r);
 }
