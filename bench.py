@@ -6,9 +6,9 @@ of 100 Mbp (seed 2) plus 2% of it duplicated as 5 kbp repeats at 10 copies
 each, 150 bp reads at 30x (20M reads, 3 Gbp), 1% substitutions, random
 strand; k=31, m=21, x=5, partition count 1024, FASTA-path (two-pass) counting.
 A step = one pass of the counting path over the whole read set, which is
-resident in HBM before timing starts: scan + pass-1 minimizer counts ->
-super-k-mer partitioning -> LDS counting with HBM spill -> count > x filter
--> sort by encoded k-mer. Prints one JSON line.
+resident in HBM before timing starts: minimizer scan + super-k-mer records
+-> partitioning (radix sort by partition) -> LDS counting with HBM spill ->
+count > x filter -> sort by encoded k-mer. Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -31,6 +31,19 @@ WORKLOADS = {
     "c1_small_1mbp": (dict(genome_len=1_000_000, genome_seed=1, read_seed=101, n_reads=200_000, read_len=150,
                            sub_rate=0.005), 31, 21, 1, 64, 1),
 }
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by the factor a plain
+    copy kernel of known size reads on the same counter), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d[workload]["kernels"][kernel]["hbm_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def decode_prefix(reads, n_frag):
@@ -203,7 +216,9 @@ def main() -> int:
     reads = M.SynthReads(M.SynthSpec(**spec_kw))
     print(f"[bench] generated {reads.n_bases} bases in {len(reads.frag_off) - 1} reads "
           f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    t_up = time.perf_counter()
     batch = M.Batch.from_packed(reads.words, reads.frag_off)
+    upload_s = time.perf_counter() - t_up  # host -> HBM copy of the packed reads (pageable source)
     cfg = M.CounterConfig(k, m, x, parts)
     order = M.ORDER_ENCODED if mode == 0 else M.ORDER_KFF_U8
 
@@ -242,19 +257,21 @@ def main() -> int:
 
     nb = reads.n_bases
     R = prof["n_records"]
+    P = prof["internal_partitions"]
     out_bytes = prof["n_out"] * 16
-    # algorithmic (compulsory) bytes per kernel group, one pass each:
-    #  pass 1 scan: packed reads in, staged records (16 B) + minimizer keys (4 B) out
-    #  partitioning: staged records + keys in, partition-ordered records + keys out
-    #  counting: partitioned records + keys in, (k-mer, count, partition) out
+    # algorithmic (compulsory) bytes per kernel group, each operand once:
+    #  k_scan: packed reads in (2 bits/base); records (16 B) + partition ids (4 B) out
+    #  partition sort: (partition id, record) pairs in and out once
+    #  k_count: partitioned records + partition bounds in; (k-mer, count, partition) out
     kernels = {
         "k_scan": (nb / 4 + 20 * R, prof["ms_scan1"]),
-        "partition_sort_gather": (40 * R, prof["ms_scan2"]),
-        "k_count": (20 * R + out_bytes, prof["ms_count"]),
+        "partition_sort": (40 * R, prof["ms_scan2"]),
+        "k_count": (16 * R + 16 * P + out_bytes, prof["ms_count"]),
     }
     dom = max(kernels, key=lambda n: kernels[n][1])
     alg, ms = kernels[dom]
     achieved = alg / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    traffic = measured_traffic(args.workload, dom)
 
     line = {
         "metric": "MC bases/s + ZI k-mer lookups/s, p99; batched vs per-file GPU, bit-exact",
@@ -274,6 +291,8 @@ def main() -> int:
                    "path": "two-pass FASTA counting" if mode == 0 else "KMC-style KFF counting",
                    "launch": "batched (persistent kernels, hipGraph)"},
         "device_ms_per_step": sum(dev_ms) / len(dev_ms),
+        "upload_ms": upload_s * 1e3,
+        "pcie_inclusive_value": nb / (ms_per_step * 1e-3 + upload_s),
         "plain_value": plain_value,
         "batched_vs_plain": (value / plain_value) if plain_value else None,
         "phase_ms": {k2: prof[k2] for k2 in ("ms_scan1", "ms_scan2", "ms_count", "ms_sort", "ms_total")},
@@ -282,7 +301,7 @@ def main() -> int:
         "device": M.device_info(),
         "copy_probe": {"bytes": copy_bytes, "ms": copy_ms, "read_plus_write_GBs": copy_gbs},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": alg, "kernel_ms": ms},
         "cpu_baseline": None,
     }

@@ -210,7 +210,9 @@ int mczi_zi_absent(const char *index_path, const char *const *query_paths, size_
 /* ---- ZI serving: a device-resident index and a continuous-batching loop ----
  * No counterpart in the reference (which runs one query set per process).
  * Requests are FASTA/FASTQ text (or packed reads); each result is the sorted
- * absent set and, on request, its simplitig FASTA text. */
+ * absent set and, on request, its simplitig FASTA text. The absent arrays
+ * are pinned host memory owned by the library: release the whole result
+ * with mczi_zi_result_free (never free()). */
 typedef struct mczi_zi_server mczi_zi_server;
 typedef struct {
     uint64_t request_id;
