@@ -403,33 +403,25 @@ int mczi_batch_from_packed(const uint32_t *words, const uint64_t *frag_off, size
     return guarded("failed to load batch", [&] {
         require(out != nullptr && frag_off != nullptr, MCZI_E_INVALID, "null argument");
         std::lock_guard<std::mutex> lk(Device::get().call_mutex());
-        HostReads r;
-        const uint64_t nb = frag_off[n_frag];
-        r.words.assign(words, words + (nb + 15) / 16);
-        r.cur_ = nb;
-        r.off.assign(frag_off, frag_off + n_frag + 1);
-        uint32_t nd = 1;
-        if (dataset) {
-            r.dataset.assign(dataset, dataset + n_frag);
-            for (size_t i = 0; i < n_frag; i++) nd = std::max(nd, dataset[i] + 1);
-        } else {
-            r.dataset.assign(n_frag, 0);
-        }
+        require(words != nullptr || frag_off[n_frag] == 0, MCZI_E_INVALID, "null argument");
         for (size_t i = 0; i < n_frag; i++)
             require(frag_off[i + 1] >= frag_off[i], MCZI_E_INVALID, "fragment offsets must not decrease");
-        r.n_datasets = nd;
+        uint32_t nd = 1;
+        if (dataset)
+            for (size_t i = 0; i < n_frag; i++) nd = std::max(nd, dataset[i] + 1);
         // input chunks of >= 128 Mi bases (BATCH_BASES, src/lib.rs:28) for the plain launch mode
+        std::vector<uint64_t> chunk_frag{0};
         uint64_t acc = 0;
         for (size_t i = 0; i < n_frag; i++) {
             acc += frag_off[i + 1] - frag_off[i];
             if (acc >= (128ull << 20) && i + 1 < n_frag) {
-                r.chunk_frag.push_back(i + 1);
+                chunk_frag.push_back(i + 1);
                 acc = 0;
             }
         }
-        auto h = new mczi_batch();
-        h->b = make_batch(std::move(r));
-        *out = h;
+        auto h = std::make_unique<mczi_batch>();
+        h->b = make_batch_packed(words, frag_off, n_frag, dataset, nd, std::move(chunk_frag));
+        *out = h.release();
     });
 }
 

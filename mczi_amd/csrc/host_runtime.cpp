@@ -9,6 +9,8 @@
 #include <cstring>
 #include <string>
 
+#include <thread>
+
 #include "internal.hpp"
 
 namespace mczi {
@@ -111,6 +113,54 @@ void *Device::host_alloc(size_t bytes) {
     std::lock_guard<std::mutex> lk(hmu_);
     hlive_[p] = want;
     return p;
+}
+
+void Device::upload(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return;
+    constexpr size_t kChunk = 64u << 20;
+    constexpr int kThreads = 8;
+    if (bytes <= (4u << 20)) {  // small: the runtime's own staging is fine
+        MCZI_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+        MCZI_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    char *stage[2] = {(char *)host_alloc(kChunk), (char *)host_alloc(kChunk)};
+    hipEvent_t ev[2];
+    MCZI_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    MCZI_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    bool used[2] = {false, false};
+    try {
+        for (size_t off = 0, i = 0; off < bytes; off += kChunk, i++) {
+            const int b = (int)(i & 1);
+            const size_t n = std::min(kChunk, bytes - off);
+            if (used[b]) MCZI_HIP(hipEventSynchronize(ev[b]));
+            const char *from = (const char *)src + off;
+            std::vector<std::thread> th;
+            const size_t per = (n + kThreads - 1) / kThreads;
+            for (int t = 0; t < kThreads; t++) {
+                const size_t a0 = (size_t)t * per;
+                if (a0 >= n) break;
+                const size_t len = std::min(per, n - a0);
+                th.emplace_back([=] { memcpy(stage[b] + a0, from + a0, len); });
+            }
+            for (auto &x : th) x.join();
+            MCZI_HIP(hipMemcpyAsync((char *)dst + off, stage[b], n, hipMemcpyHostToDevice, s));
+            MCZI_HIP(hipEventRecord(ev[b], s));
+            used[b] = true;
+        }
+        MCZI_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        host_release(stage[0]);
+        host_release(stage[1]);
+        throw;
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    host_release(stage[0]);
+    host_release(stage[1]);
 }
 
 void Device::host_release(void *p) {

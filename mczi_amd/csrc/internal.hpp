@@ -50,6 +50,10 @@ class Device {
     // blocks stay cached up to kPinnedCacheBytes.
     void *host_alloc(size_t bytes);
     void host_release(void *p);
+    // Host -> device copy of pageable memory: the source is copied by
+    // several host threads into two alternating pinned staging blocks, each
+    // sent by DMA while the next one fills. Synchronous on `s`.
+    void upload(void *dst, const void *src, size_t bytes, hipStream_t s);
     std::mutex &call_mutex() { return call_mu_; }
 
   private:
@@ -164,6 +168,9 @@ struct Batch {
 };
 
 std::unique_ptr<Batch> make_batch(HostReads &&reads);
+std::unique_ptr<Batch> make_batch_packed(const uint32_t *words, const uint64_t *frag_off, size_t n_frag,
+                                         const uint32_t *dataset, uint32_t n_datasets,
+                                         std::vector<uint64_t> &&chunk_frag);
 void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch, int order,
                  mczi_run_stats *stats);
 void batch_download(Batch &b, std::vector<mczi_counted_kmer> &out);

@@ -897,6 +897,37 @@ std::unique_ptr<Batch> make_batch(HostReads &&r) {
     return b;
 }
 
+// Batch from caller-owned packed words (no intermediate host copy of the
+// words): uploads through the device's pinned staging, 32 zero words of
+// slack after the data.
+std::unique_ptr<Batch> make_batch_packed(const uint32_t *words, const uint64_t *frag_off, size_t n_frag,
+                                         const uint32_t *dataset, uint32_t n_datasets,
+                                         std::vector<uint64_t> &&chunk_frag) {
+    auto b = std::make_unique<Batch>();
+    Device &d = Device::get();
+    const uint64_t nb = frag_off[n_frag];
+    const uint64_t nw = (nb + 15) / 16;
+    b->n_bases = nb;
+    b->n_frag = n_frag;
+    b->n_words = nw + 32;
+    b->chunk_frag = std::move(chunk_frag);
+    if (b->chunk_frag.back() != n_frag) b->chunk_frag.push_back(n_frag);
+    b->host_off.assign(frag_off, frag_off + n_frag + 1);
+    b->n_datasets = n_datasets;
+    b->words.reset(b->n_words * 4);
+    b->off.reset((n_frag + 1) * 8);
+    fill_u32(b->words.as<uint32_t>() + nw, 32, 0u, d.stream());
+    MCZI_HIP(hipStreamSynchronize(d.stream()));
+    d.upload(b->words.as<void>(), words, nw * 4, d.stream());
+    d.upload(b->off.as<void>(), frag_off, (n_frag + 1) * 8, d.stream());
+    if (dataset) {
+        b->has_dataset = true;
+        b->dataset.reset(std::max<size_t>(n_frag, 1) * 4);
+        d.upload(b->dataset.as<void>(), dataset, n_frag * 4, d.stream());
+    }
+    return b;
+}
+
 // Tile schedule: tiles of TILE positions, clipped at chunk ends, each with
 // the fragment holding its first position.
 static void build_tiles(Batch &b, McPlan &p, uint32_t k) {
