@@ -638,24 +638,31 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
                 const unsigned long long key = canon_from_mc(lo & kmask, kmask, rshift);
                 const uint32_t val = (MODE == 0) ? 1u : (1u << (uint32_t)((ry >> 56) & 31u));
                 uint32_t slot = lds_slot(key);
+                // Home slot: a plain load settles the common case (key present,
+                // or empty slot claimed by one CAS); later probes use the CAS
+                // alone, which returns whatever key holds the slot.
                 bool done = false;
-                for (int p = 0; p < CNT_MAXPROBE; p++) {
-                    const unsigned long long cur = __hip_atomic_load(
-                        &s_keys[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (cur == key) {
+                {
+                    const unsigned long long cur =
+                        __hip_atomic_load(&s_keys[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    unsigned long long prev = cur;
+                    if (cur == EMPTY_KEY) prev = atomicCAS(&s_keys[slot], EMPTY_KEY, key);
+                    if (prev == key || prev == EMPTY_KEY) {
                         val_update<MODE>(&s_vals[slot], val);
                         done = true;
-                        break;
                     }
-                    if (cur == EMPTY_KEY) {
+                }
+                if (!done) {
+                    slot = (slot + 1) & (CNT_SLOTS - 1);
+                    for (int p = 1; p < CNT_MAXPROBE; p++) {
                         const unsigned long long prev = atomicCAS(&s_keys[slot], EMPTY_KEY, key);
                         if (prev == EMPTY_KEY || prev == key) {
                             val_update<MODE>(&s_vals[slot], val);
                             done = true;
                             break;
                         }
+                        slot = (slot + 1) & (CNT_SLOTS - 1);
                     }
-                    slot = (slot + 1) & (CNT_SLOTS - 1);
                 }
                 if (!done) overflow_insert<MODE>(a, key, val, q);
             }
