@@ -9,6 +9,7 @@
 // (count + filter). fof: 637-703, 4253-4370, 2184-2247.
 #include <rocprim/block/block_scan.hpp>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -26,6 +27,7 @@ constexpr int SCAN_THREADS = 256;
 constexpr int WPT = 8;                        // windows (and hash positions) per thread
 constexpr int SPAN = SCAN_THREADS * WPT;      // hash / prefix positions per tile
 static_assert(WPT == 8, "the padded LDS indexing assumes 8 positions per thread");
+static_assert(SCAN_THREADS * WPT <= 2048, "piece starts are packed in 11 bits");
 constexpr int MAXK = 64;
 constexpr int TILE = SPAN - 2 * MAXK;         // windows per tile: tl + k - 1 < SPAN
 constexpr int MAXW = 64;                      // w = k - m + 1 <= 64
@@ -61,6 +63,7 @@ struct ScanArgs {
     uint32_t x;
     uint32_t sat;
     uint32_t p_int;
+    uint32_t p_mask;  // p_int - 1 when p_int is a power of two, else 0
     uint32_t wrec;
     uint32_t *win_out;
     unsigned long long *stats;  // [0] pieces [1] records
@@ -68,7 +71,9 @@ struct ScanArgs {
     ulonglong2 *stage_rec;
     uint32_t *stage_q;
     uint64_t stage_cap;
-    unsigned long long *stage_ctr;
+    unsigned long long *stage_ctr;  // total records (one add per block)
+    uint32_t *tile_nrec;            // count-only pass: records of each tile
+    const unsigned long long *tile_base;  // staging pass: first record slot of each tile
     unsigned long long *err;
 };
 
@@ -144,8 +149,10 @@ __device__ __forceinline__ uint32_t extract16_bases(const uint32_t *sw, int pos)
 // (sm/sliding_min.rs:279-340, sm/canonical.rs:11-29, sm/minimizers.rs:213-216),
 // then runs of equal selected hash (sm/collect.rs:38-75) cut at fragment and
 // tile boundaries ("pieces"). STAGE writes the pieces as super-k-mer records,
-// WINDOWS the selected hash of every window.
-template <int MODE>
+// WINDOWS the selected hash of every window. GW selects the window-minimum
+// scheme (w >= WPT: Gil-Werman over the thread's windows; else a direct scan
+// of the w hashes). STAGE serves the counting path, which has k <= 32.
+template <int MODE, bool GW>
 __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_scan(ScanArgs a) {
     __shared__ uint32_t s_words[WORDS_SPAN];
     __shared__ uint32_t s_hash[PADDED(TILE + MAXW)];
@@ -169,7 +176,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
 
     const int tid = threadIdx.x;
     const int k = a.k, m = a.m, w = a.w;
-    unsigned long long my_pieces = 0;
+    unsigned long long my_pieces = 0, my_records = 0;
     // each thread owns positions r0 .. r0 + WPT - 1 of a tile; P8(r0 + c),
     // the padded LDS index, is 9 * tid + c + floor(c / 8) since r0 = 8 * tid
     const int r0 = tid * WPT;
@@ -181,6 +188,8 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
         const uint64_t f0 = a.tile_frag[t];
         const int rb0 = (int)(ts & 15u);
         const int nwords = ((rb0 + tl + 160) >> 4) + 3;
+        unsigned long long tbase = 0;
+        if (MODE == STAGE && a.stage_rec != nullptr && tid == 0) tbase = a.tile_base[t];
         __syncthreads();  // previous tile fully consumed
         for (int i = tid; i < WORDS_SPAN; i += SCAN_THREADS) s_words[i] = i < nwords ? a.words[(ts >> 4) + i] : 0u;
         // Fragment boundaries relative to ts: s_fs[j] = off[f0 + j] - ts (clamped),
@@ -256,6 +265,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
         // 279-340). With w >= WPT the thread's WPT windows share the middle
         // hashes [WPT-1, w-1]; suffix minima on the left and prefix minima on
         // the right complete each window (van Herk / Gil-Werman).
+        int jfirst = 0;  // fragment holding the thread's first window
         if (r0 < tl) {
             int lo = 0, hi = nf - 1;
             while (lo < hi) {
@@ -263,6 +273,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
                 if (s_fs[mid] <= r0) lo = mid;
                 else hi = mid - 1;
             }
+            jfirst = lo;
             int j = lo;
             int fcur = s_fs[j], fnext = s_fs[j + 1];  // bounds of the fragment holding r
             const int rb = rb0 + r0;
@@ -270,14 +281,15 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             // strand vote of window i counts them over bases [i, i + k)
             const uint64_t H0 = extract32_bases(s_words, (uint64_t)rb) & 0xAAAAAAAAAAAAAAAAull;
             const uint64_t H1 = extract32_bases(s_words, (uint64_t)rb + 32) & 0xAAAAAAAAAAAAAAAAull;
-            const uint64_t H2 = k > 32 ? extract32_bases(s_words, (uint64_t)rb + 64) & 0xAAAAAAAAAAAAAAAAull : 0ull;
+            constexpr bool KBIG = MODE == WINDOWS;  // k may exceed 32
+            const uint64_t H2 =
+                KBIG && k > 32 ? extract32_bases(s_words, (uint64_t)rb + 64) & 0xAAAAAAAAAAAAAAAAull : 0ull;
             const uint64_t mlo = high_bit_mask(k < 32 ? k : 32);
-            const uint64_t mhi = k > 32 ? high_bit_mask(k - 32) : 0ull;
+            const uint64_t mhi = KBIG && k > 32 ? high_bit_mask(k - 32) : 0ull;
             // suffix minima over [i, w-1] (the shared middle folded in); prefix
             // minima over [w, w+i-1] are kept running in the window loop
             uint32_t sufL[WPT], sufR[WPT], preL = ~0u, preR = ~0u;
-            const bool gw = w >= WPT;
-            if (gw) {
+            if constexpr (GW) {
                 uint32_t midL = ~0u, midR = ~0u;
                 for (int t = WPT - 1; t <= w - 1; t++) {
                     const uint32_t h = s_hash[px(t)] & 0xffff0000u;
@@ -296,7 +308,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
 #pragma unroll
             for (int i = 0; i < WPT; i++) {
                 const int r = r0 + i;
-                if (i > 0 && gw) {
+                if (GW && i > 0) {
                     const int t = w + i - 1;
                     const uint32_t h = s_hash[px(t)] & 0xffff0000u;
                     preL = min(preL, h | (uint32_t)t);
@@ -314,12 +326,12 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
                 if (valid) {
                     const uint64_t lo64 = i ? ((H0 >> (2 * i)) | (H1 << (64 - 2 * i))) : H0;
                     uint32_t tg = (uint32_t)__popcll(lo64 & mlo);
-                    if (k > 32) {
+                    if (KBIG && k > 32) {
                         const uint64_t hi64 = i ? ((H1 >> (2 * i)) | (H2 << (64 - 2 * i))) : H1;
                         tg += (uint32_t)__popcll(hi64 & mhi);
                     }
                     int lp, rp;
-                    if (gw) {
+                    if constexpr (GW) {
                         lp = (int)(min(sufL[i], preL) & 0xffffu);
                         rp = (int)(0xffffu - (min(sufR[i], preR) & 0xffffu));
                     } else {
@@ -349,7 +361,9 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             continue;
         }
 
-        // piece starts -> compacted list
+        // piece starts -> compacted list of (start | fragment << 11), the
+        // fragment index inside the tile walked forward from the thread's
+        // first fragment
         uint32_t smask = 0;
         int cnt = 0;
         for (int i = 0; i < WPT; i++) {
@@ -363,8 +377,17 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
         }
         int np = 0;
         int o = block_exclusive_scan(cnt, &np, s_wsum);
-        for (int i = 0; i < WPT; i++)
-            if (smask & (1u << i)) s_u.run.pst[o++] = (uint32_t)(r0 + i);
+        if (smask) {
+            int j = jfirst, fnx = s_fs[jfirst + 1];
+            for (int i = 0; i < WPT; i++) {
+                const int r = r0 + i;
+                while (j + 1 < nf && fnx <= r) {
+                    j++;
+                    fnx = s_fs[j + 1];
+                }
+                if (smask & (1u << i)) s_u.run.pst[o++] = (uint32_t)r | ((uint32_t)j << 11);
+            }
+        }
         __syncthreads();
 
         // Pieces -> super-k-mer records of at most wrec windows (bases
@@ -377,38 +400,37 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
         // 3192-3197), base j of the record at bits 2j.
         int my_nrec = 0;
         for (int pi = tid; pi < np; pi += SCAN_THREADS) {
-            const int s = (int)s_u.run.pst[pi];
-            int e = (pi + 1 < np) ? (int)s_u.run.pst[pi + 1] : tl;
-            int lo = 0, hi = nf - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_fs[mid] <= s) lo = mid;
-                else hi = mid - 1;
-            }
-            const int vend = s_fs[lo + 1] - k + 1;
+            const uint32_t pw = s_u.run.pst[pi];
+            const int s = (int)(pw & 2047u);
+            int e = (pi + 1 < np) ? (int)(s_u.run.pst[pi + 1] & 2047u) : tl;
+            const int vend = s_fs[(pw >> 11) + 1] - k + 1;
             if (vend < e) e = vend;
             my_pieces++;
             my_nrec += (e - s + (int)a.wrec - 1) / (int)a.wrec;
         }
         int tot = 0;
         const int off = block_exclusive_scan(my_nrec, &tot, s_wsum);
-        if (tid == 0) s_sbase = tot ? atomicAdd(a.stage_ctr, (unsigned long long)tot) : 0ull;
+        // Output slots: the count-only pass stores each tile's record count;
+        // their exclusive scan gives the staging pass every tile's first slot
+        // (records land in tile order, no device-scope atomics per tile).
+        if (tid == 0) {
+            my_records += (unsigned long long)tot;
+            if (a.stage_rec == nullptr) a.tile_nrec[t] = (uint32_t)tot;
+            s_sbase = tbase;
+        }
         __syncthreads();
         if (a.stage_rec != nullptr) {
             uint64_t pos = s_sbase + (uint64_t)off;
             for (int pi = tid; pi < np; pi += SCAN_THREADS) {
-                const int s = (int)s_u.run.pst[pi];
-                int e = (pi + 1 < np) ? (int)s_u.run.pst[pi + 1] : tl;
-                int lo = 0, hi = nf - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (s_fs[mid] <= s) lo = mid;
-                    else hi = mid - 1;
-                }
-                const int vend = s_fs[lo + 1] - k + 1;
+                const uint32_t pw = s_u.run.pst[pi];
+                const int s = (int)(pw & 2047u);
+                const int jf = (int)(pw >> 11);
+                int e = (pi + 1 < np) ? (int)(s_u.run.pst[pi + 1] & 2047u) : tl;
+                const int vend = s_fs[jf + 1] - k + 1;
                 if (vend < e) e = vend;
-                const uint32_t q = spread_hash_u32(spread_hash_u32(s_u.run.v[P8(s)])) % a.p_int;
-                const uint64_t ds = a.dataset != nullptr ? (uint64_t)a.dataset[f0 + lo] : 0ull;
+                const uint32_t hq = spread_hash_u32(spread_hash_u32(s_u.run.v[P8(s)]));
+                const uint32_t q = a.p_mask ? (hq & a.p_mask) : hq % a.p_int;
+                const uint64_t ds = a.dataset != nullptr ? (uint64_t)a.dataset[f0 + jf] : 0ull;
                 for (int cs = s; cs < e; cs += (int)a.wrec) {
                     const int ce = min(cs + (int)a.wrec, e);
                     const uint32_t L = (uint32_t)(ce - cs + k - 1);
@@ -435,6 +457,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
     }
     __shared__ unsigned long long s_red[SCAN_THREADS / 64];
     block_sum_atomic(my_pieces, &a.stats[0], s_red);
+    if constexpr (MODE == STAGE) block_sum_atomic(my_records, a.stage_ctr, s_red);
 }
 
 // Partitioning (src/lib.rs:3483-3560 writes each super-k-mer to its
@@ -823,7 +846,7 @@ __global__ void k_gather_kmer(const unsigned long long *ks, const uint32_t *cs, 
 struct McPlan {
     uint32_t k = 0;
     // tile schedule
-    DevBuf tile_start, tile_frag, tile_len;
+    DevBuf tile_start, tile_frag, tile_len, tile_nrec, tile_base, tile_scan_tmp;
     uint64_t n_tiles = 0;
     std::vector<uint64_t> chunk_tile;  // tile index where each chunk starts
     uint64_t n_windows = 0;
@@ -1039,19 +1062,22 @@ static ScanArgs scan_args(Batch &b, McPlan &p, const mczi_counter_config &cfg) {
     sa.x = (uint32_t)cfg.threshold;
     sa.sat = (uint32_t)std::min<uint64_t>(cfg.threshold + 1, 255);
     sa.p_int = p.p_int;
+    sa.p_mask = (p.p_int & (p.p_int - 1)) == 0 ? p.p_int - 1 : 0u;
     sa.wrec = (uint32_t)(REC_MAX_BASES - k + 1);
     sa.stats = p.ctr.as<unsigned long long>() + 8;
     sa.stage_rec = p.stage_rec.as<ulonglong2>();
     sa.stage_q = p.stage_q.as<uint32_t>();
     sa.stage_cap = p.stage_cap;
     sa.stage_ctr = p.ctr.as<unsigned long long>() + 11;
+    sa.tile_nrec = p.tile_nrec.as<uint32_t>();
+    sa.tile_base = p.tile_base.as<unsigned long long>();
     sa.err = p.ctr.as<unsigned long long>() + 4;
     return sa;
 }
 
 static void launch_scan(McPlan &p, const ScanArgs &sa, int launch, hipStream_t s) {
     Device &d = Device::get();
-    static const unsigned scan_res = resident_blocks((const void *)k_scan<STAGE>, SCAN_THREADS);
+    static const unsigned scan_res = resident_blocks((const void *)k_scan<STAGE, true>, SCAN_THREADS);
     const unsigned scan_grid_max = (unsigned)d.cu_count() * scan_res;
     auto go = [&](uint64_t tb, uint64_t te) {
         if (te <= tb) return;
@@ -1059,7 +1085,8 @@ static void launch_scan(McPlan &p, const ScanArgs &sa, int launch, hipStream_t s
         ca.tile_begin = tb;
         ca.tile_end = te;
         const unsigned g = (unsigned)std::min<uint64_t>(te - tb, scan_grid_max);
-        hipLaunchKernelGGL(k_scan<STAGE>, dim3(g), dim3(SCAN_THREADS), 0, s, ca);
+        if (ca.w >= WPT) hipLaunchKernelGGL((k_scan<STAGE, true>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
+        else hipLaunchKernelGGL((k_scan<STAGE, false>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
     };
     if (launch == MCZI_LAUNCH_PLAIN) {
         for (size_t c = 0; c + 1 < p.chunk_tile.size(); c++) go(p.chunk_tile[c], p.chunk_tile[c + 1]);
@@ -1247,7 +1274,19 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
                 MCZI_HIP(hipGraphExecDestroy(p.graph));
                 p.graph = nullptr;
             }
+            p.tile_nrec.ensure(std::max<uint64_t>(p.n_tiles, 1) * 4);
+            p.tile_base.ensure(std::max<uint64_t>(p.n_tiles, 1) * 8);
             enqueue_stage_a(b, p, cfg, mode, launch, s, nullptr, false);
+            if (p.n_tiles) {
+                size_t tmp = 0;
+                MCZI_HIP(rocprim::exclusive_scan(nullptr, tmp, p.tile_nrec.as<uint32_t>(),
+                                                 p.tile_base.as<unsigned long long>(), 0ull, (size_t)p.n_tiles,
+                                                 rocprim::plus<unsigned long long>(), s));
+                p.tile_scan_tmp.ensure(tmp + 16);
+                MCZI_HIP(rocprim::exclusive_scan(p.tile_scan_tmp.as<void>(), tmp, p.tile_nrec.as<uint32_t>(),
+                                                 p.tile_base.as<unsigned long long>(), 0ull, (size_t)p.n_tiles,
+                                                 rocprim::plus<unsigned long long>(), s));
+            }
             unsigned long long tot = 0;
             MCZI_HIP(hipMemcpyAsync(&tot, p.ctr.as<unsigned long long>() + 11, 8, hipMemcpyDeviceToHost, s));
             MCZI_HIP(hipStreamSynchronize(s));
@@ -1280,7 +1319,7 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
             for (const DevBuf *db : {&b.words, &b.off, &b.dataset, &p.base, &p.cursor, &p.records, &p.ctr,
                                      &p.ovf_keys, &p.ovf_vals, &p.ovf_part, &p.out_kmer, &p.out_count,
                                      &p.out_part, &p.tile_start, &p.stage_rec, &p.stage_q, &p.pqs,
-                                     &p.part_tmp})
+                                     &p.part_tmp, &p.tile_nrec, &p.tile_base})
                 sig = mix_sig(sig, (uint64_t)(uintptr_t)db->as<void>());
             sig = mix_sig(mix_sig(sig, p.ovf_slots), p.stage_cap);
             if (!p.graph || p.graph_sig != sig) {
@@ -1472,8 +1511,9 @@ void device_run_hashes(const uint8_t *packed, uint64_t n_bases, uint32_t k, uint
     sa.wrec = 1;
     sa.win_out = win.as<uint32_t>();
     sa.stats = ctr.as<unsigned long long>();
-    hipLaunchKernelGGL(k_scan<WINDOWS>, dim3((unsigned)std::min<uint64_t>(p.n_tiles, 2048)),
-                       dim3(SCAN_THREADS), 0, s, sa);
+    const dim3 wg((unsigned)std::min<uint64_t>(p.n_tiles, 2048));
+    if (sa.w >= WPT) hipLaunchKernelGGL((k_scan<WINDOWS, true>), wg, dim3(SCAN_THREADS), 0, s, sa);
+    else hipLaunchKernelGGL((k_scan<WINDOWS, false>), wg, dim3(SCAN_THREADS), 0, s, sa);
     MCZI_HIP(hipGetLastError());
     std::vector<uint32_t> v(nw);
     MCZI_HIP(hipMemcpyAsync(v.data(), win.as<void>(), nw * 4, hipMemcpyDeviceToHost, s));
