@@ -27,6 +27,7 @@
  */
 #include "oracle.h"
 
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1217,5 +1218,110 @@ int or_parse_fastx(const char *path, int min_len, uint8_t **codes_out, uint64_t 
     *codes_out = codes;
     *off_out = off;
     *nfrag_out = on - 1;
+    return 0;
+}
+
+/* ---- full-size spot checks ---------------------------------------------- */
+typedef struct {
+    const uint32_t *words;
+    const uint64_t *off;
+    const uint32_t *dataset;
+    size_t f0, f1;
+    int k;
+    const uint64_t *tkey;
+    const int64_t *tidx;
+    uint64_t tmask;
+    uint64_t *counts;
+    uint32_t *masks;
+} spot_job;
+
+static void *spot_worker(void *arg) {
+    spot_job *j = (spot_job *)arg;
+    const int k = j->k;
+    const uint64_t kmask = k >= 32 ? ~0ull : ((1ull << (2 * k)) - 1ull);
+    const int top = 2 * (k - 1);
+    for (size_t f = j->f0; f < j->f1; f++) {
+        const uint64_t s = j->off[f], e = j->off[f + 1];
+        if (e - s < (uint64_t)k) continue;
+        const uint32_t ds = j->dataset ? j->dataset[f] : 0;
+        uint64_t fwd = 0, rev = 0;
+        for (uint64_t p = s; p < e; p++) {
+            const uint32_t code = (j->words[p >> 4] >> ((p & 15) * 2)) & 3u;
+            const uint64_t b = code ^ (code >> 1); /* MC bits */
+            fwd = ((fwd << 2) | b) & kmask;
+            rev = (rev >> 2) | ((b ^ 3u) << top);
+            if (p + 1 - s < (uint64_t)k) continue;
+            const uint64_t key = fwd < rev ? fwd : rev;
+            uint64_t h = or_spread_hash_u64(key) & j->tmask;
+            while (j->tidx[h] >= 0) {
+                if (j->tkey[h] == key) {
+                    j->counts[j->tidx[h]]++;
+                    if (j->masks) j->masks[j->tidx[h]] |= 1u << (ds & 31u);
+                    break;
+                }
+                h = (h + 1) & j->tmask;
+            }
+        }
+    }
+    return NULL;
+}
+
+int or_spot_counts(const uint32_t *words, const uint64_t *off, size_t n_frag, const uint32_t *dataset, int k,
+                   const uint64_t *query, size_t n_query, uint64_t *counts, uint32_t *masks, int threads) {
+    if (k < 1 || k > 32 || threads < 1) return 1;
+    uint64_t slots = 16;
+    while (slots < 4 * (uint64_t)n_query) slots <<= 1;
+    uint64_t *tkey = (uint64_t *)xmalloc(slots * 8);
+    int64_t *tidx = (int64_t *)xmalloc(slots * 8);
+    for (uint64_t i = 0; i < slots; i++) tidx[i] = -1;
+    for (size_t q = 0; q < n_query; q++) {
+        uint64_t h = or_spread_hash_u64(query[q]) & (slots - 1);
+        while (tidx[h] >= 0 && tkey[h] != query[q]) h = (h + 1) & (slots - 1);
+        if (tidx[h] < 0) {
+            tkey[h] = query[q];
+            tidx[h] = (int64_t)q;
+        }
+    }
+    spot_job *jobs = (spot_job *)xmalloc(sizeof(spot_job) * (size_t)threads);
+    pthread_t *th = (pthread_t *)xmalloc(sizeof(pthread_t) * (size_t)threads);
+    /* split fragments so every thread gets about the same number of bases */
+    const uint64_t total = off[n_frag] - off[0];
+    size_t f = 0;
+    for (int t = 0; t < threads; t++) {
+        spot_job *j = &jobs[t];
+        j->words = words; j->off = off; j->dataset = dataset; j->k = k;
+        j->tkey = tkey; j->tidx = tidx; j->tmask = slots - 1;
+        j->counts = (uint64_t *)calloc(n_query ? n_query : 1, 8);
+        j->masks = masks ? (uint32_t *)calloc(n_query ? n_query : 1, 4) : NULL;
+        j->f0 = f;
+        const uint64_t goal = off[0] + total / (uint64_t)threads * (uint64_t)(t + 1);
+        while (f < n_frag && (t == threads - 1 || off[f] < goal)) f++;
+        j->f1 = f;
+        pthread_create(&th[t], NULL, spot_worker, j);
+    }
+    for (size_t q = 0; q < n_query; q++) {
+        counts[q] = 0;
+        if (masks) masks[q] = 0;
+    }
+    for (int t = 0; t < threads; t++) {
+        pthread_join(th[t], NULL);
+        for (size_t q = 0; q < n_query; q++) {
+            counts[q] += jobs[t].counts[q];
+            if (masks) masks[q] |= jobs[t].masks[q];
+        }
+        free(jobs[t].counts);
+        free(jobs[t].masks);
+    }
+    /* duplicate queries share the first copy's totals */
+    for (size_t q = 0; q < n_query; q++) {
+        uint64_t h = or_spread_hash_u64(query[q]) & (slots - 1);
+        while (tkey[h] != query[q]) h = (h + 1) & (slots - 1);
+        counts[q] = counts[tidx[h]];
+        if (masks) masks[q] = masks[tidx[h]];
+    }
+    free(jobs);
+    free(th);
+    free(tkey);
+    free(tidx);
     return 0;
 }

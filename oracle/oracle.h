@@ -88,6 +88,16 @@ int or_write_kff(const char *path, const or_counted *kmers, size_t n, int k, int
 int or_parse_fastx(const char *path, int min_len, uint8_t **codes, uint64_t **off,
                    size_t *nfrag);
 
+/* ---- full-size spot checks -------------------------------------------- */
+/* Exact occurrence count and dataset-presence mask of each query k-mer
+ * (canonical MC encoding, k <= 32) over reads packed 16 bases per u32 word
+ * (packed-seq codes, base i at bits 2(i%16) of word i/16), fragment i =
+ * bases [off[i], off[i+1]). dataset may be NULL (all 0); masks may be NULL.
+ * Counts every window exactly like or_count_bruteforce, on `threads`
+ * threads over fragment ranges. */
+int or_spot_counts(const uint32_t *words, const uint64_t *off, size_t n_frag, const uint32_t *dataset, int k,
+                   const uint64_t *query, size_t n_query, uint64_t *counts, uint32_t *masks, int threads);
+
 /* ---- small helpers ----------------------------------------------------- */
 uint64_t or_canonical_u64(const uint8_t *codes, int k); /* k <= 32 */
 void or_free(void *p);

@@ -79,6 +79,8 @@ def lib() -> C.CDLL:
         L.or_parse_fastx.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.POINTER(C.c_uint8)),
                                      C.POINTER(C.POINTER(C.c_uint64)), C.POINTER(C.c_size_t)]
         L.or_free.argtypes = [C.c_void_p]
+        L.or_spot_counts.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_int, C.c_void_p,
+                                     C.c_size_t, C.c_void_p, C.c_void_p, C.c_int]
     return _lib
 
 
@@ -289,3 +291,38 @@ def encode_canonical(s: str) -> int:
         b = 3 - "ACGT".index(ch)
         r = (r << 2) | b
     return min(f, r)
+
+
+# ---------------------------------------------------------------------------
+# Full-size spot checks (reads packed 16 bases per u32 word)
+def spot_counts(words, frag_off, k, query, dataset=None, threads=16):
+    """Exact count and dataset-presence mask of each query k-mer (canonical
+    MC encoding, k <= 32) over packed reads (oracle.c or_spot_counts)."""
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    off = np.ascontiguousarray(frag_off, dtype=np.uint64)
+    q = np.ascontiguousarray(query, dtype=np.uint64)
+    ds = None if dataset is None else np.ascontiguousarray(dataset, dtype=np.uint32)
+    counts = np.zeros(q.size, dtype=np.uint64)
+    masks = np.zeros(q.size, dtype=np.uint32)
+    rc = lib().or_spot_counts(words.ctypes.data, off.ctypes.data, off.size - 1,
+                              ds.ctypes.data if ds is not None else None, k, q.ctypes.data, q.size,
+                              counts.ctypes.data, masks.ctypes.data, threads)
+    if rc:
+        raise ValueError("or_spot_counts failed")
+    return counts, masks
+
+
+def kmers_at(words, pos, k):
+    """Canonical MC encoding of the k-mer starting at each base position
+    (k <= 32) of packed reads."""
+    words = np.asarray(words, dtype=np.uint32)
+    pos = np.asarray(pos, dtype=np.uint64)
+    fwd = np.zeros(pos.size, dtype=np.uint64)
+    rev = np.zeros(pos.size, dtype=np.uint64)
+    for i in range(k):
+        p = pos + np.uint64(i)
+        c = (words[(p >> np.uint64(4)).astype(np.int64)] >> ((p & np.uint64(15)) * np.uint64(2)).astype(np.uint32)) & 3
+        b = (c ^ (c >> 1)).astype(np.uint64)
+        fwd = (fwd << np.uint64(2)) | b
+        rev = rev | ((b ^ np.uint64(3)) << np.uint64(2 * i))
+    return np.minimum(fwd, rev)

@@ -266,6 +266,31 @@ def test_c1_full_config_kff_bytes(tmp_path):
     assert nrec == 1053631
 
 
+def _spot_check(words, off, k, x, got, dataset=None, n_emitted=3000, n_windows=3000, seed=0):
+    """Full-size parity by sampling: the exact count (fof: number of datasets)
+    of random emitted k-mers, and of the k-mers at random read windows, from
+    the oracle's spot counter over the same packed reads. A window k-mer is
+    emitted iff its value is > x, with that value."""
+    rng = np.random.default_rng(seed)
+    lo = got["lo"].astype(np.uint64)
+    cnt = got["count"].astype(np.uint64)
+    assert np.all(got["hi"] == 0) and np.all(np.diff(lo) > 0)
+    idx = rng.integers(0, lo.size, size=min(n_emitted, lo.size))
+    lens = np.diff(off.astype(np.int64))
+    ok = np.nonzero(lens >= k)[0]
+    f = rng.choice(ok, size=n_windows)
+    pos = off[f].astype(np.uint64) + (rng.random(n_windows) * (lens[f] - k + 1)).astype(np.uint64)
+    wk = O.kmers_at(words, pos, k)
+    c, m = O.spot_counts(words, off, k, np.concatenate([lo[idx], wk]), dataset, 16)
+    val = c if dataset is None else np.array([bin(int(v)).count("1") for v in m], dtype=np.uint64)
+    assert np.array_equal(val[: idx.size], cnt[idx])
+    vw = val[idx.size:]
+    at = np.minimum(np.searchsorted(lo, wk), lo.size - 1)
+    found = lo[at] == wk
+    assert np.array_equal(found, vw > x)
+    assert np.array_equal(cnt[at][found], vw[found])
+
+
 def test_c2_full_size_properties():
     """Short reads at scale (100 Mbp + repeats, 20M reads): batched and plain
     launches agree exactly; output sorted, unique, all counts > x; the total
@@ -287,6 +312,44 @@ def test_c2_full_size_properties():
     # single pass (KMC-style) gives the same set and counts
     b.count(cfg, M.MODE_SINGLE_PASS)
     assert same_counted(b.download(), a)
+    _spot_check(r.words, r.frag_off, 31, 5, a)
+
+
+def test_c3_full_size_long_and_short_reads():
+    """Long reads (50 Mbp genome, seed 3; lognormal lengths, median 8 kbp,
+    sigma 0.8, clip [500, 1e5], 20x; 4/2/2% substitutions/insertions/
+    deletions) plus 150 bp reads at 10x (0.5% substitutions) in one launch,
+    k=31, m=21, x=3: batched equals plain, and sampled exact counts."""
+    long = M.SynthReads(M.SynthSpec(genome_len=50_000_000, genome_seed=3, read_seed=103, long_bases=1_000_000_000,
+                                    long_median=8000, long_sigma=0.8, long_min=500, long_max=100_000,
+                                    sub_rate=0.04, ins_rate=0.02, del_rate=0.02))
+    short = M.SynthReads(M.SynthSpec(genome_len=50_000_000, genome_seed=3, read_seed=203, n_reads=3_333_334,
+                                     read_len=150, sub_rate=0.005))
+    words, off = _concat(long, short)
+    del long, short
+    b = M.Batch.from_packed(words, off)
+    cfg = M.CounterConfig(31, 21, 3, 1024)
+    st = b.count(cfg)
+    got = b.download()
+    assert st.n_bases > 1_400_000_000 and len(got) == st.n_out
+    b.count(cfg, M.MODE_TWO_PASS, M.LAUNCH_PLAIN)
+    assert same_counted(b.download(), got)
+    _spot_check(words, off, 31, 3, got, seed=3)
+
+
+def test_c4_full_size_fof_32_datasets():
+    """32 datasets: variants of a 20 Mbp genome (seed 4) with 0.1% SNPs, each
+    sequenced as 150 bp reads at 15x with 0.5% errors (9.6 Gbp in one
+    segmented buffer), fof k=31, m=21, x=5: sampled exact dataset counts."""
+    r = M.SynthReads(M.SynthSpec(genome_len=20_000_000, genome_seed=4, read_seed=104, n_reads=2_000_000,
+                                 read_len=150, sub_rate=0.005, n_datasets=32, snp_rate=0.001))
+    b = M.Batch.from_packed(r.words, r.frag_off, r.dataset)
+    assert b.info()[2] == 32
+    st = b.count(M.CounterConfig(31, 21, 5, 1024), M.MODE_DATASETS)
+    got = b.download()
+    assert st.n_bases == 9_600_000_000 and len(got) == st.n_out
+    assert np.all(got["count"] > 5) and np.all(got["count"] <= 32)
+    _spot_check(r.words, r.frag_off, 31, 5, got, dataset=r.dataset, seed=4)
 
 
 @pytest.mark.parametrize("k,m", [(31, 21), (21, 11), (15, 9)])

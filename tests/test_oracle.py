@@ -372,3 +372,32 @@ def test_fastq_and_lowercase(tmp_path):
     codes, off = O.parse_fastx(str(p), 3)
     frags = ["".join("ACTG"[c] for c in codes[off[i]:off[i + 1]]) for i in range(len(off) - 1)]
     assert frags == ["ACGT", "ACGT", "GGGG"]
+
+
+def test_spot_counts_match_bruteforce():
+    """The full-size spot counter (threads over fragment ranges, packed
+    words) against the brute-force closed form, including k-mers that do not
+    occur and dataset-presence masks."""
+    rng = np.random.default_rng(21)
+    seqs = random_seqs(rng, 300, 20, 400)
+    codes, off, _ = O.frags_of(seqs)
+    words = np.zeros((codes.size + 15) // 16 + 4, dtype=np.uint32)
+    for j in range(16):
+        part = codes[j::16].astype(np.uint32) << (2 * j)
+        words[: part.size] |= part
+    ds = (np.arange(off.size - 1) % 5).astype(np.uint32)
+    for k in (31, 15, 5):
+        want = O.count_bruteforce(codes, off, k, 0)
+        wmask = O.count_datasets_bruteforce(codes, off, ds, 5, k, 0)
+        present = dict(zip(want["lo"].tolist(), want["count"].tolist()))
+        npres = dict(zip(wmask["lo"].tolist(), wmask["count"].tolist()))
+        q = np.concatenate([want["lo"][::7], rng.integers(0, 1 << (2 * k), size=200, dtype=np.uint64)])
+        for threads in (1, 6):
+            c, m = O.spot_counts(words, off, k, q, ds, threads)
+            for key, cnt, mask in zip(q.tolist(), c.tolist(), m.tolist()):
+                assert cnt == present.get(key, 0), (k, key)
+                assert bin(mask).count("1") == npres.get(key, 0), (k, key)
+        # kmers_at gives the canonical k-mer of a window
+        f = 3
+        p0 = int(off[f])
+        assert O.kmers_at(words, [p0], k)[0] == O.encode_canonical("".join("ACTG"[c] for c in codes[p0:p0 + k]))
