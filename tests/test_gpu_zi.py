@@ -179,3 +179,51 @@ def test_server_rejects_over_budget():
         assert len(arr) == 0
     finally:
         M._lib.mczi_zi_server_destroy(h)
+
+
+def test_zi_full_size_index_spot_check():
+    """Serving at full index size: index = canonical 31-mers of a 100 Mbp
+    genome (seed 5); one request of 200k 150 bp reads, half from the index
+    genome (0.5% substitutions) and half from an unrelated genome. Sampled
+    absent k-mers never occur in the index; sampled query windows are absent
+    exactly when they do not occur in it (oracle spot counter over the index
+    genome)."""
+    G = 100_000_000
+    ga = M.synth_genome(G, 5)
+    gb = M.synth_genome(G, 1005)
+    codes = np.unpackbits(ga.view(np.uint8)[:, None], axis=1, bitorder="little").reshape(-1, 4, 2)
+    codes = (codes[:, :, 0] | (codes[:, :, 1] << 1)).reshape(-1)[:G]
+    text = b">index\n" + np.frombuffer(b"ACTG", dtype=np.uint8)[codes].tobytes() + b"\n"
+    del codes
+    reads = M.synth_reads_from([(ga, G), (gb, G)], 200_000, 150, 0.005, 6)
+    h = C.c_void_p()
+    M._check(M._lib.mczi_zi_server_create_from_text(text, len(text), 31, 0, 64 << 30, 2, C.byref(h)))
+    try:
+        off = reads.frag_off.astype(np.uint64)
+        rid = C.c_uint64()
+        M._check(M._lib.mczi_zi_server_submit_packed(h, reads.words.ctypes.data, off.ctypes.data, off.size - 1, 0,
+                                                      C.byref(rid)))
+        p = C.POINTER(M._ZiResult)()
+        M._check(M._lib.mczi_zi_server_poll(h, rid.value, 1, C.byref(p)))
+        r = p.contents
+        absent = np.ctypeslib.as_array(r.absent_lo, shape=(r.n_absent,)).copy()
+        n_windows = r.n_windows
+        M._lib.mczi_zi_result_free(p)
+    finally:
+        M._lib.mczi_zi_server_destroy(h)
+    assert n_windows == 200_000 * 120
+    assert np.all(np.diff(absent) > 0) and 0.3 * n_windows < absent.size < n_windows
+    rng = np.random.default_rng(5)
+    gwords = ga.view(np.uint32)
+    goff = np.array([0, G], dtype=np.uint64)
+    # sampled absent k-mers: none occurs in the index genome
+    sa = absent[rng.integers(0, absent.size, size=3000)]
+    c, _ = O.spot_counts(gwords, goff, 31, sa)
+    assert np.all(c == 0)
+    # sampled query windows: absent iff not in the index genome
+    f = rng.integers(0, off.size - 1, size=3000)
+    pos = off[f] + rng.integers(0, 150 - 31 + 1, size=3000).astype(np.uint64)
+    wk = O.kmers_at(reads.words, pos, 31)
+    c, _ = O.spot_counts(gwords, goff, 31, wk)
+    at = np.minimum(np.searchsorted(absent, wk), absent.size - 1)
+    assert np.array_equal(absent[at] == wk, c == 0)
