@@ -40,7 +40,7 @@ constexpr int REC_MAX_BASES = 56;             // bases per 128-bit super-k-mer r
 
 // Count kernel geometry
 constexpr int CNT_THREADS = 256;
-constexpr int CNT_LOG2_SLOTS = 12;
+constexpr int CNT_LOG2_SLOTS = 11;
 constexpr int CNT_SLOTS = 1 << CNT_LOG2_SLOTS;  // LDS table slots (u64 key + u32 value)
 constexpr int CNT_MAXPROBE = 32;
 constexpr int OVF_MAXPROBE = 1 << 16;
