@@ -160,7 +160,7 @@ struct Batch {
     // tile schedule (depends on k) and per-run state
     std::unique_ptr<McPlan> plan;
     // results of the last count (device)
-    DevBuf out_kmer, out_count, out_part;
+    DevBuf out_kmer, out_count, out_part, out_hi;  // out_hi: high words for k > 32
     uint64_t n_out = 0;
     uint32_t k = 0;
     mczi_run_stats last{};

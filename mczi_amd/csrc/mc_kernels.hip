@@ -36,7 +36,14 @@ constexpr int WORDS_SPAN = (15 + SPAN + 160) / 16 + 4;
 // one pad element every 8, so the 64 lanes of a wave hit distinct banks.
 #define PADDED(n) ((n) + (n) / 8 + 1)
 __device__ __forceinline__ int P8(int i) { return i + (i >> 3); }
-constexpr int REC_MAX_BASES = 56;             // bases per 128-bit super-k-mer record
+constexpr int REC_MAX_BASES = 56;             // bases per 128-bit super-k-mer record (k <= 32)
+constexpr int REC_MAX_BASES_W = 112;          // bases per 256-bit record (33 <= k <= 63)
+
+// 256-bit super-k-mer record for k > 32: bases 0..95 in w[0..2], bases
+// 96..111 in w[3] bits 0..31, length at w[3] bits 48..55, dataset at 56..63.
+struct Rec32 {
+    unsigned long long w[4];
+};
 
 // Count kernel geometry
 constexpr int CNT_THREADS = 256;
@@ -69,7 +76,9 @@ struct ScanArgs {
     unsigned long long *stats;  // [0] pieces [1] records
     // staging of super-k-mer records and their internal partition (null: count only)
     ulonglong2 *stage_rec;
+    Rec32 *stage_rec_w;  // k > 32
     uint32_t *stage_q;
+    int staging;         // records are written (else only counted)
     uint64_t stage_cap;
     unsigned long long *stage_ctr;  // total records (one add per block)
     uint32_t *tile_nrec;            // count-only pass: records of each tile
@@ -152,7 +161,7 @@ __device__ __forceinline__ uint32_t extract16_bases(const uint32_t *sw, int pos)
 // WINDOWS the selected hash of every window. GW selects the window-minimum
 // scheme (w >= WPT: Gil-Werman over the thread's windows; else a direct scan
 // of the w hashes). STAGE serves the counting path, which has k <= 32.
-template <int MODE, bool GW>
+template <int MODE, bool GW, bool WIDE = false>
 __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_scan(ScanArgs a) {
     __shared__ uint32_t s_words[WORDS_SPAN];
     __shared__ uint32_t s_hash[PADDED(TILE + MAXW)];
@@ -189,7 +198,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
         const int rb0 = (int)(ts & 15u);
         const int nwords = ((rb0 + tl + 160) >> 4) + 3;
         unsigned long long tbase = 0;
-        if (MODE == STAGE && a.stage_rec != nullptr && tid == 0) tbase = a.tile_base[t];
+        if (MODE == STAGE && a.staging && tid == 0) tbase = a.tile_base[t];
         __syncthreads();  // previous tile fully consumed
         for (int i = tid; i < WORDS_SPAN; i += SCAN_THREADS) s_words[i] = i < nwords ? a.words[(ts >> 4) + i] : 0u;
         // Fragment boundaries relative to ts: s_fs[j] = off[f0 + j] - ts (clamped),
@@ -281,7 +290,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             // strand vote of window i counts them over bases [i, i + k)
             const uint64_t H0 = extract32_bases(s_words, (uint64_t)rb) & 0xAAAAAAAAAAAAAAAAull;
             const uint64_t H1 = extract32_bases(s_words, (uint64_t)rb + 32) & 0xAAAAAAAAAAAAAAAAull;
-            constexpr bool KBIG = MODE == WINDOWS;  // k may exceed 32
+            constexpr bool KBIG = MODE == WINDOWS || WIDE;  // k may exceed 32
             const uint64_t H2 =
                 KBIG && k > 32 ? extract32_bases(s_words, (uint64_t)rb + 64) & 0xAAAAAAAAAAAAAAAAull : 0ull;
             const uint64_t mlo = high_bit_mask(k < 32 ? k : 32);
@@ -415,11 +424,11 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
         // (records land in tile order, no device-scope atomics per tile).
         if (tid == 0) {
             my_records += (unsigned long long)tot;
-            if (a.stage_rec == nullptr) a.tile_nrec[t] = (uint32_t)tot;
+            if (!a.staging) a.tile_nrec[t] = (uint32_t)tot;
             s_sbase = tbase;
         }
         __syncthreads();
-        if (a.stage_rec != nullptr) {
+        if (a.staging) {
             uint64_t pos = s_sbase + (uint64_t)off;
             for (int pi = tid; pi < np; pi += SCAN_THREADS) {
                 const uint32_t pw = s_u.run.pst[pi];
@@ -434,6 +443,26 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
                 for (int cs = s; cs < e; cs += (int)a.wrec) {
                     const int ce = min(cs + (int)a.wrec, e);
                     const uint32_t L = (uint32_t)(ce - cs + k - 1);
+                    if constexpr (WIDE) {
+                        Rec32 rw;
+#pragma unroll
+                        for (int c = 0; c < 4; c++) {
+                            uint64_t x = extract32_bases(s_words, (uint64_t)(rb0 + cs + 32 * c));
+                            const int nbc = (int)L - 32 * c;  // bases of this word
+                            if (nbc <= 0) x = 0;
+                            else if (nbc < 32) x &= (1ull << (2 * nbc)) - 1ull;
+                            rw.w[c] = x ^ ((x >> 1) & 0x5555555555555555ull);  // packed code -> MC code
+                        }
+                        rw.w[3] = (rw.w[3] & 0xffffffffull) | ((uint64_t)L << 48) | (ds << 56);
+                        if (pos < a.stage_cap) {
+                            a.stage_rec_w[pos] = rw;
+                            a.stage_q[pos] = q;
+                        } else {
+                            atomicOr(a.err, 2ull);
+                        }
+                        pos++;
+                        continue;
+                    }
                     uint64_t lo64 = extract32_bases(s_words, (uint64_t)(rb0 + cs));
                     uint64_t hi64 = 0;
                     if (L < 32) lo64 &= (1ull << (2 * L)) - 1ull;
@@ -476,6 +505,9 @@ __global__ void k_part_bounds(const uint32_t *qs, uint64_t n, unsigned long long
 // ---------------------------------------------------------------------------
 struct CountArgs {
     const ulonglong2 *records;
+    const Rec32 *records_w;            // k > 32
+    unsigned long long *ovf_hi;        // k > 32: high words of spilled k-mers
+    unsigned long long *out_hi;        // k > 32: high words of emitted k-mers
     const unsigned long long *base;    // partition start
     const unsigned long long *cursor;  // partition end
     uint32_t p_int;
@@ -762,6 +794,253 @@ __global__ void k_overflow_emit(CountArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Counting for 33 <= k <= 63: canonical k-mers are 128-bit (hi < 2^62), the
+// u128 k-mers of src/lib.rs:3589-3614. A table slot (LDS or spill) is claimed
+// by a CAS of its high word to (hi | KEY_BUSY); the claimer then writes the
+// low word and releases the high word. A reader that meets a busy slot looks
+// again in the next round; the number of rounds is bounded and reports error
+// bit 8 if the bound is ever reached.
+constexpr unsigned long long KEY_BUSY = 1ull << 63;
+constexpr int SPIN_LIMIT = 1 << 20;
+
+__device__ __forceinline__ uint64_t rev_groups64(uint64_t x) {
+    x = __builtin_bitreverse64(x);
+    return ((x >> 1) & 0x5555555555555555ull) | ((x & 0x5555555555555555ull) << 1);
+}
+
+// Canonical MC encoding of the k MC-coded bases in (b:a): a holds bases
+// 0..31, b bases 32..k-1 (masked by mb), base j at bits 2j. Forward = 2-bit
+// groups reversed over 2k bits; reverse complement = complement in order.
+__device__ __forceinline__ void canon128_mc(uint64_t a, uint64_t b, int k, uint64_t mb, uint64_t &clo,
+                                            uint64_t &chi) {
+    const uint64_t fh = rev_groups64(a), fl = rev_groups64(b);
+    const int sh = 128 - 2 * k;  // 2..62
+    const uint64_t f_lo = (fl >> sh) | (fh << (64 - sh));
+    const uint64_t f_hi = fh >> sh;
+    const uint64_t r_lo = ~a, r_hi = (~b) & mb;
+    const bool fwd = f_hi < r_hi || (f_hi == r_hi && f_lo < r_lo);
+    clo = fwd ? f_lo : r_lo;
+    chi = fwd ? f_hi : r_hi;
+}
+
+// One insertion round per loop iteration for every active lane; a lane that
+// claims a slot releases it in the same round, so a lane that saw the claim
+// (busy) finds the key published when it looks again next round. No lane
+// leaves the loop early (the exit is a wave-wide vote), which keeps the
+// claimer's release ahead of the waiters' next look.
+__device__ void overflow_insert_w(const CountArgs &a, uint64_t klo, uint64_t khi, uint32_t q) {
+    uint64_t slot = spread_hash_u64(klo ^ spread_hash_u64(khi + 0x9e3779b97f4a7c15ull)) & a.ovf_mask;
+    bool done = false;
+    int probes = 0, waits = 0;
+    while (true) {
+        if (!done) {
+            unsigned long long h =
+                __hip_atomic_load(&a.ovf_hi[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            bool claimed = false;
+            if (h == EMPTY_KEY) {
+                const unsigned long long prev = atomicCAS(&a.ovf_hi[slot], EMPTY_KEY, khi | KEY_BUSY);
+                if (prev == EMPTY_KEY) claimed = true;
+                else h = prev;
+            }
+            if (claimed) {
+                a.ovf_part[slot] = q;
+                __hip_atomic_store(&a.ovf_keys[slot], klo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&a.ovf_hi[slot], khi, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                atomicAdd(&a.ctr[3], 1ull);
+                atomicAdd(&a.ovf_vals[slot], 1u);
+                done = true;
+            } else if (h & KEY_BUSY) {
+                if (++waits > SPIN_LIMIT) {
+                    atomicOr(&a.ctr[4], 8ull);
+                    done = true;
+                }
+            } else if (h == khi &&
+                       __hip_atomic_load(&a.ovf_keys[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == klo) {
+                atomicAdd(&a.ovf_vals[slot], 1u);
+                done = true;
+            } else {
+                slot = (slot + 1) & a.ovf_mask;
+                if (++probes >= OVF_MAXPROBE) {
+                    atomicOr(&a.ctr[4], 1ull);  // spill table full: host retries with more partitions
+                    done = true;
+                }
+            }
+        }
+        if (__all(done)) break;
+    }
+}
+
+// Same structure as k_count<0> (counts, emit count > x), over 256-bit
+// records and 128-bit keys.
+__global__ __launch_bounds__(CNT_THREADS) void k_count_wide(CountArgs a) {
+    __shared__ unsigned long long s_klo[CNT_SLOTS];
+    __shared__ unsigned long long s_khi[CNT_SLOTS];
+    __shared__ uint32_t s_vals[CNT_SLOTS];
+    __shared__ Rec32 s_rec[CNT_THREADS / 64][64];
+    __shared__ uint8_t s_mark[CNT_THREADS / 64][64];
+    __shared__ int s_wsum[CNT_THREADS / 64];
+    __shared__ unsigned long long s_red[CNT_THREADS / 64];
+    __shared__ unsigned long long s_obase;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wid = tid >> 6;
+    const int k = a.k;
+    const uint64_t mb = (k - 32) >= 32 ? ~0ull : ((1ull << (2 * (k - 32))) - 1ull);
+    unsigned long long my_distinct = 0;
+    for (uint32_t q = blockIdx.x; q < a.p_int; q += gridDim.x) {
+        for (int i = tid; i < CNT_SLOTS; i += CNT_THREADS) {
+            s_klo[i] = EMPTY_KEY;
+            s_khi[i] = EMPTY_KEY;
+            s_vals[i] = 0;
+        }
+        __syncthreads();
+        const uint64_t rb = a.base[q], re = a.cursor[q];
+        for (uint64_t wb = rb + (uint64_t)wid * 64; wb < re; wb += CNT_THREADS) {
+            const uint64_t ri = wb + lane;
+            Rec32 r{{0, 0, 0, 0}};
+            int n = 0;
+            if (ri < re) {
+                r = a.records_w[ri];
+                n = (int)((r.w[3] >> 48) & 255u) - k + 1;
+                if (n < 0) n = 0;
+            }
+            s_rec[wid][lane] = r;
+            const int incl = wave_incl_add(n);
+            const int total = __builtin_amdgcn_readlane(incl, 63);
+            const int my_excl = incl - n;
+            int carry = 0;
+            for (int j0 = 0; j0 < total; j0 += 64) {
+                const int j = j0 + lane;
+                s_mark[wid][lane] = 0;
+                if (n > 0 && my_excl >= j0 && my_excl < j0 + 64) s_mark[wid][my_excl - j0] = (uint8_t)(lane + 1);
+                __builtin_amdgcn_wave_barrier();
+                int mk = wave_incl_max((int)s_mark[wid][lane]);
+                mk = max(mk, carry);
+                carry = __builtin_amdgcn_readlane(mk, 63);
+                __builtin_amdgcn_wave_barrier();
+                const int o = (mk - 1) & 63;
+                const int excl = __shfl(my_excl, o, 64);
+                if (j >= total) continue;
+                const int off = j - excl;  // k-mer start inside the record (<= 112 - k)
+                const Rec32 ro = s_rec[wid][o];
+                const uint64_t W3 = ro.w[3] & 0xffffffffull;
+                const int bo = 2 * off, wi = bo >> 6, sh = bo & 63;
+                const uint64_t x0 = wi == 0 ? ro.w[0] : (wi == 1 ? ro.w[1] : ro.w[2]);
+                const uint64_t x1 = wi == 0 ? ro.w[1] : (wi == 1 ? ro.w[2] : W3);
+                const uint64_t x2 = wi == 0 ? ro.w[2] : (wi == 1 ? W3 : 0ull);
+                const uint64_t fa = sh ? ((x0 >> sh) | (x1 << (64 - sh))) : x0;
+                const uint64_t fb = (sh ? ((x1 >> sh) | (x2 << (64 - sh))) : x1) & mb;
+                uint64_t klo, khi;
+                canon128_mc(fa, fb, k, mb, klo, khi);
+                uint32_t slot = lds_slot(klo ^ (khi * 0x9E3779B97F4A7C15ull));
+                bool done = false, spill = false;
+                int probes = 0, waits = 0;
+                while (true) {  // same round structure as overflow_insert_w
+                    if (!done) {
+                        unsigned long long h =
+                            __hip_atomic_load(&s_khi[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        bool claimed = false;
+                        if (h == EMPTY_KEY) {
+                            const unsigned long long prev = atomicCAS(&s_khi[slot], EMPTY_KEY, khi | KEY_BUSY);
+                            if (prev == EMPTY_KEY) claimed = true;
+                            else h = prev;
+                        }
+                        if (claimed) {
+                            __hip_atomic_store(&s_klo[slot], klo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            __hip_atomic_store(&s_khi[slot], khi, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            atomicAdd(&s_vals[slot], 1u);
+                            done = true;
+                        } else if (h & KEY_BUSY) {
+                            if (++waits > SPIN_LIMIT) {
+                                atomicOr(&a.ctr[4], 8ull);
+                                done = true;
+                            }
+                        } else if (h == khi && __hip_atomic_load(&s_klo[slot], __ATOMIC_ACQUIRE,
+                                                                 __HIP_MEMORY_SCOPE_WORKGROUP) == klo) {
+                            atomicAdd(&s_vals[slot], 1u);
+                            done = true;
+                        } else {
+                            slot = (slot + 1) & (CNT_SLOTS - 1);
+                            if (++probes >= CNT_MAXPROBE) {
+                                spill = true;
+                                done = true;
+                            }
+                        }
+                    }
+                    if (__all(done)) break;
+                }
+                if (spill) overflow_insert_w(a, klo, khi, q);
+            }
+        }
+        __syncthreads();
+        constexpr int PER = CNT_SLOTS / CNT_THREADS;
+        uint32_t emask = 0;
+        int cnt = 0;
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int sl = i * CNT_THREADS + tid;
+            if (s_khi[sl] == EMPTY_KEY) continue;
+            my_distinct++;
+            if (s_vals[sl] > a.x) {
+                emask |= 1u << i;
+                cnt++;
+            }
+        }
+        int x = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) s_wsum[wid] = x;
+        __syncthreads();
+        int pre = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < CNT_THREADS / 64; w++) {
+            pre += (w < wid) ? s_wsum[w] : 0;
+            tot += s_wsum[w];
+        }
+        if (tid == 0) s_obase = tot ? atomicAdd(&a.ctr[1], (unsigned long long)tot) : 0ull;
+        __syncthreads();
+        unsigned long long pos = s_obase + (unsigned long long)(pre + x - cnt);
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            if (!(emask & (1u << i))) continue;
+            const int sl = i * CNT_THREADS + tid;
+            if (pos < a.out_cap) {
+                a.out_kmer[pos] = s_klo[sl];
+                a.out_hi[pos] = s_khi[sl];
+                a.out_count[pos] = s_vals[sl];
+                a.out_part[pos] = q;
+            }
+            pos++;
+        }
+        __syncthreads();
+    }
+    block_sum_atomic(my_distinct, &a.ctr[2], s_red);
+}
+
+__global__ void k_overflow_emit_wide(CountArgs a) {
+    const uint64_t n = a.ovf_mask + 1;
+    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < n; i0 += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = i0 + threadIdx.x;
+        unsigned long long hi = EMPTY_KEY;
+        uint32_t c = 0;
+        if (i < n) {
+            hi = a.ovf_hi[i];
+            if (hi != EMPTY_KEY) c = a.ovf_vals[i];
+        }
+        const bool emit = hi != EMPTY_KEY && c > a.x;
+        const unsigned long long pos = wave_append(emit, &a.ctr[1]);
+        if (emit && pos < a.out_cap) {
+            a.out_kmer[pos] = a.ovf_keys[i];
+            a.out_hi[pos] = hi;
+            a.out_count[pos] = c;
+            a.out_part[pos] = a.ovf_part[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x)
@@ -824,6 +1103,16 @@ __global__ void k_kff_keys(const unsigned long long *kmer, const uint32_t *part,
         idx[i] = (uint32_t)i;
     }
 }
+__global__ void k_iota_u32(uint32_t *p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = (uint32_t)i;
+}
+
+__global__ void k_gather_u64(const unsigned long long *src, const uint32_t *idx, unsigned long long *dst, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[idx[i]];
+}
+
 __global__ void k_gather_u32(const uint32_t *src, const uint32_t *idx, uint32_t *dst, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x)
@@ -852,6 +1141,7 @@ struct McPlan {
     uint64_t n_windows = 0;
     // state
     DevBuf base, cursor, records, ctr, ovf_keys, ovf_vals, ovf_part, stage_rec, stage_q, pqs, part_tmp;
+    DevBuf records_w, stage_rec_w, ovf_hi, out_hi;  // k > 32
     uint64_t stage_cap = 0, n_staged = 0;
     size_t part_tmp_bytes = 0;
     DevBuf out_kmer, out_count, out_part;
@@ -1063,9 +1353,11 @@ static ScanArgs scan_args(Batch &b, McPlan &p, const mczi_counter_config &cfg) {
     sa.sat = (uint32_t)std::min<uint64_t>(cfg.threshold + 1, 255);
     sa.p_int = p.p_int;
     sa.p_mask = (p.p_int & (p.p_int - 1)) == 0 ? p.p_int - 1 : 0u;
-    sa.wrec = (uint32_t)(REC_MAX_BASES - k + 1);
+    sa.wrec = (uint32_t)((k > 32 ? REC_MAX_BASES_W : REC_MAX_BASES) - k + 1);
     sa.stats = p.ctr.as<unsigned long long>() + 8;
     sa.stage_rec = p.stage_rec.as<ulonglong2>();
+    sa.stage_rec_w = p.stage_rec_w.as<Rec32>();
+    sa.staging = 1;
     sa.stage_q = p.stage_q.as<uint32_t>();
     sa.stage_cap = p.stage_cap;
     sa.stage_ctr = p.ctr.as<unsigned long long>() + 11;
@@ -1085,7 +1377,10 @@ static void launch_scan(McPlan &p, const ScanArgs &sa, int launch, hipStream_t s
         ca.tile_begin = tb;
         ca.tile_end = te;
         const unsigned g = (unsigned)std::min<uint64_t>(te - tb, scan_grid_max);
-        if (ca.w >= WPT) hipLaunchKernelGGL((k_scan<STAGE, true>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
+        const bool gw = ca.w >= WPT, wide = ca.k > 32;
+        if (wide && gw) hipLaunchKernelGGL((k_scan<STAGE, true, true>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
+        else if (wide) hipLaunchKernelGGL((k_scan<STAGE, false, true>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
+        else if (gw) hipLaunchKernelGGL((k_scan<STAGE, true>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
         else hipLaunchKernelGGL((k_scan<STAGE, false>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
     };
     if (launch == MCZI_LAUNCH_PLAIN) {
@@ -1100,7 +1395,7 @@ static void launch_scan(McPlan &p, const ScanArgs &sa, int launch, hipStream_t s
 static void enqueue_stage_a(Batch &b, McPlan &p, const mczi_counter_config &cfg, int mode,
                             int launch, hipStream_t s, Timer *tm, bool stage = true) {
     ScanArgs sa = scan_args(b, p, cfg);
-    if (!stage) sa.stage_rec = nullptr;
+    if (!stage) sa.staging = 0;
     if (mode != MCZI_MODE_DATASETS) sa.dataset = nullptr;
     unsigned long long *ctr = p.ctr.as<unsigned long long>();
     fill_u64(ctr, 16, 0, s);
@@ -1112,6 +1407,9 @@ static void enqueue_stage_a(Batch &b, McPlan &p, const mczi_counter_config &cfg,
 static CountArgs count_args(McPlan &p, const mczi_counter_config &cfg) {
     CountArgs ca{};
     ca.records = p.records.as<ulonglong2>();
+    ca.records_w = p.records_w.as<Rec32>();
+    ca.ovf_hi = p.ovf_hi.as<unsigned long long>();
+    ca.out_hi = p.out_hi.as<unsigned long long>();
     ca.base = p.base.as<unsigned long long>();
     ca.cursor = p.cursor.as<unsigned long long>();
     ca.p_int = p.p_int;
@@ -1143,7 +1441,9 @@ static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg,
     (void)b;
     (void)launch;
     const uint64_t n = p.n_staged;
+    const bool wide = cfg.k > 32;
     fill_u64(p.ovf_keys.as<unsigned long long>(), p.ovf_slots, EMPTY_KEY, s);
+    if (wide) fill_u64(p.ovf_hi.as<unsigned long long>(), p.ovf_slots, EMPTY_KEY, s);
     fill_u32(p.ovf_vals.as<uint32_t>(), p.ovf_slots, 0, s);
     fill_u64(p.base.as<unsigned long long>(), p.p_int, 0, s);
     fill_u64(p.cursor.as<unsigned long long>(), p.p_int, 0, s);
@@ -1151,9 +1451,14 @@ static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg,
     if (n) {
         const unsigned g = grid_for(n, 256, 16384);
         size_t tmp = p.part_tmp_bytes;
-        MCZI_HIP(rocprim::radix_sort_pairs(p.part_tmp.as<void>(), tmp, p.stage_q.as<uint32_t>(),
-                                           p.pqs.as<uint32_t>(), p.stage_rec.as<ulonglong2>(),
-                                           p.records.as<ulonglong2>(), (size_t)n, 0, bits_for(p.p_int), s));
+        if (wide)
+            MCZI_HIP(rocprim::radix_sort_pairs(p.part_tmp.as<void>(), tmp, p.stage_q.as<uint32_t>(),
+                                               p.pqs.as<uint32_t>(), p.stage_rec_w.as<Rec32>(),
+                                               p.records_w.as<Rec32>(), (size_t)n, 0, bits_for(p.p_int), s));
+        else
+            MCZI_HIP(rocprim::radix_sort_pairs(p.part_tmp.as<void>(), tmp, p.stage_q.as<uint32_t>(),
+                                               p.pqs.as<uint32_t>(), p.stage_rec.as<ulonglong2>(),
+                                               p.records.as<ulonglong2>(), (size_t)n, 0, bits_for(p.p_int), s));
         hipLaunchKernelGGL(k_part_bounds, dim3(g), dim3(256), 0, s, p.pqs.as<uint32_t>(), n,
                            p.base.as<unsigned long long>(), p.cursor.as<unsigned long long>());
     }
@@ -1163,7 +1468,13 @@ static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg,
     static const unsigned count_res = resident_blocks((const void *)k_count<0>, CNT_THREADS);
     const unsigned cgrid = (unsigned)std::min<uint64_t>(p.p_int, (uint64_t)d.cu_count() * count_res);
     const unsigned ogrid = grid_for(p.ovf_slots, 256, 8192);
-    if (mode == MCZI_MODE_DATASETS) {
+    if (wide) {
+        static const unsigned wide_res = resident_blocks((const void *)k_count_wide, CNT_THREADS);
+        const unsigned wgrid = (unsigned)std::min<uint64_t>(p.p_int, (uint64_t)d.cu_count() * wide_res);
+        hipLaunchKernelGGL(k_count_wide, dim3(wgrid), dim3(CNT_THREADS), 0, s, ca);
+        if (tm) tm->mark(s);
+        hipLaunchKernelGGL(k_overflow_emit_wide, dim3(ogrid), dim3(256), 0, s, ca);
+    } else if (mode == MCZI_MODE_DATASETS) {
         hipLaunchKernelGGL(k_count<1>, dim3(cgrid), dim3(CNT_THREADS), 0, s, ca);
         if (tm) tm->mark(s);
         hipLaunchKernelGGL(k_overflow_emit<1>, dim3(ogrid), dim3(256), 0, s, ca);
@@ -1186,6 +1497,35 @@ static void sort_output(Batch &b, McPlan &p, uint64_t n, int order, uint32_t p_u
     unsigned long long *kout = b.out_kmer.as<unsigned long long>();
     uint32_t *cout = b.out_count.as<uint32_t>();
     size_t tmp = 0;
+    if (k > 32) {
+        // 128-bit keys in encoded order: LSD over (lo, then hi) carrying the
+        // original positions, then a gather of lo and counts
+        require(order == MCZI_ORDER_ENCODED, MCZI_E_INVALID, "KFF u8 record order needs k <= 32");
+        const unsigned long long *hin = p.out_hi.as<unsigned long long>();
+        b.out_hi.ensure(n * 8);
+        unsigned long long *hout = b.out_hi.as<unsigned long long>();
+        p.idx.ensure(n * 4);
+        p.idx2.ensure(n * 4);
+        p.key2.ensure(n * 8);
+        p.k2out.ensure(n * 8);
+        const unsigned g = grid_for(n, 256, 8192);
+        hipLaunchKernelGGL(k_iota_u32, dim3(g), dim3(256), 0, s, p.idx.as<uint32_t>(), n);
+        size_t t2 = 0;
+        MCZI_HIP(rocprim::radix_sort_pairs(nullptr, tmp, kin, p.key2.as<unsigned long long>(), p.idx.as<uint32_t>(),
+                                           p.idx2.as<uint32_t>(), (size_t)n, 0, 64, s));
+        MCZI_HIP(rocprim::radix_sort_pairs(nullptr, t2, p.k2out.as<unsigned long long>(), hout, p.idx2.as<uint32_t>(),
+                                           p.idx.as<uint32_t>(), (size_t)n, 0, (unsigned)(2 * k - 64), s));
+        p.sort_tmp.ensure(std::max(tmp, t2) + 16);
+        MCZI_HIP(rocprim::radix_sort_pairs(p.sort_tmp.as<void>(), tmp, kin, p.key2.as<unsigned long long>(),
+                                           p.idx.as<uint32_t>(), p.idx2.as<uint32_t>(), (size_t)n, 0, 64, s));
+        hipLaunchKernelGGL(k_gather_u64, dim3(g), dim3(256), 0, s, hin, p.idx2.as<uint32_t>(),
+                           p.k2out.as<unsigned long long>(), n);
+        MCZI_HIP(rocprim::radix_sort_pairs(p.sort_tmp.as<void>(), t2, p.k2out.as<unsigned long long>(), hout,
+                                           p.idx2.as<uint32_t>(), p.idx.as<uint32_t>(), (size_t)n, 0,
+                                           (unsigned)(2 * k - 64), s));
+        hipLaunchKernelGGL(k_gather_kmer, dim3(g), dim3(256), 0, s, kin, cin, p.idx.as<uint32_t>(), kout, cout, n);
+        return;
+    }
     if (order == MCZI_ORDER_ENCODED) {
         const unsigned end_bit = (unsigned)std::min(64, 2 * k);
         MCZI_HIP(rocprim::radix_sort_pairs(nullptr, tmp, kin, kout, cin, cout, (size_t)n, 0, end_bit, s));
@@ -1235,16 +1575,19 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
     Device &d = Device::get();
     hipStream_t s = d.stream();
     const uint32_t k = cfg.k;
-    require(k >= 1 && k <= 32, MCZI_E_INVALID,
-            "the device counting path supports k <= 32 (k-mers are 64-bit on the device)");
+    require(k >= 1 && k <= 64, MCZI_E_INVALID, "k must be <= 64 because MC stores k-mers in u128");
+    const bool wide = k > 32;  // 128-bit k-mers, 256-bit records
     require(cfg.minimizer >= 1 && cfg.minimizer <= k, MCZI_E_INVALID, "minimizer size must be <= k");
     require(k - cfg.minimizer + 1 <= MAXW, MCZI_E_INVALID, "window k - m + 1 must be <= 64");
     require(k % 2 == 1, MCZI_E_INVALID,
             "k must be odd: the canonical window length l = k must be odd (simd-minimizers)");
     require(cfg.partition_count > 0, MCZI_E_INVALID, "partition count must be greater than 0");
-    if (mode == MCZI_MODE_DATASETS)
+    if (mode == MCZI_MODE_DATASETS) {
+        require(!wide, MCZI_E_INVALID, "FOFN dataset-presence mode currently requires k <= 32");
         require(b.n_datasets <= 32, MCZI_E_INVALID,
                 "the device fof path supports at most 32 datasets per batch");
+    }
+    require(!wide || order != MCZI_ORDER_KFF_U8, MCZI_E_INVALID, "KFF u8 record order needs k <= 32");
     if (!b.plan) b.plan.reset(new McPlan());
     McPlan &p = *b.plan;
     if (p.k != k || p.n_tiles == 0) build_tiles(b, p, k);
@@ -1261,8 +1604,10 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
         p.ovf_keys.ensure(ovf * 8);
         p.ovf_vals.ensure(ovf * 4);
         p.ovf_part.ensure(ovf * 4);
+        if (wide) p.ovf_hi.ensure(ovf * 8);
         p.out_cap = p.n_windows / (cfg.threshold + 1) + 16;
         p.out_kmer.ensure(p.out_cap * 8);
+        if (wide) p.out_hi.ensure(p.out_cap * 8);
         p.out_count.ensure(p.out_cap * 4);
         p.out_part.ensure(p.out_cap * 4);
 
@@ -1294,17 +1639,29 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
             p.n_staged = tot;
             p.stage_cap = tot + 16;
             p.rec_cap = tot + 16;
-            p.records.ensure(p.rec_cap * 16);
-            p.stage_rec.ensure(p.stage_cap * 16);
+            if (wide) {
+                p.records_w.ensure(p.rec_cap * sizeof(Rec32));
+                p.stage_rec_w.ensure(p.stage_cap * sizeof(Rec32));
+            } else {
+                p.records.ensure(p.rec_cap * 16);
+                p.stage_rec.ensure(p.stage_cap * 16);
+            }
             p.stage_q.ensure(p.stage_cap * 4);
             p.pqs.ensure(p.stage_cap * 4);
             p.rec_sig = cfg_sig;
         }
         {
             size_t tmp = 0;
-            MCZI_HIP(rocprim::radix_sort_pairs(nullptr, tmp, p.stage_q.as<uint32_t>(), p.pqs.as<uint32_t>(),
-                                               p.stage_rec.as<ulonglong2>(), p.records.as<ulonglong2>(),
-                                               (size_t)std::max<uint64_t>(p.n_staged, 1), 0, bits_for(p.p_int), s));
+            if (wide)
+                MCZI_HIP(rocprim::radix_sort_pairs(nullptr, tmp, p.stage_q.as<uint32_t>(), p.pqs.as<uint32_t>(),
+                                                   p.stage_rec_w.as<Rec32>(), p.records_w.as<Rec32>(),
+                                                   (size_t)std::max<uint64_t>(p.n_staged, 1), 0,
+                                                   bits_for(p.p_int), s));
+            else
+                MCZI_HIP(rocprim::radix_sort_pairs(nullptr, tmp, p.stage_q.as<uint32_t>(), p.pqs.as<uint32_t>(),
+                                                   p.stage_rec.as<ulonglong2>(), p.records.as<ulonglong2>(),
+                                                   (size_t)std::max<uint64_t>(p.n_staged, 1), 0,
+                                                   bits_for(p.p_int), s));
             p.part_tmp.ensure(tmp + 256);
             p.part_tmp_bytes = tmp;
         }
@@ -1319,7 +1676,8 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
             for (const DevBuf *db : {&b.words, &b.off, &b.dataset, &p.base, &p.cursor, &p.records, &p.ctr,
                                      &p.ovf_keys, &p.ovf_vals, &p.ovf_part, &p.out_kmer, &p.out_count,
                                      &p.out_part, &p.tile_start, &p.stage_rec, &p.stage_q, &p.pqs,
-                                     &p.part_tmp, &p.tile_nrec, &p.tile_base})
+                                     &p.part_tmp, &p.tile_nrec, &p.tile_base, &p.records_w, &p.stage_rec_w,
+                                     &p.ovf_hi, &p.out_hi})
                 sig = mix_sig(sig, (uint64_t)(uintptr_t)db->as<void>());
             sig = mix_sig(mix_sig(sig, p.ovf_slots), p.stage_cap);
             if (!p.graph || p.graph_sig != sig) {
@@ -1366,6 +1724,12 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
                                         hipMemcpyDeviceToDevice, s));
                 MCZI_HIP(hipMemcpyAsync(b.out_count.as<void>(), p.out_count.as<void>(), n_out * 4,
                                         hipMemcpyDeviceToDevice, s));
+            }
+            if (wide) {
+                b.out_hi.ensure(std::max<uint64_t>(n_out, 1) * 8);
+                if (n_out)
+                    MCZI_HIP(hipMemcpyAsync(b.out_hi.as<void>(), p.out_hi.as<void>(), n_out * 8,
+                                            hipMemcpyDeviceToDevice, s));
             }
         }
         tm.mark(s);
@@ -1434,8 +1798,13 @@ void batch_download(Batch &b, std::vector<mczi_counted_kmer> &out) {
                             d.stream()));
     MCZI_HIP(hipMemcpyAsync(cs.data(), b.out_count.as<void>(), b.n_out * 4, hipMemcpyDeviceToHost,
                             d.stream()));
+    std::vector<unsigned long long> hs;
+    if (b.k > 32) {
+        hs.resize(b.n_out);
+        MCZI_HIP(hipMemcpyAsync(hs.data(), b.out_hi.as<void>(), b.n_out * 8, hipMemcpyDeviceToHost, d.stream()));
+    }
     MCZI_HIP(hipStreamSynchronize(d.stream()));
-    for (uint64_t i = 0; i < b.n_out; i++) out[i] = mczi_counted_kmer{ks[i], 0, cs[i]};
+    for (uint64_t i = 0; i < b.n_out; i++) out[i] = mczi_counted_kmer{ks[i], hs.empty() ? 0 : hs[i], cs[i]};
 }
 
 Batch::~Batch() = default;

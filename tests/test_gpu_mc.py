@@ -65,7 +65,8 @@ def _reads(seed, genome_len=20000, n=2000, L=150, err=0.01):
 
 
 @pytest.mark.parametrize("k,m,x,P", [(31, 21, 1, 64), (31, 21, 5, 1024), (21, 11, 2, 7), (31, 19, 0, 3),
-                                     (15, 9, 3, 1024), (9, 5, 1, 16)])
+                                     (15, 9, 3, 1024), (9, 5, 1, 16),
+                                     (33, 17, 0, 1024), (41, 21, 1, 64), (45, 27, 3, 7), (63, 31, 2, 1024)])
 def test_count_inputs_matches_oracle(tmp_path, k, m, x, P):
     print(f"count_inputs k={k} m={m} x={x} P={P}", flush=True)
     reads = _reads(k + x)
@@ -387,3 +388,42 @@ def test_pieces_and_records_match_oracle_runs(k, m):
         pieces += len(bounds) - 1
         records += sum((bounds[i + 1] - bounds[i] + wrec - 1) // wrec for i in range(len(bounds) - 1))
     assert (int(st.n_pieces), int(st.n_records)) == (pieces, records)
+
+
+
+def test_wide_k_writers_and_spill(tmp_path, monkeypatch):
+    """33 <= k <= 63 (u128 k-mers, src/lib.rs:110-137): FASTA and KFF (8-byte
+    counts) bytes equal the oracle writers' over the brute-force counts; plain
+    and batched launches agree; with few, large partitions most k-mers spill
+    to the HBM table and the result is unchanged."""
+    reads = _reads(91, n=3000)
+    fa = tmp_path / "r.fa"
+    write_fasta(fa, reads)
+    for k, m, x in ((41, 21, 1), (63, 25, 0)):
+        codes, off = O.parse_fastx(str(fa), k)
+        want = O.count_bruteforce(codes, off, k, x)
+        assert np.any(want["hi"] != 0)
+        out = tmp_path / f"o{k}.fa"
+        M.count_inputs_to_fasta_path([fa], M.CounterConfig(k, m, x, 64), out)
+        O.write_simplitigs(str(tmp_path / f"w{k}.fa"), want, k)
+        assert out.read_bytes() == (tmp_path / f"w{k}.fa").read_bytes()
+        kff = tmp_path / f"o{k}.kff"
+        M.count_inputs_to_kff_path([fa], M.CounterConfig(k, m, x, 64), kff)
+        O.write_kff(str(tmp_path / f"w{k}.kff"), want, k, m, x, count_size=8, ordered=False)
+        assert kff.read_bytes() == (tmp_path / f"w{k}.kff").read_bytes()
+    r = M.SynthReads(M.SynthSpec(genome_len=200_000, genome_seed=9, read_seed=109, n_reads=40_000,
+                                 read_len=150, sub_rate=0.01))
+    want = O.count_bruteforce(r.codes(), r.frag_off, 47, 2)
+    b = M.Batch.from_packed(r.words, r.frag_off)
+    cfg = M.CounterConfig(47, 23, 2, 1024)
+    b.count(cfg)
+    assert same_counted(b.download(), want)
+    b.count(cfg, M.MODE_TWO_PASS, M.LAUNCH_PLAIN)
+    assert same_counted(b.download(), want)
+    monkeypatch.setenv("MCZI_PART_TARGET", str(10 ** 9))
+    b2 = M.Batch.from_packed(r.words, r.frag_off)
+    st = b2.count(cfg)
+    assert st.n_overflow > 0
+    assert same_counted(b2.download(), want)
+    with pytest.raises(M.MczaError, match="k <= 32"):
+        b2.count(cfg, M.MODE_SINGLE_PASS, M.LAUNCH_BATCHED, M.ORDER_KFF_U8)
