@@ -204,7 +204,7 @@ int mczi_count_inputs_to_fasta_path(const char *const *paths, size_t n, const mc
         const double t0 = now_s();
         auto v = count_inputs_impl(paths, n, cfg, MCZI_ORDER_ENCODED);
         const double t1 = now_s();
-        write_simplitig_fasta(out_path, v, cfg->k, 0, nullptr);
+        write_simplitig_fasta(out_path, v.data(), v.size(), cfg->k, 0, nullptr);
         log_phase("4_simplitig_output", now_s() - t1);
         log_phase("fasta_total", now_s() - t0);
     });
@@ -218,7 +218,7 @@ int mczi_count_datasets_to_fasta_path(const char *const *paths, size_t n,
         const double t0 = now_s();
         auto v = count_datasets_impl(paths, n, cfg);
         const double t1 = now_s();
-        write_simplitig_fasta(out_path, v, cfg->k, 0, nullptr);
+        write_simplitig_fasta(out_path, v.data(), v.size(), cfg->k, 0, nullptr);
         log_phase("3_output_streaming", now_s() - t1);
         log_phase("total", now_s() - t0);
     });
@@ -300,16 +300,16 @@ int mczi_count_datasets_to_kmer_fasta_path(const char *const *paths, size_t n,
 
 int mczi_write_fasta_path(const char *path, const mczi_counted_kmer *kmers, size_t n, uint32_t k) {
     return guarded("failed to write FASTA", [&] {
-        std::vector<mczi_counted_kmer> v(kmers, kmers + n);
-        write_simplitig_fasta(path, v, k, 0, nullptr);
+        require(kmers != nullptr || n == 0, MCZI_E_INVALID, "null k-mer array");
+        write_simplitig_fasta(path, kmers, n, k, 0, nullptr);
     });
 }
 
 int mczi_write_fasta_after_index(const char *path, const mczi_counted_kmer *kmers, size_t n, uint32_t k,
                                  size_t last_written_index, size_t *last_index) {
     return guarded("failed to write FASTA", [&] {
-        std::vector<mczi_counted_kmer> v(kmers, kmers + n);
-        write_simplitig_fasta(path, v, k, last_written_index, last_index);
+        require(kmers != nullptr || n == 0, MCZI_E_INVALID, "null k-mer array");
+        write_simplitig_fasta(path, kmers, n, k, last_written_index, last_index);
     });
 }
 

@@ -163,6 +163,57 @@ void Device::upload(void *dst, const void *src, size_t bytes, hipStream_t s) {
     host_release(stage[1]);
 }
 
+void Device::download(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return;
+    constexpr size_t kChunk = 64u << 20;
+    constexpr int kThreads = 8;
+    if (bytes <= (4u << 20)) {
+        MCZI_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        MCZI_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    char *stage[2] = {(char *)host_alloc(kChunk), (char *)host_alloc(kChunk)};
+    hipEvent_t ev[2];
+    MCZI_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    MCZI_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    auto drain = [&](int b, size_t off) {  // copy chunk `off` out of stage[b]
+        MCZI_HIP(hipEventSynchronize(ev[b]));
+        const size_t n = std::min(kChunk, bytes - off);
+        char *to = (char *)dst + off;
+        std::vector<std::thread> th;
+        const size_t per = (n + kThreads - 1) / kThreads;
+        for (int t = 0; t < kThreads; t++) {
+            const size_t a0 = (size_t)t * per;
+            if (a0 >= n) break;
+            const size_t len = std::min(per, n - a0);
+            th.emplace_back([=] { memcpy(to + a0, stage[b] + a0, len); });
+        }
+        for (auto &x : th) x.join();
+    };
+    try {
+        size_t i = 0;
+        for (size_t off = 0; off < bytes; off += kChunk, i++) {
+            const int b = (int)(i & 1);
+            const size_t n = std::min(kChunk, bytes - off);
+            MCZI_HIP(hipMemcpyAsync(stage[b], (const char *)src + off, n, hipMemcpyDeviceToHost, s));
+            MCZI_HIP(hipEventRecord(ev[b], s));
+            if (i > 0) drain(1 - b, off - kChunk);
+        }
+        drain((int)((i - 1) & 1), (i - 1) * kChunk);
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        host_release(stage[0]);
+        host_release(stage[1]);
+        throw;
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    host_release(stage[0]);
+    host_release(stage[1]);
+}
+
 void Device::host_release(void *p) {
     if (!p) return;
     std::lock_guard<std::mutex> lk(hmu_);

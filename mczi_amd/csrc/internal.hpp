@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <atomic>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -54,6 +55,10 @@ class Device {
     // several host threads into two alternating pinned staging blocks, each
     // sent by DMA while the next one fills. Synchronous on `s`.
     void upload(void *dst, const void *src, size_t bytes, hipStream_t s);
+    // Device -> host copy into pageable memory, the same way round: DMA into
+    // alternating pinned blocks, each copied out by several threads while the
+    // next one arrives. Synchronous on `s`.
+    void download(void *dst, const void *src, size_t bytes, hipStream_t s);
     std::mutex &call_mutex() { return call_mu_; }
 
   private:
@@ -186,14 +191,19 @@ void device_run_hashes(const uint8_t *packed, uint64_t n_bases, uint32_t k, uint
 
 // ---------------------------------------------------------------------------
 // Output writers (host end of the path).
-void write_simplitig_fasta(const std::string &path, const std::vector<mczi_counted_kmer> &kmers,
+void write_simplitig_fasta(const std::string &path, const mczi_counted_kmer *kmers, size_t n,
                            uint32_t k, size_t first_index, size_t *last_index);
 void write_kmer_fasta(const std::string &path, const std::vector<mczi_counted_kmer> &kmers,
                       uint32_t k);
 void write_kff(const std::string &path, const std::vector<mczi_counted_kmer> &kmers,
                const mczi_counter_config &cfg, uint64_t count_size, bool ordered,
                uint64_t saturate_at);
-std::string simplitig_fasta_text(const std::vector<mczi_counted_kmer> &kmers, uint32_t k);
+std::string simplitig_fasta_text(const mczi_counted_kmer *kmers, size_t n, uint32_t k);
+// Greedy simplitig cover (src/lib.rs:1478-1533) of `kmers` (any order,
+// duplicates collapse) as FASTA text handed to `put` in chunks; headers count
+// from first_index + 1. Returns the last index written.
+size_t emit_simplitigs(const mczi_counted_kmer *kmers, size_t n, uint32_t k, size_t first_index,
+                       const std::function<void(const char *, size_t)> &put);
 
 // Phase log lines on stderr, in the reference's formats (src/lib.rs:994-1062).
 void log_phase(const char *name, double seconds);

@@ -1128,6 +1128,20 @@ __global__ void k_gather_kmer(const unsigned long long *ks, const uint32_t *cs, 
     }
 }
 
+// Output records in the C ABI's layout (lo, hi, count as u64), so the host
+// receives them with one copy.
+__global__ void k_pack_counted(const unsigned long long *__restrict__ lo,
+                               const unsigned long long *__restrict__ hi,
+                               const uint32_t *__restrict__ cnt, unsigned long long *__restrict__ out,
+                               uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        out[3 * i] = lo[i];
+        out[3 * i + 1] = hi ? hi[i] : 0ull;
+        out[3 * i + 2] = cnt[i];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -1792,19 +1806,14 @@ void batch_download(Batch &b, std::vector<mczi_counted_kmer> &out) {
     Device &d = Device::get();
     out.resize(b.n_out);
     if (!b.n_out) return;
-    std::vector<unsigned long long> ks(b.n_out);
-    std::vector<uint32_t> cs(b.n_out);
-    MCZI_HIP(hipMemcpyAsync(ks.data(), b.out_kmer.as<void>(), b.n_out * 8, hipMemcpyDeviceToHost,
-                            d.stream()));
-    MCZI_HIP(hipMemcpyAsync(cs.data(), b.out_count.as<void>(), b.n_out * 4, hipMemcpyDeviceToHost,
-                            d.stream()));
-    std::vector<unsigned long long> hs;
-    if (b.k > 32) {
-        hs.resize(b.n_out);
-        MCZI_HIP(hipMemcpyAsync(hs.data(), b.out_hi.as<void>(), b.n_out * 8, hipMemcpyDeviceToHost, d.stream()));
-    }
-    MCZI_HIP(hipStreamSynchronize(d.stream()));
-    for (uint64_t i = 0; i < b.n_out; i++) out[i] = mczi_counted_kmer{ks[i], hs.empty() ? 0 : hs[i], cs[i]};
+    static_assert(sizeof(mczi_counted_kmer) == 24, "record layout");
+    DevBuf rec(b.n_out * sizeof(mczi_counted_kmer));
+    const unsigned g = (unsigned)std::min<uint64_t>((b.n_out + 255) / 256, (uint64_t)d.cu_count() * 16);
+    hipLaunchKernelGGL(k_pack_counted, dim3(g), dim3(256), 0, d.stream(), b.out_kmer.as<unsigned long long>(),
+                       b.k > 32 ? b.out_hi.as<unsigned long long>() : nullptr, b.out_count.as<uint32_t>(),
+                       rec.as<unsigned long long>(), b.n_out);
+    MCZI_HIP(hipGetLastError());
+    d.download(out.data(), rec.as<void>(), b.n_out * sizeof(mczi_counted_kmer), d.stream());
 }
 
 Batch::~Batch() = default;

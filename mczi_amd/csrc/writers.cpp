@@ -13,61 +13,6 @@ namespace {
 typedef unsigned __int128 u128;
 
 inline u128 enc_of(const mczi_counted_kmer &r) { return ((u128)r.hi << 64) | r.lo; }
-inline u128 kmask(uint32_t k) { return k >= 64 ? ~(u128)0 : (((u128)1 << (2 * k)) - 1); }
-
-inline uint64_t mix64(uint64_t v) {
-    v ^= v >> 33;
-    v *= 0xff51afd7ed558ccdull;
-    v ^= v >> 33;
-    v *= 0xc4ceb9fe1a85ec53ull;
-    return v ^ (v >> 33);
-}
-
-// Open-addressing set of 128-bit k-mers with removal (tombstones).
-class KmerSet {
-  public:
-    explicit KmerSet(size_t n) {
-        cap_ = 16;
-        while (cap_ < n * 2 + 16) cap_ <<= 1;
-        keys_.assign(cap_, 0);
-        st_.assign(cap_, 0);
-    }
-    void insert(u128 k) {
-        size_t i = slot(k);
-        size_t tomb = SIZE_MAX;
-        for (;;) {
-            if (st_[i] == 0) {
-                if (tomb != SIZE_MAX) i = tomb;
-                keys_[i] = k;
-                st_[i] = 1;
-                return;
-            }
-            if (st_[i] == 1 && keys_[i] == k) return;
-            if (st_[i] == 2 && tomb == SIZE_MAX) tomb = i;
-            i = (i + 1) & (cap_ - 1);
-        }
-    }
-    bool remove(u128 k) {
-        size_t i = slot(k);
-        for (;;) {
-            if (st_[i] == 0) return false;
-            if (st_[i] == 1 && keys_[i] == k) {
-                st_[i] = 2;
-                return true;
-            }
-            i = (i + 1) & (cap_ - 1);
-        }
-    }
-
-  private:
-    size_t slot(u128 k) const {
-        return (size_t)mix64((uint64_t)k ^ mix64((uint64_t)(k >> 64) + 0x9e3779b97f4a7c15ull)) &
-               (cap_ - 1);
-    }
-    size_t cap_;
-    std::vector<u128> keys_;
-    std::vector<uint8_t> st_;
-};
 
 const char kAscii[4] = {'A', 'C', 'G', 'T'};
 
@@ -126,79 +71,21 @@ void put_lines(Out &o, const std::vector<uint8_t> &seq) {
     }
 }
 
-// for_each_simplitig_encoded (src/lib.rs:1478-1533): seeds in ascending order,
-// greedy extension with A, C, G, T (canonical_suffix_extension, 1580-1602),
-// then the same on the reverse complement.
-void simplitigs(Out &o, const std::vector<mczi_counted_kmer> &kmers, uint32_t k, size_t first_index,
-                size_t *last_index) {
-    require(k > 0 && k <= 64, MCZI_E_INVALID, "k must be greater than 0 and <= 64");
-    size_t idx = first_index;
-    if (kmers.empty()) {
-        if (last_index) *last_index = idx;
-        return;
-    }
-    std::vector<u128> seeds(kmers.size());
-    for (size_t i = 0; i < kmers.size(); i++) seeds[i] = enc_of(kmers[i]);
-    if (!std::is_sorted(seeds.begin(), seeds.end())) std::sort(seeds.begin(), seeds.end());
-    seeds.erase(std::unique(seeds.begin(), seeds.end()), seeds.end());
-    KmerSet rem(seeds.size());
-    for (u128 s : seeds) rem.insert(s);
-    const u128 mk = kmask(k), mk1 = kmask(k - 1);
-    const int hs = 2 * (int)(k - 1);
-    std::vector<uint8_t> seq;
-    seq.reserve(1024);
-    char head[96];
-    for (u128 seed : seeds) {
-        if (!rem.remove(seed)) continue;
-        seq.clear();
-        for (uint32_t i = 0; i < k; i++) seq.push_back((uint8_t)((seed >> (2 * (k - 1 - i))) & 3u));
-        for (int pass = 0; pass < 2; pass++) {
-            // rolling forward / reverse-complement encodings of the last k-1 bases
-            u128 f = 0, r = 0;
-            for (size_t i = seq.size() - (k - 1); i < seq.size(); i++) {
-                f = ((f << 2) | seq[i]) & mk1;
-            }
-            for (size_t i = seq.size(); i-- > seq.size() - (k - 1);) r = (r << 2) | (3u - seq[i]);
-            for (;;) {
-                bool ext = false;
-                for (uint32_t b = 0; b < 4; b++) {
-                    const u128 fk = ((f << 2) | b) & mk;
-                    const u128 rk = ((u128)(3u - b) << hs) | r;
-                    const u128 c = fk < rk ? fk : rk;
-                    if (rem.remove(c)) {
-                        seq.push_back((uint8_t)b);
-                        f = fk & mk1;
-                        r = rk >> 2;
-                        ext = true;
-                        break;
-                    }
-                }
-                if (!ext) break;
-            }
-            std::reverse(seq.begin(), seq.end());
-            for (auto &c : seq) c = (uint8_t)(3u - c);
-        }
-        idx++;
-        int hl = snprintf(head, sizeof head, ">MC_simplitig_%zu kmers=%zu\n", idx, seq.size() - k + 1);
-        o.put(head, (size_t)hl);
-        put_lines(o, seq);
-    }
-    if (last_index) *last_index = idx;
-}
-
 }  // namespace
 
-void write_simplitig_fasta(const std::string &path, const std::vector<mczi_counted_kmer> &kmers,
+void write_simplitig_fasta(const std::string &path, const mczi_counted_kmer *kmers, size_t n,
                            uint32_t k, size_t first_index, size_t *last_index) {
     Out o(path);
-    simplitigs(o, kmers, k, first_index, last_index);
+    const size_t last =
+        emit_simplitigs(kmers, n, k, first_index, [&](const char *t, size_t len) { o.put(t, len); });
+    if (last_index) *last_index = last;
     o.finish();
 }
 
-std::string simplitig_fasta_text(const std::vector<mczi_counted_kmer> &kmers, uint32_t k) {
+std::string simplitig_fasta_text(const mczi_counted_kmer *kmers, size_t n, uint32_t k) {
     std::string s;
     Out o(&s);
-    simplitigs(o, kmers, k, 0, nullptr);
+    emit_simplitigs(kmers, n, k, 0, [&](const char *t, size_t len) { o.put(t, len); });
     return s;
 }
 

@@ -824,7 +824,10 @@ void mczi_zi_server::done_loop() {
         for (size_t i = 0; i < batch.size() && !err; i++) {
             batch[i]->absent = std::move(res.absent[i]);
             batch[i]->windows = res.windows[i];
-            if (batch[i]->make_fasta) batch[i]->fasta = simplitig_fasta_text(batch[i]->absent.counted(), idx.k);
+            if (batch[i]->make_fasta) {
+                const auto v = batch[i]->absent.counted();
+                batch[i]->fasta = simplitig_fasta_text(v.data(), v.size(), idx.k);
+            }
         }
         if (trace)
             fprintf(stderr, "ZI_TRACE done reqs=%zu wait_ms=%.2f copy_ms=%.2f post_ms=%.2f\n", batch.size(),
@@ -921,7 +924,7 @@ int mczi_zi_subtract(const char *index_path, const char *const *query_paths, siz
         auto t0 = std::chrono::steady_clock::now();
         auto v = zi_run(index_path, query_paths, n, k);
         auto t1 = std::chrono::steady_clock::now();
-        write_simplitig_fasta(out_path, v, k, 0, nullptr);
+        write_simplitig_fasta(out_path, v.data(), v.size(), k, 0, nullptr);
         auto t2 = std::chrono::steady_clock::now();
         fprintf(stderr, "ZI_PHASE\t2_query_subtraction\t%.6f\n", std::chrono::duration<double>(t1 - t0).count());
         fprintf(stderr, "ZI_PHASE\t3_simplitig_output\t%.6f\n", std::chrono::duration<double>(t2 - t1).count());

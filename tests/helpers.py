@@ -39,3 +39,27 @@ def write_fasta(path, seqs, name="r"):
 def same_counted(a, b) -> bool:
     return a.shape == b.shape and bool(np.all(a["lo"] == b["lo"]) and np.all(a["hi"] == b["hi"])
                                        and np.all(a["count"] == b["count"]))
+
+
+def canonical_kmers_np(codes, k):
+    """Canonical k-mers (A0 C1 G2 T3) of a code array as (hi, lo) uint64
+    arrays: the value is hi << 64 | lo. Vectorised over positions."""
+    codes = np.asarray(codes, dtype=np.uint64)
+    n = codes.size - k + 1
+    if n <= 0:
+        z = np.zeros(0, dtype=np.uint64)
+        return z, z
+
+    def val(get, j0, j1):
+        v = np.zeros(n, dtype=np.uint64)
+        for j in range(j0, j1):
+            v = (v << np.uint64(2)) | get(j)
+        return v
+
+    fwd = lambda j: codes[j:j + n]
+    rev = lambda j: np.uint64(3) - codes[k - 1 - j:k - 1 - j + n]
+    split = max(0, k - 32)
+    fh, fl = val(fwd, 0, split), val(fwd, split, k)
+    rh, rl = val(rev, 0, split), val(rev, split, k)
+    take_r = (rh < fh) | ((rh == fh) & (rl < fl))
+    return np.where(take_r, rh, fh), np.where(take_r, rl, fl)
