@@ -136,6 +136,12 @@ typedef struct mczi_batch mczi_batch;
 
 int mczi_batch_from_files(const char *const *paths, size_t n, uint32_t min_len,
                           int one_dataset_per_file, mczi_batch **out);
+/* Host-side ingest of one FASTA/FASTQ file into the packed layout above
+ * (for_fastx_packed_batches, src/lib.rs:3264-3308): fragments split at
+ * non-ACGT, shorter than min_len dropped. *words holds ceil(n_bases / 16)
+ * words and *frag_off n_frag + 1 offsets; both are released with mczi_free. */
+int mczi_pack_fastx(const char *path, uint32_t min_len, uint32_t **words, uint64_t **frag_off,
+                    size_t *n_frag);
 int mczi_batch_from_packed(const uint32_t *words, const uint64_t *frag_off, size_t n_frag,
                            const uint32_t *dataset, mczi_batch **out);
 int mczi_batch_info(const mczi_batch *b, uint64_t *n_bases, uint64_t *n_frag,

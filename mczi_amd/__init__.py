@@ -115,6 +115,8 @@ _sig("mczi_write_kff_path", C.c_int, [C.c_char_p, _P, C.c_size_t, C.POINTER(_Con
 _sig("mczi_decode_kmer", C.c_int, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_char_p])
 _sig("mczi_minimizer_run_hashes", C.c_int, [_P, C.c_uint64, C.c_uint32, C.c_uint32, _P, _P, C.POINTER(C.c_size_t)])
 _sig("mczi_batch_from_files", C.c_int, [_CP, C.c_size_t, C.c_uint32, C.c_int, C.POINTER(_P)])
+_sig("mczi_pack_fastx", C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.POINTER(C.c_uint32)),
+                                  C.POINTER(C.POINTER(C.c_uint64)), C.POINTER(C.c_size_t)])
 _sig("mczi_batch_from_packed", C.c_int, [_P, _P, C.c_size_t, _P, C.POINTER(_P)])
 _sig("mczi_batch_info", C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)])
 _sig("mczi_batch_free", None, [_P])
@@ -151,7 +153,7 @@ EXPORTED = [
     "mczi_count_datasets_to_fasta_path", "mczi_count_inputs_to_kff_path", "mczi_count_datasets_to_kff_path",
     "mczi_count_inputs_to_kmer_fasta_path", "mczi_count_datasets_to_kmer_fasta_path", "mczi_write_fasta_path",
     "mczi_write_fasta_after_index", "mczi_write_kff_path", "mczi_decode_kmer", "mczi_minimizer_run_hashes",
-    "mczi_batch_from_files", "mczi_batch_from_packed", "mczi_batch_info", "mczi_batch_free", "mczi_batch_count",
+    "mczi_batch_from_files", "mczi_pack_fastx", "mczi_batch_from_packed", "mczi_batch_info", "mczi_batch_free", "mczi_batch_count",
     "mczi_batch_download", "mczi_batch_autotune", "mczi_copy_probe", "mczi_device_info", "mczi_zi_subtract", "mczi_zi_absent", "mczi_zi_server_create",
     "mczi_zi_server_create_from_text", "mczi_zi_server_submit", "mczi_zi_server_submit_packed",
     "mczi_zi_server_poll", "mczi_zi_result_free",
@@ -279,6 +281,23 @@ def count_datasets_to_kmer_fasta_path(inputs, cfg: CounterConfig, out) -> int:
 
 def _counted_arr(kmers) -> np.ndarray:
     return np.ascontiguousarray(kmers, dtype=COUNTED)
+
+
+def pack_fastx(path, min_len: int = 1):
+    """Parse and 2-bit pack one FASTA/FASTQ file on the host: (words, frag_off)
+    as numpy arrays in the layout Batch.from_packed takes."""
+    wp = C.POINTER(C.c_uint32)()
+    op = C.POINTER(C.c_uint64)()
+    n = C.c_size_t()
+    _check(_lib.mczi_pack_fastx(os.fspath(path).encode(), min_len, C.byref(wp), C.byref(op), C.byref(n)))
+    try:
+        off = np.ctypeslib.as_array(op, shape=(n.value + 1,)).copy()
+        nw = int((off[-1] + 15) // 16)
+        words = np.ctypeslib.as_array(wp, shape=(nw,)).copy() if nw else np.zeros(0, np.uint32)
+    finally:
+        _lib.mczi_free(C.cast(wp, C.c_void_p))
+        _lib.mczi_free(C.cast(op, C.c_void_p))
+    return words, off
 
 
 def write_fasta_path(path, kmers, k: int) -> None:

@@ -134,6 +134,28 @@ const char *mczi_last_error(void) { return g_err.c_str(); }
 const char *mczi_version(void) { return "mczi-hip 0.1.0 (gfx950)"; }
 void mczi_free(void *p) { free(p); }
 
+int mczi_pack_fastx(const char *path, uint32_t min_len, uint32_t **words, uint64_t **frag_off,
+                    size_t *n_frag) {
+    return guarded("failed to read FASTA/FASTQ", [&] {
+        require(path && words && frag_off && n_frag, MCZI_E_INVALID, "null argument");
+        HostReads r;
+        parse_fastx_file(path, min_len, 0, r);
+        const size_t nw = (size_t)((r.n_bases() + 15) / 16);
+        *words = (uint32_t *)malloc(std::max<size_t>(nw, 1) * 4);
+        *frag_off = (uint64_t *)malloc(r.off.size() * 8);
+        if (!*words || !*frag_off) {
+            free(*words);
+            free(*frag_off);
+            *words = nullptr;
+            *frag_off = nullptr;
+            throw Error(MCZI_E_NOMEM, "out of host memory");
+        }
+        if (nw) memcpy(*words, r.words.data(), nw * 4);
+        memcpy(*frag_off, r.off.data(), r.off.size() * 8);
+        *n_frag = (size_t)r.n_frag();
+    });
+}
+
 int mczi_validate_config(const mczi_counter_config *cfg, int datasets) {
     return guarded("invalid configuration", [&] {
         require(cfg != nullptr, MCZI_E_INVALID, "null config");

@@ -4,7 +4,10 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <algorithm>
+#include <emmintrin.h>
 #include <cctype>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -282,6 +285,276 @@ struct Packer {
 };
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// FASTA on many threads, with the same fragments and bases as the one-pass
+// parser below. Pass 1 scans byte chunks on their own into fragment pieces: a
+// chunk starts inside a header line when a '>' follows the last '\n' before
+// it. The pieces are stitched across chunk ends and filtered by length; pass
+// 2 scans again and packs every kept base at its final position.
+namespace {
+
+enum : uint8_t { EV_NONE = 0, EV_BASE = 1, EV_CLOSE = 2 };
+constexpr uint64_t DROPPED = ~0ull;
+
+struct FastaChunk {
+    size_t a = 0, b = 0;
+    bool header = false;      // starts inside a header line
+    uint8_t first = EV_NONE;  // first base or fragment end seen in the chunk
+    bool open_end = false;    // the last piece runs on past the chunk end
+    uint64_t frag0 = 0, off0 = 0;  // global fragment of piece 0, its offset in it
+    std::vector<uint64_t> piece;   // piece lengths; later, output positions
+};
+
+// 16 bytes that are all of ACGTacgt.
+inline bool all_bases16(const char *p) {
+    const __m128i v = _mm_or_si128(_mm_loadu_si128((const __m128i *)p), _mm_set1_epi8(0x20));
+    const __m128i m = _mm_or_si128(_mm_or_si128(_mm_cmpeq_epi8(v, _mm_set1_epi8('a')), _mm_cmpeq_epi8(v, _mm_set1_epi8('c'))),
+                                   _mm_or_si128(_mm_cmpeq_epi8(v, _mm_set1_epi8('g')), _mm_cmpeq_epi8(v, _mm_set1_epi8('t'))));
+    return _mm_movemask_epi8(m) == 0xFFFF;
+}
+
+// Codes (A0 C1 T2 G3, which is (ascii >> 1) & 3 in both cases) of 8 bases,
+// base i at bits 2i.
+inline uint32_t codes8(const char *p) {
+    uint64_t v;
+    memcpy(&v, p, 8);
+    uint64_t x = (v >> 1) & 0x0303030303030303ull;
+    x = (x | (x >> 6)) & 0x000F000F000F000Full;
+    x = (x | (x >> 12)) & 0x000000FF000000FFull;
+    return (uint32_t)((x | (x >> 24)) & 0xFFFFu);
+}
+
+void fasta_scan(const char *buf, FastaChunk &c) {
+    bool hdr = c.header;
+    uint64_t cur = 0;
+    for (size_t i = c.a; i < c.b; i++) {
+        const char ch = buf[i];
+        if (hdr) {
+            if (ch == '\n') hdr = false;
+            continue;
+        }
+        const uint8_t code = kCodes.t[(uint8_t)ch];
+        if (code < 4) {
+            if (c.first == EV_NONE) c.first = EV_BASE;
+            cur++;
+            while (i + 17 <= c.b && all_bases16(buf + i + 1)) {
+                cur += 16;
+                i += 16;
+            }
+            continue;
+        }
+        if (code == 4) continue;
+        // '>' and every other character end the fragment
+        if (c.first == EV_NONE) c.first = EV_CLOSE;
+        if (cur) c.piece.push_back(cur);
+        cur = 0;
+        if (ch == '>') hdr = true;
+    }
+    c.open_end = cur > 0;
+    if (cur) c.piece.push_back(cur);
+}
+
+// Pack the chunk's kept bases; c.piece now holds each piece's output
+// position (or DROPPED). Output positions rise through the chunk, so each
+// word is gathered in one go: stored when all its 16 bases are this chunk's,
+// OR-ed atomically when it is shared with a neighbour.
+void fasta_pack(const char *buf, const FastaChunk &c, uint32_t *words) {
+    bool hdr = c.header, in_piece = false, keep = false;
+    size_t j = 0;
+    uint64_t pos = 0, wi = ~0ull;
+    uint32_t acc = 0;
+    int filled = 0;
+    auto flush = [&] {
+        if (wi == ~0ull) return;
+        if (filled == 16)
+            words[wi] = acc;
+        else
+            __atomic_fetch_or(&words[wi], acc, __ATOMIC_RELAXED);
+    };
+    // append n <= 8 codes (2 bits each) at pos
+    auto put = [&](uint32_t bits, int n) {
+        while (n > 0) {
+            if ((pos >> 4) != wi) {
+                flush();
+                wi = pos >> 4;
+                acc = 0;
+                filled = 0;
+            }
+            const int at = (int)(pos & 15u), take = std::min(n, 16 - at);
+            const uint32_t part = take == 16 ? bits : (bits & ((1u << (2 * take)) - 1u));
+            acc |= part << (2 * at);
+            filled += take;
+            pos += (uint64_t)take;
+            bits = take >= 16 ? 0 : bits >> (2 * take);
+            n -= take;
+        }
+    };
+    for (size_t i = c.a; i < c.b; i++) {
+        const char ch = buf[i];
+        if (hdr) {
+            if (ch == '\n') hdr = false;
+            continue;
+        }
+        const uint8_t code = kCodes.t[(uint8_t)ch];
+        if (code < 4) {
+            if (!in_piece) {
+                in_piece = true;
+                keep = c.piece[j] != DROPPED;
+                pos = c.piece[j];
+            }
+            if (keep) {
+                put(code, 1);
+                while (i + 17 <= c.b && all_bases16(buf + i + 1)) {
+                    put(codes8(buf + i + 1), 8);
+                    put(codes8(buf + i + 9), 8);
+                    i += 16;
+                }
+            } else {
+                while (i + 17 <= c.b && all_bases16(buf + i + 1)) i += 16;
+            }
+            continue;
+        }
+        if (code == 4) continue;
+        if (in_piece) j++;
+        in_piece = false;
+        if (ch == '>') hdr = true;
+    }
+    flush();
+}
+
+size_t parse_chunk_bytes(size_t len, unsigned nt) {
+    if (const char *e = getenv("MCZI_PARSE_CHUNK_BYTES")) {
+        const long long v = atoll(e);
+        if (v > 0) return (size_t)v;
+    }
+    return std::max<size_t>(size_t(4) << 20, len / (4 * (size_t)nt) + 1);
+}
+
+// buf[0] is the first record's '>'.
+void parse_fasta_parallel(const char *buf, size_t len, uint64_t min_len, uint32_t dataset,
+                          HostReads &out, size_t chunk) {
+    const unsigned nt = host_workers(len);
+    std::vector<FastaChunk> ch((len + chunk - 1) / chunk);
+    for (size_t c = 0; c < ch.size(); c++) {
+        ch[c].a = c * chunk;
+        ch[c].b = std::min(len, (c + 1) * chunk);
+    }
+    parallel_ranges(ch.size(), 1, nt, [&](size_t c0, size_t c1) {
+        for (size_t c = c0; c < c1; c++) {
+            FastaChunk &k = ch[c];
+            if (k.a > 0) {
+                const char *nl = (const char *)memrchr(buf, '\n', k.a);
+                const size_t from = nl ? (size_t)(nl - buf) + 1 : 0;
+                k.header = from < k.a && memchr(buf + from, '>', k.a - from) != nullptr;
+            }
+            fasta_scan(buf, k);
+        }
+    });
+    // stitch pieces into fragments
+    size_t n_piece = 0;
+    for (auto &k : ch) n_piece += k.piece.size();
+    std::vector<uint64_t> flen;
+    flen.reserve(n_piece);
+    out.off.reserve(out.off.size() + n_piece);
+    out.dataset.reserve(out.dataset.size() + n_piece);
+    bool open = false;
+    for (auto &k : ch) {
+        if (k.piece.empty()) {
+            if (k.first == EV_CLOSE) open = false;
+            continue;
+        }
+        size_t j0 = 0;
+        if (open && k.first == EV_BASE) {
+            k.frag0 = flen.size() - 1;
+            k.off0 = flen.back();
+            flen.back() += k.piece[0];
+            j0 = 1;
+        } else {
+            k.frag0 = flen.size();
+        }
+        for (size_t j = j0; j < k.piece.size(); j++) flen.push_back(k.piece[j]);
+        open = k.open_end;
+    }
+    // keep fragments of at least min_len bases, placed after out.cur_:
+    // flen[f] becomes the fragment's output position (or DROPPED)
+    const size_t nf = flen.size();
+    const size_t nblk = std::max<size_t>(1, std::min<size_t>(4 * nt, nf / 65536 + 1));
+    const size_t bs = (nf + nblk - 1) / nblk;
+    std::vector<uint64_t> blen(nblk + 1, 0), bcnt(nblk + 1, 0);
+    parallel_ranges(nblk, 1, nt, [&](size_t b0, size_t b1) {
+        for (size_t b = b0; b < b1; b++) {
+            uint64_t l = 0, c = 0;
+            for (size_t f = b * bs; f < std::min(nf, (b + 1) * bs); f++)
+                if (flen[f] >= min_len) {
+                    l += flen[f];
+                    c++;
+                }
+            blen[b + 1] = l;
+            bcnt[b + 1] = c;
+        }
+    });
+    for (size_t b = 0; b < nblk; b++) {
+        blen[b + 1] += blen[b];
+        bcnt[b + 1] += bcnt[b];
+    }
+    const uint64_t start = out.cur_, at = start + blen[nblk];
+    const size_t off0 = out.off.size();
+    out.off.resize(off0 + bcnt[nblk]);
+    out.dataset.resize(out.dataset.size() + bcnt[nblk], dataset);
+    parallel_ranges(nblk, 1, nt, [&](size_t b0, size_t b1) {
+        for (size_t b = b0; b < b1; b++) {
+            uint64_t p = start + blen[b];
+            size_t o = off0 + bcnt[b];
+            for (size_t f = b * bs; f < std::min(nf, (b + 1) * bs); f++) {
+                if (flen[f] >= min_len) {
+                    const uint64_t l = flen[f];
+                    flen[f] = p;
+                    p += l;
+                    out.off[o++] = p;
+                } else {
+                    flen[f] = DROPPED;
+                }
+            }
+        }
+    });
+    // pieces: output positions, and each chunk's output range
+    std::vector<uint64_t> lo(ch.size(), DROPPED), hi(ch.size(), 0);
+    parallel_ranges(ch.size(), 1, nt, [&](size_t c0, size_t c1) {
+        for (size_t c = c0; c < c1; c++) {
+            FastaChunk &k = ch[c];
+            for (size_t j = 0; j < k.piece.size(); j++) {
+                const uint64_t f = flen[k.frag0 + j], len = k.piece[j];
+                if (f == DROPPED) {
+                    k.piece[j] = DROPPED;
+                    continue;
+                }
+                const uint64_t p = f + (j == 0 ? k.off0 : 0);
+                k.piece[j] = p;
+                if (lo[c] == DROPPED) lo[c] = p;
+                hi[c] = p + len;
+            }
+        }
+    });
+    std::vector<uint64_t>().swap(flen);
+    // new words start uninitialised: words a chunk fills alone are stored
+    // whole; the (new) words at each chunk's two ends are OR-ed into, so
+    // they start at zero
+    const size_t old_words = out.words.size();
+    out.words.resize((at + 15) / 16);
+    uint32_t *words = out.words.data();
+    for (size_t c = 0; c < ch.size(); c++) {
+        if (lo[c] == DROPPED) continue;
+        for (const uint64_t w : {lo[c] >> 4, (hi[c] - 1) >> 4})
+            if (w >= old_words) words[w] = 0;
+    }
+    parallel_ranges(ch.size(), 1, nt, [&](size_t c0, size_t c1) {
+        for (size_t c = c0; c < c1; c++) fasta_pack(buf, ch[c], words);
+    });
+    out.cur_ = at;
+}
+
+}  // namespace
+
 static void parse_buffer(const char *buf, size_t len, uint64_t min_len, uint32_t dataset,
                          HostReads &out, const std::string &what) {
     size_t i = 0;
@@ -291,6 +564,13 @@ static void parse_buffer(const char *buf, size_t len, uint64_t min_len, uint32_t
         fprintf(stderr, "MC_WARN\tskipped_invalid_fastx_input\tpath\t%s\terror\t%s\n", what.c_str(),
                 "Invalid record start");
         return;
+    }
+    if (buf[i] == '>') {
+        const size_t chunk = parse_chunk_bytes(len - i, host_workers(len - i));
+        if (len - i > chunk) {
+            parse_fasta_parallel(buf + i, len - i, min_len, dataset, out, chunk);
+            return;
+        }
     }
     Packer pk(out, min_len, dataset);
     const bool fastq = buf[i] == '@';

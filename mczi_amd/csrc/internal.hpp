@@ -12,6 +12,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mczi.h"
@@ -33,6 +34,37 @@ struct Error : std::runtime_error {
 
 inline void require(bool ok, int code, const std::string &msg) {
     if (!ok) throw Error(code, msg);
+}
+
+// ---------------------------------------------------------------------------
+// Host worker threads: up to 16 (the host cores one GPU gets), one for small
+// jobs.
+inline unsigned host_workers(size_t n) {
+    if (n < (size_t(1) << 16)) return 1;
+    const unsigned hc = std::thread::hardware_concurrency();
+    return hc == 0 ? 1u : (hc < 16u ? hc : 16u);
+}
+
+// f(begin, end) over [0, n) in chunks of `grain`, on up to nt threads.
+template <class F> void parallel_ranges(size_t n, size_t grain, unsigned nt, F &&f) {
+    std::atomic<size_t> next{0};
+    auto run = [&] {
+        for (;;) {
+            const size_t b = next.fetch_add(grain, std::memory_order_relaxed);
+            if (b >= n) return;
+            f(b, b + grain < n ? b + grain : n);
+        }
+    };
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt && grain < n; t++) {
+        try {
+            th.emplace_back(run);
+        } catch (...) {
+            break;  // fewer workers; the chunks still all get done
+        }
+    }
+    run();
+    for (auto &t : th) t.join();
 }
 
 // ---------------------------------------------------------------------------
@@ -107,9 +139,20 @@ class DevBuf {
 };
 
 // ---------------------------------------------------------------------------
+// Allocator whose resize(n) leaves new elements uninitialised (resize(n, v)
+// still fills): for large buffers that are written in full right after.
+template <class T> struct NoInitAlloc : std::allocator<T> {
+    template <class U> struct rebind { using other = NoInitAlloc<U>; };
+    NoInitAlloc() = default;
+    template <class U> NoInitAlloc(const NoInitAlloc<U> &) {}
+    template <class U, class... A> void construct(U *p, A &&...a) { ::new ((void *)p) U(std::forward<A>(a)...); }
+    template <class U> void construct(U *p) { ::new ((void *)p) U; }
+};
+
+// ---------------------------------------------------------------------------
 // Packed reads in host memory: 16 bases per u32 word, fragments back to back.
 struct HostReads {
-    std::vector<uint32_t> words;     // padded with zero words at the end
+    std::vector<uint32_t, NoInitAlloc<uint32_t>> words;  // padded with zero words at the end
     std::vector<uint64_t> off{0};    // fragment boundaries (n_frag + 1)
     std::vector<uint32_t> dataset;   // dataset of each fragment (may be empty)
     std::vector<uint64_t> chunk_frag{0}; // fragment index where each input chunk starts

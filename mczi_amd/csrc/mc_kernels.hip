@@ -1217,10 +1217,8 @@ std::unique_ptr<Batch> make_batch(HostReads &&r) {
     b->n_datasets = r.n_datasets;
     b->words.reset(r.words.size() * 4);
     b->off.reset(r.off.size() * 8);
-    MCZI_HIP(hipMemcpyAsync(b->words.as<void>(), r.words.data(), r.words.size() * 4,
-                            hipMemcpyHostToDevice, d.stream()));
-    MCZI_HIP(hipMemcpyAsync(b->off.as<void>(), r.off.data(), r.off.size() * 8,
-                            hipMemcpyHostToDevice, d.stream()));
+    d.upload(b->words.as<void>(), r.words.data(), r.words.size() * 4, d.stream());
+    d.upload(b->off.as<void>(), r.off.data(), r.off.size() * 8, d.stream());
     if (!r.dataset.empty()) {
         b->has_dataset = true;
         b->dataset.reset(r.dataset.size() * 4);

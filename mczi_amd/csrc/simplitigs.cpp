@@ -96,34 +96,6 @@ template <class T> struct HugeAlloc {
 };
 template <class T> using big_vector = std::vector<T, HugeAlloc<T>>;
 
-unsigned worker_count(size_t n) {
-    if (n < (size_t(1) << 16)) return 1;
-    const unsigned hc = std::thread::hardware_concurrency();
-    return std::max(1u, std::min(hc ? hc : 1u, 16u));
-}
-
-// f(begin, end) over [0, n) in chunks of `grain`, on up to nt threads.
-template <class F> void parallel_ranges(size_t n, size_t grain, unsigned nt, F &&f) {
-    std::atomic<size_t> next{0};
-    auto run = [&] {
-        for (;;) {
-            const size_t b = next.fetch_add(grain, std::memory_order_relaxed);
-            if (b >= n) return;
-            f(b, std::min(n, b + grain));
-        }
-    };
-    std::vector<std::thread> th;
-    for (unsigned t = 1; t < nt && grain < n; t++) {
-        try {
-            th.emplace_back(run);
-        } catch (...) {
-            break;  // fewer workers; the chunks still all get done
-        }
-    }
-    run();
-    for (auto &t : th) t.join();
-}
-
 inline uint64_t ld(const uint64_t *p) { return __atomic_load_n(p, __ATOMIC_RELAXED); }
 inline void st(uint64_t *p, uint64_t v) { __atomic_store_n(p, v, __ATOMIC_RELAXED); }
 inline bool bit(const uint64_t *w, size_t i) { return (w[i >> 6] >> (i & 63)) & 1; }
@@ -196,7 +168,7 @@ template <class T> class SimplitigCover {
     SimplitigCover(const mczi_counted_kmer *kmers, size_t n, uint32_t k)
         : k_(k), mask_(Kmer<T>::mask(k)), hs_(2 * (int)k - 2) {
         keys_.resize(n);
-        nt_ = worker_count(n);
+        nt_ = host_workers(n);
         parallel_ranges(n, 1 << 16, nt_, [&](size_t b0, size_t b1) {
             for (size_t i = b0; i < b1; i++) keys_[i] = Kmer<T>::of(kmers[i]) & mask_;
         });
@@ -205,7 +177,7 @@ template <class T> class SimplitigCover {
         n_ = keys_.size();
         require(n_ < (size_t)NONE, MCZI_E_INVALID,
                 "too many k-mers for the simplitig writer (at most 2^32 - 2)");
-        nt_ = worker_count(n_);
+        nt_ = host_workers(n_);
     }
 
     size_t run(size_t first_index, const std::function<void(const char *, size_t)> &put) {

@@ -1012,7 +1012,7 @@ int mczi_zi_server_submit(mczi_zi_server *s, const char *fastx, size_t len, int 
         parse_fastx_text(fastx, len, s->idx.k, 0, r->reads);
         r->reads.finish();
         r->pin_words(r->reads.words.data(), (r->reads.n_bases() + 15) / 16);
-        std::vector<uint32_t>().swap(r->reads.words);
+        decltype(r->reads.words)().swap(r->reads.words);
         r->make_fasta = make_fasta != 0;
         std::lock_guard<std::mutex> lk(s->mu);
         require(s->fatal.empty(), MCZI_E_DEVICE, s->fatal);
