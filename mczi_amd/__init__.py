@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -174,11 +175,16 @@ def _paths(paths) -> tuple:
 
 
 def _take(ptr, n) -> np.ndarray:
-    out = np.zeros(n, dtype=COUNTED)
-    if n:
-        C.memmove(out.ctypes.data, ptr, n * COUNTED.itemsize)
-    _lib.mczi_free(C.cast(ptr, C.c_void_p))
-    return out
+    """The library's malloc'd record array as a numpy array without a copy;
+    it is released with mczi_free when the last view of it is gone."""
+    addr = C.cast(ptr, C.c_void_p).value
+    if not n or not addr:
+        if addr:
+            _lib.mczi_free(C.c_void_p(addr))
+        return np.zeros(0, dtype=COUNTED)
+    raw = np.ctypeslib.as_array(C.cast(addr, C.POINTER(C.c_uint8)), shape=(n * COUNTED.itemsize,))
+    weakref.finalize(raw, _lib.mczi_free, C.c_void_p(addr))
+    return raw.view(COUNTED)
 
 
 def copy_probe(nbytes: int, reps: int = 10) -> float:

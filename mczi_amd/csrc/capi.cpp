@@ -218,16 +218,31 @@ int mczi_count_datasets(const char *const *paths, size_t n, const mczi_counter_c
     });
 }
 
+// Count, then hand only the sorted k-mer words to the simplitig writer (the
+// counts are not written to simplitig FASTA).
+void count_to_simplitigs(const std::vector<std::string> &paths, const mczi_counter_config &cfg, int mode,
+                         const char *out_path, const char *count_phase) {
+    const double t0 = now_s();
+    auto b = load_inputs(paths, cfg.k, mode == MCZI_MODE_DATASETS);
+    if (b->n_frag) batch_count(*b, cfg, mode, MCZI_LAUNCH_BATCHED, MCZI_ORDER_ENCODED, nullptr);
+    log_phase(count_phase, now_s() - t0);
+    const size_t n_out = b->n_frag ? (size_t)b->n_out : 0;
+    const double t1 = now_s();
+    write_simplitig_fasta_keys(out_path, n_out, cfg.k, [&](uint64_t *lo, uint64_t *hi) {
+        batch_download_keys(*b, lo, hi);
+    });
+    log_phase(mode == MCZI_MODE_DATASETS ? "3_output_streaming" : "4_simplitig_output", now_s() - t1);
+}
+
 int mczi_count_inputs_to_fasta_path(const char *const *paths, size_t n, const mczi_counter_config *cfg,
                                     const char *out_path) {
     return guarded("failed to write simplitig FASTA", [&] {
-        require(out_path != nullptr, MCZI_E_INVALID, "null output path");
+        require(out_path != nullptr && cfg != nullptr, MCZI_E_INVALID, "null argument");
+        validate_normal(*cfg);
+        require(n > 0, MCZI_E_INVALID, "at least one input file is required");
         std::lock_guard<std::mutex> lk(Device::get().call_mutex());
         const double t0 = now_s();
-        auto v = count_inputs_impl(paths, n, cfg, MCZI_ORDER_ENCODED);
-        const double t1 = now_s();
-        write_simplitig_fasta(out_path, v.data(), v.size(), cfg->k, 0, nullptr);
-        log_phase("4_simplitig_output", now_s() - t1);
+        count_to_simplitigs(to_paths(paths, n), *cfg, MCZI_MODE_TWO_PASS, out_path, "3_kmer_counting_and_output");
         log_phase("fasta_total", now_s() - t0);
     });
 }
@@ -235,13 +250,20 @@ int mczi_count_inputs_to_fasta_path(const char *const *paths, size_t n, const mc
 int mczi_count_datasets_to_fasta_path(const char *const *paths, size_t n,
                                       const mczi_counter_config *cfg, const char *out_path) {
     return guarded("failed to write simplitig FASTA", [&] {
-        require(out_path != nullptr, MCZI_E_INVALID, "null output path");
+        require(out_path != nullptr && cfg != nullptr, MCZI_E_INVALID, "null argument");
+        validate_datasets(*cfg);
+        require(n > 0, MCZI_E_INVALID, "at least one dataset file is required");
         std::lock_guard<std::mutex> lk(Device::get().call_mutex());
         const double t0 = now_s();
-        auto v = count_datasets_impl(paths, n, cfg);
-        const double t1 = now_s();
-        write_simplitig_fasta(out_path, v.data(), v.size(), cfg->k, 0, nullptr);
-        log_phase("3_output_streaming", now_s() - t1);
+        if (cfg->threshold >= n) {  // src/lib.rs:644-648
+            log_phase("1_dataset_minimizer_counting", 0);
+            log_phase("2_dataset_kmer_counting", 0);
+            const double t1 = now_s();
+            write_simplitig_fasta_keys(out_path, 0, cfg->k, [](uint64_t *, uint64_t *) {});
+            log_phase("3_output_streaming", now_s() - t1);
+        } else {
+            count_to_simplitigs(to_paths(paths, n), *cfg, MCZI_MODE_DATASETS, out_path, "2_dataset_kmer_counting");
+        }
         log_phase("total", now_s() - t0);
     });
 }
@@ -496,10 +518,19 @@ int mczi_batch_autotune(mczi_batch *b, const mczi_counter_config *cfg, int mode,
 int mczi_batch_download(mczi_batch *b, mczi_counted_kmer **out, size_t *n_out) {
     return guarded("failed to download results", [&] {
         require(b != nullptr, MCZI_E_INVALID, "null batch");
+        require(out && n_out, MCZI_E_INVALID, "null output pointer");
         std::lock_guard<std::mutex> lk(Device::get().call_mutex());
-        std::vector<mczi_counted_kmer> v;
-        batch_download(*b->b, v);
-        copy_out(v, out, n_out);
+        const size_t n = (size_t)b->b->n_out;
+        mczi_counted_kmer *p = (mczi_counted_kmer *)malloc(std::max<size_t>(n, 1) * sizeof(mczi_counted_kmer));
+        require(p != nullptr, MCZI_E_NOMEM, "out of host memory");
+        try {
+            batch_download(*b->b, p);
+        } catch (...) {
+            free(p);
+            throw;
+        }
+        *out = p;
+        *n_out = n;
     });
 }
 

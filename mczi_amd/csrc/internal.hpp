@@ -222,6 +222,9 @@ std::unique_ptr<Batch> make_batch_packed(const uint32_t *words, const uint64_t *
 void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch, int order,
                  mczi_run_stats *stats);
 void batch_download(Batch &b, std::vector<mczi_counted_kmer> &out);
+void batch_download(Batch &b, mczi_counted_kmer *out);  // b.n_out records
+// Only the emitted k-mers' low (and, for k > 32, high) words.
+void batch_download_keys(Batch &b, uint64_t *lo, uint64_t *hi);
 double batch_autotune(Batch &b, const mczi_counter_config &cfg, int mode, int launch, int order);
 void free_plan(McPlan *p);
 
@@ -247,6 +250,12 @@ std::string simplitig_fasta_text(const mczi_counted_kmer *kmers, size_t n, uint3
 // from first_index + 1. Returns the last index written.
 size_t emit_simplitigs(const mczi_counted_kmer *kmers, size_t n, uint32_t k, size_t first_index,
                        const std::function<void(const char *, size_t)> &put);
+// The same from n keys that fill(lo, hi) writes as 64-bit words (hi is null
+// for k <= 32).
+size_t emit_simplitigs_keys(size_t n, uint32_t k, const std::function<void(uint64_t *, uint64_t *)> &fill,
+                            size_t first_index, const std::function<void(const char *, size_t)> &put);
+void write_simplitig_fasta_keys(const std::string &path, size_t n, uint32_t k,
+                                const std::function<void(uint64_t *, uint64_t *)> &fill);
 
 // Phase log lines on stderr, in the reference's formats (src/lib.rs:994-1062).
 void log_phase(const char *name, double seconds);

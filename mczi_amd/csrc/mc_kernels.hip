@@ -1801,8 +1801,12 @@ double batch_autotune(Batch &b, const mczi_counter_config &cfg, int mode, int la
 }
 
 void batch_download(Batch &b, std::vector<mczi_counted_kmer> &out) {
-    Device &d = Device::get();
     out.resize(b.n_out);
+    batch_download(b, out.data());
+}
+
+void batch_download(Batch &b, mczi_counted_kmer *out) {
+    Device &d = Device::get();
     if (!b.n_out) return;
     static_assert(sizeof(mczi_counted_kmer) == 24, "record layout");
     DevBuf rec(b.n_out * sizeof(mczi_counted_kmer));
@@ -1811,7 +1815,19 @@ void batch_download(Batch &b, std::vector<mczi_counted_kmer> &out) {
                        b.k > 32 ? b.out_hi.as<unsigned long long>() : nullptr, b.out_count.as<uint32_t>(),
                        rec.as<unsigned long long>(), b.n_out);
     MCZI_HIP(hipGetLastError());
-    d.download(out.data(), rec.as<void>(), b.n_out * sizeof(mczi_counted_kmer), d.stream());
+    d.download(out, rec.as<void>(), b.n_out * sizeof(mczi_counted_kmer), d.stream());
+}
+
+void batch_download_keys(Batch &b, uint64_t *lo, uint64_t *hi) {
+    if (!b.n_out) return;
+    Device &d = Device::get();
+    d.download(lo, b.out_kmer.as<void>(), b.n_out * 8, d.stream());
+    if (hi) {
+        if (b.k > 32)
+            d.download(hi, b.out_hi.as<void>(), b.n_out * 8, d.stream());
+        else
+            memset(hi, 0, b.n_out * 8);
+    }
 }
 
 Batch::~Batch() = default;

@@ -91,6 +91,10 @@ template <class T> struct HugeAlloc {
         else
             munmap(p, bytes);
     }
+    // resize(n) leaves new elements uninitialised: every array here is
+    // written in full, or created with an explicit fill value
+    template <class U> void construct(U *p) { ::new ((void *)p) U; }
+    template <class U, class... A> void construct(U *p, A &&...a) { ::new ((void *)p) U(std::forward<A>(a)...); }
     template <class U> bool operator==(const HugeAlloc<U> &) const { return true; }
     template <class U> bool operator!=(const HugeAlloc<U> &) const { return false; }
 };
@@ -172,7 +176,42 @@ template <class T> class SimplitigCover {
         parallel_ranges(n, 1 << 16, nt_, [&](size_t b0, size_t b1) {
             for (size_t i = b0; i < b1; i++) keys_[i] = Kmer<T>::of(kmers[i]) & mask_;
         });
-        if (!std::is_sorted(keys_.begin(), keys_.end())) std::sort(keys_.begin(), keys_.end());
+        finish_keys();
+    }
+    // Keys as low / high 64-bit words, written by fill(lo, hi) (hi only for
+    // k > 32).
+    SimplitigCover(size_t n, uint32_t k, const std::function<void(uint64_t *, uint64_t *)> &fill)
+        : k_(k), mask_(Kmer<T>::mask(k)), hs_(2 * (int)k - 2) {
+        nt_ = host_workers(n);
+        keys_.resize(n);
+        if (sizeof(T) == 8) {
+            fill(reinterpret_cast<uint64_t *>(keys_.data()), nullptr);
+            parallel_ranges(n, 1 << 16, nt_, [&](size_t b0, size_t b1) {
+                for (size_t i = b0; i < b1; i++) keys_[i] &= mask_;
+            });
+        } else {
+            big_vector<uint64_t> lo(n), hi(n);
+            fill(lo.data(), hi.data());
+            parallel_ranges(n, 1 << 16, nt_, [&](size_t b0, size_t b1) {
+                for (size_t i = b0; i < b1; i++) keys_[i] = (T)((((u128)hi[i] << 64) | lo[i]) & mask_);
+            });
+        }
+        finish_keys();
+    }
+
+  private:
+    // Seeds are the keys in ascending order, each once (the set semantics of
+    // the reference's HashSet).
+    void finish_keys() {
+        std::atomic<bool> sorted{true};
+        parallel_ranges(keys_.size(), 1 << 20, nt_, [&](size_t b0, size_t b1) {
+            for (size_t i = std::max<size_t>(b0, 1); i < b1; i++)
+                if (keys_[i] < keys_[i - 1]) {
+                    sorted = false;
+                    return;
+                }
+        });
+        if (!sorted) std::sort(keys_.begin(), keys_.end());
         keys_.erase(std::unique(keys_.begin(), keys_.end()), keys_.end());
         n_ = keys_.size();
         require(n_ < (size_t)NONE, MCZI_E_INVALID,
@@ -180,6 +219,7 @@ template <class T> class SimplitigCover {
         nt_ = host_workers(n_);
     }
 
+  public:
     size_t run(size_t first_index, const std::function<void(const char *, size_t)> &put) {
         if (n_ == 0) return first_index;
         const bool trace = getenv("MCZI_WRITER_TRACE") != nullptr;
@@ -685,6 +725,17 @@ template <class T> class SimplitigCover {
 };
 
 }  // namespace
+
+size_t emit_simplitigs_keys(size_t n, uint32_t k, const std::function<void(uint64_t *, uint64_t *)> &fill,
+                            size_t first_index, const std::function<void(const char *, size_t)> &put) {
+    require(k > 0 && k <= 64, MCZI_E_INVALID, "k must be greater than 0 and <= 64");
+    if (k <= 32) {
+        SimplitigCover<uint64_t> c(n, k, fill);
+        return c.run(first_index, put);
+    }
+    SimplitigCover<u128> c(n, k, fill);
+    return c.run(first_index, put);
+}
 
 size_t emit_simplitigs(const mczi_counted_kmer *kmers, size_t n, uint32_t k, size_t first_index,
                        const std::function<void(const char *, size_t)> &put) {

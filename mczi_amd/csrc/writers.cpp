@@ -82,6 +82,13 @@ void write_simplitig_fasta(const std::string &path, const mczi_counted_kmer *kme
     o.finish();
 }
 
+void write_simplitig_fasta_keys(const std::string &path, size_t n, uint32_t k,
+                                const std::function<void(uint64_t *, uint64_t *)> &fill) {
+    Out o(path);
+    emit_simplitigs_keys(n, k, fill, 0, [&](const char *t, size_t len) { o.put(t, len); });
+    o.finish();
+}
+
 std::string simplitig_fasta_text(const mczi_counted_kmer *kmers, size_t n, uint32_t k) {
     std::string s;
     Out o(&s);

@@ -177,6 +177,13 @@ def test_datasets_match_oracle(tmp_path, golden):
         got = M.count_datasets(paths, M.CounterConfig(31, 21, x, 64))
         want = O.count_datasets_bruteforce(codes, off, dsa, 7, 31, x)
         assert same_counted(got, want), x
+    # the dataset simplitig path (keys only to the writer); x >= datasets writes nothing
+    for x in (2, 7):
+        out = tmp_path / f"ds{x}.fa"
+        M.count_datasets_to_fasta_path(paths, M.CounterConfig(31, 21, x, 64), out)
+        want = O.count_datasets_bruteforce(codes, off, dsa, 7, 31, x)
+        O.write_simplitigs(str(tmp_path / "w.fa"), want, 31)
+        assert out.read_bytes() == (tmp_path / "w.fa").read_bytes(), x
 
 
 def test_batch_plain_equals_batched_and_spill(monkeypatch):
