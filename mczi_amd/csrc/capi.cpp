@@ -228,9 +228,12 @@ void count_to_simplitigs(const std::vector<std::string> &paths, const mczi_count
     log_phase(count_phase, now_s() - t0);
     const size_t n_out = b->n_frag ? (size_t)b->n_out : 0;
     const double t1 = now_s();
-    write_simplitig_fasta_keys(out_path, n_out, cfg.k, [&](uint64_t *lo, uint64_t *hi) {
-        batch_download_keys(*b, lo, hi);
-    });
+    write_simplitig_fasta_keys(
+        out_path, n_out, cfg.k, [&](uint64_t *lo, uint64_t *hi) { batch_download_keys(*b, lo, hi); },
+        [&](uint64_t *link) {
+            device_forced_links(b->out_kmer.as<unsigned long long>(),
+                                cfg.k > 32 ? b->out_hi.as<unsigned long long>() : nullptr, n_out, cfg.k, link);
+        });
     log_phase(mode == MCZI_MODE_DATASETS ? "3_output_streaming" : "4_simplitig_output", now_s() - t1);
 }
 

@@ -252,10 +252,18 @@ size_t emit_simplitigs(const mczi_counted_kmer *kmers, size_t n, uint32_t k, siz
                        const std::function<void(const char *, size_t)> &put);
 // The same from n keys that fill(lo, hi) writes as 64-bit words (hi is null
 // for k <= 32).
+// link_fill (optional) supplies the forced-link words instead of the host
+// neighbour pass: device_forced_links for keys resident in HBM.
 size_t emit_simplitigs_keys(size_t n, uint32_t k, const std::function<void(uint64_t *, uint64_t *)> &fill,
-                            size_t first_index, const std::function<void(const char *, size_t)> &put);
+                            size_t first_index, const std::function<void(const char *, size_t)> &put,
+                            const std::function<void(uint64_t *)> &link_fill = nullptr);
 void write_simplitig_fasta_keys(const std::string &path, size_t n, uint32_t k,
-                                const std::function<void(uint64_t *, uint64_t *)> &fill);
+                                const std::function<void(uint64_t *, uint64_t *)> &fill,
+                                const std::function<void(uint64_t *)> &link_fill = nullptr);
+// Forced links of n sorted unique k-mers in HBM (lo; hi for k > 32) into
+// link_host[0 .. 2n) (graph_kernels.hip).
+void device_forced_links(const unsigned long long *lo, const unsigned long long *hi, uint64_t n, uint32_t k,
+                         uint64_t *link_host);
 
 // Phase log lines on stderr, in the reference's formats (src/lib.rs:994-1062).
 void log_phase(const char *name, double seconds);

@@ -197,6 +197,7 @@ template <class T> class SimplitigCover {
             });
         }
         finish_keys();
+        keys_as_given_ = n_ == n && !resorted_;
     }
 
   private:
@@ -211,6 +212,7 @@ template <class T> class SimplitigCover {
                     return;
                 }
         });
+        resorted_ = !sorted;
         if (!sorted) std::sort(keys_.begin(), keys_.end());
         keys_.erase(std::unique(keys_.begin(), keys_.end()), keys_.end());
         n_ = keys_.size();
@@ -220,7 +222,10 @@ template <class T> class SimplitigCover {
     }
 
   public:
-    size_t run(size_t first_index, const std::function<void(const char *, size_t)> &put) {
+    // link_fill, when given, writes the 2n link words (the device pass of
+    // graph_kernels.hip) for keys that came sorted and unique.
+    size_t run(size_t first_index, const std::function<void(const char *, size_t)> &put,
+               const std::function<void(uint64_t *)> &link_fill = nullptr) {
         if (n_ == 0) return first_index;
         const bool trace = getenv("MCZI_WRITER_TRACE") != nullptr;
         double t0 = wall_s();
@@ -232,7 +237,14 @@ template <class T> class SimplitigCover {
         };
         SortedSet<T> set(keys_.data(), n_, k_, nt_);
         lap("directory");
-        big_vector<uint64_t> link = forced_links(set);
+        big_vector<uint64_t> link;
+        if (link_fill) {
+            require(keys_as_given_, MCZI_E_INVALID, "device k-mers for the simplitig writer are not sorted and unique");
+            link.resize(2 * n_);
+            link_fill(link.data());
+        } else {
+            link = forced_links(set);
+        }
         lap("links");
         lay_out_chains(link);
         big_vector<uint64_t>().swap(link);
@@ -718,6 +730,7 @@ template <class T> class SimplitigCover {
     int hs_;
     size_t n_ = 0;
     unsigned nt_ = 1;
+    bool resorted_ = false, keys_as_given_ = false;
     big_vector<T> keys_;
     big_vector<uint32_t> pos_;  // chain position of each k-mer (by sorted index)
     big_vector<uint8_t> info_;  // per chain position
@@ -727,14 +740,15 @@ template <class T> class SimplitigCover {
 }  // namespace
 
 size_t emit_simplitigs_keys(size_t n, uint32_t k, const std::function<void(uint64_t *, uint64_t *)> &fill,
-                            size_t first_index, const std::function<void(const char *, size_t)> &put) {
+                            size_t first_index, const std::function<void(const char *, size_t)> &put,
+                            const std::function<void(uint64_t *)> &link_fill) {
     require(k > 0 && k <= 64, MCZI_E_INVALID, "k must be greater than 0 and <= 64");
     if (k <= 32) {
         SimplitigCover<uint64_t> c(n, k, fill);
-        return c.run(first_index, put);
+        return c.run(first_index, put, link_fill);
     }
     SimplitigCover<u128> c(n, k, fill);
-    return c.run(first_index, put);
+    return c.run(first_index, put, link_fill);
 }
 
 size_t emit_simplitigs(const mczi_counted_kmer *kmers, size_t n, uint32_t k, size_t first_index,

@@ -83,9 +83,10 @@ void write_simplitig_fasta(const std::string &path, const mczi_counted_kmer *kme
 }
 
 void write_simplitig_fasta_keys(const std::string &path, size_t n, uint32_t k,
-                                const std::function<void(uint64_t *, uint64_t *)> &fill) {
+                                const std::function<void(uint64_t *, uint64_t *)> &fill,
+                                const std::function<void(uint64_t *)> &link_fill) {
     Out o(path);
-    emit_simplitigs_keys(n, k, fill, 0, [&](const char *t, size_t len) { o.put(t, len); });
+    emit_simplitigs_keys(n, k, fill, 0, [&](const char *t, size_t len) { o.put(t, len); }, link_fill);
     o.finish();
 }
 

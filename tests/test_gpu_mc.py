@@ -143,6 +143,26 @@ def test_fasta_bytes_match_oracle(tmp_path):
     assert out.read_bytes() == (tmp_path / "w.fa").read_bytes()
 
 
+@pytest.mark.parametrize("k,m", [(31, 21), (45, 27)])
+def test_simplitigs_with_device_links_match_host_writer(tmp_path, k, m):
+    # count -> FASTA in one call finds the writer's forced links on the device;
+    # writing the downloaded k-mers finds them on the host: same bytes. The
+    # genome has 5 kbp repeats, so the k-mer graph branches.
+    spec = M.SynthSpec(genome_len=400_000, genome_seed=9, read_seed=109, n_reads=60_000, read_len=150,
+                       sub_rate=0.002, repeat_fraction=0.05, repeat_len=5000, repeat_copies=4)
+    r = M.SynthReads(spec)
+    fa = tmp_path / "r.fa"
+    r.write_fasta(fa)
+    cfg = M.CounterConfig(k, m, 1, 64)
+    dev = tmp_path / "dev.fa"
+    M.count_inputs_to_fasta_path([fa], cfg, dev)
+    kms = M.count_inputs([fa], cfg)
+    assert kms.size > 1 << 17  # above the host writer's threading threshold
+    host = tmp_path / "host.fa"
+    M.write_fasta_path(host, kms, k)
+    assert dev.read_bytes() == host.read_bytes()
+
+
 def test_datasets_match_oracle(tmp_path, golden):
     for case in golden["datasets"]:
         paths = []
