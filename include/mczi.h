@@ -110,6 +110,19 @@ int mczi_write_fasta_path(const char *path, const mczi_counted_kmer *kmers, size
 int mczi_write_fasta_after_index(const char *path, const mczi_counted_kmer *kmers, size_t n,
                                  uint32_t k, size_t last_written_index, size_t *last_index);
 /* write_kff (src/lib.rs:1278-1293): 8-byte counts, ordered=1. */
+/* simplitig_sequences (src/lib.rs:1209-1211): the simplitigs of the greedy
+ * cover above as NUL-terminated ASCII strings, in its order. */
+int mczi_simplitig_sequences(const mczi_counted_kmer *kmers, size_t n, uint32_t k, char ***seqs,
+                             size_t *n_seqs);
+/* simplitig_sequences_from_kmers_parallel (src/lib.rs:1217-1264): k-mers split
+ * into partition_count partitions (a power of two up to 1024; 0 = the
+ * reference's default: 16 per worker thread, or MC_SIMPLITIG_PARTITIONS) by the
+ * hash of their smallest canonical minimizer-size m-mer, the greedy run on
+ * each, results concatenated in partition order. Both lists are released with
+ * mczi_free_strings. */
+int mczi_simplitig_sequences_parallel(const mczi_counted_kmer *kmers, size_t n, uint32_t k,
+                                      uint32_t minimizer, uint32_t partition_count, char ***seqs,
+                                      size_t *n_seqs);
 int mczi_write_kff_path(const char *path, const mczi_counted_kmer *kmers, size_t n,
                         const mczi_counter_config *cfg);
 /* decode_kmer (src/lib.rs:3056-3069); out must hold k bytes (no NUL added). */

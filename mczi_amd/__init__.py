@@ -113,6 +113,10 @@ _sig("mczi_write_fasta_path", C.c_int, [C.c_char_p, _P, C.c_size_t, C.c_uint32])
 _sig("mczi_write_fasta_after_index", C.c_int, [C.c_char_p, _P, C.c_size_t, C.c_uint32, C.c_size_t,
                                                C.POINTER(C.c_size_t)])
 _sig("mczi_write_kff_path", C.c_int, [C.c_char_p, _P, C.c_size_t, C.POINTER(_Config)])
+_sig("mczi_simplitig_sequences", C.c_int, [_P, C.c_size_t, C.c_uint32, C.POINTER(C.POINTER(C.c_char_p)),
+                                           C.POINTER(C.c_size_t)])
+_sig("mczi_simplitig_sequences_parallel", C.c_int, [_P, C.c_size_t, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                    C.POINTER(C.POINTER(C.c_char_p)), C.POINTER(C.c_size_t)])
 _sig("mczi_decode_kmer", C.c_int, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_char_p])
 _sig("mczi_minimizer_run_hashes", C.c_int, [_P, C.c_uint64, C.c_uint32, C.c_uint32, _P, _P, C.POINTER(C.c_size_t)])
 _sig("mczi_batch_from_files", C.c_int, [_CP, C.c_size_t, C.c_uint32, C.c_int, C.POINTER(_P)])
@@ -153,7 +157,8 @@ EXPORTED = [
     "mczi_free_strings", "mczi_count_inputs", "mczi_count_datasets", "mczi_count_inputs_to_fasta_path",
     "mczi_count_datasets_to_fasta_path", "mczi_count_inputs_to_kff_path", "mczi_count_datasets_to_kff_path",
     "mczi_count_inputs_to_kmer_fasta_path", "mczi_count_datasets_to_kmer_fasta_path", "mczi_write_fasta_path",
-    "mczi_write_fasta_after_index", "mczi_write_kff_path", "mczi_decode_kmer", "mczi_minimizer_run_hashes",
+    "mczi_write_fasta_after_index", "mczi_write_kff_path", "mczi_simplitig_sequences",
+    "mczi_simplitig_sequences_parallel", "mczi_decode_kmer", "mczi_minimizer_run_hashes",
     "mczi_batch_from_files", "mczi_pack_fastx", "mczi_batch_from_packed", "mczi_batch_info", "mczi_batch_free", "mczi_batch_count",
     "mczi_batch_download", "mczi_batch_autotune", "mczi_copy_probe", "mczi_device_info", "mczi_zi_subtract", "mczi_zi_absent", "mczi_zi_server_create",
     "mczi_zi_server_create_from_text", "mczi_zi_server_submit", "mczi_zi_server_submit_packed",
@@ -317,6 +322,30 @@ def write_fasta_after_index(path, kmers, k: int, last_written_index: int) -> int
     _check(_lib.mczi_write_fasta_after_index(os.fspath(path).encode(), a.ctypes.data if a.size else None, a.size,
                                              k, last_written_index, C.byref(last)))
     return last.value
+
+
+def _strings(pp, n) -> list:
+    try:
+        return [pp[i].decode() for i in range(n.value)]
+    finally:
+        _lib.mczi_free_strings(pp, n.value)
+
+
+def simplitig_sequences(kmers, k: int) -> list:
+    """The greedy simplitigs of the k-mers as strings (simplitig_sequences)."""
+    a = _counted_arr(kmers)
+    pp, n = C.POINTER(C.c_char_p)(), C.c_size_t()
+    _check(_lib.mczi_simplitig_sequences(a.ctypes.data if a.size else None, a.size, k, C.byref(pp), C.byref(n)))
+    return _strings(pp, n)
+
+
+def simplitig_sequences_parallel(kmers, k: int, minimizer: int, partition_count: int = 0) -> list:
+    """simplitig_sequences_from_kmers_parallel: the greedy per minimizer partition."""
+    a = _counted_arr(kmers)
+    pp, n = C.POINTER(C.c_char_p)(), C.c_size_t()
+    _check(_lib.mczi_simplitig_sequences_parallel(a.ctypes.data if a.size else None, a.size, k, minimizer,
+                                                  partition_count, C.byref(pp), C.byref(n)))
+    return _strings(pp, n)
 
 
 def write_kff(path, kmers, cfg: CounterConfig) -> None:

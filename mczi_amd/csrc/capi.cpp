@@ -360,6 +360,37 @@ int mczi_write_fasta_after_index(const char *path, const mczi_counted_kmer *kmer
     });
 }
 
+static void strings_out(const std::vector<std::string> &v, char ***seqs, size_t *n_seqs) {
+    char **arr = (char **)calloc(std::max<size_t>(v.size(), 1), sizeof(char *));
+    require(arr != nullptr, MCZI_E_NOMEM, "out of host memory");
+    for (size_t i = 0; i < v.size(); i++) {
+        arr[i] = strdup(v[i].c_str());
+        if (!arr[i]) {
+            mczi_free_strings(arr, i);
+            throw Error(MCZI_E_NOMEM, "out of host memory");
+        }
+    }
+    *seqs = arr;
+    *n_seqs = v.size();
+}
+
+int mczi_simplitig_sequences(const mczi_counted_kmer *kmers, size_t n, uint32_t k, char ***seqs,
+                             size_t *n_seqs) {
+    return guarded("failed to build simplitigs", [&] {
+        require((kmers != nullptr || n == 0) && seqs && n_seqs, MCZI_E_INVALID, "null argument");
+        strings_out(simplitig_sequences(kmers, n, k), seqs, n_seqs);
+    });
+}
+
+int mczi_simplitig_sequences_parallel(const mczi_counted_kmer *kmers, size_t n, uint32_t k,
+                                      uint32_t minimizer, uint32_t partition_count, char ***seqs,
+                                      size_t *n_seqs) {
+    return guarded("failed to build simplitigs", [&] {
+        require((kmers != nullptr || n == 0) && seqs && n_seqs, MCZI_E_INVALID, "null argument");
+        strings_out(simplitig_sequences_parallel(kmers, n, k, minimizer, partition_count), seqs, n_seqs);
+    });
+}
+
 int mczi_write_kff_path(const char *path, const mczi_counted_kmer *kmers, size_t n,
                         const mczi_counter_config *cfg) {
     return guarded("failed to write KFF", [&] {

@@ -245,6 +245,14 @@ void write_kff(const std::string &path, const std::vector<mczi_counted_kmer> &km
                const mczi_counter_config &cfg, uint64_t count_size, bool ordered,
                uint64_t saturate_at);
 std::string simplitig_fasta_text(const mczi_counted_kmer *kmers, size_t n, uint32_t k);
+// simplitig_sequences (src/lib.rs:1209-1211): the greedy's simplitigs as
+// ASCII strings, in its order.
+std::vector<std::string> simplitig_sequences(const mczi_counted_kmer *kmers, size_t n, uint32_t k);
+// simplitig_sequences_from_kmers_parallel (src/lib.rs:1217-1264): k-mers split
+// by canonical minimizer into partition_count partitions (0: the reference's
+// default), the greedy run on each, results concatenated in partition order.
+std::vector<std::string> simplitig_sequences_parallel(const mczi_counted_kmer *kmers, size_t n, uint32_t k,
+                                                      uint32_t minimizer, uint32_t partition_count);
 // Greedy simplitig cover (src/lib.rs:1478-1533) of `kmers` (any order,
 // duplicates collapse) as FASTA text handed to `put` in chunks; headers count
 // from first_index + 1. Returns the last index written.

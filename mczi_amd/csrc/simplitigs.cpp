@@ -28,6 +28,7 @@
 
 #include <sys/mman.h>
 
+#include "device_common.hpp"
 #include "internal.hpp"
 
 namespace mczi {
@@ -169,10 +170,10 @@ template <class T> class SortedSet {
 
 template <class T> class SimplitigCover {
   public:
-    SimplitigCover(const mczi_counted_kmer *kmers, size_t n, uint32_t k)
-        : k_(k), mask_(Kmer<T>::mask(k)), hs_(2 * (int)k - 2) {
+    SimplitigCover(const mczi_counted_kmer *kmers, size_t n, uint32_t k, bool one_thread = false)
+        : k_(k), mask_(Kmer<T>::mask(k)), hs_(2 * (int)k - 2), one_thread_(one_thread) {
         keys_.resize(n);
-        nt_ = host_workers(n);
+        nt_ = one_thread_ ? 1 : host_workers(n);
         parallel_ranges(n, 1 << 16, nt_, [&](size_t b0, size_t b1) {
             for (size_t i = b0; i < b1; i++) keys_[i] = Kmer<T>::of(kmers[i]) & mask_;
         });
@@ -218,15 +219,14 @@ template <class T> class SimplitigCover {
         n_ = keys_.size();
         require(n_ < (size_t)NONE, MCZI_E_INVALID,
                 "too many k-mers for the simplitig writer (at most 2^32 - 2)");
-        nt_ = host_workers(n_);
+        nt_ = one_thread_ ? 1 : host_workers(n_);
     }
 
   public:
     // link_fill, when given, writes the 2n link words (the device pass of
     // graph_kernels.hip) for keys that came sorted and unique.
-    size_t run(size_t first_index, const std::function<void(const char *, size_t)> &put,
-               const std::function<void(uint64_t *)> &link_fill = nullptr) {
-        if (n_ == 0) return first_index;
+    template <class Sink> void run(Sink &sink, const std::function<void(uint64_t *)> &link_fill = nullptr) {
+        if (n_ == 0) return;
         const bool trace = getenv("MCZI_WRITER_TRACE") != nullptr;
         double t0 = wall_s();
         auto lap = [&](const char *what) {
@@ -249,9 +249,8 @@ template <class T> class SimplitigCover {
         lay_out_chains(link);
         big_vector<uint64_t>().swap(link);
         lap("chains");
-        const size_t last = greedy(set, first_index, put);
+        greedy(set, sink);
         lap("greedy");
-        return last;
     }
 
   private:
@@ -677,17 +676,13 @@ template <class T> class SimplitigCover {
         }
     }
 
-    size_t greedy(const SortedSet<T> &set, size_t first_index,
-                  const std::function<void(const char *, size_t)> &put) {
-        static const char kAscii[4] = {'A', 'C', 'G', 'T'};
+    // The greedy proper: each simplitig, as 2-bit codes (A0 C1 G2 T3), goes to
+    // sink(seq) in the reference's order.
+    template <class Sink> void greedy(const SortedSet<T> &set, Sink &sink) {
         std::vector<uint8_t> fwd, bwd, seq;
         fwd.reserve(1 << 16);
         bwd.reserve(1 << 16);
         seq.reserve(1 << 17);
-        std::string txt;
-        txt.reserve((1 << 22) + (1 << 16));
-        size_t idx = first_index;
-        char head[96];
         for (size_t i = 0; i < n_; i++) {
             const uint32_t p = pos_[i];
             if (bit(used_.data(), p)) continue;
@@ -703,26 +698,8 @@ template <class T> class SimplitigCover {
             for (size_t j = bwd.size(); j-- > 0;) seq[t++] = (uint8_t)(3 - bwd[j]);
             for (uint32_t j = 0; j < k_; j++) seq[t++] = (uint8_t)((uint32_t)(c >> (2 * (k_ - 1 - j))) & 3);
             if (!fwd.empty()) memcpy(&seq[t], fwd.data(), fwd.size());
-            idx++;
-            const size_t len = seq.size();
-            const int hl = snprintf(head, sizeof head, ">MC_simplitig_%zu kmers=%zu\n", idx, len - k_ + 1);
-            txt.append(head, (size_t)hl);
-            const size_t o0 = txt.size();
-            txt.resize(o0 + len + (len + 79) / 80);
-            char *w = &txt[o0];
-            for (size_t a = 0; a < len; a += 80) {
-                const size_t m = std::min<size_t>(80, len - a);
-                for (size_t j = 0; j < m; j++) w[j] = kAscii[seq[a + j]];
-                w[m] = '\n';
-                w += m + 1;
-            }
-            if (txt.size() >= (1u << 22)) {
-                put(txt.data(), txt.size());
-                txt.clear();
-            }
+            sink(seq);
         }
-        if (!txt.empty()) put(txt.data(), txt.size());
-        return idx;
     }
 
     uint32_t k_;
@@ -730,7 +707,7 @@ template <class T> class SimplitigCover {
     int hs_;
     size_t n_ = 0;
     unsigned nt_ = 1;
-    bool resorted_ = false, keys_as_given_ = false;
+    bool one_thread_ = false, resorted_ = false, keys_as_given_ = false;
     big_vector<T> keys_;
     big_vector<uint32_t> pos_;  // chain position of each k-mer (by sorted index)
     big_vector<uint8_t> info_;  // per chain position
@@ -739,27 +716,190 @@ template <class T> class SimplitigCover {
 
 }  // namespace
 
+// FASTA framing of write_simplitigs (src/lib.rs:1515-1533): ">MC_simplitig_i
+// kmers=n", sequence in 80-column lines; text handed on in ~4 MiB chunks.
+class FastaSink {
+  public:
+    FastaSink(uint32_t k, size_t first_index, const std::function<void(const char *, size_t)> &put)
+        : k_(k), idx_(first_index), put_(put) {
+        txt_.reserve((1 << 22) + (1 << 16));
+    }
+    void operator()(const std::vector<uint8_t> &seq) {
+        static const char kAscii[4] = {'A', 'C', 'G', 'T'};
+        char head[96];
+        idx_++;
+        const size_t len = seq.size();
+        const int hl = snprintf(head, sizeof head, ">MC_simplitig_%zu kmers=%zu\n", idx_, len - k_ + 1);
+        txt_.append(head, (size_t)hl);
+        const size_t o0 = txt_.size();
+        txt_.resize(o0 + len + (len + 79) / 80);
+        char *w = &txt_[o0];
+        for (size_t a = 0; a < len; a += 80) {
+            const size_t m = std::min<size_t>(80, len - a);
+            for (size_t j = 0; j < m; j++) w[j] = kAscii[seq[a + j]];
+            w[m] = '\n';
+            w += m + 1;
+        }
+        if (txt_.size() >= (1u << 22)) flush();
+    }
+    size_t finish() {
+        flush();
+        return idx_;
+    }
+
+  private:
+    void flush() {
+        if (!txt_.empty()) put_(txt_.data(), txt_.size());
+        txt_.clear();
+    }
+    uint32_t k_;
+    size_t idx_;
+    const std::function<void(const char *, size_t)> &put_;
+    std::string txt_;
+};
+
+// Each simplitig as an ASCII string (decode_kmer's letters, src/lib.rs:3056-3069).
+struct StringSink {
+    std::vector<std::string> *out;
+    void operator()(const std::vector<uint8_t> &seq) {
+        static const char kAscii[4] = {'A', 'C', 'G', 'T'};
+        std::string t(seq.size(), 'A');
+        for (size_t j = 0; j < seq.size(); j++) t[j] = kAscii[seq[j]];
+        out->push_back(std::move(t));
+    }
+};
+
 size_t emit_simplitigs_keys(size_t n, uint32_t k, const std::function<void(uint64_t *, uint64_t *)> &fill,
                             size_t first_index, const std::function<void(const char *, size_t)> &put,
                             const std::function<void(uint64_t *)> &link_fill) {
-    require(k > 0 && k <= 64, MCZI_E_INVALID, "k must be greater than 0 and <= 64");
+    require(k > 0, MCZI_E_INVALID, "k must be greater than 0");
+    require(k <= 64, MCZI_E_INVALID, "k must be <= 64");
+    FastaSink sink(k, first_index, put);
     if (k <= 32) {
         SimplitigCover<uint64_t> c(n, k, fill);
-        return c.run(first_index, put, link_fill);
+        c.run(sink, link_fill);
+    } else {
+        SimplitigCover<u128> c(n, k, fill);
+        c.run(sink, link_fill);
     }
-    SimplitigCover<u128> c(n, k, fill);
-    return c.run(first_index, put, link_fill);
+    return sink.finish();
 }
 
 size_t emit_simplitigs(const mczi_counted_kmer *kmers, size_t n, uint32_t k, size_t first_index,
                        const std::function<void(const char *, size_t)> &put) {
-    require(k > 0 && k <= 64, MCZI_E_INVALID, "k must be greater than 0 and <= 64");
+    require(k > 0, MCZI_E_INVALID, "k must be greater than 0");
+    require(k <= 64, MCZI_E_INVALID, "k must be <= 64");
+    FastaSink sink(k, first_index, put);
     if (k <= 32) {
         SimplitigCover<uint64_t> c(kmers, n, k);
-        return c.run(first_index, put);
+        c.run(sink);
+    } else {
+        SimplitigCover<u128> c(kmers, n, k);
+        c.run(sink);
     }
-    SimplitigCover<u128> c(kmers, n, k);
-    return c.run(first_index, put);
+    return sink.finish();
+}
+
+namespace {
+
+// minimizer_hash_u64 / minimizer_hash_u128 (src/lib.rs:2979-2994)
+uint32_t minimizer_hash_u64(uint64_t v) {
+    const uint64_t folded = v ^ ((v >> 17) | (v << 47)) ^ (v >> 32);
+    return (uint32_t)folded * 0x9E3779B1u;
+}
+uint32_t minimizer_hash_u128(u128 v) {
+    const uint64_t h = (uint64_t)(v >> 64);
+    return minimizer_hash_u64((uint64_t)v ^ ((h << 23) | (h >> 41)));
+}
+
+// simplitig_partition (src/lib.rs:2935-2956): the smallest canonical m-mer
+// of the k-mer, hashed, masked to a power-of-two partition count.
+uint32_t simplitig_partition(u128 enc, uint32_t k, uint32_t m, uint32_t parts) {
+    const u128 mask = Kmer<u128>::mask(m);
+    u128 best = ~(u128)0;
+    for (uint32_t start = 0; start + m <= k; start++) {
+        const u128 v = (enc >> (2 * (k - start - m))) & mask;
+        const u128 r = Kmer<u128>::rc(v, m);
+        best = std::min(best, std::min(v, r));
+    }
+    return spread_hash_u32(minimizer_hash_u128(best)) & (parts - 1);
+}
+
+// simplitig_partition_count (src/lib.rs:2967-2977), with the host worker
+// count in place of Rayon's thread count.
+uint32_t simplitig_partition_count(uint32_t requested) {
+    size_t want = requested;
+    if (want == 0) {
+        want = (size_t)host_workers(size_t(1) << 20) * 16;
+        if (const char *e = getenv("MC_SIMPLITIG_PARTITIONS")) {
+            char *end = nullptr;
+            const unsigned long long v = strtoull(e, &end, 10);
+            if (end != e && *end == 0) want = (size_t)v;
+        }
+    }
+    want = std::min<size_t>(std::max<size_t>(want, 1), 1024);
+    size_t p = 1;
+    while (p < want) p <<= 1;
+    return (uint32_t)std::min<size_t>(p, 1024);
+}
+
+}  // namespace
+
+std::vector<std::string> simplitig_sequences_parallel(const mczi_counted_kmer *kmers, size_t n, uint32_t k,
+                                                      uint32_t minimizer, uint32_t partition_count) {
+    require(k > 0, MCZI_E_INVALID, "k must be greater than 0");
+    require(k <= 64, MCZI_E_INVALID, "k must be <= 64");
+    require(minimizer > 0 && minimizer <= k, MCZI_E_INVALID, "minimizer size must be > 0 and <= k");
+    require(minimizer <= 64, MCZI_E_INVALID, "minimizer size must be <= 64 because MC stores minimizers in u128");
+    const uint32_t parts = simplitig_partition_count(partition_count);
+    const unsigned nt = host_workers(n);
+    std::vector<uint32_t> part(n);
+    parallel_ranges(n, 1 << 14, nt, [&](size_t b0, size_t b1) {
+        for (size_t i = b0; i < b1; i++)
+            part[i] = simplitig_partition(((u128)kmers[i].hi << 64) | kmers[i].lo, k, minimizer, parts);
+    });
+    std::vector<size_t> start(parts + 1, 0);
+    for (size_t i = 0; i < n; i++) start[part[i] + 1]++;
+    for (uint32_t q = 0; q < parts; q++) start[q + 1] += start[q];
+    std::vector<mczi_counted_kmer> by_part(n);
+    {
+        std::vector<size_t> at(start.begin(), start.end() - 1);
+        for (size_t i = 0; i < n; i++) by_part[at[part[i]]++] = kmers[i];
+    }
+    std::vector<std::vector<std::string>> per(parts);
+    parallel_ranges(parts, 1, nt, [&](size_t q0, size_t q1) {
+        for (size_t q = q0; q < q1; q++) {
+            StringSink sink{&per[q]};
+            const mczi_counted_kmer *b = by_part.data() + start[q];
+            const size_t m = start[q + 1] - start[q];
+            if (k <= 32) {
+                SimplitigCover<uint64_t> c(b, m, k, true);
+                c.run(sink);
+            } else {
+                SimplitigCover<u128> c(b, m, k, true);
+                c.run(sink);
+            }
+        }
+    });
+    std::vector<std::string> out;
+    for (auto &v : per)
+        for (auto &t : v) out.push_back(std::move(t));
+    return out;
+}
+
+std::vector<std::string> simplitig_sequences(const mczi_counted_kmer *kmers, size_t n, uint32_t k) {
+    require(k > 0, MCZI_E_INVALID, "k must be greater than 0");
+    require(k <= 64, MCZI_E_INVALID, "k must be <= 64");
+    std::vector<std::string> out;
+    StringSink sink{&out};
+    if (k <= 32) {
+        SimplitigCover<uint64_t> c(kmers, n, k);
+        c.run(sink);
+    } else {
+        SimplitigCover<u128> c(kmers, n, k);
+        c.run(sink);
+    }
+    return out;
 }
 
 }  // namespace mczi

@@ -100,9 +100,78 @@ def test_large_sets_threaded(tmp_path, k):
 
 
 def test_text_and_empty(tmp_path):
+    # header numbering continues after first_index, in both writers
     a = _genome_set(np.random.default_rng(3), 31, 500)
-    O.write_simplitigs(str(tmp_path / "o.fa"), a, 31)
-    assert M.simplitig_fasta_text(a, 31) == (tmp_path / "o.fa").read_text() if hasattr(M, "simplitig_fasta_text") else True
+    last = M.write_fasta_after_index(tmp_path / "m.fa", a, 31, 41)
+    assert last == O.write_simplitigs(str(tmp_path / "o.fa"), a, 31, 41) > 41
+    assert (tmp_path / "m.fa").read_bytes() == (tmp_path / "o.fa").read_bytes()
+    assert (tmp_path / "m.fa").read_text().startswith(">MC_simplitig_42 ")
     e = np.zeros(0, dtype=M.COUNTED)
     assert M.write_fasta_after_index(tmp_path / "e.fa", e, 31, 4) == 4
     assert (tmp_path / "e.fa").read_bytes() == b""
+
+
+def _fasta_seqs(path):
+    seqs, cur = [], None
+    for line in open(path):
+        line = line.rstrip("\n")
+        if line.startswith(">"):
+            if cur is not None:
+                seqs.append("".join(cur))
+            cur = []
+        else:
+            cur.append(line)
+    if cur is not None:
+        seqs.append("".join(cur))
+    return seqs
+
+
+def _canon_all(seqs, k):
+    out = []
+    for s in seqs:
+        for i in range(len(s) - k + 1):
+            out.append(O.encode_canonical(s[i:i + k]))
+    return sorted(out)
+
+
+def test_simplitig_sequences_are_the_fasta_records(tmp_path):
+    rng = np.random.default_rng(21)
+    for k in (3, 7, 21, 31, 45):
+        a = _genome_set(rng, k, 2000, rc_repeats=3, repeats=2, errors=4)
+        O.write_simplitigs(str(tmp_path / "o.fa"), a, k)
+        assert M.simplitig_sequences(a, k) == _fasta_seqs(tmp_path / "o.fa"), k
+    assert M.simplitig_sequences(np.zeros(0, dtype=M.COUNTED), 31) == []
+
+
+def test_parallel_simplitigs_preserve_unique_kmer_set():
+    # the reference's own case (src/lib.rs:5395-5416): k = 3, minimizer 2
+    kms = [b"AAA", b"AAC", b"ACG", b"CGT", b"AAC"]
+    a = np.zeros(len(kms), dtype=M.COUNTED)
+    a["lo"] = [O.encode_canonical(x.decode()) for x in kms]
+    a["count"] = 1
+    a.sort(order=["hi", "lo"])
+    want = sorted({O.encode_canonical(x.decode()) for x in kms})
+    for parts in (0, 1, 2, 1024):
+        assert _canon_all(M.simplitig_sequences_parallel(a, 3, 2, parts), 3) == want, parts
+
+
+@pytest.mark.parametrize("k,m", [(5, 3), (21, 11), (31, 19), (45, 23)])
+def test_parallel_simplitigs_cover_each_kmer_once(k, m):
+    rng = np.random.default_rng(k)
+    a = _genome_set(rng, k, 20_000, rc_repeats=5, repeats=5, errors=20)
+    want = sorted((int(h) << 64) | int(l) for h, l in zip(a["hi"], a["lo"]))
+    for parts in (1, 2, 16, 1024):
+        got = M.simplitig_sequences_parallel(a, k, m, parts)
+        assert _canon_all(got, k) == want, parts
+    # one partition is the plain greedy
+    assert M.simplitig_sequences_parallel(a, k, m, 1) == M.simplitig_sequences(a, k)
+
+
+def test_parallel_simplitigs_validation():
+    a = _genome_set(np.random.default_rng(2), 9, 100)
+    with pytest.raises(M.MczaError, match="minimizer size must be > 0 and <= k"):
+        M.simplitig_sequences_parallel(a, 9, 0)
+    with pytest.raises(M.MczaError, match="minimizer size must be > 0 and <= k"):
+        M.simplitig_sequences_parallel(a, 9, 10)
+    with pytest.raises(M.MczaError, match="k must be <= 64"):
+        M.simplitig_sequences(a, 65)
