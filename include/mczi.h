@@ -266,6 +266,32 @@ int mczi_zi_server_stats(mczi_zi_server *s, uint64_t *index_kmers, uint64_t *com
                          uint64_t *lookups, double *busy_ms);
 void mczi_zi_server_destroy(mczi_zi_server *s);
 
+/* MCZI in one process (src/mczi.rs:238-451), without GGCAT and SSHash: the
+ * index inputs are counted by MC (cfg: k odd in [3, 63], minimizer in (0, k),
+ * threshold x; index_datasets != 0 = one dataset per file, fof presence,
+ * --index-fofn) and the k-mers above the threshold become the device
+ * membership index directly; the query k-mers absent from it are subtracted
+ * as in ZI. Logs MCZI phase lines and the MCZI_STAT query statistics
+ * (src/mczi.rs:1600-1650). out_path receives the simplitig FASTA of the absent
+ * canonical k-mers (the greedy writer above: the reference compacts the same
+ * set with GGCAT, so the k-mer set is identical and the layout differs);
+ * out_path NULL = --output-mode no-output. */
+typedef struct {
+    uint64_t index_kmers;                     /* index k-mers above the threshold */
+    uint64_t query_kmers_scanned;             /* query windows */
+    uint64_t query_kmers_filtered;            /* windows whose k-mer is in the index */
+    uint64_t query_kmers_not_filtered;        /* windows whose k-mer is absent */
+    uint64_t query_unique_kmers_not_filtered; /* distinct absent canonical k-mers */
+} mczi_mczi_stats;
+int mczi_mczi_subtract(const char *const *index_paths, size_t n_index, int index_datasets,
+                       const char *const *query_paths, size_t n_query, const mczi_counter_config *cfg,
+                       const char *out_path, mczi_mczi_stats *stats);
+/* The same subtraction returning the sorted absent k-mers (count 1 each),
+ * released with mczi_free. */
+int mczi_mczi_absent(const char *const *index_paths, size_t n_index, int index_datasets,
+                     const char *const *query_paths, size_t n_query, const mczi_counter_config *cfg,
+                     mczi_counted_kmer **out, size_t *n_out, mczi_mczi_stats *stats);
+
 /* ---- synthetic inputs (no counterpart in the reference; test/bench data) --
  * SplitMix64 genome and reads (bench/simulate_long_reads.rs:9-30, 166-180),
  * extended with random strand, iid errors, indels and repeats. Produces

@@ -78,6 +78,14 @@ class _SynthReads(C.Structure):
                 ("dataset", C.POINTER(C.c_uint32)), ("n_bases", C.c_uint64)]
 
 
+class McziStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("index_kmers", "query_kmers_scanned", "query_kmers_filtered",
+                                         "query_kmers_not_filtered", "query_unique_kmers_not_filtered")]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class _ZiResult(C.Structure):
     _fields_ = [("request_id", C.c_uint64), ("n_windows", C.c_uint64), ("n_absent", C.c_uint64),
                 ("absent_lo", C.POINTER(C.c_uint64)), ("absent_hi", C.POINTER(C.c_uint64)),
@@ -134,6 +142,10 @@ _sig("mczi_zi_subtract", C.c_int, [C.c_char_p, _CP, C.c_size_t, C.c_uint32, C.c_
                                    C.c_uint32, C.c_uint32])
 _sig("mczi_zi_absent", C.c_int, [C.c_char_p, _CP, C.c_size_t, C.c_uint32, C.POINTER(C.POINTER(_Counted)),
                                  C.POINTER(C.c_size_t)])
+_sig("mczi_mczi_subtract", C.c_int, [_CP, C.c_size_t, C.c_int, _CP, C.c_size_t, C.POINTER(_Config), C.c_char_p,
+                                     C.POINTER(McziStats)])
+_sig("mczi_mczi_absent", C.c_int, [_CP, C.c_size_t, C.c_int, _CP, C.c_size_t, C.POINTER(_Config),
+                                   C.POINTER(C.POINTER(_Counted)), C.POINTER(C.c_size_t), C.POINTER(McziStats)])
 _sig("mczi_zi_server_create", C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.POINTER(_P)])
 _sig("mczi_zi_server_create_from_text", C.c_int, [C.c_char_p, C.c_size_t, C.c_uint32, C.c_uint32, C.c_uint64,
                                                   C.c_uint32, C.POINTER(_P)])
@@ -160,7 +172,7 @@ EXPORTED = [
     "mczi_write_fasta_after_index", "mczi_write_kff_path", "mczi_simplitig_sequences",
     "mczi_simplitig_sequences_parallel", "mczi_decode_kmer", "mczi_minimizer_run_hashes",
     "mczi_batch_from_files", "mczi_pack_fastx", "mczi_batch_from_packed", "mczi_batch_info", "mczi_batch_free", "mczi_batch_count",
-    "mczi_batch_download", "mczi_batch_autotune", "mczi_copy_probe", "mczi_device_info", "mczi_zi_subtract", "mczi_zi_absent", "mczi_zi_server_create",
+    "mczi_batch_download", "mczi_batch_autotune", "mczi_copy_probe", "mczi_device_info", "mczi_zi_subtract", "mczi_zi_absent", "mczi_mczi_subtract", "mczi_mczi_absent", "mczi_zi_server_create",
     "mczi_zi_server_create_from_text", "mczi_zi_server_submit", "mczi_zi_server_submit_packed",
     "mczi_zi_server_poll", "mczi_zi_result_free",
     "mczi_zi_server_stats", "mczi_zi_server_destroy", "mczi_synth_generate", "mczi_synth_genome",
@@ -288,6 +300,31 @@ def count_datasets_to_kmer_fasta_path(inputs, cfg: CounterConfig, out) -> int:
     _check(_lib.mczi_count_datasets_to_kmer_fasta_path(arr, n, C.byref(cfg._c()), os.fspath(out).encode(),
                                                         C.byref(st)))
     return st.selected_kmers
+
+
+def mczi_subtract(index_inputs, query_inputs, cfg: CounterConfig, out=None, index_datasets: bool = False) -> dict:
+    """MCZI in one process: MC-count the index inputs (threshold cfg.threshold;
+    index_datasets = one dataset per file), subtract the query k-mers found
+    in that set, write the absent k-mers' simplitigs to `out` (None: stats
+    only). Returns the query statistics."""
+    ia, ni = _paths(index_inputs)
+    qa, nq = _paths(query_inputs)
+    st = McziStats()
+    _check(_lib.mczi_mczi_subtract(ia, ni, int(index_datasets), qa, nq, C.byref(cfg._c()),
+                                   None if out is None else os.fspath(out).encode(), C.byref(st)))
+    return st.as_dict()
+
+
+def mczi_absent(index_inputs, query_inputs, cfg: CounterConfig, index_datasets: bool = False):
+    """The same subtraction: (sorted absent k-mers, query statistics)."""
+    ia, ni = _paths(index_inputs)
+    qa, nq = _paths(query_inputs)
+    st = McziStats()
+    p = C.POINTER(_Counted)()
+    n = C.c_size_t()
+    _check(_lib.mczi_mczi_absent(ia, ni, int(index_datasets), qa, nq, C.byref(cfg._c()), C.byref(p), C.byref(n),
+                                 C.byref(st)))
+    return _take(p, n.value), st.as_dict()
 
 
 def _counted_arr(kmers) -> np.ndarray:

@@ -10,6 +10,8 @@
 #include <string>
 #include <vector>
 
+#include <sys/resource.h>
+
 #include "internal.hpp"
 #include "zi.hpp"
 
@@ -126,9 +128,165 @@ std::vector<mczi_counted_kmer> count_datasets_impl(const char *const *paths, siz
     return v;
 }
 
+// ---- MCZI in one process (src/mczi.rs:238-451) ------------------------------
+// log_resource_phase (src/lib.rs:1012-1021): "MCZI phaseN detail W.WWWs CPU
+// C.CCCs NMB RAM", CPU = user + system time of the phase, RAM = peak RSS.
+class ResourcePhase {
+  public:
+    ResourcePhase() : t0_(now_s()), cpu0_(cpu_s()) {}
+    void log(const char *name) const {
+        rusage u{};
+        getrusage(RUSAGE_SELF, &u);
+        const unsigned long long mb = ((unsigned long long)u.ru_maxrss + 1023) / 1024;
+        fprintf(stderr, "MCZI %s %.3fs CPU %.3fs %lluMB RAM\n", phase_display_name(name).c_str(), now_s() - t0_,
+                cpu_s() - cpu0_, mb);
+    }
+    static void zero(const char *name) {
+        rusage u{};
+        getrusage(RUSAGE_SELF, &u);
+        fprintf(stderr, "MCZI %s 0.000s CPU 0.000s %lluMB RAM\n", phase_display_name(name).c_str(),
+                ((unsigned long long)u.ru_maxrss + 1023) / 1024);
+    }
+
+  private:
+    static double cpu_s() {
+        rusage u{};
+        getrusage(RUSAGE_SELF, &u);
+        return u.ru_utime.tv_sec + u.ru_stime.tv_sec + 1e-6 * (u.ru_utime.tv_usec + u.ru_stime.tv_usec);
+    }
+    double t0_, cpu0_;
+};
+
+// log_mczi_stat with format_stat_u64's comma groups (src/mczi.rs:1600-1632)
+void mczi_stat(const char *name, uint64_t v) {
+    const std::string d = std::to_string(v);
+    std::string g;
+    for (size_t i = 0; i < d.size(); i++) {
+        if (i && (d.size() - i) % 3 == 0) g.push_back(',');
+        g.push_back(d[i]);
+    }
+    fprintf(stderr, "MCZI_STAT\t%s\t%s\n", name, g.c_str());
+}
+
+// The index inputs are counted by MC and the counted k-mers become the
+// device membership index directly (the reference goes through GGCAT
+// simplitigs and SSHash: same membership); the queries are subtracted as in
+// ZI. Returns the query scan with the sorted unique absent k-mers.
+// run()'s argument checks (src/mczi.rs:239-311), before any device work.
+void mczi_check(const char *const *ip, size_t ni, bool datasets, const char *const *qp, size_t nq,
+                const mczi_counter_config &cfg) {
+    require(cfg.k >= 3 && cfg.k <= 63, MCZI_E_INVALID, "k must be in [3, 63] for sshash-rs");
+    require(cfg.k % 2 == 1, MCZI_E_INVALID, "k must be odd for sshash-rs");
+    require(cfg.minimizer > 0 && cfg.minimizer < cfg.k, MCZI_E_INVALID, "minimizer size must be > 0 and < k");
+    require(cfg.partition_count > 0, MCZI_E_INVALID, "--partition-count must be greater than 0");
+    require(ni > 0 && ip != nullptr, MCZI_E_INVALID, "at least one index input is required");
+    require(nq > 0 && qp != nullptr, MCZI_E_INVALID, "at least one query input is required");
+    if (datasets)
+        validate_datasets(cfg);
+    else
+        validate_normal(cfg);
+}
+
+ZiScan mczi_run(const char *const *ip, size_t ni, bool datasets, const char *const *qp, size_t nq,
+                const mczi_counter_config &cfg, bool no_output, mczi_mczi_stats *stats) {
+    Device &d = Device::get();
+    std::unique_ptr<Batch> b;
+    uint64_t n_index = 0;
+    if (datasets && cfg.threshold >= ni) {  // src/mczi.rs:325-337
+        fprintf(stderr,
+                "MCZI_WARN\tindex_fofn_threshold_excludes_all_kmers\tindex_inputs\t%zu\tthreshold\t%llu\tmessage\t"
+                "no k-mer can pass a dataset-presence threshold greater than or equal to the number of index "
+                "datasets\n",
+                ni, (unsigned long long)cfg.threshold);
+        ResourcePhase::zero("1_index_minimizer_presence_counting");
+        ResourcePhase::zero("2_index_kmer_partition_counting");
+    } else {
+        ResourcePhase ph;
+        b = load_inputs(to_paths(ip, ni), cfg.k, datasets);
+        if (b->n_frag)
+            batch_count(*b, cfg, datasets ? MCZI_MODE_DATASETS : MCZI_MODE_TWO_PASS, MCZI_LAUNCH_BATCHED,
+                        MCZI_ORDER_ENCODED, nullptr);
+        n_index = b->n_frag ? b->n_out : 0;
+        if (datasets) {
+            ResourcePhase::zero("1_index_minimizer_presence_counting");
+            ph.log("2_index_kmer_partition_counting");
+        } else {
+            ph.log("1_index_kmer_counting");
+        }
+    }
+    mczi_stat("index_kmers_above_threshold", n_index);
+    ResourcePhase::zero("3_ggcat_simplitigs");  // no simplitig round trip: the k-mers go straight in
+    ZiIndex idx;
+    {
+        ResourcePhase ph;
+        zi_build_index_from_keys(idx, n_index ? b->out_kmer.as<unsigned long long>() : nullptr,
+                                 n_index && cfg.k > 32 ? b->out_hi.as<unsigned long long>() : nullptr, n_index,
+                                 cfg.k, cfg.minimizer, d.stream());
+        ph.log("4_sshash_indexing");
+    }
+    b.reset();
+    ResourcePhase ph;
+    ZiScan scan = zi_subtract_files(idx, qp, nq, d.stream());
+    ph.log(no_output ? "5_query_subtraction_no_output" : "5_query_subtraction");
+    // log_query_stats (src/mczi.rs:1634-1650)
+    mczi_stat("query_kmers_scanned", scan.windows);
+    mczi_stat("query_kmers_filtered_by_zi", scan.windows - scan.absent_windows);
+    mczi_stat("query_kmers_not_filtered_by_zi", scan.absent_windows);
+    if (stats) {
+        stats->index_kmers = n_index;
+        stats->query_kmers_scanned = scan.windows;
+        stats->query_kmers_filtered = scan.windows - scan.absent_windows;
+        stats->query_kmers_not_filtered = scan.absent_windows;
+        stats->query_unique_kmers_not_filtered = scan.absent.n;
+    }
+    return scan;
+}
+
 }  // namespace
 
 extern "C" {
+
+int mczi_mczi_subtract(const char *const *index_paths, size_t n_index, int index_datasets,
+                       const char *const *query_paths, size_t n_query, const mczi_counter_config *cfg,
+                       const char *out_path, mczi_mczi_stats *stats) {
+    return guarded("MCZI failed", [&] {
+        require(cfg != nullptr, MCZI_E_INVALID, "null config");
+        mczi_check(index_paths, n_index, index_datasets != 0, query_paths, n_query, *cfg);
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        ResourcePhase total;
+        const bool no_output = out_path == nullptr;
+        ZiScan scan = mczi_run(index_paths, n_index, index_datasets != 0, query_paths, n_query, *cfg, no_output, stats);
+        if (no_output) {
+            ResourcePhase::zero("6_no_output");
+        } else {
+            ResourcePhase ph;
+            ZiKeys &a = scan.absent;
+            write_simplitig_fasta_keys(out_path, a.n, cfg->k, [&](uint64_t *lo, uint64_t *hi) {
+                if (a.n) memcpy(lo, a.lo, a.n * 8);
+                if (hi) {
+                    if (a.hi && a.n)
+                        memcpy(hi, a.hi, a.n * 8);
+                    else if (a.n)
+                        memset(hi, 0, a.n * 8);
+                }
+            });
+            ph.log("6_ggcat_simplitig_output");
+        }
+        total.log("total");
+    });
+}
+
+int mczi_mczi_absent(const char *const *index_paths, size_t n_index, int index_datasets,
+                     const char *const *query_paths, size_t n_query, const mczi_counter_config *cfg,
+                     mczi_counted_kmer **out, size_t *n_out, mczi_mczi_stats *stats) {
+    return guarded("MCZI failed", [&] {
+        require(cfg != nullptr && out != nullptr && n_out != nullptr, MCZI_E_INVALID, "null argument");
+        mczi_check(index_paths, n_index, index_datasets != 0, query_paths, n_query, *cfg);
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        ZiScan scan = mczi_run(index_paths, n_index, index_datasets != 0, query_paths, n_query, *cfg, true, stats);
+        copy_out(scan.absent.counted(), out, n_out);
+    });
+}
 
 const char *mczi_last_error(void) { return g_err.c_str(); }
 const char *mczi_version(void) { return "mczi-hip 0.1.0 (gfx950)"; }

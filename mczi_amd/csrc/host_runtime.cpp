@@ -673,9 +673,17 @@ static std::string compact_detail(const std::string &n) {
         {"output_streaming", "output"},
         {"kmer_counting_and_output", "kmer count output"},
         {"simplitig_output", "simplitig output"},
+        {"ggcat_simplitigs", "ggcat simplitigs"},
         {"sshash_indexing", "sshash index"},
         {"query_subtraction", "query subtraction"},
+        {"query_subtraction_no_output", "query scan no output"},
+        {"query_subtraction_regular_output", "regular output"},
+        {"regular_output", "regular output"},
+        {"ggcat_simplitig_output", "ggcat output"},
+        {"reform_output", "reform output"},
+        {"no_output", "no output"},
         {"fasta_total", "fasta total"},
+        {"phase12_total", "phase12 total"},
         {"total", "total"},
     };
     for (auto &e : map)
@@ -683,17 +691,19 @@ static std::string compact_detail(const std::string &n) {
     return n;
 }
 
-void log_phase(const char *name, double seconds) {
+std::string phase_display_name(const char *name) {
     std::string n(name);
-    std::string shown;
     size_t us = n.find('_');
     if (us != std::string::npos && us > 0 && isdigit((unsigned char)n[0])) {
         bool alnum = true;
         for (size_t i = 0; i < us; i++) alnum = alnum && isalnum((unsigned char)n[i]);
-        if (alnum) shown = "phase" + n.substr(0, us) + " " + compact_detail(n.substr(us + 1));
+        if (alnum) return "phase" + n.substr(0, us) + " " + compact_detail(n.substr(us + 1));
     }
-    if (shown.empty()) shown = compact_detail(n);
-    fprintf(stderr, "MC %s %.3fs\n", shown.c_str(), seconds);
+    return compact_detail(n);
+}
+
+void log_phase(const char *name, double seconds) {
+    fprintf(stderr, "MC %s %.3fs\n", phase_display_name(name).c_str(), seconds);
 }
 
 }  // namespace mczi

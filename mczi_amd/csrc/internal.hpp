@@ -275,6 +275,8 @@ void device_forced_links(const unsigned long long *lo, const unsigned long long 
 
 // Phase log lines on stderr, in the reference's formats (src/lib.rs:994-1062).
 void log_phase(const char *name, double seconds);
+// "phaseN detail" of a phase name (display_phase_name, src/lib.rs:1027-1035).
+std::string phase_display_name(const char *name);
 
 }  // namespace mczi
 

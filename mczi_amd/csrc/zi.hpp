@@ -21,6 +21,10 @@ struct ZiIndex {
 };
 
 void zi_build_index(ZiIndex &idx, HostReads &&reads, uint32_t k, uint32_t m, hipStream_t s);
+// The index from n sorted unique canonical k-mers already in HBM (lo; hi for
+// k > 32), e.g. the output of an MC count.
+void zi_build_index_from_keys(ZiIndex &idx, const unsigned long long *lo, const unsigned long long *hi, uint64_t n,
+                              uint32_t k, uint32_t m, hipStream_t s);
 
 // One batch of query requests (fragments tagged with their request slot).
 // Sorted, unique absent k-mers of one request, in pinned host arrays from
@@ -64,6 +68,7 @@ struct ZiKeys {
 struct ZiQueryResult {
     std::vector<ZiKeys> absent;     // per request
     std::vector<uint64_t> windows;  // per request
+    std::vector<uint64_t> absent_windows;  // per request, repeats counted
     double wait_ms = 0;             // host wait for the batch's device work
 };
 
@@ -93,5 +98,13 @@ struct ZiBatchInput {
 void zi_enqueue(ZiWork &w, const ZiIndex &idx, ZiBatchInput &&in, uint32_t n_req, hipStream_t s);
 // Wait for the batch and collect per-request results.
 void zi_collect(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, ZiQueryResult &out);
+
+// All query files in one batch against idx: query windows, absent windows
+// (repeats counted) and the sorted unique absent k-mers.
+struct ZiScan {
+    ZiKeys absent;
+    uint64_t windows = 0, absent_windows = 0;
+};
+ZiScan zi_subtract_files(const ZiIndex &idx, const char *const *qp, size_t n, hipStream_t s);
 
 }  // namespace mczi

@@ -162,6 +162,44 @@ __global__ __launch_bounds__(ZI_THREADS) void k_zi_build(ZiTileArgs a) {
     }
 }
 
+// Index build from k-mers already in HBM (the counted, sorted set of an MC
+// run): the same slots and probing as k_zi_build, one key per thread.
+template <bool WIDE>
+__global__ void k_zi_insert(const unsigned long long *__restrict__ lo, const unsigned long long *__restrict__ hi,
+                            uint64_t n, unsigned long long *tlo, unsigned long long *thi, uint64_t mask) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        if constexpr (!WIDE) {
+            const unsigned long long key = lo[i];
+            uint64_t s = slot_of(key, 0) & mask;
+            for (;;) {
+                const unsigned long long cur = tlo[s];
+                if (cur == key) break;
+                if (cur == EMPTY_KEY) {
+                    const unsigned long long prev = atomicCAS(&tlo[s], EMPTY_KEY, key);
+                    if (prev == EMPTY_KEY || prev == key) break;
+                }
+                s = (s + 1) & mask;
+            }
+        } else {
+            const Key2 key{lo[i], hi[i]};
+            uint64_t s = slot_of(key.lo, key.hi) & mask;
+            for (;;) {
+                const unsigned long long ch = thi[s];
+                if (ch == key.hi && tlo[s] == key.lo) break;
+                if (ch == EMPTY_KEY) {
+                    const unsigned long long prev = atomicCAS(&thi[s], EMPTY_KEY, key.hi);
+                    if (prev == EMPTY_KEY) {
+                        tlo[s] = key.lo;
+                        break;
+                    }
+                }
+                s = (s + 1) & mask;
+            }
+        }
+    }
+}
+
 // Lookup: every window of every query fragment is looked up in the index;
 // absent canonical k-mers are appended to their request's segment. The
 // tile's bases are staged in LDS; each thread first forms the keys of its
@@ -395,6 +433,33 @@ void zi_build_index(ZiIndex &idx, HostReads &&reads, uint32_t k, uint32_t m, hip
     MCZI_HIP(hipGetLastError());
 }
 
+void zi_build_index_from_keys(ZiIndex &idx, const unsigned long long *lo, const unsigned long long *hi, uint64_t n,
+                              uint32_t k, uint32_t m, hipStream_t s) {
+    require(k <= 32 || hi != nullptr || n == 0, MCZI_E_INVALID, "k > 32 needs the high k-mer words");
+    idx.k = k;
+    idx.m = m;
+    idx.wide = k > 32;
+    uint64_t slots = 1024;
+    while (slots < n * 2) slots <<= 1;
+    idx.mask = slots - 1;
+    idx.n_kmers = n;
+    idx.lo.reset(slots * 8);
+    if (idx.wide) idx.hi.reset(slots * 8);
+    zfill(idx.lo.as<unsigned long long>(), slots, EMPTY_KEY, s);
+    if (idx.wide) zfill(idx.hi.as<unsigned long long>(), slots, EMPTY_KEY, s);
+    if (n) {
+        const unsigned g = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 1u << 20));
+        if (idx.wide)
+            hipLaunchKernelGGL(k_zi_insert<true>, dim3(g), dim3(256), 0, s, lo, hi, n, idx.lo.as<unsigned long long>(),
+                               idx.hi.as<unsigned long long>(), idx.mask);
+        else
+            hipLaunchKernelGGL(k_zi_insert<false>, dim3(g), dim3(256), 0, s, lo, hi, n,
+                               idx.lo.as<unsigned long long>(), (unsigned long long *)nullptr, idx.mask);
+    }
+    MCZI_HIP(hipStreamSynchronize(s));
+    MCZI_HIP(hipGetLastError());
+}
+
 struct ZiWork {
     // host copies kept alive until the batch's asynchronous copies are done
     std::vector<uint64_t> h_off;
@@ -594,9 +659,10 @@ void zi_collect(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, Zi
     out.wait_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tw).count();
     MCZI_HIP(hipGetLastError());
     w.h_count.assign(n_req + 1, 0);
-    std::vector<uint64_t> h_start(n_req + 1, 0);
+    std::vector<uint64_t> h_start(n_req + 1, 0), h_cursor(n_req + 1, 0);
     unsigned long long lk = 0;
     MCZI_HIP(hipMemcpyAsync(w.h_count.data(), w.seg_count.as<void>(), n_req * 8, hipMemcpyDeviceToHost, s));
+    MCZI_HIP(hipMemcpyAsync(h_cursor.data(), w.seg_cursor.as<void>(), n_req * 8, hipMemcpyDeviceToHost, s));
     MCZI_HIP(hipMemcpyAsync(h_start.data(), w.seg_start.as<void>(), n_req * 8, hipMemcpyDeviceToHost, s));
     MCZI_HIP(hipMemcpyAsync(&lk, w.ctr.as<void>(), 8, hipMemcpyDeviceToHost, s));
     MCZI_HIP(hipStreamSynchronize(s));
@@ -604,8 +670,10 @@ void zi_collect(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, Zi
     out.absent.clear();
     out.absent.resize(n_req);
     out.windows.assign(n_req, 0);
+    out.absent_windows.assign(n_req, 0);
     for (uint32_t r = 0; r < n_req; r++) {
         out.windows[r] = w.h_base[r + 1] - w.h_base[r];
+        out.absent_windows[r] = h_cursor[r] - w.h_base[r];
         const uint64_t n = w.h_count[r];
         ZiKeys &kk = out.absent[r];
         kk.n = n;
@@ -883,6 +951,30 @@ uint32_t default_m(uint32_t k) {
     uint32_t m = std::min<uint32_t>(19, k - 2);
     return m % 2 == 1 ? m : m - 1;
 }
+
+}  // namespace
+
+ZiScan mczi::zi_subtract_files(const ZiIndex &idx, const char *const *qp, size_t n, hipStream_t s) {
+    HostReads q;
+    for (size_t i = 0; i < n; i++) parse_fastx_file(qp[i], idx.k, 0, q);
+    q.finish();
+    ZiBatchInput in;
+    in.off = q.off;
+    in.frag_req.assign(q.n_frag(), 0);
+    in.n_words = (q.n_bases() + 15) / 16;
+    in.spans.push_back(ZiWordSpan{q.words.data(), in.n_words, 0});
+    auto w = zi_make_work();
+    zi_enqueue(*w, idx, std::move(in), 1, s);
+    ZiQueryResult res;
+    zi_collect(*w, idx, 1, s, res);
+    ZiScan out;
+    out.windows = res.windows[0];
+    out.absent_windows = res.absent_windows[0];
+    out.absent = std::move(res.absent[0]);
+    return out;
+}
+
+namespace {
 
 std::vector<mczi_counted_kmer> zi_run(const char *index_path, const char *const *qp, size_t n, uint32_t k) {
     Device &d = Device::get();

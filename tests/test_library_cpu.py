@@ -156,3 +156,19 @@ def test_device_entry_points_fail_loudly_without_gpu(tmp_path):
     r = subprocess.run([sys.executable, "-c", code, str(tmp_path / "x.fa")], capture_output=True, text=True,
                        cwd=ROOT, env=env, timeout=120)
     assert "ERR 3" in r.stdout, r.stdout + r.stderr
+
+
+def test_mczi_argument_checks(tmp_path):
+    # MCZI's run() checks (src/mczi.rs:239-311, 885-889) come before any device work
+    f = tmp_path / "a.fa"
+    f.write_text(">a\nACGT\n")
+    for (k, m, ds), msg in [((2, 1, False), r"k must be in \[3, 63\]"), ((65, 21, False), r"k must be in \[3, 63\]"),
+                            ((30, 21, False), "k must be odd"), ((31, 31, False), "minimizer size must be > 0 and < k"),
+                            ((31, 0, False), "minimizer size must be > 0 and < k"),
+                            ((33, 21, True), "requires k <= 32")]:
+        with pytest.raises(M.MczaError, match=msg):
+            M.mczi_subtract([f], [f], M.CounterConfig(k, m, 1), index_datasets=ds)
+    with pytest.raises(M.MczaError, match="at least one index input"):
+        M.mczi_subtract([], [f], M.CounterConfig(31, 21, 1))
+    with pytest.raises(M.MczaError, match="at least one query input"):
+        M.mczi_subtract([f], [], M.CounterConfig(31, 21, 1))
