@@ -155,6 +155,36 @@ int mczi_batch_from_files(const char *const *paths, size_t n, uint32_t min_len,
  * words and *frag_off n_frag + 1 offsets; both are released with mczi_free. */
 int mczi_pack_fastx(const char *path, uint32_t min_len, uint32_t **words, uint64_t **frag_off,
                     size_t *n_frag);
+/* On-disk formats of the reference's partitioned path, for interop and spill
+ * (big-endian integers; packed-seq codes A0 C1 T2 G3, 4 bases per byte from
+ * bit 0, unused high bits of a last byte zero):
+ * - packed-read cache (write_packed_read_cache_batch / for_packed_read_cache_
+ *   batches, src/lib.rs:3394-3450): "MCRD0001", then per fragment a u32
+ *   length and its packed bytes. The reader returns the layout of
+ *   mczi_pack_fastx; fragments shorter than min_len are skipped.
+ * - super-k-mer partitions (partition_NNNN.skm; append_record_len,
+ *   add_packed_range and read_superkmer_partition, src/lib.rs:3209-3216,
+ *   4893-4917, 2692-2711): per record a length (one byte < 255, or 0xFF and a
+ *   u32) and its packed bases. mczi_write_superkmer_partitions runs the device
+ *   scan and writes the records of each partition spread_hash_u32(key) % P to
+ *   dir/partition_NNNN.skm (record splits differ from the reference's; the
+ *   k-mer multiset of each partition is the same). mczi_count_superkmer_
+ *   partitions counts such files on the device, as the reference's phase 3
+ *   (count > x, sorted). mczi_read_superkmer_partitions returns their records
+ *   as packed fragments.
+ * - counted k-mers (.cnt; process_superkmer_partition / emit_counted_partition,
+ *   src/lib.rs:2413-2421, 2657-2678): per k-mer u128 encoding and u64 count. */
+int mczi_write_read_cache(const char *fastx_path, uint32_t min_len, const char *cache_path);
+int mczi_read_read_cache(const char *cache_path, uint32_t min_len, uint32_t **words, uint64_t **frag_off,
+                         size_t *n_frag);
+int mczi_write_superkmer_partitions(const char *const *paths, size_t n, const mczi_counter_config *cfg,
+                                    const char *dir);
+int mczi_read_superkmer_partitions(const char *const *paths, size_t n, uint32_t min_len, uint32_t **words,
+                                   uint64_t **frag_off, size_t *n_frag);
+int mczi_count_superkmer_partitions(const char *const *paths, size_t n, const mczi_counter_config *cfg,
+                                    mczi_counted_kmer **out, size_t *n_out);
+int mczi_write_counted(const char *path, const mczi_counted_kmer *kmers, size_t n);
+int mczi_read_counted(const char *path, mczi_counted_kmer **out, size_t *n_out);
 int mczi_batch_from_packed(const uint32_t *words, const uint64_t *frag_off, size_t n_frag,
                            const uint32_t *dataset, mczi_batch **out);
 int mczi_batch_info(const mczi_batch *b, uint64_t *n_bases, uint64_t *n_frag,

@@ -131,6 +131,15 @@ _sig("mczi_batch_from_files", C.c_int, [_CP, C.c_size_t, C.c_uint32, C.c_int, C.
 _sig("mczi_pack_fastx", C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.POINTER(C.c_uint32)),
                                   C.POINTER(C.POINTER(C.c_uint64)), C.POINTER(C.c_size_t)])
 _sig("mczi_batch_from_packed", C.c_int, [_P, _P, C.c_size_t, _P, C.POINTER(_P)])
+_WORDS_OUT = [C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(C.POINTER(C.c_uint64)), C.POINTER(C.c_size_t)]
+_sig("mczi_write_read_cache", C.c_int, [C.c_char_p, C.c_uint32, C.c_char_p])
+_sig("mczi_read_read_cache", C.c_int, [C.c_char_p, C.c_uint32] + _WORDS_OUT)
+_sig("mczi_write_superkmer_partitions", C.c_int, [_CP, C.c_size_t, C.POINTER(_Config), C.c_char_p])
+_sig("mczi_read_superkmer_partitions", C.c_int, [_CP, C.c_size_t, C.c_uint32] + _WORDS_OUT)
+_sig("mczi_count_superkmer_partitions", C.c_int, [_CP, C.c_size_t, C.POINTER(_Config), C.POINTER(C.POINTER(_Counted)),
+                                                  C.POINTER(C.c_size_t)])
+_sig("mczi_write_counted", C.c_int, [C.c_char_p, _P, C.c_size_t])
+_sig("mczi_read_counted", C.c_int, [C.c_char_p, C.POINTER(C.POINTER(_Counted)), C.POINTER(C.c_size_t)])
 _sig("mczi_batch_info", C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)])
 _sig("mczi_batch_free", None, [_P])
 _sig("mczi_batch_count", C.c_int, [_P, C.POINTER(_Config), C.c_int, C.c_int, C.c_int, C.POINTER(RunStats)])
@@ -171,7 +180,9 @@ EXPORTED = [
     "mczi_count_inputs_to_kmer_fasta_path", "mczi_count_datasets_to_kmer_fasta_path", "mczi_write_fasta_path",
     "mczi_write_fasta_after_index", "mczi_write_kff_path", "mczi_simplitig_sequences",
     "mczi_simplitig_sequences_parallel", "mczi_decode_kmer", "mczi_minimizer_run_hashes",
-    "mczi_batch_from_files", "mczi_pack_fastx", "mczi_batch_from_packed", "mczi_batch_info", "mczi_batch_free", "mczi_batch_count",
+    "mczi_batch_from_files", "mczi_pack_fastx", "mczi_batch_from_packed", "mczi_write_read_cache",
+    "mczi_read_read_cache", "mczi_write_superkmer_partitions", "mczi_read_superkmer_partitions",
+    "mczi_count_superkmer_partitions", "mczi_write_counted", "mczi_read_counted", "mczi_batch_info", "mczi_batch_free", "mczi_batch_count",
     "mczi_batch_download", "mczi_batch_autotune", "mczi_copy_probe", "mczi_device_info", "mczi_zi_subtract", "mczi_zi_absent", "mczi_mczi_subtract", "mczi_mczi_absent", "mczi_zi_server_create",
     "mczi_zi_server_create_from_text", "mczi_zi_server_submit", "mczi_zi_server_submit_packed",
     "mczi_zi_server_poll", "mczi_zi_result_free",
@@ -329,6 +340,64 @@ def mczi_absent(index_inputs, query_inputs, cfg: CounterConfig, index_datasets: 
 
 def _counted_arr(kmers) -> np.ndarray:
     return np.ascontiguousarray(kmers, dtype=COUNTED)
+
+
+def _words_call(f, *args):
+    wp = C.POINTER(C.c_uint32)()
+    op = C.POINTER(C.c_uint64)()
+    n = C.c_size_t()
+    _check(f(*args, C.byref(wp), C.byref(op), C.byref(n)))
+    try:
+        off = np.ctypeslib.as_array(op, shape=(n.value + 1,)).copy()
+        nw = int((off[-1] + 15) // 16)
+        words = np.ctypeslib.as_array(wp, shape=(nw,)).copy() if nw else np.zeros(0, np.uint32)
+    finally:
+        _lib.mczi_free(C.cast(wp, C.c_void_p))
+        _lib.mczi_free(C.cast(op, C.c_void_p))
+    return words, off
+
+
+def write_read_cache(fastx_path, cache_path, min_len: int = 1) -> None:
+    """Packed-read cache (.pread) of one FASTA/FASTQ file."""
+    _check(_lib.mczi_write_read_cache(os.fspath(fastx_path).encode(), min_len, os.fspath(cache_path).encode()))
+
+
+def read_read_cache(cache_path, min_len: int = 1):
+    """(words, frag_off) of a packed-read cache, as pack_fastx returns them."""
+    return _words_call(_lib.mczi_read_read_cache, os.fspath(cache_path).encode(), min_len)
+
+
+def write_superkmer_partitions(inputs, cfg: CounterConfig, out_dir) -> None:
+    """The device scan's super-k-mer records as partition_NNNN.skm files."""
+    arr, n = _paths(inputs)
+    _check(_lib.mczi_write_superkmer_partitions(arr, n, C.byref(cfg._c()), os.fspath(out_dir).encode()))
+
+
+def read_superkmer_partitions(paths, min_len: int = 1):
+    """Records of .skm files as packed fragments (words, frag_off)."""
+    arr, n = _paths(paths)
+    return _words_call(_lib.mczi_read_superkmer_partitions, arr, n, min_len)
+
+
+def count_superkmer_partitions(paths, cfg: CounterConfig) -> np.ndarray:
+    """Count the k-mers of .skm partition files on the device (count > x, sorted)."""
+    arr, n = _paths(paths)
+    p = C.POINTER(_Counted)()
+    m = C.c_size_t()
+    _check(_lib.mczi_count_superkmer_partitions(arr, n, C.byref(cfg._c()), C.byref(p), C.byref(m)))
+    return _take(p, m.value)
+
+
+def write_counted(path, kmers) -> None:
+    a = _counted_arr(kmers)
+    _check(_lib.mczi_write_counted(os.fspath(path).encode(), a.ctypes.data if a.size else None, a.size))
+
+
+def read_counted(path) -> np.ndarray:
+    p = C.POINTER(_Counted)()
+    m = C.c_size_t()
+    _check(_lib.mczi_read_counted(os.fspath(path).encode(), C.byref(p), C.byref(m)))
+    return _take(p, m.value)
 
 
 def pack_fastx(path, min_len: int = 1):

@@ -292,12 +292,7 @@ const char *mczi_last_error(void) { return g_err.c_str(); }
 const char *mczi_version(void) { return "mczi-hip 0.1.0 (gfx950)"; }
 void mczi_free(void *p) { free(p); }
 
-int mczi_pack_fastx(const char *path, uint32_t min_len, uint32_t **words, uint64_t **frag_off,
-                    size_t *n_frag) {
-    return guarded("failed to read FASTA/FASTQ", [&] {
-        require(path && words && frag_off && n_frag, MCZI_E_INVALID, "null argument");
-        HostReads r;
-        parse_fastx_file(path, min_len, 0, r);
+static void reads_out(const HostReads &r, uint32_t **words, uint64_t **frag_off, size_t *n_frag) {
         const size_t nw = (size_t)((r.n_bases() + 15) / 16);
         *words = (uint32_t *)malloc(std::max<size_t>(nw, 1) * 4);
         *frag_off = (uint64_t *)malloc(r.off.size() * 8);
@@ -311,6 +306,95 @@ int mczi_pack_fastx(const char *path, uint32_t min_len, uint32_t **words, uint64
         if (nw) memcpy(*words, r.words.data(), nw * 4);
         memcpy(*frag_off, r.off.data(), r.off.size() * 8);
         *n_frag = (size_t)r.n_frag();
+}
+
+int mczi_pack_fastx(const char *path, uint32_t min_len, uint32_t **words, uint64_t **frag_off,
+                    size_t *n_frag) {
+    return guarded("failed to read FASTA/FASTQ", [&] {
+        require(path && words && frag_off && n_frag, MCZI_E_INVALID, "null argument");
+        HostReads r;
+        parse_fastx_file(path, min_len, 0, r);
+        reads_out(r, words, frag_off, n_frag);
+    });
+}
+
+int mczi_write_read_cache(const char *fastx_path, uint32_t min_len, const char *cache_path) {
+    return guarded("failed to write packed-read cache", [&] {
+        require(fastx_path && cache_path, MCZI_E_INVALID, "null argument");
+        HostReads r;
+        parse_fastx_file(fastx_path, min_len, 0, r);
+        write_read_cache(cache_path, r);
+    });
+}
+
+int mczi_read_read_cache(const char *cache_path, uint32_t min_len, uint32_t **words, uint64_t **frag_off,
+                         size_t *n_frag) {
+    return guarded("failed to read packed-read cache", [&] {
+        require(cache_path && words && frag_off && n_frag, MCZI_E_INVALID, "null argument");
+        HostReads r;
+        read_read_cache(cache_path, min_len, r);
+        reads_out(r, words, frag_off, n_frag);
+    });
+}
+
+int mczi_read_superkmer_partitions(const char *const *paths, size_t n, uint32_t min_len, uint32_t **words,
+                                   uint64_t **frag_off, size_t *n_frag) {
+    return guarded("failed to read super-k-mer partitions", [&] {
+        require((n == 0 || paths) && words && frag_off && n_frag, MCZI_E_INVALID, "null argument");
+        HostReads r;
+        for (size_t i = 0; i < n; i++) read_superkmer_file(paths[i], min_len, r);
+        reads_out(r, words, frag_off, n_frag);
+    });
+}
+
+int mczi_write_superkmer_partitions(const char *const *paths, size_t n, const mczi_counter_config *cfg,
+                                    const char *dir) {
+    return guarded("failed to write super-k-mer partitions", [&] {
+        require(cfg && dir && (n == 0 || paths), MCZI_E_INVALID, "null argument");
+        validate_normal(*cfg);
+        require(n > 0, MCZI_E_INVALID, "at least one input file is required");
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        auto b = load_inputs(to_paths(paths, n), cfg->k, false);
+        std::vector<uint64_t> rec;
+        std::vector<uint32_t> part;
+        uint32_t rec_words = cfg->k > 32 ? 4 : 2, p_int = 0;
+        if (b->n_frag) {
+            batch_count(*b, *cfg, MCZI_MODE_TWO_PASS, MCZI_LAUNCH_BATCHED, MCZI_ORDER_NONE, nullptr);
+            batch_download_records(*b, rec, part, rec_words, p_int);
+        }
+        write_superkmer_partitions(dir, cfg->partition_count, rec, part, rec_words);
+    });
+}
+
+int mczi_count_superkmer_partitions(const char *const *paths, size_t n, const mczi_counter_config *cfg,
+                                    mczi_counted_kmer **out, size_t *n_out) {
+    return guarded("failed to count super-k-mer partitions", [&] {
+        require(cfg && out && n_out && (n == 0 || paths), MCZI_E_INVALID, "null argument");
+        validate_normal(*cfg);
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        HostReads r;
+        for (size_t i = 0; i < n; i++) read_superkmer_file(paths[i], cfg->k, r);
+        auto b = make_batch(std::move(r));
+        std::vector<mczi_counted_kmer> v;
+        if (b->n_frag) {
+            batch_count(*b, *cfg, MCZI_MODE_TWO_PASS, MCZI_LAUNCH_BATCHED, MCZI_ORDER_ENCODED, nullptr);
+            batch_download(*b, v);
+        }
+        copy_out(v, out, n_out);
+    });
+}
+
+int mczi_write_counted(const char *path, const mczi_counted_kmer *kmers, size_t n) {
+    return guarded("failed to write counted k-mers", [&] {
+        require(path && (kmers || n == 0), MCZI_E_INVALID, "null argument");
+        write_counted_file(path, kmers, n);
+    });
+}
+
+int mczi_read_counted(const char *path, mczi_counted_kmer **out, size_t *n_out) {
+    return guarded("failed to read counted k-mers", [&] {
+        require(path && out && n_out, MCZI_E_INVALID, "null argument");
+        copy_out(read_counted_file(path), out, n_out);
     });
 }
 

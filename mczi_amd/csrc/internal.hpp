@@ -223,6 +223,13 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
                  mczi_run_stats *stats);
 void batch_download(Batch &b, std::vector<mczi_counted_kmer> &out);
 void batch_download(Batch &b, mczi_counted_kmer *out);  // b.n_out records
+// The super-k-mer records of the last count, sorted by internal partition:
+// rec_words u64 words each (2 for k <= 32: bases 0-55 at 2 bits, MC codes,
+// base count at bits 112-119; 4 for k > 32: bases 0-111, count at bits
+// 240-247), with each record's internal partition (of p_int; the user
+// partition is that modulo the partition count).
+void batch_download_records(Batch &b, std::vector<uint64_t> &rec, std::vector<uint32_t> &part, uint32_t &rec_words,
+                            uint32_t &p_int);
 // Only the emitted k-mers' low (and, for k > 32, high) words.
 void batch_download_keys(Batch &b, uint64_t *lo, uint64_t *hi);
 double batch_autotune(Batch &b, const mczi_counter_config &cfg, int mode, int launch, int order);
@@ -272,6 +279,21 @@ void write_simplitig_fasta_keys(const std::string &path, size_t n, uint32_t k,
 // link_host[0 .. 2n) (graph_kernels.hip).
 void device_forced_links(const unsigned long long *lo, const unsigned long long *hi, uint64_t n, uint32_t k,
                          uint64_t *link_host);
+
+// ---------------------------------------------------------------------------
+// On-disk formats of the reference's partitioned path (interop.cpp).
+// Packed-read cache (.pread): "MCRD0001", then per fragment a u32 BE length
+// and ceil(len / 4) bytes of packed-seq codes, 4 bases per byte from bit 0.
+void write_read_cache(const std::string &path, const HostReads &r);
+void read_read_cache(const std::string &path, uint64_t min_len, HostReads &out);
+// Super-k-mer partition files (partition_NNNN.skm): per record a length
+// (one byte, or 0xFF and a u32 BE) and its packed bases.
+void write_superkmer_partitions(const std::string &dir, uint32_t partitions, const std::vector<uint64_t> &rec,
+                                const std::vector<uint32_t> &part, uint32_t rec_words);
+void read_superkmer_file(const std::string &path, uint64_t min_len, HostReads &out);
+// Counted k-mer records (.cnt): u128 BE k-mer, u64 BE count.
+void write_counted_file(const std::string &path, const mczi_counted_kmer *kmers, size_t n);
+std::vector<mczi_counted_kmer> read_counted_file(const std::string &path);
 
 // Phase log lines on stderr, in the reference's formats (src/lib.rs:994-1062).
 void log_phase(const char *name, double seconds);

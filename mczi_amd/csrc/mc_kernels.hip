@@ -1818,6 +1818,22 @@ void batch_download(Batch &b, mczi_counted_kmer *out) {
     d.download(out, rec.as<void>(), b.n_out * sizeof(mczi_counted_kmer), d.stream());
 }
 
+void batch_download_records(Batch &b, std::vector<uint64_t> &rec, std::vector<uint32_t> &part, uint32_t &rec_words,
+                            uint32_t &p_int) {
+    require(b.plan != nullptr, MCZI_E_INVALID, "the batch has not been counted");
+    McPlan &p = *b.plan;
+    Device &d = Device::get();
+    const bool wide = p.k > 32;
+    rec_words = wide ? 4 : 2;
+    p_int = p.p_int;
+    const uint64_t n = p.n_staged;
+    rec.resize(n * rec_words);
+    part.resize(n);
+    if (!n) return;
+    d.download(rec.data(), wide ? p.records_w.as<void>() : p.records.as<void>(), n * rec_words * 8, d.stream());
+    d.download(part.data(), p.pqs.as<void>(), n * 4, d.stream());
+}
+
 void batch_download_keys(Batch &b, uint64_t *lo, uint64_t *hi) {
     if (!b.n_out) return;
     Device &d = Device::get();

@@ -163,6 +163,32 @@ def test_simplitigs_with_device_links_match_host_writer(tmp_path, k, m):
     assert dev.read_bytes() == host.read_bytes()
 
 
+@pytest.mark.parametrize("k,m", [(31, 21), (45, 27)])
+def test_superkmer_partition_files_round_trip(tmp_path, k, m):
+    # reads -> device scan -> partition_NNNN.skm files -> device count of the
+    # files == count of the reads; each k-mer lives in one partition only
+    spec = M.SynthSpec(genome_len=300_000, genome_seed=4, read_seed=104, n_reads=40_000, read_len=150,
+                       sub_rate=0.005, repeat_fraction=0.05, repeat_len=2000, repeat_copies=3)
+    r = M.SynthReads(spec)
+    fa = tmp_path / "r.fa"
+    r.write_fasta(fa)
+    cfg = M.CounterConfig(k, m, 0, 16)
+    out = tmp_path / "parts"
+    out.mkdir()
+    M.write_superkmer_partitions([fa], cfg, out)
+    files = sorted(out.glob("partition_*.skm"))
+    assert [f.name for f in files] == [f"partition_{u:04d}.skm" for u in range(16)]
+    for x in (0, 2):
+        c = M.CounterConfig(k, m, x, 16)
+        assert same_counted(M.count_superkmer_partitions(files, c), M.count_inputs([fa], c)), x
+    seen = set()
+    for f in files:
+        kms = M.count_superkmer_partitions([f], cfg)
+        mine = set(zip(kms["hi"].tolist(), kms["lo"].tolist()))
+        assert not (mine & seen)
+        seen |= mine
+
+
 def test_datasets_match_oracle(tmp_path, golden):
     for case in golden["datasets"]:
         paths = []
