@@ -1128,6 +1128,83 @@ __global__ void k_gather_kmer(const unsigned long long *ks, const uint32_t *cs, 
     }
 }
 
+// Finishing pass of the output sort: the pairs are already ordered by the top
+// bits of the key (key >> shift), so only runs of equal prefixes (a handful
+// of k-mers each) are left to order. Each block takes a chunk of FIN_CHUNK
+// positions widened to whole runs (a run belongs to the chunk where it
+// starts), stages it in LDS, insertion-sorts every run there, and writes it
+// back. Run bounds depend only on the prefix, which sorting inside a run never
+// changes and which lies in the high 32 bits (shift >= 32), so reading a
+// neighbour's elements while it rewrites them still sees the same bounds. A
+// run longer than FINISH_MAX sets *flag; the caller then sorts in full.
+constexpr int FINISH_MAX = 64;
+constexpr int FIN_CHUNK = 2048;
+__global__ __launch_bounds__(256) void k_finish_runs(unsigned long long *keys, uint32_t *vals, uint64_t n,
+                                                     int shift, unsigned long long *flag) {
+    __shared__ unsigned long long sk[FIN_CHUNK + FINISH_MAX + 1];
+    __shared__ uint32_t sv[FIN_CHUNK + FINISH_MAX + 1];
+    __shared__ uint64_t s_a, s_b;
+    for (uint64_t c0 = (uint64_t)blockIdx.x * FIN_CHUNK; c0 < n; c0 += (uint64_t)gridDim.x * FIN_CHUNK) {
+        const uint64_t c1 = c0 + FIN_CHUNK < n ? c0 + FIN_CHUNK : n;
+        __syncthreads();  // the previous chunk is written back
+        if (threadIdx.x < 64) {
+            // [a, b): from the first run starting in [c0, c1) to the end of
+            // the run holding c1 - 1; the first wave tests 64 positions at
+            // once on each side (FINISH_MAX = 64)
+            const int lane = threadIdx.x;
+            uint64_t a = c0;
+            if (c0 > 0) {
+                const unsigned long long pp = keys[c0 - 1] >> shift;
+                const uint64_t i = c0 + (uint64_t)lane;
+                const bool other = i >= c1 || (keys[i] >> shift) != pp;
+                const unsigned long long m = __ballot(other);
+                a = m ? c0 + (uint64_t)__ffsll((long long)m) - 1 : c0 + 64;
+                if (!m) a = c0 + FINISH_MAX + 1;  // the run goes on too far
+            }
+            uint64_t b = a;
+            if (a < c1) {
+                const unsigned long long pl = keys[c1 - 1] >> shift;
+                const uint64_t i = c1 + (uint64_t)lane;
+                const bool other = i >= n || (keys[i] >> shift) != pl;
+                const unsigned long long m = __ballot(other);
+                b = m ? c1 + (uint64_t)__ffsll((long long)m) - 1 : c1 + FINISH_MAX;
+                if (!m && lane == 0) atomicOr(flag, 1ull);
+            }
+            if (lane == 0) {
+                if (a > c0 + FINISH_MAX) atomicOr(flag, 1ull);
+                s_a = a < c1 ? a : c1;
+                s_b = a < c1 ? b : c1;
+            }
+        }
+        __syncthreads();
+        const uint64_t a = s_a, b = s_b;
+        const int len = (int)(b - a);
+        for (int i = threadIdx.x; i < len; i += blockDim.x) {
+            sk[i] = keys[a + i];
+            sv[i] = vals[a + i];
+        }
+        __syncthreads();
+        // every element goes to its run's start plus the number of smaller
+        // keys in its run (keys are distinct)
+        for (int i = threadIdx.x; i < len; i += blockDim.x) {
+            const unsigned long long key = sk[i], pfx = key >> shift;
+            int rs = i;
+            while (rs > 0 && (sk[rs - 1] >> shift) == pfx && i - rs < FINISH_MAX) rs--;
+            int rank = 0, e = rs;
+            while (e < len && (sk[e] >> shift) == pfx && e - rs <= FINISH_MAX) {
+                rank += sk[e] < key;
+                e++;
+            }
+            if (e - rs > FINISH_MAX) {
+                atomicOr(flag, 1ull);
+                continue;
+            }
+            keys[a + rs + rank] = key;
+            vals[a + rs + rank] = sv[i];
+        }
+    }
+}
+
 // Output records in the C ABI's layout (lo, hi, count as u64), so the host
 // receives them with one copy.
 __global__ void k_pack_counted(const unsigned long long *__restrict__ lo,
@@ -1540,10 +1617,31 @@ static void sort_output(Batch &b, McPlan &p, uint64_t n, int order, uint32_t p_u
     }
     if (order == MCZI_ORDER_ENCODED) {
         const unsigned end_bit = (unsigned)std::min(64, 2 * k);
-        MCZI_HIP(rocprim::radix_sort_pairs(nullptr, tmp, kin, kout, cin, cout, (size_t)n, 0, end_bit, s));
-        p.sort_tmp.ensure(tmp + 16);
+        // Radix passes over the top 24 bits only, then the runs of equal
+        // 24-bit prefixes (about n / 2^24 k-mers each) finished in place;
+        // keys up to 56 bits, or a run too long to finish, take the full sort.
+        constexpr unsigned PREFIX_BITS = 24;
+        const bool split = end_bit >= 56 && (n >> PREFIX_BITS) < 64;
+        const unsigned begin_bit = split ? end_bit - PREFIX_BITS : 0;
+        size_t t_full = 0;
+        MCZI_HIP(rocprim::radix_sort_pairs(nullptr, tmp, kin, kout, cin, cout, (size_t)n, begin_bit, end_bit, s));
+        MCZI_HIP(rocprim::radix_sort_pairs(nullptr, t_full, kin, kout, cin, cout, (size_t)n, 0, end_bit, s));
+        p.sort_tmp.ensure(std::max(tmp, t_full) + 16);
         MCZI_HIP(rocprim::radix_sort_pairs(p.sort_tmp.as<void>(), tmp, kin, kout, cin, cout, (size_t)n,
-                                           0, end_bit, s));
+                                           begin_bit, end_bit, s));
+        if (!split) return;
+        unsigned long long *flag = p.ctr.as<unsigned long long>() + 15;
+        fill_u64(flag, 1, 0, s);
+        hipLaunchKernelGGL(k_finish_runs, dim3(grid_for(n, FIN_CHUNK, 1u << 20)), dim3(256), 0, s, kout, cout,
+                           n, (int)begin_bit, flag);
+        unsigned long long h = 0;
+        MCZI_HIP(hipMemcpyAsync(&h, flag, 8, hipMemcpyDeviceToHost, s));
+        MCZI_HIP(hipStreamSynchronize(s));
+        if (h) {
+            t_full = std::max(tmp, t_full);
+            MCZI_HIP(rocprim::radix_sort_pairs(p.sort_tmp.as<void>(), t_full, kin, kout, cin, cout, (size_t)n, 0,
+                                               end_bit, s));
+        }
         return;
     }
     // KFF u8 order: (partition % P_user, encoded & 255, encoded); two stable

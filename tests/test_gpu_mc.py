@@ -189,6 +189,22 @@ def test_superkmer_partition_files_round_trip(tmp_path, k, m):
         seen |= mine
 
 
+def test_output_order_with_long_prefix_runs(tmp_path):
+    # The encoded-order sort finishes runs of k-mers that share their top 24
+    # bits in place; 300 k-mers that start with twelve A's form one run too
+    # long for that, and the full sort must take over.
+    rng = np.random.default_rng(31)
+    reads = ["A" * 12 + "".join(rng.choice(list("ACG"), size=19)) for _ in range(300)]
+    reads += ["".join(rng.choice(list("ACGT"), size=150)) for _ in range(2000)]
+    fa = tmp_path / "r.fa"
+    write_fasta(fa, reads)
+    codes, off = O.parse_fastx(str(fa), 31)
+    want = O.count_bruteforce(codes, off, 31, 0)
+    got = M.count_inputs([fa], M.CounterConfig(31, 21, 0, 64))
+    assert same_counted(got, want)
+    assert int((want["lo"] >> np.uint64(38) == 0).sum()) >= 300
+
+
 def test_datasets_match_oracle(tmp_path, golden):
     for case in golden["datasets"]:
         paths = []
