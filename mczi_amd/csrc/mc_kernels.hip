@@ -648,12 +648,13 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
     const uint64_t kmask = k >= 32 ? ~0ull : ((1ull << (2 * k)) - 1ull);
     const int rshift = 64 - 2 * k;
     unsigned long long my_distinct = 0;
+    // the table starts empty; emit leaves it empty again for the next partition
+    for (int i = tid; i < CNT_SLOTS; i += CNT_THREADS) {
+        s_keys[i] = EMPTY_KEY;
+        s_vals[i] = 0;
+    }
+    __syncthreads();
     for (uint32_t q = blockIdx.x; q < a.p_int; q += gridDim.x) {
-        for (int i = tid; i < CNT_SLOTS; i += CNT_THREADS) {
-            s_keys[i] = EMPTY_KEY;
-            s_vals[i] = 0;
-        }
-        __syncthreads();
         const uint64_t rb = a.base[q], re = a.cursor[q];
         for (uint64_t wb = rb + (uint64_t)wid * 64; wb < re; wb += CNT_THREADS) {
             const uint64_t ri = wb + lane;
@@ -758,14 +759,17 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
         unsigned long long pos = s_obase + (unsigned long long)(pre + x - cnt);
 #pragma unroll
         for (int i = 0; i < PER; i++) {
-            if (!(emask & (1u << i))) continue;
             const int sl = i * CNT_THREADS + tid;
-            if (pos < a.out_cap) {
-                a.out_kmer[pos] = s_keys[sl];
-                a.out_count[pos] = (MODE == 0) ? s_vals[sl] : (uint32_t)__popc(s_vals[sl]);
-                a.out_part[pos] = q;
+            if (emask & (1u << i)) {
+                if (pos < a.out_cap) {
+                    a.out_kmer[pos] = s_keys[sl];
+                    a.out_count[pos] = (MODE == 0) ? s_vals[sl] : (uint32_t)__popc(s_vals[sl]);
+                    a.out_part[pos] = q;
+                }
+                pos++;
             }
-            pos++;
+            s_keys[sl] = EMPTY_KEY;  // each thread resets the slots it emitted from
+            s_vals[sl] = 0;
         }
         __syncthreads();
     }
