@@ -36,6 +36,8 @@ constexpr int WORDS_SPAN = (15 + SPAN + 160) / 16 + 4;
 // one pad element every 8, so the 64 lanes of a wave hit distinct banks.
 #define PADDED(n) ((n) + (n) / 8 + 1)
 __device__ __forceinline__ int P8(int i) { return i + (i >> 3); }
+constexpr unsigned long long STAGE_CHUNK = 8192;  // staging slots a scan block reserves at a time
+static_assert(STAGE_CHUNK >= 2 * TILE, "an abandoned staging chunk must be at least half used");
 constexpr int REC_MAX_BASES = 56;             // bases per 128-bit super-k-mer record (k <= 32)
 constexpr int REC_MAX_BASES_W = 112;          // bases per 256-bit record (33 <= k <= 63)
 
@@ -73,16 +75,13 @@ struct ScanArgs {
     uint32_t p_mask;  // p_int - 1 when p_int is a power of two, else 0
     uint32_t wrec;
     uint32_t *win_out;
-    unsigned long long *stats;  // [0] pieces [1] records
-    // staging of super-k-mer records and their internal partition (null: count only)
+    unsigned long long *stats;  // [0] pieces [4] records
+    // staging of super-k-mer records and their internal partition
     ulonglong2 *stage_rec;
     Rec32 *stage_rec_w;  // k > 32
     uint32_t *stage_q;
-    int staging;         // records are written (else only counted)
     uint64_t stage_cap;
-    unsigned long long *stage_ctr;  // total records (one add per block)
-    uint32_t *tile_nrec;            // count-only pass: records of each tile
-    const unsigned long long *tile_base;  // staging pass: first record slot of each tile
+    unsigned long long *stage_ctr;  // record slots reserved so far (the batch total at the end)
     unsigned long long *err;
 };
 
@@ -181,11 +180,12 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
     __shared__ uint8_t s_flag[PADDED(TILE)];
     __shared__ int32_t s_fs[TILE + 2];
     __shared__ int s_wsum[SCAN_THREADS / 64];
-    __shared__ unsigned long long s_sbase;
+    __shared__ unsigned long long s_sbase, s_hole_b, s_hole_e;
 
     const int tid = threadIdx.x;
     const int k = a.k, m = a.m, w = a.w;
     unsigned long long my_pieces = 0, my_records = 0;
+    unsigned long long chunk_cur = 0, chunk_end = 0;  // thread 0: the block's current staging chunk
     // each thread owns positions r0 .. r0 + WPT - 1 of a tile; P8(r0 + c),
     // the padded LDS index, is 9 * tid + c + floor(c / 8) since r0 = 8 * tid
     const int r0 = tid * WPT;
@@ -197,8 +197,6 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
         const uint64_t f0 = a.tile_frag[t];
         const int rb0 = (int)(ts & 15u);
         const int nwords = ((rb0 + tl + 160) >> 4) + 3;
-        unsigned long long tbase = 0;
-        if (MODE == STAGE && a.staging && tid == 0) tbase = a.tile_base[t];
         __syncthreads();  // previous tile fully consumed
         for (int i = tid; i < WORDS_SPAN; i += SCAN_THREADS) s_words[i] = i < nwords ? a.words[(ts >> 4) + i] : 0u;
         // Fragment boundaries relative to ts: s_fs[j] = off[f0 + j] - ts (clamped),
@@ -401,8 +399,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
 
         // Pieces -> super-k-mer records of at most wrec windows (bases
         // [start, end + k - 1), src/lib.rs:3483-3560), staged contiguously per
-        // tile (one reservation per tile; with no staging buffer only the
-        // count is taken) together with their partition: key =
+        // tile together with their partition: key =
         // spread_hash_u32(minimizer hash) (visit_direct_hash_runs), partition
         // = spread_hash_u32(key) % P (minimizer_shard, src/lib.rs:2927-2929).
         // Bases are stored in MC's 2-bit code (A0 C1 G2 T3, src/lib.rs:
@@ -419,16 +416,35 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
         }
         int tot = 0;
         const int off = block_exclusive_scan(my_nrec, &tot, s_wsum);
-        // Output slots: the count-only pass stores each tile's record count;
-        // their exclusive scan gives the staging pass every tile's first slot
-        // (records land in tile order, no device-scope atomics per tile).
+        // Staging slots: each block takes chunks of STAGE_CHUNK slots (or the
+        // tile's records, if more) from the batch-wide counter and hands out
+        // consecutive slots to its tiles. A tile that does not fit in the
+        // rest of the chunk starts a new one; the rest gets the partition id
+        // p_int, which sorts after every real partition, so partitioning
+        // leaves such padding at the end. Slots at or beyond stage_cap are
+        // counted but not written: the host then sizes the buffer from the
+        // counts and scans again. The order of tiles in the buffer is free:
+        // partitioning and counting do not depend on it.
         if (tid == 0) {
+            s_hole_b = s_hole_e = 0;
+            unsigned long long base = 0;
+            if (tot) {
+                if (chunk_cur + (unsigned long long)tot > chunk_end) {
+                    s_hole_b = chunk_cur;
+                    s_hole_e = chunk_end;
+                    const unsigned long long sz = (unsigned long long)tot > STAGE_CHUNK ? tot : STAGE_CHUNK;
+                    chunk_cur = atomicAdd(a.stage_ctr, sz);
+                    chunk_end = chunk_cur + sz;
+                }
+                base = chunk_cur;
+                chunk_cur += (unsigned long long)tot;
+            }
+            s_sbase = base;
             my_records += (unsigned long long)tot;
-            if (!a.staging) a.tile_nrec[t] = (uint32_t)tot;
-            s_sbase = tbase;
         }
         __syncthreads();
-        if (a.staging) {
+        for (uint64_t i = s_hole_b + tid; i < s_hole_e && i < a.stage_cap; i += SCAN_THREADS) a.stage_q[i] = a.p_int;
+        {
             uint64_t pos = s_sbase + (uint64_t)off;
             for (int pi = tid; pi < np; pi += SCAN_THREADS) {
                 const uint32_t pw = s_u.run.pst[pi];
@@ -484,9 +500,18 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             }
         }
     }
+    if constexpr (MODE == STAGE) {
+        __syncthreads();
+        if (tid == 0) {
+            s_hole_b = chunk_cur;
+            s_hole_e = chunk_end;
+        }
+        __syncthreads();
+        for (uint64_t i = s_hole_b + tid; i < s_hole_e && i < a.stage_cap; i += SCAN_THREADS) a.stage_q[i] = a.p_int;
+    }
     __shared__ unsigned long long s_red[SCAN_THREADS / 64];
     block_sum_atomic(my_pieces, &a.stats[0], s_red);
-    if constexpr (MODE == STAGE) block_sum_atomic(my_records, a.stage_ctr, s_red);
+    if constexpr (MODE == STAGE) block_sum_atomic(my_records, &a.stats[4], s_red);
 }
 
 // Partitioning (src/lib.rs:3483-3560 writes each super-k-mer to its
@@ -1230,14 +1255,14 @@ __global__ void k_pack_counted(const unsigned long long *__restrict__ lo,
 struct McPlan {
     uint32_t k = 0;
     // tile schedule
-    DevBuf tile_start, tile_frag, tile_len, tile_nrec, tile_base, tile_scan_tmp;
+    DevBuf tile_start, tile_frag, tile_len;
     uint64_t n_tiles = 0;
     std::vector<uint64_t> chunk_tile;  // tile index where each chunk starts
     uint64_t n_windows = 0;
     // state
     DevBuf base, cursor, records, ctr, ovf_keys, ovf_vals, ovf_part, stage_rec, stage_q, pqs, part_tmp;
     DevBuf records_w, stage_rec_w, ovf_hi, out_hi;  // k > 32
-    uint64_t stage_cap = 0, n_staged = 0;
+    uint64_t stage_cap = 0, n_staged = 0, n_records = 0;
     size_t part_tmp_bytes = 0;
     DevBuf out_kmer, out_count, out_part;
     DevBuf sort_tmp, key2, pu, idx, idx2, pu2, k2out;
@@ -1450,26 +1475,29 @@ static ScanArgs scan_args(Batch &b, McPlan &p, const mczi_counter_config &cfg) {
     sa.stats = p.ctr.as<unsigned long long>() + 8;
     sa.stage_rec = p.stage_rec.as<ulonglong2>();
     sa.stage_rec_w = p.stage_rec_w.as<Rec32>();
-    sa.staging = 1;
     sa.stage_q = p.stage_q.as<uint32_t>();
     sa.stage_cap = p.stage_cap;
     sa.stage_ctr = p.ctr.as<unsigned long long>() + 11;
-    sa.tile_nrec = p.tile_nrec.as<uint32_t>();
-    sa.tile_base = p.tile_base.as<unsigned long long>();
     sa.err = p.ctr.as<unsigned long long>() + 4;
     return sa;
 }
 
-static void launch_scan(McPlan &p, const ScanArgs &sa, int launch, hipStream_t s) {
+// Launches the scan (one grid per input chunk with the plain launch, one for
+// the batch otherwise); returns the number of blocks, or only counts them
+// when s_launch is false.
+static uint64_t launch_scan(McPlan &p, const ScanArgs &sa, int launch, hipStream_t s, bool s_launch = true) {
     Device &d = Device::get();
     static const unsigned scan_res = resident_blocks((const void *)k_scan<STAGE, true>, SCAN_THREADS);
     const unsigned scan_grid_max = (unsigned)d.cu_count() * scan_res;
+    uint64_t blocks = 0;
     auto go = [&](uint64_t tb, uint64_t te) {
         if (te <= tb) return;
         ScanArgs ca = sa;
         ca.tile_begin = tb;
         ca.tile_end = te;
         const unsigned g = (unsigned)std::min<uint64_t>(te - tb, scan_grid_max);
+        blocks += g;
+        if (!s_launch) return;
         const bool gw = ca.w >= WPT, wide = ca.k > 32;
         if (wide && gw) hipLaunchKernelGGL((k_scan<STAGE, true, true>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
         else if (wide) hipLaunchKernelGGL((k_scan<STAGE, false, true>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
@@ -1481,14 +1509,13 @@ static void launch_scan(McPlan &p, const ScanArgs &sa, int launch, hipStream_t s
     } else {
         go(0, p.n_tiles);
     }
+    return blocks;
 }
 
-// Stage A: clears, scan + record staging (with stage = false only the record
-// count). No host synchronization inside.
+// Stage A: clears, scan + record staging. No host synchronization inside.
 static void enqueue_stage_a(Batch &b, McPlan &p, const mczi_counter_config &cfg, int mode,
-                            int launch, hipStream_t s, Timer *tm, bool stage = true) {
+                            int launch, hipStream_t s, Timer *tm) {
     ScanArgs sa = scan_args(b, p, cfg);
-    if (!stage) sa.staging = 0;
     if (mode != MCZI_MODE_DATASETS) sa.dataset = nullptr;
     unsigned long long *ctr = p.ctr.as<unsigned long long>();
     fill_u64(ctr, 16, 0, s);
@@ -1533,7 +1560,8 @@ static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg,
     Device &d = Device::get();
     (void)b;
     (void)launch;
-    const uint64_t n = p.n_staged;
+    const uint64_t n = p.n_staged;   // staging slots, padding included
+    const uint64_t nr = p.n_records;  // records: the sorted prefix before the padding
     const bool wide = cfg.k > 32;
     fill_u64(p.ovf_keys.as<unsigned long long>(), p.ovf_slots, EMPTY_KEY, s);
     if (wide) fill_u64(p.ovf_hi.as<unsigned long long>(), p.ovf_slots, EMPTY_KEY, s);
@@ -1542,17 +1570,17 @@ static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg,
     fill_u64(p.cursor.as<unsigned long long>(), p.p_int, 0, s);
     if (tm) tm->mark(s);
     if (n) {
-        const unsigned g = grid_for(n, 256, 16384);
+        const unsigned g = grid_for(nr, 256, 16384);
         size_t tmp = p.part_tmp_bytes;
         if (wide)
             MCZI_HIP(rocprim::radix_sort_pairs(p.part_tmp.as<void>(), tmp, p.stage_q.as<uint32_t>(),
                                                p.pqs.as<uint32_t>(), p.stage_rec_w.as<Rec32>(),
-                                               p.records_w.as<Rec32>(), (size_t)n, 0, bits_for(p.p_int), s));
+                                               p.records_w.as<Rec32>(), (size_t)n, 0, bits_for(p.p_int + 1), s));
         else
             MCZI_HIP(rocprim::radix_sort_pairs(p.part_tmp.as<void>(), tmp, p.stage_q.as<uint32_t>(),
                                                p.pqs.as<uint32_t>(), p.stage_rec.as<ulonglong2>(),
-                                               p.records.as<ulonglong2>(), (size_t)n, 0, bits_for(p.p_int), s));
-        hipLaunchKernelGGL(k_part_bounds, dim3(g), dim3(256), 0, s, p.pqs.as<uint32_t>(), n,
+                                               p.records.as<ulonglong2>(), (size_t)n, 0, bits_for(p.p_int + 1), s));
+        hipLaunchKernelGGL(k_part_bounds, dim3(g), dim3(256), 0, s, p.pqs.as<uint32_t>(), nr,
                            p.base.as<unsigned long long>(), p.cursor.as<unsigned long long>());
     }
     if (tm) tm->mark(s);
@@ -1725,75 +1753,81 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
         p.out_count.ensure(p.out_cap * 4);
         p.out_part.ensure(p.out_cap * 4);
 
-        // Staged record count: measured once per (batch, k, m) by a count-only
-        // pass 1 (the records of a batch do not depend on x, mode or P).
+        // Staging capacity: the expected record count of a random minimizer
+        // scheme (density 2 / (w + 1) per window, plus record splits, one
+        // start per fragment and one cut per tile) with a 25% margin, never
+        // more than one record per window, plus the padding of each scan
+        // block's last chunk. The scan counts every slot it reserves; if the
+        // total exceeds the capacity, the batch is scanned again with room
+        // for twice the records (the padding of an abandoned chunk is less
+        // than its records) plus a full chunk per block.
         const uint64_t cfg_sig = mix_sig(mix_sig(k, cfg.minimizer), b.n_bases);
+        const uint32_t wrec = (uint32_t)((wide ? REC_MAX_BASES_W : REC_MAX_BASES) - k + 1);
+        const uint64_t chunk_pad = launch_scan(p, ScanArgs{}, launch, s, false) * STAGE_CHUNK;
         if (p.rec_sig != cfg_sig) {
-            if (p.graph) {
-                MCZI_HIP(hipGraphExecDestroy(p.graph));
-                p.graph = nullptr;
-            }
-            p.tile_nrec.ensure(std::max<uint64_t>(p.n_tiles, 1) * 4);
-            p.tile_base.ensure(std::max<uint64_t>(p.n_tiles, 1) * 8);
-            enqueue_stage_a(b, p, cfg, mode, launch, s, nullptr, false);
-            if (p.n_tiles) {
-                size_t tmp = 0;
-                MCZI_HIP(rocprim::exclusive_scan(nullptr, tmp, p.tile_nrec.as<uint32_t>(),
-                                                 p.tile_base.as<unsigned long long>(), 0ull, (size_t)p.n_tiles,
-                                                 rocprim::plus<unsigned long long>(), s));
-                p.tile_scan_tmp.ensure(tmp + 16);
-                MCZI_HIP(rocprim::exclusive_scan(p.tile_scan_tmp.as<void>(), tmp, p.tile_nrec.as<uint32_t>(),
-                                                 p.tile_base.as<unsigned long long>(), 0ull, (size_t)p.n_tiles,
-                                                 rocprim::plus<unsigned long long>(), s));
-            }
-            unsigned long long tot = 0;
-            MCZI_HIP(hipMemcpyAsync(&tot, p.ctr.as<unsigned long long>() + 11, 8, hipMemcpyDeviceToHost, s));
-            MCZI_HIP(hipStreamSynchronize(s));
-            require(tot < (1ull << 32), MCZI_E_INVALID, "too many super-k-mer records for one batch (limit 2^32)");
-            p.n_staged = tot;
-            p.stage_cap = tot + 16;
-            p.rec_cap = tot + 16;
-            if (wide) {
-                p.records_w.ensure(p.rec_cap * sizeof(Rec32));
-                p.stage_rec_w.ensure(p.stage_cap * sizeof(Rec32));
-            } else {
-                p.records.ensure(p.rec_cap * 16);
-                p.stage_rec.ensure(p.stage_cap * 16);
-            }
-            p.stage_q.ensure(p.stage_cap * 4);
-            p.pqs.ensure(p.stage_cap * 4);
+            const double w = (double)(k - cfg.minimizer + 1);
+            const double expect = (double)p.n_windows * (2.0 / (w + 1.0) + 1.0 / wrec) + (double)b.n_frag +
+                                  (double)p.n_tiles;
+            p.stage_cap = std::min<uint64_t>((uint64_t)(1.25 * expect) + 4096, p.n_windows) + chunk_pad + 16;
+            if (const char *e = getenv("MCZI_STAGE_CAPACITY"))  // initial capacity in slots
+                p.stage_cap = std::max<uint64_t>(strtoull(e, nullptr, 10), 1);
             p.rec_sig = cfg_sig;
         }
+        const bool want_times = stats != nullptr && !use_graph;
+        Timer tm(stats != nullptr);
+        auto t0 = std::chrono::steady_clock::now();
+        tm.mark(s);
+        unsigned long long h[16];
+        for (;;) {
+            if (wide) p.stage_rec_w.ensure(p.stage_cap * sizeof(Rec32));
+            else p.stage_rec.ensure(p.stage_cap * 16);
+            p.stage_q.ensure(p.stage_cap * 4);
+            enqueue_stage_a(b, p, cfg, mode, launch, s, want_times ? &tm : nullptr);
+            MCZI_HIP(hipMemcpyAsync(h, p.ctr.as<void>(), sizeof h, hipMemcpyDeviceToHost, s));
+            MCZI_HIP(hipStreamSynchronize(s));
+            MCZI_HIP(hipGetLastError());
+            if (h[11] <= p.stage_cap) break;
+            require(h[12] < (1ull << 32), MCZI_E_INVALID, "too many super-k-mer records for one batch (limit 2^32)");
+            require(p.stage_cap < 2 * h[12] + chunk_pad + 16, MCZI_E_DEVICE,
+                    "super-k-mer staging exceeded its bound");
+            p.stage_cap = 2 * h[12] + chunk_pad + 16;
+            tm.n = 1;
+        }
+        require(h[11] < (1ull << 32), MCZI_E_INVALID, "too many super-k-mer records for one batch (limit 2^32)");
+        p.n_staged = h[11];
+        p.n_records = h[12];
+        p.rec_cap = std::max<uint64_t>(p.rec_cap, p.n_staged + 16);
+        if (wide) p.records_w.ensure(p.rec_cap * sizeof(Rec32));
+        else p.records.ensure(p.rec_cap * 16);
+        p.pqs.ensure(p.rec_cap * 4);
         {
             size_t tmp = 0;
             if (wide)
                 MCZI_HIP(rocprim::radix_sort_pairs(nullptr, tmp, p.stage_q.as<uint32_t>(), p.pqs.as<uint32_t>(),
                                                    p.stage_rec_w.as<Rec32>(), p.records_w.as<Rec32>(),
                                                    (size_t)std::max<uint64_t>(p.n_staged, 1), 0,
-                                                   bits_for(p.p_int), s));
+                                                   bits_for(p.p_int + 1), s));
             else
                 MCZI_HIP(rocprim::radix_sort_pairs(nullptr, tmp, p.stage_q.as<uint32_t>(), p.pqs.as<uint32_t>(),
                                                    p.stage_rec.as<ulonglong2>(), p.records.as<ulonglong2>(),
                                                    (size_t)std::max<uint64_t>(p.n_staged, 1), 0,
-                                                   bits_for(p.p_int), s));
+                                                   bits_for(p.p_int + 1), s));
             p.part_tmp.ensure(tmp + 256);
             p.part_tmp_bytes = tmp;
         }
 
-        const bool want_times = stats != nullptr && !use_graph;
-        Timer tm(stats != nullptr);
-        auto t0 = std::chrono::steady_clock::now();
-        tm.mark(s);
+        // Stage B (partitioning, counting, emit) as one graph per batch
+        // layout: its sizes (records, partitions, spill slots) and buffers
+        // are fixed once the records are staged.
         if (use_graph) {
             uint64_t sig = cfg_sig;
             sig = mix_sig(mix_sig(mix_sig(sig, cfg.threshold), (uint64_t)mode), p.p_int);
             for (const DevBuf *db : {&b.words, &b.off, &b.dataset, &p.base, &p.cursor, &p.records, &p.ctr,
                                      &p.ovf_keys, &p.ovf_vals, &p.ovf_part, &p.out_kmer, &p.out_count,
                                      &p.out_part, &p.tile_start, &p.stage_rec, &p.stage_q, &p.pqs,
-                                     &p.part_tmp, &p.tile_nrec, &p.tile_base, &p.records_w, &p.stage_rec_w,
-                                     &p.ovf_hi, &p.out_hi})
+                                     &p.part_tmp, &p.records_w, &p.stage_rec_w, &p.ovf_hi, &p.out_hi})
                 sig = mix_sig(sig, (uint64_t)(uintptr_t)db->as<void>());
-            sig = mix_sig(mix_sig(sig, p.ovf_slots), p.stage_cap);
+            sig = mix_sig(mix_sig(mix_sig(sig, p.ovf_slots), p.stage_cap), p.n_staged);
             if (!p.graph || p.graph_sig != sig) {
                 if (p.graph) {
                     MCZI_HIP(hipGraphExecDestroy(p.graph));
@@ -1801,7 +1835,6 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
                 }
                 hipGraph_t g;
                 MCZI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                enqueue_stage_a(b, p, cfg, mode, launch, s, nullptr);
                 enqueue_stage_b(b, p, cfg, mode, launch, s, nullptr);
                 MCZI_HIP(hipStreamEndCapture(s, &g));
                 MCZI_HIP(hipGraphInstantiate(&p.graph, g, nullptr, nullptr, 0));
@@ -1810,16 +1843,14 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
             }
             MCZI_HIP(hipGraphLaunch(p.graph, s));
         } else {
-            enqueue_stage_a(b, p, cfg, mode, launch, s, want_times ? &tm : nullptr);
             enqueue_stage_b(b, p, cfg, mode, launch, s, want_times ? &tm : nullptr);
         }
         const int t_dev_end = tm.n;
         tm.mark(s);
-        unsigned long long h[16];
         MCZI_HIP(hipMemcpyAsync(h, p.ctr.as<void>(), sizeof h, hipMemcpyDeviceToHost, s));
         MCZI_HIP(hipStreamSynchronize(s));
         MCZI_HIP(hipGetLastError());
-        require((h[4] & 2ull) == 0 && h[11] == p.n_staged, MCZI_E_DEVICE,
+        require((h[4] & 2ull) == 0 && h[11] == p.n_staged && h[12] == p.n_records, MCZI_E_DEVICE,
                 "super-k-mer staging differs from its measured size");
         if (h[4] != 0 || h[1] > p.out_cap) {
             require(attempt < 4, MCZI_E_NOMEM, "k-mer spill table overflowed repeatedly");
@@ -1855,12 +1886,12 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
         st.n_bases = b.n_bases;
         st.n_windows = p.n_windows;
         st.n_pieces = h[8];
-        st.n_records = p.n_staged;
+        st.n_records = p.n_records;
         st.tile_windows = TILE;
         st.n_distinct = h[2] + h[3];
         st.n_overflow = h[3];
         st.n_out = n_out;
-        st.partition_bytes = p.n_staged * 16;
+        st.partition_bytes = p.n_records * 16;
         st.internal_partitions = p.p_int;
         st.lds_slots = CNT_SLOTS;
         if (want_times && t_dev_end >= 7) {
@@ -1928,7 +1959,7 @@ void batch_download_records(Batch &b, std::vector<uint64_t> &rec, std::vector<ui
     const bool wide = p.k > 32;
     rec_words = wide ? 4 : 2;
     p_int = p.p_int;
-    const uint64_t n = p.n_staged;
+    const uint64_t n = p.n_records;  // the padding slots sort after every record
     rec.resize(n * rec_words);
     part.resize(n);
     if (!n) return;

@@ -276,6 +276,34 @@ def test_batch_plain_equals_batched_and_spill(monkeypatch):
     assert same_counted(b2.download(), want)
 
 
+@pytest.mark.parametrize("k,m", [(31, 21), (45, 27)])
+def test_staging_grows_past_its_estimate(monkeypatch, k, m):
+    """The scan reserves record slots per tile from one counter; when the
+    records exceed the staging buffer's first capacity, the buffer is sized to
+    the exact total and the batch is scanned again, with the same result."""
+    print(f"staging regrow k={k} m={m}", flush=True)
+    spec = M.SynthSpec(genome_len=100_000, genome_seed=3, read_seed=103, n_reads=8_000, read_len=150,
+                       sub_rate=0.01)
+    r = M.SynthReads(spec)
+    codes = r.codes()
+    cfg = M.CounterConfig(k, m, 1, 64)
+    want = O.count_bruteforce(codes, r.frag_off, k, 1)
+    b = M.Batch.from_packed(r.words, r.frag_off)
+    st = b.count(cfg, M.MODE_TWO_PASS, M.LAUNCH_BATCHED)
+    assert same_counted(b.download(), want)
+    monkeypatch.setenv("MCZI_STAGE_CAPACITY", "1000")
+    b2 = M.Batch.from_packed(r.words, r.frag_off)
+    st2 = b2.count(cfg, M.MODE_TWO_PASS, M.LAUNCH_BATCHED)
+    assert int(st.n_records) > 1000
+    assert (int(st2.n_records), int(st2.n_pieces)) == (int(st.n_records), int(st.n_pieces))
+    assert same_counted(b2.download(), want)
+    monkeypatch.delenv("MCZI_STAGE_CAPACITY")
+    # the grown buffer is kept: later counts of the batch scan once
+    for launch in (M.LAUNCH_BATCHED, M.LAUNCH_PLAIN):
+        b2.count(cfg, M.MODE_TWO_PASS, launch)
+        assert same_counted(b2.download(), want)
+
+
 def _concat(a, b):
     """Two packed read sets in one segmented buffer (word-aligned; the gap
     between them is a short fragment that has no k-mer)."""
