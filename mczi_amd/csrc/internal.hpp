@@ -205,7 +205,11 @@ struct Batch {
     bool has_dataset = false;
     std::vector<uint64_t> chunk_frag;
     std::vector<uint64_t> host_off; // fragment offsets (host copy, for tiling)
-    // tile schedule (depends on k) and per-run state
+    // fragment lengths: how many of each length below 64, and the count and
+    // base total of the longer ones (window count for any k <= 64)
+    uint64_t len_hist[64] = {};
+    uint64_t long_cnt = 0, long_sum = 0;
+    // tile schedule (built with the batch) and per-run state
     std::unique_ptr<McPlan> plan;
     // results of the last count (device)
     DevBuf out_kmer, out_count, out_part, out_hi;  // out_hi: high words for k > 32

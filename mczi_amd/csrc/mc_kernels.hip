@@ -1311,6 +1311,8 @@ static void fill_zero_bytes(void *p, uint64_t bytes, hipStream_t s) {
                        (uint4 *)p, n16);
 }
 
+static void prepare_batch(Batch &b);
+
 std::unique_ptr<Batch> make_batch(HostReads &&r) {
     r.finish();
     auto b = std::make_unique<Batch>();
@@ -1332,6 +1334,7 @@ std::unique_ptr<Batch> make_batch(HostReads &&r) {
                                 hipMemcpyHostToDevice, d.stream()));
     }
     MCZI_HIP(hipStreamSynchronize(d.stream()));
+    prepare_batch(*b);
     return b;
 }
 
@@ -1363,20 +1366,29 @@ std::unique_ptr<Batch> make_batch_packed(const uint32_t *words, const uint64_t *
         b->dataset.reset(std::max<size_t>(n_frag, 1) * 4);
         d.upload(b->dataset.as<void>(), dataset, n_frag * 4, d.stream());
     }
+    prepare_batch(*b);
     return b;
 }
 
 // Tile schedule: tiles of TILE positions, clipped at chunk ends, each with
-// the fragment holding its first position.
-static void build_tiles(Batch &b, McPlan &p, uint32_t k) {
+// the fragment holding its first position, and the fragment length summary
+// that gives the window count for any k. Built once when the batch is
+// created; neither depends on k.
+static void build_tiles(Batch &b, McPlan &p) {
     std::vector<uint64_t> ts;
     std::vector<uint32_t> tf, tl;
     p.chunk_tile.clear();
     const auto &off = b.host_off;
-    p.n_windows = 0;
+    std::fill(std::begin(b.len_hist), std::end(b.len_hist), 0ull);
+    b.long_cnt = b.long_sum = 0;
     for (size_t i = 0; i + 1 < off.size(); i++) {
-        uint64_t n = off[i + 1] - off[i];
-        if (n >= k) p.n_windows += n - k + 1;
+        const uint64_t n = off[i + 1] - off[i];
+        if (n < 64) {
+            b.len_hist[n]++;
+        } else {
+            b.long_cnt++;
+            b.long_sum += n;
+        }
     }
     for (size_t c = 0; c + 1 < b.chunk_frag.size(); c++) {
         p.chunk_tile.push_back(ts.size());
@@ -1405,7 +1417,19 @@ static void build_tiles(Batch &b, McPlan &p, uint32_t k) {
                                 hipMemcpyHostToDevice, d.stream()));
     }
     MCZI_HIP(hipStreamSynchronize(d.stream()));
-    p.k = k;
+}
+
+static void prepare_batch(Batch &b) {
+    b.plan.reset(new McPlan());
+    build_tiles(b, *b.plan);
+}
+
+// Windows (k-mer starts) of the batch: sum over fragments of len - k + 1.
+static uint64_t windows_for(const Batch &b, uint32_t k) {
+    require(k >= 1 && k <= 64, MCZI_E_INVALID, "k must be greater than 0 and <= 64");
+    uint64_t n = b.long_sum - b.long_cnt * (k - 1);
+    for (uint32_t L = k; L < 64; L++) n += b.len_hist[L] * (L - k + 1);
+    return n;
 }
 
 static uint64_t next_pow2(uint64_t v) {
@@ -1730,9 +1754,9 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
                 "the device fof path supports at most 32 datasets per batch");
     }
     require(!wide || order != MCZI_ORDER_KFF_U8, MCZI_E_INVALID, "KFF u8 record order needs k <= 32");
-    if (!b.plan) b.plan.reset(new McPlan());
     McPlan &p = *b.plan;
-    if (p.k != k || p.n_tiles == 0) build_tiles(b, p, k);
+    p.n_windows = windows_for(b, k);
+    p.k = k;
     const bool use_graph = launch == MCZI_LAUNCH_BATCHED && getenv("MCZI_NO_GRAPH") == nullptr;
 
     double target = p.target_override > 0 ? p.target_override : tuned_target();
@@ -1953,7 +1977,7 @@ void batch_download(Batch &b, mczi_counted_kmer *out) {
 
 void batch_download_records(Batch &b, std::vector<uint64_t> &rec, std::vector<uint32_t> &part, uint32_t &rec_words,
                             uint32_t &p_int) {
-    require(b.plan != nullptr, MCZI_E_INVALID, "the batch has not been counted");
+    require(b.plan != nullptr && b.plan->k != 0, MCZI_E_INVALID, "the batch has not been counted");
     McPlan &p = *b.plan;
     Device &d = Device::get();
     const bool wide = p.k > 32;
@@ -2028,8 +2052,7 @@ void device_run_hashes(const uint8_t *packed, uint64_t n_bases, uint32_t k, uint
     for (uint64_t i = 0; i < n_bases; i++) r.append_base((packed[i / 4] >> (2 * (i % 4))) & 3u);
     r.close_fragment(k, 0);
     auto b = make_batch(std::move(r));
-    McPlan p;
-    build_tiles(*b, p, k);
+    McPlan &p = *b->plan;
     Device &d = Device::get();
     hipStream_t s = d.stream();
     const uint64_t nw = n_bases - k + 1;
