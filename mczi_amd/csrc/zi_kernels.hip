@@ -209,6 +209,9 @@ template <bool WIDE>
 __global__ __launch_bounds__(ZI_THREADS) void k_zi_lookup(ZiTileArgs a) {
     __shared__ int32_t s_fs[ZI_TILE + 2];
     __shared__ uint32_t s_words[ZI_WORDS];
+    __shared__ uint32_t s_cnt[ZI_WPT * (ZI_THREADS / 64)];  // absent k-mers per (window i, wave)
+    __shared__ uint32_t s_rmin[ZI_THREADS / 64], s_rmax[ZI_THREADS / 64];
+    __shared__ unsigned long long s_base;
     unsigned long long my_lookups = 0;
     const int lane = threadIdx.x & 63;
     for (uint64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
@@ -265,24 +268,87 @@ __global__ __launch_bounds__(ZI_THREADS) void k_zi_lookup(ZiTileArgs a) {
                 }
             }
         }
-        // append absent k-mers, aggregated per request within the wave
+        // Append the absent k-mers to their request's segment. When they all
+        // belong to one request (every tile but those that straddle request
+        // boundaries), the block takes one reservation for the tile and each
+        // wave writes its k-mers of window i as one contiguous run (order:
+        // window i, then wave); otherwise each wave reserves per window i and
+        // request. Same-address returning atomics were the lookup's bound.
+        const int wid = (int)threadIdx.x >> 6;
+        uint32_t rmin = NO_REQ, rmax = 0;
+#pragma unroll
+        for (int i = 0; i < ZI_WPT; i++)
+            if ((absent >> i) & 1u) {
+                rmin = min(rmin, req[i]);
+                rmax = max(rmax, req[i]);
+            }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            rmin = min(rmin, (uint32_t)__shfl_xor((int)rmin, o, 64));
+            rmax = max(rmax, (uint32_t)__shfl_xor((int)rmax, o, 64));
+        }
+        uint64_t abm[ZI_WPT];
 #pragma unroll
         for (int i = 0; i < ZI_WPT; i++) {
-            const bool ab = (absent >> i) & 1u;
-            uint64_t todo = __ballot(ab);
-            while (todo) {
-                const int leader = __ffsll((long long)todo) - 1;
-                const uint32_t rl = __shfl(req[i], leader, 64);
-                const uint64_t same = __ballot(ab && req[i] == rl);
-                unsigned long long base = 0;
-                if (lane == leader) base = atomicAdd(&a.seg_cursor[rl], (unsigned long long)__popcll(same));
-                base = __shfl(base, leader, 64);
-                if (ab && req[i] == rl) {
-                    const unsigned long long pos = base + __popcll(same & ((1ull << lane) - 1ull));
+            abm[i] = __ballot((absent >> i) & 1u);
+            if (lane == 0) s_cnt[i * (ZI_THREADS / 64) + wid] = (uint32_t)__popcll(abm[i]);
+        }
+        if (lane == 0) {
+            s_rmin[wid] = rmin;
+            s_rmax[wid] = rmax;
+        }
+        __syncthreads();
+        uint32_t bmin = NO_REQ, bmax = 0;
+#pragma unroll
+        for (int w = 0; w < ZI_THREADS / 64; w++) {
+            bmin = min(bmin, s_rmin[w]);
+            bmax = max(bmax, s_rmax[w]);
+        }
+        if (bmin == NO_REQ) continue;  // nothing absent in this tile (uniform over the block)
+        const unsigned long long lt = (1ull << lane) - 1ull;
+        if (bmin == bmax) {
+            if (threadIdx.x == 0) {
+                uint32_t tot = 0;
+                for (int c = 0; c < ZI_WPT * (ZI_THREADS / 64); c++) tot += s_cnt[c];
+                s_base = atomicAdd(&a.seg_cursor[bmin], (unsigned long long)tot);
+            }
+            __syncthreads();
+            unsigned long long run = s_base;  // first slot of window i's k-mers
+#pragma unroll
+            for (int i = 0; i < ZI_WPT; i++) {
+                unsigned long long before = 0, tot_i = 0;
+#pragma unroll
+                for (int w = 0; w < ZI_THREADS / 64; w++) {
+                    const uint32_t c = s_cnt[i * (ZI_THREADS / 64) + w];
+                    before += w < wid ? c : 0u;
+                    tot_i += c;
+                }
+                if ((absent >> i) & 1u) {
+                    const unsigned long long pos = run + before + (unsigned long long)__popcll(abm[i] & lt);
                     a.out_lo[pos] = key[i].lo;
                     if constexpr (WIDE) a.out_hi[pos] = key[i].hi;
                 }
-                todo &= ~same;
+                run += tot_i;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < ZI_WPT; i++) {
+                const bool ab = (absent >> i) & 1u;
+                uint64_t todo = abm[i];
+                while (todo) {
+                    const int leader = __ffsll((long long)todo) - 1;
+                    const uint32_t rl = __shfl(req[i], leader, 64);
+                    const uint64_t same = __ballot(ab && req[i] == rl);
+                    unsigned long long base = 0;
+                    if (lane == leader) base = atomicAdd(&a.seg_cursor[rl], (unsigned long long)__popcll(same));
+                    base = __shfl(base, leader, 64);
+                    if (ab && req[i] == rl) {
+                        const unsigned long long pos = base + __popcll(same & lt);
+                        a.out_lo[pos] = key[i].lo;
+                        if constexpr (WIDE) a.out_hi[pos] = key[i].hi;
+                    }
+                    todo &= ~same;
+                }
             }
         }
     }
