@@ -1157,83 +1157,6 @@ __global__ void k_gather_kmer(const unsigned long long *ks, const uint32_t *cs, 
     }
 }
 
-// Finishing pass of the output sort: the pairs are already ordered by the top
-// bits of the key (key >> shift), so only runs of equal prefixes (a handful
-// of k-mers each) are left to order. Each block takes a chunk of FIN_CHUNK
-// positions widened to whole runs (a run belongs to the chunk where it
-// starts), stages it in LDS, insertion-sorts every run there, and writes it
-// back. Run bounds depend only on the prefix, which sorting inside a run never
-// changes and which lies in the high 32 bits (shift >= 32), so reading a
-// neighbour's elements while it rewrites them still sees the same bounds. A
-// run longer than FINISH_MAX sets *flag; the caller then sorts in full.
-constexpr int FINISH_MAX = 64;
-constexpr int FIN_CHUNK = 2048;
-__global__ __launch_bounds__(256) void k_finish_runs(unsigned long long *keys, uint32_t *vals, uint64_t n,
-                                                     int shift, unsigned long long *flag) {
-    __shared__ unsigned long long sk[FIN_CHUNK + FINISH_MAX + 1];
-    __shared__ uint32_t sv[FIN_CHUNK + FINISH_MAX + 1];
-    __shared__ uint64_t s_a, s_b;
-    for (uint64_t c0 = (uint64_t)blockIdx.x * FIN_CHUNK; c0 < n; c0 += (uint64_t)gridDim.x * FIN_CHUNK) {
-        const uint64_t c1 = c0 + FIN_CHUNK < n ? c0 + FIN_CHUNK : n;
-        __syncthreads();  // the previous chunk is written back
-        if (threadIdx.x < 64) {
-            // [a, b): from the first run starting in [c0, c1) to the end of
-            // the run holding c1 - 1; the first wave tests 64 positions at
-            // once on each side (FINISH_MAX = 64)
-            const int lane = threadIdx.x;
-            uint64_t a = c0;
-            if (c0 > 0) {
-                const unsigned long long pp = keys[c0 - 1] >> shift;
-                const uint64_t i = c0 + (uint64_t)lane;
-                const bool other = i >= c1 || (keys[i] >> shift) != pp;
-                const unsigned long long m = __ballot(other);
-                a = m ? c0 + (uint64_t)__ffsll((long long)m) - 1 : c0 + 64;
-                if (!m) a = c0 + FINISH_MAX + 1;  // the run goes on too far
-            }
-            uint64_t b = a;
-            if (a < c1) {
-                const unsigned long long pl = keys[c1 - 1] >> shift;
-                const uint64_t i = c1 + (uint64_t)lane;
-                const bool other = i >= n || (keys[i] >> shift) != pl;
-                const unsigned long long m = __ballot(other);
-                b = m ? c1 + (uint64_t)__ffsll((long long)m) - 1 : c1 + FINISH_MAX;
-                if (!m && lane == 0) atomicOr(flag, 1ull);
-            }
-            if (lane == 0) {
-                if (a > c0 + FINISH_MAX) atomicOr(flag, 1ull);
-                s_a = a < c1 ? a : c1;
-                s_b = a < c1 ? b : c1;
-            }
-        }
-        __syncthreads();
-        const uint64_t a = s_a, b = s_b;
-        const int len = (int)(b - a);
-        for (int i = threadIdx.x; i < len; i += blockDim.x) {
-            sk[i] = keys[a + i];
-            sv[i] = vals[a + i];
-        }
-        __syncthreads();
-        // every element goes to its run's start plus the number of smaller
-        // keys in its run (keys are distinct)
-        for (int i = threadIdx.x; i < len; i += blockDim.x) {
-            const unsigned long long key = sk[i], pfx = key >> shift;
-            int rs = i;
-            while (rs > 0 && (sk[rs - 1] >> shift) == pfx && i - rs < FINISH_MAX) rs--;
-            int rank = 0, e = rs;
-            while (e < len && (sk[e] >> shift) == pfx && e - rs <= FINISH_MAX) {
-                rank += sk[e] < key;
-                e++;
-            }
-            if (e - rs > FINISH_MAX) {
-                atomicOr(flag, 1ull);
-                continue;
-            }
-            keys[a + rs + rank] = key;
-            vals[a + rs + rank] = sv[i];
-        }
-    }
-}
-
 // Output records in the C ABI's layout (lo, hi, count as u64), so the host
 // receives them with one copy.
 __global__ void k_pack_counted(const unsigned long long *__restrict__ lo,
@@ -1688,7 +1611,7 @@ static void sort_output(Batch &b, McPlan &p, uint64_t n, int order, uint32_t p_u
         if (!split) return;
         unsigned long long *flag = p.ctr.as<unsigned long long>() + 15;
         fill_u64(flag, 1, 0, s);
-        hipLaunchKernelGGL(k_finish_runs, dim3(grid_for(n, FIN_CHUNK, 1u << 20)), dim3(256), 0, s, kout, cout,
+        hipLaunchKernelGGL(k_finish_runs<true>, dim3(grid_for(n, FIN_CHUNK, 1u << 20)), dim3(256), 0, s, kout, cout,
                            n, (int)begin_bit, flag);
         unsigned long long h = 0;
         MCZI_HIP(hipMemcpyAsync(&h, flag, 8, hipMemcpyDeviceToHost, s));

@@ -94,9 +94,14 @@ struct ZiBatchInput {
     std::vector<ZiWordSpan> spans;
     uint64_t n_words = 0;
 };
-// Enqueue upload + lookup + per-request sort + unique on stream s (async).
+// Enqueue upload + lookup on stream s (async).
 void zi_enqueue(ZiWork &w, const ZiIndex &idx, ZiBatchInput &&in, uint32_t n_req, hipStream_t s);
-// Wait for the batch and collect per-request results.
+// Wait for the lookup, then sort and deduplicate each request's absent
+// k-mers on the device; returns with their counts known (host wait in ms).
+double zi_finish(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s);
+// Copy each request's unique absent k-mers out (after zi_finish).
+void zi_copy_out(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, ZiQueryResult &out);
+// zi_finish + zi_copy_out.
 void zi_collect(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, ZiQueryResult &out);
 
 // All query files in one batch against idx: query windows, absent windows

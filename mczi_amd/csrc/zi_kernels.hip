@@ -372,9 +372,9 @@ __global__ void k_zi_copy_u64(const unsigned long long *src, unsigned long long 
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
 }
-__global__ void k_zi_iota(uint32_t *p, uint64_t n) {
+__global__ void k_zi_iota(uint32_t *p, uint64_t n, uint64_t base) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = (uint32_t)i;
+        p[i] = (uint32_t)(base + i);
 }
 __global__ void k_zi_gather_range(const unsigned long long *src, const uint32_t *idx, unsigned long long *dst,
                                   uint64_t b, uint64_t e) {
@@ -383,7 +383,7 @@ __global__ void k_zi_gather_range(const unsigned long long *src, const uint32_t 
 }
 
 // Unique within each request's sorted segment: flag the first of every run
-// of equal keys among the segment's filled entries [base, cursor).
+// of equal keys among the segment's filled entries [base, end).
 template <bool WIDE>
 __global__ void k_zi_flags(const unsigned long long *lo, const unsigned long long *hi,
                            const unsigned long long *seg_base, const unsigned long long *seg_end, uint32_t n_seg,
@@ -535,7 +535,7 @@ struct ZiWork {
     DevBuf words, off, req, ts, tf, tl, seg_base, seg_cursor, seg_count, seg_start, olo, ohi, slo, shi, ulo, uhi, idx,
         idx2, flags, pos, tmp, ctr;
     uint64_t cap = 0;
-    std::vector<uint64_t> h_base, h_count;
+    std::vector<uint64_t> h_base, h_count, h_start, h_absent;  // per request
     uint64_t lookups = 0;
     hipEvent_t done = nullptr;
     ~ZiWork() {
@@ -609,24 +609,11 @@ void zi_enqueue(ZiWork &w, const ZiIndex &idx, ZiBatchInput &&in, uint32_t n_req
     w.seg_cursor.ensure((n_req + 1) * 8);
     w.seg_count.ensure((n_req + 1) * 8);
     w.seg_start.ensure((n_req + 1) * 8);
-    w.flags.ensure((cap + 1) * 4);
-    w.pos.ensure((cap + 1) * 4);
-    MCZI_HIP(hipMemcpyAsync(w.seg_base.as<void>(), w.h_base.data(), (n_req + 1) * 8, hipMemcpyHostToDevice, s));
     MCZI_HIP(hipMemcpyAsync(w.seg_cursor.as<void>(), w.h_base.data(), (n_req + 1) * 8, hipMemcpyHostToDevice, s));
     w.olo.ensure(cap * 8);
-    w.slo.ensure(cap * 8);
-    w.ulo.ensure(cap * 8);
-    if (wide) {
-        w.ohi.ensure(cap * 8);
-        w.shi.ensure(cap * 8);
-        w.uhi.ensure(cap * 8);
-        w.idx.ensure(cap * 4);
-        w.idx2.ensure(cap * 4);
-    }
+    if (wide) w.ohi.ensure(cap * 8);
     w.ctr.ensure(64);
     zfill(w.ctr.as<unsigned long long>(), 8, 0, s);
-    zfill(w.olo.as<unsigned long long>(), cap, EMPTY_KEY, s);
-    if (wide) zfill(w.ohi.as<unsigned long long>(), cap, EMPTY_KEY, s);
 
     ZiTileArgs a{};
     a.words = w.words.as<uint32_t>();
@@ -650,114 +637,189 @@ void zi_enqueue(ZiWork &w, const ZiIndex &idx, ZiBatchInput &&in, uint32_t n_req
         if (wide) hipLaunchKernelGGL(k_zi_lookup<true>, dim3(g), dim3(ZI_THREADS), 0, s, a);
         else hipLaunchKernelGGL(k_zi_lookup<false>, dim3(g), dim3(ZI_THREADS), 0, s, a);
     }
-    const unsigned long long *sb = w.seg_base.as<unsigned long long>();
-    const unsigned long long *se = w.seg_cursor.as<unsigned long long>();
-    const int kbits = (int)std::min<uint32_t>(64, 2 * idx.k);
-    const int hbits = 2 * (int)idx.k - 64;
-    uint64_t max_seg = 1;
-    for (uint32_t r = 0; r < n_req; r++) max_seg = std::max<uint64_t>(max_seg, seg_cap[r]);
-    // Each request's segment is sorted with a device-wide radix sort; unused
-    // entries hold the all-ones sentinel, which sorts last (no canonical
-    // k-mer of k <= 63 equals it).
-    size_t tmp = 0, t2 = 0, t3 = 0;
-    if (!wide) {
-        MCZI_HIP(rocprim::radix_sort_keys(nullptr, tmp, w.olo.as<unsigned long long>(), w.slo.as<unsigned long long>(),
-                                          (size_t)max_seg, 0, kbits, s));
-    } else {
-        MCZI_HIP(rocprim::radix_sort_pairs(nullptr, tmp, w.olo.as<unsigned long long>(), w.slo.as<unsigned long long>(),
-                                           w.idx.as<uint32_t>(), w.idx2.as<uint32_t>(), (size_t)max_seg, 0, 64, s));
-        MCZI_HIP(rocprim::radix_sort_pairs(nullptr, t2, w.shi.as<unsigned long long>(), w.uhi.as<unsigned long long>(),
-                                           w.idx2.as<uint32_t>(), w.idx.as<uint32_t>(), (size_t)max_seg, 0, hbits, s));
-    }
-    MCZI_HIP(rocprim::exclusive_scan(nullptr, t3, w.flags.as<uint32_t>(), w.pos.as<uint32_t>(), 0u, (size_t)cap + 1,
-                                     rocprim::plus<uint32_t>(), s));
-    w.tmp.ensure(std::max(std::max(tmp, t2), t3) + 16);
-    if (wide) hipLaunchKernelGGL(k_zi_iota, dim3(zgrid(cap)), dim3(256), 0, s, w.idx.as<uint32_t>(), cap);
-    for (uint32_t r = 0; r < n_req; r++) {
-        const uint64_t b0 = w.h_base[r], n = seg_cap[r];
-        if (n == 0) continue;
-        size_t ts1 = tmp, ts2 = t2;
-        if (!wide) {
-            MCZI_HIP(rocprim::radix_sort_keys(w.tmp.as<void>(), ts1, w.olo.as<unsigned long long>() + b0,
-                                              w.slo.as<unsigned long long>() + b0, (size_t)n, 0, kbits, s));
-        } else {
-            // LSD over (lo, hi) carrying original positions
-            MCZI_HIP(rocprim::radix_sort_pairs(w.tmp.as<void>(), ts1, w.olo.as<unsigned long long>() + b0,
-                                               w.slo.as<unsigned long long>() + b0, w.idx.as<uint32_t>() + b0,
-                                               w.idx2.as<uint32_t>() + b0, (size_t)n, 0, 64, s));
-            hipLaunchKernelGGL(k_zi_gather_range, dim3(zgrid(n)), dim3(256), 0, s, w.ohi.as<unsigned long long>(),
-                               w.idx2.as<uint32_t>(), w.shi.as<unsigned long long>(), b0, b0 + n);
-            MCZI_HIP(rocprim::radix_sort_pairs(w.tmp.as<void>(), ts2, w.shi.as<unsigned long long>() + b0,
-                                               w.uhi.as<unsigned long long>() + b0, w.idx2.as<uint32_t>() + b0,
-                                               w.idx.as<uint32_t>() + b0, (size_t)n, 0, hbits, s));
-            hipLaunchKernelGGL(k_zi_gather_range, dim3(zgrid(n)), dim3(256), 0, s, w.olo.as<unsigned long long>(),
-                               w.idx.as<uint32_t>(), w.slo.as<unsigned long long>(), b0, b0 + n);
-        }
-    }
-    // unique per request: flags -> exclusive scan -> scatter
-    dim3 gg(zgrid(cap), std::min<uint32_t>(std::max<uint32_t>(n_req, 1), 64));
-    const unsigned long long *khi = wide ? w.uhi.as<unsigned long long>() : nullptr;
-    if (wide)
-        hipLaunchKernelGGL(k_zi_flags<true>, gg, dim3(256), 0, s, w.slo.as<unsigned long long>(), khi, sb, se, n_req,
-                           w.flags.as<uint32_t>(), cap);
-    else
-        hipLaunchKernelGGL(k_zi_flags<false>, gg, dim3(256), 0, s, w.slo.as<unsigned long long>(), khi, sb, se, n_req,
-                           w.flags.as<uint32_t>(), cap);
-    size_t ts3 = t3;
-    MCZI_HIP(rocprim::exclusive_scan(w.tmp.as<void>(), ts3, w.flags.as<uint32_t>(), w.pos.as<uint32_t>(), 0u,
-                                     (size_t)cap + 1, rocprim::plus<uint32_t>(), s));
-    if (wide)
-        hipLaunchKernelGGL(k_zi_scatter<true>, dim3(zgrid(cap)), dim3(256), 0, s, w.slo.as<unsigned long long>(), khi,
-                           w.flags.as<uint32_t>(), w.pos.as<uint32_t>(), cap, w.ulo.as<unsigned long long>(),
-                           w.ohi.as<unsigned long long>());
-    else
-        hipLaunchKernelGGL(k_zi_scatter<false>, dim3(zgrid(cap)), dim3(256), 0, s, w.slo.as<unsigned long long>(), khi,
-                           w.flags.as<uint32_t>(), w.pos.as<uint32_t>(), cap, w.ulo.as<unsigned long long>(),
-                           (unsigned long long *)nullptr);
-    hipLaunchKernelGGL(k_zi_seg_counts, dim3(1), dim3(256), 0, s, w.pos.as<uint32_t>(), sb, n_req,
-                       w.seg_start.as<unsigned long long>(), w.seg_count.as<unsigned long long>());
     MCZI_HIP(hipEventRecord(w.done, s));
 }
 
-void zi_collect(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, ZiQueryResult &out) {
+// Sort the filled part of every request's segment (olo/ohi at h_base[r],
+// n[r] entries) into a tight layout (slo/uhi at tb[r]). k <= 32: radix
+// passes over the top 24 key bits, then the finishing pass orders each run of
+// equal prefixes in LDS (a run too long to finish makes the segment take the
+// full sort); small segments take the full sort. k > 32: LSD over (lo, hi)
+// carrying the source positions.
+static void zi_sort_segments(ZiWork &w, const ZiIndex &idx, const std::vector<uint64_t> &n,
+                             const std::vector<uint64_t> &tb, hipStream_t s) {
+    const uint32_t n_req = (uint32_t)n.size();
+    const uint64_t N = tb[n_req];
+    uint64_t max_n = 1;
+    for (uint64_t x : n) max_n = std::max(max_n, x);
+    unsigned long long *olo = w.olo.as<unsigned long long>(), *slo = w.slo.as<unsigned long long>();
+    if (!idx.wide) {
+        const unsigned kbits = std::min<unsigned>(64, 2 * idx.k);
+        constexpr unsigned PREFIX_BITS = 24;
+        constexpr uint64_t SPLIT_MIN = 1u << 16;
+        size_t t_split = 0, t_full = 0;
+        MCZI_HIP(rocprim::radix_sort_keys(nullptr, t_split, olo, slo, (size_t)max_n, kbits - std::min(kbits, PREFIX_BITS),
+                                          kbits, s));
+        MCZI_HIP(rocprim::radix_sort_keys(nullptr, t_full, olo, slo, (size_t)max_n, 0, kbits, s));
+        w.tmp.ensure(std::max(t_split, t_full) + 16);
+        unsigned long long *flag = w.ctr.as<unsigned long long>() + 7;
+        zfill(flag, 1, 0, s);
+        std::vector<uint32_t> split;
+        for (uint32_t r = 0; r < n_req; r++) {
+            if (n[r] == 0) continue;
+            const bool sp = kbits >= 56 && n[r] >= SPLIT_MIN && (n[r] >> PREFIX_BITS) < FINISH_MAX;
+            size_t t = sp ? t_split : t_full;
+            MCZI_HIP(rocprim::radix_sort_keys(w.tmp.as<void>(), t, olo + w.h_base[r], slo + tb[r], (size_t)n[r],
+                                              sp ? kbits - PREFIX_BITS : 0u, kbits, s));
+            if (sp) {
+                const unsigned g = (unsigned)std::min<uint64_t>((n[r] + FIN_CHUNK - 1) / FIN_CHUNK, 1u << 20);
+                hipLaunchKernelGGL(k_finish_runs<false>, dim3(g), dim3(256), 0, s, slo + tb[r], (uint32_t *)nullptr,
+                                   n[r], (int)(kbits - PREFIX_BITS), flag);
+                split.push_back(r);
+            }
+        }
+        if (!split.empty()) {
+            unsigned long long h = 0;
+            MCZI_HIP(hipMemcpyAsync(&h, flag, 8, hipMemcpyDeviceToHost, s));
+            MCZI_HIP(hipStreamSynchronize(s));
+            for (uint32_t r : h ? split : std::vector<uint32_t>()) {
+                size_t t = t_full;
+                MCZI_HIP(rocprim::radix_sort_keys(w.tmp.as<void>(), t, olo + w.h_base[r], slo + tb[r],
+                                                  (size_t)n[r], 0, kbits, s));
+            }
+        }
+        return;
+    }
+    const unsigned hbits = 2 * idx.k - 64;
+    unsigned long long *ohi = w.ohi.as<unsigned long long>(), *shi = w.shi.as<unsigned long long>(),
+                       *uhi = w.uhi.as<unsigned long long>();
+    uint32_t *i1 = w.idx.as<uint32_t>(), *i2 = w.idx2.as<uint32_t>();
+    size_t t1 = 0, t2 = 0;
+    MCZI_HIP(rocprim::radix_sort_pairs(nullptr, t1, olo, slo, i1, i2, (size_t)max_n, 0, 64, s));
+    MCZI_HIP(rocprim::radix_sort_pairs(nullptr, t2, shi, uhi, i2, i1, (size_t)max_n, 0, hbits, s));
+    w.tmp.ensure(std::max(t1, t2) + 16);
+    for (uint32_t r = 0; r < n_req; r++) {
+        if (n[r] == 0) continue;
+        const uint64_t b0 = tb[r], e0 = tb[r] + n[r];
+        size_t ts1 = t1, ts2 = t2;
+        hipLaunchKernelGGL(k_zi_iota, dim3(zgrid(n[r])), dim3(256), 0, s, i1 + b0, n[r], w.h_base[r]);
+        MCZI_HIP(rocprim::radix_sort_pairs(w.tmp.as<void>(), ts1, olo + w.h_base[r], slo + b0, i1 + b0, i2 + b0,
+                                           (size_t)n[r], 0, 64, s));
+        hipLaunchKernelGGL(k_zi_gather_range, dim3(zgrid(n[r])), dim3(256), 0, s, ohi, i2, shi, b0, e0);
+        MCZI_HIP(rocprim::radix_sort_pairs(w.tmp.as<void>(), ts2, shi + b0, uhi + b0, i2 + b0, i1 + b0, (size_t)n[r],
+                                           0, hbits, s));
+        hipLaunchKernelGGL(k_zi_gather_range, dim3(zgrid(n[r])), dim3(256), 0, s, olo, i1, slo, b0, e0);
+    }
+    (void)N;
+}
+
+// Wait for the lookup, then (with the filled size of every segment known)
+// sort and deduplicate each request's absent k-mers.
+double zi_finish(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s) {
     const auto tw = std::chrono::steady_clock::now();
     MCZI_HIP(hipEventSynchronize(w.done));
-    out.wait_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tw).count();
+    const double wait_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tw).count();
     MCZI_HIP(hipGetLastError());
-    w.h_count.assign(n_req + 1, 0);
-    std::vector<uint64_t> h_start(n_req + 1, 0), h_cursor(n_req + 1, 0);
+    const bool wide = idx.wide;
+    std::vector<uint64_t> h_cursor(n_req + 1, 0);
     unsigned long long lk = 0;
-    MCZI_HIP(hipMemcpyAsync(w.h_count.data(), w.seg_count.as<void>(), n_req * 8, hipMemcpyDeviceToHost, s));
     MCZI_HIP(hipMemcpyAsync(h_cursor.data(), w.seg_cursor.as<void>(), n_req * 8, hipMemcpyDeviceToHost, s));
-    MCZI_HIP(hipMemcpyAsync(h_start.data(), w.seg_start.as<void>(), n_req * 8, hipMemcpyDeviceToHost, s));
     MCZI_HIP(hipMemcpyAsync(&lk, w.ctr.as<void>(), 8, hipMemcpyDeviceToHost, s));
     MCZI_HIP(hipStreamSynchronize(s));
     w.lookups = lk;
+    std::vector<uint64_t> n(n_req, 0), tb(n_req + 1, 0);
+    for (uint32_t r = 0; r < n_req; r++) {
+        n[r] = h_cursor[r] - w.h_base[r];
+        require(n[r] <= w.h_base[r + 1] - w.h_base[r], MCZI_E_DEVICE, "absent k-mers exceed their request's windows");
+        tb[r + 1] = tb[r] + n[r];
+    }
+    const uint64_t N = tb[n_req];
+    w.h_count.assign(n_req + 1, 0);
+    w.h_start.assign(n_req + 1, 0);
+    w.h_absent = n;
+    if (N) {
+        w.slo.ensure(N * 8);
+        w.ulo.ensure(N * 8);
+        w.flags.ensure((N + 1) * 4);
+        w.pos.ensure((N + 1) * 4);
+        if (wide) {
+            w.shi.ensure(N * 8);
+            w.uhi.ensure(N * 8);
+            w.idx.ensure(N * 4);
+            w.idx2.ensure(N * 4);
+        }
+        zi_sort_segments(w, idx, n, tb, s);
+        // unique per request: flags -> exclusive scan -> scatter
+        MCZI_HIP(hipMemcpyAsync(w.seg_base.as<void>(), tb.data(), (n_req + 1) * 8, hipMemcpyHostToDevice, s));
+        const unsigned long long *sb = w.seg_base.as<unsigned long long>();
+        const unsigned long long *khi = wide ? w.uhi.as<unsigned long long>() : nullptr;
+        dim3 gg(zgrid(N), std::min<uint32_t>(std::max<uint32_t>(n_req, 1), 64));
+        if (wide)
+            hipLaunchKernelGGL(k_zi_flags<true>, gg, dim3(256), 0, s, w.slo.as<unsigned long long>(), khi, sb, sb + 1,
+                               n_req, w.flags.as<uint32_t>(), N);
+        else
+            hipLaunchKernelGGL(k_zi_flags<false>, gg, dim3(256), 0, s, w.slo.as<unsigned long long>(), khi, sb, sb + 1,
+                               n_req, w.flags.as<uint32_t>(), N);
+        size_t t3 = 0;
+        MCZI_HIP(rocprim::exclusive_scan(nullptr, t3, w.flags.as<uint32_t>(), w.pos.as<uint32_t>(), 0u, (size_t)N + 1,
+                                         rocprim::plus<uint32_t>(), s));
+        w.tmp.ensure(t3 + 16);
+        MCZI_HIP(rocprim::exclusive_scan(w.tmp.as<void>(), t3, w.flags.as<uint32_t>(), w.pos.as<uint32_t>(), 0u,
+                                         (size_t)N + 1, rocprim::plus<uint32_t>(), s));
+        if (wide)
+            hipLaunchKernelGGL(k_zi_scatter<true>, dim3(zgrid(N)), dim3(256), 0, s, w.slo.as<unsigned long long>(),
+                               khi, w.flags.as<uint32_t>(), w.pos.as<uint32_t>(), N, w.ulo.as<unsigned long long>(),
+                               w.ohi.as<unsigned long long>());
+        else
+            hipLaunchKernelGGL(k_zi_scatter<false>, dim3(zgrid(N)), dim3(256), 0, s, w.slo.as<unsigned long long>(),
+                               khi, w.flags.as<uint32_t>(), w.pos.as<uint32_t>(), N, w.ulo.as<unsigned long long>(),
+                               (unsigned long long *)nullptr);
+        hipLaunchKernelGGL(k_zi_seg_counts, dim3(1), dim3(256), 0, s, w.pos.as<uint32_t>(), sb, n_req,
+                           w.seg_start.as<unsigned long long>(), w.seg_count.as<unsigned long long>());
+        MCZI_HIP(hipMemcpyAsync(w.h_count.data(), w.seg_count.as<void>(), n_req * 8, hipMemcpyDeviceToHost, s));
+        MCZI_HIP(hipMemcpyAsync(w.h_start.data(), w.seg_start.as<void>(), n_req * 8, hipMemcpyDeviceToHost, s));
+        MCZI_HIP(hipStreamSynchronize(s));
+    }
+    return wait_ms;
+}
+
+void zi_copy_out(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, ZiQueryResult &out) {
+    const bool wide = idx.wide;
     out.absent.clear();
     out.absent.resize(n_req);
     out.windows.assign(n_req, 0);
     out.absent_windows.assign(n_req, 0);
     for (uint32_t r = 0; r < n_req; r++) {
         out.windows[r] = w.h_base[r + 1] - w.h_base[r];
-        out.absent_windows[r] = h_cursor[r] - w.h_base[r];
-        const uint64_t n = w.h_count[r];
+        out.absent_windows[r] = w.h_absent[r];
+        const uint64_t c = w.h_count[r];
         ZiKeys &kk = out.absent[r];
-        kk.n = n;
-        kk.lo = (uint64_t *)Device::get().host_alloc(std::max<uint64_t>(n, 1) * 8);
-        if (idx.wide) kk.hi = (uint64_t *)Device::get().host_alloc(std::max<uint64_t>(n, 1) * 8);
-        require(kk.lo && (!idx.wide || kk.hi), MCZI_E_NOMEM, "out of host memory for ZI results");
-        if (n)
-            MCZI_HIP(hipMemcpyAsync(kk.lo, w.ulo.as<unsigned long long>() + h_start[r], n * 8,
+        kk.n = c;
+        kk.lo = (uint64_t *)Device::get().host_alloc(std::max<uint64_t>(c, 1) * 8);
+        if (wide) kk.hi = (uint64_t *)Device::get().host_alloc(std::max<uint64_t>(c, 1) * 8);
+        require(kk.lo && (!wide || kk.hi), MCZI_E_NOMEM, "out of host memory for ZI results");
+        if (c)
+            MCZI_HIP(hipMemcpyAsync(kk.lo, w.ulo.as<unsigned long long>() + w.h_start[r], c * 8,
                                     hipMemcpyDeviceToHost, s));
-        if (idx.wide && n)
-            MCZI_HIP(hipMemcpyAsync(kk.hi, w.ohi.as<unsigned long long>() + h_start[r], n * 8,
+        if (wide && c)
+            MCZI_HIP(hipMemcpyAsync(kk.hi, w.ohi.as<unsigned long long>() + w.h_start[r], c * 8,
                                     hipMemcpyDeviceToHost, s));
     }
     MCZI_HIP(hipStreamSynchronize(s));
 }
 
+void zi_collect(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, ZiQueryResult &out) {
+    const double wait_ms = zi_finish(w, idx, n_req, s);
+    zi_copy_out(w, idx, n_req, s, out);
+    out.wait_ms = wait_ms;
+}
+
 // ---------------------------------------------------------------------------
 // Serving loop
+static double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 struct ZiRequest {
     uint64_t id = 0;
     HostReads reads;        // fragment offsets (the words live in `pin`)
@@ -765,9 +827,15 @@ struct ZiRequest {
     ~ZiRequest() {
         if (pin) Device::get().host_release(pin);
     }
+    double alloc_ms = 0;  // pinned allocation time of the last pin_words
     void pin_words(const uint32_t *src, uint64_t n_words) {
+        const double t0 = now_ms();
         pin = (uint32_t *)Device::get().host_alloc(std::max<uint64_t>(n_words, 1) * 4);
-        if (n_words) memcpy(pin, src, n_words * 4);
+        alloc_ms = now_ms() - t0;
+        const size_t grain = 1u << 20;  // words per worker chunk (4 MiB)
+        parallel_ranges(n_words, grain, host_workers(n_words), [&](size_t b, size_t e) {
+            memcpy(pin + b, src + b, (e - b) * 4);
+        });
     }
     bool make_fasta = false;
     uint64_t bytes = 0;
@@ -780,9 +848,6 @@ struct ZiRequest {
     uint64_t windows = 0;
 };
 
-static double now_ms() {
-    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
 
 }  // namespace mczi
 
@@ -804,18 +869,27 @@ struct mczi_zi_server {
         std::vector<ZiRequest *> batch;
         uint64_t bytes = 0;
         bool busy = false;
-        bool enqueued = false;
+        bool enqueued = false;  // lookup enqueued (or failed: err)
+        bool finished = false;  // absent sets sorted and counted (or failed: err)
+        int err = 0;
+        std::string err_msg;
+        double wait_ms = 0, sort_ms = 0;
         uint64_t seq = 0;
     };
     std::vector<Slot> slots;
-    uint64_t seq_issue = 0, seq_done = 0;
-    std::thread admit_thread, done_thread;
+    // Batches move through three in-order stages, each on its own thread:
+    // admit (upload + lookup enqueued), finish (sort + unique once the
+    // lookup is done), done (absent sets copied out, requests completed).
+    // A batch's copy-out overlaps the next batch's sort on the device.
+    uint64_t seq_issue = 0, seq_fin = 0, seq_done = 0;
+    std::thread admit_thread, finish_thread, done_thread;
     uint64_t completed = 0, lookups = 0;
     double busy_ms = 0;
     const bool trace = getenv("MCZI_ZI_TRACE") != nullptr;  // per-batch host timings on stderr
     std::string fatal;
 
     void admit_loop();
+    void finish_loop();
     void done_loop();
 };
 
@@ -866,6 +940,9 @@ void mczi_zi_server::admit_loop() {
         Slot &sl = slots[free_slot];
         sl.busy = true;
         sl.enqueued = false;
+        sl.finished = false;
+        sl.err = 0;
+        sl.err_msg.clear();
         sl.batch = batch;
         sl.bytes = bytes;
         sl.seq = seq_issue++;
@@ -909,23 +986,51 @@ void mczi_zi_server::admit_loop() {
             msg = e.what();
         }
         if (trace)
-            fprintf(stderr, "ZI_TRACE admit slot=%d reqs=%zu bases=%llu concat_ms=%.2f enqueue_ms=%.2f\n", free_slot,
+            fprintf(stderr, "ZI_TRACE t=%.2f admit slot=%d reqs=%zu bases=%llu concat_ms=%.2f enqueue_ms=%.2f\n", now_ms(), free_slot,
                     batch.size(), (unsigned long long)bases, ta1 - ta0, now_ms() - ta1);
         lk.lock();
-        if (err) {
-            for (auto *r : batch) {
-                r->err = err;
-                r->err_msg = msg;
-                r->done = true;
-                r->t_done = now_ms();
-            }
-            used -= sl.bytes;
-            sl.busy = false;
-            sl.batch.clear();
-            seq_done = std::max(seq_done, sl.seq + 1);
-        } else {
-            sl.enqueued = true;
+        sl.err = err;  // a failed batch still passes the later stages in order
+        sl.err_msg = msg;
+        sl.enqueued = true;
+        cv.notify_all();
+    }
+}
+
+void mczi_zi_server::finish_loop() {
+    std::unique_lock<std::mutex> lk(mu);
+    while (true) {
+        Slot *sl = nullptr;
+        for (auto &s : slots)
+            if (s.busy && s.enqueued && !s.finished && s.seq == seq_fin) sl = &s;
+        if (!sl) {
+            if (stop) return;
+            cv.wait(lk);
+            continue;
         }
+        const uint32_t n_req = (uint32_t)sl->batch.size();
+        const bool failed = sl->err != 0;
+        lk.unlock();
+        int err = 0;
+        std::string msg;
+        double wait = 0;
+        const double t0 = now_ms();
+        if (!failed) {
+            try {
+                wait = zi_finish(*sl->work, idx, n_req, sl->stream);
+            } catch (const Error &e) {
+                err = e.code;
+                msg = e.what();
+            }
+        }
+        lk.lock();
+        if (err) {
+            sl->err = err;
+            sl->err_msg = msg;
+        }
+        sl->wait_ms = wait;
+        sl->sort_ms = now_ms() - t0 - wait;
+        sl->finished = true;
+        seq_fin++;
         cv.notify_all();
     }
 }
@@ -936,23 +1041,26 @@ void mczi_zi_server::done_loop() {
         // next issued batch in order
         Slot *sl = nullptr;
         for (auto &s : slots)
-            if (s.busy && s.enqueued && s.seq == seq_done && !s.batch.empty()) sl = &s;
+            if (s.busy && s.finished && s.seq == seq_done) sl = &s;
         if (!sl) {
             if (stop) return;
             cv.wait(lk);
             continue;
         }
         std::vector<ZiRequest *> batch = sl->batch;
+        int err = sl->err;
+        std::string msg = sl->err_msg;
+        const double wait_ms = sl->wait_ms, sort_ms = sl->sort_ms;
         lk.unlock();
         ZiQueryResult res;
-        int err = 0;
-        std::string msg;
         const double t0 = now_ms();
-        try {
-            zi_collect(*sl->work, idx, (uint32_t)batch.size(), sl->stream, res);
-        } catch (const Error &e) {
-            err = e.code;
-            msg = e.what();
+        if (!err) {
+            try {
+                zi_copy_out(*sl->work, idx, (uint32_t)batch.size(), sl->stream, res);
+            } catch (const Error &e) {
+                err = e.code;
+                msg = e.what();
+            }
         }
         const double t1 = now_ms();
         for (size_t i = 0; i < batch.size() && !err; i++) {
@@ -964,11 +1072,11 @@ void mczi_zi_server::done_loop() {
             }
         }
         if (trace)
-            fprintf(stderr, "ZI_TRACE done reqs=%zu wait_ms=%.2f copy_ms=%.2f post_ms=%.2f\n", batch.size(),
-                    res.wait_ms, t1 - t0 - res.wait_ms, now_ms() - t1);
+            fprintf(stderr, "ZI_TRACE t=%.2f done reqs=%zu wait_ms=%.2f sort_ms=%.2f copy_ms=%.2f post_ms=%.2f\n",
+                    now_ms(), batch.size(), wait_ms, sort_ms, t1 - t0, now_ms() - t1);
         lk.lock();
-        busy_ms += t1 - t0;
-        lookups += sl->work ? sl->work->lookups : 0;
+        busy_ms += t1 - t0 + sort_ms;
+        lookups += sl->work && !err ? sl->work->lookups : 0;
         const double t = now_ms();
         for (auto *r : batch) {
             if (err) {
@@ -982,6 +1090,7 @@ void mczi_zi_server::done_loop() {
         used -= sl->bytes;
         sl->busy = false;
         sl->enqueued = false;
+        sl->finished = false;
         sl->batch.clear();
         seq_done++;
         cv.notify_all();
@@ -1127,6 +1236,7 @@ static int server_create(HostReads &&ir, uint32_t k, uint32_t m, uint64_t budget
         for (auto &sl : s->slots) MCZI_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
         mczi_zi_server *raw = s.get();
         s->admit_thread = std::thread([raw] { raw->admit_loop(); });
+        s->finish_thread = std::thread([raw] { raw->finish_loop(); });
         s->done_thread = std::thread([raw] { raw->done_loop(); });
         *out = s.release();
     });
@@ -1176,6 +1286,10 @@ int mczi_zi_server_submit(mczi_zi_server *s, const char *fastx, size_t len, int 
         require(s->fatal.empty(), MCZI_E_DEVICE, s->fatal);
         r->id = s->next_id++;
         *request_id = r->id;
+        if (s->trace)
+            fprintf(stderr, "ZI_TRACE t=%.2f submit id=%llu bases=%llu submit_ms=%.2f pin_alloc_ms=%.2f\n", now_ms(),
+                    (unsigned long long)r->id, (unsigned long long)r->reads.n_bases(), now_ms() - r->t_submit,
+                    r->alloc_ms);
         s->queue.push_back(r.get());
         s->reqs[r->id] = std::move(r);
         s->cv.notify_all();
@@ -1202,6 +1316,10 @@ int mczi_zi_server_submit_packed(mczi_zi_server *s, const uint32_t *words, const
         require(s->fatal.empty(), MCZI_E_DEVICE, s->fatal);
         r->id = s->next_id++;
         *request_id = r->id;
+        if (s->trace)
+            fprintf(stderr, "ZI_TRACE t=%.2f submit id=%llu bases=%llu submit_ms=%.2f pin_alloc_ms=%.2f\n", now_ms(),
+                    (unsigned long long)r->id, (unsigned long long)r->reads.n_bases(), now_ms() - r->t_submit,
+                    r->alloc_ms);
         s->queue.push_back(r.get());
         s->reqs[r->id] = std::move(r);
         s->cv.notify_all();
@@ -1290,6 +1408,7 @@ void mczi_zi_server_destroy(mczi_zi_server *s) {
         s->cv.notify_all();
     }
     if (s->admit_thread.joinable()) s->admit_thread.join();
+    if (s->finish_thread.joinable()) s->finish_thread.join();
     if (s->done_thread.joinable()) s->done_thread.join();
     for (auto &sl : s->slots) {
         sl.work.reset();
