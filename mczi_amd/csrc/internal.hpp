@@ -35,6 +35,11 @@ struct Error : std::runtime_error {
 inline void require(bool ok, int code, const std::string &msg) {
     if (!ok) throw Error(code, msg);
 }
+// Literal messages: the string is only built on failure (checks run inside
+// loops over millions of fragments).
+inline void require(bool ok, int code, const char *msg) {
+    if (!ok) throw Error(code, msg);
+}
 
 // ---------------------------------------------------------------------------
 // Host worker threads: up to 16 (the host cores one GPU gets), one for small
@@ -204,7 +209,6 @@ struct Batch {
     uint32_t n_datasets = 1;
     bool has_dataset = false;
     std::vector<uint64_t> chunk_frag;
-    std::vector<uint64_t> host_off; // fragment offsets (host copy, for tiling)
     // fragment lengths: how many of each length below 64, and the count and
     // base total of the longer ones (window count for any k <= 64)
     uint64_t len_hist[64] = {};

@@ -12,6 +12,7 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -1234,7 +1235,7 @@ static void fill_zero_bytes(void *p, uint64_t bytes, hipStream_t s) {
                        (uint4 *)p, n16);
 }
 
-static void prepare_batch(Batch &b);
+static void prepare_batch(Batch &b, const uint64_t *off);
 
 std::unique_ptr<Batch> make_batch(HostReads &&r) {
     r.finish();
@@ -1244,7 +1245,6 @@ std::unique_ptr<Batch> make_batch(HostReads &&r) {
     b->n_frag = r.n_frag();
     b->n_words = r.words.size();
     b->chunk_frag = r.chunk_frag;
-    b->host_off = r.off;
     b->n_datasets = r.n_datasets;
     b->words.reset(r.words.size() * 4);
     b->off.reset(r.off.size() * 8);
@@ -1257,7 +1257,7 @@ std::unique_ptr<Batch> make_batch(HostReads &&r) {
                                 hipMemcpyHostToDevice, d.stream()));
     }
     MCZI_HIP(hipStreamSynchronize(d.stream()));
-    prepare_batch(*b);
+    prepare_batch(*b, r.off.data());
     return b;
 }
 
@@ -1276,7 +1276,6 @@ std::unique_ptr<Batch> make_batch_packed(const uint32_t *words, const uint64_t *
     b->n_words = nw + 32;
     b->chunk_frag = std::move(chunk_frag);
     if (b->chunk_frag.back() != n_frag) b->chunk_frag.push_back(n_frag);
-    b->host_off.assign(frag_off, frag_off + n_frag + 1);
     b->n_datasets = n_datasets;
     b->words.reset(b->n_words * 4);
     b->off.reset((n_frag + 1) * 8);
@@ -1289,62 +1288,86 @@ std::unique_ptr<Batch> make_batch_packed(const uint32_t *words, const uint64_t *
         b->dataset.reset(std::max<size_t>(n_frag, 1) * 4);
         d.upload(b->dataset.as<void>(), dataset, n_frag * 4, d.stream());
     }
-    prepare_batch(*b);
+    prepare_batch(*b, frag_off);
     return b;
 }
 
 // Tile schedule: tiles of TILE positions, clipped at chunk ends, each with
 // the fragment holding its first position, and the fragment length summary
 // that gives the window count for any k. Built once when the batch is
-// created; neither depends on k.
-static void build_tiles(Batch &b, McPlan &p) {
-    std::vector<uint64_t> ts;
-    std::vector<uint32_t> tf, tl;
-    p.chunk_tile.clear();
-    const auto &off = b.host_off;
+// created, on the host worker threads; neither depends on k.
+static void build_tiles(Batch &b, McPlan &p, const uint64_t *off) {
+    const uint64_t nf = b.n_frag;
+    require(nf < (1ull << 32), MCZI_E_INVALID, "too many fragments in one batch");
+    // fragment lengths: one histogram per chunk of fragments, merged
+    const size_t grain = 1u << 20;
+    std::vector<std::array<uint64_t, 66>> part((nf + grain - 1) / grain);
+    parallel_ranges(nf, grain, host_workers(nf), [&](size_t lo, size_t hi) {
+        std::array<uint64_t, 66> h{};  // [0..63] lengths below 64, [64] count and [65] bases of longer ones
+        for (size_t i = lo; i < hi; i++) {
+            const uint64_t n = off[i + 1] - off[i];
+            if (n < 64) {
+                h[n]++;
+            } else {
+                h[64]++;
+                h[65] += n;
+            }
+        }
+        part[lo / grain] = h;
+    });
     std::fill(std::begin(b.len_hist), std::end(b.len_hist), 0ull);
     b.long_cnt = b.long_sum = 0;
-    for (size_t i = 0; i + 1 < off.size(); i++) {
-        const uint64_t n = off[i + 1] - off[i];
-        if (n < 64) {
-            b.len_hist[n]++;
-        } else {
-            b.long_cnt++;
-            b.long_sum += n;
-        }
+    for (const auto &h : part) {
+        for (int j = 0; j < 64; j++) b.len_hist[j] += h[j];
+        b.long_cnt += h[64];
+        b.long_sum += h[65];
     }
+    // tiles: chunk c covers bases [off[chunk_frag[c]], off[chunk_frag[c + 1]]);
+    // tile t of a chunk starts at its start + t * TILE, in the last fragment
+    // of the chunk whose offset is <= that position
+    p.chunk_tile.clear();
+    uint64_t n_tiles = 0;
     for (size_t c = 0; c + 1 < b.chunk_frag.size(); c++) {
-        p.chunk_tile.push_back(ts.size());
-        uint64_t f = b.chunk_frag[c], fe = b.chunk_frag[c + 1];
-        uint64_t cs = off[f], ce = off[fe];
-        for (uint64_t s = cs; s < ce; s += TILE) {
-            while (f + 1 < fe && off[f + 1] <= s) f++;
-            ts.push_back(s);
-            tf.push_back((uint32_t)f);
-            tl.push_back((uint32_t)std::min<uint64_t>(TILE, ce - s));
-        }
+        p.chunk_tile.push_back(n_tiles);
+        const uint64_t cs = off[b.chunk_frag[c]], ce = off[b.chunk_frag[c + 1]];
+        n_tiles += (ce - cs + TILE - 1) / TILE;
     }
-    p.chunk_tile.push_back(ts.size());
-    require(b.n_frag < (1ull << 32), MCZI_E_INVALID, "too many fragments in one batch");
-    p.n_tiles = ts.size();
+    p.chunk_tile.push_back(n_tiles);
+    std::vector<uint64_t> ts(n_tiles);
+    std::vector<uint32_t> tf(n_tiles), tl(n_tiles);
+    for (size_t c = 0; c + 1 < b.chunk_frag.size(); c++) {
+        const uint64_t f0 = b.chunk_frag[c], fe = b.chunk_frag[c + 1];
+        const uint64_t cs = off[f0], ce = off[fe];
+        const uint64_t t0 = p.chunk_tile[c], nt_c = p.chunk_tile[c + 1] - t0;
+        parallel_ranges(nt_c, 1u << 14, host_workers(nt_c * 16), [&](size_t lo, size_t hi) {
+            // fragment of the first tile by binary search, then walked forward
+            const uint64_t s0 = cs + lo * TILE;
+            uint64_t f = (uint64_t)(std::upper_bound(off + f0, off + fe, s0) - off) - 1;
+            for (size_t t = lo; t < hi; t++) {
+                const uint64_t s = cs + t * TILE;
+                while (f + 1 < fe && off[f + 1] <= s) f++;
+                ts[t0 + t] = s;
+                tf[t0 + t] = (uint32_t)f;
+                tl[t0 + t] = (uint32_t)std::min<uint64_t>(TILE, ce - s);
+            }
+        });
+    }
+    p.n_tiles = n_tiles;
     Device &d = Device::get();
-    p.tile_start.reset(std::max<size_t>(ts.size(), 1) * 8);
-    p.tile_frag.reset(std::max<size_t>(tf.size(), 1) * 4);
-    p.tile_len.reset(std::max<size_t>(tl.size(), 1) * 4);
-    if (!ts.empty()) {
-        MCZI_HIP(hipMemcpyAsync(p.tile_start.as<void>(), ts.data(), ts.size() * 8,
-                                hipMemcpyHostToDevice, d.stream()));
-        MCZI_HIP(hipMemcpyAsync(p.tile_frag.as<void>(), tf.data(), tf.size() * 4,
-                                hipMemcpyHostToDevice, d.stream()));
-        MCZI_HIP(hipMemcpyAsync(p.tile_len.as<void>(), tl.data(), tl.size() * 4,
-                                hipMemcpyHostToDevice, d.stream()));
+    p.tile_start.reset(std::max<size_t>(n_tiles, 1) * 8);
+    p.tile_frag.reset(std::max<size_t>(n_tiles, 1) * 4);
+    p.tile_len.reset(std::max<size_t>(n_tiles, 1) * 4);
+    if (n_tiles) {
+        d.upload(p.tile_start.as<void>(), ts.data(), n_tiles * 8, d.stream());
+        d.upload(p.tile_frag.as<void>(), tf.data(), n_tiles * 4, d.stream());
+        d.upload(p.tile_len.as<void>(), tl.data(), n_tiles * 4, d.stream());
     }
     MCZI_HIP(hipStreamSynchronize(d.stream()));
 }
 
-static void prepare_batch(Batch &b) {
+static void prepare_batch(Batch &b, const uint64_t *off) {
     b.plan.reset(new McPlan());
-    build_tiles(b, *b.plan);
+    build_tiles(b, *b.plan, off);
 }
 
 // Windows (k-mer starts) of the batch: sum over fragments of len - k + 1.
