@@ -96,15 +96,32 @@ __device__ __forceinline__ uint64_t high_bit_mask(int nbases) {
     return nbases >= 32 ? all : (all & ((1ull << (2 * nbases)) - 1ull));
 }
 
+// Inclusive scans over the 64 lanes of a wave with DPP row shifts (within
+// 16-lane rows) and row broadcasts (lane 15 -> next row, lane 31 -> rows 2-3).
+__device__ __forceinline__ int wave_incl_add(int x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return x;
+}
+// (x >= 0 for every lane)
+__device__ __forceinline__ int wave_incl_max(int x) {
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));
+    return x;
+}
+
 // Exclusive block scan of one int per thread (SCAN_THREADS threads).
 __device__ __forceinline__ int block_exclusive_scan(int v, int *total, int *s_wsum) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    int x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
+    const int x = wave_incl_add(v);
     if (lane == 63) s_wsum[wid] = x;
     __syncthreads();
     int pre = 0, tot = 0;
@@ -273,7 +290,8 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
         // 279-340). With w >= WPT the thread's WPT windows share the middle
         // hashes [WPT-1, w-1]; suffix minima on the left and prefix minima on
         // the right complete each window (van Herk / Gil-Werman).
-        int jfirst = 0;  // fragment holding the thread's first window
+        int jfirst = 0;     // fragment holding the thread's first window
+        uint32_t smask = 0;  // piece starts among the thread's windows (bit 0 provisional)
         if (r0 < tl) {
             int lo = 0, hi = nf - 1;
             while (lo < hi) {
@@ -297,6 +315,8 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             // suffix minima over [i, w-1] (the shared middle folded in); prefix
             // minima over [w, w+i-1] are kept running in the window loop
             uint32_t sufL[WPT], sufR[WPT], preL = ~0u, preR = ~0u;
+            uint32_t pv = 0;
+            bool pvalid = false;
             if constexpr (GW) {
                 uint32_t midL = ~0u, midR = ~0u;
                 for (int t = WPT - 1; t <= w - 1; t++) {
@@ -359,6 +379,12 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
                 }
                 s_u.run.v[px(i)] = v;
                 s_flag[px(i)] = (uint8_t)((valid ? 1 : 0) | (fstart ? 2 : 0));
+                // a piece starts at a valid window whose predecessor is
+                // invalid, in another fragment or selects another hash (the
+                // first window's predecessor is the previous thread's last)
+                if (valid && (i == 0 || fstart || !pvalid || pv != v)) smask |= 1u << i;
+                pv = v;
+                pvalid = valid;
             }
         }
         __syncthreads();
@@ -372,17 +398,10 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
         // piece starts -> compacted list of (start | fragment << 11), the
         // fragment index inside the tile walked forward from the thread's
         // first fragment
-        uint32_t smask = 0;
-        int cnt = 0;
-        for (int i = 0; i < WPT; i++) {
-            const int r = r0 + i;
-            if (r >= tl) break;
-            const uint8_t f = s_flag[px(i)];
-            if ((f & 1) && (r == 0 || (f & 2) || !(s_flag[px(i - 1)] & 1) || s_u.run.v[px(i - 1)] != s_u.run.v[px(i)])) {
-                smask |= 1u << i;
-                cnt++;
-            }
-        }
+        if ((smask & 1u) && r0 > 0 && !(s_flag[px(0)] & 2) && (s_flag[px(-1)] & 1) &&
+            s_u.run.v[px(-1)] == s_u.run.v[px(0)])
+            smask &= ~1u;
+        const int cnt = __popc(smask);
         int np = 0;
         int o = block_exclusive_scan(cnt, &np, s_wsum);
         if (smask) {
@@ -597,28 +616,6 @@ __device__ __forceinline__ unsigned long long wave_append(bool pred, unsigned lo
         base = __shfl(base, leader, 64);
     }
     return base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
-}
-
-// Inclusive scans over the 64 lanes of a wave with DPP row shifts (within
-// 16-lane rows) and row broadcasts (lane 15 -> next row, lane 31 -> rows 2-3).
-__device__ __forceinline__ int wave_incl_add(int x) {
-    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
-    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
-    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
-    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
-    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
-    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
-    return x;
-}
-// (x >= 0 for every lane)
-__device__ __forceinline__ int wave_incl_max(int x) {
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false));
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false));
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false));
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false));
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));
-    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));
-    return x;
 }
 
 // Canonical MC encoding (src/lib.rs:3589-3614) of the k <= 32 MC-coded bases
