@@ -209,14 +209,39 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
     const int r0 = tid * WPT;
     auto px = [&](int c) { return 9 * tid + c + (c >> 3); };
 
-    for (uint64_t t = a.tile_begin + blockIdx.x; t < a.tile_end; t += gridDim.x) {
-        const uint64_t ts = a.tile_start[t];
-        const int tl = (int)a.tile_len[t];
-        const uint64_t f0 = a.tile_frag[t];
+    // A tile's schedule entry is loaded one tile ahead, and its words and
+    // first fragment offsets while the tile before it is scanned, so a tile
+    // starts with no global round trip: each thread holds one word
+    // (WORDS_SPAN <= SCAN_THREADS) and one offset.
+    static_assert(WORDS_SPAN <= SCAN_THREADS, "one staged word per thread");
+    uint64_t t = a.tile_begin + blockIdx.x;
+    uint64_t ts = 0, f0 = 0;
+    int tl = 0;
+    if (t < a.tile_end) {
+        ts = a.tile_start[t];
+        tl = (int)a.tile_len[t];
+        f0 = a.tile_frag[t];
+    }
+    uint32_t pword = 0;
+    uint64_t poff = 0;
+    auto load_first = [&](uint64_t ts_, int tl_, uint64_t f0_) {
+        const int nw = (((int)(ts_ & 15u) + tl_ + 160) >> 4) + 3;
+        pword = tid < nw ? a.words[(ts_ >> 4) + tid] : 0u;
+        poff = f0_ + (uint64_t)tid <= a.n_frag ? a.off[f0_ + tid] : 0ull;
+    };
+    if (t < a.tile_end) load_first(ts, tl, f0);
+    uint64_t ts_n = 0, f0_n = 0;  // the next tile's entry
+    int tl_n = 0;
+    for (; t < a.tile_end; t += gridDim.x, ts = ts_n, tl = tl_n, f0 = f0_n) {
+        const uint64_t tn = t + gridDim.x;
+        if (tn < a.tile_end) {
+            ts_n = a.tile_start[tn];
+            tl_n = (int)a.tile_len[tn];
+            f0_n = a.tile_frag[tn];
+        }
         const int rb0 = (int)(ts & 15u);
-        const int nwords = ((rb0 + tl + 160) >> 4) + 3;
         __syncthreads();  // previous tile fully consumed
-        for (int i = tid; i < WORDS_SPAN; i += SCAN_THREADS) s_words[i] = i < nwords ? a.words[(ts >> 4) + i] : 0u;
+        if (tid < WORDS_SPAN) s_words[tid] = pword;
         // Fragment boundaries relative to ts: s_fs[j] = off[f0 + j] - ts (clamped),
         // for j = 0 .. nf, nf = number of fragments with a base in [ts, ts + tl).
         int nf = 0;
@@ -224,7 +249,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             const int j = j0 + tid;
             bool in = false;
             if (j <= TILE + 1 && f0 + (uint64_t)j <= a.n_frag) {
-                const int64_t rel = (int64_t)a.off[f0 + j] - (int64_t)ts;
+                const int64_t rel = (int64_t)(j0 == 0 ? poff : a.off[f0 + j]) - (int64_t)ts;
                 s_fs[j] = rel < 0 ? -1 : (rel > (1 << 30) ? (1 << 30) : (int32_t)rel);
                 in = rel < tl;
             }
@@ -233,6 +258,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             if (c < SCAN_THREADS) break;
         }
         __syncthreads();
+        if (tn < a.tile_end) load_first(ts_n, tl_n, f0_n);
 
         // m-mer hashes for relative positions i in [0, tl + w - 1). ntHash is
         // XOR-linear, so with the exclusive prefix XORs over the tile's bases
