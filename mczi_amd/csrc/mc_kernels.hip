@@ -178,7 +178,10 @@ __device__ __forceinline__ uint32_t extract16_bases(const uint32_t *sw, int pos)
 // WINDOWS the selected hash of every window. GW selects the window-minimum
 // scheme (w >= WPT: Gil-Werman over the thread's windows; else a direct scan
 // of the w hashes). STAGE serves the counting path, which has k <= 32.
-template <int MODE, bool GW, bool WIDE = false>
+// WC != 0 fixes the window length w at compile time (the reference default
+// k = 31, m = 21 has w = 11), which unrolls the minimum over the middle
+// hashes and makes every s_hash offset a constant.
+template <int MODE, bool GW, bool WIDE = false, int WC = 0>
 __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_scan(ScanArgs a) {
     __shared__ uint32_t s_words[WORDS_SPAN];
     __shared__ uint32_t s_hash[PADDED(TILE + MAXW)];
@@ -201,7 +204,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
     __shared__ unsigned long long s_sbase, s_hole_b, s_hole_e;
 
     const int tid = threadIdx.x;
-    const int k = a.k, m = a.m, w = a.w;
+    const int k = a.k, m = a.m, w = WC ? WC : a.w;
     unsigned long long my_pieces = 0, my_records = 0;
     unsigned long long chunk_cur = 0, chunk_end = 0;  // thread 0: the block's current staging chunk
     // each thread owns positions r0 .. r0 + WPT - 1 of a tile; P8(r0 + c),
@@ -1494,6 +1497,7 @@ static uint64_t launch_scan(McPlan &p, const ScanArgs &sa, int launch, hipStream
         const bool gw = ca.w >= WPT, wide = ca.k > 32;
         if (wide && gw) hipLaunchKernelGGL((k_scan<STAGE, true, true>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
         else if (wide) hipLaunchKernelGGL((k_scan<STAGE, false, true>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
+        else if (gw && ca.w == 11) hipLaunchKernelGGL((k_scan<STAGE, true, false, 11>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
         else if (gw) hipLaunchKernelGGL((k_scan<STAGE, true>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
         else hipLaunchKernelGGL((k_scan<STAGE, false>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
     };
