@@ -7,7 +7,6 @@
 // 3483-3523 / add_all_superkmers_packed 3525-3560 (pass 2) ->
 // process_superkmer_partition 2400-2425 / ..._to_kff_u8_bucketed 2512-2570
 // (count + filter). fof: 637-703, 4253-4370, 2184-2247.
-#include <rocprim/block/block_scan.hpp>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
@@ -107,6 +106,15 @@ __device__ __forceinline__ int wave_incl_add(int x) {
     x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
     return x;
 }
+__device__ __forceinline__ uint32_t wave_incl_xor(uint32_t x) {
+    x ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return x;
+}
 // (x >= 0 for every lane)
 __device__ __forceinline__ int wave_incl_max(int x) {
     x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false));
@@ -152,12 +160,6 @@ __device__ __forceinline__ void block_sum_atomic(unsigned long long v, unsigned 
     __syncthreads();
 }
 
-struct XorOp {
-    __device__ __forceinline__ unsigned long long operator()(unsigned long long x, unsigned long long y) const {
-        return x ^ y;
-    }
-};
-using ScanXor = rocprim::block_scan<unsigned long long, SCAN_THREADS>;
 
 // 8 bases starting at base index `pos` of the LDS word stream (base j at
 // bits 2j of the result).
@@ -197,7 +199,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             uint32_t pst[TILE];
         } run;
     } s_u;
-    __shared__ typename ScanXor::storage_type s_scan;
+    __shared__ uint32_t s_wxor[2][SCAN_THREADS / 64];  // per-wave XOR totals of the two prefix streams
     __shared__ uint8_t s_flag[PADDED(TILE)];
     __shared__ int32_t s_fs[TILE + 2];
     __shared__ int s_wsum[SCAN_THREADS / 64];
@@ -289,9 +291,22 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
                 xb ^= __builtin_rotateleft32(g, amt);
                 amt = (amt + 7u) & 31u;
             }
-            unsigned long long pre;
-            ScanXor().exclusive_scan(((unsigned long long)xb << 32) | xa, pre, 0ull, s_scan, XorOp());
-            const uint32_t pa = (uint32_t)pre, pb = (uint32_t)(pre >> 32);
+            // exclusive block XOR scan of (xa, xb): DPP within the wave, then
+            // the totals of the waves before this one
+            const int lane = tid & 63, wid = tid >> 6;
+            const uint32_t ia = wave_incl_xor(xa), ib = wave_incl_xor(xb);
+            if (lane == 63) {
+                s_wxor[0][wid] = ia;
+                s_wxor[1][wid] = ib;
+            }
+            __syncthreads();
+            uint32_t pa = ia ^ xa, pb = ib ^ xb;
+#pragma unroll
+            for (int q = 0; q < SCAN_THREADS / 64; q++)
+                if (q < wid) {
+                    pa ^= s_wxor[0][q];
+                    pb ^= s_wxor[1][q];
+                }
 #pragma unroll
             for (int t = 0; t < WPT; t++) {
                 s_u.pre.sa[px(t)] = pa ^ ta[t];
