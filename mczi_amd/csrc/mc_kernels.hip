@@ -807,12 +807,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
                 cnt++;
             }
         }
-        int x = cnt;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
-        }
+        const int x = wave_incl_add(cnt);
         if (lane == 63) s_wsum[wid] = x;
         __syncthreads();
         int pre = 0, tot = 0;
@@ -1056,12 +1051,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count_wide(CountArgs a) {
                 cnt++;
             }
         }
-        int x = cnt;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
-        }
+        const int x = wave_incl_add(cnt);
         if (lane == 63) s_wsum[wid] = x;
         __syncthreads();
         int pre = 0, tot = 0;
