@@ -347,15 +347,16 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             int j = lo;
             int fcur = s_fs[j], fnext = s_fs[j + 1];  // bounds of the fragment holding r
             const int rb = rb0 + r0;
-            // T/G bits (bit 2b+1 of each code) of bases r0 .. r0 + 95; the
-            // strand vote of window i counts them over bases [i, i + k)
-            const uint64_t H0 = extract32_bases(s_words, (uint64_t)rb) & 0xAAAAAAAAAAAAAAAAull;
-            const uint64_t H1 = extract32_bases(s_words, (uint64_t)rb + 32) & 0xAAAAAAAAAAAAAAAAull;
+            // Strand vote: the number of T/G bases (bit 2b+1 of each code) in
+            // window i's bases [i, i + k). Window 0 counts them; window i
+            // then drops base i - 1 (from H0) and adds base i - 1 + k (from
+            // E, the bases from k on).
             constexpr bool KBIG = MODE == WINDOWS || WIDE;  // k may exceed 32
-            const uint64_t H2 =
-                KBIG && k > 32 ? extract32_bases(s_words, (uint64_t)rb + 64) & 0xAAAAAAAAAAAAAAAAull : 0ull;
-            const uint64_t mlo = high_bit_mask(k < 32 ? k : 32);
-            const uint64_t mhi = KBIG && k > 32 ? high_bit_mask(k - 32) : 0ull;
+            const uint64_t H0 = extract32_bases(s_words, (uint64_t)rb) & 0xAAAAAAAAAAAAAAAAull;
+            const uint32_t E = (uint32_t)extract32_bases(s_words, (uint64_t)rb + (uint64_t)k);
+            uint32_t tg = (uint32_t)__popcll(H0 & high_bit_mask(k < 32 ? k : 32));
+            if (KBIG && k > 32)
+                tg += (uint32_t)__popcll(extract32_bases(s_words, (uint64_t)rb + 32) & high_bit_mask(k - 32));
             // suffix minima over [i, w-1] (the shared middle folded in); prefix
             // minima over [w, w+i-1] are kept running in the window loop
             uint32_t sufL[WPT], sufR[WPT], preL = ~0u, preR = ~0u;
@@ -387,6 +388,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
                     preR = min(preR, h | (0xffffu - (uint32_t)t));
                 }
                 if (r >= tl) break;
+                if (i > 0) tg = tg + ((E >> (2 * i - 1)) & 1u) - (((uint32_t)H0 >> (2 * i - 1)) & 1u);
                 while (j + 1 < nf && fnext <= r) {
                     j++;
                     fcur = fnext;
@@ -396,12 +398,6 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
                 const bool valid = r + k <= fnext;
                 uint32_t v = 0;
                 if (valid) {
-                    const uint64_t lo64 = i ? ((H0 >> (2 * i)) | (H1 << (64 - 2 * i))) : H0;
-                    uint32_t tg = (uint32_t)__popcll(lo64 & mlo);
-                    if (KBIG && k > 32) {
-                        const uint64_t hi64 = i ? ((H1 >> (2 * i)) | (H2 << (64 - 2 * i))) : H1;
-                        tg += (uint32_t)__popcll(hi64 & mhi);
-                    }
                     int lp, rp;
                     if constexpr (GW) {
                         lp = (int)(min(sufL[i], preL) & 0xffffu);
