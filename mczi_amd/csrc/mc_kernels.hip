@@ -336,6 +336,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
         // the right complete each window (van Herk / Gil-Werman).
         int jfirst = 0;     // fragment holding the thread's first window
         uint32_t smask = 0;  // piece starts among the thread's windows (bit 0 provisional)
+        uint32_t jd = 0;     // 4 bits per window: its fragment minus jfirst (at most WPT)
         if (r0 < tl) {
             int lo = 0, hi = nf - 1;
             while (lo < hi) {
@@ -394,6 +395,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
                     fcur = fnext;
                     fnext = s_fs[j + 1];
                 }
+                jd |= (uint32_t)(j - jfirst) << (4 * i);
                 const bool fstart = fcur == r;
                 const bool valid = r + k <= fnext;
                 uint32_t v = 0;
@@ -444,16 +446,10 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
         const int cnt = __popc(smask);
         int np = 0;
         int o = block_exclusive_scan(cnt, &np, s_wsum);
-        if (smask) {
-            int j = jfirst, fnx = s_fs[jfirst + 1];
-            for (int i = 0; i < WPT; i++) {
-                const int r = r0 + i;
-                while (j + 1 < nf && fnx <= r) {
-                    j++;
-                    fnx = s_fs[j + 1];
-                }
-                if (smask & (1u << i)) s_u.run.pst[o++] = (uint32_t)r | ((uint32_t)j << 11);
-            }
+        for (uint32_t sm = smask; sm; sm &= sm - 1) {
+            const int i = __builtin_ctz(sm);
+            const uint32_t j = (uint32_t)jfirst + ((jd >> (4 * i)) & 15u);
+            s_u.run.pst[o++] = (uint32_t)(r0 + i) | (j << 11);
         }
         __syncthreads();
 
@@ -472,7 +468,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             const int vend = s_fs[(pw >> 11) + 1] - k + 1;
             if (vend < e) e = vend;
             my_pieces++;
-            my_nrec += (e - s + (int)a.wrec - 1) / (int)a.wrec;
+            my_nrec += e - s <= (int)a.wrec ? 1 : (e - s + (int)a.wrec - 1) / (int)a.wrec;
         }
         int tot = 0;
         const int off = block_exclusive_scan(my_nrec, &tot, s_wsum);
