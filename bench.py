@@ -83,6 +83,31 @@ def cpu_baseline(reads, k, m, x, parts, mode, seconds_target=15.0):
                       f"{'two-pass FASTA path' if mode == 0 else 'KMC-style KFF path'}, {dt:.1f} s"}
 
 
+def many_inputs(M, n_inputs: int = 4000, reads_per_input: int = 50, steps: int = 3):
+    """Many small inputs in one packed batch (reads of a 1 Mbp random genome, 150 bp
+    reads at 0.5% substitutions; k31 m21 x1 P64): the batched launch scans
+    them all at once, the plain launch one input per launch. Both count the
+    same k-mers; returns bases/s of each and their ratio."""
+    import numpy as np
+    r = M.SynthReads(M.SynthSpec(genome_len=1_000_000, genome_seed=1, read_seed=101,
+                                 n_reads=n_inputs * reads_per_input, read_len=150, sub_rate=0.005))
+    marks = np.arange(0, n_inputs * reads_per_input + 1, reads_per_input, dtype=np.uint64)
+    b = M.Batch.from_packed(r.words, r.frag_off, input_frag=marks)
+    cfg = M.CounterConfig(31, 21, 1, 64)
+    res = {"inputs": n_inputs, "reads_per_input": reads_per_input, "bases": r.n_bases}
+    outs = []
+    for name, launch in (("batched", M.LAUNCH_BATCHED), ("plain", M.LAUNCH_PLAIN)):
+        st = b.count(cfg, M.MODE_TWO_PASS, launch)  # warm-up (plan, graph)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = b.count(cfg, M.MODE_TWO_PASS, launch)
+        res[name + "_bases_per_s"] = r.n_bases * steps / (time.perf_counter() - t0)
+        outs.append(st.n_out)
+    assert outs[0] == outs[1], "batched and plain counts differ"
+    res["batched_vs_plain"] = res["batched_bases_per_s"] / res["plain_bases_per_s"]
+    return res
+
+
 def zi_serving(M, n_requests: int, k: int, seed: int = 5):
     """ZI served subtraction: index = canonical k-mers of a 100 Mbp genome,
     requests of 150 bp reads (half from the indexed genome, half from an
@@ -315,6 +340,7 @@ def main() -> int:
             batch.count(cfg, mode, M.LAUNCH_PLAIN, order)
         plain_value = reads.n_bases * 2 / (time.perf_counter() - tp)
     del os.environ["MCZI_NO_GRAPH"]
+    many = None if args.no_plain else many_inputs(M)
 
     # plain device copy of 2 GiB: achievable HBM bandwidth on this box, and
     # the reference dispatch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE
@@ -362,6 +388,7 @@ def main() -> int:
         "pcie_inclusive_value": nb / (ms_per_step * 1e-3 + upload_s),
         "plain_value": plain_value,
         "batched_vs_plain": (value / plain_value) if plain_value else None,
+        "many_inputs": many,
         "phase_ms": {k2: prof[k2] for k2 in ("ms_scan1", "ms_scan2", "ms_count", "ms_sort", "ms_total")},
         "stats": {k2: last[k2] for k2 in ("n_windows", "n_pieces", "n_records", "n_distinct", "n_overflow",
                                            "n_out", "internal_partitions")},

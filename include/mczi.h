@@ -187,6 +187,16 @@ int mczi_write_counted(const char *path, const mczi_counted_kmer *kmers, size_t 
 int mczi_read_counted(const char *path, mczi_counted_kmer **out, size_t *n_out);
 int mczi_batch_from_packed(const uint32_t *words, const uint64_t *frag_off, size_t n_frag,
                            const uint32_t *dataset, mczi_batch **out);
+/* Same, with the inputs the caller packed together marked: input c holds
+ * fragments [input_frag[c], input_frag[c+1]) (input_frag[0] = 0,
+ * input_frag[n_inputs] = n_frag, non-decreasing). The plain launch mode then
+ * scans one input per launch, as the reference reads one file at a time in
+ * batches of at most BATCH_BASES (src/lib.rs:28, for_fastx_packed_batches
+ * src/lib.rs:3264-3308); inputs above 128 Mi bases are split further. The
+ * batched mode and the results do not depend on the marks. */
+int mczi_batch_from_packed_inputs(const uint32_t *words, const uint64_t *frag_off, size_t n_frag,
+                                  const uint32_t *dataset, const uint64_t *input_frag, size_t n_inputs,
+                                  mczi_batch **out);
 int mczi_batch_info(const mczi_batch *b, uint64_t *n_bases, uint64_t *n_frag,
                     uint32_t *n_datasets);
 void mczi_batch_free(mczi_batch *b);

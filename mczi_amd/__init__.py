@@ -131,6 +131,7 @@ _sig("mczi_batch_from_files", C.c_int, [_CP, C.c_size_t, C.c_uint32, C.c_int, C.
 _sig("mczi_pack_fastx", C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.POINTER(C.c_uint32)),
                                   C.POINTER(C.POINTER(C.c_uint64)), C.POINTER(C.c_size_t)])
 _sig("mczi_batch_from_packed", C.c_int, [_P, _P, C.c_size_t, _P, C.POINTER(_P)])
+_sig("mczi_batch_from_packed_inputs", C.c_int, [_P, _P, C.c_size_t, _P, _P, C.c_size_t, C.POINTER(_P)])
 _WORDS_OUT = [C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(C.POINTER(C.c_uint64)), C.POINTER(C.c_size_t)]
 _sig("mczi_write_read_cache", C.c_int, [C.c_char_p, C.c_uint32, C.c_char_p])
 _sig("mczi_read_read_cache", C.c_int, [C.c_char_p, C.c_uint32] + _WORDS_OUT)
@@ -180,7 +181,8 @@ EXPORTED = [
     "mczi_count_inputs_to_kmer_fasta_path", "mczi_count_datasets_to_kmer_fasta_path", "mczi_write_fasta_path",
     "mczi_write_fasta_after_index", "mczi_write_kff_path", "mczi_simplitig_sequences",
     "mczi_simplitig_sequences_parallel", "mczi_decode_kmer", "mczi_minimizer_run_hashes",
-    "mczi_batch_from_files", "mczi_pack_fastx", "mczi_batch_from_packed", "mczi_write_read_cache",
+    "mczi_batch_from_files", "mczi_pack_fastx", "mczi_batch_from_packed", "mczi_batch_from_packed_inputs",
+    "mczi_write_read_cache",
     "mczi_read_read_cache", "mczi_write_superkmer_partitions", "mczi_read_superkmer_partitions",
     "mczi_count_superkmer_partitions", "mczi_write_counted", "mczi_read_counted", "mczi_batch_info", "mczi_batch_free", "mczi_batch_count",
     "mczi_batch_download", "mczi_batch_autotune", "mczi_copy_probe", "mczi_device_info", "mczi_zi_subtract", "mczi_zi_absent", "mczi_mczi_subtract", "mczi_mczi_absent", "mczi_zi_server_create",
@@ -493,13 +495,23 @@ class Batch:
         return cls(h)
 
     @classmethod
-    def from_packed(cls, words: np.ndarray, frag_off: np.ndarray, dataset: np.ndarray | None = None) -> "Batch":
+    def from_packed(cls, words: np.ndarray, frag_off: np.ndarray, dataset: np.ndarray | None = None,
+                    input_frag: np.ndarray | None = None) -> "Batch":
+        """Packed reads to a device batch. input_frag (optional) marks where
+        each packed input starts (fragment indices, 0 first, n_frag last):
+        the plain launch mode then scans one input per launch."""
         words = np.ascontiguousarray(words, dtype=np.uint32)
         frag_off = np.ascontiguousarray(frag_off, dtype=np.uint64)
         ds = None if dataset is None else np.ascontiguousarray(dataset, dtype=np.uint32)
         h = C.c_void_p()
-        _check(_lib.mczi_batch_from_packed(words.ctypes.data, frag_off.ctypes.data, frag_off.size - 1,
-                                           ds.ctypes.data if ds is not None else None, C.byref(h)))
+        if input_frag is None:
+            _check(_lib.mczi_batch_from_packed(words.ctypes.data, frag_off.ctypes.data, frag_off.size - 1,
+                                               ds.ctypes.data if ds is not None else None, C.byref(h)))
+        else:
+            inp = np.ascontiguousarray(input_frag, dtype=np.uint64)
+            _check(_lib.mczi_batch_from_packed_inputs(words.ctypes.data, frag_off.ctypes.data, frag_off.size - 1,
+                                                      ds.ctypes.data if ds is not None else None,
+                                                      inp.ctypes.data, inp.size - 1, C.byref(h)))
         return cls(h)
 
     def info(self) -> tuple[int, int, int]:

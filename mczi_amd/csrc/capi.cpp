@@ -718,23 +718,41 @@ int mczi_batch_from_files(const char *const *paths, size_t n, uint32_t min_len, 
     });
 }
 
-int mczi_batch_from_packed(const uint32_t *words, const uint64_t *frag_off, size_t n_frag,
-                           const uint32_t *dataset, mczi_batch **out) {
+// Packed reads -> device batch; input_frag (optional) marks where each
+// packed input starts (the plain launch mode scans one input per launch).
+static int batch_from_packed(const uint32_t *words, const uint64_t *frag_off, size_t n_frag,
+                             const uint32_t *dataset, const uint64_t *input_frag, size_t n_inputs,
+                             mczi_batch **out) {
     return guarded("failed to load batch", [&] {
         require(out != nullptr && frag_off != nullptr, MCZI_E_INVALID, "null argument");
-        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
         require(words != nullptr || frag_off[n_frag] == 0, MCZI_E_INVALID, "null argument");
         for (size_t i = 0; i < n_frag; i++)
             require(frag_off[i + 1] >= frag_off[i], MCZI_E_INVALID, "fragment offsets must not decrease");
+        if (input_frag) {
+            require(n_inputs >= 1 && input_frag[0] == 0 && input_frag[n_inputs] == n_frag, MCZI_E_INVALID,
+                    "input marks must start at fragment 0 and end at n_frag");
+            for (size_t c = 0; c < n_inputs; c++)
+                require(input_frag[c + 1] >= input_frag[c], MCZI_E_INVALID, "input marks must not decrease");
+        }
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
         uint32_t nd = 1;
         if (dataset)
             for (size_t i = 0; i < n_frag; i++) nd = std::max(nd, dataset[i] + 1);
-        // input chunks of >= 128 Mi bases (BATCH_BASES, src/lib.rs:28) for the plain launch mode
+        // input chunks of >= 128 Mi bases (BATCH_BASES, src/lib.rs:28) for the
+        // plain launch mode, each input starting a chunk of its own
         std::vector<uint64_t> chunk_frag{0};
         uint64_t acc = 0;
+        size_t c = 1;
         for (size_t i = 0; i < n_frag; i++) {
             acc += frag_off[i + 1] - frag_off[i];
-            if (acc >= (128ull << 20) && i + 1 < n_frag) {
+            bool cut = acc >= (128ull << 20);
+            if (input_frag) {
+                while (c < n_inputs && input_frag[c] <= i + 1) {
+                    cut = cut || input_frag[c] == i + 1;
+                    c++;
+                }
+            }
+            if (cut && i + 1 < n_frag && chunk_frag.back() != i + 1) {
                 chunk_frag.push_back(i + 1);
                 acc = 0;
             }
@@ -743,6 +761,19 @@ int mczi_batch_from_packed(const uint32_t *words, const uint64_t *frag_off, size
         h->b = make_batch_packed(words, frag_off, n_frag, dataset, nd, std::move(chunk_frag));
         *out = h.release();
     });
+}
+
+int mczi_batch_from_packed(const uint32_t *words, const uint64_t *frag_off, size_t n_frag,
+                           const uint32_t *dataset, mczi_batch **out) {
+    return batch_from_packed(words, frag_off, n_frag, dataset, nullptr, 0, out);
+}
+
+int mczi_batch_from_packed_inputs(const uint32_t *words, const uint64_t *frag_off, size_t n_frag,
+                                  const uint32_t *dataset, const uint64_t *input_frag, size_t n_inputs,
+                                  mczi_batch **out) {
+    if (input_frag == nullptr)
+        return guarded("failed to load batch", [&] { require(false, MCZI_E_INVALID, "null argument"); });
+    return batch_from_packed(words, frag_off, n_frag, dataset, input_frag, n_inputs, out);
 }
 
 int mczi_batch_info(const mczi_batch *b, uint64_t *n_bases, uint64_t *n_frag, uint32_t *n_datasets) {

@@ -304,6 +304,21 @@ def test_staging_grows_past_its_estimate(monkeypatch, k, m):
         assert same_counted(b2.download(), want)
 
 
+def test_many_small_inputs_plain_equals_batched():
+    """Inputs marked in one packed batch: the plain mode scans one input
+    per launch (here 600 inputs of 20 reads), the batched mode all of them in
+    one; both equal brute force."""
+    r = M.SynthReads(M.SynthSpec(genome_len=50_000, genome_seed=11, read_seed=111, n_reads=12_000, read_len=150,
+                                 sub_rate=0.01))
+    marks = np.arange(0, 12_001, 20, dtype=np.uint64)
+    want = O.count_bruteforce(r.codes(), r.frag_off, 31, 1)
+    b = M.Batch.from_packed(r.words, r.frag_off, input_frag=marks)
+    cfg = M.CounterConfig(31, 21, 1, 64)
+    for launch in (M.LAUNCH_PLAIN, M.LAUNCH_BATCHED, M.LAUNCH_PLAIN):
+        b.count(cfg, M.MODE_TWO_PASS, launch)
+        assert same_counted(b.download(), want), launch
+
+
 def _concat(a, b):
     """Two packed read sets in one segmented buffer (word-aligned; the gap
     between them is a short fragment that has no k-mer)."""

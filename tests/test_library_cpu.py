@@ -172,3 +172,15 @@ def test_mczi_argument_checks(tmp_path):
         M.mczi_subtract([], [f], M.CounterConfig(31, 21, 1))
     with pytest.raises(M.MczaError, match="at least one query input"):
         M.mczi_subtract([f], [], M.CounterConfig(31, 21, 1))
+
+
+def test_packed_input_marks_are_checked_before_device_work():
+    import numpy as np
+    words = np.zeros(4, dtype=np.uint32)
+    off = np.array([0, 16, 32, 48], dtype=np.uint64)
+    for marks, msg in [([1, 3], "start at fragment 0"), ([0, 2], "end at n_frag"),
+                       ([0, 2, 1, 3], "must not decrease")]:
+        with pytest.raises(M.MczaError, match=msg):
+            M.Batch.from_packed(words, off, input_frag=np.array(marks, dtype=np.uint64))
+    with pytest.raises(M.MczaError, match="fragment offsets must not decrease"):
+        M.Batch.from_packed(words, np.array([0, 32, 16, 48], dtype=np.uint64), input_frag=np.array([0, 3]))
