@@ -285,6 +285,7 @@ def main() -> int:
     ap.add_argument("--zi-requests", type=int, default=48)
     ap.add_argument("--no-zi", action="store_true")
     ap.add_argument("--zi-only", action="store_true", help="run only the ZI serving measurement")
+    ap.add_argument("--zi-k63-only", action="store_true", help="run only the ZI serving measurement at k=63")
     ap.add_argument("--e2e", action="store_true",
                     help="run only the end-to-end count -> simplitig FASTA measurement (file on disk)")
     args = ap.parse_args()
@@ -297,6 +298,9 @@ def main() -> int:
 
     if args.zi_only:
         print(json.dumps({"zi": zi_serving(M, args.zi_requests, 31)}), flush=True)
+        return 0
+    if args.zi_k63_only:
+        print(json.dumps({"zi_k63": zi_serving(M, args.zi_requests, 63)}), flush=True)
         return 0
     if args.e2e:
         print(json.dumps({"e2e": end_to_end(M, args.workload, not args.no_cpu_baseline)}), flush=True)
@@ -403,6 +407,8 @@ def main() -> int:
         del batch
         line["zi"] = zi_serving(M, args.zi_requests, 31)
         print(f"[bench] zi: {line['zi']}", file=sys.stderr, flush=True)
+        line["zi_k63"] = zi_serving(M, args.zi_requests, 63)
+        print(f"[bench] zi k63: {line['zi_k63']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(reads, k, m, x, parts, mode)
     print(json.dumps(line), flush=True)
