@@ -634,6 +634,193 @@ int or_count_two_pass(const or_frags *f, int k, int m, uint64_t x, size_t P, or_
     return 0;
 }
 
+/* Multithreaded two-pass count: the phases of count_inputs (src/lib.rs:
+ * 920-992) with the reference's parallel structure (Rayon over input
+ * batches and over partitions, ShardedMinimizerCounts shards):
+ * 1. each thread counts the minimizers of its fragments into its own P
+ *    shards (saturating at x + 1); shards are merged per shard, which is
+ *    exact because every partial count is already clamped at x + 1;
+ * 2. each thread writes the filtered super-k-mers of its fragments into its
+ *    own P partition buffers;
+ * 3. each partition (all threads' buffers) is counted by one thread.
+ * The result is sorted as in or_count_two_pass. */
+typedef struct {
+    const or_frags *f;
+    int k, m;
+    uint64_t x;
+    size_t P, lo, hi;  /* fragments [lo, hi) in phases 1-2, partitions p = t mod T in 3 */
+    int t, T;
+    mintab *shards;          /* phase 1: this thread's P shards */
+    mintab **all_shards;     /* phase 1 merge: every thread's shards */
+    mintab *merged;          /* phase 2: merged shards (read only) */
+    bbuf *parts;             /* phase 2: this thread's P buffers */
+    bbuf **all_parts;        /* phase 3: every thread's buffers */
+    or_counted *res;         /* phase 3: survivors */
+    size_t nres, capres;
+} mt_job;
+
+static void *mt_minimizers(void *arg) {
+    mt_job *j = (mt_job *)arg;
+    size_t mx = max_frag_len(j->f) + 1;
+    run_t *runs = (run_t *)xmalloc(mx * sizeof(run_t));
+    uint32_t *th = (uint32_t *)xmalloc(mx * sizeof(uint32_t)), *ts = (uint32_t *)xmalloc(mx * sizeof(uint32_t));
+    for (size_t i = j->lo; i < j->hi; i++) {
+        const uint8_t *c = j->f->codes + j->f->off[i];
+        size_t n = j->f->off[i + 1] - j->f->off[i];
+        if (n < (size_t)j->k) continue;
+        size_t r = fragment_runs(c, n, j->k, j->m, runs, th, ts);
+        for (size_t q = 0; q < r; q++) {
+            size_t len = runs[q].end - runs[q].start;
+            mintab_add(&j->shards[or_spread_hash_u32(runs[q].key) % j->P], runs[q].key,
+                       (uint8_t)(len < 255 ? len : 255));
+        }
+    }
+    free(runs);
+    free(th);
+    free(ts);
+    return NULL;
+}
+
+static void *mt_merge(void *arg) {
+    mt_job *j = (mt_job *)arg;
+    for (size_t s = (size_t)j->t; s < j->P; s += (size_t)j->T) {
+        mintab *dst = &j->merged[s];
+        mintab_init(dst, 64, (uint8_t)(j->x + 1));
+        for (int u = 0; u < j->T; u++) {
+            const mintab *src = &j->all_shards[u][s];
+            for (size_t i = 0; i < src->cap; i++)
+                if (src->counts[i]) mintab_add(dst, src->keys[i], src->counts[i]);
+        }
+    }
+    return NULL;
+}
+
+static void *mt_partition(void *arg) {
+    mt_job *j = (mt_job *)arg;
+    size_t mx = max_frag_len(j->f) + 1;
+    run_t *runs = (run_t *)xmalloc(mx * sizeof(run_t));
+    uint32_t *th = (uint32_t *)xmalloc(mx * sizeof(uint32_t)), *ts = (uint32_t *)xmalloc(mx * sizeof(uint32_t));
+    for (size_t i = j->lo; i < j->hi; i++) {
+        const uint8_t *c = j->f->codes + j->f->off[i];
+        size_t n = j->f->off[i + 1] - j->f->off[i];
+        if (n < (size_t)j->k) continue;
+        uint8_t *packed = pack_codes(c, n);
+        size_t plen = (n + 3) / 4;
+        size_t r = fragment_runs(c, n, j->k, j->m, runs, th, ts);
+        for (size_t q = 0; q < r; q++) {
+            size_t shard = or_spread_hash_u32(runs[q].key) % j->P;
+            if (!((uint64_t)mintab_get(&j->merged[shard], runs[q].key) > j->x)) continue;
+            size_t send = runs[q].end + (size_t)j->k - 1;
+            if (send > n) continue;
+            append_superkmer(&j->parts[shard], packed, plen, runs[q].start, send - runs[q].start);
+        }
+        free(packed);
+    }
+    free(runs);
+    free(th);
+    free(ts);
+    return NULL;
+}
+
+static void *mt_count(void *arg) {
+    mt_job *j = (mt_job *)arg;
+    for (size_t p = (size_t)j->t; p < j->P; p += (size_t)j->T) {
+        map128 counts;
+        map_init(&counts, 1024);
+        count_ctx ctx = {&counts, j->k};
+        for (int u = 0; u < j->T; u++) for_each_record(&j->all_parts[u][p], count_record, &ctx);
+        for (size_t i = 0; i < counts.cap; i++) {
+            if (counts.state[i] != 1 || !(counts.vals[i] > j->x)) continue; /* src/lib.rs:2417 */
+            if (j->nres == j->capres) {
+                j->capres = j->capres ? j->capres * 2 : 1024;
+                j->res = (or_counted *)xrealloc(j->res, j->capres * sizeof(or_counted));
+            }
+            j->res[j->nres].lo = (uint64_t)counts.keys[i];
+            j->res[j->nres].hi = (uint64_t)(counts.keys[i] >> 64);
+            j->res[j->nres].count = counts.vals[i];
+            j->nres++;
+        }
+        map_free(&counts);
+    }
+    return NULL;
+}
+
+static void mt_run(mt_job *jobs, int T, void *(*fn)(void *)) {
+    pthread_t *th = (pthread_t *)xmalloc(sizeof(pthread_t) * (size_t)T);
+    for (int t = 0; t < T; t++) pthread_create(&th[t], NULL, fn, &jobs[t]);
+    for (int t = 0; t < T; t++) pthread_join(th[t], NULL);
+    free(th);
+}
+
+int or_count_two_pass_mt(const or_frags *f, int k, int m, uint64_t x, size_t P, int threads, or_counted **out,
+                         size_t *n_out) {
+    if (k < 1 || k > 64 || m < 1 || m > k || P == 0 || x >= 255 || (k % 2) == 0 || threads < 1) return -1;
+    const int T = threads;
+    mt_job *jobs = (mt_job *)xcalloc((size_t)T, sizeof(mt_job));
+    mintab **all_shards = (mintab **)xmalloc(sizeof(mintab *) * (size_t)T);
+    bbuf **all_parts = (bbuf **)xmalloc(sizeof(bbuf *) * (size_t)T);
+    mintab *merged = (mintab *)xcalloc(P, sizeof(mintab));
+    /* fragment ranges of about equal bases */
+    const uint64_t total = f->off[f->nfrag];
+    size_t lo = 0;
+    for (int t = 0; t < T; t++) {
+        size_t hi = lo;
+        const uint64_t goal = total / (uint64_t)T * (uint64_t)(t + 1);
+        while (hi < f->nfrag && (t == T - 1 || f->off[hi] < goal)) hi++;
+        mt_job *j = &jobs[t];
+        j->f = f;
+        j->k = k;
+        j->m = m;
+        j->x = x;
+        j->P = P;
+        j->lo = lo;
+        j->hi = hi;
+        j->t = t;
+        j->T = T;
+        j->shards = (mintab *)xmalloc(P * sizeof(mintab));
+        for (size_t s = 0; s < P; s++) mintab_init(&j->shards[s], 64, (uint8_t)(x + 1));
+        j->parts = (bbuf *)xcalloc(P, sizeof(bbuf));
+        j->all_shards = all_shards;
+        j->all_parts = all_parts;
+        j->merged = merged;
+        all_shards[t] = j->shards;
+        all_parts[t] = j->parts;
+        lo = hi;
+    }
+    mt_run(jobs, T, mt_minimizers);
+    mt_run(jobs, T, mt_merge);
+    mt_run(jobs, T, mt_partition);
+    mt_run(jobs, T, mt_count);
+    size_t r = 0;
+    for (int t = 0; t < T; t++) r += jobs[t].nres;
+    or_counted *res = (or_counted *)xmalloc((r ? r : 1) * sizeof(or_counted));
+    r = 0;
+    for (int t = 0; t < T; t++) {
+        if (jobs[t].nres) memcpy(res + r, jobs[t].res, jobs[t].nres * sizeof(or_counted));
+        r += jobs[t].nres;
+        free(jobs[t].res);
+        for (size_t s = 0; s < P; s++) {
+            free(jobs[t].shards[s].keys);
+            free(jobs[t].shards[s].counts);
+            free(jobs[t].parts[s].d);
+        }
+        free(jobs[t].shards);
+        free(jobs[t].parts);
+    }
+    for (size_t s = 0; s < P; s++) {
+        free(merged[s].keys);
+        free(merged[s].counts);
+    }
+    free(merged);
+    free(all_shards);
+    free(all_parts);
+    free(jobs);
+    if (r) qsort(res, r, sizeof(or_counted), cmp_counted); /* src/lib.rs:285 */
+    *out = res;
+    *n_out = r;
+    return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* KMC-style single-pass KFF u8 path (src/lib.rs:599-635, 2427-2593)        */
 /* ------------------------------------------------------------------------ */

@@ -61,6 +61,11 @@ int or_count_bruteforce(const or_frags *f, int k, uint64_t threshold,
 int or_count_two_pass(const or_frags *f, int k, int m, uint64_t threshold,
                       size_t partitions, or_counted **out, size_t *n_out,
                       uint64_t *partition_bytes, uint64_t *minimizers_kept);
+/* The same two-pass count on `threads` host threads, as the reference runs
+ * it under Rayon: per-thread minimizer counts merged per shard, per-thread
+ * partition buffers, partitions counted in parallel. Same output. */
+int or_count_two_pass_mt(const or_frags *f, int k, int m, uint64_t threshold,
+                         size_t partitions, int threads, or_counted **out, size_t *n_out);
 /* KMC-style single-pass KFF u8 path: full KFF file bytes. */
 int or_kff_u8_kmc(const or_frags *f, int k, int m, uint64_t threshold,
                   size_t partitions, uint8_t **bytes, size_t *n_bytes,

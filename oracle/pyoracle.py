@@ -65,6 +65,8 @@ def lib() -> C.CDLL:
         L.or_count_two_pass.argtypes = [C.POINTER(_Frags), C.c_int, C.c_int, C.c_uint64, C.c_size_t,
                                         C.POINTER(C.POINTER(_Counted)), C.POINTER(C.c_size_t),
                                         C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.or_count_two_pass_mt.argtypes = [C.POINTER(_Frags), C.c_int, C.c_int, C.c_uint64, C.c_size_t, C.c_int,
+                                           C.POINTER(C.POINTER(_Counted)), C.POINTER(C.c_size_t)]
         L.or_kff_u8_kmc.argtypes = [C.POINTER(_Frags), C.c_int, C.c_int, C.c_uint64, C.c_size_t,
                                     C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t), C.POINTER(C.c_uint64)]
         L.or_count_datasets.argtypes = [C.POINTER(_Frags), C.c_size_t, C.c_int, C.c_int, C.c_uint64, C.c_size_t,
@@ -189,6 +191,16 @@ def count_two_pass(codes, off, k, m, x, partitions):
     if rc != 0:
         raise ValueError("oracle rejected configuration")
     return _take_counted(p, n.value), pb.value, kept.value
+
+
+def count_two_pass_mt(codes, off, k, m, x, partitions, threads):
+    f, keep = _frags(codes, off)
+    p = C.POINTER(_Counted)()
+    n = C.c_size_t()
+    rc = lib().or_count_two_pass_mt(C.byref(f), k, m, x, partitions, threads, C.byref(p), C.byref(n))
+    if rc != 0:
+        raise ValueError("oracle rejected configuration")
+    return _take_counted(p, n.value)
 
 
 def kff_u8_kmc(codes, off, k, m, x, partitions) -> tuple[bytes, int]:

@@ -196,6 +196,10 @@ def test_two_pass_equals_bruteforce_random():
         bf = O.count_bruteforce(codes, off, k, x)
         tp, _, _ = O.count_two_pass(codes, off, k, m, x, P)
         assert np.array_equal(bf, tp), (k, m, x, P)
+        # the multithreaded form (the CPU baseline) gives the same array at
+        # any thread count, including more threads than fragments
+        for t in (1, 3, 8, 450):
+            assert np.array_equal(O.count_two_pass_mt(codes, off, k, m, x, P, t), tp), (k, m, x, P, t)
 
 
 def test_planted_mistake_is_caught():
