@@ -312,11 +312,14 @@ def test_many_small_inputs_plain_equals_batched():
                                  sub_rate=0.01))
     marks = np.arange(0, 12_001, 20, dtype=np.uint64)
     want = O.count_bruteforce(r.codes(), r.frag_off, 31, 1)
-    b = M.Batch.from_packed(r.words, r.frag_off, input_frag=marks)
     cfg = M.CounterConfig(31, 21, 1, 64)
-    for launch in (M.LAUNCH_PLAIN, M.LAUNCH_BATCHED, M.LAUNCH_PLAIN):
-        b.count(cfg, M.MODE_TWO_PASS, launch)
-        assert same_counted(b.download(), want), launch
+    # the same reads marked with empty inputs between and around them
+    gaps = np.sort(np.concatenate([marks, marks[::7], [0, 0, 12_000]])).astype(np.uint64)
+    for m in (marks, gaps):
+        b = M.Batch.from_packed(r.words, r.frag_off, input_frag=m)
+        for launch in (M.LAUNCH_PLAIN, M.LAUNCH_BATCHED, M.LAUNCH_PLAIN):
+            b.count(cfg, M.MODE_TWO_PASS, launch)
+            assert same_counted(b.download(), want), (launch, m.size)
 
 
 def _concat(a, b):
