@@ -73,3 +73,30 @@ def test_zi_command_line_set_difference(tmp_path):
     qc, qo = O.parse_fastx(str(q), 31)
     O.write_simplitigs(str(tmp_path / "w.fa"), O.zi_absent(ic, io, qc, qo, 31), 31)
     assert out.read_bytes() == (tmp_path / "w.fa").read_bytes()
+
+
+def test_mc_command_line_fofn(tmp_path):
+    """--fofn: one dataset per listed file, -x is a dataset-presence threshold."""
+    rng = np.random.default_rng(23)
+    base = _genome(rng, 5000)
+    paths, seqs, ds = [], [], []
+    for d in range(5):
+        g = list(base)
+        for _ in range(20):
+            q = int(rng.integers(0, len(g)))
+            g[q] = "ACGT"[("ACGT".index(g[q]) + 1) % 4]
+        g = "".join(g)
+        rs = [g[p:p + 150] for p in rng.integers(0, len(g) - 150, size=150)]
+        p = tmp_path / f"r{d}.fa"
+        write_fasta(p, rs)
+        paths.append(p)
+        seqs += rs
+        ds += [d] * len(rs)
+    fofn = tmp_path / "list.txt"
+    fofn.write_text("".join(f"{p}\n" for p in paths))
+    codes, off, dsa = O.frags_of(seqs, datasets=ds)
+    out = tmp_path / "o.fa"
+    r = _run([MC, "--fofn", "-i", fofn, "-k", 31, "-m", 21, "-x", 2, "-o", out])
+    assert r.returncode == 0, r.stderr
+    O.write_simplitigs(str(tmp_path / "w.fa"), O.count_datasets_bruteforce(codes, off, dsa, 5, 31, 2), 31)
+    assert out.read_bytes() == (tmp_path / "w.fa").read_bytes()
