@@ -61,6 +61,11 @@ typedef struct {
 #define MCZI_DEFAULT_PARTITION_COUNT 1024u
 
 const char *mczi_last_error(void);
+/* Host worker threads of later calls (ingest, writers, upload staging): the
+ * global Rayon pool size the MC and ZI command lines set from -t/--threads
+ * (src/main.rs:73-79; ThreadPoolBuilder::num_threads). 0 restores the
+ * default: every host core, at most 16. */
+int mczi_set_threads(uint32_t n);
 const char *mczi_version(void);
 void mczi_free(void *p);
 
@@ -257,7 +262,11 @@ int mczi_device_info(char *buf, size_t len);
 
 /* ---- ZI (src/zi.rs) -------------------------------------------------------- */
 /* ZI main path (src/zi.rs:64-100): absent query k-mers as simplitig FASTA.
- * m == 0 selects default_minimizer_size (src/zi.rs:109-112). */
+ * m == 0 selects default_minimizer_size (src/zi.rs:109-112). threads > 0
+ * sets the host worker threads as mczi_set_threads does (0 keeps them).
+ * ram_limit_gib bounds the device memory of one query pass: query fragments
+ * beyond it are looked up in several passes whose absent sets are merged
+ * (the reference bounds SSHash construction memory with it). */
 int mczi_zi_subtract(const char *index_path, const char *const *query_paths, size_t n,
                      uint32_t k, uint32_t m, const char *out_path, uint32_t threads,
                      uint32_t ram_limit_gib);

@@ -106,6 +106,7 @@ def _sig(name, res, args):
 
 _sig("mczi_last_error", C.c_char_p, [])
 _sig("mczi_version", C.c_char_p, [])
+_sig("mczi_set_threads", C.c_int, [C.c_uint32])
 _sig("mczi_free", None, [_P])
 _sig("mczi_validate_config", C.c_int, [C.POINTER(_Config), C.c_int])
 _sig("mczi_expand_fofns", C.c_int, [_CP, C.c_size_t, C.POINTER(C.POINTER(C.c_char_p)), C.POINTER(C.c_size_t)])
@@ -175,7 +176,7 @@ _sig("mczi_synth_free", None, [C.POINTER(_SynthReads)])
 
 #: every symbol include/mczi.h declares
 EXPORTED = [
-    "mczi_last_error", "mczi_version", "mczi_free", "mczi_validate_config", "mczi_expand_fofns",
+    "mczi_last_error", "mczi_set_threads", "mczi_version", "mczi_free", "mczi_validate_config", "mczi_expand_fofns",
     "mczi_free_strings", "mczi_count_inputs", "mczi_count_datasets", "mczi_count_inputs_to_fasta_path",
     "mczi_count_datasets_to_fasta_path", "mczi_count_inputs_to_kff_path", "mczi_count_datasets_to_kff_path",
     "mczi_count_inputs_to_kmer_fasta_path", "mczi_count_datasets_to_kmer_fasta_path", "mczi_write_fasta_path",
@@ -229,6 +230,11 @@ def device_info() -> dict:
     buf = C.create_string_buffer(4096)
     _check(_lib.mczi_device_info(buf, 4096))
     return json.loads(buf.value.decode())
+
+
+def set_threads(n: int) -> None:
+    """Host worker threads of later calls (0: every core, at most 16)."""
+    _check(_lib.mczi_set_threads(n))
 
 
 def version() -> str:

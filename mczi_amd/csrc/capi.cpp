@@ -289,6 +289,11 @@ int mczi_mczi_absent(const char *const *index_paths, size_t n_index, int index_d
 }
 
 const char *mczi_last_error(void) { return g_err.c_str(); }
+
+int mczi_set_threads(uint32_t n) {
+    g_host_threads.store(n, std::memory_order_relaxed);
+    return MCZI_OK;
+}
 const char *mczi_version(void) { return "mczi-hip 0.1.0 (gfx950)"; }
 void mczi_free(void *p) { free(p); }
 
@@ -738,6 +743,31 @@ static int batch_from_packed(const uint32_t *words, const uint64_t *frag_off, si
         uint32_t nd = 1;
         if (dataset)
             for (size_t i = 0; i < n_frag; i++) nd = std::max(nd, dataset[i] + 1);
+        // empty fragments are dropped (dataset ids and input marks follow)
+        std::vector<uint64_t> c_off, c_inp;
+        std::vector<uint32_t> c_ds;
+        const std::vector<uint64_t> keep = nonempty_fragments(frag_off, n_frag);
+        if (!keep.empty() || (n_frag && frag_off[n_frag] == frag_off[0])) {
+            c_off.reserve(keep.size() + 1);
+            c_off.push_back(frag_off[0]);
+            for (uint64_t i : keep) c_off.push_back(frag_off[i + 1]);
+            if (dataset) {
+                c_ds.reserve(keep.size());
+                for (uint64_t i : keep) c_ds.push_back(dataset[i]);
+            }
+            if (input_frag) {
+                c_inp.resize(n_inputs + 1);
+                size_t j = 0;
+                for (size_t c = 0; c <= n_inputs; c++) {
+                    while (j < keep.size() && keep[j] < input_frag[c]) j++;
+                    c_inp[c] = j;
+                }
+                input_frag = c_inp.data();
+            }
+            frag_off = c_off.data();
+            n_frag = keep.size();
+            if (dataset) dataset = c_ds.data();
+        }
         // input chunks of >= 128 Mi bases (BATCH_BASES, src/lib.rs:28) for the
         // plain launch mode, each input starting a chunk of its own
         std::vector<uint64_t> chunk_frag{0};

@@ -103,6 +103,8 @@ int main(int argc, char **argv) {
     }
     mczi_counter_config cfg{(uint32_t)k, (uint32_t)m, x, (uint32_t)parts};
     if (k > 0xffffffffull || m > 0xffffffffull || parts > 0xffffffffull) return fail("value out of range");
+    // the global worker pool (src/main.rs:73-79)
+    if (threads) mczi_set_threads((uint32_t)threads);
     std::vector<const char *> p;
     for (auto &s : inputs) p.push_back(s.c_str());
     int rc;

@@ -18,6 +18,8 @@
 
 namespace mczi {
 
+std::atomic<unsigned> g_host_threads{0};
+
 Device &Device::get() {
     static Device d;
     return d;

@@ -42,10 +42,14 @@ inline void require(bool ok, int code, const char *msg) {
 }
 
 // ---------------------------------------------------------------------------
-// Host worker threads: up to 16 (the host cores one GPU gets), one for small
-// jobs.
+// Host worker threads: the limit set by mczi_set_threads (the reference's
+// global Rayon pool size, src/main.rs:73-79), else up to 16 (the host cores
+// one GPU gets); one for small jobs.
+extern std::atomic<unsigned> g_host_threads;
 inline unsigned host_workers(size_t n) {
     if (n < (size_t(1) << 16)) return 1;
+    const unsigned lim = g_host_threads.load(std::memory_order_relaxed);
+    if (lim) return lim;
     const unsigned hc = std::thread::hardware_concurrency();
     return hc == 0 ? 1u : (hc < 16u ? hc : 16u);
 }
@@ -192,6 +196,22 @@ struct HostReads {
     }
     uint64_t cur_ = 0;
 };
+
+// Empty fragments (frag_off[i] == frag_off[i + 1]) hold no window, but the
+// device scan assumes at most one fragment start per base position (it keeps
+// each window's fragment as a small offset from its thread's first). Returns
+// the indices of the non-empty fragments when some are empty, else an empty
+// vector (nothing to drop).
+inline std::vector<uint64_t> nonempty_fragments(const uint64_t *frag_off, size_t n_frag) {
+    size_t empty = 0;
+    for (size_t i = 0; i < n_frag; i++) empty += frag_off[i + 1] == frag_off[i];
+    std::vector<uint64_t> keep;
+    if (!empty) return keep;
+    keep.reserve(n_frag - empty);
+    for (size_t i = 0; i < n_frag; i++)
+        if (frag_off[i + 1] != frag_off[i]) keep.push_back(i);
+    return keep;
+}
 
 // Parse FASTA/FASTQ (headers and qualities ignored, split at non-ACGT,
 // fragments shorter than min_len dropped; src/lib.rs:3264-3308).

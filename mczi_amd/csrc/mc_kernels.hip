@@ -8,6 +8,7 @@
 // process_superkmer_partition 2400-2425 / ..._to_kff_u8_bucketed 2512-2570
 // (count + filter). fof: 637-703, 4253-4370, 2184-2247.
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce_by_key.hpp>
 #include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
@@ -38,14 +39,29 @@ constexpr int WORDS_SPAN = (15 + SPAN + 160) / 16 + 4;
 __device__ __forceinline__ int P8(int i) { return i + (i >> 3); }
 constexpr unsigned long long STAGE_CHUNK = 8192;  // staging slots a scan block reserves at a time
 static_assert(STAGE_CHUNK >= 2 * TILE, "an abandoned staging chunk must be at least half used");
-constexpr int REC_MAX_BASES = 56;             // bases per 128-bit super-k-mer record (k <= 32)
-constexpr int REC_MAX_BASES_W = 112;          // bases per 256-bit record (33 <= k <= 63)
+// 128-bit super-k-mer record (k <= 32): bases in MC's 2-bit code, base j at
+// bits 2j: bases 0..31 in x, bases 32..43 in y bits 0..23; y bits 24..29 hold
+// the base count, 30..34 the dataset (fof), 35..63 the internal partition q.
+// A staging slot no record filled holds y = ~0 (partition REC_PAD_Q).
+constexpr int REC_MAX_BASES = 44;
+constexpr uint32_t REC_PAD_Q = 0x1FFFFFFFu;
+__host__ __device__ __forceinline__ uint32_t rec_q(const ulonglong2 &r) { return (uint32_t)(r.y >> 35); }
+__host__ __device__ __forceinline__ int rec_len(const ulonglong2 &r) { return (int)((r.y >> 24) & 63u); }
+__host__ __device__ __forceinline__ uint32_t rec_ds(const ulonglong2 &r) { return (uint32_t)(r.y >> 30) & 31u; }
 
-// 256-bit super-k-mer record for k > 32: bases 0..95 in w[0..2], bases
-// 96..111 in w[3] bits 0..31, length at w[3] bits 48..55, dataset at 56..63.
+// 256-bit record for 33 <= k <= 63: bases 0..95 in w[0..2], bases 96..103 in
+// w[3] bits 0..15, base count at w[3] bits 16..23, partition at bits 32..63
+// (REC_PAD_QW in an unfilled slot).
+constexpr int REC_MAX_BASES_W = 104;
+constexpr uint32_t REC_PAD_QW = 0xFFFFFFFFu;
 struct Rec32 {
     unsigned long long w[4];
 };
+__host__ __device__ __forceinline__ uint32_t rec_q(const Rec32 &r) { return (uint32_t)(r.w[3] >> 32); }
+__host__ __device__ __forceinline__ int rec_len(const Rec32 &r) { return (int)((r.w[3] >> 16) & 255u); }
+
+// Internal partitions per batch: at most 2^22 (two 11-bit digits).
+constexpr uint32_t MAX_INTERNAL_PARTITIONS = 1u << 22;
 
 // Count kernel geometry
 constexpr int CNT_THREADS = 256;
@@ -74,12 +90,12 @@ struct ScanArgs {
     uint32_t p_int;
     uint32_t p_mask;  // p_int - 1 when p_int is a power of two, else 0
     uint32_t wrec;
+    uint32_t ds_lo;   // fof: first dataset of the group of 32 being staged
     uint32_t *win_out;
     unsigned long long *stats;  // [0] pieces [4] records
     // staging of super-k-mer records and their internal partition
     ulonglong2 *stage_rec;
     Rec32 *stage_rec_w;  // k > 32
-    uint32_t *stage_q;
     uint64_t stage_cap;
     unsigned long long *stage_ctr;  // record slots reserved so far (the batch total at the end)
     unsigned long long *err;
@@ -168,6 +184,16 @@ __device__ __forceinline__ uint32_t extract16_bases(const uint32_t *sw, int pos)
     const uint32_t sh = (uint32_t)(pos & 15) * 2u;
     const uint64_t v = (uint64_t)sw[w] | ((uint64_t)sw[w + 1] << 32);
     return (uint32_t)(v >> sh) & 0xffffu;
+}
+
+// Staging slots [b, e) that no record fills (the rest of an abandoned chunk)
+// are marked with the padding partition, which partitioning skips.
+template <bool WIDE>
+__device__ __forceinline__ void stage_pad(const ScanArgs &a, uint64_t b, uint64_t e) {
+    for (uint64_t i = b + threadIdx.x; i < e && i < a.stage_cap; i += SCAN_THREADS) {
+        if constexpr (WIDE) a.stage_rec_w[i] = Rec32{{0ull, 0ull, 0ull, ~0ull}};
+        else a.stage_rec[i] = make_ulonglong2(0ull, ~0ull);
+    }
 }
 
 // One tile = up to TILE consecutive base positions of the segmented stream,
@@ -468,6 +494,8 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             const int vend = s_fs[(pw >> 11) + 1] - k + 1;
             if (vend < e) e = vend;
             my_pieces++;
+            // fof: only the datasets of the current group of 32 are staged
+            if (a.dataset != nullptr && a.dataset[f0 + (pw >> 11)] - a.ds_lo >= 32u) continue;
             my_nrec += e - s <= (int)a.wrec ? 1 : (e - s + (int)a.wrec - 1) / (int)a.wrec;
         }
         int tot = 0;
@@ -475,9 +503,8 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
         // Staging slots: each block takes chunks of STAGE_CHUNK slots (or the
         // tile's records, if more) from the batch-wide counter and hands out
         // consecutive slots to its tiles. A tile that does not fit in the
-        // rest of the chunk starts a new one; the rest gets the partition id
-        // p_int, which sorts after every real partition, so partitioning
-        // leaves such padding at the end. Slots at or beyond stage_cap are
+        // rest of the chunk starts a new one; the rest is marked as padding,
+        // which partitioning skips. Slots at or beyond stage_cap are
         // counted but not written: the host then sizes the buffer from the
         // counts and scans again. The order of tiles in the buffer is free:
         // partitioning and counting do not depend on it.
@@ -499,7 +526,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             my_records += (unsigned long long)tot;
         }
         __syncthreads();
-        for (uint64_t i = s_hole_b + tid; i < s_hole_e && i < a.stage_cap; i += SCAN_THREADS) a.stage_q[i] = a.p_int;
+        stage_pad<WIDE>(a, s_hole_b, s_hole_e);
         {
             uint64_t pos = s_sbase + (uint64_t)off;
             for (int pi = tid; pi < np; pi += SCAN_THREADS) {
@@ -511,7 +538,8 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
                 if (vend < e) e = vend;
                 const uint32_t hq = spread_hash_u32(spread_hash_u32(s_u.run.v[P8(s)]));
                 const uint32_t q = a.p_mask ? (hq & a.p_mask) : hq % a.p_int;
-                const uint64_t ds = a.dataset != nullptr ? (uint64_t)a.dataset[f0 + jf] : 0ull;
+                const uint64_t ds = a.dataset != nullptr ? (uint64_t)(a.dataset[f0 + jf] - a.ds_lo) : 0ull;
+                if (ds >= 32u) continue;
                 for (int cs = s; cs < e; cs += (int)a.wrec) {
                     const int ce = min(cs + (int)a.wrec, e);
                     const uint32_t L = (uint32_t)(ce - cs + k - 1);
@@ -525,10 +553,9 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
                             else if (nbc < 32) x &= (1ull << (2 * nbc)) - 1ull;
                             rw.w[c] = x ^ ((x >> 1) & 0x5555555555555555ull);  // packed code -> MC code
                         }
-                        rw.w[3] = (rw.w[3] & 0xffffffffull) | ((uint64_t)L << 48) | (ds << 56);
+                        rw.w[3] = (rw.w[3] & 0xffffull) | ((uint64_t)L << 16) | ((uint64_t)q << 32);
                         if (pos < a.stage_cap) {
                             a.stage_rec_w[pos] = rw;
-                            a.stage_q[pos] = q;
                         } else {
                             atomicOr(a.err, 2ull);
                         }
@@ -544,10 +571,9 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
                     }
                     lo64 ^= (lo64 >> 1) & 0x5555555555555555ull;  // packed code -> MC code
                     hi64 ^= (hi64 >> 1) & 0x5555555555555555ull;
-                    hi64 |= ((uint64_t)L << 48) | (ds << 56);
+                    hi64 |= ((uint64_t)L << 24) | (ds << 30) | ((uint64_t)q << 35);
                     if (pos < a.stage_cap) {
                         a.stage_rec[pos] = make_ulonglong2(lo64, hi64);
-                        a.stage_q[pos] = q;
                     } else {
                         atomicOr(a.err, 2ull);
                     }
@@ -563,24 +589,291 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             s_hole_e = chunk_end;
         }
         __syncthreads();
-        for (uint64_t i = s_hole_b + tid; i < s_hole_e && i < a.stage_cap; i += SCAN_THREADS) a.stage_q[i] = a.p_int;
+        stage_pad<WIDE>(a, s_hole_b, s_hole_e);
     }
     __shared__ unsigned long long s_red[SCAN_THREADS / 64];
     block_sum_atomic(my_pieces, &a.stats[0], s_red);
     if constexpr (MODE == STAGE) block_sum_atomic(my_records, &a.stats[4], s_red);
 }
 
-// Partitioning (src/lib.rs:3483-3560 writes each super-k-mer to its
-// partition's bucket): the staged (partition, record) pairs are radix-sorted
-// by partition (rocPRIM onesweep, records moved as 16-byte values), then each
-// partition's [start, end) is marked; no device-scope atomics per record.
-__global__ void k_part_bounds(const uint32_t *qs, uint64_t n, unsigned long long *pstart,
-                              unsigned long long *pend) {
-    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t q = qs[j];
-        if (j == 0 || qs[j - 1] != q) pstart[q] = j;
-        if (j + 1 == n || qs[j + 1] != q) pend[q] = j + 1;
+// ---------------------------------------------------------------------------
+// Partitioning (src/lib.rs:3483-3560 and 4821-4925 append each super-k-mer to
+// the file of its partition): the staged records move into one contiguous
+// bucket per internal partition q in HBM, in two counting passes over the
+// digits of q (which each record carries).
+// - Coarse pass, digit q >> fbits: every block counts the digits of a fixed
+//   range of staging slots (k_part_count), the counts are scanned per digit
+//   over the blocks (k_part_colscan) and over the digits (k_part_bucketscan),
+//   and every block moves its records to its slots (k_part_coarse).
+// - Fine pass, one coarse bucket per block (k_part_fine): it counts the fine
+//   digits q & (2^fbits - 1), writes the [start, end) of each of its
+//   partitions, and moves the records.
+// A move goes through an LDS tile of PT_THREADS * ITEMS records ordered by
+// digit (local ranks from LDS atomics, starts from a block scan), so the
+// records of one digit leave the tile as one run of consecutive stores.
+// Padding slots are dropped by the coarse pass. Record order inside a
+// partition is free: counting does not depend on it.
+constexpr int PT_THREADS = 512;
+constexpr int PT_MAX_DIGITS = 2048;
+
+template <class R> struct PartRec;
+template <> struct PartRec<ulonglong2> {
+    static constexpr int ITEMS = 16;
+    static constexpr uint32_t PAD = REC_PAD_Q;
+    // the partition of record i, from its high word only
+    __device__ static uint32_t q_at(const ulonglong2 *p, uint64_t i) {
+        return (uint32_t)(reinterpret_cast<const unsigned long long *>(p)[2 * i + 1] >> 35);
     }
+};
+template <> struct PartRec<Rec32> {
+    static constexpr int ITEMS = 8;
+    static constexpr uint32_t PAD = REC_PAD_QW;
+    __device__ static uint32_t q_at(const Rec32 *p, uint64_t i) {
+        return (uint32_t)(reinterpret_cast<const unsigned long long *>(p)[4 * i + 3] >> 32);
+    }
+};
+
+struct PartArgs {
+    const unsigned long long *n_slots;  // staged slots, padding included (device counter)
+    uint64_t cap;                       // slots of each record buffer
+    uint32_t p_int;
+    int fbits;                          // fine digit bits
+    uint32_t nc, nf;                    // coarse buckets, fine digits
+    uint32_t nblk;                      // blocks of the coarse pass
+    uint32_t *hist;                     // [nc][nblk] coarse digit counts, then their prefix over blocks
+    unsigned long long *cbase;          // [nc + 1] bucket starts, then the total
+    unsigned long long *pstart, *pend;  // [p_int] partition bounds
+    unsigned long long *err;
+};
+
+// The tile-aligned range of staging slots of coarse-pass block blockIdx.x.
+__device__ __forceinline__ void pt_range(const PartArgs &a, uint64_t tile_n, uint64_t &b, uint64_t &e) {
+    const uint64_t n = min((uint64_t)*a.n_slots, a.cap);
+    const uint64_t tiles = (n + tile_n - 1) / tile_n;
+    const uint64_t per = (tiles + a.nblk - 1) / a.nblk * tile_n;
+    b = min(n, (uint64_t)blockIdx.x * per);
+    e = min(n, b + per);
+}
+
+// Exclusive scan of s_cnt[0, nd) into s_start (nd <= PT_MAX_DIGITS); returns
+// the total. Each thread sums PT_MAX_DIGITS / PT_THREADS consecutive digits.
+__device__ __forceinline__ uint32_t pt_digit_scan(const uint32_t *s_cnt, uint32_t *s_start, uint32_t nd,
+                                                  uint32_t *s_wsum) {
+    constexpr int PER = PT_MAX_DIGITS / PT_THREADS;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    uint32_t v[PER], sum = 0;
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const uint32_t d = (uint32_t)(tid * PER + i);
+        v[i] = d < nd ? s_cnt[d] : 0u;
+        sum += v[i];
+    }
+    const uint32_t incl = (uint32_t)wave_incl_add((int)sum);
+    if (lane == 63) s_wsum[wid] = incl;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < PT_THREADS / 64; w++) {
+        const uint32_t x = s_wsum[w];
+        pre += w < wid ? x : 0u;
+        tot += x;
+    }
+    uint32_t run = pre + incl - sum;
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const uint32_t d = (uint32_t)(tid * PER + i);
+        if (d < nd) s_start[d] = run;
+        run += v[i];
+    }
+    __syncthreads();
+    return tot;
+}
+
+// Moves the records of slots [tb, te) (at most one tile) to dst: digit
+// (q >> shift) & dmask, the next free slot of each digit in s_cur (advanced
+// here). Padding records are skipped. s_cnt is zero on entry and on return.
+template <class R>
+__device__ __forceinline__ void pt_move_tile(const R *__restrict__ src, R *__restrict__ dst, uint64_t tb,
+                                             uint64_t te, int shift, uint32_t dmask, uint32_t nd, uint64_t cap,
+                                             R *s_stage, uint32_t *s_cnt, uint32_t *s_start, uint32_t *s_cur,
+                                             uint32_t *s_wsum, unsigned long long *err) {
+    constexpr int IT = PartRec<R>::ITEMS;
+    const int tid = threadIdx.x;
+    R rec[IT];
+    int dig[IT];
+    uint32_t rank[IT];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const uint64_t i = tb + (uint64_t)it * PT_THREADS + tid;
+        dig[it] = -1;
+        if (i < te) {
+            rec[it] = src[i];
+            const uint32_t q = rec_q(rec[it]);
+            if (q != PartRec<R>::PAD) dig[it] = (int)((q >> shift) & dmask);
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < IT; it++)
+        if (dig[it] >= 0) rank[it] = atomicAdd(&s_cnt[dig[it]], 1u);
+    __syncthreads();
+    const uint32_t total = pt_digit_scan(s_cnt, s_start, nd, s_wsum);
+#pragma unroll
+    for (int it = 0; it < IT; it++)
+        if (dig[it] >= 0) s_stage[s_start[dig[it]] + rank[it]] = rec[it];
+    __syncthreads();
+    for (uint32_t j = tid; j < total; j += PT_THREADS) {
+        const R r = s_stage[j];
+        const uint32_t d = (rec_q(r) >> shift) & dmask;
+        const uint64_t pos = (uint64_t)s_cur[d] + (j - s_start[d]);
+        if (pos < cap) dst[pos] = r;
+        else atomicOr(err, 16ull);
+    }
+    __syncthreads();
+    for (uint32_t d = tid; d < nd; d += PT_THREADS) {
+        s_cur[d] += s_cnt[d];
+        s_cnt[d] = 0;
+    }
+    __syncthreads();
+}
+
+template <class R>
+__global__ __launch_bounds__(PT_THREADS) void k_part_count(const R *__restrict__ src, PartArgs a) {
+    __shared__ uint32_t s_h[PT_MAX_DIGITS];
+    constexpr uint64_t TN = (uint64_t)PT_THREADS * PartRec<R>::ITEMS;
+    for (uint32_t d = threadIdx.x; d < a.nc; d += PT_THREADS) s_h[d] = 0;
+    __syncthreads();
+    uint64_t b, e;
+    pt_range(a, TN, b, e);
+    for (uint64_t t = b; t < e; t += TN) {
+        uint32_t q[PartRec<R>::ITEMS];
+#pragma unroll
+        for (int it = 0; it < PartRec<R>::ITEMS; it++) {
+            const uint64_t i = t + (uint64_t)it * PT_THREADS + threadIdx.x;
+            q[it] = i < e ? PartRec<R>::q_at(src, i) : PartRec<R>::PAD;
+        }
+#pragma unroll
+        for (int it = 0; it < PartRec<R>::ITEMS; it++)
+            if (q[it] != PartRec<R>::PAD) atomicAdd(&s_h[q[it] >> a.fbits], 1u);
+    }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < a.nc; d += PT_THREADS) a.hist[(uint64_t)d * a.nblk + blockIdx.x] = s_h[d];
+}
+
+// Per coarse digit (one block each): exclusive prefix of its counts over the
+// blocks, in place; the digit's total into cbase.
+__global__ __launch_bounds__(256) void k_part_colscan(PartArgs a) {
+    __shared__ uint32_t s_w[4];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    uint32_t *col = a.hist + (uint64_t)blockIdx.x * a.nblk;
+    const uint32_t per = (a.nblk + 255) / 256;
+    const uint32_t b = min(a.nblk, (uint32_t)tid * per), e = min(a.nblk, b + per);
+    uint32_t sum = 0;
+    for (uint32_t i = b; i < e; i++) sum += col[i];
+    const uint32_t incl = (uint32_t)wave_incl_add((int)sum);
+    if (lane == 63) s_w[wid] = incl;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+    for (int w = 0; w < 4; w++) {
+        pre += w < wid ? s_w[w] : 0u;
+        tot += s_w[w];
+    }
+    uint32_t run = pre + incl - sum;
+    for (uint32_t i = b; i < e; i++) {
+        const uint32_t v = col[i];
+        col[i] = run;
+        run += v;
+    }
+    if (tid == 0) a.cbase[blockIdx.x] = tot;
+}
+
+// Bucket starts: exclusive prefix of the coarse digit totals (one block).
+__global__ __launch_bounds__(1024) void k_part_bucketscan(PartArgs a) {
+    __shared__ unsigned long long s_w[16];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const uint32_t per = (a.nc + 1023) / 1024;
+    const uint32_t b = min(a.nc, (uint32_t)tid * per), e = min(a.nc, b + per);
+    unsigned long long sum = 0;
+    for (uint32_t i = b; i < e; i++) sum += a.cbase[i];
+    unsigned long long incl = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned long long y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) s_w[wid] = incl;
+    __syncthreads();
+    unsigned long long pre = 0, tot = 0;
+    for (int w = 0; w < 16; w++) {
+        pre += w < wid ? s_w[w] : 0ull;
+        tot += s_w[w];
+    }
+    __syncthreads();
+    unsigned long long run = pre + incl - sum;
+    for (uint32_t i = b; i < e; i++) {
+        const unsigned long long v = a.cbase[i];
+        a.cbase[i] = run;
+        run += v;
+    }
+    if (tid == 0) a.cbase[a.nc] = tot;
+}
+
+template <class R>
+__global__ __launch_bounds__(PT_THREADS) void k_part_coarse(const R *__restrict__ src, R *__restrict__ dst,
+                                                            PartArgs a) {
+    __shared__ R s_stage[PT_THREADS * PartRec<R>::ITEMS];
+    __shared__ uint32_t s_cnt[PT_MAX_DIGITS], s_start[PT_MAX_DIGITS], s_cur[PT_MAX_DIGITS];
+    __shared__ uint32_t s_wsum[PT_THREADS / 64];
+    constexpr uint64_t TN = (uint64_t)PT_THREADS * PartRec<R>::ITEMS;
+    for (uint32_t d = threadIdx.x; d < a.nc; d += PT_THREADS) {
+        s_cnt[d] = 0;
+        s_cur[d] = (uint32_t)(a.cbase[d] + a.hist[(uint64_t)d * a.nblk + blockIdx.x]);
+    }
+    __syncthreads();
+    uint64_t b, e;
+    pt_range(a, TN, b, e);
+    for (uint64_t t = b; t < e; t += TN)
+        pt_move_tile<R>(src, dst, t, min(e, t + TN), a.fbits, 0xffffffffu, a.nc, a.cap, s_stage, s_cnt, s_start,
+                        s_cur, s_wsum, a.err);
+}
+
+template <class R>
+__global__ __launch_bounds__(PT_THREADS) void k_part_fine(const R *__restrict__ src, R *__restrict__ dst,
+                                                          PartArgs a) {
+    __shared__ R s_stage[PT_THREADS * PartRec<R>::ITEMS];
+    __shared__ uint32_t s_cnt[PT_MAX_DIGITS], s_start[PT_MAX_DIGITS], s_cur[PT_MAX_DIGITS];
+    __shared__ uint32_t s_wsum[PT_THREADS / 64];
+    constexpr uint64_t TN = (uint64_t)PT_THREADS * PartRec<R>::ITEMS;
+    const uint32_t bkt = blockIdx.x, fmask = a.nf - 1;
+    const uint64_t bs = a.cbase[bkt], be = a.cbase[bkt + 1];
+    for (uint32_t d = threadIdx.x; d < a.nf; d += PT_THREADS) s_cnt[d] = 0;
+    __syncthreads();
+    for (uint64_t t = bs; t < be; t += TN) {
+        uint32_t q[PartRec<R>::ITEMS];
+#pragma unroll
+        for (int it = 0; it < PartRec<R>::ITEMS; it++) {
+            const uint64_t i = t + (uint64_t)it * PT_THREADS + threadIdx.x;
+            q[it] = i < be ? PartRec<R>::q_at(src, i) : PartRec<R>::PAD;
+        }
+#pragma unroll
+        for (int it = 0; it < PartRec<R>::ITEMS; it++)
+            if (q[it] != PartRec<R>::PAD) atomicAdd(&s_cnt[q[it] & fmask], 1u);
+    }
+    __syncthreads();
+    pt_digit_scan(s_cnt, s_start, a.nf, s_wsum);
+    for (uint32_t f = threadIdx.x; f < a.nf; f += PT_THREADS) {
+        const uint32_t q = (bkt << a.fbits) | f;
+        const uint64_t st = bs + s_start[f];
+        if (q < a.p_int) {
+            a.pstart[q] = st;
+            a.pend[q] = st + s_cnt[f];
+        }
+        s_cur[f] = (uint32_t)st;
+        s_cnt[f] = 0;
+    }
+    __syncthreads();
+    for (uint64_t t = bs; t < be; t += TN)
+        pt_move_tile<R>(src, dst, t, min(be, t + TN), 0, fmask, a.nf, a.cap, s_stage, s_cnt, s_start, s_cur, s_wsum,
+                        a.err);
 }
 
 // ---------------------------------------------------------------------------
@@ -602,7 +895,8 @@ struct CountArgs {
     uint32_t *out_count;
     uint32_t *out_part;
     uint64_t out_cap;
-    // [0] work queue [1] out cursor [2] distinct (LDS) [3] distinct (HBM) [4] error bits
+    // [1] output slots reserved (padding included) [2] distinct (LDS)
+    // [3] distinct (HBM) [4] error bits [6] padding slots
     unsigned long long *ctr;
 };
 
@@ -654,6 +948,61 @@ __device__ __forceinline__ unsigned long long wave_append(bool pred, unsigned lo
     return base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
+// Output slots are reserved in chunks of OUT_CHUNK per block: one device-scope
+// atomic per chunk instead of one per partition. The unused rest of an
+// abandoned chunk, and of each block's last one, holds padding entries (key
+// ~0, partition OUT_PAD) that sort after every k-mer; ctr[6] counts them.
+constexpr unsigned long long OUT_CHUNK = 1024;
+constexpr uint32_t OUT_PAD = 0xFFFFFFFFu;
+
+template <bool WIDE>
+__device__ __forceinline__ void out_pad(const CountArgs &a, unsigned long long b, unsigned long long e) {
+    for (unsigned long long i = b + threadIdx.x; i < e && i < a.out_cap; i += blockDim.x) {
+        a.out_kmer[i] = EMPTY_KEY;
+        if constexpr (WIDE) a.out_hi[i] = EMPTY_KEY;
+        a.out_count[i] = 0;
+        a.out_part[i] = OUT_PAD;
+    }
+}
+
+// Thread 0 keeps the block's current chunk [o_cur, o_end) in registers. For
+// tot survivors of a partition: *s_obase gets their first slot; the padded
+// rest of an abandoned chunk goes to s_pad[0..1] (written by every thread
+// after the caller's barrier).
+__device__ __forceinline__ void out_reserve(const CountArgs &a, unsigned long long tot, unsigned long long &o_cur,
+                                            unsigned long long &o_end, unsigned long long &my_pad,
+                                            unsigned long long *s_obase, unsigned long long *s_pad) {
+    if (threadIdx.x != 0) return;
+    s_pad[0] = s_pad[1] = 0;
+    if (!tot) return;
+    if (o_cur + tot > o_end) {
+        s_pad[0] = o_cur;
+        s_pad[1] = o_end;
+        my_pad += o_end - o_cur;
+        const unsigned long long sz = tot > OUT_CHUNK ? tot : OUT_CHUNK;
+        o_cur = atomicAdd(&a.ctr[1], sz);
+        o_end = o_cur + sz;
+    }
+    *s_obase = o_cur;
+    o_cur += tot;
+}
+
+// The end of a count kernel: the rest of the block's last chunk is padded and
+// the padding total added to ctr[6].
+template <bool WIDE>
+__device__ __forceinline__ void out_finish(const CountArgs &a, unsigned long long o_cur, unsigned long long o_end,
+                                           unsigned long long my_pad, unsigned long long *s_pad) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_pad[0] = o_cur;
+        s_pad[1] = o_end;
+        my_pad += o_end - o_cur;
+        if (my_pad) atomicAdd(&a.ctr[6], my_pad);
+    }
+    __syncthreads();
+    out_pad<WIDE>(a, s_pad[0], s_pad[1]);
+}
+
 // Canonical MC encoding (src/lib.rs:3589-3614) of the k <= 32 MC-coded bases
 // in x (base j at bits 2j, x already masked to 2k bits): forward = 2-bit
 // groups reversed (first base most significant), reverse complement =
@@ -700,10 +1049,11 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
     __shared__ uint8_t s_mark[CNT_THREADS / 64][64];    // record starts in the current 64 k-mers
     __shared__ int s_wsum[CNT_THREADS / 64];
     __shared__ unsigned long long s_red[CNT_THREADS / 64];
-    __shared__ unsigned long long s_obase;
+    __shared__ unsigned long long s_obase, s_pad[2];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wid = tid >> 6;
     const int k = a.k;
+    unsigned long long o_cur = 0, o_end = 0, my_pad = 0;  // thread 0: the block's output chunk
     const uint64_t kmask = k >= 32 ? ~0ull : ((1ull << (2 * k)) - 1ull);
     const int rshift = 64 - 2 * k;
     unsigned long long my_distinct = 0;
@@ -721,7 +1071,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
             int n = 0;
             if (ri < re) {
                 r = a.records[ri];
-                n = (int)((r.y >> 48) & 255u) - k + 1;
+                n = rec_len(r) - k + 1;
                 if (n < 0) n = 0;
             }
             s_rec[wid][lane] = r;
@@ -747,11 +1097,11 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
                 const int off = j - excl;  // k-mer start inside the record
                 const ulonglong2 ro = s_rec[wid][o];
                 const unsigned long long rx = ro.x, ry = ro.y;
-                const uint64_t rd = ry & 0xFFFFFFFFFFFFull;  // bases 32..55
+                const uint64_t rd = ry & 0xFFFFFFull;  // bases 32..43
                 const int sh = 2 * off;
                 const uint64_t lo = sh == 0 ? rx : (sh < 64 ? ((rx >> sh) | (rd << (64 - sh))) : (rd >> (sh - 64)));
                 const unsigned long long key = canon_from_mc(lo & kmask, kmask, rshift);
-                const uint32_t val = (MODE == 0) ? 1u : (1u << (uint32_t)((ry >> 56) & 31u));
+                const uint32_t val = (MODE == 0) ? 1u : (1u << (uint32_t)((ry >> 30) & 31u));
                 uint32_t slot = lds_slot(key);
                 // Home slot: a plain load settles the common case (key present,
                 // or empty slot claimed by one CAS); later probes use the CAS
@@ -808,8 +1158,9 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
             pre += (w < wid) ? s_wsum[w] : 0;
             tot += s_wsum[w];
         }
-        if (tid == 0) s_obase = tot ? atomicAdd(&a.ctr[1], (unsigned long long)tot) : 0ull;
+        out_reserve(a, (unsigned long long)tot, o_cur, o_end, my_pad, &s_obase, s_pad);
         __syncthreads();
+        out_pad<false>(a, s_pad[0], s_pad[1]);
         unsigned long long pos = s_obase + (unsigned long long)(pre + x - cnt);
 #pragma unroll
         for (int i = 0; i < PER; i++) {
@@ -827,6 +1178,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
         }
         __syncthreads();
     }
+    out_finish<false>(a, o_cur, o_end, my_pad, s_pad);
     block_sum_atomic(my_distinct, &a.ctr[2], s_red);
 }
 
@@ -938,10 +1290,11 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count_wide(CountArgs a) {
     __shared__ uint8_t s_mark[CNT_THREADS / 64][64];
     __shared__ int s_wsum[CNT_THREADS / 64];
     __shared__ unsigned long long s_red[CNT_THREADS / 64];
-    __shared__ unsigned long long s_obase;
+    __shared__ unsigned long long s_obase, s_pad[2];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wid = tid >> 6;
     const int k = a.k;
+    unsigned long long o_cur = 0, o_end = 0, my_pad = 0;  // thread 0: the block's output chunk
     const uint64_t mb = (k - 32) >= 32 ? ~0ull : ((1ull << (2 * (k - 32))) - 1ull);
     unsigned long long my_distinct = 0;
     for (uint32_t q = blockIdx.x; q < a.p_int; q += gridDim.x) {
@@ -958,7 +1311,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count_wide(CountArgs a) {
             int n = 0;
             if (ri < re) {
                 r = a.records_w[ri];
-                n = (int)((r.w[3] >> 48) & 255u) - k + 1;
+                n = rec_len(r) - k + 1;
                 if (n < 0) n = 0;
             }
             s_rec[wid][lane] = r;
@@ -978,9 +1331,9 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count_wide(CountArgs a) {
                 const int o = (mk - 1) & 63;
                 const int excl = __shfl(my_excl, o, 64);
                 if (j >= total) continue;
-                const int off = j - excl;  // k-mer start inside the record (<= 112 - k)
+                const int off = j - excl;  // k-mer start inside the record (<= 104 - k)
                 const Rec32 ro = s_rec[wid][o];
-                const uint64_t W3 = ro.w[3] & 0xffffffffull;
+                const uint64_t W3 = ro.w[3] & 0xffffull;  // bases 96..103
                 const int bo = 2 * off, wi = bo >> 6, sh = bo & 63;
                 const uint64_t x0 = wi == 0 ? ro.w[0] : (wi == 1 ? ro.w[1] : ro.w[2]);
                 const uint64_t x1 = wi == 0 ? ro.w[1] : (wi == 1 ? ro.w[2] : W3);
@@ -1052,8 +1405,9 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count_wide(CountArgs a) {
             pre += (w < wid) ? s_wsum[w] : 0;
             tot += s_wsum[w];
         }
-        if (tid == 0) s_obase = tot ? atomicAdd(&a.ctr[1], (unsigned long long)tot) : 0ull;
+        out_reserve(a, (unsigned long long)tot, o_cur, o_end, my_pad, &s_obase, s_pad);
         __syncthreads();
+        out_pad<true>(a, s_pad[0], s_pad[1]);
         unsigned long long pos = s_obase + (unsigned long long)(pre + x - cnt);
 #pragma unroll
         for (int i = 0; i < PER; i++) {
@@ -1069,6 +1423,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count_wide(CountArgs a) {
         }
         __syncthreads();
     }
+    out_finish<true>(a, o_cur, o_end, my_pad, s_pad);
     block_sum_atomic(my_distinct, &a.ctr[2], s_red);
 }
 
@@ -1152,7 +1507,7 @@ __global__ void k_kff_keys(const unsigned long long *kmer, const uint32_t *part,
          i += (uint64_t)gridDim.x * blockDim.x) {
         const unsigned long long K = kmer[i];
         key2[i] = ((K & 255ull) << 56) | (K >> 8);
-        pu[i] = part[i] % p_user;
+        pu[i] = part[i] == OUT_PAD ? p_user : part[i] % p_user;  // padding sorts last
         idx[i] = (uint32_t)i;
     }
 }
@@ -1207,15 +1562,20 @@ struct McPlan {
     std::vector<uint64_t> chunk_tile;  // tile index where each chunk starts
     uint64_t n_windows = 0;
     // state
-    DevBuf base, cursor, records, ctr, ovf_keys, ovf_vals, ovf_part, stage_rec, stage_q, pqs, part_tmp;
+    // records: the coarse pass's output; stage_rec: the staging slots, then
+    // the records in partition order (fine pass); base / cursor: each
+    // partition's [start, end); hist / cbase: coarse digit counts per block
+    // and bucket starts
+    DevBuf base, cursor, records, ctr, ovf_keys, ovf_vals, ovf_part, stage_rec, hist, cbase;
     DevBuf records_w, stage_rec_w, ovf_hi, out_hi;  // k > 32
     uint64_t stage_cap = 0, n_staged = 0, n_records = 0;
-    size_t part_tmp_bytes = 0;
+    uint32_t part_blocks = 0;  // blocks of the coarse pass
+    uint32_t ds_lo = 0;        // fof: first dataset of the group being counted
     DevBuf out_kmer, out_count, out_part;
     DevBuf sort_tmp, key2, pu, idx, idx2, pu2, k2out;
     uint64_t ovf_slots = 0;
     uint32_t p_int = 0;
-    uint64_t rec_cap = 0, out_cap = 0;
+    uint64_t out_cap = 0;
     uint64_t rec_sig = 0;
     double target_override = 0;
     // graph cache
@@ -1413,7 +1773,7 @@ static uint64_t next_pow2(uint64_t v) {
 // q % P_user, so the reference's output order is kept.
 static uint32_t choose_partitions(uint64_t n_windows, uint32_t p_user, double target) {
     uint64_t p = p_user;
-    while ((double)n_windows / (double)p > target && p * 2 <= (1ull << 24)) p *= 2;
+    while ((double)n_windows / (double)p > target && p * 2 <= MAX_INTERNAL_PARTITIONS) p *= 2;
     return (uint32_t)p;
 }
 
@@ -1468,7 +1828,6 @@ static ScanArgs scan_args(Batch &b, McPlan &p, const mczi_counter_config &cfg) {
     sa.stats = p.ctr.as<unsigned long long>() + 8;
     sa.stage_rec = p.stage_rec.as<ulonglong2>();
     sa.stage_rec_w = p.stage_rec_w.as<Rec32>();
-    sa.stage_q = p.stage_q.as<uint32_t>();
     sa.stage_cap = p.stage_cap;
     sa.stage_ctr = p.ctr.as<unsigned long long>() + 11;
     sa.err = p.ctr.as<unsigned long long>() + 4;
@@ -1511,6 +1870,7 @@ static void enqueue_stage_a(Batch &b, McPlan &p, const mczi_counter_config &cfg,
                             int launch, hipStream_t s, Timer *tm) {
     ScanArgs sa = scan_args(b, p, cfg);
     if (mode != MCZI_MODE_DATASETS) sa.dataset = nullptr;
+    sa.ds_lo = p.ds_lo;
     unsigned long long *ctr = p.ctr.as<unsigned long long>();
     fill_u64(ctr, 16, 0, s);
     if (tm) tm->mark(s);
@@ -1520,8 +1880,8 @@ static void enqueue_stage_a(Batch &b, McPlan &p, const mczi_counter_config &cfg,
 
 static CountArgs count_args(McPlan &p, const mczi_counter_config &cfg) {
     CountArgs ca{};
-    ca.records = p.records.as<ulonglong2>();
-    ca.records_w = p.records_w.as<Rec32>();
+    ca.records = p.stage_rec.as<ulonglong2>();  // in partition order after the fine pass
+    ca.records_w = p.stage_rec_w.as<Rec32>();
     ca.ovf_hi = p.ovf_hi.as<unsigned long long>();
     ca.out_hi = p.out_hi.as<unsigned long long>();
     ca.base = p.base.as<unsigned long long>();
@@ -1547,35 +1907,70 @@ static unsigned bits_for(uint64_t n) {
     return b;
 }
 
-// Stage B: records into partition order (radix sort of (partition, record)
-// pairs), per-partition LDS counting with HBM spill, emit.
+// Partition-pass geometry: coarse digit q >> fbits, fine digit q & (2^fbits - 1).
+struct PartGeom {
+    int fbits = 0;
+    uint32_t nc = 1, nf = 1;
+};
+static PartGeom part_geom(uint32_t p_int) {
+    PartGeom g;
+    const unsigned bits = bits_for(p_int);
+    g.fbits = (int)(bits / 2);
+    g.nf = 1u << g.fbits;
+    g.nc = (uint32_t)(((uint64_t)p_int + g.nf - 1) >> g.fbits);
+    return g;
+}
+
+static PartArgs part_args(McPlan &p) {
+    const PartGeom g = part_geom(p.p_int);
+    PartArgs pa{};
+    pa.n_slots = p.ctr.as<unsigned long long>() + 11;
+    pa.cap = p.stage_cap;
+    pa.p_int = p.p_int;
+    pa.fbits = g.fbits;
+    pa.nc = g.nc;
+    pa.nf = g.nf;
+    pa.nblk = p.part_blocks;
+    pa.hist = p.hist.as<uint32_t>();
+    pa.cbase = p.cbase.as<unsigned long long>();
+    pa.pstart = p.base.as<unsigned long long>();
+    pa.pend = p.cursor.as<unsigned long long>();
+    pa.err = p.ctr.as<unsigned long long>() + 4;
+    return pa;
+}
+
+// Stage B: records into partition buckets (two counting passes), then per
+// partition LDS counting with HBM spill and the emit. Every launch is a
+// kernel of this file (no library call, no memset), and no size is read back
+// by the host, so the stage can be captured in a graph with stage A.
 static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg, int mode,
                             int launch, hipStream_t s, Timer *tm) {
     Device &d = Device::get();
     (void)b;
     (void)launch;
-    const uint64_t n = p.n_staged;   // staging slots, padding included
-    const uint64_t nr = p.n_records;  // records: the sorted prefix before the padding
     const bool wide = cfg.k > 32;
     fill_u64(p.ovf_keys.as<unsigned long long>(), p.ovf_slots, EMPTY_KEY, s);
     if (wide) fill_u64(p.ovf_hi.as<unsigned long long>(), p.ovf_slots, EMPTY_KEY, s);
     fill_u32(p.ovf_vals.as<uint32_t>(), p.ovf_slots, 0, s);
-    fill_u64(p.base.as<unsigned long long>(), p.p_int, 0, s);
-    fill_u64(p.cursor.as<unsigned long long>(), p.p_int, 0, s);
     if (tm) tm->mark(s);
-    if (n) {
-        const unsigned g = grid_for(nr, 256, 16384);
-        size_t tmp = p.part_tmp_bytes;
-        if (wide)
-            MCZI_HIP(rocprim::radix_sort_pairs(p.part_tmp.as<void>(), tmp, p.stage_q.as<uint32_t>(),
-                                               p.pqs.as<uint32_t>(), p.stage_rec_w.as<Rec32>(),
-                                               p.records_w.as<Rec32>(), (size_t)n, 0, bits_for(p.p_int + 1), s));
-        else
-            MCZI_HIP(rocprim::radix_sort_pairs(p.part_tmp.as<void>(), tmp, p.stage_q.as<uint32_t>(),
-                                               p.pqs.as<uint32_t>(), p.stage_rec.as<ulonglong2>(),
-                                               p.records.as<ulonglong2>(), (size_t)n, 0, bits_for(p.p_int + 1), s));
-        hipLaunchKernelGGL(k_part_bounds, dim3(g), dim3(256), 0, s, p.pqs.as<uint32_t>(), nr,
-                           p.base.as<unsigned long long>(), p.cursor.as<unsigned long long>());
+    const PartArgs pa = part_args(p);
+    if (wide) {
+        hipLaunchKernelGGL(k_part_count<Rec32>, dim3(pa.nblk), dim3(PT_THREADS), 0, s, p.stage_rec_w.as<Rec32>(), pa);
+        hipLaunchKernelGGL(k_part_colscan, dim3(pa.nc), dim3(256), 0, s, pa);
+        hipLaunchKernelGGL(k_part_bucketscan, dim3(1), dim3(1024), 0, s, pa);
+        hipLaunchKernelGGL(k_part_coarse<Rec32>, dim3(pa.nblk), dim3(PT_THREADS), 0, s, p.stage_rec_w.as<Rec32>(),
+                           p.records_w.as<Rec32>(), pa);
+        hipLaunchKernelGGL(k_part_fine<Rec32>, dim3(pa.nc), dim3(PT_THREADS), 0, s, p.records_w.as<Rec32>(),
+                           p.stage_rec_w.as<Rec32>(), pa);
+    } else {
+        hipLaunchKernelGGL(k_part_count<ulonglong2>, dim3(pa.nblk), dim3(PT_THREADS), 0, s,
+                           p.stage_rec.as<ulonglong2>(), pa);
+        hipLaunchKernelGGL(k_part_colscan, dim3(pa.nc), dim3(256), 0, s, pa);
+        hipLaunchKernelGGL(k_part_bucketscan, dim3(1), dim3(1024), 0, s, pa);
+        hipLaunchKernelGGL(k_part_coarse<ulonglong2>, dim3(pa.nblk), dim3(PT_THREADS), 0, s,
+                           p.stage_rec.as<ulonglong2>(), p.records.as<ulonglong2>(), pa);
+        hipLaunchKernelGGL(k_part_fine<ulonglong2>, dim3(pa.nc), dim3(PT_THREADS), 0, s, p.records.as<ulonglong2>(),
+                           p.stage_rec.as<ulonglong2>(), pa);
     }
     if (tm) tm->mark(s);
     CountArgs ca = count_args(p, cfg);
@@ -1601,9 +1996,50 @@ static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg,
     if (tm) tm->mark(s);
 }
 
-static void sort_output(Batch &b, McPlan &p, uint64_t n, int order, uint32_t p_user, int k,
+// Blocks of the count grid (the output padding is at most one chunk each).
+static uint64_t count_grid_blocks(const McPlan &p, bool wide) {
+    Device &d = Device::get();
+    static const unsigned count_res = resident_blocks((const void *)k_count<0>, CNT_THREADS);
+    static const unsigned wide_res = resident_blocks((const void *)k_count_wide, CNT_THREADS);
+    return std::min<uint64_t>(p.p_int, (uint64_t)d.cu_count() * (wide ? wide_res : count_res));
+}
+
+// Output entries [0, n_slots) without the padding, in their order (the
+// unsorted output order).
+__global__ void k_compact_out(const unsigned long long *kin, const unsigned long long *hin, const uint32_t *cin,
+                              const uint32_t *pin, uint64_t n, unsigned long long *kout, unsigned long long *hout,
+                              uint32_t *cout, unsigned long long *cursor) {
+    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < n; i0 += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = i0 + threadIdx.x;
+        const bool keep = i < n && pin[i] != OUT_PAD;
+        const unsigned long long pos = wave_append(keep, cursor);
+        if (keep) {
+            kout[pos] = kin[i];
+            if (hin) hout[pos] = hin[i];
+            cout[pos] = cin[i];
+        }
+    }
+}
+
+// The n output slots (n_real k-mers and the padding, which sorts last) into
+// b.out_* in the requested order; the first n_real entries are the k-mers.
+static void sort_output(Batch &b, McPlan &p, uint64_t n, uint64_t n_real, int order, uint32_t p_user, int k,
                         hipStream_t s) {
-    if (order == MCZI_ORDER_NONE || n < 2) return;
+    if (order == MCZI_ORDER_NONE || n_real < 2) {
+        // drop the padding, keep the emit order
+        b.out_kmer.ensure(std::max<uint64_t>(n_real, 1) * 8);
+        b.out_count.ensure(std::max<uint64_t>(n_real, 1) * 4);
+        if (k > 32) b.out_hi.ensure(std::max<uint64_t>(n_real, 1) * 8);
+        if (!n) return;
+        unsigned long long *cur = p.ctr.as<unsigned long long>() + 14;
+        fill_u64(cur, 1, 0, s);
+        hipLaunchKernelGGL(k_compact_out, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s,
+                           p.out_kmer.as<unsigned long long>(), k > 32 ? p.out_hi.as<unsigned long long>() : nullptr,
+                           p.out_count.as<uint32_t>(), p.out_part.as<uint32_t>(), n,
+                           b.out_kmer.as<unsigned long long>(), k > 32 ? b.out_hi.as<unsigned long long>() : nullptr,
+                           b.out_count.as<uint32_t>(), cur);
+        return;
+    }
     require(n < (1ull << 32), MCZI_E_INVALID, "too many output k-mers for one sort");
     unsigned long long *kin = p.out_kmer.as<unsigned long long>();
     uint32_t *cin = p.out_count.as<uint32_t>();
@@ -1642,7 +2078,9 @@ static void sort_output(Batch &b, McPlan &p, uint64_t n, int order, uint32_t p_u
         return;
     }
     if (order == MCZI_ORDER_ENCODED) {
-        const unsigned end_bit = (unsigned)std::min(64, 2 * k);
+        // keys use 2k <= 62 bits (k odd); bit 2k is set only in the padding,
+        // which therefore sorts after every k-mer in any top-bit pass
+        const unsigned end_bit = (unsigned)std::min(64, 2 * k + 1);
         // Radix passes over the top 24 bits only, then the runs of equal
         // 24-bit prefixes (about n / 2^24 k-mers each) finished in place;
         // keys up to 56 bits, or a run too long to finish, take the full sort.
@@ -1658,8 +2096,8 @@ static void sort_output(Batch &b, McPlan &p, uint64_t n, int order, uint32_t p_u
         if (!split) return;
         unsigned long long *flag = p.ctr.as<unsigned long long>() + 15;
         fill_u64(flag, 1, 0, s);
-        hipLaunchKernelGGL(k_finish_runs<true>, dim3(grid_for(n, FIN_CHUNK, 1u << 20)), dim3(256), 0, s, kout, cout,
-                           n, (int)begin_bit, flag);
+        hipLaunchKernelGGL(k_finish_runs<true>, dim3(grid_for(n_real, FIN_CHUNK, 1u << 20)), dim3(256), 0, s, kout,
+                           cout, n_real, (int)begin_bit, flag);
         unsigned long long h = 0;
         MCZI_HIP(hipMemcpyAsync(&h, flag, 8, hipMemcpyDeviceToHost, s));
         MCZI_HIP(hipStreamSynchronize(s));
@@ -1693,8 +2131,7 @@ static void sort_output(Batch &b, McPlan &p, uint64_t n, int order, uint32_t p_u
                                        p.idx2.as<uint32_t>(), (size_t)n, 0, 64, s));
     hipLaunchKernelGGL(k_gather_u32, dim3(g), dim3(256), 0, s, p.pu.as<uint32_t>(), p.idx2.as<uint32_t>(),
                        p.pu2.as<uint32_t>(), n);
-    unsigned pbits = 1;
-    while ((1ull << pbits) < p_user) pbits++;
+    const unsigned pbits = bits_for((uint64_t)p_user + 1);  // the padding sorts as p_user
     MCZI_HIP(rocprim::radix_sort_pairs(p.sort_tmp.as<void>(), tmp2, p.pu2.as<uint32_t>(),
                                        p.pu.as<uint32_t>(), p.idx2.as<uint32_t>(), p.idx.as<uint32_t>(),
                                        (size_t)n, 0, pbits, s));
@@ -1704,6 +2141,220 @@ static void sort_output(Batch &b, McPlan &p, uint64_t n, int order, uint32_t p_u
 
 static uint64_t mix_sig(uint64_t h, uint64_t v) {
     return spread_hash_u64(h ^ (v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2)));
+}
+
+// One count of the batch on the device (for fof, of the datasets
+// [p.ds_lo, p.ds_lo + 32)): scan, partitioning, counting and emit, as one
+// graph with the batched launch. The output slots (padding included) stay in
+// p.out_*; h receives the counters. Staging that outgrew its capacity, or a
+// spill table that filled, is redone with more room.
+static void count_pass(Batch &b, McPlan &p, const mczi_counter_config &cfg, int mode, int launch, Timer &tm,
+                       bool want_times, unsigned long long *h) {
+    Device &d = Device::get();
+    hipStream_t s = d.stream();
+    const uint32_t k = cfg.k;
+    const bool wide = k > 32;
+    const bool use_graph = launch == MCZI_LAUNCH_BATCHED && getenv("MCZI_NO_GRAPH") == nullptr;
+    double target = p.target_override > 0 ? p.target_override : tuned_target();
+    static const unsigned part_res =
+        std::min(resident_blocks((const void *)k_part_coarse<ulonglong2>, PT_THREADS),
+                 resident_blocks((const void *)k_part_coarse<Rec32>, PT_THREADS));
+    for (int attempt = 0;; attempt++) {
+        p.p_int = choose_partitions(p.n_windows, cfg.partition_count, target);
+        const PartGeom g = part_geom(p.p_int);
+        require(g.nc <= PT_MAX_DIGITS && g.nf <= PT_MAX_DIGITS, MCZI_E_INVALID,
+                "too many internal partitions for the device partitioning");
+        p.base.ensure(((size_t)p.p_int + 1) * 8);
+        p.cursor.ensure(((size_t)p.p_int + 1) * 8);
+        p.ctr.ensure(16 * 8);
+        p.part_blocks = (uint32_t)d.cu_count() * part_res;
+        p.hist.ensure((size_t)g.nc * p.part_blocks * 4);
+        p.cbase.ensure(((size_t)g.nc + 1) * 8);
+        const uint64_t ovf = next_pow2(std::max<uint64_t>(p.n_windows / 16, 1 << 20)) << attempt;
+        p.ovf_slots = ovf;
+        p.ovf_keys.ensure(ovf * 8);
+        p.ovf_vals.ensure(ovf * 4);
+        p.ovf_part.ensure(ovf * 4);
+        if (wide) p.ovf_hi.ensure(ovf * 8);
+        // every emitted k-mer has more than x windows; plus one padded chunk per count block
+        p.out_cap = p.n_windows / (cfg.threshold + 1) + 16 + count_grid_blocks(p, wide) * OUT_CHUNK;
+        p.out_kmer.ensure(p.out_cap * 8);
+        if (wide) p.out_hi.ensure(p.out_cap * 8);
+        p.out_count.ensure(p.out_cap * 4);
+        p.out_part.ensure(p.out_cap * 4);
+
+        // Staging capacity: the expected record count of a random minimizer
+        // scheme (density 2 / (w + 1) per window, plus record splits, one
+        // start per fragment and one cut per tile) with a 25% margin, never
+        // more than one record per window, plus the padding of each scan
+        // block's last chunk. The scan counts every slot it reserves; if the
+        // total exceeds the capacity, the step runs again with room for twice
+        // the records (the padding of an abandoned chunk is less than its
+        // records) plus a full chunk per block.
+        const uint64_t cfg_sig = mix_sig(mix_sig(k, cfg.minimizer), b.n_bases);
+        const uint32_t wrec = (uint32_t)((wide ? REC_MAX_BASES_W : REC_MAX_BASES) - k + 1);
+        const uint64_t chunk_pad = launch_scan(p, ScanArgs{}, launch, s, false) * STAGE_CHUNK;
+        if (p.rec_sig != cfg_sig) {
+            const double w = (double)(k - cfg.minimizer + 1);
+            const double expect = (double)p.n_windows * (2.0 / (w + 1.0) + 1.0 / wrec) + (double)b.n_frag +
+                                  (double)p.n_tiles;
+            p.stage_cap = std::min<uint64_t>((uint64_t)(1.25 * expect) + 4096, p.n_windows) + chunk_pad + 16;
+            if (const char *e = getenv("MCZI_STAGE_CAPACITY"))  // initial capacity in slots
+                p.stage_cap = std::max<uint64_t>(strtoull(e, nullptr, 10), 1);
+            p.rec_sig = cfg_sig;
+        }
+        require(p.stage_cap < (1ull << 32), MCZI_E_INVALID,
+                "too many super-k-mer records for one batch (limit 2^32): split the input");
+        if (wide) {
+            p.stage_rec_w.ensure(p.stage_cap * sizeof(Rec32));
+            p.records_w.ensure(p.stage_cap * sizeof(Rec32));
+        } else {
+            p.stage_rec.ensure(p.stage_cap * 16);
+            p.records.ensure(p.stage_cap * 16);
+        }
+
+        tm.n = 0;
+        tm.mark(s);
+        if (use_graph) {
+            uint64_t sig = cfg_sig;
+            sig = mix_sig(mix_sig(mix_sig(mix_sig(sig, cfg.threshold), (uint64_t)mode), p.p_int), p.ds_lo);
+            for (const DevBuf *db : {&b.words, &b.off, &b.dataset, &p.base, &p.cursor, &p.records, &p.ctr,
+                                     &p.ovf_keys, &p.ovf_vals, &p.ovf_part, &p.out_kmer, &p.out_count,
+                                     &p.out_part, &p.tile_start, &p.tile_frag, &p.tile_len, &p.stage_rec,
+                                     &p.hist, &p.cbase, &p.records_w, &p.stage_rec_w, &p.ovf_hi, &p.out_hi})
+                sig = mix_sig(sig, (uint64_t)(uintptr_t)db->as<void>());
+            sig = mix_sig(mix_sig(mix_sig(sig, p.ovf_slots), p.stage_cap), p.out_cap);
+            sig = mix_sig(mix_sig(sig, p.part_blocks), (uint64_t)launch);
+            if (!p.graph || p.graph_sig != sig) {
+                if (p.graph) {
+                    MCZI_HIP(hipGraphExecDestroy(p.graph));
+                    p.graph = nullptr;
+                }
+                hipGraph_t gr;
+                MCZI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                enqueue_stage_a(b, p, cfg, mode, launch, s, nullptr);
+                enqueue_stage_b(b, p, cfg, mode, launch, s, nullptr);
+                MCZI_HIP(hipStreamEndCapture(s, &gr));
+                MCZI_HIP(hipGraphInstantiate(&p.graph, gr, nullptr, nullptr, 0));
+                MCZI_HIP(hipGraphDestroy(gr));
+                p.graph_sig = sig;
+            }
+            MCZI_HIP(hipGraphLaunch(p.graph, s));
+        } else {
+            enqueue_stage_a(b, p, cfg, mode, launch, s, want_times ? &tm : nullptr);
+            enqueue_stage_b(b, p, cfg, mode, launch, s, want_times ? &tm : nullptr);
+        }
+        tm.mark(s);
+        MCZI_HIP(hipMemcpyAsync(h, p.ctr.as<void>(), 16 * 8, hipMemcpyDeviceToHost, s));
+        MCZI_HIP(hipStreamSynchronize(s));
+        MCZI_HIP(hipGetLastError());
+        if (h[11] > p.stage_cap || (h[4] & 2ull)) {
+            // staging outgrew its capacity: everything after the scan ran on
+            // the slots that fit, and is redone
+            require(h[12] < (1ull << 32), MCZI_E_INVALID, "too many super-k-mer records for one batch (limit 2^32)");
+            require(p.stage_cap < 2 * h[12] + chunk_pad + 16, MCZI_E_DEVICE, "super-k-mer staging exceeded its bound");
+            p.stage_cap = 2 * h[12] + chunk_pad + 16;
+            attempt--;
+            continue;
+        }
+        require((h[4] & 16ull) == 0, MCZI_E_DEVICE, "partitioning moved a record out of its bucket");
+        require((h[4] & 8ull) == 0, MCZI_E_DEVICE, "k-mer table insertion did not settle");
+        p.n_staged = h[11];
+        p.n_records = h[12];
+        if ((h[4] & 1ull) || h[1] > p.out_cap) {
+            require(attempt < 4, MCZI_E_NOMEM, "k-mer spill table overflowed repeatedly");
+            target /= 4;
+            continue;
+        }
+        return;
+    }
+}
+
+__global__ void k_gt_compact(const unsigned long long *keys, const uint32_t *vals, const unsigned long long *n_ptr,
+                             uint32_t x, unsigned long long *kout, uint32_t *cout, unsigned long long *cursor) {
+    const uint64_t n = *n_ptr;
+    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < n; i0 += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = i0 + threadIdx.x;
+        const bool keep = i < n && vals[i] > x;
+        const unsigned long long pos = wave_append(keep, cursor);
+        if (keep) {
+            kout[pos] = keys[i];
+            cout[pos] = vals[i];
+        }
+    }
+}
+
+// fof with more than 32 datasets (count_dataset_presence_counts_u64,
+// src/lib.rs:637-703, keeps u32 presence counts for any number of
+// datasets): each group of 32 datasets is counted as its own pass
+// (presence = popcount of the group's 32-bit masks, every k-mer present in
+// the group kept); the per-group (k-mer, presence) pairs accumulate in HBM,
+// are sorted by k-mer and summed per k-mer, and the sums > x are emitted in
+// encoded order.
+static void count_dataset_groups(Batch &b, McPlan &p, const mczi_counter_config &cfg, int launch, Timer &tm,
+                                 unsigned long long *h, uint64_t &n_out, uint64_t &n_distinct) {
+    Device &d = Device::get();
+    hipStream_t s = d.stream();
+    mczi_counter_config gcfg = cfg;
+    gcfg.threshold = 0;  // keep every k-mer present in the group
+    const uint32_t groups = (b.n_datasets + 31) / 32;
+    DevBuf acc_k, acc_c;
+    uint64_t used = 0, cap = 0;
+    unsigned long long *cur = p.ctr.as<unsigned long long>() + 14;
+    for (uint32_t gi = 0; gi < groups; gi++) {
+        p.ds_lo = gi * 32;
+        count_pass(b, p, gcfg, MCZI_MODE_DATASETS, launch, tm, false, h);
+        const uint64_t n_slots = h[1], n_real = h[1] - h[6];
+        if (used + n_real > cap) {  // grow, keeping what is there
+            const uint64_t nc = std::max<uint64_t>(2 * cap, used + n_real) + 1024;
+            DevBuf nk(nc * 8), nv(nc * 4);
+            if (used) {
+                MCZI_HIP(hipMemcpyAsync(nk.as<void>(), acc_k.as<void>(), used * 8, hipMemcpyDeviceToDevice, s));
+                MCZI_HIP(hipMemcpyAsync(nv.as<void>(), acc_c.as<void>(), used * 4, hipMemcpyDeviceToDevice, s));
+            }
+            acc_k = std::move(nk);
+            acc_c = std::move(nv);
+            cap = nc;
+        }
+        MCZI_HIP(hipMemcpyAsync(cur, &used, 8, hipMemcpyHostToDevice, s));
+        if (n_slots)
+            hipLaunchKernelGGL(k_compact_out, dim3(grid_for(n_slots, 256, 8192)), dim3(256), 0, s,
+                               p.out_kmer.as<unsigned long long>(), nullptr, p.out_count.as<uint32_t>(),
+                               p.out_part.as<uint32_t>(), n_slots, acc_k.as<unsigned long long>(), nullptr,
+                               acc_c.as<uint32_t>(), cur);
+        used += n_real;
+        MCZI_HIP(hipStreamSynchronize(s));
+    }
+    p.ds_lo = 0;
+    p.graph_sig = 0;  // the captured step belongs to the last group
+    // sort the pairs by k-mer, sum per k-mer, keep sums > x
+    const unsigned end_bit = (unsigned)std::min<uint32_t>(64, 2 * cfg.k);
+    DevBuf sk(std::max<uint64_t>(used, 1) * 8), sv(std::max<uint64_t>(used, 1) * 4);
+    DevBuf uk(std::max<uint64_t>(used, 1) * 8), uv(std::max<uint64_t>(used, 1) * 4), nu(16);
+    size_t t1 = 0, t2 = 0;
+    MCZI_HIP(rocprim::radix_sort_pairs(nullptr, t1, acc_k.as<unsigned long long>(), sk.as<unsigned long long>(),
+                                       acc_c.as<uint32_t>(), sv.as<uint32_t>(), (size_t)used, 0, end_bit, s));
+    MCZI_HIP(rocprim::reduce_by_key(nullptr, t2, sk.as<unsigned long long>(), sv.as<uint32_t>(), (size_t)used,
+                                    uk.as<unsigned long long>(), uv.as<uint32_t>(), nu.as<unsigned long long>(),
+                                    rocprim::plus<uint32_t>(), rocprim::equal_to<unsigned long long>(), s));
+    DevBuf tmp(std::max(t1, t2) + 16);
+    MCZI_HIP(rocprim::radix_sort_pairs(tmp.as<void>(), t1, acc_k.as<unsigned long long>(), sk.as<unsigned long long>(),
+                                       acc_c.as<uint32_t>(), sv.as<uint32_t>(), (size_t)used, 0, end_bit, s));
+    MCZI_HIP(rocprim::reduce_by_key(tmp.as<void>(), t2, sk.as<unsigned long long>(), sv.as<uint32_t>(), (size_t)used,
+                                    uk.as<unsigned long long>(), uv.as<uint32_t>(), nu.as<unsigned long long>(),
+                                    rocprim::plus<uint32_t>(), rocprim::equal_to<unsigned long long>(), s));
+    b.out_kmer.ensure(std::max<uint64_t>(used, 1) * 8);
+    b.out_count.ensure(std::max<uint64_t>(used, 1) * 4);
+    fill_u64(cur, 1, 0, s);
+    hipLaunchKernelGGL(k_gt_compact, dim3(grid_for(std::max<uint64_t>(used, 1), 256, 8192)), dim3(256), 0, s,
+                       uk.as<unsigned long long>(), uv.as<uint32_t>(), nu.as<unsigned long long>(),
+                       (uint32_t)cfg.threshold, b.out_kmer.as<unsigned long long>(), b.out_count.as<uint32_t>(), cur);
+    unsigned long long r[2] = {0, 0};
+    MCZI_HIP(hipMemcpyAsync(&r[0], cur, 8, hipMemcpyDeviceToHost, s));
+    MCZI_HIP(hipMemcpyAsync(&r[1], nu.as<void>(), 8, hipMemcpyDeviceToHost, s));
+    MCZI_HIP(hipStreamSynchronize(s));
+    n_out = r[0];
+    n_distinct = r[1];
 }
 
 void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch, int order,
@@ -1718,191 +2369,69 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
     require(k % 2 == 1, MCZI_E_INVALID,
             "k must be odd: the canonical window length l = k must be odd (simd-minimizers)");
     require(cfg.partition_count > 0, MCZI_E_INVALID, "partition count must be greater than 0");
-    if (mode == MCZI_MODE_DATASETS) {
-        require(!wide, MCZI_E_INVALID, "FOFN dataset-presence mode currently requires k <= 32");
-        require(b.n_datasets <= 32, MCZI_E_INVALID,
-                "the device fof path supports at most 32 datasets per batch");
-    }
+    require(cfg.partition_count <= MAX_INTERNAL_PARTITIONS, MCZI_E_INVALID,
+            "partition count above 4194304 is not supported on the device");
+    if (mode == MCZI_MODE_DATASETS) require(!wide, MCZI_E_INVALID, "FOFN dataset-presence mode currently requires k <= 32");
     require(!wide || order != MCZI_ORDER_KFF_U8, MCZI_E_INVALID, "KFF u8 record order needs k <= 32");
     McPlan &p = *b.plan;
     p.n_windows = windows_for(b, k);
     p.k = k;
     const bool use_graph = launch == MCZI_LAUNCH_BATCHED && getenv("MCZI_NO_GRAPH") == nullptr;
-
-    double target = p.target_override > 0 ? p.target_override : tuned_target();
-    for (int attempt = 0;; attempt++) {
-        p.p_int = choose_partitions(p.n_windows, cfg.partition_count, target);
-        p.base.ensure(((size_t)p.p_int + 1) * 8);
-        p.cursor.ensure(((size_t)p.p_int + 1) * 8);
-        p.ctr.ensure(16 * 8);
-        const uint64_t ovf = next_pow2(std::max<uint64_t>(p.n_windows / 16, 1 << 20)) << attempt;
-        p.ovf_slots = ovf;
-        p.ovf_keys.ensure(ovf * 8);
-        p.ovf_vals.ensure(ovf * 4);
-        p.ovf_part.ensure(ovf * 4);
-        if (wide) p.ovf_hi.ensure(ovf * 8);
-        p.out_cap = p.n_windows / (cfg.threshold + 1) + 16;
-        p.out_kmer.ensure(p.out_cap * 8);
-        if (wide) p.out_hi.ensure(p.out_cap * 8);
-        p.out_count.ensure(p.out_cap * 4);
-        p.out_part.ensure(p.out_cap * 4);
-
-        // Staging capacity: the expected record count of a random minimizer
-        // scheme (density 2 / (w + 1) per window, plus record splits, one
-        // start per fragment and one cut per tile) with a 25% margin, never
-        // more than one record per window, plus the padding of each scan
-        // block's last chunk. The scan counts every slot it reserves; if the
-        // total exceeds the capacity, the batch is scanned again with room
-        // for twice the records (the padding of an abandoned chunk is less
-        // than its records) plus a full chunk per block.
-        const uint64_t cfg_sig = mix_sig(mix_sig(k, cfg.minimizer), b.n_bases);
-        const uint32_t wrec = (uint32_t)((wide ? REC_MAX_BASES_W : REC_MAX_BASES) - k + 1);
-        const uint64_t chunk_pad = launch_scan(p, ScanArgs{}, launch, s, false) * STAGE_CHUNK;
-        if (p.rec_sig != cfg_sig) {
-            const double w = (double)(k - cfg.minimizer + 1);
-            const double expect = (double)p.n_windows * (2.0 / (w + 1.0) + 1.0 / wrec) + (double)b.n_frag +
-                                  (double)p.n_tiles;
-            p.stage_cap = std::min<uint64_t>((uint64_t)(1.25 * expect) + 4096, p.n_windows) + chunk_pad + 16;
-            if (const char *e = getenv("MCZI_STAGE_CAPACITY"))  // initial capacity in slots
-                p.stage_cap = std::max<uint64_t>(strtoull(e, nullptr, 10), 1);
-            p.rec_sig = cfg_sig;
-        }
-        const bool want_times = stats != nullptr && !use_graph;
-        Timer tm(stats != nullptr);
+    const bool want_times = stats != nullptr && !use_graph;
+    Timer tm(stats != nullptr);
+    unsigned long long h[16];
+    mczi_run_stats st{};
+    b.k = k;
+    if (mode == MCZI_MODE_DATASETS && b.n_datasets > 32) {
         auto t0 = std::chrono::steady_clock::now();
-        tm.mark(s);
-        unsigned long long h[16];
-        for (;;) {
-            if (wide) p.stage_rec_w.ensure(p.stage_cap * sizeof(Rec32));
-            else p.stage_rec.ensure(p.stage_cap * 16);
-            p.stage_q.ensure(p.stage_cap * 4);
-            enqueue_stage_a(b, p, cfg, mode, launch, s, want_times ? &tm : nullptr);
-            MCZI_HIP(hipMemcpyAsync(h, p.ctr.as<void>(), sizeof h, hipMemcpyDeviceToHost, s));
-            MCZI_HIP(hipStreamSynchronize(s));
-            MCZI_HIP(hipGetLastError());
-            if (h[11] <= p.stage_cap) break;
-            require(h[12] < (1ull << 32), MCZI_E_INVALID, "too many super-k-mer records for one batch (limit 2^32)");
-            require(p.stage_cap < 2 * h[12] + chunk_pad + 16, MCZI_E_DEVICE,
-                    "super-k-mer staging exceeded its bound");
-            p.stage_cap = 2 * h[12] + chunk_pad + 16;
-            tm.n = 1;
-        }
-        require(h[11] < (1ull << 32), MCZI_E_INVALID, "too many super-k-mer records for one batch (limit 2^32)");
-        p.n_staged = h[11];
-        p.n_records = h[12];
-        p.rec_cap = std::max<uint64_t>(p.rec_cap, p.n_staged + 16);
-        if (wide) p.records_w.ensure(p.rec_cap * sizeof(Rec32));
-        else p.records.ensure(p.rec_cap * 16);
-        p.pqs.ensure(p.rec_cap * 4);
-        {
-            size_t tmp = 0;
-            if (wide)
-                MCZI_HIP(rocprim::radix_sort_pairs(nullptr, tmp, p.stage_q.as<uint32_t>(), p.pqs.as<uint32_t>(),
-                                                   p.stage_rec_w.as<Rec32>(), p.records_w.as<Rec32>(),
-                                                   (size_t)std::max<uint64_t>(p.n_staged, 1), 0,
-                                                   bits_for(p.p_int + 1), s));
-            else
-                MCZI_HIP(rocprim::radix_sort_pairs(nullptr, tmp, p.stage_q.as<uint32_t>(), p.pqs.as<uint32_t>(),
-                                                   p.stage_rec.as<ulonglong2>(), p.records.as<ulonglong2>(),
-                                                   (size_t)std::max<uint64_t>(p.n_staged, 1), 0,
-                                                   bits_for(p.p_int + 1), s));
-            p.part_tmp.ensure(tmp + 256);
-            p.part_tmp_bytes = tmp;
-        }
-
-        // Stage B (partitioning, counting, emit) as one graph per batch
-        // layout: its sizes (records, partitions, spill slots) and buffers
-        // are fixed once the records are staged.
-        if (use_graph) {
-            uint64_t sig = cfg_sig;
-            sig = mix_sig(mix_sig(mix_sig(sig, cfg.threshold), (uint64_t)mode), p.p_int);
-            for (const DevBuf *db : {&b.words, &b.off, &b.dataset, &p.base, &p.cursor, &p.records, &p.ctr,
-                                     &p.ovf_keys, &p.ovf_vals, &p.ovf_part, &p.out_kmer, &p.out_count,
-                                     &p.out_part, &p.tile_start, &p.stage_rec, &p.stage_q, &p.pqs,
-                                     &p.part_tmp, &p.records_w, &p.stage_rec_w, &p.ovf_hi, &p.out_hi})
-                sig = mix_sig(sig, (uint64_t)(uintptr_t)db->as<void>());
-            sig = mix_sig(mix_sig(mix_sig(sig, p.ovf_slots), p.stage_cap), p.n_staged);
-            if (!p.graph || p.graph_sig != sig) {
-                if (p.graph) {
-                    MCZI_HIP(hipGraphExecDestroy(p.graph));
-                    p.graph = nullptr;
-                }
-                hipGraph_t g;
-                MCZI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                enqueue_stage_b(b, p, cfg, mode, launch, s, nullptr);
-                MCZI_HIP(hipStreamEndCapture(s, &g));
-                MCZI_HIP(hipGraphInstantiate(&p.graph, g, nullptr, nullptr, 0));
-                MCZI_HIP(hipGraphDestroy(g));
-                p.graph_sig = sig;
-            }
-            MCZI_HIP(hipGraphLaunch(p.graph, s));
-        } else {
-            enqueue_stage_b(b, p, cfg, mode, launch, s, want_times ? &tm : nullptr);
-        }
-        const int t_dev_end = tm.n;
-        tm.mark(s);
-        MCZI_HIP(hipMemcpyAsync(h, p.ctr.as<void>(), sizeof h, hipMemcpyDeviceToHost, s));
-        MCZI_HIP(hipStreamSynchronize(s));
-        MCZI_HIP(hipGetLastError());
-        require((h[4] & 2ull) == 0 && h[11] == p.n_staged && h[12] == p.n_records, MCZI_E_DEVICE,
-                "super-k-mer staging differs from its measured size");
-        if (h[4] != 0 || h[1] > p.out_cap) {
-            require(attempt < 4, MCZI_E_NOMEM, "k-mer spill table overflowed repeatedly");
-            target /= 4;
-            continue;
-        }
-        const uint64_t n_out = h[1];
-        const int tsort0 = tm.n;
-        tm.mark(s);
-        sort_output(b, p, n_out, order, cfg.partition_count, (int)k, s);
-        if (order == MCZI_ORDER_NONE || n_out == 1) {
-            b.out_kmer.ensure(std::max<uint64_t>(n_out, 1) * 8);
-            b.out_count.ensure(std::max<uint64_t>(n_out, 1) * 4);
-            if (n_out) {
-                MCZI_HIP(hipMemcpyAsync(b.out_kmer.as<void>(), p.out_kmer.as<void>(), n_out * 8,
-                                        hipMemcpyDeviceToDevice, s));
-                MCZI_HIP(hipMemcpyAsync(b.out_count.as<void>(), p.out_count.as<void>(), n_out * 4,
-                                        hipMemcpyDeviceToDevice, s));
-            }
-            if (wide) {
-                b.out_hi.ensure(std::max<uint64_t>(n_out, 1) * 8);
-                if (n_out)
-                    MCZI_HIP(hipMemcpyAsync(b.out_hi.as<void>(), p.out_hi.as<void>(), n_out * 8,
-                                            hipMemcpyDeviceToDevice, s));
-            }
-        }
-        tm.mark(s);
-        MCZI_HIP(hipStreamSynchronize(s));
-        auto t1 = std::chrono::steady_clock::now();
+        uint64_t n_out = 0, n_distinct = 0;
+        count_dataset_groups(b, p, cfg, launch, tm, h, n_out, n_distinct);
         b.n_out = n_out;
-        b.k = k;
-        mczi_run_stats st{};
         st.n_bases = b.n_bases;
         st.n_windows = p.n_windows;
-        st.n_pieces = h[8];
-        st.n_records = p.n_records;
-        st.tile_windows = TILE;
-        st.n_distinct = h[2] + h[3];
-        st.n_overflow = h[3];
+        st.n_distinct = n_distinct;
         st.n_out = n_out;
-        st.partition_bytes = p.n_records * 16;
         st.internal_partitions = p.p_int;
         st.lds_slots = CNT_SLOTS;
-        if (want_times && t_dev_end >= 7) {
-            // marks: 0 start, 1 clears, 2 scan + staging, 3 clears,
-            // 4 partition sort + bounds, 5 count, 6 spill emit
-            st.ms_scan1 = tm.ms(1, 2);
-            st.ms_scan2 = tm.ms(3, 4);
-            st.ms_count = tm.ms(4, 6);
-        }
-        st.ms_total = tm.ms(0, t_dev_end);
-        st.ms_sort = tm.ms(tsort0, tsort0 + 1);
-        (void)t0;
-        (void)t1;
+        st.tile_windows = TILE;
+        st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         b.last = st;
         if (stats) *stats = st;
         return;
     }
+    p.ds_lo = 0;
+    count_pass(b, p, cfg, mode, launch, tm, want_times, h);
+    const int t_dev_end = tm.n - 1;
+    const uint64_t n_slots = h[1], n_real = h[1] - h[6];
+    const int tsort0 = tm.n;
+    tm.mark(s);
+    sort_output(b, p, n_slots, n_real, order, cfg.partition_count, (int)k, s);
+    tm.mark(s);
+    MCZI_HIP(hipStreamSynchronize(s));
+    MCZI_HIP(hipGetLastError());
+    b.n_out = n_real;
+    st.n_bases = b.n_bases;
+    st.n_windows = p.n_windows;
+    st.n_pieces = h[8];
+    st.n_records = p.n_records;
+    st.tile_windows = TILE;
+    st.n_distinct = h[2] + h[3];
+    st.n_overflow = h[3];
+    st.n_out = n_real;
+    st.partition_bytes = p.n_records * (wide ? sizeof(Rec32) : 16);
+    st.internal_partitions = p.p_int;
+    st.lds_slots = CNT_SLOTS;
+    if (want_times && t_dev_end >= 7) {
+        // marks: 0 start, 1 clears, 2 scan + staging, 3 clears,
+        // 4 partitioning, 5 count, 6 spill emit
+        st.ms_scan1 = tm.ms(1, 2);
+        st.ms_scan2 = tm.ms(3, 4);
+        st.ms_count = tm.ms(4, 6);
+    }
+    st.ms_total = tm.ms(0, t_dev_end);
+    st.ms_sort = tm.ms(tsort0, tsort0 + 1);
+    b.last = st;
+    if (stats) *stats = st;
 }
 
 // Autotuner: sweep the expected k-mer instances per internal partition (the
@@ -1953,12 +2482,27 @@ void batch_download_records(Batch &b, std::vector<uint64_t> &rec, std::vector<ui
     const bool wide = p.k > 32;
     rec_words = wide ? 4 : 2;
     p_int = p.p_int;
-    const uint64_t n = p.n_records;  // the padding slots sort after every record
+    const uint64_t n = p.n_records;  // partition order, no padding
     rec.resize(n * rec_words);
     part.resize(n);
     if (!n) return;
-    d.download(rec.data(), wide ? p.records_w.as<void>() : p.records.as<void>(), n * rec_words * 8, d.stream());
-    d.download(part.data(), p.pqs.as<void>(), n * 4, d.stream());
+    d.download(rec.data(), wide ? p.stage_rec_w.as<void>() : p.stage_rec.as<void>(), n * rec_words * 8, d.stream());
+    // the device layout to the one the interop writer reads: bases from bit 0,
+    // base count at bits 48..55 of the last word; partition beside
+    parallel_ranges(n, 1u << 16, host_workers(n), [&](size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; i++) {
+            uint64_t *r = rec.data() + i * rec_words;
+            if (wide) {
+                const Rec32 x{{r[0], r[1], r[2], r[3]}};
+                part[i] = rec_q(x);
+                r[3] = (r[3] & 0xffffull) | ((uint64_t)rec_len(x) << 48);
+            } else {
+                const ulonglong2 x = make_ulonglong2(r[0], r[1]);
+                part[i] = rec_q(x);
+                r[1] = (r[1] & 0xffffffull) | ((uint64_t)rec_len(x) << 48);
+            }
+        }
+    });
 }
 
 void batch_download_keys(Batch &b, uint64_t *lo, uint64_t *hi) {

@@ -148,7 +148,7 @@ void synth_generate(const mczi_synth_spec &spec, mczi_synth_reads &out) {
     // generate in parallel, each read from its own seed
     const size_t nr = reads.size();
     std::vector<std::vector<uint8_t>> seqs(nr);
-    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    unsigned nt = host_workers(size_t(1) << 20);
     std::vector<std::thread> th;
     for (unsigned t = 0; t < nt; t++) {
         th.emplace_back([&, t]() {
@@ -204,7 +204,7 @@ void synth_reads_from(const uint32_t *const *gw, const uint64_t *gl, uint32_t ng
     out.dataset = (uint32_t *)malloc(std::max<uint64_t>(n_reads, 1) * 4);
     require(out.words && out.frag_off && out.dataset, MCZI_E_NOMEM, "synthetic reads: out of memory");
     for (uint64_t i = 0; i <= n_reads; i++) out.frag_off[i] = i * read_len;
-    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    unsigned nt = host_workers(size_t(1) << 20);
     std::vector<std::thread> th;
     for (unsigned t = 0; t < nt; t++) {
         th.emplace_back([&, t]() {

@@ -1151,7 +1151,41 @@ ZiScan mczi::zi_subtract_files(const ZiIndex &idx, const char *const *qp, size_t
 
 namespace {
 
-std::vector<mczi_counted_kmer> zi_run(const char *index_path, const char *const *qp, size_t n, uint32_t k) {
+// Device bytes one query pass holds per fragment of L bases: its packed
+// words and offset, and the absent-k-mer segment slots and sort buffers of
+// its windows.
+static uint64_t zi_pass_bytes(uint64_t L, uint32_t k) {
+    const uint64_t per_window = k > 32 ? 40 : 24;
+    return L / 4 + 16 + (L >= k ? (L - k + 1) * per_window : 0);
+}
+
+// Query fragments [f0, f1) of q as one lookup pass; returns the sorted
+// unique absent k-mers.
+static std::vector<mczi_counted_kmer> zi_pass(const ZiIndex &idx, const HostReads &q, size_t f0, size_t f1,
+                                              hipStream_t s) {
+    // the pass starts at the word holding its first base; the bases of that
+    // word before it form a leading fragment of no request (shorter than k)
+    const uint64_t w0 = q.off[f0] / 16, lead = q.off[f0] - w0 * 16;
+    ZiBatchInput in;
+    in.off.assign(1, 0);
+    if (lead) in.off.push_back(lead);
+    for (size_t i = f0 + 1; i <= f1; i++) in.off.push_back(q.off[i] - w0 * 16);
+    in.frag_req.assign(in.off.size() - 1, 0);
+    if (lead) in.frag_req[0] = NO_REQ;
+    in.n_words = (in.off.back() + 15) / 16;
+    in.spans.push_back(ZiWordSpan{q.words.data() + w0, in.n_words, 0});
+    auto w = zi_make_work();
+    zi_enqueue(*w, idx, std::move(in), 1, s);
+    ZiQueryResult res;
+    zi_collect(*w, idx, 1, s, res);
+    return res.absent[0].counted();
+}
+
+// The query fragments are looked up in passes of at most ram_bytes of
+// device memory each (all at once when ram_bytes is 0); the sorted absent
+// sets of the passes are merged.
+std::vector<mczi_counted_kmer> zi_run(const char *index_path, const char *const *qp, size_t n, uint32_t k,
+                                      uint64_t ram_bytes = 0) {
     Device &d = Device::get();
     ZiIndex idx;
     HostReads ir;
@@ -1160,16 +1194,38 @@ std::vector<mczi_counted_kmer> zi_run(const char *index_path, const char *const 
     HostReads q;
     for (size_t i = 0; i < n; i++) parse_fastx_file(qp[i], k, 0, q);
     q.finish();
-    ZiBatchInput in;
-    in.off = q.off;
-    in.frag_req.assign(q.n_frag(), 0);
-    in.n_words = (q.n_bases() + 15) / 16;
-    in.spans.push_back(ZiWordSpan{q.words.data(), in.n_words, 0});
-    auto w = zi_make_work();
-    zi_enqueue(*w, idx, std::move(in), 1, d.stream());
-    ZiQueryResult res;
-    zi_collect(*w, idx, 1, d.stream(), res);
-    return res.absent[0].counted();
+    const size_t nf = q.n_frag();
+    if (const char *e = getenv("MCZI_ZI_PASS_BYTES"))  // pass size override (tests)
+        ram_bytes = strtoull(e, nullptr, 10);
+    std::vector<mczi_counted_kmer> acc, merged;
+    size_t f0 = 0;
+    do {
+        size_t f1 = f0;
+        uint64_t est = 0;
+        while (f1 < nf) {
+            const uint64_t e = zi_pass_bytes(q.off[f1 + 1] - q.off[f1], k);
+            if (ram_bytes && f1 > f0 && est + e > ram_bytes) break;
+            est += e;
+            f1++;
+        }
+        std::vector<mczi_counted_kmer> v = zi_pass(idx, q, f0, f1, d.stream());
+        if (f0 == 0) {
+            acc = std::move(v);
+        } else {
+            auto less = [](const mczi_counted_kmer &a, const mczi_counted_kmer &b) {
+                return a.hi != b.hi ? a.hi < b.hi : a.lo < b.lo;
+            };
+            auto same = [](const mczi_counted_kmer &a, const mczi_counted_kmer &b) {
+                return a.hi == b.hi && a.lo == b.lo;
+            };
+            merged.resize(acc.size() + v.size());
+            merged.erase(std::merge(acc.begin(), acc.end(), v.begin(), v.end(), merged.begin(), less), merged.end());
+            merged.erase(std::unique(merged.begin(), merged.end(), same), merged.end());
+            acc.swap(merged);
+        }
+        f0 = f1;
+    } while (f0 < nf);
+    return acc;
 }
 }  // namespace
 
@@ -1180,16 +1236,15 @@ extern "C" {
 
 int mczi_zi_subtract(const char *index_path, const char *const *query_paths, size_t n, uint32_t k, uint32_t m,
                      const char *out_path, uint32_t threads, uint32_t ram_limit_gib) {
-    (void)threads;
-    (void)ram_limit_gib;
     int rc = zguard("ZI subtraction failed", [&] {
         require(index_path && out_path && (n == 0 || query_paths), MCZI_E_INVALID, "null argument");
         if (m == 0) m = default_m(k);
         check_k(k, m);
         require(n > 0, MCZI_E_INVALID, "at least one query FASTA is required");
+        if (threads) g_host_threads.store(threads, std::memory_order_relaxed);  // src/zi.rs -t
         std::lock_guard<std::mutex> lk(Device::get().call_mutex());
         auto t0 = std::chrono::steady_clock::now();
-        auto v = zi_run(index_path, query_paths, n, k);
+        auto v = zi_run(index_path, query_paths, n, k, (uint64_t)ram_limit_gib << 30);
         auto t1 = std::chrono::steady_clock::now();
         write_simplitig_fasta(out_path, v.data(), v.size(), k, 0, nullptr);
         auto t2 = std::chrono::steady_clock::now();
@@ -1310,7 +1365,15 @@ int mczi_zi_server_submit_packed(mczi_zi_server *s, const uint32_t *words, const
         const uint64_t nb = frag_off[n_frag];
         r->pin_words(words, (nb + 15) / 16);
         h.cur_ = nb;
-        h.off.assign(frag_off, frag_off + n_frag + 1);
+        // empty fragments are dropped: the lookup's tile walk assumes at most
+        // one fragment start per base position
+        const std::vector<uint64_t> keep = nonempty_fragments(frag_off, n_frag);
+        if (!keep.empty() || (n_frag && nb == frag_off[0])) {
+            h.off.assign(1, frag_off[0]);
+            for (uint64_t i : keep) h.off.push_back(frag_off[i + 1]);
+        } else {
+            h.off.assign(frag_off, frag_off + n_frag + 1);
+        }
         r->make_fasta = make_fasta != 0;
         std::lock_guard<std::mutex> lk(s->mu);
         require(s->fatal.empty(), MCZI_E_DEVICE, s->fatal);

@@ -63,3 +63,35 @@ def canonical_kmers_np(codes, k):
     rh, rl = val(rev, 0, split), val(rev, split, k)
     take_r = (rh < fh) | ((rh == fh) & (rl < fl))
     return np.where(take_r, rh, fh), np.where(take_r, rl, fl)
+
+
+def pack_words(codes) -> np.ndarray:
+    """Packed-seq codes (A0 C1 T2 G3) as 16 bases per little-endian u32 word,
+    with 32 zero words of slack."""
+    codes = np.asarray(codes, dtype=np.uint32)
+    n = codes.size
+    w = np.zeros((n + 15) // 16 + 32, dtype=np.uint32)
+    for j in range(16):
+        part = codes[j::16]
+        w[: part.size] |= part << np.uint32(2 * j)
+    return w
+
+
+def with_empty_fragments(rng, seqs, runs=6, run_len=40):
+    """Fragment offsets of seqs (concatenated) with runs of empty fragments
+    inserted at random fragment boundaries; returns (codes, off, src) where
+    src[i] is the index in seqs of fragment i (-1 for an empty one)."""
+    codes = np.concatenate([O.codes_of(s) for s in seqs]).astype(np.uint8)
+    lens = [len(s) for s in seqs]
+    at = set(int(x) for x in rng.integers(0, len(seqs) + 1, size=runs))
+    off, src, pos = [0], [], 0
+    for i in range(len(seqs) + 1):
+        if i in at:
+            for _ in range(run_len):
+                off.append(pos)
+                src.append(-1)
+        if i < len(seqs):
+            pos += lens[i]
+            off.append(pos)
+            src.append(i)
+    return codes, np.array(off, dtype=np.uint64), np.array(src, dtype=np.int64)

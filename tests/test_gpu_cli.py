@@ -46,7 +46,7 @@ def test_mc_command_line_fasta_and_kff(tmp_path):
     write_fasta(fa, _reads(rng, _genome(rng, 15000), 2500))
     codes, off = O.parse_fastx(str(fa), 31)
     out = tmp_path / "o.fa"
-    r = _run([MC, "-i", fa, "-k", 31, "-m", 21, "-x", 1, "-o", out])
+    r = _run([MC, "-i", fa, "-k", 31, "-m", 21, "-x", 1, "-t", 3, "-o", out])
     assert r.returncode == 0, r.stderr
     assert "MC phase" in r.stderr
     O.write_simplitigs(str(tmp_path / "w.fa"), O.count_bruteforce(codes, off, 31, 1), 31)

@@ -11,6 +11,7 @@ import pytest
 import mczi_amd as M
 import pyoracle as O
 from helpers import as_pairs, random_seqs, revcomp, same_counted, write_fasta
+from kff_reader import KffFile, kmers_u64
 
 pytestmark = pytest.mark.gpu
 
@@ -378,8 +379,18 @@ def test_c1_full_config_kff_bytes(tmp_path):
     out = tmp_path / "c1.kff"
     M.count_inputs_to_kff_path([fa], M.CounterConfig(31, 21, 1, 64), out)
     want, nrec = O.kff_u8_kmc(r.codes(), r.frag_off, 31, 21, 1, 64)
-    assert out.read_bytes() == want
+    data = out.read_bytes()
+    assert data == want
     assert nrec == 1053631
+    # read back by the independent KFF reader: the closed form
+    # {K : c(K) > 1} with count min(c(K), 2), each k-mer once
+    f = KffFile(data, decode_records=False)
+    assert f.variables["k"] == 31 and f.variables["data_size"] == 1 and f.canonical == 1
+    keys, vals = kmers_u64(f, data)
+    bf = O.count_bruteforce(r.codes(), r.frag_off, 31, 1)
+    order = np.argsort(keys)
+    assert np.array_equal(keys[order], bf["lo"].astype(np.uint64))
+    assert np.array_equal(vals[order], np.minimum(bf["count"].astype(np.uint64), 2))
 
 
 def _spot_check(words, off, k, x, got, dataset=None, n_emitted=3000, n_windows=3000, seed=0):

@@ -250,3 +250,27 @@ def test_zi_full_size_index_spot_check():
     c, _ = O.spot_counts(gwords, goff, 31, wk)
     at = np.minimum(np.searchsorted(absent, wk), absent.size - 1)
     assert np.array_equal(absent[at] == wk, c == 0)
+
+
+def test_zi_query_passes_equal_one_pass(tmp_path, monkeypatch):
+    """ram_limit_gib splits the query fragments into several lookup passes
+    (forced small here through MCZI_ZI_PASS_BYTES); the merged absent set and
+    the FASTA bytes equal one pass and the oracle."""
+    rng = np.random.default_rng(12)
+    genome = _genome(rng, 30000)
+    other = _genome(rng, 30000)
+    write_fasta(tmp_path / "idx.fa", [genome])
+    write_fasta(tmp_path / "q.fa", _queries(rng, genome, other, 900))
+    ic, io = O.parse_fastx(str(tmp_path / "idx.fa"), 1)
+    qc, qo = O.parse_fastx(str(tmp_path / "q.fa"), 31)
+    want = O.zi_absent(ic, io, qc, qo, 31)
+    outs = []
+    for pass_bytes in (None, "200000", "5000"):
+        if pass_bytes:
+            monkeypatch.setenv("MCZI_ZI_PASS_BYTES", pass_bytes)
+        out = tmp_path / f"o{pass_bytes}.fa"
+        M._check(M._lib.mczi_zi_subtract(str(tmp_path / "idx.fa").encode(), M._paths([tmp_path / "q.fa"])[0], 1, 31,
+                                         0, str(out).encode(), 4, 8))
+        outs.append(out.read_bytes())
+        assert same_counted(zi_absent(tmp_path / "idx.fa", [tmp_path / "q.fa"], 31), want)
+    assert outs[0] == outs[1] == outs[2]
