@@ -689,17 +689,16 @@ __device__ __forceinline__ uint32_t pt_digit_scan(const uint32_t *s_cnt, uint32_
     return tot;
 }
 
-// Moves the records of slots [tb, te) (at most one tile) to dst: digit
+// Moves the records of one tile (rec[], slots [tb, te)) to dst: digit
 // (q >> shift) & dmask, the next free slot of each digit in s_cur (advanced
 // here). Padding records are skipped. s_cnt is zero on entry and on return.
 template <class R>
-__device__ __forceinline__ void pt_move_tile(const R *__restrict__ src, R *__restrict__ dst, uint64_t tb,
+__device__ __forceinline__ void pt_move_tile(const R (&rec)[PartRec<R>::ITEMS], R *__restrict__ dst, uint64_t tb,
                                              uint64_t te, int shift, uint32_t dmask, uint32_t nd, uint64_t cap,
                                              R *s_stage, uint32_t *s_cnt, uint32_t *s_start, uint32_t *s_cur,
                                              uint32_t *s_wsum, unsigned long long *err) {
     constexpr int IT = PartRec<R>::ITEMS;
     const int tid = threadIdx.x;
-    R rec[IT];
     int dig[IT];
     uint32_t rank[IT];
 #pragma unroll
@@ -707,7 +706,6 @@ __device__ __forceinline__ void pt_move_tile(const R *__restrict__ src, R *__res
         const uint64_t i = tb + (uint64_t)it * PT_THREADS + tid;
         dig[it] = -1;
         if (i < te) {
-            rec[it] = src[i];
             const uint32_t q = rec_q(rec[it]);
             if (q != PartRec<R>::PAD) dig[it] = (int)((q >> shift) & dmask);
         }
@@ -736,27 +734,84 @@ __device__ __forceinline__ void pt_move_tile(const R *__restrict__ src, R *__res
     __syncthreads();
 }
 
+// Moves slots [b, e) tile by tile; the next tile's records are loaded into
+// registers before the current tile is ranked and stored, so the loads are in
+// flight during the LDS phases (block barriers wait on LDS only).
+template <class R>
+__device__ __forceinline__ void pt_move_range(const R *__restrict__ src, R *__restrict__ dst, uint64_t b, uint64_t e,
+                                              int shift, uint32_t dmask, uint32_t nd, uint64_t cap, R *s_stage,
+                                              uint32_t *s_cnt, uint32_t *s_start, uint32_t *s_cur, uint32_t *s_wsum,
+                                              unsigned long long *err) {
+    constexpr int IT = PartRec<R>::ITEMS;
+    constexpr uint64_t TN = (uint64_t)PT_THREADS * IT;
+    R cur[IT], nxt[IT];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const uint64_t i = b + (uint64_t)it * PT_THREADS + threadIdx.x;
+        if (i < e) cur[it] = src[i];
+    }
+    for (uint64_t t = b; t < e; t += TN) {
+        const uint64_t tn = t + TN;
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const uint64_t i = tn + (uint64_t)it * PT_THREADS + threadIdx.x;
+            if (i < e) nxt[it] = src[i];
+        }
+        pt_move_tile<R>(cur, dst, t, min(e, tn), shift, dmask, nd, cap, s_stage, s_cnt, s_start, s_cur, s_wsum, err);
+#pragma unroll
+        for (int it = 0; it < IT; it++) cur[it] = nxt[it];
+    }
+}
+
+// Digit counts of slots [b, e) into s_h (LDS atomics), padding skipped; the
+// next tile's partition words are loaded before the current tile is counted.
+template <class R>
+__device__ __forceinline__ void pt_count_range(const R *__restrict__ src, uint64_t b, uint64_t e, int shift,
+                                               uint32_t dmask, uint32_t *s_h) {
+    constexpr int IT = PartRec<R>::ITEMS;
+    constexpr uint64_t TN = (uint64_t)PT_THREADS * IT;
+    uint32_t cur[IT], nxt[IT];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const uint64_t i = b + (uint64_t)it * PT_THREADS + threadIdx.x;
+        cur[it] = i < e ? PartRec<R>::q_at(src, i) : PartRec<R>::PAD;
+    }
+    for (uint64_t t = b; t < e; t += TN) {
+        const uint64_t tn = t + TN;
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const uint64_t i = tn + (uint64_t)it * PT_THREADS + threadIdx.x;
+            nxt[it] = i < e ? PartRec<R>::q_at(src, i) : PartRec<R>::PAD;
+        }
+#pragma unroll
+        for (int it = 0; it < IT; it++)
+            if (cur[it] != PartRec<R>::PAD) atomicAdd(&s_h[(cur[it] >> shift) & dmask], 1u);
+#pragma unroll
+        for (int it = 0; it < IT; it++) cur[it] = nxt[it];
+    }
+}
+
+// Coarse digit counts per coarse-pass block range: PT_COUNT_SPLIT blocks share
+// each range (so the count runs at several blocks per CU) and add their
+// counts to the zeroed hist.
+constexpr int PT_COUNT_SPLIT = 4;
 template <class R>
 __global__ __launch_bounds__(PT_THREADS) void k_part_count(const R *__restrict__ src, PartArgs a) {
     __shared__ uint32_t s_h[PT_MAX_DIGITS];
     constexpr uint64_t TN = (uint64_t)PT_THREADS * PartRec<R>::ITEMS;
     for (uint32_t d = threadIdx.x; d < a.nc; d += PT_THREADS) s_h[d] = 0;
     __syncthreads();
-    uint64_t b, e;
-    pt_range(a, TN, b, e);
-    for (uint64_t t = b; t < e; t += TN) {
-        uint32_t q[PartRec<R>::ITEMS];
-#pragma unroll
-        for (int it = 0; it < PartRec<R>::ITEMS; it++) {
-            const uint64_t i = t + (uint64_t)it * PT_THREADS + threadIdx.x;
-            q[it] = i < e ? PartRec<R>::q_at(src, i) : PartRec<R>::PAD;
-        }
-#pragma unroll
-        for (int it = 0; it < PartRec<R>::ITEMS; it++)
-            if (q[it] != PartRec<R>::PAD) atomicAdd(&s_h[q[it] >> a.fbits], 1u);
-    }
+    const uint32_t g = blockIdx.x / PT_COUNT_SPLIT, part = blockIdx.x % PT_COUNT_SPLIT;
+    const uint64_t n = min((uint64_t)*a.n_slots, a.cap);
+    const uint64_t tiles = (n + TN - 1) / TN;
+    const uint64_t per = (tiles + a.nblk - 1) / a.nblk * TN;
+    const uint64_t gb = min(n, (uint64_t)g * per), ge = min(n, gb + per);
+    const uint64_t sub = (per / TN + PT_COUNT_SPLIT - 1) / PT_COUNT_SPLIT * TN;
+    const uint64_t b = min(ge, gb + part * sub), e = min(ge, b + sub);
+    pt_count_range<R>(src, b, e, a.fbits, 0xffffffffu, s_h);
     __syncthreads();
-    for (uint32_t d = threadIdx.x; d < a.nc; d += PT_THREADS) a.hist[(uint64_t)d * a.nblk + blockIdx.x] = s_h[d];
+    for (uint32_t d = threadIdx.x; d < a.nc; d += PT_THREADS)
+        if (s_h[d]) atomicAdd(&a.hist[(uint64_t)d * a.nblk + g], s_h[d]);
 }
 
 // Per coarse digit (one block each): exclusive prefix of its counts over the
@@ -831,9 +886,8 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_coarse(const R *__restrict_
     __syncthreads();
     uint64_t b, e;
     pt_range(a, TN, b, e);
-    for (uint64_t t = b; t < e; t += TN)
-        pt_move_tile<R>(src, dst, t, min(e, t + TN), a.fbits, 0xffffffffu, a.nc, a.cap, s_stage, s_cnt, s_start,
-                        s_cur, s_wsum, a.err);
+    pt_move_range<R>(src, dst, b, e, a.fbits, 0xffffffffu, a.nc, a.cap, s_stage, s_cnt, s_start, s_cur, s_wsum,
+                     a.err);
 }
 
 template <class R>
@@ -847,17 +901,7 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_fine(const R *__restrict__ 
     const uint64_t bs = a.cbase[bkt], be = a.cbase[bkt + 1];
     for (uint32_t d = threadIdx.x; d < a.nf; d += PT_THREADS) s_cnt[d] = 0;
     __syncthreads();
-    for (uint64_t t = bs; t < be; t += TN) {
-        uint32_t q[PartRec<R>::ITEMS];
-#pragma unroll
-        for (int it = 0; it < PartRec<R>::ITEMS; it++) {
-            const uint64_t i = t + (uint64_t)it * PT_THREADS + threadIdx.x;
-            q[it] = i < be ? PartRec<R>::q_at(src, i) : PartRec<R>::PAD;
-        }
-#pragma unroll
-        for (int it = 0; it < PartRec<R>::ITEMS; it++)
-            if (q[it] != PartRec<R>::PAD) atomicAdd(&s_cnt[q[it] & fmask], 1u);
-    }
+    pt_count_range<R>(src, bs, be, 0, fmask, s_cnt);
     __syncthreads();
     pt_digit_scan(s_cnt, s_start, a.nf, s_wsum);
     for (uint32_t f = threadIdx.x; f < a.nf; f += PT_THREADS) {
@@ -871,9 +915,7 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_fine(const R *__restrict__ 
         s_cnt[f] = 0;
     }
     __syncthreads();
-    for (uint64_t t = bs; t < be; t += TN)
-        pt_move_tile<R>(src, dst, t, min(be, t + TN), 0, fmask, a.nf, a.cap, s_stage, s_cnt, s_start, s_cur, s_wsum,
-                        a.err);
+    pt_move_range<R>(src, dst, bs, be, 0, fmask, a.nf, a.cap, s_stage, s_cnt, s_start, s_cur, s_wsum, a.err);
 }
 
 // ---------------------------------------------------------------------------
@@ -1955,7 +1997,9 @@ static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg,
     if (tm) tm->mark(s);
     const PartArgs pa = part_args(p);
     if (wide) {
-        hipLaunchKernelGGL(k_part_count<Rec32>, dim3(pa.nblk), dim3(PT_THREADS), 0, s, p.stage_rec_w.as<Rec32>(), pa);
+        fill_u32(pa.hist, (uint64_t)pa.nc * pa.nblk, 0u, s);
+        hipLaunchKernelGGL(k_part_count<Rec32>, dim3(pa.nblk * PT_COUNT_SPLIT), dim3(PT_THREADS), 0, s,
+                           p.stage_rec_w.as<Rec32>(), pa);
         hipLaunchKernelGGL(k_part_colscan, dim3(pa.nc), dim3(256), 0, s, pa);
         hipLaunchKernelGGL(k_part_bucketscan, dim3(1), dim3(1024), 0, s, pa);
         hipLaunchKernelGGL(k_part_coarse<Rec32>, dim3(pa.nblk), dim3(PT_THREADS), 0, s, p.stage_rec_w.as<Rec32>(),
@@ -1963,7 +2007,8 @@ static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg,
         hipLaunchKernelGGL(k_part_fine<Rec32>, dim3(pa.nc), dim3(PT_THREADS), 0, s, p.records_w.as<Rec32>(),
                            p.stage_rec_w.as<Rec32>(), pa);
     } else {
-        hipLaunchKernelGGL(k_part_count<ulonglong2>, dim3(pa.nblk), dim3(PT_THREADS), 0, s,
+        fill_u32(pa.hist, (uint64_t)pa.nc * pa.nblk, 0u, s);
+        hipLaunchKernelGGL(k_part_count<ulonglong2>, dim3(pa.nblk * PT_COUNT_SPLIT), dim3(PT_THREADS), 0, s,
                            p.stage_rec.as<ulonglong2>(), pa);
         hipLaunchKernelGGL(k_part_colscan, dim3(pa.nc), dim3(256), 0, s, pa);
         hipLaunchKernelGGL(k_part_bucketscan, dim3(1), dim3(1024), 0, s, pa);
