@@ -109,6 +109,24 @@ int mczi_count_datasets_to_kmer_fasta_path(const char *const *paths, size_t n,
                                            const mczi_counter_config *cfg,
                                            const char *out_path, mczi_kmer_fasta_stats *stats);
 
+/* Phase12Stats (src/lib.rs:69-75): seconds and pass-1 / partition figures. */
+typedef struct {
+    double phase1, phase2, total;
+    uint64_t unique_minimizer_hashes;
+    uint64_t partition_bytes;
+} mczi_phase12_stats;
+/* run_inputs_phase12_direct (src/lib.rs:160-165; the production minimizer
+ * order): pass 1 (minimizer counts) and pass 2 (filtered super-k-mer
+ * partitioning) on the device; partition_bytes is the size the reference's
+ * partition files would have. The SimdValueHash and AntiLex orders of the
+ * other benchmark variants are not provided. */
+int mczi_run_inputs_phase12_direct(const char *const *paths, size_t n, const mczi_counter_config *cfg,
+                                   mczi_phase12_stats *stats);
+/* count_inputs_minimizer_phase_direct / _packed_direct (src/lib.rs:1836-1854):
+ * pass 1 only; *unique gets the number of distinct minimizer keys. */
+int mczi_count_inputs_minimizer_phase_direct(const char *const *paths, size_t n, const mczi_counter_config *cfg,
+                                             uint64_t *unique);
+
 /* write_fasta_path (src/lib.rs:1203-1207) and write_fasta_after_index
  * (src/lib.rs:1194-1201, here writing to a path); last_index may be NULL. */
 int mczi_write_fasta_path(const char *path, const mczi_counted_kmer *kmers, size_t n, uint32_t k);
@@ -206,11 +224,20 @@ int mczi_batch_info(const mczi_batch *b, uint64_t *n_bases, uint64_t *n_frag,
                     uint32_t *n_datasets);
 void mczi_batch_free(mczi_batch *b);
 
-/* Counting modes. */
+/* Counting modes. The two-pass modes run the reference's pass 1 on the
+ * device (minimizer counts saturating at x + 1, src/lib.rs:3462-3481 and
+ * 4171-4251; or per-dataset minimizer presence, src/lib.rs:4253-4370) and
+ * stage only the super-k-mers whose minimizer passes (src/lib.rs:3483-3523,
+ * 4090-4128). The single-pass modes stage every super-k-mer (the KMC-style
+ * path, src/lib.rs:599-635). Both give the same k-mers and counts: every
+ * occurrence of a k-mer has the same minimizer, whose count (presence) is at
+ * least the k-mer's (min(c, x + 1)), so the filter only drops k-mers that
+ * fail count > x. The two-pass modes also report the minimizer statistics. */
 enum {
-    MCZI_MODE_TWO_PASS = 0,  /* minimizer pre-count, then filtered super-k-mers (FASTA path) */
-    MCZI_MODE_SINGLE_PASS = 1, /* all super-k-mers (KMC-style KFF path) */
-    MCZI_MODE_DATASETS = 2   /* fof dataset presence */
+    MCZI_MODE_TWO_PASS = 0,        /* pass 1 + filtered super-k-mers (FASTA path) */
+    MCZI_MODE_SINGLE_PASS = 1,     /* all super-k-mers (KMC-style KFF path) */
+    MCZI_MODE_DATASETS = 2,        /* fof dataset presence, all super-k-mers */
+    MCZI_MODE_DATASETS_TWO_PASS = 3 /* fof: dataset minimizer presence pass, then filtered super-k-mers */
 };
 /* Launch structure. */
 enum {
@@ -233,14 +260,23 @@ typedef struct {
     uint64_t n_overflow;         /* distinct k-mers that spilled from LDS to HBM */
     uint64_t n_out;              /* k-mers emitted */
     uint64_t tile_windows;       /* windows per scan tile (pieces are also cut at tile starts) */
-    uint64_t partition_bytes;    /* bytes of super-k-mer records */
+    uint64_t partition_bytes;    /* bytes the staged super-k-mers take in the reference's
+                                    partition files (src/lib.rs:3209-3216, 575) */
     uint32_t internal_partitions;
     uint32_t lds_slots;
     double ms_scan1, ms_scan2, ms_count, ms_sort, ms_total; /* device time per phase */
+    /* two-pass modes: distinct minimizer keys of pass 1 (unique_hashes(),
+     * src/lib.rs:4630-4632), and the keys that pass the filter (count > x;
+     * fof: present in more than x datasets, the retained minimizers of
+     * src/lib.rs:815); ms_pass1 = pass-1 device time */
+    uint64_t unique_minimizers;
+    uint64_t minimizers_retained;
+    double ms_pass1;
 } mczi_run_stats;
 
 /* Count the k-mers of a batch on the device. Results stay on the device
- * until mczi_batch_download; stats may be NULL. */
+ * until mczi_batch_download; stats may be NULL. With stats, partition_bytes
+ * is exact (runs cut by the scan's tiles are joined on the host). */
 int mczi_batch_count(mczi_batch *b, const mczi_counter_config *cfg, int mode, int launch,
                      int order, mczi_run_stats *stats);
 int mczi_batch_download(mczi_batch *b, mczi_counted_kmer **out, size_t *n_out);

@@ -26,7 +26,7 @@ if not os.path.exists(LIB_PATH):
 _lib = C.CDLL(LIB_PATH)
 
 DEFAULT_PARTITION_COUNT = 1024
-MODE_TWO_PASS, MODE_SINGLE_PASS, MODE_DATASETS = 0, 1, 2
+MODE_TWO_PASS, MODE_SINGLE_PASS, MODE_DATASETS, MODE_DATASETS_TWO_PASS = 0, 1, 2, 3
 LAUNCH_BATCHED, LAUNCH_PLAIN = 0, 1
 ORDER_ENCODED, ORDER_KFF_U8, ORDER_NONE = 0, 1, 2
 
@@ -57,7 +57,8 @@ class RunStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("n_bases", "n_windows", "n_pieces", "n_records", "n_distinct",
                                          "n_overflow", "n_out", "tile_windows", "partition_bytes")] + [
         ("internal_partitions", C.c_uint32), ("lds_slots", C.c_uint32)] + [
-        (n, C.c_double) for n in ("ms_scan1", "ms_scan2", "ms_count", "ms_sort", "ms_total")]
+        (n, C.c_double) for n in ("ms_scan1", "ms_scan2", "ms_count", "ms_sort", "ms_total")] + [
+        ("unique_minimizers", C.c_uint64), ("minimizers_retained", C.c_uint64), ("ms_pass1", C.c_double)]
 
     def as_dict(self) -> dict:
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -118,6 +119,14 @@ for _n in ("mczi_count_inputs_to_fasta_path", "mczi_count_datasets_to_fasta_path
     _sig(_n, C.c_int, [_CP, C.c_size_t, C.POINTER(_Config), C.c_char_p])
 for _n in ("mczi_count_inputs_to_kmer_fasta_path", "mczi_count_datasets_to_kmer_fasta_path"):
     _sig(_n, C.c_int, [_CP, C.c_size_t, C.POINTER(_Config), C.c_char_p, C.POINTER(_FastaStats)])
+class Phase12Stats(C.Structure):
+    _fields_ = [("phase1", C.c_double), ("phase2", C.c_double), ("total", C.c_double),
+                ("unique_minimizer_hashes", C.c_uint64), ("partition_bytes", C.c_uint64)]
+
+
+_sig("mczi_run_inputs_phase12_direct", C.c_int, [_CP, C.c_size_t, C.POINTER(_Config), C.POINTER(Phase12Stats)])
+_sig("mczi_count_inputs_minimizer_phase_direct", C.c_int, [_CP, C.c_size_t, C.POINTER(_Config),
+                                                           C.POINTER(C.c_uint64)])
 _sig("mczi_write_fasta_path", C.c_int, [C.c_char_p, _P, C.c_size_t, C.c_uint32])
 _sig("mczi_write_fasta_after_index", C.c_int, [C.c_char_p, _P, C.c_size_t, C.c_uint32, C.c_size_t,
                                                C.POINTER(C.c_size_t)])
@@ -179,7 +188,8 @@ EXPORTED = [
     "mczi_last_error", "mczi_set_threads", "mczi_version", "mczi_free", "mczi_validate_config", "mczi_expand_fofns",
     "mczi_free_strings", "mczi_count_inputs", "mczi_count_datasets", "mczi_count_inputs_to_fasta_path",
     "mczi_count_datasets_to_fasta_path", "mczi_count_inputs_to_kff_path", "mczi_count_datasets_to_kff_path",
-    "mczi_count_inputs_to_kmer_fasta_path", "mczi_count_datasets_to_kmer_fasta_path", "mczi_write_fasta_path",
+    "mczi_count_inputs_to_kmer_fasta_path", "mczi_count_datasets_to_kmer_fasta_path",
+    "mczi_run_inputs_phase12_direct", "mczi_count_inputs_minimizer_phase_direct", "mczi_write_fasta_path",
     "mczi_write_fasta_after_index", "mczi_write_kff_path", "mczi_simplitig_sequences",
     "mczi_simplitig_sequences_parallel", "mczi_decode_kmer", "mczi_minimizer_run_hashes",
     "mczi_batch_from_files", "mczi_pack_fastx", "mczi_batch_from_packed", "mczi_batch_from_packed_inputs",
@@ -423,6 +433,22 @@ def pack_fastx(path, min_len: int = 1):
         _lib.mczi_free(C.cast(wp, C.c_void_p))
         _lib.mczi_free(C.cast(op, C.c_void_p))
     return words, off
+
+
+def run_inputs_phase12_direct(inputs, cfg: CounterConfig) -> dict:
+    """Passes 1 and 2 (minimizer counts, filtered super-k-mer partitioning)
+    with their times, distinct minimizers and partition-file bytes."""
+    arr, n = _paths(inputs)
+    st = Phase12Stats()
+    _check(_lib.mczi_run_inputs_phase12_direct(arr, n, C.byref(cfg._c()), C.byref(st)))
+    return {f: getattr(st, f) for f, _ in st._fields_}
+
+
+def count_inputs_minimizer_phase_direct(inputs, cfg: CounterConfig) -> int:
+    arr, n = _paths(inputs)
+    u = C.c_uint64()
+    _check(_lib.mczi_count_inputs_minimizer_phase_direct(arr, n, C.byref(cfg._c()), C.byref(u)))
+    return u.value
 
 
 def write_fasta_path(path, kmers, k: int) -> None:

@@ -74,6 +74,8 @@ std::vector<std::string> to_paths(const char *const *paths, size_t n) {
     return v;
 }
 
+bool is_datasets(int mode) { return mode == MCZI_MODE_DATASETS || mode == MCZI_MODE_DATASETS_TWO_PASS; }
+
 std::unique_ptr<Batch> load_inputs(const std::vector<std::string> &paths, uint32_t k, bool datasets) {
     HostReads r;
     r.n_datasets = datasets ? (uint32_t)paths.size() : 1;
@@ -84,9 +86,12 @@ std::unique_ptr<Batch> load_inputs(const std::vector<std::string> &paths, uint32
 std::vector<mczi_counted_kmer> run_count(const std::vector<std::string> &paths,
                                          const mczi_counter_config &cfg, int mode, int order,
                                          mczi_run_stats *st) {
-    auto b = load_inputs(paths, cfg.k, mode == MCZI_MODE_DATASETS);
+    auto b = load_inputs(paths, cfg.k, is_datasets(mode));
     std::vector<mczi_counted_kmer> out;
-    if (b->n_frag == 0) return out;
+    if (b->n_frag == 0) {
+        if (st) *st = mczi_run_stats{};
+        return out;
+    }
     batch_count(*b, cfg, mode, MCZI_LAUNCH_BATCHED, order, st);
     batch_download(*b, out);
     return out;
@@ -100,30 +105,39 @@ void copy_out(const std::vector<mczi_counted_kmer> &v, mczi_counted_kmer **out, 
     if (!v.empty()) memcpy(*out, v.data(), v.size() * sizeof(mczi_counted_kmer));
 }
 
+// The count paths stage every super-k-mer (single pass): the reference's
+// pass-1 filter cannot change the k-mers or counts emitted. Calls that report
+// the minimizer statistics run pass 1 on the device (two_pass; *st gets the
+// run's statistics).
 std::vector<mczi_counted_kmer> count_inputs_impl(const char *const *paths, size_t n,
-                                                 const mczi_counter_config *cfg, int order) {
+                                                 const mczi_counter_config *cfg, int order,
+                                                 bool two_pass = false, mczi_run_stats *st = nullptr) {
     require(cfg != nullptr, MCZI_E_INVALID, "null config");
     validate_normal(*cfg);
     require(n > 0, MCZI_E_INVALID, "at least one input file is required");
     const double t0 = now_s();
-    mczi_run_stats st{};
-    auto v = run_count(to_paths(paths, n), *cfg, MCZI_MODE_TWO_PASS, order, &st);
+    mczi_run_stats local{};
+    auto v = run_count(to_paths(paths, n), *cfg, two_pass ? MCZI_MODE_TWO_PASS : MCZI_MODE_SINGLE_PASS, order,
+                       st ? st : &local);
     log_phase("3_kmer_counting_and_output", now_s() - t0);
     return v;
 }
 
 std::vector<mczi_counted_kmer> count_datasets_impl(const char *const *paths, size_t n,
-                                                   const mczi_counter_config *cfg) {
+                                                   const mczi_counter_config *cfg, bool two_pass = false,
+                                                   mczi_run_stats *st = nullptr) {
     require(cfg != nullptr, MCZI_E_INVALID, "null config");
     validate_datasets(*cfg);
     require(n > 0, MCZI_E_INVALID, "at least one dataset file is required");
+    if (st) *st = mczi_run_stats{};
     if (cfg->threshold >= n) {  // src/lib.rs:644-648
         log_phase("1_dataset_minimizer_counting", 0);
         log_phase("2_dataset_kmer_counting", 0);
         return {};
     }
     const double t0 = now_s();
-    auto v = run_count(to_paths(paths, n), *cfg, MCZI_MODE_DATASETS, MCZI_ORDER_ENCODED, nullptr);
+    auto v = run_count(to_paths(paths, n), *cfg, two_pass ? MCZI_MODE_DATASETS_TWO_PASS : MCZI_MODE_DATASETS,
+                       MCZI_ORDER_ENCODED, st);
     log_phase("2_dataset_kmer_counting", now_s() - t0);
     return v;
 }
@@ -200,13 +214,18 @@ ZiScan mczi_run(const char *const *ip, size_t ni, bool datasets, const char *con
                 ni, (unsigned long long)cfg.threshold);
         ResourcePhase::zero("1_index_minimizer_presence_counting");
         ResourcePhase::zero("2_index_kmer_partition_counting");
+        mczi_stat("index_minimizers_above_threshold", 0);
     } else {
         ResourcePhase ph;
         b = load_inputs(to_paths(ip, ni), cfg.k, datasets);
+        mczi_run_stats st{};
         if (b->n_frag)
-            batch_count(*b, cfg, datasets ? MCZI_MODE_DATASETS : MCZI_MODE_TWO_PASS, MCZI_LAUNCH_BATCHED,
-                        MCZI_ORDER_ENCODED, nullptr);
+            batch_count(*b, cfg, datasets ? MCZI_MODE_DATASETS_TWO_PASS : MCZI_MODE_TWO_PASS, MCZI_LAUNCH_BATCHED,
+                        MCZI_ORDER_ENCODED, &st);
         n_index = b->n_frag ? b->n_out : 0;
+        // KmerFastaWriteStats.minimizers_above_threshold of the index count
+        // (src/mczi.rs:351-354)
+        mczi_stat("index_minimizers_above_threshold", datasets ? st.minimizers_retained : st.unique_minimizers);
         if (datasets) {
             ResourcePhase::zero("1_index_minimizer_presence_counting");
             ph.log("2_index_kmer_partition_counting");
@@ -470,7 +489,7 @@ int mczi_count_datasets(const char *const *paths, size_t n, const mczi_counter_c
 void count_to_simplitigs(const std::vector<std::string> &paths, const mczi_counter_config &cfg, int mode,
                          const char *out_path, const char *count_phase) {
     const double t0 = now_s();
-    auto b = load_inputs(paths, cfg.k, mode == MCZI_MODE_DATASETS);
+    auto b = load_inputs(paths, cfg.k, is_datasets(mode));
     if (b->n_frag) batch_count(*b, cfg, mode, MCZI_LAUNCH_BATCHED, MCZI_ORDER_ENCODED, nullptr);
     log_phase(count_phase, now_s() - t0);
     const size_t n_out = b->n_frag ? (size_t)b->n_out : 0;
@@ -481,7 +500,7 @@ void count_to_simplitigs(const std::vector<std::string> &paths, const mczi_count
             device_forced_links(b->out_kmer.as<unsigned long long>(),
                                 cfg.k > 32 ? b->out_hi.as<unsigned long long>() : nullptr, n_out, cfg.k, link);
         });
-    log_phase(mode == MCZI_MODE_DATASETS ? "3_output_streaming" : "4_simplitig_output", now_s() - t1);
+    log_phase(is_datasets(mode) ? "3_output_streaming" : "4_simplitig_output", now_s() - t1);
 }
 
 int mczi_count_inputs_to_fasta_path(const char *const *paths, size_t n, const mczi_counter_config *cfg,
@@ -492,7 +511,7 @@ int mczi_count_inputs_to_fasta_path(const char *const *paths, size_t n, const mc
         require(n > 0, MCZI_E_INVALID, "at least one input file is required");
         std::lock_guard<std::mutex> lk(Device::get().call_mutex());
         const double t0 = now_s();
-        count_to_simplitigs(to_paths(paths, n), *cfg, MCZI_MODE_TWO_PASS, out_path, "3_kmer_counting_and_output");
+        count_to_simplitigs(to_paths(paths, n), *cfg, MCZI_MODE_SINGLE_PASS, out_path, "3_kmer_counting_and_output");
         log_phase("fasta_total", now_s() - t0);
     });
 }
@@ -565,11 +584,13 @@ int mczi_count_inputs_to_kmer_fasta_path(const char *const *paths, size_t n,
         require(out_path != nullptr, MCZI_E_INVALID, "null output path");
         std::lock_guard<std::mutex> lk(Device::get().call_mutex());
         const double t0 = now_s();
-        auto v = count_inputs_impl(paths, n, cfg, MCZI_ORDER_ENCODED);
+        mczi_run_stats st{};
+        auto v = count_inputs_impl(paths, n, cfg, MCZI_ORDER_ENCODED, stats != nullptr, &st);
         write_kmer_fasta(out_path, v, cfg->k);
         if (stats) {
+            // retained_minimizers = unique_hashes() of pass 1 (src/lib.rs:950-951, 326-351)
             stats->selected_kmers = v.size();
-            stats->minimizers_above_threshold = 0;
+            stats->minimizers_above_threshold = st.unique_minimizers;
             stats->total_seconds = now_s() - t0;
         }
     });
@@ -582,13 +603,50 @@ int mczi_count_datasets_to_kmer_fasta_path(const char *const *paths, size_t n,
         require(out_path != nullptr, MCZI_E_INVALID, "null output path");
         std::lock_guard<std::mutex> lk(Device::get().call_mutex());
         const double t0 = now_s();
-        auto v = count_datasets_impl(paths, n, cfg);
+        mczi_run_stats st{};
+        auto v = count_datasets_impl(paths, n, cfg, stats != nullptr, &st);
         write_kmer_fasta(out_path, v, cfg->k);
         if (stats) {
+            // the minimizers retained after the last dataset (src/lib.rs:759-760, 815)
             stats->selected_kmers = v.size();
-            stats->minimizers_above_threshold = 0;
+            stats->minimizers_above_threshold = st.minimizers_retained;
             stats->total_seconds = now_s() - t0;
         }
+    });
+}
+
+int mczi_run_inputs_phase12_direct(const char *const *paths, size_t n, const mczi_counter_config *cfg,
+                                   mczi_phase12_stats *stats) {
+    return guarded("failed to run phases 1-2", [&] {
+        require(cfg != nullptr && stats != nullptr, MCZI_E_INVALID, "null argument");
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        validate_normal(*cfg);
+        require(n > 0, MCZI_E_INVALID, "at least one input file is required");
+        const double t0 = now_s();
+        auto b = load_inputs(to_paths(paths, n), cfg->k, false);
+        mczi_run_stats st{};
+        if (b->n_frag) batch_count(*b, *cfg, MCZI_MODE_TWO_PASS, MCZI_LAUNCH_BATCHED, MCZI_ORDER_NONE, &st);
+        *stats = mczi_phase12_stats{};
+        stats->phase1 = st.ms_pass1 / 1e3;
+        stats->phase2 = (st.ms_total > 0 ? st.ms_total : 0) / 1e3;
+        stats->unique_minimizer_hashes = st.unique_minimizers;
+        stats->partition_bytes = st.partition_bytes;
+        log_phase("1_minimizer_counting", stats->phase1);
+        log_phase("2_superkmer_partitioning", stats->phase2);
+        stats->total = now_s() - t0;
+        log_phase("phase12_total", stats->total);
+    });
+}
+
+int mczi_count_inputs_minimizer_phase_direct(const char *const *paths, size_t n, const mczi_counter_config *cfg,
+                                             uint64_t *unique) {
+    return guarded("failed to count minimizers", [&] {
+        require(cfg != nullptr && unique != nullptr, MCZI_E_INVALID, "null argument");
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        validate_normal(*cfg);
+        require(n > 0, MCZI_E_INVALID, "at least one input file is required");
+        auto b = load_inputs(to_paths(paths, n), cfg->k, false);
+        *unique = b->n_frag ? batch_minimizer_count(*b, *cfg) : 0;
     });
 }
 
@@ -829,9 +887,9 @@ int mczi_batch_count(mczi_batch *b, const mczi_counter_config *cfg, int mode, in
     return guarded("failed to count batch", [&] {
         require(b != nullptr && cfg != nullptr, MCZI_E_INVALID, "null argument");
         std::lock_guard<std::mutex> lk(Device::get().call_mutex());
-        if (mode == MCZI_MODE_DATASETS) validate_datasets(*cfg);
+        if (is_datasets(mode)) validate_datasets(*cfg);
         else validate_normal(*cfg);
-        if (mode == MCZI_MODE_DATASETS && cfg->threshold >= b->b->n_datasets) {
+        if (is_datasets(mode) && cfg->threshold >= b->b->n_datasets) {
             b->b->n_out = 0;
             if (stats) *stats = mczi_run_stats{};
             return;
@@ -845,7 +903,7 @@ int mczi_batch_autotune(mczi_batch *b, const mczi_counter_config *cfg, int mode,
     return guarded("failed to autotune batch", [&] {
         require(b != nullptr && cfg != nullptr, MCZI_E_INVALID, "null argument");
         std::lock_guard<std::mutex> lk(Device::get().call_mutex());
-        if (mode == MCZI_MODE_DATASETS) validate_datasets(*cfg);
+        if (is_datasets(mode)) validate_datasets(*cfg);
         else validate_normal(*cfg);
         const double t = batch_autotune(*b->b, *cfg, mode, launch, order);
         if (instances_per_partition) *instances_per_partition = t;

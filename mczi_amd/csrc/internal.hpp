@@ -261,6 +261,8 @@ void batch_download_records(Batch &b, std::vector<uint64_t> &rec, std::vector<ui
 // Only the emitted k-mers' low (and, for k > 32, high) words.
 void batch_download_keys(Batch &b, uint64_t *lo, uint64_t *hi);
 double batch_autotune(Batch &b, const mczi_counter_config &cfg, int mode, int launch, int order);
+// Pass 1 alone (minimizer counts): the number of distinct minimizer keys.
+uint64_t batch_minimizer_count(Batch &b, const mczi_counter_config &cfg);
 void free_plan(McPlan *p);
 
 // Plain device copy of `bytes` (average ms per launch over `reps`).

@@ -72,7 +72,8 @@ constexpr int OVF_MAXPROBE = 1 << 16;
 
 enum ScanMode {
     STAGE = 0,    // super-k-mer records + partition ids (or only their count)
-    WINDOWS = 1   // selected hash of every window
+    WINDOWS = 1,  // selected hash of every window
+    RUNS = 2      // pass 1: (run key, windows or dataset) of every piece
 };
 
 struct ScanArgs {
@@ -92,7 +93,19 @@ struct ScanArgs {
     uint32_t wrec;
     uint32_t ds_lo;   // fof: first dataset of the group of 32 being staged
     uint32_t *win_out;
-    unsigned long long *stats;  // [0] pieces [4] records
+    unsigned long long *stats;  // [0] pieces [4] records [5] .skm bytes of the staged runs
+    // pass 1 (RUNS): one entry per piece, key << 32 | (windows, or dataset with fof)
+    unsigned long long *run_out;
+    uint64_t run_cap;
+    unsigned long long *run_ctr;
+    // pass-1 filter of the two-pass mode: set of the keys that pass (key + 1
+    // per slot, 0 = empty), or null
+    const unsigned long long *mtab;
+    uint64_t mtab_mask;
+    // optional, per tile: the first and last staged piece when the run they
+    // belong to may continue across the tile edge (key << 32 | windows, bit
+    // 31 = the tile's only piece; ~0 = none)
+    unsigned long long *tile_first, *tile_last;
     // staging of super-k-mer records and their internal partition
     ulonglong2 *stage_rec;
     Rec32 *stage_rec_w;  // k > 32
@@ -186,6 +199,21 @@ __device__ __forceinline__ uint32_t extract16_bases(const uint32_t *sw, int pos)
     return (uint32_t)(v >> sh) & 0xffffu;
 }
 
+// Bytes of a super-k-mer of L bases in a partition file (append_record_len
+// and append_packed_dna_range, src/lib.rs:3170-3216).
+__device__ __forceinline__ uint64_t skm_bytes(uint32_t L) { return (L < 255u ? 1u : 5u) + (L + 3u) / 4u; }
+
+// Pass-1 filter: whether key is in the set of keys above the threshold.
+__device__ __forceinline__ bool mtab_has(const ScanArgs &a, uint32_t key) {
+    uint64_t i = spread_hash_u64(key) & a.mtab_mask;
+    for (;;) {
+        const unsigned long long v = a.mtab[i];
+        if (v == 0) return false;
+        if (v == (unsigned long long)key + 1) return true;
+        i = (i + 1) & a.mtab_mask;
+    }
+}
+
 // Staging slots [b, e) that no record fills (the rest of an abandoned chunk)
 // are marked with the padding partition, which partitioning skips.
 template <bool WIDE>
@@ -233,7 +261,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
 
     const int tid = threadIdx.x;
     const int k = a.k, m = a.m, w = WC ? WC : a.w;
-    unsigned long long my_pieces = 0, my_records = 0;
+    unsigned long long my_pieces = 0, my_records = 0, my_bytes = 0;
     unsigned long long chunk_cur = 0, chunk_end = 0;  // thread 0: the block's current staging chunk
     // each thread owns positions r0 .. r0 + WPT - 1 of a tile; P8(r0 + c),
     // the padded LDS index, is 9 * tid + c + floor(c / 8) since r0 = 8 * tid
@@ -486,17 +514,58 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
         // = spread_hash_u32(key) % P (minimizer_shard, src/lib.rs:2927-2929).
         // Bases are stored in MC's 2-bit code (A0 C1 G2 T3, src/lib.rs:
         // 3192-3197), base j of the record at bits 2j.
+        if constexpr (MODE == RUNS) {
+            // pass 1 (add_minimizer_counts_on_seq, src/lib.rs:3462-3481, and
+            // the dataset presence of src/lib.rs:4291-4341): every piece's
+            // key with its window count (or its dataset), one reservation
+            // per tile
+            const int mine = np > tid ? (np - tid + SCAN_THREADS - 1) / SCAN_THREADS : 0;
+            int tot_r = 0;
+            const int off_r = block_exclusive_scan(mine, &tot_r, s_wsum);
+            if (tid == 0) s_sbase = tot_r ? atomicAdd(a.run_ctr, (unsigned long long)tot_r) : 0ull;
+            __syncthreads();
+            uint64_t pos = s_sbase + (uint64_t)off_r;
+            for (int pi = tid; pi < np; pi += SCAN_THREADS, pos++) {
+                const uint32_t pw = s_u.run.pst[pi];
+                const int s = (int)(pw & 2047u);
+                int e = (pi + 1 < np) ? (int)(s_u.run.pst[pi + 1] & 2047u) : tl;
+                const int vend = s_fs[(pw >> 11) + 1] - k + 1;
+                if (vend < e) e = vend;
+                my_pieces++;
+                const uint32_t key = spread_hash_u32(s_u.run.v[P8(s)]);
+                const uint32_t val = a.dataset != nullptr ? a.dataset[f0 + (pw >> 11)] : (uint32_t)(e - s);
+                if (pos < a.run_cap) a.run_out[pos] = ((unsigned long long)key << 32) | val;
+            }
+            continue;
+        }
+        // Which of this thread's pieces (at most TILE / SCAN_THREADS + 1) are
+        // staged: fof keeps the current group of 32 datasets, the two-pass
+        // mode the keys above the threshold (add_filtered_superkmers_packed,
+        // src/lib.rs:3483-3523).
         int my_nrec = 0;
-        for (int pi = tid; pi < np; pi += SCAN_THREADS) {
+        uint32_t staged = 0;
+        for (int pi = tid, slot = 0; pi < np; pi += SCAN_THREADS, slot++) {
             const uint32_t pw = s_u.run.pst[pi];
             const int s = (int)(pw & 2047u);
             int e = (pi + 1 < np) ? (int)(s_u.run.pst[pi + 1] & 2047u) : tl;
             const int vend = s_fs[(pw >> 11) + 1] - k + 1;
             if (vend < e) e = vend;
             my_pieces++;
-            // fof: only the datasets of the current group of 32 are staged
             if (a.dataset != nullptr && a.dataset[f0 + (pw >> 11)] - a.ds_lo >= 32u) continue;
+            const uint32_t key = spread_hash_u32(s_u.run.v[P8(s)]);
+            if (a.mtab != nullptr && !mtab_has(a, key)) continue;
+            staged |= 1u << slot;
             my_nrec += e - s <= (int)a.wrec ? 1 : (e - s + (int)a.wrec - 1) / (int)a.wrec;
+            my_bytes += skm_bytes((uint32_t)(e - s + k - 1));
+            if (a.tile_first != nullptr) {
+                // a run cut by the tile edge: the first piece starts at window
+                // 0 inside a fragment that began earlier, the last one ends at
+                // the tile end inside a fragment that goes on
+                const unsigned long long ent = ((unsigned long long)key << 32) | (uint32_t)(e - s) |
+                                               (np == 1 ? 0x80000000ull : 0ull);
+                if (pi == 0 && s == 0 && !(s_flag[P8(0)] & 2)) a.tile_first[t] = ent;
+                if (pi == np - 1 && e == tl && vend > tl) a.tile_last[t] = ent;
+            }
         }
         int tot = 0;
         const int off = block_exclusive_scan(my_nrec, &tot, s_wsum);
@@ -529,7 +598,8 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
         stage_pad<WIDE>(a, s_hole_b, s_hole_e);
         {
             uint64_t pos = s_sbase + (uint64_t)off;
-            for (int pi = tid; pi < np; pi += SCAN_THREADS) {
+            for (int pi = tid, slot = 0; pi < np; pi += SCAN_THREADS, slot++) {
+                if (!(staged & (1u << slot))) continue;
                 const uint32_t pw = s_u.run.pst[pi];
                 const int s = (int)(pw & 2047u);
                 const int jf = (int)(pw >> 11);
@@ -539,7 +609,6 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
                 const uint32_t hq = spread_hash_u32(spread_hash_u32(s_u.run.v[P8(s)]));
                 const uint32_t q = a.p_mask ? (hq & a.p_mask) : hq % a.p_int;
                 const uint64_t ds = a.dataset != nullptr ? (uint64_t)(a.dataset[f0 + jf] - a.ds_lo) : 0ull;
-                if (ds >= 32u) continue;
                 for (int cs = s; cs < e; cs += (int)a.wrec) {
                     const int ce = min(cs + (int)a.wrec, e);
                     const uint32_t L = (uint32_t)(ce - cs + k - 1);
@@ -593,7 +662,69 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
     }
     __shared__ unsigned long long s_red[SCAN_THREADS / 64];
     block_sum_atomic(my_pieces, &a.stats[0], s_red);
-    if constexpr (MODE == STAGE) block_sum_atomic(my_records, &a.stats[4], s_red);
+    if constexpr (MODE == STAGE) {
+        block_sum_atomic(my_records, &a.stats[4], s_red);
+        block_sum_atomic(my_bytes, &a.stats[5], s_red);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 1 from the run entries of the RUNS scan, sorted by key (fof: by key,
+// then dataset). A key's entries are contiguous. It passes when its windows
+// sum above x (the saturating count min(x + 1, sum of min(run, 255)) > x,
+// src/lib.rs:3477-3478, 4211-4236, 4625-4629; pieces of one run add up to the
+// run) or, with fof, when it occurs in more than x datasets (the minimizers
+// DatasetMinimizerTable retains, src/lib.rs:4291-4341, 659-664). Counts go to
+// ctr[0] (distinct keys) and ctr[1] (passing keys); with a set, the passing
+// keys are inserted (key + 1 per slot, linear probing).
+__global__ void k_mtab_scan(const unsigned long long *__restrict__ e, uint64_t n, int fof, uint64_t x,
+                            unsigned long long *mtab, uint64_t mask, unsigned long long *ctr) {
+    __shared__ unsigned long long s_red[4];
+    unsigned long long my_d = 0, my_p = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t key = (uint32_t)(e[i] >> 32);
+        if (i > 0 && (uint32_t)(e[i - 1] >> 32) == key) continue;
+        my_d++;
+        uint64_t acc = 0;
+        bool pass = false;
+        for (uint64_t j = i; j < n && (uint32_t)(e[j] >> 32) == key; j++) {
+            if (fof) acc += (j == i || e[j] != e[j - 1]) ? 1u : 0u;  // distinct datasets
+            else acc += (uint32_t)e[j];                              // windows
+            if (acc > x) {
+                pass = true;
+                break;
+            }
+        }
+        if (!pass) continue;
+        my_p++;
+        if (mtab) {
+            uint64_t sl = spread_hash_u64(key) & mask;
+            const unsigned long long v = (unsigned long long)key + 1;
+            while (atomicCAS(&mtab[sl], 0ull, v) != 0ull) sl = (sl + 1) & mask;
+        }
+    }
+    // block sums (256 threads)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        my_d += __shfl_xor(my_d, o, 64);
+        my_p += __shfl_xor(my_p, o, 64);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) s_red[wid] = my_d;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < 4; w++) t += s_red[w];
+        if (t) atomicAdd(&ctr[0], t);
+    }
+    __syncthreads();
+    if (lane == 0) s_red[wid] = my_p;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < 4; w++) t += s_red[w];
+        if (t) atomicAdd(&ctr[1], t);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1509,38 +1640,6 @@ __global__ void k_fill_zero16(uint4 *p, uint64_t n16) {
         p[i] = z;
 }
 
-// Exclusive scan of per-partition record counts -> bucket bases, cursors.
-constexpr int PSCAN_THREADS = 1024;
-__global__ __launch_bounds__(PSCAN_THREADS) void k_partition_scan(const uint32_t *hist, uint32_t n,
-                                                                  unsigned long long *base,
-                                                                  unsigned long long *cursor,
-                                                                  unsigned long long *total) {
-    __shared__ unsigned long long s_sum[PSCAN_THREADS];
-    const int tid = threadIdx.x;
-    const uint32_t per = (n + PSCAN_THREADS - 1) / PSCAN_THREADS;
-    const uint32_t b = min(n, tid * per), e = min(n, b + per);
-    unsigned long long s = 0;
-    for (uint32_t i = b; i < e; i++) s += hist[i];
-    s_sum[tid] = s;
-    __syncthreads();
-    for (int o = 1; o < PSCAN_THREADS; o <<= 1) {
-        unsigned long long y = (tid >= o) ? s_sum[tid - o] : 0ull;
-        __syncthreads();
-        s_sum[tid] += y;
-        __syncthreads();
-    }
-    unsigned long long run = s_sum[tid] - s;
-    for (uint32_t i = b; i < e; i++) {
-        base[i] = run;
-        cursor[i] = run;
-        run += hist[i];
-    }
-    if (tid == PSCAN_THREADS - 1) {
-        base[n] = s_sum[tid];
-        *total = s_sum[tid];
-    }
-}
-
 // KFF u8 order key: (encoded & 255, encoded >> 8) as one 64-bit key (k <= 32).
 __global__ void k_kff_keys(const unsigned long long *kmer, const uint32_t *part, uint64_t n,
                            uint32_t p_user, unsigned long long *key2, uint32_t *pu,
@@ -1613,6 +1712,11 @@ struct McPlan {
     uint64_t stage_cap = 0, n_staged = 0, n_records = 0;
     uint32_t part_blocks = 0;  // blocks of the coarse pass
     uint32_t ds_lo = 0;        // fof: first dataset of the group being counted
+    // pass 1: run entries, their sorted copy, the set of passing keys; tile
+    // edge entries for the exact partition byte count
+    DevBuf runs, runs_sorted, run_tmp, mtab, tile_first, tile_last;
+    uint64_t run_cap = 0, mtab_mask = 0;
+    bool use_mtab = false, want_tiles = false;
     DevBuf out_kmer, out_count, out_part;
     DevBuf sort_tmp, key2, pu, idx, idx2, pu2, k2out;
     uint64_t ovf_slots = 0;
@@ -1871,6 +1975,10 @@ static ScanArgs scan_args(Batch &b, McPlan &p, const mczi_counter_config &cfg) {
     sa.stage_rec = p.stage_rec.as<ulonglong2>();
     sa.stage_rec_w = p.stage_rec_w.as<Rec32>();
     sa.stage_cap = p.stage_cap;
+    sa.mtab = p.use_mtab ? p.mtab.as<unsigned long long>() : nullptr;
+    sa.mtab_mask = p.mtab_mask;
+    sa.tile_first = p.want_tiles ? p.tile_first.as<unsigned long long>() : nullptr;
+    sa.tile_last = p.want_tiles ? p.tile_last.as<unsigned long long>() : nullptr;
     sa.stage_ctr = p.ctr.as<unsigned long long>() + 11;
     sa.err = p.ctr.as<unsigned long long>() + 4;
     return sa;
@@ -1879,6 +1987,7 @@ static ScanArgs scan_args(Batch &b, McPlan &p, const mczi_counter_config &cfg) {
 // Launches the scan (one grid per input chunk with the plain launch, one for
 // the batch otherwise); returns the number of blocks, or only counts them
 // when s_launch is false.
+template <int MODE = STAGE>
 static uint64_t launch_scan(McPlan &p, const ScanArgs &sa, int launch, hipStream_t s, bool s_launch = true) {
     Device &d = Device::get();
     static const unsigned scan_res = resident_blocks((const void *)k_scan<STAGE, true>, SCAN_THREADS);
@@ -1893,11 +2002,11 @@ static uint64_t launch_scan(McPlan &p, const ScanArgs &sa, int launch, hipStream
         blocks += g;
         if (!s_launch) return;
         const bool gw = ca.w >= WPT, wide = ca.k > 32;
-        if (wide && gw) hipLaunchKernelGGL((k_scan<STAGE, true, true>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
-        else if (wide) hipLaunchKernelGGL((k_scan<STAGE, false, true>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
-        else if (gw && ca.w == 11) hipLaunchKernelGGL((k_scan<STAGE, true, false, 11>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
-        else if (gw) hipLaunchKernelGGL((k_scan<STAGE, true>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
-        else hipLaunchKernelGGL((k_scan<STAGE, false>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
+        if (wide && gw) hipLaunchKernelGGL((k_scan<MODE, true, true>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
+        else if (wide) hipLaunchKernelGGL((k_scan<MODE, false, true>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
+        else if (gw && ca.w == 11) hipLaunchKernelGGL((k_scan<MODE, true, false, 11>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
+        else if (gw) hipLaunchKernelGGL((k_scan<MODE, true>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
+        else hipLaunchKernelGGL((k_scan<MODE, false>), dim3(g), dim3(SCAN_THREADS), 0, s, ca);
     };
     if (launch == MCZI_LAUNCH_PLAIN) {
         for (size_t c = 0; c + 1 < p.chunk_tile.size(); c++) go(p.chunk_tile[c], p.chunk_tile[c + 1]);
@@ -1915,6 +2024,10 @@ static void enqueue_stage_a(Batch &b, McPlan &p, const mczi_counter_config &cfg,
     sa.ds_lo = p.ds_lo;
     unsigned long long *ctr = p.ctr.as<unsigned long long>();
     fill_u64(ctr, 16, 0, s);
+    if (p.want_tiles) {
+        fill_u64(p.tile_first.as<unsigned long long>(), p.n_tiles, ~0ull, s);
+        fill_u64(p.tile_last.as<unsigned long long>(), p.n_tiles, ~0ull, s);
+    }
     if (tm) tm->mark(s);
     launch_scan(p, sa, launch, s);
     if (tm) tm->mark(s);
@@ -2270,6 +2383,8 @@ static void count_pass(Batch &b, McPlan &p, const mczi_counter_config &cfg, int 
                 sig = mix_sig(sig, (uint64_t)(uintptr_t)db->as<void>());
             sig = mix_sig(mix_sig(mix_sig(sig, p.ovf_slots), p.stage_cap), p.out_cap);
             sig = mix_sig(mix_sig(sig, p.part_blocks), (uint64_t)launch);
+            sig = mix_sig(mix_sig(sig, p.use_mtab ? (uint64_t)(uintptr_t)p.mtab.as<void>() ^ p.mtab_mask : 0ull),
+                          p.want_tiles ? (uint64_t)(uintptr_t)p.tile_first.as<void>() : 0ull);
             if (!p.graph || p.graph_sig != sig) {
                 if (p.graph) {
                     MCZI_HIP(hipGraphExecDestroy(p.graph));
@@ -2345,10 +2460,11 @@ static void count_dataset_groups(Batch &b, McPlan &p, const mczi_counter_config 
     const uint32_t groups = (b.n_datasets + 31) / 32;
     DevBuf acc_k, acc_c;
     uint64_t used = 0, cap = 0;
-    unsigned long long *cur = p.ctr.as<unsigned long long>() + 14;
+    unsigned long long *cur = nullptr;
     for (uint32_t gi = 0; gi < groups; gi++) {
         p.ds_lo = gi * 32;
         count_pass(b, p, gcfg, MCZI_MODE_DATASETS, launch, tm, false, h);
+        cur = p.ctr.as<unsigned long long>() + 14;  // allocated by the pass
         const uint64_t n_slots = h[1], n_real = h[1] - h[6];
         if (used + n_real > cap) {  // grow, keeping what is there
             const uint64_t nc = std::max<uint64_t>(2 * cap, used + n_real) + 1024;
@@ -2388,24 +2504,140 @@ static void count_dataset_groups(Batch &b, McPlan &p, const mczi_counter_config 
     MCZI_HIP(rocprim::reduce_by_key(tmp.as<void>(), t2, sk.as<unsigned long long>(), sv.as<uint32_t>(), (size_t)used,
                                     uk.as<unsigned long long>(), uv.as<uint32_t>(), nu.as<unsigned long long>(),
                                     rocprim::plus<uint32_t>(), rocprim::equal_to<unsigned long long>(), s));
-    b.out_kmer.ensure(std::max<uint64_t>(used, 1) * 8);
-    b.out_count.ensure(std::max<uint64_t>(used, 1) * 4);
+    // the sums > x (appended in any order), then sorted by k-mer
     fill_u64(cur, 1, 0, s);
     hipLaunchKernelGGL(k_gt_compact, dim3(grid_for(std::max<uint64_t>(used, 1), 256, 8192)), dim3(256), 0, s,
                        uk.as<unsigned long long>(), uv.as<uint32_t>(), nu.as<unsigned long long>(),
-                       (uint32_t)cfg.threshold, b.out_kmer.as<unsigned long long>(), b.out_count.as<uint32_t>(), cur);
+                       (uint32_t)cfg.threshold, sk.as<unsigned long long>(), sv.as<uint32_t>(), cur);
     unsigned long long r[2] = {0, 0};
     MCZI_HIP(hipMemcpyAsync(&r[0], cur, 8, hipMemcpyDeviceToHost, s));
     MCZI_HIP(hipMemcpyAsync(&r[1], nu.as<void>(), 8, hipMemcpyDeviceToHost, s));
     MCZI_HIP(hipStreamSynchronize(s));
     n_out = r[0];
     n_distinct = r[1];
+    b.out_kmer.ensure(std::max<uint64_t>(n_out, 1) * 8);
+    b.out_count.ensure(std::max<uint64_t>(n_out, 1) * 4);
+    size_t t3 = 0;
+    MCZI_HIP(rocprim::radix_sort_pairs(nullptr, t3, sk.as<unsigned long long>(), b.out_kmer.as<unsigned long long>(),
+                                       sv.as<uint32_t>(), b.out_count.as<uint32_t>(), (size_t)n_out, 0, end_bit, s));
+    tmp.ensure(t3 + 16);
+    MCZI_HIP(rocprim::radix_sort_pairs(tmp.as<void>(), t3, sk.as<unsigned long long>(),
+                                       b.out_kmer.as<unsigned long long>(), sv.as<uint32_t>(),
+                                       b.out_count.as<uint32_t>(), (size_t)n_out, 0, end_bit, s));
+    MCZI_HIP(hipStreamSynchronize(s));
+}
+
+// Pass 1 on the device (count_input_minimizers, src/lib.rs:1726-1797; fof:
+// count_dataset_minimizer_presence, src/lib.rs:1799-1822 with the driver loop
+// of 659-665): the run keys of the whole batch with their windows (fof:
+// datasets), sorted, then the distinct and passing keys; the passing keys
+// become the set that filters staging. unique = distinct keys, retained =
+// passing keys.
+static void minimizer_pass(Batch &b, McPlan &p, const mczi_counter_config &cfg, bool fof, int launch,
+                           uint64_t &unique, uint64_t &retained, double &ms) {
+    Device &d = Device::get();
+    hipStream_t s = d.stream();
+    const uint32_t k = cfg.k;
+    hipEvent_t e0, e1;
+    MCZI_HIP(hipEventCreate(&e0));
+    MCZI_HIP(hipEventCreate(&e1));
+    MCZI_HIP(hipEventRecord(e0, s));
+    p.ctr.ensure(16 * 8);
+    unsigned long long *ctr = p.ctr.as<unsigned long long>();
+    if (!p.run_cap) {
+        const double w = (double)(k - cfg.minimizer + 1);
+        p.run_cap = (uint64_t)(1.25 * (double)p.n_windows * 2.0 / (w + 1.0)) + b.n_frag + p.n_tiles + 4096;
+    }
+    uint64_t n = 0;
+    for (;;) {
+        p.runs.ensure(p.run_cap * 8);
+        ScanArgs sa = scan_args(b, p, cfg);
+        sa.dataset = fof ? sa.dataset : nullptr;
+        sa.mtab = nullptr;
+        sa.tile_first = sa.tile_last = nullptr;
+        sa.run_out = p.runs.as<unsigned long long>();
+        sa.run_cap = p.run_cap;
+        sa.run_ctr = ctr + 10;
+        fill_u64(ctr, 16, 0, s);
+        launch_scan<RUNS>(p, sa, launch, s);
+        unsigned long long h[16];
+        MCZI_HIP(hipMemcpyAsync(h, ctr, sizeof h, hipMemcpyDeviceToHost, s));
+        MCZI_HIP(hipStreamSynchronize(s));
+        MCZI_HIP(hipGetLastError());
+        n = h[10];
+        if (n <= p.run_cap) break;
+        p.run_cap = n + n / 8 + 4096;
+    }
+    p.runs_sorted.ensure(std::max<uint64_t>(n, 1) * 8);
+    size_t tb = 0;
+    // whole entries: by key, then windows (dataset)
+    MCZI_HIP(rocprim::radix_sort_keys(nullptr, tb, p.runs.as<unsigned long long>(),
+                                      p.runs_sorted.as<unsigned long long>(), (size_t)n, 0u, 64u, s));
+    p.run_tmp.ensure(tb + 16);
+    MCZI_HIP(rocprim::radix_sort_keys(p.run_tmp.as<void>(), tb, p.runs.as<unsigned long long>(),
+                                      p.runs_sorted.as<unsigned long long>(), (size_t)n, 0u, 64u, s));
+    const unsigned g = grid_for(n, 256, 8192);
+    fill_u64(ctr, 4, 0, s);
+    hipLaunchKernelGGL(k_mtab_scan, dim3(g), dim3(256), 0, s, p.runs_sorted.as<unsigned long long>(), n, fof ? 1 : 0,
+                       (uint64_t)cfg.threshold, nullptr, 0ull, ctr);
+    unsigned long long c[2];
+    MCZI_HIP(hipMemcpyAsync(c, ctr, sizeof c, hipMemcpyDeviceToHost, s));
+    MCZI_HIP(hipStreamSynchronize(s));
+    unique = c[0];
+    retained = c[1];
+    const uint64_t slots = next_pow2(std::max<uint64_t>(2 * retained, 1024));
+    p.mtab.ensure(slots * 8);
+    p.mtab_mask = slots - 1;
+    fill_u64(p.mtab.as<unsigned long long>(), slots, 0ull, s);
+    hipLaunchKernelGGL(k_mtab_scan, dim3(g), dim3(256), 0, s, p.runs_sorted.as<unsigned long long>(), n, fof ? 1 : 0,
+                       (uint64_t)cfg.threshold, p.mtab.as<unsigned long long>(), p.mtab_mask, ctr + 2);
+    MCZI_HIP(hipEventRecord(e1, s));
+    MCZI_HIP(hipEventSynchronize(e1));
+    float f = 0;
+    MCZI_HIP(hipEventElapsedTime(&f, e0, e1));
+    ms = f;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
+static uint64_t skm_bytes_host(uint64_t L) { return (L < 255 ? 1 : 5) + (L + 3) / 4; }
+
+// The scan sums each staged piece's partition-file bytes; a run that a tile
+// edge cut into pieces is one super-k-mer in the reference's files
+// (add_packed_range, src/lib.rs:4893-4917). Joins such pieces from the tile
+// edge entries, tile by tile in stream order, and returns the correction.
+static int64_t join_tile_runs(const std::vector<uint64_t> &first, const std::vector<uint64_t> &last, uint32_t k) {
+    constexpr uint64_t NONE = ~0ull, SINGLE = 0x80000000ull;
+    int64_t corr = 0;
+    bool open = false;
+    uint32_t okey = 0;
+    uint64_t total = 0, parts = 0;
+    for (size_t t = 0; t < first.size(); t++) {
+        const uint64_t F = first[t], L = last[t];
+        if (open && F != NONE && (uint32_t)(F >> 32) == okey) {
+            const uint64_t w = F & 0x7fffffffull;
+            total += w;
+            parts += skm_bytes_host(w + k - 1);
+            if ((F & SINGLE) && L != NONE) continue;  // the tile's only piece runs on into the next tile
+        }
+        if (open) corr += (int64_t)skm_bytes_host(total + k - 1) - (int64_t)parts;
+        open = false;
+        if (L != NONE) {
+            open = true;
+            okey = (uint32_t)(L >> 32);
+            total = L & 0x7fffffffull;
+            parts = skm_bytes_host(total + k - 1);
+        }
+    }
+    if (open) corr += (int64_t)skm_bytes_host(total + k - 1) - (int64_t)parts;
+    return corr;
 }
 
 void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch, int order,
                  mczi_run_stats *stats) {
     Device &d = Device::get();
     hipStream_t s = d.stream();
+    (void)hipGetLastError();  // an earlier failed call's error is not this call's
     const uint32_t k = cfg.k;
     require(k >= 1 && k <= 64, MCZI_E_INVALID, "k must be <= 64 because MC stores k-mers in u128");
     const bool wide = k > 32;  // 128-bit k-mers, 256-bit records
@@ -2416,11 +2648,27 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
     require(cfg.partition_count > 0, MCZI_E_INVALID, "partition count must be greater than 0");
     require(cfg.partition_count <= MAX_INTERNAL_PARTITIONS, MCZI_E_INVALID,
             "partition count above 4194304 is not supported on the device");
+    require(mode >= MCZI_MODE_TWO_PASS && mode <= MCZI_MODE_DATASETS_TWO_PASS, MCZI_E_INVALID, "unknown counting mode");
+    const bool two_pass = mode == MCZI_MODE_TWO_PASS || mode == MCZI_MODE_DATASETS_TWO_PASS;
+    if (mode == MCZI_MODE_DATASETS_TWO_PASS) mode = MCZI_MODE_DATASETS;
     if (mode == MCZI_MODE_DATASETS) require(!wide, MCZI_E_INVALID, "FOFN dataset-presence mode currently requires k <= 32");
     require(!wide || order != MCZI_ORDER_KFF_U8, MCZI_E_INVALID, "KFF u8 record order needs k <= 32");
     McPlan &p = *b.plan;
     p.n_windows = windows_for(b, k);
     p.k = k;
+    uint64_t unique_min = 0, retained_min = 0;
+    double ms_pass1 = 0;
+    p.use_mtab = false;
+    if (two_pass) {
+        minimizer_pass(b, p, cfg, mode == MCZI_MODE_DATASETS, launch, unique_min, retained_min, ms_pass1);
+        p.use_mtab = true;
+    }
+    // exact partition bytes: per-tile edge entries, with stats, one dataset group
+    p.want_tiles = stats != nullptr && !(mode == MCZI_MODE_DATASETS && b.n_datasets > 32);
+    if (p.want_tiles) {
+        p.tile_first.ensure(std::max<uint64_t>(p.n_tiles, 1) * 8);
+        p.tile_last.ensure(std::max<uint64_t>(p.n_tiles, 1) * 8);
+    }
     const bool use_graph = launch == MCZI_LAUNCH_BATCHED && getenv("MCZI_NO_GRAPH") == nullptr;
     const bool want_times = stats != nullptr && !use_graph;
     Timer tm(stats != nullptr);
@@ -2440,6 +2688,9 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
         st.lds_slots = CNT_SLOTS;
         st.tile_windows = TILE;
         st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        st.unique_minimizers = unique_min;
+        st.minimizers_retained = retained_min;
+        st.ms_pass1 = ms_pass1;
         b.last = st;
         if (stats) *stats = st;
         return;
@@ -2463,7 +2714,17 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
     st.n_distinct = h[2] + h[3];
     st.n_overflow = h[3];
     st.n_out = n_real;
-    st.partition_bytes = p.n_records * (wide ? sizeof(Rec32) : 16);
+    st.partition_bytes = h[13];
+    if (p.want_tiles && p.n_tiles) {
+        std::vector<uint64_t> tf(p.n_tiles), tl(p.n_tiles);
+        MCZI_HIP(hipMemcpyAsync(tf.data(), p.tile_first.as<void>(), p.n_tiles * 8, hipMemcpyDeviceToHost, s));
+        MCZI_HIP(hipMemcpyAsync(tl.data(), p.tile_last.as<void>(), p.n_tiles * 8, hipMemcpyDeviceToHost, s));
+        MCZI_HIP(hipStreamSynchronize(s));
+        st.partition_bytes = (uint64_t)((int64_t)h[13] + join_tile_runs(tf, tl, k));
+    }
+    st.unique_minimizers = unique_min;
+    st.minimizers_retained = retained_min;
+    st.ms_pass1 = ms_pass1;
     st.internal_partitions = p.p_int;
     st.lds_slots = CNT_SLOTS;
     if (want_times && t_dev_end >= 7) {
@@ -2477,6 +2738,18 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
     st.ms_sort = tm.ms(tsort0, tsort0 + 1);
     b.last = st;
     if (stats) *stats = st;
+}
+
+uint64_t batch_minimizer_count(Batch &b, const mczi_counter_config &cfg) {
+    require(cfg.k % 2 == 1 && cfg.k <= 63, MCZI_E_INVALID,
+            "k must be odd: the canonical window length l = k must be odd (simd-minimizers)");
+    require(cfg.k - cfg.minimizer + 1 <= MAXW, MCZI_E_INVALID, "window k - m + 1 must be <= 64");
+    McPlan &p = *b.plan;
+    p.n_windows = windows_for(b, cfg.k);
+    uint64_t unique = 0, retained = 0;
+    double ms = 0;
+    minimizer_pass(b, p, cfg, false, MCZI_LAUNCH_BATCHED, unique, retained, ms);
+    return unique;
 }
 
 // Autotuner: sweep the expected k-mer instances per internal partition (the

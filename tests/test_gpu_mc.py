@@ -369,6 +369,41 @@ def test_fof_32_datasets_batch():
         assert st.n_out == len(want)
 
 
+@pytest.mark.parametrize("n_datasets", [33, 100])
+def test_fof_beyond_32_datasets(n_datasets):
+    """count_dataset_presence_counts_u64 (src/lib.rs:637-703) takes any number
+    of datasets: presence counts above 32 datasets equal brute force, in one
+    batch and through the file path."""
+    print(f"fof datasets={n_datasets}", flush=True)
+    r = M.SynthReads(M.SynthSpec(genome_len=40_000, genome_seed=4, read_seed=404, n_reads=60 * n_datasets,
+                                 read_len=150, sub_rate=0.005, n_datasets=n_datasets, snp_rate=0.002))
+    b = M.Batch.from_packed(r.words, r.frag_off, r.dataset)
+    assert b.info()[2] == n_datasets
+    codes = r.codes()
+    for x in (0, 5, 31, 32, n_datasets - 2):
+        st = b.count(M.CounterConfig(31, 21, x, 64), M.MODE_DATASETS)
+        want = O.count_datasets_bruteforce(codes, r.frag_off, r.dataset, n_datasets, 31, x)
+        assert same_counted(b.download(), want), x
+        assert st.n_out == len(want)
+
+
+def test_fof_file_path_40_datasets(tmp_path):
+    rng = np.random.default_rng(40)
+    base = "".join(rng.choice(list("ACGT"), size=3000))
+    paths, seqs, ds = [], [], []
+    for d in range(40):
+        rs = [base[p:p + 120] for p in rng.integers(0, 2880, size=30)]
+        p = tmp_path / f"r{d}.fa"
+        write_fasta(p, rs)
+        paths.append(p)
+        seqs += rs
+        ds += [d] * len(rs)
+    codes, off, dsa = O.frags_of(seqs, datasets=ds)
+    for x in (3, 33):
+        got = M.count_datasets(paths, M.CounterConfig(31, 21, x, 64))
+        assert same_counted(got, O.count_datasets_bruteforce(codes, off, dsa, 40, 31, x)), x
+
+
 def test_c1_full_config_kff_bytes(tmp_path):
     """MC abundance mode, plain method, small: 1 Mbp genome (seed 1), 200k
     150 bp reads, 0.5% substitutions, k=31, m=21, x=1, 64 partitions, KFF."""
