@@ -161,6 +161,17 @@ int mczi_decode_kmer(uint64_t lo, uint64_t hi, uint32_t k, char *out);
  * of runs (deduplicated selected hashes and the first window of each). */
 int mczi_minimizer_run_hashes(const uint8_t *packed, uint64_t n_bases, uint32_t k, uint32_t m,
                               uint32_t *hashes, uint32_t *starts, size_t *n_runs);
+/* No counterpart in the reference: the same as mczi_minimizer_run_hashes
+ * (same arguments, outputs and errors) for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops): the
+ * device memory of a call is a fixed array the library holds on each GPU it
+ * runs on (about 69 MiB, kept until the process ends), so no device
+ * allocation is made. n_bases is at most 16777216 (2^24): longer sequences
+ * are rejected with MCZI_E_INVALID. Calls on one GPU share that array; the
+ * library serializes them, and calls that bypass its lock and overlap on one
+ * GPU would race. */
+int mczi_minimizer_run_hashes_noalloc(const uint8_t *packed, uint64_t n_bases, uint32_t k, uint32_t m,
+                                      uint32_t *hashes, uint32_t *starts, size_t *n_runs);
 
 /* ---- batched device API ---------------------------------------------------
  * A device-resident segmented buffer of reads (many files / datasets packed

@@ -137,6 +137,8 @@ _sig("mczi_simplitig_sequences_parallel", C.c_int, [_P, C.c_size_t, C.c_uint32, 
                                                     C.POINTER(C.POINTER(C.c_char_p)), C.POINTER(C.c_size_t)])
 _sig("mczi_decode_kmer", C.c_int, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_char_p])
 _sig("mczi_minimizer_run_hashes", C.c_int, [_P, C.c_uint64, C.c_uint32, C.c_uint32, _P, _P, C.POINTER(C.c_size_t)])
+_sig("mczi_minimizer_run_hashes_noalloc", C.c_int, [_P, C.c_uint64, C.c_uint32, C.c_uint32, _P, _P,
+                                                    C.POINTER(C.c_size_t)])
 _sig("mczi_batch_from_files", C.c_int, [_CP, C.c_size_t, C.c_uint32, C.c_int, C.POINTER(_P)])
 _sig("mczi_pack_fastx", C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.POINTER(C.c_uint32)),
                                   C.POINTER(C.POINTER(C.c_uint64)), C.POINTER(C.c_size_t)])
@@ -191,7 +193,7 @@ EXPORTED = [
     "mczi_count_inputs_to_kmer_fasta_path", "mczi_count_datasets_to_kmer_fasta_path",
     "mczi_run_inputs_phase12_direct", "mczi_count_inputs_minimizer_phase_direct", "mczi_write_fasta_path",
     "mczi_write_fasta_after_index", "mczi_write_kff_path", "mczi_simplitig_sequences",
-    "mczi_simplitig_sequences_parallel", "mczi_decode_kmer", "mczi_minimizer_run_hashes",
+    "mczi_simplitig_sequences_parallel", "mczi_decode_kmer", "mczi_minimizer_run_hashes", "mczi_minimizer_run_hashes_noalloc",
     "mczi_batch_from_files", "mczi_pack_fastx", "mczi_batch_from_packed", "mczi_batch_from_packed_inputs",
     "mczi_write_read_cache",
     "mczi_read_read_cache", "mczi_write_superkmer_partitions", "mczi_read_superkmer_partitions",

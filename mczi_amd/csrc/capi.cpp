@@ -734,6 +734,23 @@ int mczi_minimizer_run_hashes(const uint8_t *packed, uint64_t n_bases, uint32_t 
     });
 }
 
+int mczi_minimizer_run_hashes_noalloc(const uint8_t *packed, uint64_t n_bases, uint32_t k, uint32_t m,
+                                      uint32_t *hashes, uint32_t *starts, size_t *n_runs) {
+    return guarded("failed to compute minimizer runs", [&] {
+        require(n_runs != nullptr, MCZI_E_INVALID, "null output pointer");
+        require(k >= 1 && k <= 64 && m >= 1 && m <= k && k - m + 1 <= 64, MCZI_E_INVALID,
+                "invalid k / m for the minimizer scan");
+        require(k % 2 == 1, MCZI_E_INVALID, "window length l = k must be odd");
+        require(n_bases <= NOALLOC_RUN_HASHES_MAX_BASES, MCZI_E_INVALID,
+                "the allocation-free minimizer scan takes at most 16777216 bases per call");
+        *n_runs = 0;
+        if (n_bases < k) return;
+        require(packed && hashes && starts, MCZI_E_INVALID, "null argument");
+        std::lock_guard<std::mutex> lk(Device::get().call_mutex());
+        device_run_hashes_noalloc(packed, n_bases, k, m, hashes, starts, n_runs);
+    });
+}
+
 int mczi_copy_probe(uint64_t bytes, int reps, double *ms_per_copy) {
     return guarded("copy probe failed", [&] {
         require(ms_per_copy != nullptr, MCZI_E_INVALID, "null output pointer");

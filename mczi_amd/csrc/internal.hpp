@@ -268,6 +268,12 @@ void free_plan(McPlan *p);
 // Plain device copy of `bytes` (average ms per launch over `reps`).
 double device_copy_probe(uint64_t bytes, int reps);
 
+// The same without any device allocation (a fixed file-scope scratch array;
+// n_bases <= 2^24, the caller checks the arguments).
+void device_run_hashes_noalloc(const uint8_t *packed, uint64_t n_bases, uint32_t k, uint32_t m, uint32_t *hashes,
+                               uint32_t *starts, size_t *n_runs);
+constexpr uint64_t NOALLOC_RUN_HASHES_MAX_BASES = 1ull << 24;
+
 // Device scan boundary for one sequence (run_hashes).
 void device_run_hashes(const uint8_t *packed, uint64_t n_bases, uint32_t k, uint32_t m,
                        std::vector<uint32_t> &hashes, std::vector<uint32_t> &starts);

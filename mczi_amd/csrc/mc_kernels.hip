@@ -2922,4 +2922,117 @@ void device_run_hashes(const uint8_t *packed, uint64_t n_bases, uint32_t k, uint
     }
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free run hashes (mczi_minimizer_run_hashes_noalloc): the device
+// memory of a call lives in a file-scope array of fixed size, so the call
+// makes no device allocation. In host code the array's name is only a
+// stand-in: each GPU's copy is reached through the address
+// hipGetSymbolAddress returns with that GPU current, fetched once per device
+// and kept. Calls on one GPU share its copy (the library's call lock
+// serializes them; concurrent use of the same GPU's copy outside it would
+// race).
+constexpr uint64_t NOALLOC_MAX_BASES = 1ull << 24;
+constexpr size_t na_align(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t NA_TILES = (NOALLOC_MAX_BASES + TILE - 1) / TILE;
+constexpr size_t NA_CTR = 0;                                        // 16 counters
+constexpr size_t NA_OFF = 256;                                      // fragment bounds {0, n}
+constexpr size_t NA_TS = 512;                                       // tile starts (u64)
+constexpr size_t NA_TF = NA_TS + na_align(NA_TILES * 8);            // tile fragments (u32)
+constexpr size_t NA_TL = NA_TF + na_align(NA_TILES * 4);            // tile lengths (u32)
+constexpr size_t NA_WORDS = NA_TL + na_align(NA_TILES * 4);         // packed words + 64 slack words
+constexpr size_t NA_WIN = NA_WORDS + na_align((NOALLOC_MAX_BASES / 16 + 64) * 4);  // selected hash per window
+constexpr size_t NA_BYTES = NA_WIN + na_align(NOALLOC_MAX_BASES * 4);
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+// The device address of this GPU's scratch, from hipGetSymbolAddress on the
+// first call with the device current.
+static unsigned char *noalloc_scratch(int dev) {
+    static std::mutex mu;
+    static unsigned char *ptr[64] = {};
+    require(dev >= 0 && dev < 64, MCZI_E_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[dev]) {
+        void *a = nullptr;
+        MCZI_HIP(hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)));
+        require(a != nullptr, MCZI_E_DEVICE, "no device address for the scratch array");
+        ptr[dev] = static_cast<unsigned char *>(a);
+    }
+    return ptr[dev];
+}
+
+// Counters zeroed, fragment bounds and the tile schedule of one sequence of n
+// bases, and the packed words from word w0 on zeroed (the copy of the input
+// fills the words below n; the rest is slack the scan reads).
+__global__ void k_noalloc_setup(unsigned char *base, uint64_t n, uint64_t n_tiles, uint64_t w0, uint64_t w1) {
+    unsigned long long *ctr = reinterpret_cast<unsigned long long *>(base + NA_CTR);
+    uint64_t *off = reinterpret_cast<uint64_t *>(base + NA_OFF);
+    uint64_t *ts = reinterpret_cast<uint64_t *>(base + NA_TS);
+    uint32_t *tf = reinterpret_cast<uint32_t *>(base + NA_TF);
+    uint32_t *tl = reinterpret_cast<uint32_t *>(base + NA_TL);
+    uint32_t *words = reinterpret_cast<uint32_t *>(base + NA_WORDS);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint64_t i = i0; i < 16; i += stride) ctr[i] = 0;
+    if (i0 == 0) {
+        off[0] = 0;
+        off[1] = n;
+    }
+    for (uint64_t t = i0; t < n_tiles; t += stride) {
+        ts[t] = t * TILE;
+        tf[t] = 0;
+        tl[t] = (uint32_t)min((uint64_t)TILE, n - t * TILE);
+    }
+    for (uint64_t w = w0 + i0; w < w1; w += stride) words[w] = 0;
+}
+
+void device_run_hashes_noalloc(const uint8_t *packed, uint64_t n_bases, uint32_t k, uint32_t m, uint32_t *hashes,
+                               uint32_t *starts, size_t *n_runs) {
+    Device &d = Device::get();
+    hipStream_t s = d.stream();
+    MCZI_HIP(hipSetDevice(d.device()));
+    unsigned char *base = noalloc_scratch(d.device());
+    const uint64_t nw = n_bases - k + 1;
+    const uint64_t n_tiles = (n_bases + TILE - 1) / TILE;
+    const uint64_t n_bytes = (n_bases + 3) / 4;
+    const uint64_t w1 = (n_bases + 15) / 16 + 64;
+    hipLaunchKernelGGL(k_noalloc_setup, dim3((unsigned)std::min<uint64_t>((w1 + 255) / 256 + 1, 1024)), dim3(256), 0,
+                       s, base, n_bases, n_tiles, n_bytes / 4, w1);
+    MCZI_HIP(hipMemcpyAsync(base + NA_WORDS, packed, n_bytes, hipMemcpyHostToDevice, s));
+    ScanArgs sa{};
+    sa.words = reinterpret_cast<const uint32_t *>(base + NA_WORDS);
+    sa.off = reinterpret_cast<const uint64_t *>(base + NA_OFF);
+    sa.tile_start = reinterpret_cast<const uint64_t *>(base + NA_TS);
+    sa.tile_frag = reinterpret_cast<const uint32_t *>(base + NA_TF);
+    sa.tile_len = reinterpret_cast<const uint32_t *>(base + NA_TL);
+    sa.tile_begin = 0;
+    sa.tile_end = n_tiles;
+    sa.n_frag = 1;
+    sa.k = (int)k;
+    sa.m = (int)m;
+    sa.w = (int)(k - m + 1);
+    sa.p_int = 1;
+    sa.wrec = 1;
+    sa.win_out = reinterpret_cast<uint32_t *>(base + NA_WIN);
+    sa.stats = reinterpret_cast<unsigned long long *>(base + NA_CTR);
+    const dim3 g((unsigned)std::min<uint64_t>(n_tiles, 2048));
+    if (sa.w >= WPT) hipLaunchKernelGGL((k_scan<WINDOWS, true>), g, dim3(SCAN_THREADS), 0, s, sa);
+    else hipLaunchKernelGGL((k_scan<WINDOWS, false>), g, dim3(SCAN_THREADS), 0, s, sa);
+    MCZI_HIP(hipGetLastError());
+    MCZI_HIP(hipMemcpyAsync(hashes, base + NA_WIN, nw * 4, hipMemcpyDeviceToHost, s));
+    MCZI_HIP(hipStreamSynchronize(s));
+    // runs of equal selected hash, in place (the write index never passes the read index)
+    size_t r = 0;
+    uint32_t prev = 0;
+    for (uint64_t j = 0; j < nw; j++) {
+        const uint32_t v = hashes[j];
+        if (j == 0 || v != prev) {
+            hashes[r] = v;
+            starts[r] = (uint32_t)j;
+            r++;
+        }
+        prev = v;
+    }
+    *n_runs = r;
+}
+
 }  // namespace mczi

@@ -4,6 +4,7 @@ Device entry points are not called here (no GPU in this container); the
 writers and checks below are host code and are compared byte for byte with
 the oracle's restatement of the reference writers.
 """
+import ctypes as C
 import os
 import re
 
@@ -184,3 +185,15 @@ def test_packed_input_marks_are_checked_before_device_work():
             M.Batch.from_packed(words, off, input_frag=np.array(marks, dtype=np.uint64))
     with pytest.raises(M.MczaError, match="fragment offsets must not decrease"):
         M.Batch.from_packed(words, np.array([0, 32, 16, 48], dtype=np.uint64), input_frag=np.array([0, 3]))
+
+
+def test_noalloc_run_hashes_rejects_oversize_and_short_inputs():
+    # argument checks run before any device work (no GPU needed)
+    h = np.zeros(4, dtype=np.uint32)
+    st = np.zeros(4, dtype=np.uint32)
+    r = C.c_size_t(99)
+    rc = M._lib.mczi_minimizer_run_hashes_noalloc(None, (1 << 24) + 1, 31, 21, h.ctypes.data, st.ctypes.data,
+                                                  C.byref(r))
+    assert rc == 1 and b"at most 16777216 bases" in M._lib.mczi_last_error()
+    rc = M._lib.mczi_minimizer_run_hashes_noalloc(None, 10, 31, 21, h.ctypes.data, st.ctypes.data, C.byref(r))
+    assert rc == 0 and r.value == 0
