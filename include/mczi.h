@@ -326,8 +326,9 @@ int mczi_zi_absent(const char *index_path, const char *const *query_paths, size_
  * No counterpart in the reference (which runs one query set per process).
  * Requests are FASTA/FASTQ text (or packed reads); each result is the sorted
  * absent set and, on request, its simplitig FASTA text. The absent arrays
- * are pinned host memory owned by the library: release the whole result
- * with mczi_zi_result_free (never free()). */
+ * are host memory owned by the library (pinned while the host lets the
+ * process pin more, else ordinary): release the whole result with
+ * mczi_zi_result_free (never free()). */
 typedef struct mczi_zi_server mczi_zi_server;
 typedef struct {
     uint64_t request_id;
@@ -360,6 +361,9 @@ int mczi_zi_server_poll(mczi_zi_server *s, uint64_t request_id, int wait,
 void mczi_zi_result_free(mczi_zi_result *r);
 int mczi_zi_server_stats(mczi_zi_server *s, uint64_t *index_kmers, uint64_t *completed,
                          uint64_t *lookups, double *busy_ms);
+/* Lookups done so far and the summed duration of the lookup kernels that did
+ * them (HIP events on the slot streams; no counterpart in the reference). */
+int mczi_zi_server_lookup_stats(mczi_zi_server *s, uint64_t *lookups, double *lookup_kernel_ms);
 void mczi_zi_server_destroy(mczi_zi_server *s);
 
 /* MCZI in one process (src/mczi.rs:238-451), without GGCAT and SSHash: the

@@ -177,6 +177,7 @@ _sig("mczi_zi_server_poll", C.c_int, [_P, C.c_uint64, C.c_int, C.POINTER(C.POINT
 _sig("mczi_zi_result_free", None, [C.POINTER(_ZiResult)])
 _sig("mczi_zi_server_stats", C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                        C.POINTER(C.c_double)])
+_sig("mczi_zi_server_lookup_stats", C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_double)])
 _sig("mczi_zi_server_destroy", None, [_P])
 _sig("mczi_synth_generate", C.c_int, [C.POINTER(_SynthSpec), C.POINTER(_SynthReads)])
 _sig("mczi_synth_genome", C.c_int, [C.c_uint64, C.c_uint64, C.POINTER(C.POINTER(C.c_uint32))])
@@ -201,7 +202,7 @@ EXPORTED = [
     "mczi_batch_download", "mczi_batch_autotune", "mczi_copy_probe", "mczi_device_info", "mczi_zi_subtract", "mczi_zi_absent", "mczi_mczi_subtract", "mczi_mczi_absent", "mczi_zi_server_create",
     "mczi_zi_server_create_from_text", "mczi_zi_server_submit", "mczi_zi_server_submit_packed",
     "mczi_zi_server_poll", "mczi_zi_result_free",
-    "mczi_zi_server_stats", "mczi_zi_server_destroy", "mczi_synth_generate", "mczi_synth_genome",
+    "mczi_zi_server_stats", "mczi_zi_server_lookup_stats", "mczi_zi_server_destroy", "mczi_synth_generate", "mczi_synth_genome",
     "mczi_synth_write_fasta", "mczi_synth_free", "mczi_synth_reads_from",
 ]
 

@@ -83,7 +83,7 @@ void Device::release(void *p) {
     live_.erase(it);
 }
 
-void *Device::host_alloc(size_t bytes) {
+void *Device::host_alloc(size_t bytes, bool allow_pageable) {
     size_t want = 64u << 10;
     while (want < bytes) want <<= 1;
     {
@@ -111,6 +111,13 @@ void *Device::host_alloc(size_t bytes) {
         e = hipHostMalloc(&p, want, hipHostMallocDefault);
         if (e != hipSuccess) {
             (void)hipGetLastError();
+            if (allow_pageable) {
+                p = malloc(want);
+                if (!p) throw Error(MCZI_E_NOMEM, "host allocation of " + std::to_string(want) + " bytes failed");
+                std::lock_guard<std::mutex> lk(hmu_);
+                hpage_[p] = want;
+                return p;
+            }
             throw Error(MCZI_E_NOMEM, "pinned host allocation of " + std::to_string(want) +
                                           " bytes failed: " + hipGetErrorString(e));
         }
@@ -223,7 +230,14 @@ void Device::host_release(void *p) {
     if (!p) return;
     std::lock_guard<std::mutex> lk(hmu_);
     auto it = hlive_.find(p);
-    if (it == hlive_.end()) return;
+    if (it == hlive_.end()) {
+        auto pg = hpage_.find(p);
+        if (pg != hpage_.end()) {
+            hpage_.erase(pg);
+            free(p);
+        }
+        return;
+    }
     const size_t n = it->second;
     hlive_.erase(it);
     if (hcached_ + n > kPinnedCacheBytes) {

@@ -90,7 +90,10 @@ class Device {
     // Cached pinned host allocation (hipHostMalloc, power-of-two size
     // classes): device <-> host copies into it run as plain DMA. Released
     // blocks stay cached up to kPinnedCacheBytes.
-    void *host_alloc(size_t bytes);
+    // With allow_pageable, a request the host cannot pin (a process may pin
+    // only so much) is served from ordinary memory instead of failing;
+    // host_release frees either kind.
+    void *host_alloc(size_t bytes, bool allow_pageable = false);
     void host_release(void *p);
     // Host -> device copy of pageable memory: the source is copied by
     // several host threads into two alternating pinned staging blocks, each
@@ -115,6 +118,7 @@ class Device {
     std::mutex hmu_;
     std::multimap<size_t, void *> hfree_;
     std::map<void *, size_t> hlive_;
+    std::map<void *, size_t> hpage_;  // ordinary memory handed out in place of pinned
     size_t hcached_ = 0;
 };
 

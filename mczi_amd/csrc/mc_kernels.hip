@@ -68,7 +68,7 @@ constexpr int CNT_THREADS = 256;
 constexpr int CNT_LOG2_SLOTS = 11;
 constexpr int CNT_SLOTS = 1 << CNT_LOG2_SLOTS;  // LDS table slots (u64 key + u32 value)
 constexpr int CNT_MAXPROBE = 32;
-constexpr int OVF_MAXPROBE = 1 << 16;
+constexpr int OVF_MAXPROBE = 1 << 12;  // at most half-full tables keep probe runs far shorter
 
 enum ScanMode {
     STAGE = 0,    // super-k-mer records + partition ids (or only their count)
@@ -669,6 +669,44 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
 }
 
 // ---------------------------------------------------------------------------
+// Partition-file bytes of runs a tile edge cut into pieces (the scan counted
+// each piece as a super-k-mer; the reference writes one per run,
+// add_packed_range, src/lib.rs:4893-4917). Each chain of pieces joined across tile
+// edges is walked by the thread of its first edge; the correction (bytes of
+// the joined run minus those of its pieces) is added to *corr as a two's
+// complement 64-bit value.
+__global__ void k_tile_join(const unsigned long long *first, const unsigned long long *last, uint64_t n_tiles,
+                            uint32_t k, unsigned long long *corr) {
+    constexpr unsigned long long NONE = ~0ull, SINGLE = 0x80000000ull;
+    auto joins = [&](uint64_t t) {  // the run of tile t - 1's last piece goes on in tile t
+        const unsigned long long L = last[t - 1], F = first[t];
+        return L != NONE && F != NONE && (L >> 32) == (F >> 32);
+    };
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x + 1; t < n_tiles;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        if (!joins(t)) continue;
+        // not the first edge of its chain if tile t - 1 is one piece joined from t - 2
+        if (t >= 2 && (first[t - 1] & SINGLE) && first[t - 1] != NONE && joins(t - 1)) continue;
+        uint64_t total = last[t - 1] & 0x7fffffffull;
+        uint64_t parts = (total + k - 1 < 255 ? 1 : 5) + (total + k - 1 + 3) / 4;
+        uint64_t j = t;
+        for (;;) {
+            const uint64_t w = first[j] & 0x7fffffffull;
+            total += w;
+            parts += (w + k - 1 < 255 ? 1 : 5) + (w + k - 1 + 3) / 4;
+            if ((first[j] & SINGLE) && last[j] != NONE && j + 1 < n_tiles && joins(j + 1)) {
+                j++;
+                continue;
+            }
+            break;
+        }
+        const uint64_t L = total + k - 1;
+        const long long d = (long long)((L < 255 ? 1 : 5) + (L + 3) / 4) - (long long)parts;
+        if (d) atomicAdd(corr, (unsigned long long)d);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Pass 1 from the run entries of the RUNS scan, sorted by key (fof: by key,
 // then dataset). A key's entries are contiguous. It passes when its windows
 // sum above x (the saturating count min(x + 1, sum of min(run, 255)) > x,
@@ -1079,9 +1117,20 @@ __device__ __forceinline__ void val_update(uint32_t *p, uint32_t v) {
     else atomicOr(p, v);
 }
 
+// The spill table takes new keys only while it is at most half full; past
+// that every spill fails fast with error bit 1 and the host counts again with
+// smaller partitions (and a larger table), instead of probing a full table.
+__device__ __forceinline__ bool spill_table_full(const CountArgs &a) {
+    return __hip_atomic_load(&a.ctr[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > (a.ovf_mask + 1) / 2;
+}
+
 template <int MODE>
 __device__ void overflow_insert(const CountArgs &a, unsigned long long key, uint32_t val,
                                 uint32_t q) {
+    if (spill_table_full(a)) {
+        atomicOr(&a.ctr[4], 1ull);
+        return;
+    }
     uint64_t slot = spread_hash_u64(key ^ 0x9e3779b97f4a7c15ull) & a.ovf_mask;
     for (int p = 0; p < OVF_MAXPROBE; p++) {
         const unsigned long long cur =
@@ -1413,7 +1462,8 @@ __device__ __forceinline__ void canon128_mc(uint64_t a, uint64_t b, int k, uint6
 // claimer's release ahead of the waiters' next look.
 __device__ void overflow_insert_w(const CountArgs &a, uint64_t klo, uint64_t khi, uint32_t q) {
     uint64_t slot = spread_hash_u64(klo ^ spread_hash_u64(khi + 0x9e3779b97f4a7c15ull)) & a.ovf_mask;
-    bool done = false;
+    bool done = spill_table_full(a);
+    if (done) atomicOr(&a.ctr[4], 1ull);
     int probes = 0, waits = 0;
     while (true) {
         if (!done) {
@@ -1724,6 +1774,7 @@ struct McPlan {
     uint64_t out_cap = 0;
     uint64_t rec_sig = 0;
     double target_override = 0;
+    double effective_target = 0;  // instances per partition of the last count (after spill retries)
     // graph cache
     hipGraphExec_t graph = nullptr;
     uint64_t graph_sig = 0;
@@ -2030,6 +2081,10 @@ static void enqueue_stage_a(Batch &b, McPlan &p, const mczi_counter_config &cfg,
     }
     if (tm) tm->mark(s);
     launch_scan(p, sa, launch, s);
+    if (p.want_tiles && p.n_tiles > 1)
+        hipLaunchKernelGGL(k_tile_join, dim3(grid_for(p.n_tiles, 256, 4096)), dim3(256), 0, s,
+                           p.tile_first.as<unsigned long long>(), p.tile_last.as<unsigned long long>(), p.n_tiles,
+                           (uint32_t)cfg.k, ctr + 9);
     if (tm) tm->mark(s);
 }
 
@@ -2426,6 +2481,7 @@ static void count_pass(Batch &b, McPlan &p, const mczi_counter_config &cfg, int 
             target /= 4;
             continue;
         }
+        p.effective_target = target;
         return;
     }
 }
@@ -2600,39 +2656,6 @@ static void minimizer_pass(Batch &b, McPlan &p, const mczi_counter_config &cfg, 
     (void)hipEventDestroy(e1);
 }
 
-static uint64_t skm_bytes_host(uint64_t L) { return (L < 255 ? 1 : 5) + (L + 3) / 4; }
-
-// The scan sums each staged piece's partition-file bytes; a run that a tile
-// edge cut into pieces is one super-k-mer in the reference's files
-// (add_packed_range, src/lib.rs:4893-4917). Joins such pieces from the tile
-// edge entries, tile by tile in stream order, and returns the correction.
-static int64_t join_tile_runs(const std::vector<uint64_t> &first, const std::vector<uint64_t> &last, uint32_t k) {
-    constexpr uint64_t NONE = ~0ull, SINGLE = 0x80000000ull;
-    int64_t corr = 0;
-    bool open = false;
-    uint32_t okey = 0;
-    uint64_t total = 0, parts = 0;
-    for (size_t t = 0; t < first.size(); t++) {
-        const uint64_t F = first[t], L = last[t];
-        if (open && F != NONE && (uint32_t)(F >> 32) == okey) {
-            const uint64_t w = F & 0x7fffffffull;
-            total += w;
-            parts += skm_bytes_host(w + k - 1);
-            if ((F & SINGLE) && L != NONE) continue;  // the tile's only piece runs on into the next tile
-        }
-        if (open) corr += (int64_t)skm_bytes_host(total + k - 1) - (int64_t)parts;
-        open = false;
-        if (L != NONE) {
-            open = true;
-            okey = (uint32_t)(L >> 32);
-            total = L & 0x7fffffffull;
-            parts = skm_bytes_host(total + k - 1);
-        }
-    }
-    if (open) corr += (int64_t)skm_bytes_host(total + k - 1) - (int64_t)parts;
-    return corr;
-}
-
 void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch, int order,
                  mczi_run_stats *stats) {
     Device &d = Device::get();
@@ -2663,8 +2686,8 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
         minimizer_pass(b, p, cfg, mode == MCZI_MODE_DATASETS, launch, unique_min, retained_min, ms_pass1);
         p.use_mtab = true;
     }
-    // exact partition bytes: per-tile edge entries, with stats, one dataset group
-    p.want_tiles = stats != nullptr && !(mode == MCZI_MODE_DATASETS && b.n_datasets > 32);
+    // exact partition bytes: per-tile edge entries joined on the device (one dataset group)
+    p.want_tiles = !(mode == MCZI_MODE_DATASETS && b.n_datasets > 32);
     if (p.want_tiles) {
         p.tile_first.ensure(std::max<uint64_t>(p.n_tiles, 1) * 8);
         p.tile_last.ensure(std::max<uint64_t>(p.n_tiles, 1) * 8);
@@ -2714,14 +2737,7 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
     st.n_distinct = h[2] + h[3];
     st.n_overflow = h[3];
     st.n_out = n_real;
-    st.partition_bytes = h[13];
-    if (p.want_tiles && p.n_tiles) {
-        std::vector<uint64_t> tf(p.n_tiles), tl(p.n_tiles);
-        MCZI_HIP(hipMemcpyAsync(tf.data(), p.tile_first.as<void>(), p.n_tiles * 8, hipMemcpyDeviceToHost, s));
-        MCZI_HIP(hipMemcpyAsync(tl.data(), p.tile_last.as<void>(), p.n_tiles * 8, hipMemcpyDeviceToHost, s));
-        MCZI_HIP(hipStreamSynchronize(s));
-        st.partition_bytes = (uint64_t)((int64_t)h[13] + join_tile_runs(tf, tl, k));
-    }
+    st.partition_bytes = h[13] + h[9];  // piece bytes + the join correction (two's complement)
     st.unique_minimizers = unique_min;
     st.minimizers_retained = retained_min;
     st.ms_pass1 = ms_pass1;
@@ -2757,17 +2773,20 @@ uint64_t batch_minimizer_count(Batch &b, const mczi_counter_config &cfg) {
 double batch_autotune(Batch &b, const mczi_counter_config &cfg, int mode, int launch, int order) {
     if (!b.plan) b.plan.reset(new McPlan());
     McPlan &p = *b.plan;
-    const double cands[] = {0.5, 1.0, 2.0, 4.0, 8.0};
+    // a candidate whose tables spill past half the spill table is retried by
+    // the count at a quarter of its size; the size that ran is what counts
+    const double cands[] = {0.5, 1.0, 2.0, 4.0};
     double best = 0, best_ms = 1e300;
     for (double c : cands) {
         p.target_override = c * CNT_SLOTS;
         mczi_run_stats st{};
         batch_count(b, cfg, mode, launch, order, &st);  // calibrates / captures
+        p.target_override = p.effective_target;
         batch_count(b, cfg, mode, launch, order, &st);
         const double ms = st.ms_total + st.ms_sort;
         if (ms < best_ms) {
             best_ms = ms;
-            best = p.target_override;
+            best = p.effective_target;
         }
     }
     p.target_override = best;

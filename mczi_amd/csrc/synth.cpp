@@ -145,44 +145,76 @@ void synth_generate(const mczi_synth_spec &spec, mczi_synth_reads &out) {
             }
         }
     }
-    // generate in parallel, each read from its own seed
+    // generate in parallel, each read from its own seed (contiguous read
+    // ranges per thread, bases kept in a per-thread buffer)
     const size_t nr = reads.size();
-    std::vector<std::vector<uint8_t>> seqs(nr);
-    unsigned nt = host_workers(size_t(1) << 20);
-    std::vector<std::thread> th;
-    for (unsigned t = 0; t < nt; t++) {
-        th.emplace_back([&, t]() {
-            std::vector<uint8_t> tmp;
-            for (size_t i = t; i < nr; i += nt) {
-                SplitMix64 rng(read_seed(spec.read_seed, i));
-                const bool is_long = reads[i].len != spec.read_len || spec.n_reads == 0;
-                (void)is_long;
-                sample_read(genomes[spec.n_datasets > 0 ? reads[i].dataset : 0], reads[i].len,
-                            spec.sub_rate, spec.ins_rate, spec.del_rate, rng, tmp);
-                seqs[i] = tmp;
-            }
-        });
+    const unsigned nt = std::max(1u, host_workers(size_t(1) << 20));
+    std::vector<std::vector<uint8_t>> tbuf(nt);
+    std::vector<uint64_t> rlen(nr);
+    auto rng_of = [&](unsigned t, size_t &b, size_t &e) {
+        b = nr * t / nt;
+        e = nr * (t + 1) / nt;
+    };
+    {
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nt; t++) {
+            th.emplace_back([&, t]() {
+                size_t b, e;
+                rng_of(t, b, e);
+                std::vector<uint8_t> tmp;
+                for (size_t i = b; i < e; i++) {
+                    SplitMix64 rng(read_seed(spec.read_seed, i));
+                    sample_read(genomes[spec.n_datasets > 0 ? reads[i].dataset : 0], reads[i].len, spec.sub_rate,
+                                spec.ins_rate, spec.del_rate, rng, tmp);
+                    rlen[i] = tmp.size();
+                    tbuf[t].insert(tbuf[t].end(), tmp.begin(), tmp.end());
+                }
+            });
+        }
+        for (auto &x : th) x.join();
     }
-    for (auto &x : th) x.join();
-    uint64_t total = 0;
-    for (auto &s : seqs) total += s.size();
-    out.n_bases = total;
     out.n_frag = nr;
-    out.n_words = (total + 15) / 16 + 32;
-    out.words = (uint32_t *)calloc(out.n_words, 4);
     out.frag_off = (uint64_t *)malloc((nr + 1) * 8);
     out.dataset = (uint32_t *)malloc(std::max<size_t>(nr, 1) * 4);
-    require(out.words && out.frag_off && out.dataset, MCZI_E_NOMEM, "synthetic reads: out of memory");
-    uint64_t p = 0;
+    require(out.frag_off && out.dataset, MCZI_E_NOMEM, "synthetic reads: out of memory");
     out.frag_off[0] = 0;
     for (size_t i = 0; i < nr; i++) {
-        for (uint8_t c : seqs[i]) {
-            out.words[p >> 4] |= (uint32_t)c << ((p & 15u) * 2u);
-            p++;
-        }
-        out.frag_off[i + 1] = p;
+        out.frag_off[i + 1] = out.frag_off[i] + rlen[i];
         out.dataset[i] = reads[i].dataset;
-        std::vector<uint8_t>().swap(seqs[i]);
+    }
+    const uint64_t total = out.frag_off[nr];
+    out.n_bases = total;
+    out.n_words = (total + 15) / 16 + 32;
+    out.words = (uint32_t *)calloc(out.n_words, 4);
+    require(out.words != nullptr, MCZI_E_NOMEM, "synthetic reads: out of memory");
+    // pack in parallel: each thread's bases are one contiguous range; only the
+    // words at its two ends can be shared with a neighbour (atomic OR there)
+    {
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nt; t++) {
+            th.emplace_back([&, t]() {
+                size_t b, e;
+                rng_of(t, b, e);
+                const uint64_t p0 = out.frag_off[b], p1 = out.frag_off[e];
+                if (p0 == p1) return;
+                const uint64_t wf = p0 >> 4, wl = (p1 - 1) >> 4;
+                const std::vector<uint8_t> &src = tbuf[t];
+                uint32_t acc = 0;
+                uint64_t w = wf;
+                for (uint64_t p = p0; p < p1; p++) {
+                    if ((p >> 4) != w) {
+                        if (w == wf || w == wl) __atomic_fetch_or(&out.words[w], acc, __ATOMIC_RELAXED);
+                        else out.words[w] = acc;
+                        w = p >> 4;
+                        acc = 0;
+                    }
+                    acc |= (uint32_t)src[p - p0] << ((p & 15u) * 2u);
+                }
+                __atomic_fetch_or(&out.words[w], acc, __ATOMIC_RELAXED);
+                std::vector<uint8_t>().swap(tbuf[t]);
+            });
+        }
+        for (auto &x : th) x.join();
     }
 }
 

@@ -538,14 +538,21 @@ struct ZiWork {
     std::vector<uint64_t> h_base, h_count, h_start, h_absent;  // per request
     uint64_t lookups = 0;
     hipEvent_t done = nullptr;
+    hipEvent_t l0 = nullptr, l1 = nullptr;  // around the lookup kernel
+    bool timed = false;
+    double lookup_ms = 0;  // the lookup kernel's duration, known after zi_finish
     ~ZiWork() {
         if (done) (void)hipEventDestroy(done);
+        if (l0) (void)hipEventDestroy(l0);
+        if (l1) (void)hipEventDestroy(l1);
     }
 };
 
 std::unique_ptr<ZiWork> zi_make_work() {
     auto w = std::make_unique<ZiWork>();
     MCZI_HIP(hipEventCreateWithFlags(&w->done, hipEventDisableTiming));
+    MCZI_HIP(hipEventCreate(&w->l0));
+    MCZI_HIP(hipEventCreate(&w->l1));
     return w;
 }
 void zi_free_work(ZiWork *w) { delete w; }
@@ -632,10 +639,13 @@ void zi_enqueue(ZiWork &w, const ZiIndex &idx, ZiBatchInput &&in, uint32_t n_req
     a.out_lo = w.olo.as<unsigned long long>();
     a.out_hi = wide ? w.ohi.as<unsigned long long>() : nullptr;
     a.ctr = w.ctr.as<unsigned long long>();
+    w.timed = a.n_tiles != 0;
     if (a.n_tiles) {
         const unsigned g = (unsigned)std::min<uint64_t>(a.n_tiles, (uint64_t)Device::get().cu_count() * 8);
+        MCZI_HIP(hipEventRecord(w.l0, s));
         if (wide) hipLaunchKernelGGL(k_zi_lookup<true>, dim3(g), dim3(ZI_THREADS), 0, s, a);
         else hipLaunchKernelGGL(k_zi_lookup<false>, dim3(g), dim3(ZI_THREADS), 0, s, a);
+        MCZI_HIP(hipEventRecord(w.l1, s));
     }
     MCZI_HIP(hipEventRecord(w.done, s));
 }
@@ -720,6 +730,12 @@ double zi_finish(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s) {
     MCZI_HIP(hipEventSynchronize(w.done));
     const double wait_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tw).count();
     MCZI_HIP(hipGetLastError());
+    w.lookup_ms = 0;
+    if (w.timed) {
+        float f = 0;
+        MCZI_HIP(hipEventElapsedTime(&f, w.l0, w.l1));
+        w.lookup_ms = f;
+    }
     const bool wide = idx.wide;
     std::vector<uint64_t> h_cursor(n_req + 1, 0);
     unsigned long long lk = 0;
@@ -795,8 +811,8 @@ void zi_copy_out(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, Z
         const uint64_t c = w.h_count[r];
         ZiKeys &kk = out.absent[r];
         kk.n = c;
-        kk.lo = (uint64_t *)Device::get().host_alloc(std::max<uint64_t>(c, 1) * 8);
-        if (wide) kk.hi = (uint64_t *)Device::get().host_alloc(std::max<uint64_t>(c, 1) * 8);
+        kk.lo = (uint64_t *)Device::get().host_alloc(std::max<uint64_t>(c, 1) * 8, true);
+        if (wide) kk.hi = (uint64_t *)Device::get().host_alloc(std::max<uint64_t>(c, 1) * 8, true);
         require(kk.lo && (!wide || kk.hi), MCZI_E_NOMEM, "out of host memory for ZI results");
         if (c)
             MCZI_HIP(hipMemcpyAsync(kk.lo, w.ulo.as<unsigned long long>() + w.h_start[r], c * 8,
@@ -823,14 +839,17 @@ static double now_ms() {
 struct ZiRequest {
     uint64_t id = 0;
     HostReads reads;        // fragment offsets (the words live in `pin`)
-    uint32_t *pin = nullptr;  // packed words in pinned host memory
+    uint32_t *pin = nullptr;  // packed words in pinned host memory (ordinary memory past the pin limit)
     ~ZiRequest() {
         if (pin) Device::get().host_release(pin);
     }
     double alloc_ms = 0;  // pinned allocation time of the last pin_words
+    // The request's words copied out of the caller's memory at submit, into
+    // pinned memory (plain DMA at admission) while the host lets the process
+    // pin more, else into ordinary memory (the runtime stages that upload).
     void pin_words(const uint32_t *src, uint64_t n_words) {
         const double t0 = now_ms();
-        pin = (uint32_t *)Device::get().host_alloc(std::max<uint64_t>(n_words, 1) * 4);
+        pin = (uint32_t *)Device::get().host_alloc(std::max<uint64_t>(n_words, 1) * 4, true);
         alloc_ms = now_ms() - t0;
         const size_t grain = 1u << 20;  // words per worker chunk (4 MiB)
         parallel_ranges(n_words, grain, host_workers(n_words), [&](size_t b, size_t e) {
@@ -885,6 +904,7 @@ struct mczi_zi_server {
     std::thread admit_thread, finish_thread, done_thread;
     uint64_t completed = 0, lookups = 0;
     double busy_ms = 0;
+    double lookup_kernel_ms = 0;  // summed duration of the lookup kernels
     const bool trace = getenv("MCZI_ZI_TRACE") != nullptr;  // per-batch host timings on stderr
     std::string fatal;
 
@@ -1077,6 +1097,7 @@ void mczi_zi_server::done_loop() {
         lk.lock();
         busy_ms += t1 - t0 + sort_ms;
         lookups += sl->work && !err ? sl->work->lookups : 0;
+        lookup_kernel_ms += sl->work && !err ? sl->work->lookup_ms : 0;
         const double t = now_ms();
         for (auto *r : batch) {
             if (err) {
@@ -1454,6 +1475,14 @@ int mczi_zi_server_stats(mczi_zi_server *s, uint64_t *index_kmers, uint64_t *com
     if (completed) *completed = s->completed;
     if (lookups) *lookups = s->lookups;
     if (busy_ms) *busy_ms = s->busy_ms;
+    return MCZI_OK;
+}
+
+int mczi_zi_server_lookup_stats(mczi_zi_server *s, uint64_t *lookups, double *lookup_kernel_ms) {
+    if (!s) return MCZI_E_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (lookups) *lookups = s->lookups;
+    if (lookup_kernel_ms) *lookup_kernel_ms = s->lookup_kernel_ms;
     return MCZI_OK;
 }
 

@@ -30,6 +30,7 @@
 #include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <time.h>
 #include <string.h>
 
 typedef unsigned __int128 u128;
@@ -1194,6 +1195,50 @@ int or_zi_absent(const or_frags *index, const or_frags *query, int k, or_counted
     free(qv);
     *out = res;
     *n_out = r;
+    return 0;
+}
+
+/* ZI's query loop on one thread (collect_query_kmers_absent_from_index,
+ * src/zi.rs:152-175): the index k-mers go into a hash set, then every query
+ * window's canonical k-mer is looked up and the absent ones are inserted into
+ * a second set. The two phases are timed apart (*build_s, *query_s); this is
+ * the CPU baseline of the served lookups, and it returns the same absent set
+ * size as or_zi_absent. */
+static double or_now(void) {
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+int or_zi_lookup_timed(const or_frags *index, const or_frags *query, int k, uint64_t *lookups,
+                       uint64_t *absent_unique, double *build_s, double *query_s) {
+    const double t0 = or_now();
+    size_t mx = max_frag_len(index);
+    if (max_frag_len(query) > mx) mx = max_frag_len(query);
+    u128 *cw = (u128 *)xmalloc((mx + 1) * sizeof(u128));
+    map128 idx;
+    map_init(&idx, 1024);
+    for (size_t i = 0; i < index->nfrag; i++) {
+        size_t n = canon_windows(index->codes + index->off[i], index->off[i + 1] - index->off[i], k, cw);
+        for (size_t w = 0; w < n; w++) *map_slot(&idx, cw[w], 1) = 1;
+    }
+    const double t1 = or_now();
+    map128 absent;
+    map_init(&absent, 1024);
+    uint64_t nl = 0;
+    for (size_t i = 0; i < query->nfrag; i++) {
+        size_t n = canon_windows(query->codes + query->off[i], query->off[i + 1] - query->off[i], k, cw);
+        nl += n;
+        for (size_t w = 0; w < n; w++)
+            if (!map_slot(&idx, cw[w], 0)) *map_slot(&absent, cw[w], 1) = 1;
+    }
+    const double t2 = or_now();
+    *lookups = nl;
+    *absent_unique = absent.used;
+    *build_s = t1 - t0;
+    *query_s = t2 - t1;
+    map_free(&idx);
+    map_free(&absent);
+    free(cw);
     return 0;
 }
 

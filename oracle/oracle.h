@@ -80,6 +80,10 @@ int or_count_datasets_bruteforce(const or_frags *f, size_t n_datasets, int k,
 /* ---- ZI ---------------------------------------------------------------- */
 int or_zi_absent(const or_frags *index, const or_frags *query, int k,
                  or_counted **out, size_t *n_out);
+/* ZI's one-thread query loop (hash-set index, per-window lookups), with the
+ * index build and the lookups timed apart. */
+int or_zi_lookup_timed(const or_frags *index, const or_frags *query, int k, uint64_t *lookups,
+                       uint64_t *absent_unique, double *build_s, double *query_s);
 
 /* ---- writers ----------------------------------------------------------- */
 /* Simplitig FASTA (sorted input). Returns index of last simplitig written. */

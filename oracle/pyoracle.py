@@ -75,6 +75,8 @@ def lib() -> C.CDLL:
                                                    C.POINTER(C.POINTER(_Counted)), C.POINTER(C.c_size_t)]
         L.or_zi_absent.argtypes = [C.POINTER(_Frags), C.POINTER(_Frags), C.c_int,
                                    C.POINTER(C.POINTER(_Counted)), C.POINTER(C.c_size_t)]
+        L.or_zi_lookup_timed.argtypes = [C.POINTER(_Frags), C.POINTER(_Frags), C.c_int, C.POINTER(C.c_uint64),
+                                         C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.or_write_simplitigs.argtypes = [C.c_char_p, C.c_void_p, C.c_size_t, C.c_int, C.c_size_t, C.POINTER(C.c_size_t)]
         L.or_write_kmer_fasta.argtypes = [C.c_char_p, C.c_void_p, C.c_size_t, C.c_int]
         L.or_write_kff.argtypes = [C.c_char_p, C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int]
@@ -244,6 +246,17 @@ def zi_absent(icodes, ioff, qcodes, qoff, k) -> np.ndarray:
     rc = lib().or_zi_absent(C.byref(fi), C.byref(fq), k, C.byref(p), C.byref(n))
     assert rc == 0
     return _take_counted(p, n.value)
+
+
+def zi_lookup_timed(icodes, ioff, qcodes, qoff, k):
+    """(lookups, unique absent, index build s, query s) of ZI's one-thread
+    query loop."""
+    fi, k1 = _frags(icodes, ioff)
+    fq, k2 = _frags(qcodes, qoff)
+    nl, na = C.c_uint64(), C.c_uint64()
+    b, q = C.c_double(), C.c_double()
+    assert lib().or_zi_lookup_timed(C.byref(fi), C.byref(fq), k, C.byref(nl), C.byref(na), C.byref(b), C.byref(q)) == 0
+    return nl.value, na.value, b.value, q.value
 
 
 def write_simplitigs(path, kmers: np.ndarray, k, first_index=0) -> int:

@@ -405,3 +405,16 @@ def test_spot_counts_match_bruteforce():
         f = 3
         p0 = int(off[f])
         assert O.kmers_at(words, [p0], k)[0] == O.encode_canonical("".join("ACTG"[c] for c in codes[p0:p0 + k]))
+
+
+def test_zi_lookup_loop_matches_set_difference():
+    rng = np.random.default_rng(77)
+    g = "".join(rng.choice(list("ACGT"), size=4000))
+    other = "".join(rng.choice(list("ACGT"), size=4000))
+    qs = [g[p:p + 150] for p in rng.integers(0, 3850, size=60)] + [other[p:p + 150] for p in rng.integers(0, 3850, size=60)]
+    ic, io, _ = O.frags_of([g])
+    qc, qo, _ = O.frags_of(qs)
+    for k in (21, 31, 45):
+        nl, na, _, _ = O.zi_lookup_timed(ic, io, qc, qo, k)
+        assert nl == 120 * (150 - k + 1)
+        assert na == len(O.zi_absent(ic, io, qc, qo, k))
