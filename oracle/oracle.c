@@ -1498,6 +1498,98 @@ static void *spot_worker(void *arg) {
     return NULL;
 }
 
+/* The same exact occurrence count for 33 <= k <= 64 (u128 canonical k-mers,
+ * queries given as lo / hi words), over fragments split among threads. */
+typedef struct {
+    const uint32_t *words;
+    const uint64_t *off;
+    size_t f0, f1;
+    int k;
+    const u128 *tkey;
+    const int64_t *tidx;
+    uint64_t tmask;
+    uint64_t *counts;
+} spot128_job;
+
+static void *spot128_worker(void *arg) {
+    spot128_job *j = (spot128_job *)arg;
+    const int k = j->k;
+    const u128 mask = kmask128(k);
+    for (size_t f = j->f0; f < j->f1; f++) {
+        const uint64_t s = j->off[f], e = j->off[f + 1];
+        if (e - s < (uint64_t)k) continue;
+        u128 fwd = 0, rev = 0;
+        for (uint64_t p = s; p < e; p++) {
+            const uint32_t code = (j->words[p >> 4] >> ((p & 15) * 2)) & 3u;
+            const u128 b = (u128)(code ^ (code >> 1));
+            fwd = ((fwd << 2) | b) & mask;
+            rev = (rev >> 2) | ((b ^ 3u) << (2 * (k - 1)));
+            if (p + 1 - s < (uint64_t)k) continue;
+            const u128 key = fwd < rev ? fwd : rev;
+            uint64_t h = h128(key) & j->tmask;
+            while (j->tidx[h] >= 0) {
+                if (j->tkey[h] == key) {
+                    j->counts[j->tidx[h]]++;
+                    break;
+                }
+                h = (h + 1) & j->tmask;
+            }
+        }
+    }
+    return NULL;
+}
+
+int or_spot_counts128(const uint32_t *words, const uint64_t *off, size_t n_frag, int k, const uint64_t *qlo,
+                      const uint64_t *qhi, size_t n_query, uint64_t *counts, int threads) {
+    if (k < 1 || k > 64 || threads < 1) return 1;
+    uint64_t slots = 16;
+    while (slots < 4 * (uint64_t)n_query) slots <<= 1;
+    u128 *tkey = (u128 *)xmalloc(slots * sizeof(u128));
+    int64_t *tidx = (int64_t *)xmalloc(slots * 8);
+    for (uint64_t i = 0; i < slots; i++) tidx[i] = -1;
+    for (size_t q = 0; q < n_query; q++) {
+        const u128 key = ((u128)qhi[q] << 64) | qlo[q];
+        uint64_t h = h128(key) & (slots - 1);
+        while (tidx[h] >= 0 && tkey[h] != key) h = (h + 1) & (slots - 1);
+        if (tidx[h] < 0) {
+            tkey[h] = key;
+            tidx[h] = (int64_t)q;
+        }
+    }
+    spot128_job *jobs = (spot128_job *)xmalloc(sizeof(spot128_job) * (size_t)threads);
+    pthread_t *th = (pthread_t *)xmalloc(sizeof(pthread_t) * (size_t)threads);
+    const uint64_t total = off[n_frag] - off[0];
+    size_t f = 0;
+    for (int t = 0; t < threads; t++) {
+        spot128_job *j = &jobs[t];
+        j->words = words; j->off = off; j->k = k;
+        j->tkey = tkey; j->tidx = tidx; j->tmask = slots - 1;
+        j->counts = (uint64_t *)calloc(n_query ? n_query : 1, 8);
+        j->f0 = f;
+        const uint64_t goal = off[0] + total / (uint64_t)threads * (uint64_t)(t + 1);
+        while (f < n_frag && (t == threads - 1 || off[f] < goal)) f++;
+        j->f1 = f;
+        pthread_create(&th[t], NULL, spot128_worker, j);
+    }
+    for (size_t q = 0; q < n_query; q++) counts[q] = 0;
+    for (int t = 0; t < threads; t++) {
+        pthread_join(th[t], NULL);
+        for (size_t q = 0; q < n_query; q++) counts[q] += jobs[t].counts[q];
+        free(jobs[t].counts);
+    }
+    for (size_t q = 0; q < n_query; q++) {
+        const u128 key = ((u128)qhi[q] << 64) | qlo[q];
+        uint64_t h = h128(key) & (slots - 1);
+        while (tkey[h] != key) h = (h + 1) & (slots - 1);
+        counts[q] = counts[tidx[h]];
+    }
+    free(jobs);
+    free(th);
+    free(tkey);
+    free(tidx);
+    return 0;
+}
+
 int or_spot_counts(const uint32_t *words, const uint64_t *off, size_t n_frag, const uint32_t *dataset, int k,
                    const uint64_t *query, size_t n_query, uint64_t *counts, uint32_t *masks, int threads) {
     if (k < 1 || k > 32 || threads < 1) return 1;

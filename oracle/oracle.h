@@ -85,6 +85,11 @@ int or_zi_absent(const or_frags *index, const or_frags *query, int k,
 int or_zi_lookup_timed(const or_frags *index, const or_frags *query, int k, uint64_t *lookups,
                        uint64_t *absent_unique, double *build_s, double *query_s);
 
+/* Exact occurrence counts of 128-bit canonical query k-mers (33 <= k <= 64)
+ * over packed reads (16 bases per u32 word). */
+int or_spot_counts128(const uint32_t *words, const uint64_t *off, size_t n_frag, int k, const uint64_t *qlo,
+                      const uint64_t *qhi, size_t n_query, uint64_t *counts, int threads);
+
 /* ---- writers ----------------------------------------------------------- */
 /* Simplitig FASTA (sorted input). Returns index of last simplitig written. */
 int or_write_simplitigs(const char *path, const or_counted *kmers, size_t n,

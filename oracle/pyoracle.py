@@ -75,6 +75,8 @@ def lib() -> C.CDLL:
                                                    C.POINTER(C.POINTER(_Counted)), C.POINTER(C.c_size_t)]
         L.or_zi_absent.argtypes = [C.POINTER(_Frags), C.POINTER(_Frags), C.c_int,
                                    C.POINTER(C.POINTER(_Counted)), C.POINTER(C.c_size_t)]
+        L.or_spot_counts128.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_void_p,
+                                        C.c_size_t, C.c_void_p, C.c_int]
         L.or_zi_lookup_timed.argtypes = [C.POINTER(_Frags), C.POINTER(_Frags), C.c_int, C.POINTER(C.c_uint64),
                                          C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.or_write_simplitigs.argtypes = [C.c_char_p, C.c_void_p, C.c_size_t, C.c_int, C.c_size_t, C.POINTER(C.c_size_t)]
@@ -335,6 +337,48 @@ def spot_counts(words, frag_off, k, query, dataset=None, threads=16):
     if rc:
         raise ValueError("or_spot_counts failed")
     return counts, masks
+
+
+def spot_counts128(words, frag_off, k, qlo, qhi, threads=16):
+    """Exact count of each 128-bit canonical query k-mer (lo, hi words;
+    33 <= k <= 64) over packed reads."""
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    off = np.ascontiguousarray(frag_off, dtype=np.uint64)
+    lo = np.ascontiguousarray(qlo, dtype=np.uint64)
+    hi = np.ascontiguousarray(qhi, dtype=np.uint64)
+    counts = np.zeros(lo.size, dtype=np.uint64)
+    rc = lib().or_spot_counts128(words.ctypes.data, off.ctypes.data, off.size - 1, k, lo.ctypes.data, hi.ctypes.data,
+                                 lo.size, counts.ctypes.data, threads)
+    if rc:
+        raise ValueError("or_spot_counts128 failed")
+    return counts
+
+
+def kmers_at128(words, pos, k):
+    """Canonical MC encoding of the k-mer (k <= 64) at each base position, as
+    (lo, hi) uint64 arrays."""
+    words = np.asarray(words, dtype=np.uint32)
+    pos = np.asarray(pos, dtype=np.uint64)
+    n = pos.size
+    fh, fl, rh, rl = (np.zeros(n, dtype=np.uint64) for _ in range(4))
+    for i in range(k):
+        p = pos + np.uint64(i)
+        c = (words[(p >> np.uint64(4)).astype(np.int64)] >> ((p & np.uint64(15)) * np.uint64(2)).astype(np.uint32)) & 3
+        b = (c ^ (c >> 1)).astype(np.uint64)
+        # forward: shift the 128-bit value left by 2 and add b
+        fh = (fh << np.uint64(2)) | (fl >> np.uint64(62))
+        fl = (fl << np.uint64(2)) | b
+        # reverse complement: base i's complement at bit position 2i
+        sh = 2 * i
+        cb = b ^ np.uint64(3)
+        if sh < 64:
+            rl = rl | (cb << np.uint64(sh))
+        else:
+            rh = rh | (cb << np.uint64(sh - 64))
+    if 2 * k < 128:
+        fh = fh & np.uint64((1 << (2 * k - 64)) - 1) if k > 32 else np.zeros(n, dtype=np.uint64)
+    take_r = (rh < fh) | ((rh == fh) & (rl < fl))
+    return np.where(take_r, rl, fl), np.where(take_r, rh, fh)
 
 
 def kmers_at(words, pos, k):

@@ -188,6 +188,108 @@ def test_server_results_same_alone_and_in_mix(k):
         assert same_counted(alone[i][0], mixed[i][0]) and alone[i][1] == mixed[i][1]
 
 
+def _codes_of(words, n):
+    """Packed-seq codes (A0 C1 T2 G3) of the first n bases of packed words."""
+    b = np.unpackbits(np.ascontiguousarray(words).view(np.uint8)[:, None], axis=1, bitorder="little")
+    b = b.reshape(-1, 4, 2)
+    return (b[:, :, 0] | (b[:, :, 1] << 1)).reshape(-1)[:n].astype(np.uint8)
+
+
+@pytest.mark.parametrize("k,m", [(31, 19), (63, 23)])
+def test_server_request_stream_spot_check(k, m):
+    """The request stream's shape at reduced scale: 48 requests whose read
+    counts are log-uniform in [10, 1e5], half from the index genome (0.5%
+    substitutions) and half unrelated, submitted from three threads with
+    exponential gaps against a 4 Mbp index. Every result has the request's
+    window count and a strictly increasing absent set; the eight smallest
+    equal the oracle's set difference; for every request, 200 sampled query
+    windows are absent exactly when the oracle spot counter finds them 0
+    times in the index genome, and 200 sampled absent k-mers occur 0 times."""
+    import time
+    G = 4_000_000
+    ga = M.synth_genome(G, 15)
+    gb = M.synth_genome(G, 1015)
+    gcodes = _codes_of(ga, G)
+    text = b">index\n" + np.frombuffer(b"ACTG", dtype=np.uint8)[gcodes].tobytes() + b"\n"
+    rng = np.random.default_rng(40 + k)
+    n_req = 48
+    counts = np.exp(rng.uniform(np.log(10), np.log(1e5), size=n_req)).astype(np.int64)
+    reqs = [M.synth_reads_from([(ga, G), (gb, G)], int(c), 150, 0.005, 500 + i) for i, c in enumerate(counts)]
+    gaps = rng.exponential(0.002, size=n_req)
+    h = C.c_void_p()
+    M._check(M._lib.mczi_zi_server_create_from_text(text, len(text), k, m, 8 << 30, 4, C.byref(h)))
+    ids = [None] * n_req
+    errs = []
+
+    def sub(which):
+        try:
+            for i in which:
+                time.sleep(gaps[i])
+                off = reqs[i].frag_off.astype(np.uint64)
+                rid = C.c_uint64()
+                M._check(M._lib.mczi_zi_server_submit_packed(h, reqs[i].words.ctypes.data, off.ctypes.data,
+                                                              off.size - 1, 0, C.byref(rid)))
+                ids[i] = rid.value
+        except Exception as e:  # reported after join
+            errs.append(e)
+
+    results = []
+    try:
+        ths = [threading.Thread(target=sub, args=(range(t, n_req, 3),)) for t in range(3)]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        assert not errs, errs
+        for i in range(n_req):
+            p = C.POINTER(M._ZiResult)()
+            M._check(M._lib.mczi_zi_server_poll(h, ids[i], 1, C.byref(p)))
+            r = p.contents
+            lo = np.ctypeslib.as_array(r.absent_lo, shape=(r.n_absent,)).copy() if r.n_absent else np.zeros(0, np.uint64)
+            hi = (np.ctypeslib.as_array(r.absent_hi, shape=(r.n_absent,)).copy() if k > 32 and r.n_absent
+                  else np.zeros_like(lo))
+            results.append((r.n_windows, lo, hi))
+            M._lib.mczi_zi_result_free(p)
+    finally:
+        M._lib.mczi_zi_server_destroy(h)
+    gwords = ga.view(np.uint32)
+    goff = np.array([0, G], dtype=np.uint64)
+    ic, io = gcodes, np.array([0, G], dtype=np.uint64)
+    smallest = set(np.argsort(counts)[:8].tolist())
+    for i, (n_windows, lo, hi) in enumerate(results):
+        rq = reqs[i]
+        off = rq.frag_off.astype(np.uint64)
+        assert n_windows == int(counts[i]) * (150 - k + 1), i
+        assert np.all((hi[1:] > hi[:-1]) | ((hi[1:] == hi[:-1]) & (lo[1:] > lo[:-1]))), i
+        if i in smallest:
+            qc = _codes_of(rq.words, int(off[-1]))
+            want = O.zi_absent(ic, io, qc, off, k)
+            got = np.zeros(lo.size, dtype=M.COUNTED)
+            got["lo"], got["hi"], got["count"] = lo, hi, 1
+            assert same_counted(got, want), i
+        f = rng.integers(0, off.size - 1, size=200)
+        pos = off[f] + rng.integers(0, 150 - k + 1, size=200).astype(np.uint64)
+        sa = rng.integers(0, max(lo.size, 1), size=200 if lo.size else 0)
+        if k <= 32:
+            wk = O.kmers_at(rq.words, pos, k)
+            c, _ = O.spot_counts(gwords, goff, k, wk)
+            at = np.minimum(np.searchsorted(lo, wk), max(lo.size - 1, 0))
+            is_absent = (lo[at] == wk) if lo.size else np.zeros(wk.size, bool)
+            assert np.array_equal(is_absent, c == 0), i
+            c, _ = O.spot_counts(gwords, goff, k, lo[sa])
+        else:
+            wlo, whi = O.kmers_at128(rq.words, pos, k)
+            c = O.spot_counts128(gwords, goff, k, wlo, whi)
+            left, right = np.searchsorted(hi, whi, "left"), np.searchsorted(hi, whi, "right")
+            is_absent = []
+            for a, b, x in zip(left, right, wlo):
+                j = a + np.searchsorted(lo[a:b], x)
+                is_absent.append(bool(j < b and lo[j] == x))
+            assert np.array_equal(np.array(is_absent), c == 0), i
+            c = O.spot_counts128(gwords, goff, k, lo[sa], hi[sa])
+        assert np.all(c == 0), i
+
+
 def test_server_rejects_over_budget():
     rng = np.random.default_rng(1)
     genome = _genome(rng, 5000)
@@ -204,13 +306,14 @@ def test_server_rejects_over_budget():
         M._lib.mczi_zi_server_destroy(h)
 
 
-def test_zi_full_size_index_spot_check():
-    """Serving at full index size: index = canonical 31-mers of a 100 Mbp
+@pytest.mark.parametrize("k,m", [(31, 0), (63, 23)])
+def test_zi_full_size_index_spot_check(k, m):
+    """Serving at full index size: index = canonical k-mers of a 100 Mbp
     genome (seed 5); one request of 200k 150 bp reads, half from the index
     genome (0.5% substitutions) and half from an unrelated genome. Sampled
     absent k-mers never occur in the index; sampled query windows are absent
     exactly when they do not occur in it (oracle spot counter over the index
-    genome)."""
+    genome). k=63 with m=23 is the 128-bit request-stream configuration."""
     G = 100_000_000
     ga = M.synth_genome(G, 5)
     gb = M.synth_genome(G, 1005)
@@ -220,7 +323,8 @@ def test_zi_full_size_index_spot_check():
     del codes
     reads = M.synth_reads_from([(ga, G), (gb, G)], 200_000, 150, 0.005, 6)
     h = C.c_void_p()
-    M._check(M._lib.mczi_zi_server_create_from_text(text, len(text), 31, 0, 64 << 30, 2, C.byref(h)))
+    M._check(M._lib.mczi_zi_server_create_from_text(text, len(text), k, m, 64 << 30, 2, C.byref(h)))
+    del text
     try:
         off = reads.frag_off.astype(np.uint64)
         rid = C.c_uint64()
@@ -229,27 +333,43 @@ def test_zi_full_size_index_spot_check():
         p = C.POINTER(M._ZiResult)()
         M._check(M._lib.mczi_zi_server_poll(h, rid.value, 1, C.byref(p)))
         r = p.contents
-        absent = np.ctypeslib.as_array(r.absent_lo, shape=(r.n_absent,)).copy()
+        lo = np.ctypeslib.as_array(r.absent_lo, shape=(r.n_absent,)).copy()
+        hi = (np.ctypeslib.as_array(r.absent_hi, shape=(r.n_absent,)).copy() if k > 32
+              else np.zeros_like(lo))
         n_windows = r.n_windows
         M._lib.mczi_zi_result_free(p)
     finally:
         M._lib.mczi_zi_server_destroy(h)
-    assert n_windows == 200_000 * 120
-    assert np.all(np.diff(absent) > 0) and 0.3 * n_windows < absent.size < n_windows
+    assert n_windows == 200_000 * (150 - k + 1)
+    # strictly increasing as 128-bit values
+    assert np.all((hi[1:] > hi[:-1]) | ((hi[1:] == hi[:-1]) & (lo[1:] > lo[:-1])))
+    assert 0.3 * n_windows < lo.size < n_windows
     rng = np.random.default_rng(5)
     gwords = ga.view(np.uint32)
     goff = np.array([0, G], dtype=np.uint64)
-    # sampled absent k-mers: none occurs in the index genome
-    sa = absent[rng.integers(0, absent.size, size=3000)]
-    c, _ = O.spot_counts(gwords, goff, 31, sa)
-    assert np.all(c == 0)
-    # sampled query windows: absent iff not in the index genome
+    sa = rng.integers(0, lo.size, size=3000)
     f = rng.integers(0, off.size - 1, size=3000)
-    pos = off[f] + rng.integers(0, 150 - 31 + 1, size=3000).astype(np.uint64)
-    wk = O.kmers_at(reads.words, pos, 31)
-    c, _ = O.spot_counts(gwords, goff, 31, wk)
-    at = np.minimum(np.searchsorted(absent, wk), absent.size - 1)
-    assert np.array_equal(absent[at] == wk, c == 0)
+    pos = off[f] + rng.integers(0, 150 - k + 1, size=3000).astype(np.uint64)
+    if k <= 32:
+        # sampled absent k-mers: none occurs in the index genome
+        c, _ = O.spot_counts(gwords, goff, k, lo[sa])
+        assert np.all(c == 0)
+        # sampled query windows: absent iff not in the index genome
+        wk = O.kmers_at(reads.words, pos, k)
+        c, _ = O.spot_counts(gwords, goff, k, wk)
+        at = np.minimum(np.searchsorted(lo, wk), lo.size - 1)
+        assert np.array_equal(lo[at] == wk, c == 0)
+    else:
+        c = O.spot_counts128(gwords, goff, k, lo[sa], hi[sa])
+        assert np.all(c == 0)
+        wlo, whi = O.kmers_at128(reads.words, pos, k)
+        c = O.spot_counts128(gwords, goff, k, wlo, whi)
+        left, right = np.searchsorted(hi, whi, "left"), np.searchsorted(hi, whi, "right")
+        is_absent = np.array([a < b and lo[a + np.searchsorted(lo[a:b], x)] == x if a + np.searchsorted(lo[a:b], x) < b
+                              else False for a, b, x in zip(left, right, wlo)])
+        assert np.array_equal(is_absent, c == 0)
+        # both outcomes occur among the sampled windows
+        assert 0 < is_absent.sum() < is_absent.size
 
 
 def test_zi_query_passes_equal_one_pass(tmp_path, monkeypatch):

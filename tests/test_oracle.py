@@ -407,6 +407,36 @@ def test_spot_counts_match_bruteforce():
         assert O.kmers_at(words, [p0], k)[0] == O.encode_canonical("".join("ACTG"[c] for c in codes[p0:p0 + k]))
 
 
+def test_spot_counts128_match_bruteforce():
+    """The 128-bit spot counter (33 <= k <= 64) and kmers_at128 against the
+    brute-force counter, with a repeated region so counts exceed one and
+    random keys that do not occur."""
+    rng = np.random.default_rng(22)
+    seqs = random_seqs(rng, 200, 70, 500)
+    seqs += seqs[:15]
+    codes, off, _ = O.frags_of(seqs)
+    words = np.zeros((codes.size + 15) // 16 + 4, dtype=np.uint32)
+    for j in range(16):
+        part = codes[j::16].astype(np.uint32) << (2 * j)
+        words[: part.size] |= part
+    for k in (33, 47, 63, 64):
+        want = O.count_bruteforce(codes, off, k, 0)
+        present = {(lo, hi): c for lo, hi, c in zip(want["lo"].tolist(), want["hi"].tolist(),
+                                                    want["count"].tolist())}
+        assert max(present.values()) > 1
+        qlo = np.concatenate([want["lo"][::5], rng.integers(0, 1 << 63, size=100, dtype=np.uint64)])
+        qhi = np.concatenate([want["hi"][::5], rng.integers(0, 1 << (2 * k - 64), size=100, dtype=np.uint64)])
+        for threads in (1, 5):
+            c = O.spot_counts128(words, off, k, qlo, qhi, threads)
+            for lo, hi, cnt in zip(qlo.tolist(), qhi.tolist(), c.tolist()):
+                assert cnt == present.get((lo, hi), 0), (k, lo, hi)
+        f = 4
+        p0 = int(off[f])
+        lo, hi = O.kmers_at128(words, [p0], k)
+        v = O.encode_canonical("".join("ACTG"[c] for c in codes[p0:p0 + k]))
+        assert (int(hi[0]) << 64 | int(lo[0])) == v
+
+
 def test_zi_lookup_loop_matches_set_difference():
     rng = np.random.default_rng(77)
     g = "".join(rng.choice(list("ACGT"), size=4000))
