@@ -655,10 +655,24 @@ void parse_fastx_file(const std::string &path, uint64_t min_len, uint32_t datase
     if (map == MAP_FAILED) throw Error(MCZI_E_IO, "failed to map " + path);
     madvise(map, len, MADV_SEQUENTIAL);
     const char *buf = (const char *)map;
-    // Chunk boundaries every >= 128 Mi bases (BATCH_BASES, src/lib.rs:28)
-    // so the plain launch mode can issue one launch per chunk.
-    parse_buffer(buf, len, min_len, dataset, out, path);
-    munmap(map, len);
+    // gzip / xz / zstd inputs are decompressed whole into host memory first
+    const InputCompression zc = input_compression((const uint8_t *)buf, len);
+    if (zc != InputCompression::None) {
+        std::string text;
+        try {
+            decompress_input((const uint8_t *)buf, len, zc, path, text);
+        } catch (...) {
+            munmap(map, len);
+            throw;
+        }
+        munmap(map, len);
+        parse_buffer(text.data(), text.size(), min_len, dataset, out, path);
+    } else {
+        // Chunk boundaries every >= 128 Mi bases (BATCH_BASES, src/lib.rs:28)
+        // so the plain launch mode can issue one launch per chunk.
+        parse_buffer(buf, len, min_len, dataset, out, path);
+        munmap(map, len);
+    }
     // split this file's fragments into chunks of >= 128 Mi bases
     const uint64_t batch = 128ull << 20;
     uint64_t first = out.chunk_frag.back();

@@ -11,6 +11,7 @@
 #include <memory>
 #include <mutex>
 #include <stdexcept>
+#include <cstdio>
 #include <string>
 #include <thread>
 #include <vector>
@@ -220,6 +221,27 @@ inline std::vector<uint64_t> nonempty_fragments(const uint64_t *frag_off, size_t
 // Parse FASTA/FASTQ (headers and qualities ignored, split at non-ACGT,
 // fragments shorter than min_len dropped; src/lib.rs:3264-3308).
 void parse_fastx_file(const std::string &path, uint64_t min_len, uint32_t dataset, HostReads &out);
+
+// ---- compression (compress.cpp) ----
+enum class InputCompression { None, Gzip, Xz, Zstd };
+// By magic bytes: gzip 1f 8b, xz fd '7zXZ' 00, zstd 28 b5 2f fd.
+InputCompression input_compression(const uint8_t *p, size_t len);
+void decompress_input(const uint8_t *p, size_t len, InputCompression c, const std::string &what, std::string &out);
+// 1 gzip, 2 xz, 3 zstd by the path's extension, else 0.
+int output_compression_kind(const std::string &path);
+// Streams compressed output of one kind into an open file.
+class Compressor {
+  public:
+    Compressor(FILE *fp, const std::string &path, int kind);
+    ~Compressor();
+    Compressor(const Compressor &) = delete;
+    Compressor &operator=(const Compressor &) = delete;
+    void write(const char *p, size_t n, bool finish);
+
+  private:
+    struct Impl;
+    Impl *m_;
+};
 void parse_fastx_text(const char *text, size_t len, uint64_t min_len, uint32_t dataset,
                       HostReads &out);
 

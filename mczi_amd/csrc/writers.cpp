@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 
 #include "internal.hpp"
@@ -18,13 +19,16 @@ const char kAscii[4] = {'A', 'C', 'G', 'T'};
 
 class Out {
   public:
+    // .gz / .gzip, .xz and .zst / .zstd paths are written compressed
     explicit Out(const std::string &path) : path_(path) {
         fp_ = fopen(path.c_str(), "wb");
         if (!fp_) throw Error(MCZI_E_IO, "failed to create " + path);
+        if (const int kind = output_compression_kind(path)) z_.reset(new Compressor(fp_, path, kind));
         buf_.reserve(1 << 22);
     }
     explicit Out(std::string *sink) : sink_(sink) {}
     ~Out() {
+        z_.reset();
         if (fp_) fclose(fp_);
     }
     void put(const char *s, size_t n) {
@@ -36,15 +40,22 @@ class Out {
         if (buf_.size() >= (1u << 22)) flush();
     }
     void put(const std::string &s) { put(s.data(), s.size()); }
-    void flush() {
-        if (fp_ && !buf_.empty()) {
+    void flush(bool last = false) {
+        if (!fp_) return;
+        if (z_) {
+            if (!buf_.empty() || last) z_->write(buf_.data(), buf_.size(), last);
+            buf_.clear();
+            return;
+        }
+        if (!buf_.empty()) {
             if (fwrite(buf_.data(), 1, buf_.size(), fp_) != buf_.size())
                 throw Error(MCZI_E_IO, "failed to write " + path_);
             buf_.clear();
         }
     }
     void finish() {
-        flush();
+        flush(true);
+        z_.reset();
         if (fp_) {
             if (fclose(fp_) != 0) {
                 fp_ = nullptr;
@@ -59,6 +70,7 @@ class Out {
     FILE *fp_ = nullptr;
     std::string *sink_ = nullptr;
     std::string buf_;
+    std::unique_ptr<Compressor> z_;
 };
 
 void put_lines(Out &o, const std::vector<uint8_t> &seq) {
