@@ -265,6 +265,9 @@ struct Batch {
     DevBuf out_kmer, out_count, out_part, out_hi;  // out_hi: high words for k > 32
     uint64_t n_out = 0;
     uint32_t k = 0;
+    // (lo, hi) -> what the 32-bit device count of that k-mer lacks (counts
+    // of 2^32 and more; added on download)
+    std::map<std::pair<uint64_t, uint64_t>, uint64_t> carries;
     mczi_run_stats last{};
     ~Batch();
 };

@@ -273,30 +273,38 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
     // starts with no global round trip: each thread holds one word
     // (WORDS_SPAN <= SCAN_THREADS) and one offset.
     static_assert(WORDS_SPAN <= SCAN_THREADS, "one staged word per thread");
+    // Only the fragment offsets a tile can need are loaded: its fragments
+    // lie in [tile_frag[t], tile_frag[t + 1]] (the next tile starts at or
+    // after its end), so entries 0 .. tile_frag[t + 1] - tile_frag[t] + 1;
+    // the rest read as "beyond the tile".
     uint64_t t = a.tile_begin + blockIdx.x;
-    uint64_t ts = 0, f0 = 0;
+    uint64_t ts = 0, f0 = 0, f1 = 0;
     int tl = 0;
+    auto next_frag = [&](uint64_t t_) -> uint64_t { return t_ + 1 < a.tile_end ? a.tile_frag[t_ + 1] : a.n_frag; };
     if (t < a.tile_end) {
         ts = a.tile_start[t];
         tl = (int)a.tile_len[t];
         f0 = a.tile_frag[t];
+        f1 = next_frag(t);
     }
     uint32_t pword = 0;
     uint64_t poff = 0;
-    auto load_first = [&](uint64_t ts_, int tl_, uint64_t f0_) {
+    auto load_first = [&](uint64_t ts_, int tl_, uint64_t f0_, uint64_t f1_) {
         const int nw = (((int)(ts_ & 15u) + tl_ + 160) >> 4) + 3;
         pword = tid < nw ? a.words[(ts_ >> 4) + tid] : 0u;
-        poff = f0_ + (uint64_t)tid <= a.n_frag ? a.off[f0_ + tid] : 0ull;
+        const uint64_t j = f0_ + (uint64_t)tid;
+        poff = (j <= f1_ + 1 && j <= a.n_frag) ? a.off[j] : (~0ull >> 1);
     };
-    if (t < a.tile_end) load_first(ts, tl, f0);
-    uint64_t ts_n = 0, f0_n = 0;  // the next tile's entry
+    if (t < a.tile_end) load_first(ts, tl, f0, f1);
+    uint64_t ts_n = 0, f0_n = 0, f1_n = 0;  // the next tile's entry
     int tl_n = 0;
-    for (; t < a.tile_end; t += gridDim.x, ts = ts_n, tl = tl_n, f0 = f0_n) {
+    for (; t < a.tile_end; t += gridDim.x, ts = ts_n, tl = tl_n, f0 = f0_n, f1 = f1_n) {
         const uint64_t tn = t + gridDim.x;
         if (tn < a.tile_end) {
             ts_n = a.tile_start[tn];
             tl_n = (int)a.tile_len[tn];
             f0_n = a.tile_frag[tn];
+            f1_n = next_frag(tn);
         }
         const int rb0 = (int)(ts & 15u);
         __syncthreads();  // previous tile fully consumed
@@ -317,7 +325,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             if (c < SCAN_THREADS) break;
         }
         __syncthreads();
-        if (tn < a.tile_end) load_first(ts_n, tl_n, f0_n);
+        if (tn < a.tile_end) load_first(ts_n, tl_n, f0_n, f1_n);
 
         // m-mer hashes for relative positions i in [0, tl + w - 1). ntHash is
         // XOR-linear, so with the exclusive prefix XORs over the tile's bases
@@ -1106,15 +1114,47 @@ struct CountArgs {
     uint32_t *out_count;
     uint32_t *out_part;
     uint64_t out_cap;
+    // k-mers whose 32-bit count wrapped: one (lo, hi) entry per wrap, each
+    // worth 2^32 occurrences (ctr[7] entries, at most CARRY_CAP)
+    unsigned long long *carry_lo, *carry_hi;
     // [1] output slots reserved (padding included) [2] distinct (LDS)
-    // [3] distinct (HBM) [4] error bits [6] padding slots
+    // [3] distinct (HBM) [4] error bits [6] padding slots [7] carry entries
     unsigned long long *ctr;
 };
 
+// Counts are 32-bit in the tables; the reference's counts are u64
+// (add_count saturates a u64, src/lib.rs:3003-3010). A k-mer seen 2^32 times
+// in one batch wraps its 32-bit count: the wrapping add appends the k-mer to
+// the carry list, the emit keeps it whatever its low count, and the host adds
+// 2^32 per entry to its count on download. Error bit 32: list full.
+constexpr uint32_t CARRY_CAP = 4096;
+
+__device__ __noinline__ void carry_add(const CountArgs &a, uint64_t lo, uint64_t hi) {
+    const unsigned long long i = atomicAdd(&a.ctr[7], 1ull);
+    if (i < CARRY_CAP) {
+        a.carry_lo[i] = lo;
+        a.carry_hi[i] = hi;
+    } else {
+        atomicOr(&a.ctr[4], 32ull);
+    }
+}
+
+// Whether (lo, hi) has a carry entry (a linear pass over the short list).
+__device__ __noinline__ bool has_carry(const CountArgs &a, uint64_t lo, uint64_t hi) {
+    const unsigned long long n =
+        min((unsigned long long)CARRY_CAP, __hip_atomic_load(&a.ctr[7], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    for (unsigned long long i = 0; i < n; i++)
+        if (a.carry_lo[i] == lo && a.carry_hi[i] == hi) return true;
+    return false;
+}
+
+// Adds one occurrence (MODE 0) or ORs a dataset bit (MODE 1); true when a
+// count wrapped past 2^32 - 1.
 template <int MODE>
-__device__ __forceinline__ void val_update(uint32_t *p, uint32_t v) {
-    if constexpr (MODE == 0) atomicAdd(p, v);
-    else atomicOr(p, v);
+__device__ __forceinline__ bool val_update(uint32_t *p, uint32_t v) {
+    if constexpr (MODE == 0) return atomicAdd(p, v) > 0xFFFFFFFFu - v;
+    atomicOr(p, v);
+    return false;
 }
 
 // The spill table takes new keys only while it is at most half full; past
@@ -1136,7 +1176,7 @@ __device__ void overflow_insert(const CountArgs &a, unsigned long long key, uint
         const unsigned long long cur =
             __hip_atomic_load(&a.ovf_keys[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (cur == key) {
-            val_update<MODE>(&a.ovf_vals[slot], val);
+            if (val_update<MODE>(&a.ovf_vals[slot], val)) carry_add(a, key, 0);
             return;
         }
         if (cur == EMPTY_KEY) {
@@ -1148,7 +1188,7 @@ __device__ void overflow_insert(const CountArgs &a, unsigned long long key, uint
                 return;
             }
             if (prev == key) {
-                val_update<MODE>(&a.ovf_vals[slot], val);
+                if (val_update<MODE>(&a.ovf_vals[slot], val)) carry_add(a, key, 0);
                 return;
             }
         }
@@ -1272,11 +1312,13 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
     __shared__ int s_wsum[CNT_THREADS / 64];
     __shared__ unsigned long long s_red[CNT_THREADS / 64];
     __shared__ unsigned long long s_obase, s_pad[2];
+    __shared__ int s_wrap;  // a count wrapped in the current partition
     const int tid = threadIdx.x;
     const int lane = tid & 63, wid = tid >> 6;
     const int k = a.k;
     unsigned long long o_cur = 0, o_end = 0, my_pad = 0;  // thread 0: the block's output chunk
     const uint64_t kmask = k >= 32 ? ~0ull : ((1ull << (2 * k)) - 1ull);
+    if (tid == 0) s_wrap = 0;
     const int rshift = 64 - 2 * k;
     unsigned long long my_distinct = 0;
     // the table starts empty; emit leaves it empty again for the next partition
@@ -1335,7 +1377,10 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
                     unsigned long long prev = cur;
                     if (cur == EMPTY_KEY) prev = atomicCAS(&s_keys[slot], EMPTY_KEY, key);
                     if (prev == key || prev == EMPTY_KEY) {
-                        val_update<MODE>(&s_vals[slot], val);
+                        if (val_update<MODE>(&s_vals[slot], val)) {
+                            carry_add(a, key, 0);
+                            s_wrap = 1;
+                        }
                         done = true;
                     }
                 }
@@ -1344,7 +1389,10 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
                     for (int p = 1; p < CNT_MAXPROBE; p++) {
                         const unsigned long long prev = atomicCAS(&s_keys[slot], EMPTY_KEY, key);
                         if (prev == EMPTY_KEY || prev == key) {
-                            val_update<MODE>(&s_vals[slot], val);
+                            if (val_update<MODE>(&s_vals[slot], val)) {
+                                carry_add(a, key, 0);
+                                s_wrap = 1;
+                            }
                             done = true;
                             break;
                         }
@@ -1359,6 +1407,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
         constexpr int PER = CNT_SLOTS / CNT_THREADS;
         uint32_t emask = 0;
         int cnt = 0;
+        const bool wrapped = MODE == 0 && s_wrap;
 #pragma unroll
         for (int i = 0; i < PER; i++) {
             const int sl = i * CNT_THREADS + tid;
@@ -1366,7 +1415,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
             if (key == EMPTY_KEY) continue;
             my_distinct++;
             const uint32_t c = (MODE == 0) ? s_vals[sl] : (uint32_t)__popc(s_vals[sl]);
-            if (c > a.x) {
+            if (c > a.x || (wrapped && has_carry(a, key, 0))) {
                 emask |= 1u << i;
                 cnt++;
             }
@@ -1398,6 +1447,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
             s_keys[sl] = EMPTY_KEY;  // each thread resets the slots it emitted from
             s_vals[sl] = 0;
         }
+        if (tid == 0) s_wrap = 0;
         __syncthreads();
     }
     out_finish<false>(a, o_cur, o_end, my_pad, s_pad);
@@ -1415,7 +1465,7 @@ __global__ void k_overflow_emit(CountArgs a) {
             key = a.ovf_keys[i];
             if (key != EMPTY_KEY) c = (MODE == 0) ? a.ovf_vals[i] : (uint32_t)__popc(a.ovf_vals[i]);
         }
-        const bool emit = key != EMPTY_KEY && c > a.x;
+        const bool emit = key != EMPTY_KEY && (c > a.x || (MODE == 0 && a.ctr[7] && has_carry(a, key, 0)));
         const unsigned long long pos = wave_append(emit, &a.ctr[1]);
         if (emit && pos < a.out_cap) {
             a.out_kmer[pos] = key;
@@ -1489,7 +1539,7 @@ __device__ void overflow_insert_w(const CountArgs &a, uint64_t klo, uint64_t khi
                 }
             } else if (h == khi &&
                        __hip_atomic_load(&a.ovf_keys[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == klo) {
-                atomicAdd(&a.ovf_vals[slot], 1u);
+                if (atomicAdd(&a.ovf_vals[slot], 1u) == 0xFFFFFFFFu) carry_add(a, klo, khi);
                 done = true;
             } else {
                 slot = (slot + 1) & a.ovf_mask;
@@ -1520,12 +1570,14 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count_wide(CountArgs a) {
     unsigned long long o_cur = 0, o_end = 0, my_pad = 0;  // thread 0: the block's output chunk
     const uint64_t mb = (k - 32) >= 32 ? ~0ull : ((1ull << (2 * (k - 32))) - 1ull);
     unsigned long long my_distinct = 0;
+    __shared__ int s_wrap;  // a count wrapped in the current partition
     for (uint32_t q = blockIdx.x; q < a.p_int; q += gridDim.x) {
         for (int i = tid; i < CNT_SLOTS; i += CNT_THREADS) {
             s_klo[i] = EMPTY_KEY;
             s_khi[i] = EMPTY_KEY;
             s_vals[i] = 0;
         }
+        if (tid == 0) s_wrap = 0;
         __syncthreads();
         const uint64_t rb = a.base[q], re = a.cursor[q];
         for (uint64_t wb = rb + (uint64_t)wid * 64; wb < re; wb += CNT_THREADS) {
@@ -1590,7 +1642,10 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count_wide(CountArgs a) {
                             }
                         } else if (h == khi && __hip_atomic_load(&s_klo[slot], __ATOMIC_ACQUIRE,
                                                                  __HIP_MEMORY_SCOPE_WORKGROUP) == klo) {
-                            atomicAdd(&s_vals[slot], 1u);
+                            if (atomicAdd(&s_vals[slot], 1u) == 0xFFFFFFFFu) {
+                                carry_add(a, klo, khi);
+                                s_wrap = 1;
+                            }
                             done = true;
                         } else {
                             slot = (slot + 1) & (CNT_SLOTS - 1);
@@ -1609,12 +1664,13 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count_wide(CountArgs a) {
         constexpr int PER = CNT_SLOTS / CNT_THREADS;
         uint32_t emask = 0;
         int cnt = 0;
+        const bool wrapped = s_wrap;
 #pragma unroll
         for (int i = 0; i < PER; i++) {
             const int sl = i * CNT_THREADS + tid;
             if (s_khi[sl] == EMPTY_KEY) continue;
             my_distinct++;
-            if (s_vals[sl] > a.x) {
+            if (s_vals[sl] > a.x || (wrapped && has_carry(a, s_klo[sl], s_khi[sl]))) {
                 emask |= 1u << i;
                 cnt++;
             }
@@ -1660,7 +1716,7 @@ __global__ void k_overflow_emit_wide(CountArgs a) {
             hi = a.ovf_hi[i];
             if (hi != EMPTY_KEY) c = a.ovf_vals[i];
         }
-        const bool emit = hi != EMPTY_KEY && c > a.x;
+        const bool emit = hi != EMPTY_KEY && (c > a.x || (a.ctr[7] && has_carry(a, a.ovf_keys[i], hi)));
         const unsigned long long pos = wave_append(emit, &a.ctr[1]);
         if (emit && pos < a.out_cap) {
             a.out_kmer[pos] = a.ovf_keys[i];
@@ -1768,6 +1824,7 @@ struct McPlan {
     uint64_t run_cap = 0, mtab_mask = 0;
     bool use_mtab = false, want_tiles = false;
     DevBuf out_kmer, out_count, out_part;
+    DevBuf carry;  // wrapped counts: CARRY_CAP low words, then CARRY_CAP high words
     DevBuf sort_tmp, key2, pu, idx, idx2, pu2, k2out;
     uint64_t ovf_slots = 0;
     uint32_t p_int = 0;
@@ -2107,6 +2164,8 @@ static CountArgs count_args(McPlan &p, const mczi_counter_config &cfg) {
     ca.out_count = p.out_count.as<uint32_t>();
     ca.out_part = p.out_part.as<uint32_t>();
     ca.out_cap = p.out_cap;
+    ca.carry_lo = p.carry.as<unsigned long long>();
+    ca.carry_hi = ca.carry_lo + CARRY_CAP;
     ca.ctr = p.ctr.as<unsigned long long>();
     return ca;
 }
@@ -2380,6 +2439,7 @@ static void count_pass(Batch &b, McPlan &p, const mczi_counter_config &cfg, int 
         p.base.ensure(((size_t)p.p_int + 1) * 8);
         p.cursor.ensure(((size_t)p.p_int + 1) * 8);
         p.ctr.ensure(16 * 8);
+        p.carry.ensure(2 * CARRY_CAP * 8);
         p.part_blocks = (uint32_t)d.cu_count() * part_res;
         p.hist.ensure((size_t)g.nc * p.part_blocks * 4);
         p.cbase.ensure(((size_t)g.nc + 1) * 8);
@@ -2474,6 +2534,8 @@ static void count_pass(Batch &b, McPlan &p, const mczi_counter_config &cfg, int 
         }
         require((h[4] & 16ull) == 0, MCZI_E_DEVICE, "partitioning moved a record out of its bucket");
         require((h[4] & 8ull) == 0, MCZI_E_DEVICE, "k-mer table insertion did not settle");
+        require((h[4] & 32ull) == 0, MCZI_E_NOMEM,
+                "more than 4096 k-mer counts passed 2^32 in one batch (count carry list full)");
         p.n_staged = h[11];
         p.n_records = h[12];
         if ((h[4] & 1ull) || h[1] > p.out_cap) {
@@ -2699,6 +2761,7 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
     mczi_run_stats st{};
     b.k = k;
     if (mode == MCZI_MODE_DATASETS && b.n_datasets > 32) {
+        b.carries.clear();
         auto t0 = std::chrono::steady_clock::now();
         uint64_t n_out = 0, n_distinct = 0;
         count_dataset_groups(b, p, cfg, launch, tm, h, n_out, n_distinct);
@@ -2721,6 +2784,12 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
     p.ds_lo = 0;
     count_pass(b, p, cfg, mode, launch, tm, want_times, h);
     const int t_dev_end = tm.n - 1;
+    b.carries.clear();
+    if (h[7]) {  // counts that passed 2^32: 2^32 per carry entry
+        std::vector<unsigned long long> c(2 * CARRY_CAP);
+        d.download(c.data(), p.carry.as<void>(), c.size() * 8, s);
+        for (uint64_t i = 0; i < h[7]; i++) b.carries[{c[i], c[CARRY_CAP + i]}] += 1ull << 32;
+    }
     const uint64_t n_slots = h[1], n_real = h[1] - h[6];
     const int tsort0 = tm.n;
     tm.mark(s);
@@ -2809,6 +2878,11 @@ void batch_download(Batch &b, mczi_counted_kmer *out) {
                        rec.as<unsigned long long>(), b.n_out);
     MCZI_HIP(hipGetLastError());
     d.download(out, rec.as<void>(), b.n_out * sizeof(mczi_counted_kmer), d.stream());
+    if (!b.carries.empty())
+        for (uint64_t i = 0; i < b.n_out; i++) {
+            const auto it = b.carries.find({out[i].lo, out[i].hi});
+            if (it != b.carries.end()) out[i].count += it->second;
+        }
 }
 
 void batch_download_records(Batch &b, std::vector<uint64_t> &rec, std::vector<uint32_t> &part, uint32_t &rec_words,
