@@ -283,6 +283,9 @@ typedef struct {
     uint64_t unique_minimizers;
     uint64_t minimizers_retained;
     double ms_pass1;
+    /* partition-group passes the count took (1: one pass; more when one
+     * pass would not fit in device memory, or MCZI_GROUP_PASSES asks) */
+    uint64_t partition_groups;
 } mczi_run_stats;
 
 /* Count the k-mers of a batch on the device. Results stay on the device

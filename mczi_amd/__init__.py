@@ -58,7 +58,8 @@ class RunStats(C.Structure):
                                          "n_overflow", "n_out", "tile_windows", "partition_bytes")] + [
         ("internal_partitions", C.c_uint32), ("lds_slots", C.c_uint32)] + [
         (n, C.c_double) for n in ("ms_scan1", "ms_scan2", "ms_count", "ms_sort", "ms_total")] + [
-        ("unique_minimizers", C.c_uint64), ("minimizers_retained", C.c_uint64), ("ms_pass1", C.c_double)]
+        ("unique_minimizers", C.c_uint64), ("minimizers_retained", C.c_uint64), ("ms_pass1", C.c_double),
+        ("partition_groups", C.c_uint64)]
 
     def as_dict(self) -> dict:
         return {n: getattr(self, n) for n, _ in self._fields_}

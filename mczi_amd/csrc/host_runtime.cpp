@@ -83,6 +83,13 @@ void Device::release(void *p) {
     live_.erase(it);
 }
 
+size_t Device::pooled_bytes() {
+    std::lock_guard<std::mutex> lk(mu_);
+    size_t t = 0;
+    for (const auto &kv : free_) t += kv.first;
+    return t;
+}
+
 void *Device::host_alloc(size_t bytes, bool allow_pageable) {
     size_t want = 64u << 10;
     while (want < bytes) want <<= 1;

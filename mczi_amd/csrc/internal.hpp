@@ -88,6 +88,7 @@ class Device {
     // Cached device allocation (rounded to 2 MiB); contents undefined.
     void *alloc(size_t bytes);
     void release(void *p);
+    size_t pooled_bytes();  // released device blocks kept for reuse
     // Cached pinned host allocation (hipHostMalloc, power-of-two size
     // classes): device <-> host copies into it run as plain DMA. Released
     // blocks stay cached up to kPinnedCacheBytes.
@@ -146,6 +147,10 @@ class DevBuf {
     void ensure(size_t bytes) { if (bytes > n_) reset(bytes); }
     template <typename T> T *as() const { return static_cast<T *>(p_); }
     size_t size() const { return n_; }
+    void swap(DevBuf &o) noexcept {
+        std::swap(p_, o.p_);
+        std::swap(n_, o.n_);
+    }
 
   private:
     void *p_ = nullptr;

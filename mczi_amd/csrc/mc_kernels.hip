@@ -92,6 +92,9 @@ struct ScanArgs {
     uint32_t p_mask;  // p_int - 1 when p_int is a power of two, else 0
     uint32_t wrec;
     uint32_t ds_lo;   // fof: first dataset of the group of 32 being staged
+    // partition-group passes: stage only the pieces whose key hash h has
+    // (h * grp_n) >> 32 == grp_i (grp_n <= 1: every piece)
+    uint32_t grp_n, grp_i;
     uint32_t *win_out;
     unsigned long long *stats;  // [0] pieces [4] records [5] .skm bytes of the staged runs
     // pass 1 (RUNS): one entry per piece, key << 32 | (windows, or dataset with fof)
@@ -562,6 +565,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             if (a.dataset != nullptr && a.dataset[f0 + (pw >> 11)] - a.ds_lo >= 32u) continue;
             const uint32_t key = spread_hash_u32(s_u.run.v[P8(s)]);
             if (a.mtab != nullptr && !mtab_has(a, key)) continue;
+            if (a.grp_n > 1 && (uint32_t)(((uint64_t)spread_hash_u32(key) * a.grp_n) >> 32) != a.grp_i) continue;
             staged |= 1u << slot;
             my_nrec += e - s <= (int)a.wrec ? 1 : (e - s + (int)a.wrec - 1) / (int)a.wrec;
             my_bytes += skm_bytes((uint32_t)(e - s + k - 1));
@@ -1825,6 +1829,12 @@ struct McPlan {
     bool use_mtab = false, want_tiles = false;
     DevBuf out_kmer, out_count, out_part;
     DevBuf carry;  // wrapped counts: CARRY_CAP low words, then CARRY_CAP high words
+    // partition-group passes (records beyond one device pass): grp_n passes,
+    // the current one grp_i; outputs gather in acc_* between passes
+    uint32_t grp_n = 1, grp_i = 0;
+    uint64_t grp_sig = 0;
+    double out_scale = 1.0;
+    DevBuf acc_kmer, acc_hi, acc_count, acc_part;
     DevBuf sort_tmp, key2, pu, idx, idx2, pu2, k2out;
     uint64_t ovf_slots = 0;
     uint32_t p_int = 0;
@@ -2089,6 +2099,8 @@ static ScanArgs scan_args(Batch &b, McPlan &p, const mczi_counter_config &cfg) {
     sa.tile_last = p.want_tiles ? p.tile_last.as<unsigned long long>() : nullptr;
     sa.stage_ctr = p.ctr.as<unsigned long long>() + 11;
     sa.err = p.ctr.as<unsigned long long>() + 4;
+    sa.grp_n = p.grp_n;
+    sa.grp_i = p.grp_i;
     return sa;
 }
 
@@ -2415,13 +2427,35 @@ static uint64_t mix_sig(uint64_t h, uint64_t v) {
     return spread_hash_u64(h ^ (v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2)));
 }
 
+constexpr uint32_t MAX_GROUPS = 4096;
+
+static uint64_t plan_held_bytes(const McPlan &p) {
+    uint64_t t = 0;
+    for (const DevBuf *db : {&p.stage_rec, &p.records, &p.stage_rec_w, &p.records_w, &p.out_kmer, &p.out_hi,
+                             &p.out_count, &p.out_part, &p.ovf_keys, &p.ovf_vals, &p.ovf_part, &p.ovf_hi})
+        t += db->size();
+    return t;
+}
+
+// Device memory one counting pass may use: MCZI_DEVICE_BUDGET bytes, or 85%
+// of what is free, pooled for reuse, or held by this plan's pass buffers.
+static uint64_t device_budget(const McPlan &p) {
+    if (const char *e = getenv("MCZI_DEVICE_BUDGET")) return std::max<uint64_t>(strtoull(e, nullptr, 10), 1 << 20);
+    size_t fr = 0, tot = 0;
+    MCZI_HIP(hipMemGetInfo(&fr, &tot));
+    return (uint64_t)(0.85 * (double)(fr + Device::get().pooled_bytes() + plan_held_bytes(p)));
+}
+
 // One count of the batch on the device (for fof, of the datasets
-// [p.ds_lo, p.ds_lo + 32)): scan, partitioning, counting and emit, as one
-// graph with the batched launch. The output slots (padding included) stay in
-// p.out_*; h receives the counters. Staging that outgrew its capacity, or a
-// spill table that filled, is redone with more room.
-static void count_pass(Batch &b, McPlan &p, const mczi_counter_config &cfg, int mode, int launch, Timer &tm,
-                       bool want_times, unsigned long long *h) {
+// [p.ds_lo, p.ds_lo + 32); with partition groups, of group p.grp_i): scan,
+// partitioning, counting and emit, as one graph with the batched launch. The
+// output slots (padding included) stay in p.out_*; h receives the counters.
+// Staging that outgrew its capacity, or a spill table that filled, is redone
+// with more room. Returns false, with p.grp_n raised, when the pass would not
+// fit in device memory or would stage 2^32 records or more: the caller then
+// counts in that many partition-group passes.
+static bool count_pass_one(Batch &b, McPlan &p, const mczi_counter_config &cfg, int mode, int launch, Timer &tm,
+                           bool want_times, unsigned long long *h) {
     Device &d = Device::get();
     hipStream_t s = d.stream();
     const uint32_t k = cfg.k;
@@ -2431,8 +2465,11 @@ static void count_pass(Batch &b, McPlan &p, const mczi_counter_config &cfg, int 
     static const unsigned part_res =
         std::min(resident_blocks((const void *)k_part_coarse<ulonglong2>, PT_THREADS),
                  resident_blocks((const void *)k_part_coarse<Rec32>, PT_THREADS));
+    const uint32_t G = std::max<uint32_t>(p.grp_n, 1);
+    const uint64_t win = std::max<uint64_t>(p.n_windows / G, 1);  // windows of this pass (expected)
+    const uint64_t budget = device_budget(p);
     for (int attempt = 0;; attempt++) {
-        p.p_int = choose_partitions(p.n_windows, cfg.partition_count, target);
+        p.p_int = choose_partitions(win, cfg.partition_count, target);
         const PartGeom g = part_geom(p.p_int);
         require(g.nc <= PT_MAX_DIGITS && g.nf <= PT_MAX_DIGITS, MCZI_E_INVALID,
                 "too many internal partitions for the device partitioning");
@@ -2443,18 +2480,12 @@ static void count_pass(Batch &b, McPlan &p, const mczi_counter_config &cfg, int 
         p.part_blocks = (uint32_t)d.cu_count() * part_res;
         p.hist.ensure((size_t)g.nc * p.part_blocks * 4);
         p.cbase.ensure(((size_t)g.nc + 1) * 8);
-        const uint64_t ovf = next_pow2(std::max<uint64_t>(p.n_windows / 16, 1 << 20)) << attempt;
-        p.ovf_slots = ovf;
-        p.ovf_keys.ensure(ovf * 8);
-        p.ovf_vals.ensure(ovf * 4);
-        p.ovf_part.ensure(ovf * 4);
-        if (wide) p.ovf_hi.ensure(ovf * 8);
-        // every emitted k-mer has more than x windows; plus one padded chunk per count block
-        p.out_cap = p.n_windows / (cfg.threshold + 1) + 16 + count_grid_blocks(p, wide) * OUT_CHUNK;
-        p.out_kmer.ensure(p.out_cap * 8);
-        if (wide) p.out_hi.ensure(p.out_cap * 8);
-        p.out_count.ensure(p.out_cap * 4);
-        p.out_part.ensure(p.out_cap * 4);
+        const uint64_t ovf = next_pow2(std::max<uint64_t>(win / 16, 1 << 20)) << attempt;
+        // every emitted k-mer has more than x windows (out_scale > 1 covers a
+        // partition group holding more than its share); plus one padded chunk
+        // per count block
+        const uint64_t out_cap =
+            (uint64_t)((double)(win / (cfg.threshold + 1)) * p.out_scale) + 16 + count_grid_blocks(p, wide) * OUT_CHUNK;
 
         // Staging capacity: the expected record count of a random minimizer
         // scheme (density 2 / (w + 1) per window, plus record splits, one
@@ -2464,20 +2495,40 @@ static void count_pass(Batch &b, McPlan &p, const mczi_counter_config &cfg, int 
         // total exceeds the capacity, the step runs again with room for twice
         // the records (the padding of an abandoned chunk is less than its
         // records) plus a full chunk per block.
-        const uint64_t cfg_sig = mix_sig(mix_sig(k, cfg.minimizer), b.n_bases);
+        const uint64_t cfg_sig = mix_sig(mix_sig(mix_sig(k, cfg.minimizer), b.n_bases), G);
         const uint32_t wrec = (uint32_t)((wide ? REC_MAX_BASES_W : REC_MAX_BASES) - k + 1);
         const uint64_t chunk_pad = launch_scan(p, ScanArgs{}, launch, s, false) * STAGE_CHUNK;
         if (p.rec_sig != cfg_sig) {
             const double w = (double)(k - cfg.minimizer + 1);
-            const double expect = (double)p.n_windows * (2.0 / (w + 1.0) + 1.0 / wrec) + (double)b.n_frag +
-                                  (double)p.n_tiles;
+            const double expect =
+                ((double)p.n_windows * (2.0 / (w + 1.0) + 1.0 / wrec) + (double)b.n_frag + (double)p.n_tiles) / G;
             p.stage_cap = std::min<uint64_t>((uint64_t)(1.25 * expect) + 4096, p.n_windows) + chunk_pad + 16;
             if (const char *e = getenv("MCZI_STAGE_CAPACITY"))  // initial capacity in slots
                 p.stage_cap = std::max<uint64_t>(strtoull(e, nullptr, 10), 1);
             p.rec_sig = cfg_sig;
         }
+        // device memory of this pass; more than the budget (or 2^32 staging
+        // slots) splits the count into partition groups
+        const uint64_t rec_bytes = wide ? sizeof(Rec32) : 16;
+        const uint64_t need = 2 * p.stage_cap * rec_bytes + out_cap * (wide ? 24 : 16) + ovf * (wide ? 24 : 16);
+        if (attempt == 0 && (need > budget || p.stage_cap >= (1ull << 32)) && G < MAX_GROUPS) {
+            const uint64_t f = std::max<uint64_t>((need + budget - 1) / std::max<uint64_t>(budget, 1),
+                                                  (p.stage_cap >> 31) + 1);
+            p.grp_n = (uint32_t)std::min<uint64_t>(MAX_GROUPS, (uint64_t)G * next_pow2(std::max<uint64_t>(f, 2)));
+            return false;
+        }
+        p.ovf_slots = ovf;
+        p.ovf_keys.ensure(ovf * 8);
+        p.ovf_vals.ensure(ovf * 4);
+        p.ovf_part.ensure(ovf * 4);
+        if (wide) p.ovf_hi.ensure(ovf * 8);
+        p.out_cap = out_cap;
+        p.out_kmer.ensure(p.out_cap * 8);
+        if (wide) p.out_hi.ensure(p.out_cap * 8);
+        p.out_count.ensure(p.out_cap * 4);
+        p.out_part.ensure(p.out_cap * 4);
         require(p.stage_cap < (1ull << 32), MCZI_E_INVALID,
-                "too many super-k-mer records for one batch (limit 2^32): split the input");
+                "too many super-k-mer records for one pass (limit 2^32)");
         if (wide) {
             p.stage_rec_w.ensure(p.stage_cap * sizeof(Rec32));
             p.records_w.ensure(p.stage_cap * sizeof(Rec32));
@@ -2491,6 +2542,7 @@ static void count_pass(Batch &b, McPlan &p, const mczi_counter_config &cfg, int 
         if (use_graph) {
             uint64_t sig = cfg_sig;
             sig = mix_sig(mix_sig(mix_sig(mix_sig(sig, cfg.threshold), (uint64_t)mode), p.p_int), p.ds_lo);
+            sig = mix_sig(mix_sig(sig, p.grp_n), p.grp_i);
             for (const DevBuf *db : {&b.words, &b.off, &b.dataset, &p.base, &p.cursor, &p.records, &p.ctr,
                                      &p.ovf_keys, &p.ovf_vals, &p.ovf_part, &p.out_kmer, &p.out_count,
                                      &p.out_part, &p.tile_start, &p.tile_frag, &p.tile_len, &p.stage_rec,
@@ -2525,10 +2577,19 @@ static void count_pass(Batch &b, McPlan &p, const mczi_counter_config &cfg, int 
         MCZI_HIP(hipGetLastError());
         if (h[11] > p.stage_cap || (h[4] & 2ull)) {
             // staging outgrew its capacity: everything after the scan ran on
-            // the slots that fit, and is redone
-            require(h[12] < (1ull << 32), MCZI_E_INVALID, "too many super-k-mer records for one batch (limit 2^32)");
-            require(p.stage_cap < 2 * h[12] + chunk_pad + 16, MCZI_E_DEVICE, "super-k-mer staging exceeded its bound");
-            p.stage_cap = 2 * h[12] + chunk_pad + 16;
+            // the slots that fit, and is redone (in more partition groups if
+            // the room it needs is beyond the budget)
+            const uint64_t cap2 = 2 * h[12] + chunk_pad + 16;
+            const uint64_t need2 = 2 * cap2 * rec_bytes + p.out_cap * (wide ? 24 : 16) + ovf * (wide ? 24 : 16);
+            if ((need2 > budget || cap2 >= (1ull << 32)) && G < MAX_GROUPS) {
+                const uint64_t f = std::max<uint64_t>((need2 + budget - 1) / std::max<uint64_t>(budget, 1),
+                                                      (cap2 >> 31) + 1);
+                p.grp_n = (uint32_t)std::min<uint64_t>(MAX_GROUPS, (uint64_t)G * next_pow2(std::max<uint64_t>(f, 2)));
+                return false;
+            }
+            require(cap2 < (1ull << 32), MCZI_E_INVALID, "too many super-k-mer records for one pass (limit 2^32)");
+            require(p.stage_cap < cap2, MCZI_E_DEVICE, "super-k-mer staging exceeded its bound");
+            p.stage_cap = cap2;
             attempt--;
             continue;
         }
@@ -2538,12 +2599,143 @@ static void count_pass(Batch &b, McPlan &p, const mczi_counter_config &cfg, int 
                 "more than 4096 k-mer counts passed 2^32 in one batch (count carry list full)");
         p.n_staged = h[11];
         p.n_records = h[12];
+        if (h[1] > p.out_cap && G > 1) {
+            // this group holds more than its share of the output
+            require(attempt < 8, MCZI_E_NOMEM, "output slots overflowed repeatedly");
+            p.out_scale *= 2;
+            continue;
+        }
         if ((h[4] & 1ull) || h[1] > p.out_cap) {
             require(attempt < 4, MCZI_E_NOMEM, "k-mer spill table overflowed repeatedly");
             target /= 4;
             continue;
         }
         p.effective_target = target;
+        return true;
+    }
+}
+
+// Carries of the pass just counted (ctr[7] entries) into b.carries.
+static void collect_carries(Batch &b, McPlan &p, uint64_t n) {
+    if (!n) return;
+    Device &d = Device::get();
+    std::vector<unsigned long long> c(2 * CARRY_CAP);
+    d.download(c.data(), p.carry.as<void>(), c.size() * 8, d.stream());
+    for (uint64_t i = 0; i < n && i < CARRY_CAP; i++) b.carries[{c[i], c[CARRY_CAP + i]}] += 1ull << 32;
+}
+
+// Output entries [0, n) without the padding appended at cursor (k-mer, high
+// word, count, partition).
+__global__ void k_append_out(const unsigned long long *kin, const unsigned long long *hin, const uint32_t *cin,
+                             const uint32_t *pin, uint64_t n, unsigned long long *kout, unsigned long long *hout,
+                             uint32_t *cout, uint32_t *pout, unsigned long long *cursor) {
+    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < n; i0 += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = i0 + threadIdx.x;
+        const bool keep = i < n && pin[i] != OUT_PAD;
+        const unsigned long long pos = wave_append(keep, cursor);
+        if (keep) {
+            kout[pos] = kin[i];
+            if (hout) hout[pos] = hin[i];
+            cout[pos] = cin[i];
+            pout[pos] = pin[i];
+        }
+    }
+}
+
+// A count that does not fit one device pass (src/lib.rs spills the
+// super-k-mers of every partition to disk and counts the partitions one by
+// one, 2803-2845): the partitions are split into p.grp_n groups by their key
+// hash, and each group is one pass over the resident batch that stages only
+// its own super-k-mers, so each pass needs about 1 / grp_n of the staging,
+// table and output memory. Every k-mer lives in one group (its minimizer
+// fixes its group), so the groups' outputs are disjoint; they gather in HBM
+// and leave in p.out_* (h[1] entries, no padding) like one pass's output.
+// MCZI_GROUP_PASSES sets the group count; otherwise it is 1 unless a pass
+// would exceed the device budget (free device memory, or
+// MCZI_DEVICE_BUDGET bytes), and it is kept for later counts of the same
+// configuration.
+static void count_pass(Batch &b, McPlan &p, const mczi_counter_config &cfg, int mode, int launch, Timer &tm,
+                       bool want_times, unsigned long long *h) {
+    Device &d = Device::get();
+    hipStream_t s = d.stream();
+    const bool wide = cfg.k > 32;
+    const uint64_t gsig = mix_sig(mix_sig(mix_sig(mix_sig(cfg.k, cfg.minimizer), cfg.threshold), b.n_bases), mode);
+    if (p.grp_sig != gsig) {
+        p.grp_n = 1;
+        if (const char *e = getenv("MCZI_GROUP_PASSES"))
+            p.grp_n = (uint32_t)std::min<uint64_t>(MAX_GROUPS, std::max<uint64_t>(strtoull(e, nullptr, 10), 1));
+        p.out_scale = p.grp_n > 1 ? 1.5 : 1.0;
+        p.grp_sig = gsig;
+    }
+    b.carries.clear();
+    for (;;) {
+        const uint32_t G = p.grp_n;
+        if (G <= 1) {
+            p.grp_i = 0;
+            if (!count_pass_one(b, p, cfg, mode, launch, tm, want_times, h)) {
+                p.out_scale = 1.5;
+                continue;
+            }
+            collect_carries(b, p, h[7]);
+            return;
+        }
+        unsigned long long hs[16] = {};
+        uint64_t used = 0, cap = 0;
+        bool again = false;
+        for (uint32_t gi = 0; gi < G; gi++) {
+            p.grp_i = gi;
+            if (!count_pass_one(b, p, cfg, mode, launch, tm, want_times, h)) {
+                again = true;
+                break;
+            }
+            collect_carries(b, p, h[7]);
+            const uint64_t n_slots = h[1], n_real = h[1] - h[6];
+            if (used + n_real > cap) {  // grow, keeping what is there
+                const uint64_t nc = std::max<uint64_t>(2 * cap, used + n_real) + 1024;
+                DevBuf nk(nc * 8), nc4(nc * 4), np(nc * 4), nh(wide ? nc * 8 : 0);
+                if (used) {
+                    MCZI_HIP(hipMemcpyAsync(nk.as<void>(), p.acc_kmer.as<void>(), used * 8, hipMemcpyDeviceToDevice, s));
+                    MCZI_HIP(hipMemcpyAsync(nc4.as<void>(), p.acc_count.as<void>(), used * 4, hipMemcpyDeviceToDevice, s));
+                    MCZI_HIP(hipMemcpyAsync(np.as<void>(), p.acc_part.as<void>(), used * 4, hipMemcpyDeviceToDevice, s));
+                    if (wide)
+                        MCZI_HIP(hipMemcpyAsync(nh.as<void>(), p.acc_hi.as<void>(), used * 8, hipMemcpyDeviceToDevice, s));
+                }
+                MCZI_HIP(hipStreamSynchronize(s));
+                p.acc_kmer = std::move(nk);
+                p.acc_count = std::move(nc4);
+                p.acc_part = std::move(np);
+                p.acc_hi = std::move(nh);
+                cap = nc;
+            }
+            if (n_slots) {
+                unsigned long long *cur = p.ctr.as<unsigned long long>() + 14;
+                fill_u64(cur, 1, 0, s);
+                hipLaunchKernelGGL(k_append_out, dim3(grid_for(n_slots, 256, 8192)), dim3(256), 0, s,
+                                   p.out_kmer.as<unsigned long long>(),
+                                   wide ? p.out_hi.as<unsigned long long>() : nullptr, p.out_count.as<uint32_t>(),
+                                   p.out_part.as<uint32_t>(), n_slots, p.acc_kmer.as<unsigned long long>() + used,
+                                   wide ? p.acc_hi.as<unsigned long long>() + used : nullptr,
+                                   p.acc_count.as<uint32_t>() + used, p.acc_part.as<uint32_t>() + used, cur);
+                MCZI_HIP(hipGetLastError());
+            }
+            used += n_real;
+            for (int i : {2, 3, 9, 11, 12, 13}) hs[i] += h[i];
+            if (gi == 0) hs[8] = h[8];  // every pass scans every piece
+        }
+        if (again) {
+            b.carries.clear();
+            continue;
+        }
+        MCZI_HIP(hipStreamSynchronize(s));
+        p.out_kmer.swap(p.acc_kmer);
+        p.out_count.swap(p.acc_count);
+        p.out_part.swap(p.acc_part);
+        if (wide) p.out_hi.swap(p.acc_hi);
+        p.out_cap = cap;
+        for (int i = 0; i < 16; i++) h[i] = hs[i];
+        h[1] = used;
+        p.n_records = hs[12];
+        p.n_staged = hs[11];
         return;
     }
 }
@@ -2761,7 +2953,6 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
     mczi_run_stats st{};
     b.k = k;
     if (mode == MCZI_MODE_DATASETS && b.n_datasets > 32) {
-        b.carries.clear();
         auto t0 = std::chrono::steady_clock::now();
         uint64_t n_out = 0, n_distinct = 0;
         count_dataset_groups(b, p, cfg, launch, tm, h, n_out, n_distinct);
@@ -2777,6 +2968,7 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
         st.unique_minimizers = unique_min;
         st.minimizers_retained = retained_min;
         st.ms_pass1 = ms_pass1;
+        st.partition_groups = p.grp_n;
         b.last = st;
         if (stats) *stats = st;
         return;
@@ -2784,12 +2976,6 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
     p.ds_lo = 0;
     count_pass(b, p, cfg, mode, launch, tm, want_times, h);
     const int t_dev_end = tm.n - 1;
-    b.carries.clear();
-    if (h[7]) {  // counts that passed 2^32: 2^32 per carry entry
-        std::vector<unsigned long long> c(2 * CARRY_CAP);
-        d.download(c.data(), p.carry.as<void>(), c.size() * 8, s);
-        for (uint64_t i = 0; i < h[7]; i++) b.carries[{c[i], c[CARRY_CAP + i]}] += 1ull << 32;
-    }
     const uint64_t n_slots = h[1], n_real = h[1] - h[6];
     const int tsort0 = tm.n;
     tm.mark(s);
@@ -2812,6 +2998,7 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
     st.ms_pass1 = ms_pass1;
     st.internal_partitions = p.p_int;
     st.lds_slots = CNT_SLOTS;
+    st.partition_groups = p.grp_n;
     if (want_times && t_dev_end >= 7) {
         // marks: 0 start, 1 clears, 2 scan + staging, 3 clears,
         // 4 partitioning, 5 count, 6 spill emit
@@ -2889,6 +3076,7 @@ void batch_download_records(Batch &b, std::vector<uint64_t> &rec, std::vector<ui
                             uint32_t &p_int) {
     require(b.plan != nullptr && b.plan->k != 0, MCZI_E_INVALID, "the batch has not been counted");
     McPlan &p = *b.plan;
+    require(p.grp_n <= 1, MCZI_E_INVALID, "a count in partition groups keeps no super-k-mer records");
     Device &d = Device::get();
     const bool wide = p.k > 32;
     rec_words = wide ? 4 : 2;
