@@ -11,7 +11,7 @@ The other workloads are reported beside it, each with the same fields:
       1% substitutions; k31 m21 x5; FASTA-path counting;
   c3: 50 Mbp genome (seed 3), lognormal long reads (median 8 kbp, sigma 0.8,
       [500, 100000]) at 20x with 8% errors (4/2/2 sub/ins/del) and 150 bp
-      reads at 10x in one batch; k31 m21 x3; autotuned partitioning;
+      reads at 10x in one batch; k31 m21 x3;
   c1: 1 Mbp genome (seed 1), 200k 150 bp reads, 0.5% substitutions; k31 m21
       x1, 64 partitions, KFF (KMC-style single pass);
   zi: served subtraction, index = canonical k-mers of a 100 Mbp genome
@@ -21,7 +21,9 @@ The other workloads are reported beside it, each with the same fields:
 A step = one pass of the counting path over the whole read set, resident in
 HBM before timing starts: minimizer scan + super-k-mer records -> two-pass
 partitioning into buckets -> per-partition LDS counting with HBM spill ->
-count > x -> output order. `plain` runs the same counts with one scan launch
+count > x -> output order. Each MC workload is autotuned first (LDS table
+size, table fill and so the partition count, scan blocks per CU; untimed).
+`plain` runs the same counts with one scan launch
 per input (dataset file) or per 128 Mi bases; `cpu_baseline` is the oracle's
 restatement of the reference CPU path timed on a stated sample.
 """
@@ -120,13 +122,13 @@ def make_c4(M):
 
 # name: (maker, k, m, x, partitions, mode, order, autotune, description)
 MC = {
-    "c4_fof_32_datasets_20mbp": (make_c4, 31, 21, 5, 1024, "datasets", "encoded", False,
-                                 "fof dataset presence, 32 datasets in one segmented batch"),
-    "c2_short_reads_100mbp": (make_c2, 31, 21, 5, 1024, "single", "encoded", False,
-                              "FASTA-path counting (the pass-1 filter is output-neutral and not run)"),
+    "c4_fof_32_datasets_20mbp": (make_c4, 31, 21, 5, 1024, "datasets", "encoded", True,
+                                 "fof dataset presence, 32 datasets in one segmented batch (autotuned)"),
+    "c2_short_reads_100mbp": (make_c2, 31, 21, 5, 1024, "single", "encoded", True,
+                              "FASTA-path counting, every super-k-mer staged (autotuned)"),
     "c3_long_and_short_reads_50mbp": (make_c3, 31, 21, 3, 1024, "single", "encoded", True,
-                                      "long and short reads in one batch, autotuned partitioning"),
-    "c1_small_1mbp": (make_c1, 31, 21, 1, 64, "single", "kff", False, "KMC-style KFF counting"),
+                                      "long and short reads in one batch (autotuned)"),
+    "c1_small_1mbp": (make_c1, 31, 21, 1, 64, "single", "kff", True, "KMC-style KFF counting (autotuned)"),
 }
 
 
@@ -203,8 +205,10 @@ def mc_workload(M, name, steps, warmup, cpu=True, plain=True):
     print(f"[bench] {name}: uploaded ({upload_s:.2f} s)", file=sys.stderr, flush=True)
     tuned = None
     if tune:
+        t_tune = time.perf_counter()
         tuned = batch.autotune(cfg, mode, M.LAUNCH_BATCHED, order)
-        print(f"[bench] {name}: autotuned, {tuned:.0f} k-mer instances per partition", file=sys.stderr, flush=True)
+        print(f"[bench] {name}: autotuned in {time.perf_counter() - t_tune:.1f} s, {tuned:.0f} k-mer instances "
+              f"per partition", file=sys.stderr, flush=True)
     for i in range(warmup):
         st = batch.count(cfg, mode, M.LAUNCH_BATCHED, order)
         print(f"[bench] {name} warmup {i}: {st.ms_total + st.ms_sort:.1f} ms device, out {st.n_out}",
@@ -220,7 +224,9 @@ def mc_workload(M, name, steps, warmup, cpu=True, plain=True):
            "bases": nb, "fragments": int(len(off) - 1), "generate_s": gen_s,
            "value": nb * steps / wall, "unit": "bases/s", "ms_per_step": 1e3 * wall / steps,
            "device_ms_per_step": sum(dev_ms) / len(dev_ms), "upload_ms": upload_s * 1e3,
-           "pcie_inclusive_value": nb / (wall / steps + upload_s), "autotuned_instances_per_partition": tuned}
+           "pcie_inclusive_value": nb / (wall / steps + upload_s), "autotuned_instances_per_partition": tuned,
+           "tuned": {"lds_slots": st.lds_slots, "internal_partitions": st.internal_partitions,
+                     "scan_blocks_per_cu": st.scan_blocks_per_cu}}
     # the two-pass mode (the reference's pass 1 run on the device, filtering staging)
     two = M.MODE_DATASETS_TWO_PASS if mode == M.MODE_DATASETS else M.MODE_TWO_PASS
     st2 = batch.count(cfg, two, M.LAUNCH_BATCHED, order)

@@ -286,6 +286,9 @@ typedef struct {
     /* partition-group passes the count took (1: one pass; more when one
      * pass would not fit in device memory, or MCZI_GROUP_PASSES asks) */
     uint64_t partition_groups;
+    /* scan blocks per CU of the persistent scan grid (tuned; see
+     * mczi_batch_autotune) */
+    uint64_t scan_blocks_per_cu;
 } mczi_run_stats;
 
 /* Count the k-mers of a batch on the device. Results stay on the device
@@ -294,10 +297,14 @@ typedef struct {
 int mczi_batch_count(mczi_batch *b, const mczi_counter_config *cfg, int mode, int launch,
                      int order, mczi_run_stats *stats);
 int mczi_batch_download(mczi_batch *b, mczi_counted_kmer **out, size_t *n_out);
-/* Autotuner (no counterpart in the reference): times the count at several
- * internal partition sizes (expected k-mer instances per LDS table) and keeps
- * the fastest for later mczi_batch_count calls on this batch; the result
- * does not change the output. *instances_per_partition gets the choice. */
+/* Autotuner (no counterpart in the reference): times the count over the LDS
+ * table size (2^10, 2^11, 2^12 slots; k <= 32), the expected k-mer instances
+ * per internal partition (which sets the partition count) and the scan's
+ * blocks per CU, keeps the fastest for later mczi_batch_count calls on this
+ * batch, and appends it to the tuning cache (MCZI_TUNE_CACHE, else
+ * $HOME/.cache/mczi/autotune.tsv), which later counts of the same device,
+ * k, m, mode, order and batch shape use. The result does not change the
+ * output. *instances_per_partition gets the chosen fill. */
 int mczi_batch_autotune(mczi_batch *b, const mczi_counter_config *cfg, int mode, int launch, int order,
                         double *instances_per_partition);
 

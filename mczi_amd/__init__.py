@@ -59,7 +59,7 @@ class RunStats(C.Structure):
         ("internal_partitions", C.c_uint32), ("lds_slots", C.c_uint32)] + [
         (n, C.c_double) for n in ("ms_scan1", "ms_scan2", "ms_count", "ms_sort", "ms_total")] + [
         ("unique_minimizers", C.c_uint64), ("minimizers_retained", C.c_uint64), ("ms_pass1", C.c_double),
-        ("partition_groups", C.c_uint64)]
+        ("partition_groups", C.c_uint64), ("scan_blocks_per_cu", C.c_uint64)]
 
     def as_dict(self) -> dict:
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -11,6 +11,8 @@
 #include <rocprim/device/device_reduce_by_key.hpp>
 #include <rocprim/device/device_scan.hpp>
 
+#include <sys/stat.h>
+
 #include <algorithm>
 #include <array>
 #include <chrono>
@@ -240,6 +242,97 @@ __device__ __forceinline__ void stage_pad(const ScanArgs &a, uint64_t b, uint64_
 // WC != 0 fixes the window length w at compile time (the reference default
 // k = 31, m = 21 has w = 11), which unrolls the minimum over the middle
 // hashes and makes every s_hash offset a constant.
+// A piece of a tile as staging needs it: windows [s, e), its internal
+// partition q and dataset within the group, and whether it is staged.
+struct PieceInfo {
+    int s, e;
+    uint32_t q, ds;
+    bool staged;
+};
+
+// Piece pi of the tile (pst: starts | fragment << 11; v: selected hash per
+// window): its window range clipped to its fragment's last window, whether it
+// is staged (fof: the current group of 32 datasets; two-pass: a key above the
+// threshold, add_filtered_superkmers_packed, src/lib.rs:3483-3523; partition
+// groups: the current group), its partition and its record / byte counts.
+__device__ __forceinline__ PieceInfo piece_info(const ScanArgs &a, int pi, int np, int tl, int k, uint64_t f0,
+                                                uint64_t t, const uint32_t *pst, const int32_t *s_fs,
+                                                const uint32_t *s_v, const uint8_t *s_flag, int &my_nrec,
+                                                unsigned long long &my_bytes) {
+    PieceInfo P{0, 0, 0u, 0u, false};
+    const uint32_t pw = pst[pi];
+    P.s = (int)(pw & 2047u);
+    const int jf = (int)(pw >> 11);
+    int e = (pi + 1 < np) ? (int)(pst[pi + 1] & 2047u) : tl;
+    const int vend = s_fs[jf + 1] - k + 1;
+    if (vend < e) e = vend;
+    P.e = e;
+    if (a.dataset != nullptr) {
+        P.ds = a.dataset[f0 + jf] - a.ds_lo;
+        if (P.ds >= 32u) return P;
+    }
+    const uint32_t key = spread_hash_u32(s_v[P8(P.s)]);
+    if (a.mtab != nullptr && !mtab_has(a, key)) return P;
+    const uint32_t hq = spread_hash_u32(key);
+    if (a.grp_n > 1 && (uint32_t)(((uint64_t)hq * a.grp_n) >> 32) != a.grp_i) return P;
+    P.staged = true;
+    P.q = a.p_mask ? (hq & a.p_mask) : hq % a.p_int;
+    my_nrec += e - P.s <= (int)a.wrec ? 1 : (e - P.s + (int)a.wrec - 1) / (int)a.wrec;
+    my_bytes += skm_bytes((uint32_t)(e - P.s + k - 1));
+    if (a.tile_first != nullptr) {
+        // a run cut by the tile edge: the first piece starts at window 0
+        // inside a fragment that began earlier, the last one ends at the tile
+        // end inside a fragment that goes on
+        const unsigned long long ent =
+            ((unsigned long long)key << 32) | (uint32_t)(e - P.s) | (np == 1 ? 0x80000000ull : 0ull);
+        if (pi == 0 && P.s == 0 && !(s_flag[P8(0)] & 2)) a.tile_first[t] = ent;
+        if (pi == np - 1 && e == tl && vend > tl) a.tile_last[t] = ent;
+    }
+    return P;
+}
+
+// The records of a staged piece from slot pos on (bases [start, end + k - 1)
+// of each chunk of at most wrec windows, src/lib.rs:3483-3560); returns the
+// next slot. Bases are stored in MC's 2-bit code (A0 C1 G2 T3,
+// src/lib.rs:3192-3197), base j of the record at bits 2j.
+template <bool WIDE>
+__device__ __forceinline__ uint64_t write_piece(const ScanArgs &a, const PieceInfo &P, uint64_t pos,
+                                                const uint32_t *s_words, int rb0, int k) {
+    for (int cs = P.s; cs < P.e; cs += (int)a.wrec) {
+        const int ce = min(cs + (int)a.wrec, P.e);
+        const uint32_t L = (uint32_t)(ce - cs + k - 1);
+        if constexpr (WIDE) {
+            Rec32 rw;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                uint64_t x = extract32_bases(s_words, (uint64_t)(rb0 + cs + 32 * c));
+                const int nbc = (int)L - 32 * c;  // bases of this word
+                if (nbc <= 0) x = 0;
+                else if (nbc < 32) x &= (1ull << (2 * nbc)) - 1ull;
+                rw.w[c] = x ^ ((x >> 1) & 0x5555555555555555ull);  // packed code -> MC code
+            }
+            rw.w[3] = (rw.w[3] & 0xffffull) | ((uint64_t)L << 16) | ((uint64_t)P.q << 32);
+            if (pos < a.stage_cap) a.stage_rec_w[pos] = rw;
+            else atomicOr(a.err, 2ull);
+        } else {
+            uint64_t lo64 = extract32_bases(s_words, (uint64_t)(rb0 + cs));
+            uint64_t hi64 = 0;
+            if (L < 32) lo64 &= (1ull << (2 * L)) - 1ull;
+            if (L > 32) {
+                hi64 = extract32_bases(s_words, (uint64_t)(rb0 + cs + 32));
+                hi64 &= (1ull << (2 * (L - 32))) - 1ull;
+            }
+            lo64 ^= (lo64 >> 1) & 0x5555555555555555ull;  // packed code -> MC code
+            hi64 ^= (hi64 >> 1) & 0x5555555555555555ull;
+            hi64 |= ((uint64_t)L << 24) | ((uint64_t)P.ds << 30) | ((uint64_t)P.q << 35);
+            if (pos < a.stage_cap) a.stage_rec[pos] = make_ulonglong2(lo64, hi64);
+            else atomicOr(a.err, 2ull);
+        }
+        pos++;
+    }
+    return pos;
+}
+
 template <int MODE, bool GW, bool WIDE = false, int WC = 0>
 __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_scan(ScanArgs a) {
     __shared__ uint32_t s_words[WORDS_SPAN];
@@ -549,118 +642,53 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             }
             continue;
         }
-        // Which of this thread's pieces (at most TILE / SCAN_THREADS + 1) are
-        // staged: fof keeps the current group of 32 datasets, the two-pass
-        // mode the keys above the threshold (add_filtered_superkmers_packed,
-        // src/lib.rs:3483-3523).
-        int my_nrec = 0;
-        uint32_t staged = 0;
-        for (int pi = tid, slot = 0; pi < np; pi += SCAN_THREADS, slot++) {
-            const uint32_t pw = s_u.run.pst[pi];
-            const int s = (int)(pw & 2047u);
-            int e = (pi + 1 < np) ? (int)(s_u.run.pst[pi + 1] & 2047u) : tl;
-            const int vend = s_fs[(pw >> 11) + 1] - k + 1;
-            if (vend < e) e = vend;
-            my_pieces++;
-            if (a.dataset != nullptr && a.dataset[f0 + (pw >> 11)] - a.ds_lo >= 32u) continue;
-            const uint32_t key = spread_hash_u32(s_u.run.v[P8(s)]);
-            if (a.mtab != nullptr && !mtab_has(a, key)) continue;
-            if (a.grp_n > 1 && (uint32_t)(((uint64_t)spread_hash_u32(key) * a.grp_n) >> 32) != a.grp_i) continue;
-            staged |= 1u << slot;
-            my_nrec += e - s <= (int)a.wrec ? 1 : (e - s + (int)a.wrec - 1) / (int)a.wrec;
-            my_bytes += skm_bytes((uint32_t)(e - s + k - 1));
-            if (a.tile_first != nullptr) {
-                // a run cut by the tile edge: the first piece starts at window
-                // 0 inside a fragment that began earlier, the last one ends at
-                // the tile end inside a fragment that goes on
-                const unsigned long long ent = ((unsigned long long)key << 32) | (uint32_t)(e - s) |
-                                               (np == 1 ? 0x80000000ull : 0ull);
-                if (pi == 0 && s == 0 && !(s_flag[P8(0)] & 2)) a.tile_first[t] = ent;
-                if (pi == np - 1 && e == tl && vend > tl) a.tile_last[t] = ent;
-            }
-        }
-        int tot = 0;
-        const int off = block_exclusive_scan(my_nrec, &tot, s_wsum);
-        // Staging slots: each block takes chunks of STAGE_CHUNK slots (or the
-        // tile's records, if more) from the batch-wide counter and hands out
-        // consecutive slots to its tiles. A tile that does not fit in the
-        // rest of the chunk starts a new one; the rest is marked as padding,
-        // which partitioning skips. Slots at or beyond stage_cap are
-        // counted but not written: the host then sizes the buffer from the
-        // counts and scans again. The order of tiles in the buffer is free:
-        // partitioning and counting do not depend on it.
-        if (tid == 0) {
-            s_hole_b = s_hole_e = 0;
-            unsigned long long base = 0;
-            if (tot) {
-                if (chunk_cur + (unsigned long long)tot > chunk_end) {
-                    s_hole_b = chunk_cur;
-                    s_hole_e = chunk_end;
-                    const unsigned long long sz = (unsigned long long)tot > STAGE_CHUNK ? tot : STAGE_CHUNK;
-                    chunk_cur = atomicAdd(a.stage_ctr, sz);
-                    chunk_end = chunk_cur + sz;
+        // Pieces are staged in rounds of 2 * SCAN_THREADS (one round for
+        // almost every tile): each thread takes two pieces of the round, keeps
+        // their data in registers between counting and writing their records,
+        // and the round's records get consecutive staging slots.
+        my_pieces += (uint64_t)(np > tid ? (np - tid + SCAN_THREADS - 1) / SCAN_THREADS : 0);
+        for (int rb = 0; rb < np; rb += 2 * SCAN_THREADS) {
+            int my_nrec = 0;
+            PieceInfo P0{0, 0, 0u, 0u, false}, P1{0, 0, 0u, 0u, false};
+            if (rb + tid < np)
+                P0 = piece_info(a, rb + tid, np, tl, k, f0, t, s_u.run.pst, s_fs, s_u.run.v, s_flag, my_nrec,
+                                my_bytes);
+            if (rb + tid + SCAN_THREADS < np)
+                P1 = piece_info(a, rb + tid + SCAN_THREADS, np, tl, k, f0, t, s_u.run.pst, s_fs, s_u.run.v, s_flag,
+                                my_nrec, my_bytes);
+            int tot = 0;
+            const int off = block_exclusive_scan(my_nrec, &tot, s_wsum);
+            // Staging slots: each block takes chunks of STAGE_CHUNK slots (or
+            // the round's records, if more) from the batch-wide counter and
+            // hands out consecutive slots to its rounds. A round that does not
+            // fit in the rest of the chunk starts a new one; the rest is marked
+            // as padding, which partitioning skips. Slots at or beyond
+            // stage_cap are counted but not written: the host then sizes the
+            // buffer from the counts and scans again. The order of records in
+            // the buffer is free: partitioning and counting do not depend on it.
+            if (tid == 0) {
+                s_hole_b = s_hole_e = 0;
+                unsigned long long base = 0;
+                if (tot) {
+                    if (chunk_cur + (unsigned long long)tot > chunk_end) {
+                        s_hole_b = chunk_cur;
+                        s_hole_e = chunk_end;
+                        const unsigned long long sz = (unsigned long long)tot > STAGE_CHUNK ? tot : STAGE_CHUNK;
+                        chunk_cur = atomicAdd(a.stage_ctr, sz);
+                        chunk_end = chunk_cur + sz;
+                    }
+                    base = chunk_cur;
+                    chunk_cur += (unsigned long long)tot;
                 }
-                base = chunk_cur;
-                chunk_cur += (unsigned long long)tot;
+                s_sbase = base;
+                my_records += (unsigned long long)tot;
             }
-            s_sbase = base;
-            my_records += (unsigned long long)tot;
-        }
-        __syncthreads();
-        stage_pad<WIDE>(a, s_hole_b, s_hole_e);
-        {
+            __syncthreads();
+            stage_pad<WIDE>(a, s_hole_b, s_hole_e);
             uint64_t pos = s_sbase + (uint64_t)off;
-            for (int pi = tid, slot = 0; pi < np; pi += SCAN_THREADS, slot++) {
-                if (!(staged & (1u << slot))) continue;
-                const uint32_t pw = s_u.run.pst[pi];
-                const int s = (int)(pw & 2047u);
-                const int jf = (int)(pw >> 11);
-                int e = (pi + 1 < np) ? (int)(s_u.run.pst[pi + 1] & 2047u) : tl;
-                const int vend = s_fs[jf + 1] - k + 1;
-                if (vend < e) e = vend;
-                const uint32_t hq = spread_hash_u32(spread_hash_u32(s_u.run.v[P8(s)]));
-                const uint32_t q = a.p_mask ? (hq & a.p_mask) : hq % a.p_int;
-                const uint64_t ds = a.dataset != nullptr ? (uint64_t)(a.dataset[f0 + jf] - a.ds_lo) : 0ull;
-                for (int cs = s; cs < e; cs += (int)a.wrec) {
-                    const int ce = min(cs + (int)a.wrec, e);
-                    const uint32_t L = (uint32_t)(ce - cs + k - 1);
-                    if constexpr (WIDE) {
-                        Rec32 rw;
-#pragma unroll
-                        for (int c = 0; c < 4; c++) {
-                            uint64_t x = extract32_bases(s_words, (uint64_t)(rb0 + cs + 32 * c));
-                            const int nbc = (int)L - 32 * c;  // bases of this word
-                            if (nbc <= 0) x = 0;
-                            else if (nbc < 32) x &= (1ull << (2 * nbc)) - 1ull;
-                            rw.w[c] = x ^ ((x >> 1) & 0x5555555555555555ull);  // packed code -> MC code
-                        }
-                        rw.w[3] = (rw.w[3] & 0xffffull) | ((uint64_t)L << 16) | ((uint64_t)q << 32);
-                        if (pos < a.stage_cap) {
-                            a.stage_rec_w[pos] = rw;
-                        } else {
-                            atomicOr(a.err, 2ull);
-                        }
-                        pos++;
-                        continue;
-                    }
-                    uint64_t lo64 = extract32_bases(s_words, (uint64_t)(rb0 + cs));
-                    uint64_t hi64 = 0;
-                    if (L < 32) lo64 &= (1ull << (2 * L)) - 1ull;
-                    if (L > 32) {
-                        hi64 = extract32_bases(s_words, (uint64_t)(rb0 + cs + 32));
-                        hi64 &= (1ull << (2 * (L - 32))) - 1ull;
-                    }
-                    lo64 ^= (lo64 >> 1) & 0x5555555555555555ull;  // packed code -> MC code
-                    hi64 ^= (hi64 >> 1) & 0x5555555555555555ull;
-                    hi64 |= ((uint64_t)L << 24) | (ds << 30) | ((uint64_t)q << 35);
-                    if (pos < a.stage_cap) {
-                        a.stage_rec[pos] = make_ulonglong2(lo64, hi64);
-                    } else {
-                        atomicOr(a.err, 2ull);
-                    }
-                    pos++;
-                }
-            }
+            if (P0.staged) pos = write_piece<WIDE>(a, P0, pos, s_words, rb0, k);
+            if (P1.staged) pos = write_piece<WIDE>(a, P1, pos, s_words, rb0, k);
+            if (rb + 2 * SCAN_THREADS < np) __syncthreads();  // s_sbase / s_hole of the next round
         }
     }
     if constexpr (MODE == STAGE) {
@@ -1133,7 +1161,7 @@ struct CountArgs {
 // 2^32 per entry to its count on download. Error bit 32: list full.
 constexpr uint32_t CARRY_CAP = 4096;
 
-__device__ __noinline__ void carry_add(const CountArgs &a, uint64_t lo, uint64_t hi) {
+__device__ __forceinline__ void carry_add(const CountArgs &a, uint64_t lo, uint64_t hi) {
     const unsigned long long i = atomicAdd(&a.ctr[7], 1ull);
     if (i < CARRY_CAP) {
         a.carry_lo[i] = lo;
@@ -1144,7 +1172,7 @@ __device__ __noinline__ void carry_add(const CountArgs &a, uint64_t lo, uint64_t
 }
 
 // Whether (lo, hi) has a carry entry (a linear pass over the short list).
-__device__ __noinline__ bool has_carry(const CountArgs &a, uint64_t lo, uint64_t hi) {
+__device__ __forceinline__ bool has_carry(const CountArgs &a, uint64_t lo, uint64_t hi) {
     const unsigned long long n =
         min((unsigned long long)CARRY_CAP, __hip_atomic_load(&a.ctr[7], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     for (unsigned long long i = 0; i < n; i++)
@@ -1161,11 +1189,14 @@ __device__ __forceinline__ bool val_update(uint32_t *p, uint32_t v) {
     return false;
 }
 
-// The spill table takes new keys only while it is at most half full; past
-// that every spill fails fast with error bit 1 and the host counts again with
-// smaller partitions (and a larger table), instead of probing a full table.
+// The spill table takes new keys only while at most an eighth of it is used
+// (one distinct spilled k-mer per 128 windows); past that every spill fails
+// fast with error bit 1 and the host counts again with smaller partitions
+// (and a larger table), instead of probing a crowded table. Tables that
+// overflow that much are far too full for their partitions: a normal count
+// spills a few hundred k-mers.
 __device__ __forceinline__ bool spill_table_full(const CountArgs &a) {
-    return __hip_atomic_load(&a.ctr[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > (a.ovf_mask + 1) / 2;
+    return __hip_atomic_load(&a.ctr[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > (a.ovf_mask + 1) / 8;
 }
 
 template <int MODE>
@@ -1283,11 +1314,12 @@ __device__ __forceinline__ uint64_t canon_from_mc(uint64_t x, uint64_t mask, int
 
 // LDS table slot of a k-mer: multiplicative mix of both 32-bit halves (the
 // spill table uses spread_hash_u64, an independent function).
+template <int LS = CNT_LOG2_SLOTS>
 __device__ __forceinline__ uint32_t lds_slot(uint64_t key) {
     uint32_t h = ((uint32_t)key * 0x9E3779B1u) ^ ((uint32_t)(key >> 32) * 0x85EBCA77u);
     h ^= h >> 16;
     h *= 0x2C1B3C6Du;
-    return h >> (32 - CNT_LOG2_SLOTS);
+    return h >> (32 - LS);
 }
 
 // Per partition: all k-mers of its super-k-mer records go into an LDS
@@ -1307,10 +1339,11 @@ __device__ __forceinline__ uint32_t lds_slot(uint64_t key) {
 // from the 128-bit record
 // (the same canonical k-mers add_packed_superkmer_kmer_counts rolls,
 // src/lib.rs:3589-3614).
-template <int MODE>
+template <int MODE, int LS = CNT_LOG2_SLOTS>
 __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
-    __shared__ unsigned long long s_keys[CNT_SLOTS];
-    __shared__ uint32_t s_vals[CNT_SLOTS];
+    constexpr int SLOTS = 1 << LS;  // LDS table slots (the tuner picks 2^10 .. 2^12)
+    __shared__ unsigned long long s_keys[SLOTS];
+    __shared__ uint32_t s_vals[SLOTS];
     __shared__ ulonglong2 s_rec[CNT_THREADS / 64][64];  // each wave's current 64 records
     __shared__ uint8_t s_mark[CNT_THREADS / 64][64];    // record starts in the current 64 k-mers
     __shared__ int s_wsum[CNT_THREADS / 64];
@@ -1326,7 +1359,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
     const int rshift = 64 - 2 * k;
     unsigned long long my_distinct = 0;
     // the table starts empty; emit leaves it empty again for the next partition
-    for (int i = tid; i < CNT_SLOTS; i += CNT_THREADS) {
+    for (int i = tid; i < SLOTS; i += CNT_THREADS) {
         s_keys[i] = EMPTY_KEY;
         s_vals[i] = 0;
     }
@@ -1370,7 +1403,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
                 const uint64_t lo = sh == 0 ? rx : (sh < 64 ? ((rx >> sh) | (rd << (64 - sh))) : (rd >> (sh - 64)));
                 const unsigned long long key = canon_from_mc(lo & kmask, kmask, rshift);
                 const uint32_t val = (MODE == 0) ? 1u : (1u << (uint32_t)((ry >> 30) & 31u));
-                uint32_t slot = lds_slot(key);
+                uint32_t slot = lds_slot<LS>(key);
                 // Home slot: a plain load settles the common case (key present,
                 // or empty slot claimed by one CAS); later probes use the CAS
                 // alone, which returns whatever key holds the slot.
@@ -1389,7 +1422,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
                     }
                 }
                 if (!done) {
-                    slot = (slot + 1) & (CNT_SLOTS - 1);
+                    slot = (slot + 1) & (SLOTS - 1);
                     for (int p = 1; p < CNT_MAXPROBE; p++) {
                         const unsigned long long prev = atomicCAS(&s_keys[slot], EMPTY_KEY, key);
                         if (prev == EMPTY_KEY || prev == key) {
@@ -1400,7 +1433,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
                             done = true;
                             break;
                         }
-                        slot = (slot + 1) & (CNT_SLOTS - 1);
+                        slot = (slot + 1) & (SLOTS - 1);
                     }
                 }
                 if (!done) overflow_insert<MODE>(a, key, val, q);
@@ -1408,7 +1441,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
         }
         __syncthreads();
         // emit: count survivors, one reservation per partition, then write
-        constexpr int PER = CNT_SLOTS / CNT_THREADS;
+        constexpr int PER = SLOTS / CNT_THREADS;
         uint32_t emask = 0;
         int cnt = 0;
         const bool wrapped = MODE == 0 && s_wrap;
@@ -1841,6 +1874,13 @@ struct McPlan {
     uint64_t out_cap = 0;
     uint64_t rec_sig = 0;
     double target_override = 0;
+    // tuned launch geometry: LDS table of 2^cnt_ls slots (k <= 32), scan
+    // blocks per CU (0: every resident block); tune_sig names the
+    // configuration the tuning cache was last consulted for
+    int cnt_ls = CNT_LOG2_SLOTS;
+    uint32_t scan_bpc = 0;
+    uint64_t tune_sig = 0;
+    double cache_target = 0;  // instances per partition from the tuning cache
     double effective_target = 0;  // instances per partition of the last count (after spill retries)
     // graph cache
     hipGraphExec_t graph = nullptr;
@@ -2041,10 +2081,10 @@ static uint32_t choose_partitions(uint64_t n_windows, uint32_t p_user, double ta
     return (uint32_t)p;
 }
 
-static double tuned_target() {
+static double tuned_target(const McPlan &p) {
     const char *e = getenv("MCZI_PART_TARGET");
     if (e && *e) return std::max(64.0, atof(e));
-    return 2.0 * CNT_SLOTS;
+    return 2.0 * (double)(1 << p.cnt_ls);
 }
 
 struct Timer {
@@ -2107,11 +2147,16 @@ static ScanArgs scan_args(Batch &b, McPlan &p, const mczi_counter_config &cfg) {
 // Launches the scan (one grid per input chunk with the plain launch, one for
 // the batch otherwise); returns the number of blocks, or only counts them
 // when s_launch is false.
+static unsigned scan_blocks_per_cu(const McPlan &p) {
+    static const unsigned scan_res = resident_blocks((const void *)k_scan<STAGE, true>, SCAN_THREADS);
+    return p.scan_bpc ? std::min<unsigned>(p.scan_bpc, scan_res) : scan_res;
+}
+
 template <int MODE = STAGE>
 static uint64_t launch_scan(McPlan &p, const ScanArgs &sa, int launch, hipStream_t s, bool s_launch = true) {
     Device &d = Device::get();
-    static const unsigned scan_res = resident_blocks((const void *)k_scan<STAGE, true>, SCAN_THREADS);
-    const unsigned scan_grid_max = (unsigned)d.cu_count() * scan_res;
+    const unsigned per_cu = scan_blocks_per_cu(p);
+    const unsigned scan_grid_max = (unsigned)d.cu_count() * per_cu;
     uint64_t blocks = 0;
     auto go = [&](uint64_t tb, uint64_t te) {
         if (te <= tb) return;
@@ -2224,6 +2269,25 @@ static PartArgs part_args(McPlan &p) {
 // partition LDS counting with HBM spill and the emit. Every launch is a
 // kernel of this file (no library call, no memset), and no size is read back
 // by the host, so the stage can be captured in a graph with stage A.
+// The count kernel with a table of 2^ls slots (2^10, 2^11 or 2^12).
+using CountFn = void (*)(CountArgs);
+template <int MODE>
+static CountFn count_kernel(int ls) {
+    return ls == 10 ? k_count<MODE, 10> : ls == 12 ? k_count<MODE, 12> : k_count<MODE, 11>;
+}
+
+// Blocks of the count grid: every resident block, at most one per partition
+// (the output padding is at most one chunk per block).
+static uint64_t count_grid_blocks(const McPlan &p, bool wide) {
+    Device &d = Device::get();
+    static const unsigned res[3] = {resident_blocks((const void *)k_count<0, 10>, CNT_THREADS),
+                                    resident_blocks((const void *)k_count<0, 11>, CNT_THREADS),
+                                    resident_blocks((const void *)k_count<0, 12>, CNT_THREADS)};
+    static const unsigned wide_res = resident_blocks((const void *)k_count_wide, CNT_THREADS);
+    const unsigned r = wide ? wide_res : res[std::min(std::max(p.cnt_ls, 10), 12) - 10];
+    return std::min<uint64_t>(p.p_int, (uint64_t)d.cu_count() * r);
+}
+
 static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg, int mode,
                             int launch, hipStream_t s, Timer *tm) {
     Device &d = Device::get();
@@ -2259,8 +2323,7 @@ static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg,
     if (tm) tm->mark(s);
     CountArgs ca = count_args(p, cfg);
     // persistent grid: exactly the blocks that are resident at once
-    static const unsigned count_res = resident_blocks((const void *)k_count<0>, CNT_THREADS);
-    const unsigned cgrid = (unsigned)std::min<uint64_t>(p.p_int, (uint64_t)d.cu_count() * count_res);
+    const unsigned cgrid = (unsigned)count_grid_blocks(p, wide);
     const unsigned ogrid = grid_for(p.ovf_slots, 256, 8192);
     if (wide) {
         static const unsigned wide_res = resident_blocks((const void *)k_count_wide, CNT_THREADS);
@@ -2269,24 +2332,17 @@ static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg,
         if (tm) tm->mark(s);
         hipLaunchKernelGGL(k_overflow_emit_wide, dim3(ogrid), dim3(256), 0, s, ca);
     } else if (mode == MCZI_MODE_DATASETS) {
-        hipLaunchKernelGGL(k_count<1>, dim3(cgrid), dim3(CNT_THREADS), 0, s, ca);
+        hipLaunchKernelGGL(count_kernel<1>(p.cnt_ls), dim3(cgrid), dim3(CNT_THREADS), 0, s, ca);
         if (tm) tm->mark(s);
         hipLaunchKernelGGL(k_overflow_emit<1>, dim3(ogrid), dim3(256), 0, s, ca);
     } else {
-        hipLaunchKernelGGL(k_count<0>, dim3(cgrid), dim3(CNT_THREADS), 0, s, ca);
+        hipLaunchKernelGGL(count_kernel<0>(p.cnt_ls), dim3(cgrid), dim3(CNT_THREADS), 0, s, ca);
         if (tm) tm->mark(s);
         hipLaunchKernelGGL(k_overflow_emit<0>, dim3(ogrid), dim3(256), 0, s, ca);
     }
     if (tm) tm->mark(s);
 }
 
-// Blocks of the count grid (the output padding is at most one chunk each).
-static uint64_t count_grid_blocks(const McPlan &p, bool wide) {
-    Device &d = Device::get();
-    static const unsigned count_res = resident_blocks((const void *)k_count<0>, CNT_THREADS);
-    static const unsigned wide_res = resident_blocks((const void *)k_count_wide, CNT_THREADS);
-    return std::min<uint64_t>(p.p_int, (uint64_t)d.cu_count() * (wide ? wide_res : count_res));
-}
 
 // Output entries [0, n_slots) without the padding, in their order (the
 // unsorted output order).
@@ -2461,7 +2517,9 @@ static bool count_pass_one(Batch &b, McPlan &p, const mczi_counter_config &cfg, 
     const uint32_t k = cfg.k;
     const bool wide = k > 32;
     const bool use_graph = launch == MCZI_LAUNCH_BATCHED && getenv("MCZI_NO_GRAPH") == nullptr;
-    double target = p.target_override > 0 ? p.target_override : tuned_target();
+    double target = p.target_override > 0 ? p.target_override
+                    : p.cache_target > 0  ? p.cache_target
+                                          : tuned_target(p);
     static const unsigned part_res =
         std::min(resident_blocks((const void *)k_part_coarse<ulonglong2>, PT_THREADS),
                  resident_blocks((const void *)k_part_coarse<Rec32>, PT_THREADS));
@@ -2543,6 +2601,7 @@ static bool count_pass_one(Batch &b, McPlan &p, const mczi_counter_config &cfg, 
             uint64_t sig = cfg_sig;
             sig = mix_sig(mix_sig(mix_sig(mix_sig(sig, cfg.threshold), (uint64_t)mode), p.p_int), p.ds_lo);
             sig = mix_sig(mix_sig(sig, p.grp_n), p.grp_i);
+            sig = mix_sig(mix_sig(sig, (uint64_t)p.cnt_ls), p.scan_bpc);
             for (const DevBuf *db : {&b.words, &b.off, &b.dataset, &p.base, &p.cursor, &p.records, &p.ctr,
                                      &p.ovf_keys, &p.ovf_vals, &p.ovf_part, &p.out_kmer, &p.out_count,
                                      &p.out_part, &p.tile_start, &p.tile_frag, &p.tile_len, &p.stage_rec,
@@ -2610,6 +2669,9 @@ static bool count_pass_one(Batch &b, McPlan &p, const mczi_counter_config &cfg, 
             target /= 4;
             continue;
         }
+        // a fill that had to shrink is where the next count of this
+        // configuration starts (an explicit override stays as it is)
+        if (attempt > 0 && p.target_override <= 0) p.cache_target = target;
         p.effective_target = target;
         return true;
     }
@@ -2910,6 +2972,8 @@ static void minimizer_pass(Batch &b, McPlan &p, const mczi_counter_config &cfg, 
     (void)hipEventDestroy(e1);
 }
 
+static void apply_tuning_cache(Batch &b, McPlan &p, const mczi_counter_config &cfg, int mode, int order);
+
 void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch, int order,
                  mczi_run_stats *stats) {
     Device &d = Device::get();
@@ -2933,6 +2997,8 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
     McPlan &p = *b.plan;
     p.n_windows = windows_for(b, k);
     p.k = k;
+    if (p.target_override <= 0) apply_tuning_cache(b, p, cfg, mode, order);
+    if (wide) p.cnt_ls = CNT_LOG2_SLOTS;  // the 128-bit table has one size
     uint64_t unique_min = 0, retained_min = 0;
     double ms_pass1 = 0;
     p.use_mtab = false;
@@ -2962,13 +3028,14 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
         st.n_distinct = n_distinct;
         st.n_out = n_out;
         st.internal_partitions = p.p_int;
-        st.lds_slots = CNT_SLOTS;
+        st.lds_slots = 1u << p.cnt_ls;
         st.tile_windows = TILE;
         st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         st.unique_minimizers = unique_min;
         st.minimizers_retained = retained_min;
         st.ms_pass1 = ms_pass1;
         st.partition_groups = p.grp_n;
+        st.scan_blocks_per_cu = scan_blocks_per_cu(p);
         b.last = st;
         if (stats) *stats = st;
         return;
@@ -2997,8 +3064,9 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
     st.minimizers_retained = retained_min;
     st.ms_pass1 = ms_pass1;
     st.internal_partitions = p.p_int;
-    st.lds_slots = CNT_SLOTS;
+    st.lds_slots = wide ? CNT_SLOTS : 1u << p.cnt_ls;
     st.partition_groups = p.grp_n;
+    st.scan_blocks_per_cu = scan_blocks_per_cu(p);
     if (want_times && t_dev_end >= 7) {
         // marks: 0 start, 1 clears, 2 scan + staging, 3 clears,
         // 4 partitioning, 5 count, 6 spill emit
@@ -3024,28 +3092,154 @@ uint64_t batch_minimizer_count(Batch &b, const mczi_counter_config &cfg) {
     return unique;
 }
 
-// Autotuner: sweep the expected k-mer instances per internal partition (the
-// LDS table fill) and keep the fastest; results do not depend on it.
+// ---- autotuner and its cache ------------------------------------------------
+// Tuned dimensions (none changes a result): the count's LDS table size
+// (2^10, 2^11 or 2^12 slots; k <= 32), the k-mer instances per internal
+// partition (the table fill, which sets the partition count), and the scan's
+// blocks per CU (its persistent grid; each block is four wavefronts over a
+// 1920-window tile, so a read gets one wavefront per 480 of its windows).
+
+// Tuning-cache key: device, k, m, mode, order and the batch's shape (log2 of
+// its bases and of its mean fragment length, its dataset count class).
+static std::string tune_key(const Batch &b, const mczi_counter_config &cfg, int mode, int order) {
+    static const std::string dev = [] {
+        hipDeviceProp_t pr;
+        MCZI_HIP(hipGetDeviceProperties(&pr, Device::get().device()));
+        return std::string(pr.gcnArchName) + ":" + std::to_string(pr.multiProcessorCount);
+    }();
+    auto lg = [](uint64_t v) {
+        int r = 0;
+        while (v > 1) {
+            v >>= 1;
+            r++;
+        }
+        return r;
+    };
+    const uint64_t mean = b.n_frag ? b.n_bases / b.n_frag : 0;
+    const int dsc = b.n_datasets <= 1 ? 1 : b.n_datasets <= 32 ? 32 : 33;
+    char buf[160];
+    snprintf(buf, sizeof buf, "%s:k%u:m%u:mode%d:order%d:b%d:L%d:d%d", dev.c_str(), cfg.k, cfg.minimizer, mode, order,
+             lg(b.n_bases), lg(mean), dsc);
+    return buf;
+}
+
+// MCZI_TUNE_CACHE, else $HOME/.cache/mczi/autotune.tsv; lines of
+// key, table bits, instances per partition, scan blocks per CU, ms.
+static std::string tune_path() {
+    if (const char *e = getenv("MCZI_TUNE_CACHE")) return e;
+    if (const char *h = getenv("HOME")) return std::string(h) + "/.cache/mczi/autotune.tsv";
+    return "";
+}
+
+static std::mutex g_tune_mu;
+
+static bool tune_lookup(const std::string &key, int &ls, double &target, unsigned &bpc) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const std::string path = tune_path();
+    if (path.empty()) return false;
+    FILE *f = fopen(path.c_str(), "r");
+    if (!f) return false;
+    char line[512];
+    bool found = false;
+    while (fgets(line, sizeof line, f)) {
+        char k[256];
+        int l = 0;
+        double t = 0, ms = 0;
+        unsigned c = 0;
+        if (sscanf(line, "%255s %d %lf %u %lf", k, &l, &t, &c, &ms) == 5 && key == k && l >= 10 && l <= 12 &&
+            t > 0) {
+            ls = l;
+            target = t;
+            bpc = c;
+            found = true;  // the last entry for a key wins
+        }
+    }
+    fclose(f);
+    return found;
+}
+
+static void tune_store(const std::string &key, int ls, double target, unsigned bpc, double ms) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const std::string path = tune_path();
+    if (path.empty()) return;
+    const size_t slash = path.find_last_of('/');
+    if (slash != std::string::npos) {
+        // create the directory chain (best effort: the cache is optional)
+        for (size_t i = 1; i <= slash; i++)
+            if (path[i] == '/' || i == slash) (void)mkdir(path.substr(0, i).c_str(), 0755);
+    }
+    if (FILE *f = fopen(path.c_str(), "a")) {
+        fprintf(f, "%s\t%d\t%.1f\t%u\t%.3f\n", key.c_str(), ls, target, bpc, ms);
+        fclose(f);
+    }
+}
+
+// Applies the cached tuning of this configuration, if any, once per
+// configuration (an explicit override or an autotune run wins).
+static void apply_tuning_cache(Batch &b, McPlan &p, const mczi_counter_config &cfg, int mode, int order) {
+    const std::string key = tune_key(b, cfg, mode, order);
+    const uint64_t sig = spread_hash_u64(std::hash<std::string>()(key));
+    if (p.tune_sig == sig) return;
+    p.tune_sig = sig;
+    p.cnt_ls = CNT_LOG2_SLOTS;
+    p.scan_bpc = 0;
+    p.cache_target = 0;
+    int ls;
+    double target;
+    unsigned bpc;
+    if (cfg.k <= 32 && tune_lookup(key, ls, target, bpc)) {
+        p.cnt_ls = ls;
+        p.cache_target = target;
+        p.scan_bpc = bpc;
+    }
+}
+
+// Autotuner: times the count over the table sizes and table fills, then the
+// scan grid, keeps the fastest, and records it in the tuning cache.
 double batch_autotune(Batch &b, const mczi_counter_config &cfg, int mode, int launch, int order) {
     if (!b.plan) b.plan.reset(new McPlan());
     McPlan &p = *b.plan;
-    // a candidate whose tables spill past half the spill table is retried by
-    // the count at a quarter of its size; the size that ran is what counts
-    const double cands[] = {0.5, 1.0, 2.0, 4.0};
-    double best = 0, best_ms = 1e300;
-    for (double c : cands) {
-        p.target_override = c * CNT_SLOTS;
+    const bool wide = cfg.k > 32;
+    auto timed = [&](int ls, double target, unsigned bpc) {
+        p.cnt_ls = ls;
+        p.scan_bpc = bpc;
+        // a candidate whose tables spill past an eighth of the spill table is retried
+        // by the count at a quarter of its size; the size that ran is what
+        // counts
+        p.target_override = target;
         mczi_run_stats st{};
         batch_count(b, cfg, mode, launch, order, &st);  // calibrates / captures
         p.target_override = p.effective_target;
         batch_count(b, cfg, mode, launch, order, &st);
-        const double ms = st.ms_total + st.ms_sort;
+        return st.ms_total + st.ms_sort;
+    };
+    int best_ls = CNT_LOG2_SLOTS;
+    unsigned best_bpc = 0;
+    double best = 0, best_ms = 1e300;
+    const std::vector<int> sizes = wide ? std::vector<int>{CNT_LOG2_SLOTS} : std::vector<int>{10, 11, 12};
+    for (int ls : sizes)
+        for (double c : {0.5, 1.0, 2.0, 4.0}) {
+            const double ms = timed(ls, c * (double)(1 << ls), 0);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = p.effective_target;
+                best_ls = ls;
+            }
+        }
+    for (unsigned bpc : {2u, 3u}) {
+        const double ms = timed(best_ls, best, bpc);
         if (ms < best_ms) {
             best_ms = ms;
-            best = p.effective_target;
+            best_bpc = bpc;
         }
     }
+    p.cnt_ls = best_ls;
+    p.scan_bpc = best_bpc;
     p.target_override = best;
+    const int m = mode == MCZI_MODE_DATASETS_TWO_PASS ? MCZI_MODE_DATASETS
+                  : mode == MCZI_MODE_TWO_PASS        ? MCZI_MODE_SINGLE_PASS
+                                                      : mode;
+    if (!wide) tune_store(tune_key(b, cfg, m, order), best_ls, best, best_bpc, best_ms);
     return best;
 }
 
