@@ -94,13 +94,16 @@ void *Device::host_alloc(size_t bytes, bool allow_pageable) {
     size_t want = 64u << 10;
     while (want < bytes) want <<= 1;
     {
+        // a cached block up to 4x the size is reused: pinning fresh pages
+        // costs far more than the memory a larger block holds
         std::lock_guard<std::mutex> lk(hmu_);
-        auto it = hfree_.find(want);
-        if (it != hfree_.end()) {
+        auto it = hfree_.lower_bound(want);
+        if (it != hfree_.end() && it->first <= 4 * want) {
             void *p = it->second;
+            const size_t have = it->first;
             hfree_.erase(it);
-            hcached_ -= want;
-            hlive_[p] = want;
+            hcached_ -= have;
+            hlive_[p] = have;
             return p;
         }
     }

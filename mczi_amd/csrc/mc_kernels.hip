@@ -114,6 +114,7 @@ struct ScanArgs {
     // staging of super-k-mer records and their internal partition
     ulonglong2 *stage_rec;
     Rec32 *stage_rec_w;  // k > 32
+    uint32_t *stage_q;   // the partition of each staging slot (padding: the pad partition), or null
     uint64_t stage_cap;
     unsigned long long *stage_ctr;  // record slots reserved so far (the batch total at the end)
     unsigned long long *err;
@@ -226,6 +227,7 @@ __device__ __forceinline__ void stage_pad(const ScanArgs &a, uint64_t b, uint64_
     for (uint64_t i = b + threadIdx.x; i < e && i < a.stage_cap; i += SCAN_THREADS) {
         if constexpr (WIDE) a.stage_rec_w[i] = Rec32{{0ull, 0ull, 0ull, ~0ull}};
         else a.stage_rec[i] = make_ulonglong2(0ull, ~0ull);
+        if (a.stage_q) a.stage_q[i] = WIDE ? REC_PAD_QW : REC_PAD_Q;
     }
 }
 
@@ -312,8 +314,12 @@ __device__ __forceinline__ uint64_t write_piece(const ScanArgs &a, const PieceIn
                 rw.w[c] = x ^ ((x >> 1) & 0x5555555555555555ull);  // packed code -> MC code
             }
             rw.w[3] = (rw.w[3] & 0xffffull) | ((uint64_t)L << 16) | ((uint64_t)P.q << 32);
-            if (pos < a.stage_cap) a.stage_rec_w[pos] = rw;
-            else atomicOr(a.err, 2ull);
+            if (pos < a.stage_cap) {
+                a.stage_rec_w[pos] = rw;
+                if (a.stage_q) a.stage_q[pos] = P.q;
+            } else {
+                atomicOr(a.err, 2ull);
+            }
         } else {
             uint64_t lo64 = extract32_bases(s_words, (uint64_t)(rb0 + cs));
             uint64_t hi64 = 0;
@@ -325,8 +331,12 @@ __device__ __forceinline__ uint64_t write_piece(const ScanArgs &a, const PieceIn
             lo64 ^= (lo64 >> 1) & 0x5555555555555555ull;  // packed code -> MC code
             hi64 ^= (hi64 >> 1) & 0x5555555555555555ull;
             hi64 |= ((uint64_t)L << 24) | ((uint64_t)P.ds << 30) | ((uint64_t)P.q << 35);
-            if (pos < a.stage_cap) a.stage_rec[pos] = make_ulonglong2(lo64, hi64);
-            else atomicOr(a.err, 2ull);
+            if (pos < a.stage_cap) {
+                a.stage_rec[pos] = make_ulonglong2(lo64, hi64);
+                if (a.stage_q) a.stage_q[pos] = P.q;
+            } else {
+                atomicOr(a.err, 2ull);
+            }
         }
         pos++;
     }
@@ -853,6 +863,12 @@ struct PartArgs {
     unsigned long long *cbase;          // [nc + 1] bucket starts, then the total
     unsigned long long *pstart, *pend;  // [p_int] partition bounds
     unsigned long long *err;
+    // the partition word of each record, 4 bytes apart: of the staging slots
+    // (read by the coarse count) and of the coarse pass's output (written by
+    // the coarse pass, read by the fine count), so that counting reads 4
+    // bytes per record instead of the record's line
+    const uint32_t *stage_q;
+    uint32_t *coarse_q;
 };
 
 // The tile-aligned range of staging slots of coarse-pass block blockIdx.x.
@@ -902,10 +918,11 @@ __device__ __forceinline__ uint32_t pt_digit_scan(const uint32_t *s_cnt, uint32_
 // (q >> shift) & dmask, the next free slot of each digit in s_cur (advanced
 // here). Padding records are skipped. s_cnt is zero on entry and on return.
 template <class R>
-__device__ __forceinline__ void pt_move_tile(const R (&rec)[PartRec<R>::ITEMS], R *__restrict__ dst, uint64_t tb,
-                                             uint64_t te, int shift, uint32_t dmask, uint32_t nd, uint64_t cap,
-                                             R *s_stage, uint32_t *s_cnt, uint32_t *s_start, uint32_t *s_cur,
-                                             uint32_t *s_wsum, unsigned long long *err) {
+__device__ __forceinline__ void pt_move_tile(const R (&rec)[PartRec<R>::ITEMS], R *__restrict__ dst,
+                                             uint32_t *__restrict__ dst_q, uint64_t tb, uint64_t te, int shift,
+                                             uint32_t dmask, uint32_t nd, uint64_t cap, R *s_stage, uint32_t *s_cnt,
+                                             uint32_t *s_start, uint32_t *s_cur, uint32_t *s_wsum,
+                                             unsigned long long *err) {
     constexpr int IT = PartRec<R>::ITEMS;
     const int tid = threadIdx.x;
     int dig[IT];
@@ -930,10 +947,15 @@ __device__ __forceinline__ void pt_move_tile(const R (&rec)[PartRec<R>::ITEMS], 
     __syncthreads();
     for (uint32_t j = tid; j < total; j += PT_THREADS) {
         const R r = s_stage[j];
-        const uint32_t d = (rec_q(r) >> shift) & dmask;
+        const uint32_t q = rec_q(r);
+        const uint32_t d = (q >> shift) & dmask;
         const uint64_t pos = (uint64_t)s_cur[d] + (j - s_start[d]);
-        if (pos < cap) dst[pos] = r;
-        else atomicOr(err, 16ull);
+        if (pos < cap) {
+            dst[pos] = r;
+            if (dst_q) dst_q[pos] = q;
+        } else {
+            atomicOr(err, 16ull);
+        }
     }
     __syncthreads();
     for (uint32_t d = tid; d < nd; d += PT_THREADS) {
@@ -947,9 +969,10 @@ __device__ __forceinline__ void pt_move_tile(const R (&rec)[PartRec<R>::ITEMS], 
 // registers before the current tile is ranked and stored, so the loads are in
 // flight during the LDS phases (block barriers wait on LDS only).
 template <class R>
-__device__ __forceinline__ void pt_move_range(const R *__restrict__ src, R *__restrict__ dst, uint64_t b, uint64_t e,
-                                              int shift, uint32_t dmask, uint32_t nd, uint64_t cap, R *s_stage,
-                                              uint32_t *s_cnt, uint32_t *s_start, uint32_t *s_cur, uint32_t *s_wsum,
+__device__ __forceinline__ void pt_move_range(const R *__restrict__ src, R *__restrict__ dst,
+                                              uint32_t *__restrict__ dst_q, uint64_t b, uint64_t e, int shift,
+                                              uint32_t dmask, uint32_t nd, uint64_t cap, R *s_stage, uint32_t *s_cnt,
+                                              uint32_t *s_start, uint32_t *s_cur, uint32_t *s_wsum,
                                               unsigned long long *err) {
     constexpr int IT = PartRec<R>::ITEMS;
     constexpr uint64_t TN = (uint64_t)PT_THREADS * IT;
@@ -966,7 +989,8 @@ __device__ __forceinline__ void pt_move_range(const R *__restrict__ src, R *__re
             const uint64_t i = tn + (uint64_t)it * PT_THREADS + threadIdx.x;
             if (i < e) nxt[it] = src[i];
         }
-        pt_move_tile<R>(cur, dst, t, min(e, tn), shift, dmask, nd, cap, s_stage, s_cnt, s_start, s_cur, s_wsum, err);
+        pt_move_tile<R>(cur, dst, dst_q, t, min(e, tn), shift, dmask, nd, cap, s_stage, s_cnt, s_start, s_cur,
+                        s_wsum, err);
 #pragma unroll
         for (int it = 0; it < IT; it++) cur[it] = nxt[it];
     }
@@ -975,22 +999,23 @@ __device__ __forceinline__ void pt_move_range(const R *__restrict__ src, R *__re
 // Digit counts of slots [b, e) into s_h (LDS atomics), padding skipped; the
 // next tile's partition words are loaded before the current tile is counted.
 template <class R>
-__device__ __forceinline__ void pt_count_range(const R *__restrict__ src, uint64_t b, uint64_t e, int shift,
-                                               uint32_t dmask, uint32_t *s_h) {
+__device__ __forceinline__ void pt_count_range(const R *__restrict__ src, const uint32_t *__restrict__ qarr,
+                                               uint64_t b, uint64_t e, int shift, uint32_t dmask, uint32_t *s_h) {
     constexpr int IT = PartRec<R>::ITEMS;
     constexpr uint64_t TN = (uint64_t)PT_THREADS * IT;
     uint32_t cur[IT], nxt[IT];
+    auto q_of = [&](uint64_t i) { return qarr ? qarr[i] : PartRec<R>::q_at(src, i); };
 #pragma unroll
     for (int it = 0; it < IT; it++) {
         const uint64_t i = b + (uint64_t)it * PT_THREADS + threadIdx.x;
-        cur[it] = i < e ? PartRec<R>::q_at(src, i) : PartRec<R>::PAD;
+        cur[it] = i < e ? q_of(i) : PartRec<R>::PAD;
     }
     for (uint64_t t = b; t < e; t += TN) {
         const uint64_t tn = t + TN;
 #pragma unroll
         for (int it = 0; it < IT; it++) {
             const uint64_t i = tn + (uint64_t)it * PT_THREADS + threadIdx.x;
-            nxt[it] = i < e ? PartRec<R>::q_at(src, i) : PartRec<R>::PAD;
+            nxt[it] = i < e ? q_of(i) : PartRec<R>::PAD;
         }
 #pragma unroll
         for (int it = 0; it < IT; it++)
@@ -1017,7 +1042,7 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_count(const R *__restrict__
     const uint64_t gb = min(n, (uint64_t)g * per), ge = min(n, gb + per);
     const uint64_t sub = (per / TN + PT_COUNT_SPLIT - 1) / PT_COUNT_SPLIT * TN;
     const uint64_t b = min(ge, gb + part * sub), e = min(ge, b + sub);
-    pt_count_range<R>(src, b, e, a.fbits, 0xffffffffu, s_h);
+    pt_count_range<R>(src, a.stage_q, b, e, a.fbits, 0xffffffffu, s_h);
     __syncthreads();
     for (uint32_t d = threadIdx.x; d < a.nc; d += PT_THREADS)
         if (s_h[d]) atomicAdd(&a.hist[(uint64_t)d * a.nblk + g], s_h[d]);
@@ -1095,8 +1120,8 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_coarse(const R *__restrict_
     __syncthreads();
     uint64_t b, e;
     pt_range(a, TN, b, e);
-    pt_move_range<R>(src, dst, b, e, a.fbits, 0xffffffffu, a.nc, a.cap, s_stage, s_cnt, s_start, s_cur, s_wsum,
-                     a.err);
+    pt_move_range<R>(src, dst, a.coarse_q, b, e, a.fbits, 0xffffffffu, a.nc, a.cap, s_stage, s_cnt, s_start, s_cur,
+                     s_wsum, a.err);
 }
 
 template <class R>
@@ -1110,7 +1135,7 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_fine(const R *__restrict__ 
     const uint64_t bs = a.cbase[bkt], be = a.cbase[bkt + 1];
     for (uint32_t d = threadIdx.x; d < a.nf; d += PT_THREADS) s_cnt[d] = 0;
     __syncthreads();
-    pt_count_range<R>(src, bs, be, 0, fmask, s_cnt);
+    pt_count_range<R>(src, a.coarse_q, bs, be, 0, fmask, s_cnt);
     __syncthreads();
     pt_digit_scan(s_cnt, s_start, a.nf, s_wsum);
     for (uint32_t f = threadIdx.x; f < a.nf; f += PT_THREADS) {
@@ -1124,7 +1149,8 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_fine(const R *__restrict__ 
         s_cnt[f] = 0;
     }
     __syncthreads();
-    pt_move_range<R>(src, dst, bs, be, 0, fmask, a.nf, a.cap, s_stage, s_cnt, s_start, s_cur, s_wsum, a.err);
+    pt_move_range<R>(src, dst, nullptr, bs, be, 0, fmask, a.nf, a.cap, s_stage, s_cnt, s_start, s_cur, s_wsum,
+                     a.err);
 }
 
 // ---------------------------------------------------------------------------
@@ -1851,6 +1877,7 @@ struct McPlan {
     // partition's [start, end); hist / cbase: coarse digit counts per block
     // and bucket starts
     DevBuf base, cursor, records, ctr, ovf_keys, ovf_vals, ovf_part, stage_rec, hist, cbase;
+    DevBuf stage_q, coarse_q;  // partition words of the staging slots and of the coarse output
     DevBuf records_w, stage_rec_w, ovf_hi, out_hi;  // k > 32
     uint64_t stage_cap = 0, n_staged = 0, n_records = 0;
     uint32_t part_blocks = 0;  // blocks of the coarse pass
@@ -2139,6 +2166,7 @@ static ScanArgs scan_args(Batch &b, McPlan &p, const mczi_counter_config &cfg) {
     sa.tile_last = p.want_tiles ? p.tile_last.as<unsigned long long>() : nullptr;
     sa.stage_ctr = p.ctr.as<unsigned long long>() + 11;
     sa.err = p.ctr.as<unsigned long long>() + 4;
+    sa.stage_q = p.stage_q.as<uint32_t>();
     sa.grp_n = p.grp_n;
     sa.grp_i = p.grp_i;
     return sa;
@@ -2262,6 +2290,8 @@ static PartArgs part_args(McPlan &p) {
     pa.pstart = p.base.as<unsigned long long>();
     pa.pend = p.cursor.as<unsigned long long>();
     pa.err = p.ctr.as<unsigned long long>() + 4;
+    pa.stage_q = p.stage_q.as<uint32_t>();
+    pa.coarse_q = p.coarse_q.as<uint32_t>();
     return pa;
 }
 
@@ -2567,7 +2597,7 @@ static bool count_pass_one(Batch &b, McPlan &p, const mczi_counter_config &cfg, 
         }
         // device memory of this pass; more than the budget (or 2^32 staging
         // slots) splits the count into partition groups
-        const uint64_t rec_bytes = wide ? sizeof(Rec32) : 16;
+        const uint64_t rec_bytes = (wide ? sizeof(Rec32) : 16) + 4;  // + the partition word
         const uint64_t need = 2 * p.stage_cap * rec_bytes + out_cap * (wide ? 24 : 16) + ovf * (wide ? 24 : 16);
         if (attempt == 0 && (need > budget || p.stage_cap >= (1ull << 32)) && G < MAX_GROUPS) {
             const uint64_t f = std::max<uint64_t>((need + budget - 1) / std::max<uint64_t>(budget, 1),
@@ -2594,6 +2624,8 @@ static bool count_pass_one(Batch &b, McPlan &p, const mczi_counter_config &cfg, 
             p.stage_rec.ensure(p.stage_cap * 16);
             p.records.ensure(p.stage_cap * 16);
         }
+        p.stage_q.ensure(p.stage_cap * 4);
+        p.coarse_q.ensure(p.stage_cap * 4);
 
         tm.n = 0;
         tm.mark(s);
@@ -2605,7 +2637,8 @@ static bool count_pass_one(Batch &b, McPlan &p, const mczi_counter_config &cfg, 
             for (const DevBuf *db : {&b.words, &b.off, &b.dataset, &p.base, &p.cursor, &p.records, &p.ctr,
                                      &p.ovf_keys, &p.ovf_vals, &p.ovf_part, &p.out_kmer, &p.out_count,
                                      &p.out_part, &p.tile_start, &p.tile_frag, &p.tile_len, &p.stage_rec,
-                                     &p.hist, &p.cbase, &p.records_w, &p.stage_rec_w, &p.ovf_hi, &p.out_hi})
+                                     &p.hist, &p.cbase, &p.records_w, &p.stage_rec_w, &p.ovf_hi, &p.out_hi,
+                                     &p.stage_q, &p.coarse_q})
                 sig = mix_sig(sig, (uint64_t)(uintptr_t)db->as<void>());
             sig = mix_sig(mix_sig(mix_sig(sig, p.ovf_slots), p.stage_cap), p.out_cap);
             sig = mix_sig(mix_sig(sig, p.part_blocks), (uint64_t)launch);

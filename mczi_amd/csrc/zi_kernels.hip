@@ -1091,9 +1091,13 @@ void mczi_zi_server::done_loop() {
                 batch[i]->fasta = simplitig_fasta_text(v.data(), v.size(), idx.k);
             }
         }
-        if (trace)
-            fprintf(stderr, "ZI_TRACE t=%.2f done reqs=%zu wait_ms=%.2f sort_ms=%.2f copy_ms=%.2f post_ms=%.2f\n",
-                    now_ms(), batch.size(), wait_ms, sort_ms, t1 - t0, now_ms() - t1);
+        if (trace) {
+            uint64_t nb = 0;
+            for (size_t i = 0; i < batch.size() && !err; i++) nb += batch[i]->absent.n * (idx.wide ? 16 : 8);
+            fprintf(stderr,
+                    "ZI_TRACE t=%.2f done reqs=%zu wait_ms=%.2f sort_ms=%.2f copy_ms=%.2f post_ms=%.2f copy_bytes=%llu\n",
+                    now_ms(), batch.size(), wait_ms, sort_ms, t1 - t0, now_ms() - t1, (unsigned long long)nb);
+        }
         lk.lock();
         busy_ms += t1 - t0 + sort_ms;
         lookups += sl->work && !err ? sl->work->lookups : 0;
