@@ -1130,7 +1130,6 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_fine(const R *__restrict__ 
     __shared__ R s_stage[PT_THREADS * PartRec<R>::ITEMS];
     __shared__ uint32_t s_cnt[PT_MAX_DIGITS], s_start[PT_MAX_DIGITS], s_cur[PT_MAX_DIGITS];
     __shared__ uint32_t s_wsum[PT_THREADS / 64];
-    constexpr uint64_t TN = (uint64_t)PT_THREADS * PartRec<R>::ITEMS;
     const uint32_t bkt = blockIdx.x, fmask = a.nf - 1;
     const uint64_t bs = a.cbase[bkt], be = a.cbase[bkt + 1];
     for (uint32_t d = threadIdx.x; d < a.nf; d += PT_THREADS) s_cnt[d] = 0;
@@ -1326,52 +1325,108 @@ __device__ __forceinline__ void out_finish(const CountArgs &a, unsigned long lon
     out_pad<WIDE>(a, s_pad[0], s_pad[1]);
 }
 
-// Canonical MC encoding (src/lib.rs:3589-3614) of the k <= 32 MC-coded bases
-// in x (base j at bits 2j, x already masked to 2k bits): forward = 2-bit
-// groups reversed (first base most significant), reverse complement =
-// complemented groups in stored order.
-__device__ __forceinline__ uint64_t canon_from_mc(uint64_t x, uint64_t mask, int rshift) {
-    uint64_t r = __builtin_bitreverse64(x);
-    r = ((r >> 1) & 0x5555555555555555ull) | ((r & 0x5555555555555555ull) << 1);
-    const uint64_t fwd = r >> rshift;
-    const uint64_t rc = (~x) & mask;
-    return fwd < rc ? fwd : rc;
-}
-
-// LDS table slot of a k-mer: multiplicative mix of both 32-bit halves (the
-// spill table uses spread_hash_u64, an independent function).
+// LDS table slot of a k-mer: one multiplicative mix of the low half and the
+// rotated high half (the spill table uses spread_hash_u64, an independent
+// function).
 template <int LS = CNT_LOG2_SLOTS>
 __device__ __forceinline__ uint32_t lds_slot(uint64_t key) {
-    uint32_t h = ((uint32_t)key * 0x9E3779B1u) ^ ((uint32_t)(key >> 32) * 0x85EBCA77u);
-    h ^= h >> 16;
-    h *= 0x2C1B3C6Du;
+    const uint32_t h = ((uint32_t)key ^ __builtin_rotateleft32((uint32_t)(key >> 32), 16)) * 0x9E3779B1u;
     return h >> (32 - LS);
 }
 
+// Adds val to key's entry of the LDS table (claiming an empty slot), or to the
+// HBM spill table when 32 probes find neither. The home slot is read first:
+// lanes that read one slot share the read, and a present key needs no CAS.
+// WRAP: the add returns the old count to catch a wrap past 2^32 - 1 (only a
+// partition with that many k-mer instances can wrap one).
+template <int MODE, int LS, bool WRAP>
+__device__ __forceinline__ void lds_insert(const CountArgs &a, unsigned long long *s_keys, uint32_t *s_vals,
+                                           unsigned long long key, uint32_t val, uint32_t q, int *s_wrap) {
+    constexpr int SLOTS = 1 << LS;
+    uint32_t slot = lds_slot<LS>(key);
+    auto add = [&](uint32_t sl) {
+        if constexpr (MODE == 0 && WRAP) {
+            if (atomicAdd(&s_vals[sl], val) > 0xFFFFFFFFu - val) {
+                carry_add(a, key, 0);
+                *s_wrap = 1;
+            }
+        } else if constexpr (MODE == 0) {
+            __hip_atomic_fetch_add(&s_vals[sl], val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+            __hip_atomic_fetch_or(&s_vals[sl], val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    };
+    {
+        const unsigned long long cur = __hip_atomic_load(&s_keys[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        unsigned long long prev = cur;
+        if (cur == EMPTY_KEY) prev = atomicCAS(&s_keys[slot], EMPTY_KEY, key);
+        if (prev == key || prev == EMPTY_KEY) {
+            add(slot);
+            return;
+        }
+    }
+    slot = (slot + 1) & (SLOTS - 1);
+    for (int p = 1; p < CNT_MAXPROBE; p++) {
+        const unsigned long long prev = atomicCAS(&s_keys[slot], EMPTY_KEY, key);
+        if (prev == EMPTY_KEY || prev == key) {
+            add(slot);
+            return;
+        }
+        slot = (slot + 1) & (SLOTS - 1);
+    }
+    overflow_insert<MODE>(a, key, val, q);
+}
+
+// Records of a partition are counted in rounds of up to CNT_ROUND records.
+// At high coverage most records of a partition are copies of a few
+// super-k-mers, so a round first merges equal records (same bases and length)
+// in an LDS table: the first copy represents them and carries their number
+// (MODE 0) or the OR of their dataset bits (MODE 1). Only representatives are
+// expanded into k-mers, sorted by k-mer count (longest first) so that the 64
+// records a wave takes have about the same length and the lanes roll them in
+// step.
+constexpr int CNT_ROUND = 512;
+constexpr int CNT_NBUCKET = 64;               // k-mers per record <= REC_MAX_BASES
+constexpr int CNT_LOG2_DSLOTS = 10;           // record-merge table: 2 slots per record
+constexpr int CNT_DSLOTS = 1 << CNT_LOG2_DSLOTS;
+static_assert(REC_MAX_BASES < CNT_NBUCKET, "one bucket per k-mer count");
+static_assert(CNT_DSLOTS >= 2 * CNT_ROUND, "the merge table is at most half full");
+constexpr unsigned long long REC_KEY_Y = 0x3FFFFFFFull;  // bases 32..43 and the base count
+
+__device__ __forceinline__ uint32_t rec_merge_slot(uint64_t x, uint64_t ykey) {
+    const uint32_t h = ((uint32_t)x * 0x9E3779B1u) ^ ((uint32_t)(x >> 32) * 0x85EBCA77u) ^
+                       ((uint32_t)ykey * 0xC2B2AE3Du);
+    return h >> (32 - CNT_LOG2_DSLOTS);
+}
+
 // Per partition: all k-mers of its super-k-mer records go into an LDS
-// open-addressing table (CNT_SLOTS slots, linear probing, insert-only); a
-// k-mer whose probe window is full goes to the HBM spill table instead, so
-// each k-mer lives in exactly one of the two. Then count > x (MODE 0: counts,
+// open-addressing table (2^LS slots, linear probing, insert-only); a k-mer
+// whose probe window is full goes to the HBM spill table instead, so each
+// k-mer lives in exactly one of the two. Then count > x (MODE 0: counts,
 // src/lib.rs:2417 / 2502) or popcount(mask) > x (MODE 1: dataset presence,
 // src/lib.rs:2603) is emitted with one output reservation per partition.
 // The reference's pass-1 minimizer filter (two-pass 3462-3523, fof
-// 4291-4341) is not run: every occurrence of a k-mer K has the same
+// 4291-4341) is not needed here: every occurrence of a k-mer K has the same
 // minimizer (the scheme is canonical), so that minimizer's window sum
 // (dataset set) is >= c(K) (covers K's datasets), and the filter can only
 // drop k-mers that fail "> x" anyway; the emitted set and counts are equal.
-// Work is balanced per wave: 64 records are loaded (and kept in LDS), their
-// k-mers numbered by a wave prefix sum, and lane j takes k-mer j (its owner
-// record found by a max-scan of record-start markers), extracting it directly
-// from the 128-bit record
-// (the same canonical k-mers add_packed_superkmer_kmer_counts rolls,
-// src/lib.rs:3589-3614).
+// Each lane rolls the canonical k-mers of one record as
+// add_packed_superkmer_kmer_counts does (src/lib.rs:3589-3614): forward
+// = (forward << 2 | b) & mask, reverse complement = rc >> 2 | (3 - b) << 2(k-1),
+// key = min of the two; only the first k-mer of a record is built whole.
+// The next round's records (of this partition or the block's next one) are
+// loaded while the current round is counted.
 template <int MODE, int LS = CNT_LOG2_SLOTS>
 __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
     constexpr int SLOTS = 1 << LS;  // LDS table slots (the tuner picks 2^10 .. 2^12)
+    constexpr int PER_T = CNT_ROUND / CNT_THREADS;
     __shared__ unsigned long long s_keys[SLOTS];
     __shared__ uint32_t s_vals[SLOTS];
-    __shared__ ulonglong2 s_rec[CNT_THREADS / 64][64];  // each wave's current 64 records
-    __shared__ uint8_t s_mark[CNT_THREADS / 64][64];    // record starts in the current 64 k-mers
+    __shared__ ulonglong2 s_raw[CNT_ROUND];     // the round's records
+    __shared__ uint32_t s_dtab[CNT_DSLOTS];     // merge table: record index + 1, 0 = empty
+    __shared__ uint32_t s_agg[CNT_ROUND];       // per representative: copies, or OR of dataset bits
+    __shared__ uint16_t s_order[CNT_ROUND];     // representatives, most k-mers first
+    __shared__ uint32_t s_bucket[CNT_NBUCKET];  // representatives per k-mer count (descending)
     __shared__ int s_wsum[CNT_THREADS / 64];
     __shared__ unsigned long long s_red[CNT_THREADS / 64];
     __shared__ unsigned long long s_obase, s_pad[2];
@@ -1381,91 +1436,158 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
     const int k = a.k;
     unsigned long long o_cur = 0, o_end = 0, my_pad = 0;  // thread 0: the block's output chunk
     const uint64_t kmask = k >= 32 ? ~0ull : ((1ull << (2 * k)) - 1ull);
-    if (tid == 0) s_wrap = 0;
     const int rshift = 64 - 2 * k;
+    const int rcsh = 2 * k - 2;
+    if (tid == 0) s_wrap = 0;
     unsigned long long my_distinct = 0;
-    // the table starts empty; emit leaves it empty again for the next partition
+    // the tables start empty; each round and each emit leave them empty again
     for (int i = tid; i < SLOTS; i += CNT_THREADS) {
         s_keys[i] = EMPTY_KEY;
         s_vals[i] = 0;
     }
+    for (int i = tid; i < CNT_DSLOTS; i += CNT_THREADS) s_dtab[i] = 0;
+    for (int i = tid; i < CNT_ROUND; i += CNT_THREADS) s_agg[i] = 0;
+    if (tid < CNT_NBUCKET) s_bucket[tid] = 0;
     __syncthreads();
-    for (uint32_t q = blockIdx.x; q < a.p_int; q += gridDim.x) {
-        const uint64_t rb = a.base[q], re = a.cursor[q];
-        for (uint64_t wb = rb + (uint64_t)wid * 64; wb < re; wb += CNT_THREADS) {
-            const uint64_t ri = wb + lane;
-            ulonglong2 r = make_ulonglong2(0, 0);
-            int n = 0;
-            if (ri < re) {
-                r = a.records[ri];
-                n = rec_len(r) - k + 1;
-                if (n < 0) n = 0;
+
+    const uint32_t G = gridDim.x;
+    uint32_t q = blockIdx.x;
+    uint64_t rb = 0, re = 0, rbn = 0, ren = 0;  // this partition's and the next one's records
+    if (q < a.p_int) {
+        rb = a.base[q];
+        re = a.cursor[q];
+    }
+    if (q + G < a.p_int) {
+        rbn = a.base[q + G];
+        ren = a.cursor[q + G];
+    }
+    ulonglong2 rr[PER_T];
+    uint64_t lb = 0, le = 0;  // the records held in rr: [lb, le)
+    auto load_round = [&](uint64_t b, uint64_t e) {
+        lb = b;
+        le = e;
+#pragma unroll
+        for (int i = 0; i < PER_T; i++) {
+            const uint64_t ri = b + (uint64_t)(i * CNT_THREADS + tid);
+            if (ri < e) rr[i] = a.records[ri];
+        }
+    };
+    load_round(rb, min(rb + CNT_ROUND, re));
+    for (; q < a.p_int; q += G) {
+        // at most REC_MAX_BASES k-mer instances per record: fewer than 2^32
+        // in the partition and no count can wrap
+        const bool may_wrap = MODE == 0 && (re - rb) * (uint64_t)REC_MAX_BASES >= (1ull << 32);
+        for (uint64_t cb = rb; cb < re; cb += CNT_ROUND) {
+            const uint64_t ce = min(cb + CNT_ROUND, re);
+            if (lb != cb || le != ce) load_round(cb, ce);  // not prefetched (after an empty partition)
+            const int nr = (int)(ce - cb);
+#pragma unroll
+            for (int i = 0; i < PER_T; i++) {
+                const int li = i * CNT_THREADS + tid;
+                if (li < nr) s_raw[li] = rr[i];
             }
-            s_rec[wid][lane] = r;
-            const int incl = wave_incl_add(n);
-            const int total = __builtin_amdgcn_readlane(incl, 63);
-            const int my_excl = incl - n;
-            int carry = 0;
-            for (int j0 = 0; j0 < total; j0 += 64) {
-                const int j = j0 + lane;
-                // owner of k-mer j: the last record starting at or before j.
-                // Each record starting in this chunk leaves lane + 1 at its
-                // start; an inclusive max-scan spreads it forward.
-                s_mark[wid][lane] = 0;
-                if (n > 0 && my_excl >= j0 && my_excl < j0 + 64) s_mark[wid][my_excl - j0] = (uint8_t)(lane + 1);
-                __builtin_amdgcn_wave_barrier();
-                int mk = wave_incl_max((int)s_mark[wid][lane]);
-                mk = max(mk, carry);
-                carry = __builtin_amdgcn_readlane(mk, 63);
-                __builtin_amdgcn_wave_barrier();
-                const int o = (mk - 1) & 63;
-                const int excl = __shfl(my_excl, o, 64);
-                if (j >= total) continue;
-                const int off = j - excl;  // k-mer start inside the record
-                const ulonglong2 ro = s_rec[wid][o];
-                const unsigned long long rx = ro.x, ry = ro.y;
-                const uint64_t rd = ry & 0xFFFFFFull;  // bases 32..43
-                const int sh = 2 * off;
-                const uint64_t lo = sh == 0 ? rx : (sh < 64 ? ((rx >> sh) | (rd << (64 - sh))) : (rd >> (sh - 64)));
-                const unsigned long long key = canon_from_mc(lo & kmask, kmask, rshift);
-                const uint32_t val = (MODE == 0) ? 1u : (1u << (uint32_t)((ry >> 30) & 31u));
-                uint32_t slot = lds_slot<LS>(key);
-                // Home slot: a plain load settles the common case (key present,
-                // or empty slot claimed by one CAS); later probes use the CAS
-                // alone, which returns whatever key holds the slot.
-                bool done = false;
-                {
-                    const unsigned long long cur =
-                        __hip_atomic_load(&s_keys[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    unsigned long long prev = cur;
-                    if (cur == EMPTY_KEY) prev = atomicCAS(&s_keys[slot], EMPTY_KEY, key);
-                    if (prev == key || prev == EMPTY_KEY) {
-                        if (val_update<MODE>(&s_vals[slot], val)) {
-                            carry_add(a, key, 0);
-                            s_wrap = 1;
-                        }
-                        done = true;
-                    }
-                }
-                if (!done) {
-                    slot = (slot + 1) & (SLOTS - 1);
-                    for (int p = 1; p < CNT_MAXPROBE; p++) {
-                        const unsigned long long prev = atomicCAS(&s_keys[slot], EMPTY_KEY, key);
-                        if (prev == EMPTY_KEY || prev == key) {
-                            if (val_update<MODE>(&s_vals[slot], val)) {
-                                carry_add(a, key, 0);
-                                s_wrap = 1;
-                            }
-                            done = true;
+            __syncthreads();
+            // merge equal records; representatives are bucketed by k-mer count
+            int bk[PER_T];
+            uint32_t rk[PER_T];
+#pragma unroll
+            for (int i = 0; i < PER_T; i++) {
+                const int li = i * CNT_THREADS + tid;
+                bk[i] = -1;
+                rk[i] = 0;
+                if (li < nr) {
+                    const uint64_t x = rr[i].x, ykey = rr[i].y & REC_KEY_Y;
+                    uint32_t s = rec_merge_slot(x, ykey);
+                    int rep = li;
+                    for (int p = 0; p < CNT_DSLOTS; p++) {
+                        const uint32_t prev = atomicCAS(&s_dtab[s], 0u, (uint32_t)li + 1u);
+                        if (prev == 0) break;
+                        const ulonglong2 o = s_raw[prev - 1];
+                        if (o.x == x && (o.y & REC_KEY_Y) == ykey) {
+                            rep = (int)prev - 1;
                             break;
                         }
-                        slot = (slot + 1) & (SLOTS - 1);
+                        s = (s + 1) & (CNT_DSLOTS - 1);
+                    }
+                    if constexpr (MODE == 0) atomicAdd(&s_agg[rep], 1u);
+                    else atomicOr(&s_agg[rep], 1u << (uint32_t)((rr[i].y >> 30) & 31u));
+                    if (rep == li) {
+                        const int n = max(rec_len(rr[i]) - k + 1, 0);
+                        bk[i] = CNT_NBUCKET - 1 - n;
+                        rk[i] = atomicAdd(&s_bucket[bk[i]], 1u);
                     }
                 }
-                if (!done) overflow_insert<MODE>(a, key, val, q);
             }
+            // the next round's records load while this one is counted
+            {
+                uint64_t nb = rbn, ne = min(rbn + CNT_ROUND, ren);
+                if (ce < re) {
+                    nb = ce;
+                    ne = min(ce + CNT_ROUND, re);
+                }
+                load_round(nb, ne);
+            }
+            __syncthreads();
+            // bucket starts (every wave scans the 64 counts), then the
+            // representatives' order
+            int nrep;
+            {
+                const uint32_t c = s_bucket[lane];
+                const int incl = wave_incl_add((int)c);
+                nrep = __builtin_amdgcn_readlane(incl, 63);
+                const int st = incl - (int)c;
+#pragma unroll
+                for (int i = 0; i < PER_T; i++) {
+                    const int at = __shfl(st, bk[i] < 0 ? 0 : bk[i], 64);
+                    if (bk[i] >= 0) s_order[at + (int)rk[i]] = (uint16_t)(i * CNT_THREADS + tid);
+                }
+            }
+            for (int i = tid; i < CNT_DSLOTS; i += CNT_THREADS) s_dtab[i] = 0;
+            __syncthreads();
+            if (tid < CNT_NBUCKET) s_bucket[tid] = 0;  // next round (ordered by its barriers)
+            // group g of 256 representatives: lane l of wave w takes the
+            // (4 l + w)-th, so that records with equal k-mers at equal steps
+            // (copies of one locus with an error or cut, adjacent in the
+            // order) mostly fall into different waves
+            for (int g = 0; g < nrep; g += CNT_THREADS) {
+                const int li = g + 4 * lane + wid;
+                ulonglong2 r = make_ulonglong2(0ull, 0ull);
+                int n = 0;
+                uint32_t val = 0;
+                if (li < nrep) {
+                    const int idx = s_order[li];
+                    r = s_raw[idx];
+                    val = s_agg[idx];
+                    s_agg[idx] = 0;
+                    n = max(rec_len(r) - k + 1, 0);
+                }
+                // lane 0 holds the wave's longest record
+                const int nmax = __builtin_amdgcn_readfirstlane(n);
+                const uint64_t x = r.x, y = r.y & 0xFFFFFFull;  // bases 0..31, 32..43
+                const uint64_t w0 = x & kmask;
+                uint64_t fwd = __builtin_bitreverse64(w0);
+                fwd = (((fwd >> 1) & 0x5555555555555555ull) | ((fwd & 0x5555555555555555ull) << 1)) >> rshift;
+                uint64_t rc = (~w0) & kmask;
+                // bases k, k + 1, ... still to come (those past 43 read as 0)
+                uint64_t la = k >= 32 ? y : ((x >> (2 * k)) | (y << (64 - 2 * k)));
+                for (int j = 0; j < nmax; j++) {
+                    if (j > 0) {
+                        // k <= 11: bases k + 32 on are past the first 32 of la
+                        if (j == 33) la = y >> (2 * k);
+                        const uint32_t b = (uint32_t)la & 3u;
+                        la >>= 2;
+                        fwd = ((fwd << 2) | b) & kmask;
+                        rc = (rc >> 2) | ((uint64_t)(b ^ 3u) << rcsh);
+                    }
+                    if (j < n) {
+                        const unsigned long long key = fwd < rc ? fwd : rc;
+                        if (may_wrap) lds_insert<MODE, LS, true>(a, s_keys, s_vals, key, val, q, &s_wrap);
+                        else lds_insert<MODE, LS, false>(a, s_keys, s_vals, key, val, q, &s_wrap);
+                    }
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
         // emit: count survivors, one reservation per partition, then write
         constexpr int PER = SLOTS / CNT_THREADS;
         uint32_t emask = 0;
@@ -1512,6 +1634,14 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
         }
         if (tid == 0) s_wrap = 0;
         __syncthreads();
+        // the next partition; the bounds of the one after it load now
+        rb = rbn;
+        re = ren;
+        rbn = ren = 0;
+        if ((uint64_t)q + 2ull * G < a.p_int) {
+            rbn = a.base[q + 2 * G];
+            ren = a.cursor[q + 2 * G];
+        }
     }
     out_finish<false>(a, o_cur, o_end, my_pad, s_pad);
     block_sum_atomic(my_distinct, &a.ctr[2], s_red);
