@@ -364,8 +364,16 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
     __shared__ int32_t s_fs[TILE + 2];
     __shared__ int s_wsum[SCAN_THREADS / 64];
     __shared__ unsigned long long s_sbase, s_hole_b, s_hole_e;
+    // ntHash terms by (tid % 4, position in the thread, base code):
+    // (rotr(F[c], a), rotl(F[c ^ 2], a)) with a = 7 (8 tid + t) mod 32
+    __shared__ uint2 s_seed[4 * WPT * 4];
 
     const int tid = threadIdx.x;
+    if (tid < 4 * WPT * 4) {
+        const uint32_t c = (uint32_t)tid & 3u, tt = ((uint32_t)tid >> 2) % WPT, row = (uint32_t)tid / (WPT * 4);
+        const uint32_t amt = (7u * (8u * row + tt)) & 31u;
+        s_seed[tid] = make_uint2(__builtin_rotateright32(nt_seed(c), amt), __builtin_rotateleft32(nt_seed(c ^ 2u), amt));
+    }
     const int k = a.k, m = a.m, w = WC ? WC : a.w;
     unsigned long long my_pieces = 0, my_records = 0, my_bytes = 0;
     unsigned long long chunk_cur = 0, chunk_end = 0;  // thread 0: the block's current staging chunk
@@ -444,20 +452,17 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
         // warm-up.
         {
             const uint32_t bw = extract16_bases(s_words, rb0 + r0);
-            uint32_t amt = (uint32_t)(7 * r0) & 31u;
+            // the terms of position r0 + t come from the thread's row of the
+            // seed table (the rotation 7 (r0 + t) depends on tid % 4 and t)
+            const uint2 *seed_row = s_seed + (tid & 3) * (WPT * 4);
             uint32_t ta[WPT], tb[WPT], xa = 0, xb = 0;
 #pragma unroll
             for (int t = 0; t < WPT; t++) {
                 ta[t] = xa;
                 tb[t] = xb;
-                const uint32_t c = (bw >> (2 * t)) & 3u;
-                const uint32_t s_lo = (c & 1u) ? NT_SEED1 : NT_SEED0;
-                const uint32_t s_hi = (c & 1u) ? NT_SEED3 : NT_SEED2;
-                const uint32_t f = (c & 2u) ? s_hi : s_lo;  // F[c]
-                const uint32_t g = (c & 2u) ? s_lo : s_hi;  // F[c ^ 2]
-                xa ^= __builtin_rotateright32(f, amt);
-                xb ^= __builtin_rotateleft32(g, amt);
-                amt = (amt + 7u) & 31u;
+                const uint2 fg = seed_row[t * 4 + ((bw >> (2 * t)) & 3u)];
+                xa ^= fg.x;
+                xb ^= fg.y;
             }
             // exclusive block XOR scan of (xa, xb): DPP within the wave, then
             // the totals of the waves before this one
