@@ -733,6 +733,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
 __global__ void k_tile_join(const unsigned long long *first, const unsigned long long *last, uint64_t n_tiles,
                             uint32_t k, unsigned long long *corr) {
     constexpr unsigned long long NONE = ~0ull, SINGLE = 0x80000000ull;
+    unsigned long long my_corr = 0;
     auto joins = [&](uint64_t t) {  // the run of tile t - 1's last piece goes on in tile t
         const unsigned long long L = last[t - 1], F = first[t];
         return L != NONE && F != NONE && (L >> 32) == (F >> 32);
@@ -756,9 +757,10 @@ __global__ void k_tile_join(const unsigned long long *first, const unsigned long
             break;
         }
         const uint64_t L = total + k - 1;
-        const long long d = (long long)((L < 255 ? 1 : 5) + (L + 3) / 4) - (long long)parts;
-        if (d) atomicAdd(corr, (unsigned long long)d);
+        my_corr += (unsigned long long)((long long)((L < 255 ? 1 : 5) + (L + 3) / 4) - (long long)parts);
     }
+    __shared__ unsigned long long s_red[4];
+    block_sum_atomic(my_corr, corr, s_red);  // one atomic per block (two's complement sum)
 }
 
 // ---------------------------------------------------------------------------
@@ -2703,7 +2705,11 @@ static bool count_pass_one(Batch &b, McPlan &p, const mczi_counter_config &cfg, 
         p.part_blocks = (uint32_t)d.cu_count() * part_res;
         p.hist.ensure((size_t)g.nc * p.part_blocks * 4);
         p.cbase.ensure(((size_t)g.nc + 1) * 8);
-        const uint64_t ovf = next_pow2(std::max<uint64_t>(win / 16, 1 << 20)) << attempt;
+        // a normal count spills well under one distinct k-mer per 10^4
+        // windows; the table takes new keys up to an eighth of its slots, and
+        // a count that needs more is redone with smaller partitions and a
+        // table twice the size (fill and emit scan the whole table each step)
+        const uint64_t ovf = next_pow2(std::max<uint64_t>(win / 256, 1 << 20)) << attempt;
         // every emitted k-mer has more than x windows (out_scale > 1 covers a
         // partition group holding more than its share); plus one padded chunk
         // per count block
