@@ -259,6 +259,7 @@ struct PieceInfo {
 // groups: the current group), its partition and its record / byte counts.
 __device__ __forceinline__ PieceInfo piece_info(const ScanArgs &a, int pi, int np, int tl, int k, uint64_t f0,
                                                 uint64_t t, const uint32_t *pst, const int32_t *s_fs,
+                                                const uint8_t *s_ds,
                                                 const uint32_t *s_v, const uint8_t *s_flag, int &my_nrec,
                                                 unsigned long long &my_bytes) {
     PieceInfo P{0, 0, 0u, 0u, false};
@@ -270,7 +271,7 @@ __device__ __forceinline__ PieceInfo piece_info(const ScanArgs &a, int pi, int n
     if (vend < e) e = vend;
     P.e = e;
     if (a.dataset != nullptr) {
-        P.ds = a.dataset[f0 + jf] - a.ds_lo;
+        P.ds = jf < SCAN_THREADS ? (uint32_t)s_ds[jf] : a.dataset[f0 + jf] - a.ds_lo;
         if (P.ds >= 32u) return P;
     }
     const uint32_t key = spread_hash_u32(s_v[P8(P.s)]);
@@ -346,13 +347,13 @@ __device__ __forceinline__ uint64_t write_piece(const ScanArgs &a, const PieceIn
 template <int MODE, bool GW, bool WIDE = false, int WC = 0>
 __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_scan(ScanArgs a) {
     __shared__ uint32_t s_words[WORDS_SPAN];
-    __shared__ uint32_t s_hash[PADDED(TILE + MAXW)];
+    __shared__ uint32_t s_hash[PADDED(SPAN)];
     // prefix XORs live only until the hashes are written; the selected
     // hashes and piece starts reuse their space
     __shared__ union {
         struct {
-            uint32_t sa[PADDED(SPAN)];
-            uint32_t sb[PADDED(SPAN)];
+            uint32_t sa[PADDED(SPAN + MAXW)];
+            uint32_t sb[PADDED(SPAN + MAXW)];
         } pre;
         struct {
             uint32_t v[PADDED(TILE)];
@@ -362,6 +363,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
     __shared__ uint32_t s_wxor[2][SCAN_THREADS / 64];  // per-wave XOR totals of the two prefix streams
     __shared__ uint8_t s_flag[PADDED(TILE)];
     __shared__ int32_t s_fs[TILE + 2];
+    __shared__ uint8_t s_ds[SCAN_THREADS];  // fof: dataset - ds_lo of the tile's first fragments (255: other group)
     __shared__ int s_wsum[SCAN_THREADS / 64];
     __shared__ unsigned long long s_sbase, s_hole_b, s_hole_e;
     // ntHash terms by (tid % 4, position in the thread, base code):
@@ -403,11 +405,17 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
     }
     uint32_t pword = 0;
     uint64_t poff = 0;
+    uint32_t pds = 255u;  // the dataset (within the staged group) of fragment f0 + tid of the next tile
     auto load_first = [&](uint64_t ts_, int tl_, uint64_t f0_, uint64_t f1_) {
         const int nw = (((int)(ts_ & 15u) + tl_ + 160) >> 4) + 3;
         pword = tid < nw ? a.words[(ts_ >> 4) + tid] : 0u;
         const uint64_t j = f0_ + (uint64_t)tid;
         poff = (j <= f1_ + 1 && j <= a.n_frag) ? a.off[j] : (~0ull >> 1);
+        pds = 255u;
+        if (MODE == STAGE && a.dataset != nullptr && j <= f1_ && j < a.n_frag) {
+            const uint32_t d = a.dataset[j] - a.ds_lo;
+            pds = d < 32u ? d : 255u;
+        }
     };
     if (t < a.tile_end) load_first(ts, tl, f0, f1);
     uint64_t ts_n = 0, f0_n = 0, f1_n = 0;  // the next tile's entry
@@ -423,6 +431,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
         const int rb0 = (int)(ts & 15u);
         __syncthreads();  // previous tile fully consumed
         if (tid < WORDS_SPAN) s_words[tid] = pword;
+        if (MODE == STAGE) s_ds[tid] = (uint8_t)pds;
         // Fragment boundaries relative to ts: s_fs[j] = off[f0 + j] - ts (clamped),
         // for j = 0 .. nf, nf = number of fragments with a base in [ts, ts + tl).
         int nf = 0;
@@ -486,16 +495,15 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
                 s_u.pre.sb[px(t)] = pb ^ tb[t];
             }
             __syncthreads();
-            const int nh = tl + w - 1;
             uint32_t ai = (uint32_t)(7 * r0) & 31u;
             const uint32_t am = (uint32_t)(7 * (m - 1)) & 31u;
 #pragma unroll
             for (int t = 0; t < WPT; t++) {
-                if (r0 + t < nh) {
-                    const uint32_t A = s_u.pre.sa[px(t + m)] ^ pa ^ ta[t];
-                    const uint32_t Bv = s_u.pre.sb[px(t + m)] ^ pb ^ tb[t];
-                    s_hash[px(t)] = __builtin_rotateleft32(A, (ai + am) & 31u) + __builtin_rotateright32(Bv, ai);
-                }
+                // positions past nh - 1 get values that no window reads (the
+                // arrays are padded for them)
+                const uint32_t A = s_u.pre.sa[px(t + m)] ^ pa ^ ta[t];
+                const uint32_t Bv = s_u.pre.sb[px(t + m)] ^ pb ^ tb[t];
+                s_hash[px(t)] = __builtin_rotateleft32(A, (ai + am) & 31u) + __builtin_rotateright32(Bv, ai);
                 ai = (ai + 7u) & 31u;
             }
         }
@@ -666,10 +674,10 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             int my_nrec = 0;
             PieceInfo P0{0, 0, 0u, 0u, false}, P1{0, 0, 0u, 0u, false};
             if (rb + tid < np)
-                P0 = piece_info(a, rb + tid, np, tl, k, f0, t, s_u.run.pst, s_fs, s_u.run.v, s_flag, my_nrec,
+                P0 = piece_info(a, rb + tid, np, tl, k, f0, t, s_u.run.pst, s_fs, s_ds, s_u.run.v, s_flag, my_nrec,
                                 my_bytes);
             if (rb + tid + SCAN_THREADS < np)
-                P1 = piece_info(a, rb + tid + SCAN_THREADS, np, tl, k, f0, t, s_u.run.pst, s_fs, s_u.run.v, s_flag,
+                P1 = piece_info(a, rb + tid + SCAN_THREADS, np, tl, k, f0, t, s_u.run.pst, s_fs, s_ds, s_u.run.v, s_flag,
                                 my_nrec, my_bytes);
             int tot = 0;
             const int off = block_exclusive_scan(my_nrec, &tot, s_wsum);
