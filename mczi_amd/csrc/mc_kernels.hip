@@ -571,7 +571,9 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
                 }
                 if (r >= tl) break;
                 if (i > 0) tg = tg + ((E >> (2 * i - 1)) & 1u) - (((uint32_t)H0 >> (2 * i - 1)) & 1u);
-                while (j + 1 < nf && fnext <= r) {
+                // fragments are never empty (batches drop empty ones), so one
+                // window step crosses at most one fragment start
+                if (j + 1 < nf && fnext <= r) {
                     j++;
                     fcur = fnext;
                     fnext = s_fs[j + 1];
@@ -580,12 +582,16 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
                 const bool fstart = fcur == r;
                 const bool valid = r + k <= fnext;
                 uint32_t v = 0;
-                if (valid) {
+                if constexpr (GW) {
+                    // read for every window (the positions are in the tile's
+                    // hashes either way); kept only for valid ones
+                    const int lp = (int)(min(sufL[i], preL) & 0xffffu);
+                    const int rp = (int)(0xffffu - (min(sufR[i], preR) & 0xffffu));
+                    v = s_hash[px((2u * tg > (uint32_t)k) ? lp : rp)];
+                    v = valid ? v : 0u;
+                } else if (valid) {
                     int lp, rp;
-                    if constexpr (GW) {
-                        lp = (int)(min(sufL[i], preL) & 0xffffu);
-                        rp = (int)(0xffffu - (min(sufR[i], preR) & 0xffffu));
-                    } else {
+                    {
                         uint32_t best = s_hash[px(i)] >> 16;
                         lp = rp = i;
                         for (int ii = 1; ii < w; ii++) {
