@@ -1406,9 +1406,10 @@ __device__ __forceinline__ void lds_insert(const CountArgs &a, unsigned long lon
 // expanded into k-mers, sorted by k-mer count (longest first) so that the 64
 // records a wave takes have about the same length and the lanes roll them in
 // step.
-constexpr int CNT_ROUND = 512;
+constexpr int CNT_THREADS_K = 512;            // threads of a k <= 32 count block
+constexpr int CNT_ROUND = 2 * CNT_THREADS_K;
 constexpr int CNT_NBUCKET = 64;               // k-mers per record <= REC_MAX_BASES
-constexpr int CNT_LOG2_DSLOTS = 10;           // record-merge table: 2 slots per record
+constexpr int CNT_LOG2_DSLOTS = 11;           // record-merge table: 2 slots per record
 constexpr int CNT_DSLOTS = 1 << CNT_LOG2_DSLOTS;
 static_assert(REC_MAX_BASES < CNT_NBUCKET, "one bucket per k-mer count");
 static_assert(CNT_DSLOTS >= 2 * CNT_ROUND, "the merge table is at most half full");
@@ -1438,9 +1439,9 @@ __device__ __forceinline__ uint32_t rec_merge_slot(uint64_t x, uint64_t ykey) {
 // The next round's records (of this partition or the block's next one) are
 // loaded while the current round is counted.
 template <int MODE, int LS = CNT_LOG2_SLOTS>
-__global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
+__global__ __launch_bounds__(CNT_THREADS_K) void k_count(CountArgs a) {
     constexpr int SLOTS = 1 << LS;  // LDS table slots (the tuner picks 2^10 .. 2^12)
-    constexpr int PER_T = CNT_ROUND / CNT_THREADS;
+    constexpr int PER_T = CNT_ROUND / CNT_THREADS_K;
     __shared__ unsigned long long s_keys[SLOTS];
     __shared__ uint32_t s_vals[SLOTS];
     __shared__ ulonglong2 s_raw[CNT_ROUND];     // the round's records
@@ -1448,8 +1449,8 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
     __shared__ uint32_t s_agg[CNT_ROUND];       // per representative: copies, or OR of dataset bits
     __shared__ uint16_t s_order[CNT_ROUND];     // representatives, most k-mers first
     __shared__ uint32_t s_bucket[CNT_NBUCKET];  // representatives per k-mer count (descending)
-    __shared__ int s_wsum[CNT_THREADS / 64];
-    __shared__ unsigned long long s_red[CNT_THREADS / 64];
+    __shared__ int s_wsum[CNT_THREADS_K / 64];
+    __shared__ unsigned long long s_red[CNT_THREADS_K / 64];
     __shared__ unsigned long long s_obase, s_pad[2];
     __shared__ int s_wrap;  // a count wrapped in the current partition
     const int tid = threadIdx.x;
@@ -1462,12 +1463,12 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
     if (tid == 0) s_wrap = 0;
     unsigned long long my_distinct = 0;
     // the tables start empty; each round and each emit leave them empty again
-    for (int i = tid; i < SLOTS; i += CNT_THREADS) {
+    for (int i = tid; i < SLOTS; i += CNT_THREADS_K) {
         s_keys[i] = EMPTY_KEY;
         s_vals[i] = 0;
     }
-    for (int i = tid; i < CNT_DSLOTS; i += CNT_THREADS) s_dtab[i] = 0;
-    for (int i = tid; i < CNT_ROUND; i += CNT_THREADS) s_agg[i] = 0;
+    for (int i = tid; i < CNT_DSLOTS; i += CNT_THREADS_K) s_dtab[i] = 0;
+    for (int i = tid; i < CNT_ROUND; i += CNT_THREADS_K) s_agg[i] = 0;
     if (tid < CNT_NBUCKET) s_bucket[tid] = 0;
     __syncthreads();
 
@@ -1489,7 +1490,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
         le = e;
 #pragma unroll
         for (int i = 0; i < PER_T; i++) {
-            const uint64_t ri = b + (uint64_t)(i * CNT_THREADS + tid);
+            const uint64_t ri = b + (uint64_t)(i * CNT_THREADS_K + tid);
             if (ri < e) rr[i] = a.records[ri];
         }
     };
@@ -1504,7 +1505,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
             const int nr = (int)(ce - cb);
 #pragma unroll
             for (int i = 0; i < PER_T; i++) {
-                const int li = i * CNT_THREADS + tid;
+                const int li = i * CNT_THREADS_K + tid;
                 if (li < nr) s_raw[li] = rr[i];
             }
             __syncthreads();
@@ -1513,7 +1514,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
             uint32_t rk[PER_T];
 #pragma unroll
             for (int i = 0; i < PER_T; i++) {
-                const int li = i * CNT_THREADS + tid;
+                const int li = i * CNT_THREADS_K + tid;
                 bk[i] = -1;
                 rk[i] = 0;
                 if (li < nr) {
@@ -1560,18 +1561,19 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
 #pragma unroll
                 for (int i = 0; i < PER_T; i++) {
                     const int at = __shfl(st, bk[i] < 0 ? 0 : bk[i], 64);
-                    if (bk[i] >= 0) s_order[at + (int)rk[i]] = (uint16_t)(i * CNT_THREADS + tid);
+                    if (bk[i] >= 0) s_order[at + (int)rk[i]] = (uint16_t)(i * CNT_THREADS_K + tid);
                 }
             }
-            for (int i = tid; i < CNT_DSLOTS; i += CNT_THREADS) s_dtab[i] = 0;
+            for (int i = tid; i < CNT_DSLOTS; i += CNT_THREADS_K) s_dtab[i] = 0;
             __syncthreads();
             if (tid < CNT_NBUCKET) s_bucket[tid] = 0;  // next round (ordered by its barriers)
-            // group g of 256 representatives: lane l of wave w takes the
-            // (4 l + w)-th, so that records with equal k-mers at equal steps
-            // (copies of one locus with an error or cut, adjacent in the
-            // order) mostly fall into different waves
-            for (int g = 0; g < nrep; g += CNT_THREADS) {
-                const int li = g + 4 * lane + wid;
+            // group g of CNT_THREADS_K representatives, in runs of 256: lane
+            // l of wave w takes the (4 l + w % 4)-th of run w / 4, so that
+            // records with equal k-mers at equal steps (copies of one locus
+            // with an error or cut, of equal length) spread over four waves
+            // while each wave's records stay close in length
+            for (int g = 0; g < nrep; g += CNT_THREADS_K) {
+                const int li = g + (wid >> 2) * 256 + 4 * lane + (wid & 3);
                 ulonglong2 r = make_ulonglong2(0ull, 0ull);
                 int n = 0;
                 uint32_t val = 0;
@@ -1610,13 +1612,13 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
             __syncthreads();
         }
         // emit: count survivors, one reservation per partition, then write
-        constexpr int PER = SLOTS / CNT_THREADS;
+        constexpr int PER = SLOTS / CNT_THREADS_K;
         uint32_t emask = 0;
         int cnt = 0;
         const bool wrapped = MODE == 0 && s_wrap;
 #pragma unroll
         for (int i = 0; i < PER; i++) {
-            const int sl = i * CNT_THREADS + tid;
+            const int sl = i * CNT_THREADS_K + tid;
             const unsigned long long key = s_keys[sl];
             if (key == EMPTY_KEY) continue;
             my_distinct++;
@@ -1631,7 +1633,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
         __syncthreads();
         int pre = 0, tot = 0;
 #pragma unroll
-        for (int w = 0; w < CNT_THREADS / 64; w++) {
+        for (int w = 0; w < CNT_THREADS_K / 64; w++) {
             pre += (w < wid) ? s_wsum[w] : 0;
             tot += s_wsum[w];
         }
@@ -1641,7 +1643,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count(CountArgs a) {
         unsigned long long pos = s_obase + (unsigned long long)(pre + x - cnt);
 #pragma unroll
         for (int i = 0; i < PER; i++) {
-            const int sl = i * CNT_THREADS + tid;
+            const int sl = i * CNT_THREADS_K + tid;
             if (emask & (1u << i)) {
                 if (pos < a.out_cap) {
                     a.out_kmer[pos] = s_keys[sl];
@@ -2461,9 +2463,9 @@ static CountFn count_kernel(int ls) {
 // (the output padding is at most one chunk per block).
 static uint64_t count_grid_blocks(const McPlan &p, bool wide) {
     Device &d = Device::get();
-    static const unsigned res[3] = {resident_blocks((const void *)k_count<0, 10>, CNT_THREADS),
-                                    resident_blocks((const void *)k_count<0, 11>, CNT_THREADS),
-                                    resident_blocks((const void *)k_count<0, 12>, CNT_THREADS)};
+    static const unsigned res[3] = {resident_blocks((const void *)k_count<0, 10>, CNT_THREADS_K),
+                                    resident_blocks((const void *)k_count<0, 11>, CNT_THREADS_K),
+                                    resident_blocks((const void *)k_count<0, 12>, CNT_THREADS_K)};
     static const unsigned wide_res = resident_blocks((const void *)k_count_wide, CNT_THREADS);
     const unsigned r = wide ? wide_res : res[std::min(std::max(p.cnt_ls, 10), 12) - 10];
     return std::min<uint64_t>(p.p_int, (uint64_t)d.cu_count() * r);
@@ -2513,11 +2515,11 @@ static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg,
         if (tm) tm->mark(s);
         hipLaunchKernelGGL(k_overflow_emit_wide, dim3(ogrid), dim3(256), 0, s, ca);
     } else if (mode == MCZI_MODE_DATASETS) {
-        hipLaunchKernelGGL(count_kernel<1>(p.cnt_ls), dim3(cgrid), dim3(CNT_THREADS), 0, s, ca);
+        hipLaunchKernelGGL(count_kernel<1>(p.cnt_ls), dim3(cgrid), dim3(CNT_THREADS_K), 0, s, ca);
         if (tm) tm->mark(s);
         hipLaunchKernelGGL(k_overflow_emit<1>, dim3(ogrid), dim3(256), 0, s, ca);
     } else {
-        hipLaunchKernelGGL(count_kernel<0>(p.cnt_ls), dim3(cgrid), dim3(CNT_THREADS), 0, s, ca);
+        hipLaunchKernelGGL(count_kernel<0>(p.cnt_ls), dim3(cgrid), dim3(CNT_THREADS_K), 0, s, ca);
         if (tm) tm->mark(s);
         hipLaunchKernelGGL(k_overflow_emit<0>, dim3(ogrid), dim3(256), 0, s, ca);
     }
