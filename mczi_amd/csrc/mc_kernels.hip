@@ -205,6 +205,16 @@ __device__ __forceinline__ uint32_t extract16_bases(const uint32_t *sw, int pos)
     return (uint32_t)(v >> sh) & 0xffffu;
 }
 
+// 32 bases starting at base `pos` of the LDS word stream (base j at bits 2j):
+// two 32-bit funnel shifts over three words, no branch on the bit offset. The
+// word after the last one read must exist (the word stream is padded).
+__device__ __forceinline__ uint64_t lds_bases32(const uint32_t *sw, int pos) {
+    const int i = pos >> 4;
+    const uint32_t sh = (uint32_t)(pos & 15) * 2u;
+    const uint32_t w0 = sw[i], w1 = sw[i + 1], w2 = sw[i + 2];
+    return ((uint64_t)__builtin_amdgcn_alignbit(w2, w1, sh) << 32) | __builtin_amdgcn_alignbit(w1, w0, sh);
+}
+
 // Bytes of a super-k-mer of L bases in a partition file (append_record_len
 // and append_packed_dna_range, src/lib.rs:3170-3216).
 __device__ __forceinline__ uint64_t skm_bytes(uint32_t L) { return (L < 255u ? 1u : 5u) + (L + 3u) / 4u; }
@@ -308,7 +318,7 @@ __device__ __forceinline__ uint64_t write_piece(const ScanArgs &a, const PieceIn
             Rec32 rw;
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-                uint64_t x = extract32_bases(s_words, (uint64_t)(rb0 + cs + 32 * c));
+                uint64_t x = lds_bases32(s_words, rb0 + cs + 32 * c);
                 const int nbc = (int)L - 32 * c;  // bases of this word
                 if (nbc <= 0) x = 0;
                 else if (nbc < 32) x &= (1ull << (2 * nbc)) - 1ull;
@@ -322,11 +332,11 @@ __device__ __forceinline__ uint64_t write_piece(const ScanArgs &a, const PieceIn
                 atomicOr(a.err, 2ull);
             }
         } else {
-            uint64_t lo64 = extract32_bases(s_words, (uint64_t)(rb0 + cs));
+            uint64_t lo64 = lds_bases32(s_words, rb0 + cs);
             uint64_t hi64 = 0;
             if (L < 32) lo64 &= (1ull << (2 * L)) - 1ull;
             if (L > 32) {
-                hi64 = extract32_bases(s_words, (uint64_t)(rb0 + cs + 32));
+                hi64 = lds_bases32(s_words, rb0 + cs + 32);
                 hi64 &= (1ull << (2 * (L - 32))) - 1ull;
             }
             lo64 ^= (lo64 >> 1) & 0x5555555555555555ull;  // packed code -> MC code
@@ -534,11 +544,11 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             // then drops base i - 1 (from H0) and adds base i - 1 + k (from
             // E, the bases from k on).
             constexpr bool KBIG = MODE == WINDOWS || WIDE;  // k may exceed 32
-            const uint64_t H0 = extract32_bases(s_words, (uint64_t)rb) & 0xAAAAAAAAAAAAAAAAull;
-            const uint32_t E = (uint32_t)extract32_bases(s_words, (uint64_t)rb + (uint64_t)k);
+            const uint64_t H0 = lds_bases32(s_words, rb) & 0xAAAAAAAAAAAAAAAAull;
+            const uint32_t E = (uint32_t)lds_bases32(s_words, rb + k);
             uint32_t tg = (uint32_t)__popcll(H0 & high_bit_mask(k < 32 ? k : 32));
             if (KBIG && k > 32)
-                tg += (uint32_t)__popcll(extract32_bases(s_words, (uint64_t)rb + 32) & high_bit_mask(k - 32));
+                tg += (uint32_t)__popcll(lds_bases32(s_words, rb + 32) & high_bit_mask(k - 32));
             // suffix minima over [i, w-1] (the shared middle folded in); prefix
             // minima over [w, w+i-1] are kept running in the window loop
             uint32_t sufL[WPT], sufR[WPT], preL = ~0u, preR = ~0u;
