@@ -366,12 +366,12 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             uint32_t sb[PADDED(SPAN + MAXW)];
         } pre;
         struct {
-            uint32_t v[PADDED(TILE)];
+            uint32_t v[PADDED(SPAN)];
             uint32_t pst[TILE];
         } run;
     } s_u;
     __shared__ uint32_t s_wxor[2][SCAN_THREADS / 64];  // per-wave XOR totals of the two prefix streams
-    __shared__ uint8_t s_flag[PADDED(TILE)];
+    __shared__ uint8_t s_flag[PADDED(SPAN)];
     __shared__ int32_t s_fs[TILE + 2];
     __shared__ uint8_t s_ds[SCAN_THREADS];  // fof: dataset - ds_lo of the tile's first fragments (255: other group)
     __shared__ int s_wsum[SCAN_THREADS / 64];
@@ -579,7 +579,9 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
                     preL = min(preL, h | (uint32_t)t);
                     preR = min(preR, h | (0xffffu - (uint32_t)t));
                 }
-                if (r >= tl) break;
+                // windows past the tile end are computed as invalid (no break,
+                // so the loop has no nested exec masks; the LDS arrays are
+                // padded for them)
                 if (i > 0) tg = tg + ((E >> (2 * i - 1)) & 1u) - (((uint32_t)H0 >> (2 * i - 1)) & 1u);
                 // fragments are never empty (batches drop empty ones), so one
                 // window step crosses at most one fragment start
@@ -590,7 +592,7 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
                 }
                 jd |= (uint32_t)(j - jfirst) << (4 * i);
                 const bool fstart = fcur == r;
-                const bool valid = r + k <= fnext;
+                const bool valid = r + k <= fnext && r < tl;
                 uint32_t v = 0;
                 if constexpr (GW) {
                     // read for every window (the positions are in the tile's
