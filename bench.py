@@ -350,7 +350,7 @@ def zi_stream(M, k, n_requests=10_000, pool_reads=1 << 24, seed=5, cpu=True, lo=
         # `window_bytes` of request reads are outstanding, results collected
         # in order (continuous batching behind a bounded client queue)
         t0 = time.perf_counter()
-        window_bytes = 6 << 30
+        window_bytes = 1 << 30
         burst, queue, nxt, outstanding = [], [], 0, 0
         while len(burst) < n_requests:
             while nxt < n_requests and (not queue or outstanding + int(counts[nxt]) * 150 // 4 <= window_bytes):

@@ -90,9 +90,58 @@ size_t Device::pooled_bytes() {
     return t;
 }
 
+bool Device::add_slab_locked() {
+    if (slab_bytes_ + kSlabBytes > kSlabCapBytes) return false;
+    void *p = nullptr;
+    if (hipHostMalloc(&p, kSlabBytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    slabs_[(char *)p] = kSlabBytes;
+    slab_bytes_ += kSlabBytes;
+    bfree_[kOrders - 1].insert((char *)p);
+    return true;
+}
+
+// Smallest free block of at least `order`, split down to it (the upper
+// halves go to the free lists); nullptr when no slab can be added.
+char *Device::buddy_take_locked(int order) {
+    int j = order;
+    while (j < kOrders && bfree_[j].empty()) j++;
+    if (j == kOrders) {
+        if (!add_slab_locked()) return nullptr;
+        j = kOrders - 1;
+    }
+    char *p = *bfree_[j].begin();
+    bfree_[j].erase(bfree_[j].begin());
+    while (j > order) {
+        j--;
+        bfree_[j].insert(p + (kMinBlock << j));
+    }
+    blive_[p] = order;
+    return p;
+}
+
+void Device::host_reserve(size_t bytes) {
+    std::lock_guard<std::mutex> lk(hmu_);
+    while (slab_bytes_ < bytes && add_slab_locked()) {
+    }
+}
+
+size_t Device::host_slab_bytes() {
+    std::lock_guard<std::mutex> lk(hmu_);
+    return slab_bytes_;
+}
+
 void *Device::host_alloc(size_t bytes, bool allow_pageable) {
-    size_t want = 64u << 10;
+    size_t want = kMinBlock;
     while (want < bytes) want <<= 1;
+    if (want <= kSlabBytes) {
+        int order = 0;
+        while ((kMinBlock << order) < want) order++;
+        std::lock_guard<std::mutex> lk(hmu_);
+        if (char *p = buddy_take_locked(order)) return p;
+    }
     {
         // a cached block up to 4x the size is reused: pinning fresh pages
         // costs far more than the memory a larger block holds
@@ -239,6 +288,25 @@ void Device::download(void *dst, const void *src, size_t bytes, hipStream_t s) {
 void Device::host_release(void *p) {
     if (!p) return;
     std::lock_guard<std::mutex> lk(hmu_);
+    auto bl = blive_.find((char *)p);
+    if (bl != blive_.end()) {
+        int o = bl->second;
+        blive_.erase(bl);
+        char *q = (char *)p;
+        auto sl = slabs_.upper_bound(q);
+        --sl;  // the slab holding q
+        char *const base = sl->first;
+        while (o < kOrders - 1) {
+            char *const buddy = base + ((size_t)(q - base) ^ (kMinBlock << o));
+            auto f = bfree_[o].find(buddy);
+            if (f == bfree_[o].end()) break;
+            bfree_[o].erase(f);
+            q = std::min(q, buddy);
+            o++;
+        }
+        bfree_[o].insert(q);
+        return;
+    }
     auto it = hlive_.find(p);
     if (it == hlive_.end()) {
         auto pg = hpage_.find(p);

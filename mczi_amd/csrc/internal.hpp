@@ -8,6 +8,8 @@
 #include <atomic>
 #include <functional>
 #include <map>
+#include <set>
+#include <unordered_map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -89,14 +91,24 @@ class Device {
     void *alloc(size_t bytes);
     void release(void *p);
     size_t pooled_bytes();  // released device blocks kept for reuse
-    // Cached pinned host allocation (hipHostMalloc, power-of-two size
-    // classes): device <-> host copies into it run as plain DMA. Released
-    // blocks stay cached up to kPinnedCacheBytes.
+    // Pinned host allocation: device <-> host copies into it run as plain
+    // DMA. Blocks up to kSlabBytes come from a buddy allocator over 1 GiB
+    // slabs of hipHostMalloc memory (power-of-two blocks from 64 KiB; a
+    // released block merges with its free buddy), so a steady stream of
+    // requests of any mix of sizes reuses pinned pages instead of pinning
+    // fresh ones (about 29 ms per GiB, and it stalls the threads copying into
+    // other pinned blocks meanwhile). Slabs stay pinned until the process
+    // ends, at most kSlabCapBytes of them. Larger blocks are pinned on their
+    // own and cached by size up to kPinnedCacheBytes.
     // With allow_pageable, a request the host cannot pin (a process may pin
     // only so much) is served from ordinary memory instead of failing;
-    // host_release frees either kind.
+    // host_release frees every kind.
     void *host_alloc(size_t bytes, bool allow_pageable = false);
     void host_release(void *p);
+    // Pins slabs until at least `bytes` of them exist (a server's staging
+    // arena, taken once at creation rather than on the request path).
+    void host_reserve(size_t bytes);
+    size_t host_slab_bytes();
     // Host -> device copy of pageable memory: the source is copied by
     // several host threads into two alternating pinned staging blocks, each
     // sent by DMA while the next one fills. Synchronous on `s`.
@@ -117,11 +129,22 @@ class Device {
     std::multimap<size_t, void *> free_;
     std::map<void *, size_t> live_;
     static constexpr size_t kPinnedCacheBytes = 16ull << 30;
+    static constexpr size_t kMinBlock = 64u << 10;
+    static constexpr int kOrders = 15;  // 64 KiB << 14 = 1 GiB
+    static constexpr size_t kSlabBytes = kMinBlock << (kOrders - 1);
+    static constexpr size_t kSlabCapBytes = 48ull << 30;
     std::mutex hmu_;
     std::multimap<size_t, void *> hfree_;
     std::map<void *, size_t> hlive_;
     std::map<void *, size_t> hpage_;  // ordinary memory handed out in place of pinned
     size_t hcached_ = 0;
+    // buddy arena (under hmu_)
+    std::set<char *> bfree_[kOrders];
+    std::unordered_map<char *, int> blive_;  // live block -> order
+    std::map<char *, size_t> slabs_;         // slab base -> bytes
+    size_t slab_bytes_ = 0;
+    bool add_slab_locked();
+    char *buddy_take_locked(int order);
 };
 
 // RAII device buffer from the cache.

@@ -70,6 +70,7 @@ struct ZiQueryResult {
     std::vector<uint64_t> windows;  // per request
     std::vector<uint64_t> absent_windows;  // per request, repeats counted
     double wait_ms = 0;             // host wait for the batch's device work
+    double alloc_ms = 0;            // host time spent allocating the result arrays
 };
 
 // Device memory a batch of `bases` query bases in `frags` fragments needs.

@@ -799,6 +799,10 @@ double zi_finish(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s) {
     return wait_ms;
 }
 
+static double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 void zi_copy_out(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, ZiQueryResult &out) {
     const bool wide = idx.wide;
     out.absent.clear();
@@ -811,9 +815,11 @@ void zi_copy_out(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, Z
         const uint64_t c = w.h_count[r];
         ZiKeys &kk = out.absent[r];
         kk.n = c;
+        const double ta = now_ms();
         kk.lo = (uint64_t *)Device::get().host_alloc(std::max<uint64_t>(c, 1) * 8, true);
         if (wide) kk.hi = (uint64_t *)Device::get().host_alloc(std::max<uint64_t>(c, 1) * 8, true);
         require(kk.lo && (!wide || kk.hi), MCZI_E_NOMEM, "out of host memory for ZI results");
+        out.alloc_ms += now_ms() - ta;
         if (c)
             MCZI_HIP(hipMemcpyAsync(kk.lo, w.ulo.as<unsigned long long>() + w.h_start[r], c * 8,
                                     hipMemcpyDeviceToHost, s));
@@ -832,9 +838,6 @@ void zi_collect(ZiWork &w, const ZiIndex &idx, uint32_t n_req, hipStream_t s, Zi
 
 // ---------------------------------------------------------------------------
 // Serving loop
-static double now_ms() {
-    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
 
 struct ZiRequest {
     uint64_t id = 0;
@@ -852,10 +855,13 @@ struct ZiRequest {
         pin = (uint32_t *)Device::get().host_alloc(std::max<uint64_t>(n_words, 1) * 4, true);
         alloc_ms = now_ms() - t0;
         const size_t grain = 1u << 20;  // words per worker chunk (4 MiB)
+        const double t1 = now_ms();
         parallel_ranges(n_words, grain, host_workers(n_words), [&](size_t b, size_t e) {
             memcpy(pin + b, src + b, (e - b) * 4);
         });
+        copy_ms = now_ms() - t1;
     }
+    double copy_ms = 0;  // time of the copy into the request's words
     bool make_fasta = false;
     uint64_t bytes = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
@@ -1042,6 +1048,11 @@ void mczi_zi_server::finish_loop() {
                 msg = e.what();
             }
         }
+        if (!failed && !err)
+            for (ZiRequest *r : sl->batch) {  // the words reached the device: their pinned block is free
+                Device::get().host_release(r->pin);
+                r->pin = nullptr;
+            }
         lk.lock();
         if (err) {
             sl->err = err;
@@ -1095,8 +1106,10 @@ void mczi_zi_server::done_loop() {
             uint64_t nb = 0;
             for (size_t i = 0; i < batch.size() && !err; i++) nb += batch[i]->absent.n * (idx.wide ? 16 : 8);
             fprintf(stderr,
-                    "ZI_TRACE t=%.2f done reqs=%zu wait_ms=%.2f sort_ms=%.2f copy_ms=%.2f post_ms=%.2f copy_bytes=%llu\n",
-                    now_ms(), batch.size(), wait_ms, sort_ms, t1 - t0, now_ms() - t1, (unsigned long long)nb);
+                    "ZI_TRACE t=%.2f done reqs=%zu wait_ms=%.2f sort_ms=%.2f copy_ms=%.2f alloc_ms=%.2f post_ms=%.2f "
+                    "copy_bytes=%llu pinned_gib=%.1f\n",
+                    now_ms(), batch.size(), wait_ms, sort_ms, t1 - t0, res.alloc_ms, now_ms() - t1,
+                    (unsigned long long)nb, Device::get().host_slab_bytes() / double(1ull << 30));
         }
         lk.lock();
         busy_ms += t1 - t0 + sort_ms;
@@ -1311,6 +1324,14 @@ static int server_create(HostReads &&ir, uint32_t k, uint32_t m, uint64_t budget
             zi_build_index(s->idx, std::move(ir), k, m, d.stream());
         }
         s->budget = budget ? budget : (8ull << 30);
+        // pinned staging for request words and absent sets, taken once here:
+        // pinning on the request path costs about 29 ms per GiB and stalls
+        // the copies into other pinned blocks (MCZI_ZI_HOST_ARENA_GIB, default 16)
+        {
+            const char *e = getenv("MCZI_ZI_HOST_ARENA_GIB");
+            const uint64_t gib = e ? strtoull(e, nullptr, 10) : 16;
+            d.host_reserve(gib << 30);
+        }
         s->max_inflight = std::max<uint32_t>(1, std::min<uint32_t>(max_inflight ? max_inflight : 2, 8));
         s->slots.resize(s->max_inflight);
         for (auto &sl : s->slots) MCZI_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
@@ -1367,9 +1388,9 @@ int mczi_zi_server_submit(mczi_zi_server *s, const char *fastx, size_t len, int 
         r->id = s->next_id++;
         *request_id = r->id;
         if (s->trace)
-            fprintf(stderr, "ZI_TRACE t=%.2f submit id=%llu bases=%llu submit_ms=%.2f pin_alloc_ms=%.2f\n", now_ms(),
-                    (unsigned long long)r->id, (unsigned long long)r->reads.n_bases(), now_ms() - r->t_submit,
-                    r->alloc_ms);
+            fprintf(stderr, "ZI_TRACE t=%.2f submit id=%llu bases=%llu submit_ms=%.2f pin_alloc_ms=%.2f pin_copy_ms=%.2f\n",
+                    now_ms(), (unsigned long long)r->id, (unsigned long long)r->reads.n_bases(),
+                    now_ms() - r->t_submit, r->alloc_ms, r->copy_ms);
         s->queue.push_back(r.get());
         s->reqs[r->id] = std::move(r);
         s->cv.notify_all();
@@ -1405,9 +1426,9 @@ int mczi_zi_server_submit_packed(mczi_zi_server *s, const uint32_t *words, const
         r->id = s->next_id++;
         *request_id = r->id;
         if (s->trace)
-            fprintf(stderr, "ZI_TRACE t=%.2f submit id=%llu bases=%llu submit_ms=%.2f pin_alloc_ms=%.2f\n", now_ms(),
-                    (unsigned long long)r->id, (unsigned long long)r->reads.n_bases(), now_ms() - r->t_submit,
-                    r->alloc_ms);
+            fprintf(stderr, "ZI_TRACE t=%.2f submit id=%llu bases=%llu submit_ms=%.2f pin_alloc_ms=%.2f pin_copy_ms=%.2f\n",
+                    now_ms(), (unsigned long long)r->id, (unsigned long long)r->reads.n_bases(),
+                    now_ms() - r->t_submit, r->alloc_ms, r->copy_ms);
         s->queue.push_back(r.get());
         s->reqs[r->id] = std::move(r);
         s->cv.notify_all();
