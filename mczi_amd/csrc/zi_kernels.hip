@@ -382,6 +382,75 @@ __global__ void k_zi_gather_range(const unsigned long long *src, const uint32_t 
         dst[i] = src[idx[i]];
 }
 
+// The k > 32 counterpart of k_finish_runs: (hi, lo) pairs already ordered
+// by hi >> shift (shift >= 32, so a run's bounds read the same while its
+// elements are rewritten); each run of equal prefixes is ordered by the full
+// 128-bit key in LDS, every element moving to its run's start plus its
+// stable rank. A run longer than FINISH_MAX sets *flag (the caller then sorts
+// the segment in full).
+__global__ __launch_bounds__(256) void k_finish_runs_wide(unsigned long long *hi, unsigned long long *lo, uint64_t n,
+                                                          int shift, unsigned long long *flag) {
+    __shared__ unsigned long long sh[FIN_CHUNK + FINISH_MAX + 1], sl[FIN_CHUNK + FINISH_MAX + 1];
+    __shared__ uint64_t s_a, s_b;
+    for (uint64_t c0 = (uint64_t)blockIdx.x * FIN_CHUNK; c0 < n; c0 += (uint64_t)gridDim.x * FIN_CHUNK) {
+        const uint64_t c1 = c0 + FIN_CHUNK < n ? c0 + FIN_CHUNK : n;
+        __syncthreads();  // the previous chunk is written back
+        if (threadIdx.x < 64) {
+            // [a, b): from the first run starting in [c0, c1) to the end of
+            // the run holding c1 - 1 (the first wave tests 64 positions at
+            // once on each side)
+            const int lane = threadIdx.x;
+            uint64_t a = c0;
+            if (c0 > 0) {
+                const unsigned long long pp = hi[c0 - 1] >> shift;
+                const uint64_t i = c0 + (uint64_t)lane;
+                const bool other = i >= c1 || (hi[i] >> shift) != pp;
+                const unsigned long long m = __ballot(other);
+                a = m ? c0 + (uint64_t)__ffsll((long long)m) - 1 : c0 + FINISH_MAX + 1;
+            }
+            uint64_t b = a;
+            if (a < c1) {
+                const unsigned long long pl = hi[c1 - 1] >> shift;
+                const uint64_t i = c1 + (uint64_t)lane;
+                const bool other = i >= n || (hi[i] >> shift) != pl;
+                const unsigned long long m = __ballot(other);
+                b = m ? c1 + (uint64_t)__ffsll((long long)m) - 1 : c1 + FINISH_MAX;
+                if (!m && lane == 0) atomicOr(flag, 1ull);
+            }
+            if (lane == 0) {
+                if (a > c0 + FINISH_MAX) atomicOr(flag, 1ull);
+                s_a = a < c1 ? a : c1;
+                s_b = a < c1 ? b : c1;
+            }
+        }
+        __syncthreads();
+        const uint64_t a = s_a, b = s_b;
+        const int len = (int)(b - a);
+        for (int i = threadIdx.x; i < len; i += blockDim.x) {
+            sh[i] = hi[a + i];
+            sl[i] = lo[a + i];
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < len; i += blockDim.x) {
+            const unsigned long long kh = sh[i], kl = sl[i], pfx = kh >> shift;
+            int rs = i;
+            while (rs > 0 && (sh[rs - 1] >> shift) == pfx && i - rs < FINISH_MAX) rs--;
+            int rank = 0, e = rs;
+            while (e < len && (sh[e] >> shift) == pfx && e - rs <= FINISH_MAX) {
+                const unsigned long long eh = sh[e], el = sl[e];
+                rank += eh < kh || (eh == kh && (el < kl || (el == kl && e < i)));
+                e++;
+            }
+            if (e - rs > FINISH_MAX) {
+                atomicOr(flag, 1ull);
+                continue;
+            }
+            hi[a + rs + rank] = kh;
+            lo[a + rs + rank] = kl;
+        }
+    }
+}
+
 // Unique within each request's sorted segment: flag the first of every run
 // of equal keys among the segment's filled entries [base, end).
 template <bool WIDE>
@@ -704,12 +773,22 @@ static void zi_sort_segments(ZiWork &w, const ZiIndex &idx, const std::vector<ui
     unsigned long long *ohi = w.ohi.as<unsigned long long>(), *shi = w.shi.as<unsigned long long>(),
                        *uhi = w.uhi.as<unsigned long long>();
     uint32_t *i1 = w.idx.as<uint32_t>(), *i2 = w.idx2.as<uint32_t>();
-    size_t t1 = 0, t2 = 0;
+    // k >= 60 (the high word holds at least 56 key bits): large segments
+    // sort (hi, position) by the top 24 key bits only, gather lo, and order
+    // the short runs of equal prefixes by the full key in LDS; otherwise (and
+    // when a run is too long) LSD over lo, then hi, carrying positions.
+    constexpr unsigned PREFIX_BITS = 24;
+    constexpr uint64_t SPLIT_MIN = 1u << 16;
+    const bool can_split = hbits >= PREFIX_BITS + 32;
+    size_t t1 = 0, t2 = 0, tp = 0;
     MCZI_HIP(rocprim::radix_sort_pairs(nullptr, t1, olo, slo, i1, i2, (size_t)max_n, 0, 64, s));
     MCZI_HIP(rocprim::radix_sort_pairs(nullptr, t2, shi, uhi, i2, i1, (size_t)max_n, 0, hbits, s));
-    w.tmp.ensure(std::max(t1, t2) + 16);
-    for (uint32_t r = 0; r < n_req; r++) {
-        if (n[r] == 0) continue;
+    if (can_split)
+        MCZI_HIP(rocprim::radix_sort_pairs(nullptr, tp, ohi, uhi, i1, i2, (size_t)max_n, hbits - PREFIX_BITS, hbits,
+                                           s));
+    w.tmp.ensure(std::max(std::max(t1, t2), tp) + 16);
+    unsigned long long *flag = w.ctr.as<unsigned long long>() + 7;
+    auto full_sort = [&](uint32_t r) {
         const uint64_t b0 = tb[r], e0 = tb[r] + n[r];
         size_t ts1 = t1, ts2 = t2;
         hipLaunchKernelGGL(k_zi_iota, dim3(zgrid(n[r])), dim3(256), 0, s, i1 + b0, n[r], w.h_base[r]);
@@ -719,6 +798,32 @@ static void zi_sort_segments(ZiWork &w, const ZiIndex &idx, const std::vector<ui
         MCZI_HIP(rocprim::radix_sort_pairs(w.tmp.as<void>(), ts2, shi + b0, uhi + b0, i2 + b0, i1 + b0, (size_t)n[r],
                                            0, hbits, s));
         hipLaunchKernelGGL(k_zi_gather_range, dim3(zgrid(n[r])), dim3(256), 0, s, olo, i1, slo, b0, e0);
+    };
+    std::vector<uint32_t> split;
+    if (can_split) zfill(flag, 1, 0, s);
+    for (uint32_t r = 0; r < n_req; r++) {
+        if (n[r] == 0) continue;
+        if (!(can_split && n[r] >= SPLIT_MIN && (n[r] >> PREFIX_BITS) < FINISH_MAX)) {
+            full_sort(r);
+            continue;
+        }
+        const uint64_t b0 = tb[r], e0 = tb[r] + n[r];
+        size_t ts = tp;
+        hipLaunchKernelGGL(k_zi_iota, dim3(zgrid(n[r])), dim3(256), 0, s, i1 + b0, n[r], w.h_base[r]);
+        MCZI_HIP(rocprim::radix_sort_pairs(w.tmp.as<void>(), ts, ohi + w.h_base[r], uhi + b0, i1 + b0, i2 + b0,
+                                           (size_t)n[r], hbits - PREFIX_BITS, hbits, s));
+        hipLaunchKernelGGL(k_zi_gather_range, dim3(zgrid(n[r])), dim3(256), 0, s, olo, i2, slo, b0, e0);
+        const unsigned g = (unsigned)std::min<uint64_t>((n[r] + FIN_CHUNK - 1) / FIN_CHUNK, 1u << 20);
+        hipLaunchKernelGGL(k_finish_runs_wide, dim3(g), dim3(256), 0, s, uhi + b0, slo + b0, n[r],
+                           (int)(hbits - PREFIX_BITS), flag);
+        split.push_back(r);
+    }
+    if (!split.empty()) {
+        unsigned long long h = 0;
+        MCZI_HIP(hipMemcpyAsync(&h, flag, 8, hipMemcpyDeviceToHost, s));
+        MCZI_HIP(hipStreamSynchronize(s));
+        if (h)
+            for (uint32_t r : split) full_sort(r);
     }
     (void)N;
 }

@@ -59,13 +59,15 @@ def test_zi_absent_matches_oracle(tmp_path, k):
     assert same_counted(got, want), (len(got), len(want))
 
 
+@pytest.mark.parametrize("k", [31, 63])
 @pytest.mark.parametrize("repeat", [0, 100])
-def test_zi_large_request_sort_paths(tmp_path, repeat):
+def test_zi_large_request_sort_paths(tmp_path, repeat, k):
     """A request with more than 65536 absent windows is ordered by radix
-    passes over the top 24 key bits plus the finishing pass (equal k-mers
-    included); with one read repeated 100 times, runs of equal prefixes exceed
-    64 and the request takes the full sort. Both equal the oracle's set."""
-    print(f"zi large request repeat={repeat}", flush=True)
+    passes over the top 24 key bits (k = 63: of the high word, carrying the
+    low word's position) plus the finishing pass (equal k-mers included);
+    with one read repeated 100 times, runs of equal prefixes exceed 64 and
+    the request takes the full sort. Both equal the oracle's set."""
+    print(f"zi large request repeat={repeat} k={k}", flush=True)
     rng = np.random.default_rng(7 + repeat)
     genome = _genome(rng, 20000)
     other = _genome(rng, 40000)
@@ -74,10 +76,10 @@ def test_zi_large_request_sort_paths(tmp_path, repeat):
     reads += [reads[1]] * repeat
     q = tmp_path / "q.fa"
     write_fasta(q, reads)
-    got = zi_absent(tmp_path / "idx.fa", [q], 31)
+    got = zi_absent(tmp_path / "idx.fa", [q], k)
     ic, io = O.parse_fastx(str(tmp_path / "idx.fa"), 1)
-    qc, qo = O.parse_fastx(str(q), 31)
-    want = O.zi_absent(ic, io, qc, qo, 31)
+    qc, qo = O.parse_fastx(str(q), k)
+    want = O.zi_absent(ic, io, qc, qo, k)
     assert len(want) > 20000
     assert same_counted(got, want), (len(got), len(want))
 
