@@ -66,6 +66,19 @@ def measured_traffic(workload: str, kernel: str):
         return None
 
 
+def zi_traffic(k: int, lookups: int):
+    """HBM bytes of the stream's lookup kernels: the bytes per lookup that
+    the committed PMC summary measured on a request stream of the same
+    shapes (FETCH_SIZE and WRITE_SIZE over every dispatch, scaled like
+    measured_traffic), times the stream's lookups; None without it."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
+            per = json.load(f)["zi_request_stream"]["kernels"][f"k_zi_lookup_k{k}"]["bytes_per_lookup"]
+        return per * lookups
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def decode_range(words, b0, b1):
     """Packed-seq codes (one byte per base) of bases [b0, b1)."""
     import numpy as np
@@ -414,9 +427,10 @@ def zi_stream(M, k, n_requests=10_000, pool_reads=1 << 24, seed=5, cpu=True, lo=
         "plain_p99_ms": float(np.percentile([x[2] for x in plain], 99)),
         "lookup_kernel_lookups_per_s": (l1 - l0) / (kernel_ms * 1e-3) if kernel_ms > 0 else 0.0,
         "roofline": {"bound": "hbm", "kernel": "k_zi_lookup", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": zi_traffic(k, l1 - l0),
                      "algorithmic_bytes": (l1 - l0) * 64, "kernel_ms": kernel_ms,
-                     "note": "64 B per lookup (one index line); lookups/s ceiling 1.25e11"},
+                     "note": "64 B per lookup (one index line); lookups/s ceiling 1.25e11; traffic = the "
+                             "measured HBM bytes per lookup of the committed PMC summary x lookups"},
         "cpu_baseline": None,
     }
     if cpu:
