@@ -23,9 +23,13 @@ HBM before timing starts: minimizer scan + super-k-mer records -> two-pass
 partitioning into buckets -> per-partition LDS counting with HBM spill ->
 count > x -> output order. Each MC workload is autotuned first (LDS table
 size, table fill and so the partition count, scan blocks per CU; untimed).
-`plain` runs the same counts with one scan launch
-per input (dataset file) or per 128 Mi bases; `cpu_baseline` is the oracle's
-restatement of the reference CPU path timed on a stated sample.
+`plain` is the per-file GPU path: with several input files (c3: the long-read
+and short-read files; c4: the 32 dataset files) each file is counted as a job
+of its own (scan, partitioning and count of its tiles, every k-mer kept) and
+the per-file (k-mer, count) pairs are merged on the device (sorted, summed,
+count > x); a single-file workload scans one 128 Mi-base chunk per launch.
+`cpu_baseline` is the oracle's restatement of the reference CPU path timed
+on a stated sample.
 """
 from __future__ import annotations
 
@@ -116,13 +120,17 @@ def make_c2(M):
 
 
 def make_c3(M):
+    import numpy as np
     long = M.SynthReads(M.SynthSpec(genome_len=50_000_000, genome_seed=3, read_seed=103, long_bases=1_000_000_000,
                                     long_median=8000, long_sigma=0.8, long_min=500, long_max=100_000,
                                     sub_rate=0.04, ins_rate=0.02, del_rate=0.02))
     short = M.SynthReads(M.SynthSpec(genome_len=50_000_000, genome_seed=3, read_seed=203, n_reads=3_333_334,
                                      read_len=150, sub_rate=0.005))
     words, off = concat_reads(long, short)
-    return (long, short), words, off, None, None
+    # two inputs: the long-read file (and the word-alignment gap) and the short-read file
+    n_long = len(off) - 1 - (len(short.frag_off) - 1)
+    marks = np.array([0, n_long, len(off) - 1], dtype=np.uint64)
+    return (long, short), words, off, None, marks
 
 
 def make_c4(M):
@@ -258,8 +266,9 @@ def mc_workload(M, name, steps, warmup, cpu=True, plain=True):
             stp = batch.count(cfg, mode, M.LAUNCH_PLAIN, order)
         res["plain_value"] = nb * 2 / (time.perf_counter() - tp)
         res["batched_vs_plain"] = res["value"] / res["plain_value"]
-        res["plain_launch"] = ("one scan launch per dataset file" if marks is not None
-                               else "one scan launch per 128 Mi bases")
+        res["plain_launch"] = ("each input file counted on its own (scan, partitioning, count of every k-mer), "
+                               "the per-file counts merged on the device" if marks is not None
+                               else "one scan launch per 128 Mi bases, one count")
         assert stp.n_out == last["n_out"], "plain and batched counts differ"
     del os.environ["MCZI_NO_GRAPH"]
     R = prof["n_records"]

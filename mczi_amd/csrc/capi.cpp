@@ -862,8 +862,13 @@ static int batch_from_packed(const uint32_t *words, const uint64_t *frag_off, si
                 acc = 0;
             }
         }
+        // input boundaries, empty inputs dropped
+        std::vector<uint64_t> inputs{0};
+        if (input_frag)
+            for (size_t c = 1; c <= n_inputs; c++)
+                if (input_frag[c] != inputs.back()) inputs.push_back(input_frag[c]);
         auto h = std::make_unique<mczi_batch>();
-        h->b = make_batch_packed(words, frag_off, n_frag, dataset, nd, std::move(chunk_frag));
+        h->b = make_batch_packed(words, frag_off, n_frag, dataset, nd, std::move(chunk_frag), std::move(inputs));
         *out = h.release();
     });
 }

@@ -710,7 +710,7 @@ static void parse_buffer(const char *buf, size_t len, uint64_t min_len, uint32_t
 void parse_fastx_text(const char *text, size_t len, uint64_t min_len, uint32_t dataset,
                       HostReads &out) {
     parse_buffer(text, len, min_len, dataset, out, "<memory>");
-    out.end_chunk();
+    out.end_input();
 }
 
 void parse_fastx_file(const std::string &path, uint64_t min_len, uint32_t dataset, HostReads &out) {
@@ -725,7 +725,7 @@ void parse_fastx_file(const std::string &path, uint64_t min_len, uint32_t datase
     size_t len = (size_t)st.st_size;
     if (len == 0) {
         close(fd);
-        out.end_chunk();
+        out.end_input();
         return;
     }
     void *map = mmap(nullptr, len, PROT_READ, MAP_PRIVATE, fd, 0);
@@ -762,7 +762,7 @@ void parse_fastx_file(const std::string &path, uint64_t min_len, uint32_t datase
             acc = 0;
         }
     }
-    out.end_chunk();
+    out.end_input();
 }
 
 // ---------------------------------------------------------------------------

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <array>
 #include <atomic>
 #include <functional>
 #include <map>
@@ -198,6 +199,7 @@ struct HostReads {
     std::vector<uint64_t> off{0};    // fragment boundaries (n_frag + 1)
     std::vector<uint32_t> dataset;   // dataset of each fragment (may be empty)
     std::vector<uint64_t> chunk_frag{0}; // fragment index where each input chunk starts
+    std::vector<uint64_t> input_frag{0}; // fragment index where each input (file) starts
     uint32_t n_datasets = 1;
     uint64_t n_bases() const { return off.back(); }
     uint64_t n_frag() const { return off.size() - 1; }
@@ -222,6 +224,11 @@ struct HostReads {
     }
     void end_chunk() {
         if (chunk_frag.back() != n_frag()) chunk_frag.push_back(n_frag());
+    }
+    // End of one input (file); an input with no fragment leaves no entry.
+    void end_input() {
+        end_chunk();
+        if (input_frag.back() != n_frag()) input_frag.push_back(n_frag());
     }
     void finish() {
         words.resize((cur_ + 15) / 16 + 32, 0);
@@ -287,6 +294,15 @@ struct Batch {
     // base total of the longer ones (window count for any k <= 64)
     uint64_t len_hist[64] = {};
     uint64_t long_cnt = 0, long_sum = 0;
+    // Inputs (files, or the caller's input marks): fragment boundaries (each
+    // input starts a chunk), the dataset of each input when all its
+    // fragments share one (else NO_DATASET), and per input the fragment
+    // length summary above ([0..63] lengths below 64, [64] count and [65]
+    // bases of the longer ones).
+    static constexpr uint32_t NO_DATASET = 0xFFFFFFFFu;
+    std::vector<uint64_t> input_frag;
+    std::vector<uint32_t> input_ds;
+    std::vector<std::array<uint64_t, 66>> input_hist;
     // tile schedule (built with the batch) and per-run state
     std::unique_ptr<McPlan> plan;
     // results of the last count (device)
@@ -303,7 +319,7 @@ struct Batch {
 std::unique_ptr<Batch> make_batch(HostReads &&reads);
 std::unique_ptr<Batch> make_batch_packed(const uint32_t *words, const uint64_t *frag_off, size_t n_frag,
                                          const uint32_t *dataset, uint32_t n_datasets,
-                                         std::vector<uint64_t> &&chunk_frag);
+                                         std::vector<uint64_t> &&chunk_frag, std::vector<uint64_t> &&input_frag);
 void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch, int order,
                  mczi_run_stats *stats);
 void batch_download(Batch &b, std::vector<mczi_counted_kmer> &out);

@@ -1303,6 +1303,40 @@ __device__ __forceinline__ unsigned long long wave_append(bool pred, unsigned lo
     return base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
+// Stream compaction for kernels of SCAN_THREADS threads: the indexes of
+// [0, n) that keep() accepts get consecutive output slots, one atomic
+// reservation per SCAN_THREADS * CMP_ITEMS indexes (a wave-level append
+// would take one same-address atomic per 64 indexes, which serialize);
+// inside a reservation, slots follow the thread, then the round.
+constexpr int CMP_ITEMS = 16;
+template <class Keep, class Emit>
+__device__ __forceinline__ void block_compact(uint64_t n, unsigned long long *cursor, Keep keep, Emit emit) {
+    __shared__ int s_w[SCAN_THREADS / 64];
+    __shared__ unsigned long long s_base;
+    const uint64_t step = (uint64_t)SCAN_THREADS * CMP_ITEMS;
+    for (uint64_t b0 = (uint64_t)blockIdx.x * step; b0 < n; b0 += (uint64_t)gridDim.x * step) {
+        uint32_t m = 0;
+        int cnt = 0;
+#pragma unroll
+        for (int it = 0; it < CMP_ITEMS; it++) {
+            const uint64_t i = b0 + (uint64_t)it * SCAN_THREADS + threadIdx.x;
+            if (i < n && keep(i)) {
+                m |= 1u << it;
+                cnt++;
+            }
+        }
+        int tot = 0;
+        const int off = block_exclusive_scan(cnt, &tot, s_w);
+        if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor, (unsigned long long)tot) : 0ull;
+        __syncthreads();
+        unsigned long long pos = s_base + (unsigned long long)off;
+#pragma unroll
+        for (int it = 0; it < CMP_ITEMS; it++)
+            if ((m >> it) & 1u) emit(b0 + (uint64_t)it * SCAN_THREADS + threadIdx.x, pos++);
+        __syncthreads();  // s_base is rewritten next round
+    }
+}
+
 // Output slots are reserved in chunks of OUT_CHUNK per block: one device-scope
 // atomic per chunk instead of one per partition. The unused rest of an
 // abandoned chunk, and of each block's last one, holds padding entries (key
@@ -2036,6 +2070,9 @@ struct McPlan {
     uint64_t n_tiles = 0;
     std::vector<uint64_t> chunk_tile;  // tile index where each chunk starts
     uint64_t n_windows = 0;
+    // the tiles a scan covers: [tile_lo, tile_hi) (the whole batch, except
+    // while one input is counted on its own)
+    uint64_t tile_lo = 0, tile_hi = ~0ull;
     // state
     // records: the coarse pass's output; stage_rec: the staging slots, then
     // the records in partition order (fine pass); base / cursor: each
@@ -2116,6 +2153,21 @@ static void fill_zero_bytes(void *p, uint64_t bytes, hipStream_t s) {
 
 static void prepare_batch(Batch &b, const uint64_t *off);
 
+// The dataset of each input when all its fragments share one.
+static void set_input_datasets(Batch &b, const uint32_t *dataset) {
+    const size_t ni = b.input_frag.size() - 1;
+    b.input_ds.assign(ni, dataset ? Batch::NO_DATASET : 0u);
+    if (!dataset) return;
+    for (size_t i = 0; i < ni; i++) {
+        const uint64_t f0 = b.input_frag[i], f1 = b.input_frag[i + 1];
+        if (f1 <= f0) continue;
+        const uint32_t d = dataset[f0];
+        bool same = true;
+        for (uint64_t f = f0 + 1; f < f1 && same; f++) same = dataset[f] == d;
+        if (same) b.input_ds[i] = d;
+    }
+}
+
 std::unique_ptr<Batch> make_batch(HostReads &&r) {
     r.finish();
     auto b = std::make_unique<Batch>();
@@ -2124,6 +2176,9 @@ std::unique_ptr<Batch> make_batch(HostReads &&r) {
     b->n_frag = r.n_frag();
     b->n_words = r.words.size();
     b->chunk_frag = r.chunk_frag;
+    b->input_frag = r.input_frag;
+    if (b->input_frag.back() != b->n_frag) b->input_frag.push_back(b->n_frag);
+    set_input_datasets(*b, r.dataset.empty() ? nullptr : r.dataset.data());
     b->n_datasets = r.n_datasets;
     b->words.reset(r.words.size() * 4);
     b->off.reset(r.off.size() * 8);
@@ -2145,7 +2200,7 @@ std::unique_ptr<Batch> make_batch(HostReads &&r) {
 // slack after the data.
 std::unique_ptr<Batch> make_batch_packed(const uint32_t *words, const uint64_t *frag_off, size_t n_frag,
                                          const uint32_t *dataset, uint32_t n_datasets,
-                                         std::vector<uint64_t> &&chunk_frag) {
+                                         std::vector<uint64_t> &&chunk_frag, std::vector<uint64_t> &&input_frag) {
     auto b = std::make_unique<Batch>();
     Device &d = Device::get();
     const uint64_t nb = frag_off[n_frag];
@@ -2155,6 +2210,10 @@ std::unique_ptr<Batch> make_batch_packed(const uint32_t *words, const uint64_t *
     b->n_words = nw + 32;
     b->chunk_frag = std::move(chunk_frag);
     if (b->chunk_frag.back() != n_frag) b->chunk_frag.push_back(n_frag);
+    b->input_frag = std::move(input_frag);
+    if (b->input_frag.empty()) b->input_frag.push_back(0);
+    if (b->input_frag.back() != n_frag) b->input_frag.push_back(n_frag);
+    set_input_datasets(*b, dataset);
     b->n_datasets = n_datasets;
     b->words.reset(b->n_words * 4);
     b->off.reset((n_frag + 1) * 8);
@@ -2201,6 +2260,23 @@ static void build_tiles(Batch &b, McPlan &p, const uint64_t *off) {
         b.long_cnt += h[64];
         b.long_sum += h[65];
     }
+    // the same summary per input
+    const size_t ni = b.input_frag.size() - 1;
+    b.input_hist.assign(ni, std::array<uint64_t, 66>{});
+    parallel_ranges(ni, 64, host_workers(nf), [&](size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; i++) {
+            std::array<uint64_t, 66> &h = b.input_hist[i];
+            for (uint64_t f = b.input_frag[i]; f < b.input_frag[i + 1]; f++) {
+                const uint64_t n = off[f + 1] - off[f];
+                if (n < 64) {
+                    h[n]++;
+                } else {
+                    h[64]++;
+                    h[65] += n;
+                }
+            }
+        }
+    });
     // tiles: chunk c covers bases [off[chunk_frag[c]], off[chunk_frag[c + 1]]);
     // tile t of a chunk starts at its start + t * TILE, in the last fragment
     // of the chunk whose offset is <= that position
@@ -2352,6 +2428,8 @@ static uint64_t launch_scan(McPlan &p, const ScanArgs &sa, int launch, hipStream
     const unsigned scan_grid_max = (unsigned)d.cu_count() * per_cu;
     uint64_t blocks = 0;
     auto go = [&](uint64_t tb, uint64_t te) {
+        tb = std::max(tb, p.tile_lo);
+        te = std::min(te, p.tile_hi);
         if (te <= tb) return;
         ScanArgs ca = sa;
         ca.tile_begin = tb;
@@ -2539,21 +2617,20 @@ static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg,
 }
 
 
-// Output entries [0, n_slots) without the padding, in their order (the
-// unsorted output order).
-__global__ void k_compact_out(const unsigned long long *kin, const unsigned long long *hin, const uint32_t *cin,
-                              const uint32_t *pin, uint64_t n, unsigned long long *kout, unsigned long long *hout,
-                              uint32_t *cout, unsigned long long *cursor) {
-    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < n; i0 += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t i = i0 + threadIdx.x;
-        const bool keep = i < n && pin[i] != OUT_PAD;
-        const unsigned long long pos = wave_append(keep, cursor);
-        if (keep) {
+// Output entries [0, n_slots) without the padding (the unsorted output
+// order: no order is kept). 256-thread blocks.
+__global__ __launch_bounds__(SCAN_THREADS) void k_compact_out(const unsigned long long *kin,
+                                                              const unsigned long long *hin, const uint32_t *cin,
+                                                              const uint32_t *pin, uint64_t n,
+                                                              unsigned long long *kout, unsigned long long *hout,
+                                                              uint32_t *cout, unsigned long long *cursor) {
+    block_compact(
+        n, cursor, [&](uint64_t i) { return pin[i] != OUT_PAD; },
+        [&](uint64_t i, unsigned long long pos) {
             kout[pos] = kin[i];
             if (hin) hout[pos] = hin[i];
             cout[pos] = cin[i];
-        }
-    }
+        });
 }
 
 // The n output slots (n_real k-mers and the padding, which sorts last) into
@@ -2741,8 +2818,12 @@ static bool count_pass_one(Batch &b, McPlan &p, const mczi_counter_config &cfg, 
         // every emitted k-mer has more than x windows (out_scale > 1 covers a
         // partition group holding more than its share); plus one padded chunk
         // per count block
-        const uint64_t out_cap =
-            (uint64_t)((double)(win / (cfg.threshold + 1)) * p.out_scale) + 16 + count_grid_blocks(p, wide) * OUT_CHUNK;
+        // (threshold 0 keeps every distinct k-mer: a partition's survivors
+        // may then abandon up to their own number of slots of a chunk, so
+        // the room doubles)
+        const uint64_t out_cap = (uint64_t)((double)(win / (cfg.threshold + 1)) * p.out_scale *
+                                            (cfg.threshold == 0 ? 2.0 : 1.0)) +
+                                 16 + count_grid_blocks(p, wide) * OUT_CHUNK;
 
         // Staging capacity: the expected record count of a random minimizer
         // scheme (density 2 / (w + 1) per window, plus record splits, one
@@ -2860,8 +2941,9 @@ static bool count_pass_one(Batch &b, McPlan &p, const mczi_counter_config &cfg, 
                 "more than 4096 k-mer counts passed 2^32 in one batch (count carry list full)");
         p.n_staged = h[11];
         p.n_records = h[12];
-        if (h[1] > p.out_cap && G > 1) {
-            // this group holds more than its share of the output
+        if (h[1] > p.out_cap && (G > 1 || !(h[4] & 1ull))) {
+            // this group holds more than its share of the output, or the
+            // chunk padding took more room than planned
             require(attempt < 8, MCZI_E_NOMEM, "output slots overflowed repeatedly");
             p.out_scale *= 2;
             continue;
@@ -2890,20 +2972,17 @@ static void collect_carries(Batch &b, McPlan &p, uint64_t n) {
 
 // Output entries [0, n) without the padding appended at cursor (k-mer, high
 // word, count, partition).
-__global__ void k_append_out(const unsigned long long *kin, const unsigned long long *hin, const uint32_t *cin,
+__global__ __launch_bounds__(SCAN_THREADS) void k_append_out(const unsigned long long *kin, const unsigned long long *hin, const uint32_t *cin,
                              const uint32_t *pin, uint64_t n, unsigned long long *kout, unsigned long long *hout,
                              uint32_t *cout, uint32_t *pout, unsigned long long *cursor) {
-    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < n; i0 += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t i = i0 + threadIdx.x;
-        const bool keep = i < n && pin[i] != OUT_PAD;
-        const unsigned long long pos = wave_append(keep, cursor);
-        if (keep) {
+    block_compact(
+        n, cursor, [&](uint64_t i) { return pin[i] != OUT_PAD; },
+        [&](uint64_t i, unsigned long long pos) {
             kout[pos] = kin[i];
             if (hout) hout[pos] = hin[i];
             cout[pos] = cin[i];
             pout[pos] = pin[i];
-        }
-    }
+        });
 }
 
 // A count that does not fit one device pass (src/lib.rs spills the
@@ -3004,64 +3083,24 @@ static void count_pass(Batch &b, McPlan &p, const mczi_counter_config &cfg, int 
     }
 }
 
-__global__ void k_gt_compact(const unsigned long long *keys, const uint32_t *vals, const unsigned long long *n_ptr,
-                             uint32_t x, unsigned long long *kout, uint32_t *cout, unsigned long long *cursor) {
+__global__ __launch_bounds__(SCAN_THREADS) void k_gt_compact(const unsigned long long *keys, const uint32_t *vals,
+                                                             const unsigned long long *n_ptr, uint32_t x,
+                                                             unsigned long long *kout, uint32_t *cout,
+                                                             unsigned long long *cursor) {
     const uint64_t n = *n_ptr;
-    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < n; i0 += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t i = i0 + threadIdx.x;
-        const bool keep = i < n && vals[i] > x;
-        const unsigned long long pos = wave_append(keep, cursor);
-        if (keep) {
+    block_compact(
+        n, cursor, [&](uint64_t i) { return vals[i] > x; },
+        [&](uint64_t i, unsigned long long pos) {
             kout[pos] = keys[i];
             cout[pos] = vals[i];
-        }
-    }
+        });
 }
 
-// fof with more than 32 datasets (count_dataset_presence_counts_u64,
-// src/lib.rs:637-703, keeps u32 presence counts for any number of
-// datasets): each group of 32 datasets is counted as its own pass
-// (presence = popcount of the group's 32-bit masks, every k-mer present in
-// the group kept); the per-group (k-mer, presence) pairs accumulate in HBM,
-// are sorted by k-mer and summed per k-mer, and the sums > x are emitted in
-// encoded order.
-static void count_dataset_groups(Batch &b, McPlan &p, const mczi_counter_config &cfg, int launch, Timer &tm,
-                                 unsigned long long *h, uint64_t &n_out, uint64_t &n_distinct) {
-    Device &d = Device::get();
-    hipStream_t s = d.stream();
-    mczi_counter_config gcfg = cfg;
-    gcfg.threshold = 0;  // keep every k-mer present in the group
-    const uint32_t groups = (b.n_datasets + 31) / 32;
-    DevBuf acc_k, acc_c;
-    uint64_t used = 0, cap = 0;
-    unsigned long long *cur = nullptr;
-    for (uint32_t gi = 0; gi < groups; gi++) {
-        p.ds_lo = gi * 32;
-        count_pass(b, p, gcfg, MCZI_MODE_DATASETS, launch, tm, false, h);
-        cur = p.ctr.as<unsigned long long>() + 14;  // allocated by the pass
-        const uint64_t n_slots = h[1], n_real = h[1] - h[6];
-        if (used + n_real > cap) {  // grow, keeping what is there
-            const uint64_t nc = std::max<uint64_t>(2 * cap, used + n_real) + 1024;
-            DevBuf nk(nc * 8), nv(nc * 4);
-            if (used) {
-                MCZI_HIP(hipMemcpyAsync(nk.as<void>(), acc_k.as<void>(), used * 8, hipMemcpyDeviceToDevice, s));
-                MCZI_HIP(hipMemcpyAsync(nv.as<void>(), acc_c.as<void>(), used * 4, hipMemcpyDeviceToDevice, s));
-            }
-            acc_k = std::move(nk);
-            acc_c = std::move(nv);
-            cap = nc;
-        }
-        MCZI_HIP(hipMemcpyAsync(cur, &used, 8, hipMemcpyHostToDevice, s));
-        if (n_slots)
-            hipLaunchKernelGGL(k_compact_out, dim3(grid_for(n_slots, 256, 8192)), dim3(256), 0, s,
-                               p.out_kmer.as<unsigned long long>(), nullptr, p.out_count.as<uint32_t>(),
-                               p.out_part.as<uint32_t>(), n_slots, acc_k.as<unsigned long long>(), nullptr,
-                               acc_c.as<uint32_t>(), cur);
-        used += n_real;
-        MCZI_HIP(hipStreamSynchronize(s));
-    }
-    p.ds_lo = 0;
-    p.graph_sig = 0;  // the captured step belongs to the last group
+// Sorts the accumulated (k-mer, value) pairs by k-mer, sums the values of
+// each k-mer, and writes the sums > x to the batch's output in encoded
+// order; n_distinct = the distinct k-mers. cur: a device counter.
+static void merge_kmer_values(Batch &b, DevBuf &acc_k, DevBuf &acc_c, uint64_t used, const mczi_counter_config &cfg,
+                              unsigned long long *cur, hipStream_t s, uint64_t &n_out, uint64_t &n_distinct) {
     // sort the pairs by k-mer, sum per k-mer, keep sums > x
     const unsigned end_bit = (unsigned)std::min<uint32_t>(64, 2 * cfg.k);
     DevBuf sk(std::max<uint64_t>(used, 1) * 8), sv(std::max<uint64_t>(used, 1) * 4);
@@ -3099,6 +3138,170 @@ static void count_dataset_groups(Batch &b, McPlan &p, const mczi_counter_config 
                                        b.out_kmer.as<unsigned long long>(), sv.as<uint32_t>(),
                                        b.out_count.as<uint32_t>(), (size_t)n_out, 0, end_bit, s));
     MCZI_HIP(hipStreamSynchronize(s));
+}
+
+// fof with more than 32 datasets (count_dataset_presence_counts_u64,
+// src/lib.rs:637-703, keeps u32 presence counts for any number of
+// datasets): each group of 32 datasets is counted as its own pass
+// (presence = popcount of the group's 32-bit masks, every k-mer present in
+// the group kept); the per-group (k-mer, presence) pairs accumulate in HBM,
+// are sorted by k-mer and summed per k-mer, and the sums > x are emitted in
+// encoded order.
+static void count_dataset_groups(Batch &b, McPlan &p, const mczi_counter_config &cfg, int launch, Timer &tm,
+                                 unsigned long long *h, uint64_t &n_out, uint64_t &n_distinct) {
+    Device &d = Device::get();
+    hipStream_t s = d.stream();
+    mczi_counter_config gcfg = cfg;
+    gcfg.threshold = 0;  // keep every k-mer present in the group
+    const uint32_t groups = (b.n_datasets + 31) / 32;
+    DevBuf acc_k, acc_c;
+    uint64_t used = 0, cap = 0;
+    unsigned long long *cur = nullptr;
+    for (uint32_t gi = 0; gi < groups; gi++) {
+        p.ds_lo = gi * 32;
+        count_pass(b, p, gcfg, MCZI_MODE_DATASETS, launch, tm, false, h);
+        cur = p.ctr.as<unsigned long long>() + 14;  // allocated by the pass
+        const uint64_t n_slots = h[1], n_real = h[1] - h[6];
+        if (used + n_real > cap) {  // grow, keeping what is there
+            const uint64_t nc = std::max<uint64_t>(2 * cap, used + n_real) + 1024;
+            DevBuf nk(nc * 8), nv(nc * 4);
+            if (used) {
+                MCZI_HIP(hipMemcpyAsync(nk.as<void>(), acc_k.as<void>(), used * 8, hipMemcpyDeviceToDevice, s));
+                MCZI_HIP(hipMemcpyAsync(nv.as<void>(), acc_c.as<void>(), used * 4, hipMemcpyDeviceToDevice, s));
+            }
+            acc_k = std::move(nk);
+            acc_c = std::move(nv);
+            cap = nc;
+        }
+        fill_u64(cur, 1, used, s);
+        if (n_slots)
+            hipLaunchKernelGGL(k_compact_out, dim3(grid_for(n_slots, 256, 8192)), dim3(256), 0, s,
+                               p.out_kmer.as<unsigned long long>(), nullptr, p.out_count.as<uint32_t>(),
+                               p.out_part.as<uint32_t>(), n_slots, acc_k.as<unsigned long long>(), nullptr,
+                               acc_c.as<uint32_t>(), cur);
+        used += n_real;
+        MCZI_HIP(hipStreamSynchronize(s));
+    }
+    p.ds_lo = 0;
+    p.graph_sig = 0;  // the captured step belongs to the last group
+    merge_kmer_values(b, acc_k, acc_c, used, cfg, cur, s, n_out, n_distinct);
+}
+
+// Windows of the fragments summarised by one input's length histogram.
+static uint64_t windows_of(const std::array<uint64_t, 66> &h, uint32_t k) {
+    uint64_t n = h[65] - h[64] * (k - 1);
+    for (uint32_t L = k; L < 64; L++) n += h[L] * (L - k + 1);
+    return n;
+}
+
+// The plain launch over several inputs counts each input (file) as a job of
+// its own, as one call per file would: only its tiles are scanned, then
+// partitioned and counted with every k-mer kept (threshold 0), and its
+// (k-mer, count) pairs -- fof, an input of one dataset: (k-mer, 1) --
+// accumulate in HBM. The pairs of all inputs are then merged by k-mer
+// (sorted, summed: the count over the inputs, or the number of datasets
+// holding the k-mer, as the reference's per-dataset passes give it,
+// src/lib.rs:637-703, 2227-2247) and the sums > x emitted in encoded order.
+// The result equals the batched count's; the batched launch does not merge.
+static void count_per_input(Batch &b, McPlan &p, const mczi_counter_config &cfg, int mode, Timer &tm,
+                            unsigned long long *h, uint64_t &n_out, uint64_t &n_distinct) {
+    Device &d = Device::get();
+    hipStream_t s = d.stream();
+    mczi_counter_config icfg = cfg;
+    icfg.threshold = 0;
+    const uint64_t all_windows = p.n_windows;
+    DevBuf acc_k, acc_c;
+    uint64_t used = 0, cap = 0;
+    unsigned long long *cur = nullptr;
+    size_t c = 0;
+    uint64_t win_done = 0;
+    unsigned long long hs[16] = {};
+    const bool trace = getenv("MCZI_PLAIN_TRACE") != nullptr;  // per-input pass and merge times on stderr
+    // An input alone repeats its k-mers less than the whole batch, which the
+    // tuned table fill was chosen for: its partitions hold at most one k-mer
+    // instance per two LDS slots (so the table stays at most half full even
+    // when every k-mer is distinct), and the batch's tuned fill is left as
+    // it is.
+    struct Restore {  // the plan's batch-wide state, also when a pass throws
+        McPlan &p;
+        const uint64_t windows;
+        const double override_, effective;
+        ~Restore() {
+            p.target_override = override_;
+            p.effective_target = effective;
+            p.tile_lo = 0;
+            p.tile_hi = ~0ull;
+            p.n_windows = windows;
+            p.rec_sig = 0;
+            p.ds_lo = 0;
+            p.graph_sig = 0;
+        }
+    } restore{p, all_windows, p.target_override, p.effective_target};
+    {
+        const double t = p.target_override > 0 ? p.target_override
+                         : p.cache_target > 0  ? p.cache_target
+                                               : tuned_target(p);
+        p.target_override = std::min(t, 0.5 * (double)(1u << p.cnt_ls));
+    }
+    for (size_t i = 0; i + 1 < b.input_frag.size(); i++) {
+        const uint64_t f0 = b.input_frag[i], f1 = b.input_frag[i + 1];
+        // every input starts a chunk: its tiles are those of chunks [c, ce)
+        while (c + 1 < b.chunk_frag.size() && b.chunk_frag[c] < f0) c++;
+        size_t ce = c;
+        while (ce + 1 < b.chunk_frag.size() && b.chunk_frag[ce] < f1) ce++;
+        require(b.chunk_frag[c] == f0 && b.chunk_frag[ce] == f1, MCZI_E_DEVICE, "an input does not start a chunk");
+        const uint64_t win = windows_of(b.input_hist[i], cfg.k);
+        if (win == 0) continue;
+        p.tile_lo = p.chunk_tile[c];
+        p.tile_hi = p.chunk_tile[ce];
+        p.n_windows = win;
+        p.rec_sig = 0;  // staging capacity for this input's size
+        p.ds_lo = mode == MCZI_MODE_DATASETS ? (b.input_ds[i] & ~31u) : 0u;
+        const auto ti = std::chrono::steady_clock::now();
+        count_pass(b, p, icfg, mode, MCZI_LAUNCH_PLAIN, tm, false, h);
+        if (trace)
+            fprintf(stderr, "MC_PLAIN_TRACE input %zu windows %llu pass_ms %.2f out %llu\n", i,
+                    (unsigned long long)win,
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ti).count(),
+                    (unsigned long long)(h[1] - h[6]));
+        cur = p.ctr.as<unsigned long long>() + 14;
+        const uint64_t n_slots = h[1], n_real = h[1] - h[6];
+        win_done += win;
+        if (used + n_real > cap) {  // grow, keeping what is there: room for the
+            // inputs still to come at the rate of those counted so far
+            const double proj = 1.1 * (double)(used + n_real) * (double)all_windows / (double)win_done;
+            const uint64_t nc = std::max<uint64_t>(std::max<uint64_t>(2 * cap, used + n_real), (uint64_t)proj) + 1024;
+            DevBuf nk(nc * 8), nv(nc * 4);
+            if (used) {
+                MCZI_HIP(hipMemcpyAsync(nk.as<void>(), acc_k.as<void>(), used * 8, hipMemcpyDeviceToDevice, s));
+                MCZI_HIP(hipMemcpyAsync(nv.as<void>(), acc_c.as<void>(), used * 4, hipMemcpyDeviceToDevice, s));
+                MCZI_HIP(hipStreamSynchronize(s));
+            }
+            acc_k = std::move(nk);
+            acc_c = std::move(nv);
+            cap = nc;
+        }
+        fill_u64(cur, 1, used, s);
+        if (n_slots)
+            hipLaunchKernelGGL(k_compact_out, dim3(grid_for(n_slots, 256, 8192)), dim3(256), 0, s,
+                               p.out_kmer.as<unsigned long long>(), nullptr, p.out_count.as<uint32_t>(),
+                               p.out_part.as<uint32_t>(), n_slots, acc_k.as<unsigned long long>(), nullptr,
+                               acc_c.as<uint32_t>(), cur);
+        used += n_real;
+        for (int j : {8, 11, 12, 13}) hs[j] += h[j];
+        c = ce;
+    }
+    MCZI_HIP(hipStreamSynchronize(s));
+    if (!cur) {  // no input has a window
+        p.ctr.ensure(16 * 8);
+        cur = p.ctr.as<unsigned long long>() + 14;
+    }
+    const auto tm0 = std::chrono::steady_clock::now();
+    merge_kmer_values(b, acc_k, acc_c, used, cfg, cur, s, n_out, n_distinct);
+    if (trace)
+        fprintf(stderr, "MC_PLAIN_TRACE merge pairs %llu ms %.2f\n", (unsigned long long)used,
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tm0).count());
+    for (int j = 0; j < 16; j++) h[j] = hs[j];
 }
 
 // Pass 1 on the device (count_input_minimizers, src/lib.rs:1726-1797; fof:
@@ -3220,10 +3423,20 @@ void batch_count(Batch &b, const mczi_counter_config &cfg, int mode, int launch,
     unsigned long long h[16];
     mczi_run_stats st{};
     b.k = k;
-    if (mode == MCZI_MODE_DATASETS && b.n_datasets > 32) {
+    // the plain launch over several inputs: each input counted on its own,
+    // then merged (encoded order, k <= 32; a k-mer's count over the inputs
+    // fits the 32-bit sums when the batch has fewer than 2^32 windows; fof:
+    // every input holds one dataset)
+    bool per_input = launch == MCZI_LAUNCH_PLAIN && order == MCZI_ORDER_ENCODED && !wide &&
+                     b.input_frag.size() > 2 && (mode == MCZI_MODE_DATASETS || p.n_windows < (1ull << 32));
+    if (per_input && mode == MCZI_MODE_DATASETS)
+        for (uint32_t dsi : b.input_ds) per_input = per_input && dsi != Batch::NO_DATASET;
+    if (per_input || (mode == MCZI_MODE_DATASETS && b.n_datasets > 32)) {
         auto t0 = std::chrono::steady_clock::now();
         uint64_t n_out = 0, n_distinct = 0;
-        count_dataset_groups(b, p, cfg, launch, tm, h, n_out, n_distinct);
+        if (per_input) count_per_input(b, p, cfg, mode, tm, h, n_out, n_distinct);
+        else count_dataset_groups(b, p, cfg, launch, tm, h, n_out, n_distinct);
+        b.carries.clear();  // the merged sums are whole counts
         b.n_out = n_out;
         st.n_bases = b.n_bases;
         st.n_windows = p.n_windows;
