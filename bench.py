@@ -496,9 +496,25 @@ def main() -> int:
             "host": host_info(), "device": device,
             "copy_probe": {"bytes": copy_bytes, "ms": copy_ms,
                            "read_plus_write_GBs": 2 * copy_bytes / (copy_ms * 1e-3) / 1e9}}
+    # The ZI stream runs first, in the fresh process: its served path is bound by
+    # device-to-host copies into the pinned memory the server takes when it is
+    # created, and measured 15-20% slower when it followed the MC workloads.
+    if args.only is None or args.only == "zi":
+        line["zi"] = {}
+        for k in (31, 63):
+            line["zi"][f"k{k}"] = zi_stream(M, k, args.zi_requests, args.zi_pool_reads, cpu=cpu)
+            print(f"[bench] zi k{k}: {json.dumps({x: line['zi'][f'k{k}'][x] for x in ('served_lookups_per_s', 'served_vs_plain', 'p99_ms')})}",
+                  file=sys.stderr, flush=True)
+        if line["value"] is None:
+            z = line["zi"]["k31"]
+            line["value"], line["unit"] = z["served_lookups_per_s"], "lookups/s"
+            line["config"] = {"workload": "zi_served_100mbp_index_k31", "requests": z["requests"],
+                              "reads": z["reads"]}
+            line["roofline"] = z["roofline"]
+            line["cpu_baseline"] = z["cpu_baseline"]
     if args.only != "zi":
         head = mc_workload(M, args.workload, args.steps, args.warmup, cpu, not args.no_plain)
-        line["value"] = head["value"]
+        line["value"], line["unit"] = head["value"], "bases/s"
         line["ms_per_step"] = head["ms_per_step"]
         line["config"] = {"workload": head["workload"], "k": head["k"], "m": head["m"], "x": head["x"],
                           "partition_count": head["partition_count"], "bases": head["bases"],
@@ -515,19 +531,6 @@ def main() -> int:
                 if name == args.workload:
                     continue
                 line["workloads"][name] = mc_workload(M, name, args.steps, args.warmup, cpu, not args.no_plain)
-    if args.only is None or args.only == "zi":
-        line["zi"] = {}
-        for k in (31, 63):
-            line["zi"][f"k{k}"] = zi_stream(M, k, args.zi_requests, args.zi_pool_reads, cpu=cpu)
-            print(f"[bench] zi k{k}: {json.dumps({x: line['zi'][f'k{k}'][x] for x in ('served_lookups_per_s', 'served_vs_plain', 'p99_ms')})}",
-                  file=sys.stderr, flush=True)
-        if line["value"] is None:
-            z = line["zi"]["k31"]
-            line["value"], line["unit"] = z["served_lookups_per_s"], "lookups/s"
-            line["config"] = {"workload": "zi_served_100mbp_index_k31", "requests": z["requests"],
-                              "reads": z["reads"]}
-            line["roofline"] = z["roofline"]
-            line["cpu_baseline"] = z["cpu_baseline"]
     print(json.dumps(line), flush=True)
     return 0
 

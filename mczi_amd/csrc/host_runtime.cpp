@@ -51,6 +51,7 @@ void *Device::alloc(size_t bytes) {
         if (it != free_.end() && it->first <= want * 2) {
             void *p = it->second;
             live_[p] = it->first;
+            pooled_ -= it->first;
             free_.erase(it);
             return p;
         }
@@ -60,10 +61,25 @@ void *Device::alloc(size_t bytes) {
     if (e != hipSuccess) {
         (void)hipGetLastError();
         std::lock_guard<std::mutex> lk(mu_);
-        for (auto &kv : free_) (void)hipFree(kv.second);
-        free_.clear();
+        // free cached blocks, largest first, until the request fits; then all
+        size_t freed = 0;
+        while (!free_.empty() && freed < want) {
+            auto last = std::prev(free_.end());
+            freed += last->first;
+            pooled_ -= last->first;
+            (void)hipFree(last->second);
+            free_.erase(last);
+        }
         p = nullptr;
         e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            for (auto &kv : free_) (void)hipFree(kv.second);
+            free_.clear();
+            pooled_ = 0;
+            p = nullptr;
+            e = hipMalloc(&p, want);
+        }
         if (e != hipSuccess) {
             (void)hipGetLastError();
             throw Error(MCZI_E_NOMEM, "device allocation of " + std::to_string(want) +
@@ -79,15 +95,19 @@ void Device::release(void *p) {
     std::lock_guard<std::mutex> lk(mu_);
     auto it = live_.find(p);
     if (it == live_.end()) return;
-    free_.emplace(it->second, p);
+    const size_t n = it->second;
     live_.erase(it);
+    if (pooled_ + n > kDeviceCacheBytes) {  // the cache is full: give the block back
+        (void)hipFree(p);
+        return;
+    }
+    free_.emplace(n, p);
+    pooled_ += n;
 }
 
 size_t Device::pooled_bytes() {
     std::lock_guard<std::mutex> lk(mu_);
-    size_t t = 0;
-    for (const auto &kv : free_) t += kv.first;
-    return t;
+    return pooled_;
 }
 
 bool Device::add_slab_locked() {

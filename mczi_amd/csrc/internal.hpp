@@ -89,6 +89,10 @@ class Device {
     int cu_count() const { return cus_; }
     int device() const { return dev_; }
     // Cached device allocation (rounded to 2 MiB); contents undefined.
+    // Released blocks stay cached up to kDeviceCacheBytes (beyond that they
+    // are freed, so that one large job does not keep the device's memory
+    // from the next); a failed allocation frees cached blocks, largest
+    // first, and tries again.
     void *alloc(size_t bytes);
     void release(void *p);
     size_t pooled_bytes();  // released device blocks kept for reuse
@@ -129,6 +133,8 @@ class Device {
     std::mutex call_mu_;
     std::multimap<size_t, void *> free_;
     std::map<void *, size_t> live_;
+    size_t pooled_ = 0;  // bytes in free_
+    static constexpr size_t kDeviceCacheBytes = 128ull << 30;
     static constexpr size_t kPinnedCacheBytes = 16ull << 30;
     static constexpr size_t kMinBlock = 64u << 10;
     static constexpr int kOrders = 15;  // 64 KiB << 14 = 1 GiB

@@ -1212,9 +1212,10 @@ void mczi_zi_server::done_loop() {
             for (size_t i = 0; i < batch.size() && !err; i++) nb += batch[i]->absent.n * (idx.wide ? 16 : 8);
             fprintf(stderr,
                     "ZI_TRACE t=%.2f done reqs=%zu wait_ms=%.2f sort_ms=%.2f copy_ms=%.2f alloc_ms=%.2f post_ms=%.2f "
-                    "copy_bytes=%llu pinned_gib=%.1f\n",
+                    "copy_bytes=%llu pinned_gib=%.1f pooled_gib=%.1f\n",
                     now_ms(), batch.size(), wait_ms, sort_ms, t1 - t0, res.alloc_ms, now_ms() - t1,
-                    (unsigned long long)nb, Device::get().host_slab_bytes() / double(1ull << 30));
+                    (unsigned long long)nb, Device::get().host_slab_bytes() / double(1ull << 30),
+                    Device::get().pooled_bytes() / double(1ull << 30));
         }
         lk.lock();
         busy_ms += t1 - t0 + sort_ms;
