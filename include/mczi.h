@@ -253,7 +253,11 @@ enum {
 /* Launch structure. */
 enum {
     MCZI_LAUNCH_BATCHED = 0, /* one persistent launch per pass over the whole buffer (graph) */
-    MCZI_LAUNCH_PLAIN = 1    /* one launch per input chunk (per file / 128 Mi bases) */
+    MCZI_LAUNCH_PLAIN = 1    /* per-file path: with several inputs (files / input marks) and the
+                                encoded order (k <= 32; fof: one dataset per input), each input
+                                is counted on its own and the per-file counts are merged on the
+                                device; otherwise one scan launch per input chunk (per file /
+                                128 Mi bases) and one count. Same output as BATCHED. */
 };
 /* Output order. */
 enum {
