@@ -387,6 +387,58 @@ def test_fof_beyond_32_datasets(n_datasets):
         assert st.n_out == len(want)
 
 
+@pytest.mark.parametrize("n_datasets", [8, 40])
+def test_fof_per_file_plain_equals_batched(n_datasets):
+    """The plain launch over one input per dataset (the per-file path) counts
+    each input on its own and merges the per-file presence on the device:
+    equal to the batched count and to brute force, in both fof modes; inputs
+    that mix datasets keep the single-count plain launch, also equal."""
+    print(f"fof per-file datasets={n_datasets}", flush=True)
+    r = M.SynthReads(M.SynthSpec(genome_len=40_000, genome_seed=4, read_seed=505, n_reads=60, read_len=150,
+                                 sub_rate=0.005, n_datasets=n_datasets, snp_rate=0.002))
+    n = len(r.frag_off) - 1
+    assert n == 60 * n_datasets and np.array_equal(r.dataset[::60], np.arange(n_datasets))
+    marks = np.arange(0, n + 1, 60, dtype=np.uint64)
+    b = M.Batch.from_packed(r.words, r.frag_off, r.dataset, input_frag=marks)
+    codes = r.codes()
+    for x in (0, 3, n_datasets - 2):
+        want = O.count_datasets_bruteforce(codes, r.frag_off, r.dataset, n_datasets, 31, x)
+        cfg = M.CounterConfig(31, 21, x, 64)
+        for mode in (M.MODE_DATASETS, M.MODE_DATASETS_TWO_PASS):
+            for launch in (M.LAUNCH_PLAIN, M.LAUNCH_BATCHED):
+                st = b.count(cfg, mode, launch)
+                assert same_counted(b.download(), want), (x, mode, launch)
+                assert st.n_out == len(want)
+    mixed = np.array([0, 90, n], dtype=np.uint64)
+    b2 = M.Batch.from_packed(r.words, r.frag_off, r.dataset, input_frag=mixed)
+    want = O.count_datasets_bruteforce(codes, r.frag_off, r.dataset, n_datasets, 31, 3)
+    b2.count(M.CounterConfig(31, 21, 3, 64), M.MODE_DATASETS, M.LAUNCH_PLAIN)
+    assert same_counted(b2.download(), want)
+
+
+def test_counts_per_file_plain_thresholds():
+    """The per-file plain path in the counting modes: three inputs (two
+    genomes, the second sampled twice), thresholds 0 and 2, single and
+    two-pass: the merged per-file counts equal brute force over all inputs."""
+    a = M.SynthReads(M.SynthSpec(genome_len=30_000, genome_seed=21, read_seed=121, n_reads=2000, read_len=150,
+                                 sub_rate=0.01))
+    c = M.SynthReads(M.SynthSpec(genome_len=20_000, genome_seed=22, read_seed=122, n_reads=3000, read_len=150,
+                                 sub_rate=0.005))
+    words, off = _concat(a, c)
+    n = len(off) - 1
+    n_a = n - (len(c.frag_off) - 1)
+    marks = np.array([0, n_a, n_a + 1500, n], dtype=np.uint64)
+    b = M.Batch.from_packed(words, off, input_frag=marks)
+    codes = np.concatenate([a.codes(), np.zeros(int(off[n_a]) - int(a.frag_off[-1]), dtype=np.uint8), c.codes()])
+    for x in (0, 2):
+        want = O.count_bruteforce(codes, off, 31, x)
+        cfg = M.CounterConfig(31, 21, x, 64)
+        for mode in (M.MODE_SINGLE_PASS, M.MODE_TWO_PASS):
+            st = b.count(cfg, mode, M.LAUNCH_PLAIN)
+            assert same_counted(b.download(), want), (x, mode)
+            assert st.n_out == len(want)
+
+
 def test_fof_file_path_40_datasets(tmp_path):
     rng = np.random.default_rng(40)
     base = "".join(rng.choice(list("ACGT"), size=3000))
