@@ -1125,6 +1125,257 @@ int or_count_datasets(const or_frags *f, size_t D, int k, int m, uint64_t x, siz
     return 0;
 }
 
+/* The same dataset-presence count on `threads` host threads, as the
+ * reference runs it under Rayon: datasets in order (the accept-new and
+ * finish steps depend on it, src/lib.rs:659-665), the reads of a dataset in
+ * parallel. Phase 1: each thread computes the run keys of its share of the
+ * dataset's fragments, split by shard; thread t then applies the keys of
+ * shards s = t (mod T) (SEEN-guarded, so the order of application does not
+ * change a state) and finishes those shards. Phase 2: each thread collects
+ * the k-mers of retained runs of its fragments into T buckets by
+ * spread_hash_u64(k-mer) (the reference's partitions, src/lib.rs:2931-2933);
+ * thread t sorts and deduplicates bucket t of the dataset and adds 1 to its
+ * own presence map. Same output as or_count_datasets. */
+typedef struct ds_job_s {
+    const or_frags *f;
+    int k, m;
+    uint64_t x;
+    size_t P;
+    int t, T;
+    const size_t *frag;   /* the dataset's fragments (length >= k) */
+    size_t lo, hi;        /* this thread's share of them */
+    int accept_new;
+    size_t remaining;
+    dstab *shards;
+    struct ds_job_s *all;
+    uint32_t **keys;      /* phase 1: per shard group u = shard % T */
+    size_t *nkeys, *capkeys;
+    uint64_t **bk;        /* phase 2: per bucket */
+    size_t *bn, *bcap;
+    map128 pres;          /* phase 2: presence of the k-mers of bucket t */
+} ds_job;
+
+static void push_u32(uint32_t **a, size_t *n, size_t *cap, uint32_t v) {
+    if (*n == *cap) {
+        *cap = *cap ? *cap * 2 : 1024;
+        *a = (uint32_t *)xrealloc(*a, *cap * sizeof(uint32_t));
+    }
+    (*a)[(*n)++] = v;
+}
+static void push_u64(uint64_t **a, size_t *n, size_t *cap, uint64_t v) {
+    if (*n == *cap) {
+        *cap = *cap ? *cap * 2 : 1024;
+        *a = (uint64_t *)xrealloc(*a, *cap * sizeof(uint64_t));
+    }
+    (*a)[(*n)++] = v;
+}
+
+static void *ds_runs(void *arg) {
+    ds_job *j = (ds_job *)arg;
+    size_t mx = max_frag_len(j->f) + 1;
+    run_t *runs = (run_t *)xmalloc(mx * sizeof(run_t));
+    uint32_t *th = (uint32_t *)xmalloc(mx * sizeof(uint32_t)), *ts = (uint32_t *)xmalloc(mx * sizeof(uint32_t));
+    for (int u = 0; u < j->T; u++) j->nkeys[u] = 0;
+    for (size_t q = j->lo; q < j->hi; q++) {
+        size_t i = j->frag[q];
+        const uint8_t *c = j->f->codes + j->f->off[i];
+        size_t n = j->f->off[i + 1] - j->f->off[i];
+        size_t r = fragment_runs(c, n, j->k, j->m, runs, th, ts);
+        for (size_t w = 0; w < r; w++) {
+            size_t u = (or_spread_hash_u32(runs[w].key) % j->P) % (size_t)j->T;
+            push_u32(&j->keys[u], &j->nkeys[u], &j->capkeys[u], runs[w].key);
+        }
+    }
+    free(runs);
+    free(th);
+    free(ts);
+    return NULL;
+}
+
+static void *ds_apply(void *arg) {
+    ds_job *j = (ds_job *)arg;
+    const ds_job *all = (const ds_job *)j->all;
+    for (int v = 0; v < j->T; v++)
+        for (size_t i = 0; i < all[v].nkeys[j->t]; i++) {
+            uint32_t key = all[v].keys[j->t][i];
+            dstab_add_seen(&j->shards[or_spread_hash_u32(key) % j->P], key, j->accept_new);
+        }
+    for (size_t s = (size_t)j->t; s < j->P; s += (size_t)j->T)
+        dstab_rebuild(&j->shards[s], 0, 1, j->remaining, j->x);
+    return NULL;
+}
+
+static void *ds_kmers(void *arg) {
+    ds_job *j = (ds_job *)arg;
+    size_t mx = max_frag_len(j->f) + 1;
+    run_t *runs = (run_t *)xmalloc(mx * sizeof(run_t));
+    uint32_t *th = (uint32_t *)xmalloc(mx * sizeof(uint32_t)), *ts = (uint32_t *)xmalloc(mx * sizeof(uint32_t));
+    u128 *cw = (u128 *)xmalloc(mx * sizeof(u128));
+    for (int u = 0; u < j->T; u++) j->bn[u] = 0;
+    for (size_t q = j->lo; q < j->hi; q++) {
+        size_t i = j->frag[q];
+        const uint8_t *c = j->f->codes + j->f->off[i];
+        size_t n = j->f->off[i + 1] - j->f->off[i];
+        size_t r = fragment_runs(c, n, j->k, j->m, runs, th, ts);
+        canon_windows(c, n, j->k, cw);
+        for (size_t w = 0; w < r; w++) {
+            if (!dstab_above(&j->shards[or_spread_hash_u32(runs[w].key) % j->P], runs[w].key, j->x)) continue;
+            for (size_t p = runs[w].start; p < runs[w].end; p++) {
+                uint64_t km = (uint64_t)cw[p];
+                size_t u = (size_t)(or_spread_hash_u64(km) % (uint64_t)j->T);
+                push_u64(&j->bk[u], &j->bn[u], &j->bcap[u], km);
+            }
+        }
+    }
+    free(runs);
+    free(th);
+    free(ts);
+    free(cw);
+    return NULL;
+}
+
+static void *ds_presence(void *arg) {
+    ds_job *j = (ds_job *)arg;
+    const ds_job *all = (const ds_job *)j->all;
+    size_t n = 0;
+    for (int v = 0; v < j->T; v++) n += all[v].bn[j->t];
+    if (!n) return NULL;
+    uint64_t *a = (uint64_t *)xmalloc(n * sizeof(uint64_t));
+    n = 0;
+    for (int v = 0; v < j->T; v++) {
+        memcpy(a + n, all[v].bk[j->t], all[v].bn[j->t] * sizeof(uint64_t));
+        n += all[v].bn[j->t];
+    }
+    qsort(a, n, sizeof(uint64_t), cmp_u64);
+    for (size_t i = 0; i < n; i++)
+        if (i == 0 || a[i] != a[i - 1]) (*map_slot(&j->pres, (u128)a[i], 1))++;
+    free(a);
+    return NULL;
+}
+
+int or_count_datasets_mt(const or_frags *f, size_t D, int k, int m, uint64_t x, size_t P, int threads,
+                         or_counted **out, size_t *n_out, uint64_t *kept) {
+    if (k < 1 || k > 32 || m < 1 || m > k || P == 0 || x >= COUNT_MASK || (k % 2) == 0 || D == 0 ||
+        !f->dataset || threads < 1)
+        return -1;
+    *out = NULL;
+    *n_out = 0;
+    if (kept) *kept = 0;
+    if (x >= D) return 0; /* src/lib.rs:644-648 */
+    const int T = threads;
+    /* the fragments (length >= k) of each dataset, in input order */
+    size_t *cnt = (size_t *)xcalloc(D + 1, sizeof(size_t));
+    for (size_t i = 0; i < f->nfrag; i++)
+        if (f->dataset[i] < D && f->off[i + 1] - f->off[i] >= (uint64_t)k) cnt[f->dataset[i] + 1]++;
+    for (size_t d = 0; d < D; d++) cnt[d + 1] += cnt[d];
+    size_t *frag = (size_t *)xmalloc((cnt[D] ? cnt[D] : 1) * sizeof(size_t));
+    size_t *fill = (size_t *)xmalloc((D ? D : 1) * sizeof(size_t));
+    memcpy(fill, cnt, D * sizeof(size_t));
+    for (size_t i = 0; i < f->nfrag; i++)
+        if (f->dataset[i] < D && f->off[i + 1] - f->off[i] >= (uint64_t)k) frag[fill[f->dataset[i]]++] = i;
+    free(fill);
+    dstab *shards = (dstab *)xmalloc(P * sizeof(dstab));
+    for (size_t s = 0; s < P; s++) dstab_init(&shards[s], 64);
+    ds_job *jobs = (ds_job *)xcalloc((size_t)T, sizeof(ds_job));
+    for (int t = 0; t < T; t++) {
+        ds_job *j = &jobs[t];
+        j->f = f;
+        j->k = k;
+        j->m = m;
+        j->x = x;
+        j->P = P;
+        j->t = t;
+        j->T = T;
+        j->shards = shards;
+        j->all = jobs;
+        j->keys = (uint32_t **)xcalloc((size_t)T, sizeof(uint32_t *));
+        j->nkeys = (size_t *)xcalloc((size_t)T, sizeof(size_t));
+        j->capkeys = (size_t *)xcalloc((size_t)T, sizeof(size_t));
+        j->bk = (uint64_t **)xcalloc((size_t)T, sizeof(uint64_t *));
+        j->bn = (size_t *)xcalloc((size_t)T, sizeof(size_t));
+        j->bcap = (size_t *)xcalloc((size_t)T, sizeof(size_t));
+        map_init(&j->pres, 1024);
+    }
+    /* a dataset's fragments split among the threads */
+    #define DS_SPLIT(d)                                                          \
+        for (int t = 0; t < T; t++) {                                            \
+            size_t nd = cnt[(d) + 1] - cnt[(d)];                                 \
+            jobs[t].frag = frag + cnt[(d)];                                      \
+            jobs[t].lo = nd * (size_t)t / (size_t)T;                             \
+            jobs[t].hi = nd * (size_t)(t + 1) / (size_t)T;                       \
+        }
+    for (size_t d = 0; d < D; d++) {
+        DS_SPLIT(d)
+        for (int t = 0; t < T; t++) {
+            jobs[t].remaining = D - d - 1;
+            jobs[t].accept_new = (1 + (uint64_t)jobs[t].remaining) > x; /* src/lib.rs:661 */
+        }
+        pthread_t *th = (pthread_t *)xmalloc(sizeof(pthread_t) * (size_t)T);
+        for (int t = 0; t < T; t++) pthread_create(&th[t], NULL, ds_runs, &jobs[t]);
+        for (int t = 0; t < T; t++) pthread_join(th[t], NULL);
+        for (int t = 0; t < T; t++) pthread_create(&th[t], NULL, ds_apply, &jobs[t]);
+        for (int t = 0; t < T; t++) pthread_join(th[t], NULL);
+        free(th);
+    }
+    uint64_t retained = 0;
+    for (size_t s = 0; s < P; s++) retained += shards[s].items;
+    if (kept) *kept = retained;
+    if (retained) {
+        for (size_t d = 0; d < D; d++) {
+            DS_SPLIT(d)
+            pthread_t *th = (pthread_t *)xmalloc(sizeof(pthread_t) * (size_t)T);
+            for (int t = 0; t < T; t++) pthread_create(&th[t], NULL, ds_kmers, &jobs[t]);
+            for (int t = 0; t < T; t++) pthread_join(th[t], NULL);
+            for (int t = 0; t < T; t++) pthread_create(&th[t], NULL, ds_presence, &jobs[t]);
+            for (int t = 0; t < T; t++) pthread_join(th[t], NULL);
+            free(th);
+        }
+    }
+    #undef DS_SPLIT
+    or_counted *res = NULL;
+    size_t r = 0, rcap = 0;
+    for (int t = 0; t < T; t++) {
+        const map128 *pm = &jobs[t].pres;
+        for (size_t i = 0; i < pm->cap; i++) {
+            if (pm->state[i] != 1 || !(pm->vals[i] > x)) continue;
+            if (r == rcap) {
+                rcap = rcap ? rcap * 2 : 1024;
+                res = (or_counted *)xrealloc(res, rcap * sizeof(or_counted));
+            }
+            res[r].lo = (uint64_t)pm->keys[i];
+            res[r].hi = 0;
+            res[r].count = pm->vals[i];
+            r++;
+        }
+    }
+    for (int t = 0; t < T; t++) {
+        ds_job *j = &jobs[t];
+        for (int u = 0; u < T; u++) {
+            free(j->keys[u]);
+            free(j->bk[u]);
+        }
+        free(j->keys);
+        free(j->nkeys);
+        free(j->capkeys);
+        free(j->bk);
+        free(j->bn);
+        free(j->bcap);
+        map_free(&j->pres);
+    }
+    free(jobs);
+    for (size_t s = 0; s < P; s++) {
+        free(shards[s].keys);
+        free(shards[s].states);
+    }
+    free(shards);
+    free(frag);
+    free(cnt);
+    if (r) qsort(res, r, sizeof(or_counted), cmp_counted);
+    *out = res;
+    *n_out = r;
+    return 0;
+}
+
 int or_count_datasets_bruteforce(const or_frags *f, size_t D, int k, uint64_t x,
                                  or_counted **out, size_t *n_out) {
     map128 pres;

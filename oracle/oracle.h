@@ -74,6 +74,11 @@ int or_kff_u8_kmc(const or_frags *f, int k, int m, uint64_t threshold,
 int or_count_datasets(const or_frags *f, size_t n_datasets, int k, int m,
                       uint64_t threshold, size_t partitions,
                       or_counted **out, size_t *n_out, uint64_t *minimizers_kept);
+/* The same on `threads` host threads (datasets in order, each dataset's
+ * reads in parallel, as the reference's Rayon path). Same output. */
+int or_count_datasets_mt(const or_frags *f, size_t n_datasets, int k, int m,
+                         uint64_t threshold, size_t partitions, int threads,
+                         or_counted **out, size_t *n_out, uint64_t *minimizers_kept);
 int or_count_datasets_bruteforce(const or_frags *f, size_t n_datasets, int k,
                                  uint64_t threshold, or_counted **out, size_t *n_out);
 

@@ -71,6 +71,9 @@ def lib() -> C.CDLL:
                                     C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t), C.POINTER(C.c_uint64)]
         L.or_count_datasets.argtypes = [C.POINTER(_Frags), C.c_size_t, C.c_int, C.c_int, C.c_uint64, C.c_size_t,
                                         C.POINTER(C.POINTER(_Counted)), C.POINTER(C.c_size_t), C.POINTER(C.c_uint64)]
+        L.or_count_datasets_mt.argtypes = [C.POINTER(_Frags), C.c_size_t, C.c_int, C.c_int, C.c_uint64,
+                                           C.c_size_t, C.c_int, C.POINTER(C.POINTER(_Counted)),
+                                           C.POINTER(C.c_size_t), C.POINTER(C.c_uint64)]
         L.or_count_datasets_bruteforce.argtypes = [C.POINTER(_Frags), C.c_size_t, C.c_int, C.c_uint64,
                                                    C.POINTER(C.POINTER(_Counted)), C.POINTER(C.c_size_t)]
         L.or_zi_absent.argtypes = [C.POINTER(_Frags), C.POINTER(_Frags), C.c_int,
@@ -226,6 +229,18 @@ def count_datasets(codes, off, ds, n_datasets, k, m, x, partitions):
     n = C.c_size_t()
     kept = C.c_uint64()
     rc = lib().or_count_datasets(C.byref(f), n_datasets, k, m, x, partitions, C.byref(p), C.byref(n), C.byref(kept))
+    if rc != 0:
+        raise ValueError("oracle rejected configuration")
+    return _take_counted(p, n.value), kept.value
+
+
+def count_datasets_mt(codes, off, ds, n_datasets, k, m, x, partitions, threads):
+    f, keep = _frags(codes, off, ds)
+    p = C.POINTER(_Counted)()
+    n = C.c_size_t()
+    kept = C.c_uint64()
+    rc = lib().or_count_datasets_mt(C.byref(f), n_datasets, k, m, x, partitions, threads, C.byref(p), C.byref(n),
+                                    C.byref(kept))
     if rc != 0:
         raise ValueError("oracle rejected configuration")
     return _take_counted(p, n.value), kept.value

@@ -270,8 +270,13 @@ def test_datasets_equal_bruteforce_random():
     codes, off, dsa = O.frags_of(seqs, datasets=ds)
     for x in (0, 2, 4, 6):
         bf = O.count_datasets_bruteforce(codes, off, dsa, 6, 21, x)
-        got, _ = O.count_datasets(codes, off, dsa, 6, 21, 11, x, 32)
+        got, kept = O.count_datasets(codes, off, dsa, 6, 21, 11, x, 32)
         assert np.array_equal(bf, got), x
+        # the multithreaded form (the fof CPU baseline): same k-mers, counts
+        # and retained minimizers at any thread count
+        for t in (1, 3, 16, 500):
+            got_mt, kept_mt = O.count_datasets_mt(codes, off, dsa, 6, 21, 11, x, 32, t)
+            assert np.array_equal(got_mt, got) and kept_mt == kept, (x, t)
 
 
 # ---------------------------------------------------------------------------
