@@ -1257,9 +1257,11 @@ __device__ __forceinline__ bool spill_table_full(const CountArgs &a) {
     return __hip_atomic_load(&a.ctr[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > (a.ovf_mask + 1) / 8;
 }
 
+// (kept out of line: the spill path is rare, and inlined into every probe
+// site it cost the count kernel registers and instruction cache)
 template <int MODE>
 __device__ void overflow_insert(const CountArgs &a, unsigned long long key, uint32_t val,
-                                uint32_t q) {
+                                                          uint32_t q) {
     if (spill_table_full(a)) {
         atomicOr(&a.ctr[4], 1ull);
         return;
@@ -1485,12 +1487,26 @@ __device__ __forceinline__ uint32_t rec_merge_slot(uint64_t x, uint64_t ykey) {
 // The next round's records (of this partition or the block's next one) are
 // loaded while the current round is counted.
 template <int MODE, int LS = CNT_LOG2_SLOTS>
-__global__ __launch_bounds__(CNT_THREADS_K) void k_count(CountArgs a) {
+// At most 80 VGPRs (six waves per SIMD): with 2^11 or 2^10 table slots three
+// or four blocks then fit a CU (LDS 50 / 38 KiB per block); 2^12 slots keep
+// two blocks (74 KiB).
+__global__ __launch_bounds__(CNT_THREADS_K) __attribute__((amdgpu_waves_per_eu(6))) void k_count(CountArgs a) {
     constexpr int SLOTS = 1 << LS;  // LDS table slots (the tuner picks 2^10 .. 2^12)
     constexpr int PER_T = CNT_ROUND / CNT_THREADS_K;
     __shared__ unsigned long long s_keys[SLOTS];
     __shared__ uint32_t s_vals[SLOTS];
-    __shared__ ulonglong2 s_raw[CNT_ROUND];     // the round's records
+    // the round's records, 12 bytes each: bases 0..31, then bases 32..43 and
+    // the base count (what merging compares and rolling reads; the partition
+    // is the block's, the dataset travels in s_agg)
+    __shared__ uint32_t s_raw[CNT_ROUND * 3];
+    auto raw_put = [&](int i, const ulonglong2 &r) {
+        s_raw[3 * i] = (uint32_t)r.x;
+        s_raw[3 * i + 1] = (uint32_t)(r.x >> 32);
+        s_raw[3 * i + 2] = (uint32_t)r.y;
+    };
+    auto raw_get = [&](int i) {
+        return make_ulonglong2((uint64_t)s_raw[3 * i] | ((uint64_t)s_raw[3 * i + 1] << 32), (uint64_t)s_raw[3 * i + 2]);
+    };
     __shared__ uint32_t s_dtab[CNT_DSLOTS];     // merge table: record index + 1, 0 = empty
     __shared__ uint32_t s_agg[CNT_ROUND];       // per representative: copies, or OR of dataset bits
     __shared__ uint16_t s_order[CNT_ROUND];     // representatives, most k-mers first
@@ -1552,7 +1568,7 @@ __global__ __launch_bounds__(CNT_THREADS_K) void k_count(CountArgs a) {
 #pragma unroll
             for (int i = 0; i < PER_T; i++) {
                 const int li = i * CNT_THREADS_K + tid;
-                if (li < nr) s_raw[li] = rr[i];
+                if (li < nr) raw_put(li, rr[i]);
             }
             __syncthreads();
             // merge equal records; representatives are bucketed by k-mer count
@@ -1570,7 +1586,7 @@ __global__ __launch_bounds__(CNT_THREADS_K) void k_count(CountArgs a) {
                     for (int p = 0; p < CNT_DSLOTS; p++) {
                         const uint32_t prev = atomicCAS(&s_dtab[s], 0u, (uint32_t)li + 1u);
                         if (prev == 0) break;
-                        const ulonglong2 o = s_raw[prev - 1];
+                        const ulonglong2 o = raw_get((int)prev - 1);
                         if (o.x == x && (o.y & REC_KEY_Y) == ykey) {
                             rep = (int)prev - 1;
                             break;
@@ -1625,7 +1641,7 @@ __global__ __launch_bounds__(CNT_THREADS_K) void k_count(CountArgs a) {
                 uint32_t val = 0;
                 if (li < nrep) {
                     const int idx = s_order[li];
-                    r = s_raw[idx];
+                    r = raw_get(idx);
                     val = s_agg[idx];
                     s_agg[idx] = 0;
                     n = max(rec_len(r) - k + 1, 0);
