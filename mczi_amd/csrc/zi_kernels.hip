@@ -6,6 +6,8 @@
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
+#include <unistd.h>
+
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -1432,11 +1434,13 @@ static int server_create(HostReads &&ir, uint32_t k, uint32_t m, uint64_t budget
         s->budget = budget ? budget : (8ull << 30);
         // pinned staging for request words and absent sets, taken once here:
         // pinning on the request path costs about 29 ms per GiB and stalls
-        // the copies into other pinned blocks (MCZI_ZI_HOST_ARENA_GIB, default 16)
+        // the copies into other pinned blocks (MCZI_ZI_HOST_ARENA_GIB; default
+        // 16 GiB, at most an eighth of the host's memory)
         {
             const char *e = getenv("MCZI_ZI_HOST_ARENA_GIB");
-            const uint64_t gib = e ? strtoull(e, nullptr, 10) : 16;
-            d.host_reserve(gib << 30);
+            const uint64_t host = (uint64_t)sysconf(_SC_PHYS_PAGES) * (uint64_t)sysconf(_SC_PAGESIZE);
+            const uint64_t bytes = e ? strtoull(e, nullptr, 10) << 30 : std::min<uint64_t>(16ull << 30, host / 8);
+            d.host_reserve(bytes);
         }
         s->max_inflight = std::max<uint32_t>(1, std::min<uint32_t>(max_inflight ? max_inflight : 2, 8));
         s->slots.resize(s->max_inflight);
