@@ -82,3 +82,29 @@ def test_group_passes_kff_order_and_datasets(reads, monkeypatch):
                          monkeypatch=monkeypatch)
         assert st.partition_groups == 3 and one.size > 100
         assert np.array_equal(got, one), nd
+
+
+@pytest.mark.parametrize("mode", [M.MODE_SINGLE_PASS, M.MODE_DATASETS])
+def test_per_file_plain_with_group_passes(reads, monkeypatch, mode):
+    """The plain launch's per-file path with every file counted in forced
+    group passes gives the one-pass batched result (counts; fof with one
+    dataset per input, 40 datasets)."""
+    n = len(reads.frag_off) - 1
+    ds = reads.dataset if mode == M.MODE_DATASETS else None
+    per = n // 40
+    assert per * 40 == n and np.array_equal(reads.dataset[::per], np.arange(40))
+    marks = np.arange(0, n + 1, per, dtype=np.uint64)
+    cfg = M.CounterConfig(k=31, minimizer=21, threshold=3)
+    one, _ = _count(reads, cfg, mode, datasets=ds, monkeypatch=monkeypatch)
+    for key in ("MCZI_GROUP_PASSES", "MCZI_DEVICE_BUDGET"):
+        monkeypatch.delenv(key, raising=False)
+    monkeypatch.setenv("MCZI_GROUP_PASSES", "3")
+    b = M.Batch.from_packed(reads.words, reads.frag_off, ds, input_frag=marks)
+    try:
+        st = b.count(cfg, mode=mode, launch=M.LAUNCH_PLAIN)
+        got = b.download()
+    finally:
+        b.close()
+        monkeypatch.delenv("MCZI_GROUP_PASSES", raising=False)
+    print(f"per-file plain mode={mode}: {got.size} k-mers", flush=True)
+    assert np.array_equal(got, one) and st.n_out == one.size
