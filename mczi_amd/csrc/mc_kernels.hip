@@ -872,17 +872,10 @@ template <class R> struct PartRec;
 template <> struct PartRec<ulonglong2> {
     static constexpr int ITEMS = 16;
     static constexpr uint32_t PAD = REC_PAD_Q;
-    // the partition of record i, from its high word only
-    __device__ static uint32_t q_at(const ulonglong2 *p, uint64_t i) {
-        return (uint32_t)(reinterpret_cast<const unsigned long long *>(p)[2 * i + 1] >> 35);
-    }
 };
 template <> struct PartRec<Rec32> {
     static constexpr int ITEMS = 8;
     static constexpr uint32_t PAD = REC_PAD_QW;
-    __device__ static uint32_t q_at(const Rec32 *p, uint64_t i) {
-        return (uint32_t)(reinterpret_cast<const unsigned long long *>(p)[4 * i + 3] >> 32);
-    }
 };
 
 struct PartArgs {
@@ -978,18 +971,38 @@ __device__ __forceinline__ void pt_move_tile(const R (&rec)[PartRec<R>::ITEMS], 
     for (int it = 0; it < IT; it++)
         if (dig[it] >= 0) s_stage[s_start[dig[it]] + rank[it]] = rec[it];
     __syncthreads();
-    for (uint32_t j = tid; j < total; j += PT_THREADS) {
-        const R r = s_stage[j];
-        const uint32_t q = rec_q(r);
-        const uint32_t d = (q >> shift) & dmask;
-        const uint64_t pos = (uint64_t)s_cur[d] + (j - s_start[d]);
-        if (pos < cap) {
-            dst[pos] = r;
-            if (dst_q) dst_q[pos] = q;
-        } else {
-            atomicOr(err, 16ull);
+    // The thread's slots j = it * PT_THREADS + tid of the ordered tile, in
+    // fixed phases (all record reads, then all digit offsets, then the
+    // stores) so that each phase's LDS reads are in flight together rather
+    // than one dependent chain per record.
+    R out[IT];
+    uint32_t oq[IT];
+    uint64_t pos[IT];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const uint32_t j = (uint32_t)(it * PT_THREADS + tid);
+        if (j < total) out[it] = s_stage[j];
+    }
+    bool over = false;
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const uint32_t j = (uint32_t)(it * PT_THREADS + tid);
+        pos[it] = ~0ull;
+        if (j < total) {
+            oq[it] = rec_q(out[it]);
+            const uint32_t d = (oq[it] >> shift) & dmask;
+            pos[it] = (uint64_t)s_cur[d] + (j - s_start[d]);
+            over |= pos[it] >= cap;
         }
     }
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        if (pos[it] < cap) {
+            dst[pos[it]] = out[it];
+            if (dst_q) dst_q[pos[it]] = oq[it];
+        }
+    }
+    if (over) atomicOr(err, 16ull);
     __syncthreads();
     for (uint32_t d = tid; d < nd; d += PT_THREADS) {
         s_cur[d] += s_cnt[d];
@@ -1009,19 +1022,16 @@ __device__ __forceinline__ void pt_move_range(const R *__restrict__ src, R *__re
                                               unsigned long long *err) {
     constexpr int IT = PartRec<R>::ITEMS;
     constexpr uint64_t TN = (uint64_t)PT_THREADS * IT;
+    if (b >= e) return;
+    // slots past e load slot e - 1 (pt_move_tile ignores them): unconditional
+    // loads keep the prefetch's wait counts exact
     R cur[IT], nxt[IT];
 #pragma unroll
-    for (int it = 0; it < IT; it++) {
-        const uint64_t i = b + (uint64_t)it * PT_THREADS + threadIdx.x;
-        if (i < e) cur[it] = src[i];
-    }
+    for (int it = 0; it < IT; it++) cur[it] = src[min(b + (uint64_t)it * PT_THREADS + threadIdx.x, e - 1)];
     for (uint64_t t = b; t < e; t += TN) {
         const uint64_t tn = t + TN;
 #pragma unroll
-        for (int it = 0; it < IT; it++) {
-            const uint64_t i = tn + (uint64_t)it * PT_THREADS + threadIdx.x;
-            if (i < e) nxt[it] = src[i];
-        }
+        for (int it = 0; it < IT; it++) nxt[it] = src[min(tn + (uint64_t)it * PT_THREADS + threadIdx.x, e - 1)];
         pt_move_tile<R>(cur, dst, dst_q, t, min(e, tn), shift, dmask, nd, cap, s_stage, s_cnt, s_start, s_cur,
                         s_wsum, err);
 #pragma unroll
@@ -1029,30 +1039,27 @@ __device__ __forceinline__ void pt_move_range(const R *__restrict__ src, R *__re
     }
 }
 
-// Digit counts of slots [b, e) into s_h (LDS atomics), padding skipped; the
-// next tile's partition words are loaded before the current tile is counted.
+// Digit counts of slots [b, e) into s_h (LDS atomics), padding skipped, from
+// the records' partition words qarr; the next tile's words are loaded before
+// the current tile is counted (unconditionally, slots past e reading slot
+// e - 1 and counted as padding, so that the loads stay in flight).
 template <class R>
-__device__ __forceinline__ void pt_count_range(const R *__restrict__ src, const uint32_t *__restrict__ qarr,
-                                               uint64_t b, uint64_t e, int shift, uint32_t dmask, uint32_t *s_h) {
+__device__ __forceinline__ void pt_count_range(const uint32_t *__restrict__ qarr, uint64_t b, uint64_t e, int shift,
+                                               uint32_t dmask, uint32_t *s_h) {
     constexpr int IT = PartRec<R>::ITEMS;
     constexpr uint64_t TN = (uint64_t)PT_THREADS * IT;
+    if (b >= e) return;
     uint32_t cur[IT], nxt[IT];
-    auto q_of = [&](uint64_t i) { return qarr ? qarr[i] : PartRec<R>::q_at(src, i); };
 #pragma unroll
-    for (int it = 0; it < IT; it++) {
-        const uint64_t i = b + (uint64_t)it * PT_THREADS + threadIdx.x;
-        cur[it] = i < e ? q_of(i) : PartRec<R>::PAD;
-    }
+    for (int it = 0; it < IT; it++) cur[it] = qarr[min(b + (uint64_t)it * PT_THREADS + threadIdx.x, e - 1)];
     for (uint64_t t = b; t < e; t += TN) {
         const uint64_t tn = t + TN;
 #pragma unroll
-        for (int it = 0; it < IT; it++) {
-            const uint64_t i = tn + (uint64_t)it * PT_THREADS + threadIdx.x;
-            nxt[it] = i < e ? q_of(i) : PartRec<R>::PAD;
-        }
+        for (int it = 0; it < IT; it++) nxt[it] = qarr[min(tn + (uint64_t)it * PT_THREADS + threadIdx.x, e - 1)];
 #pragma unroll
         for (int it = 0; it < IT; it++)
-            if (cur[it] != PartRec<R>::PAD) atomicAdd(&s_h[(cur[it] >> shift) & dmask], 1u);
+            if (t + (uint64_t)it * PT_THREADS + threadIdx.x < e && cur[it] != PartRec<R>::PAD)
+                atomicAdd(&s_h[(cur[it] >> shift) & dmask], 1u);
 #pragma unroll
         for (int it = 0; it < IT; it++) cur[it] = nxt[it];
     }
@@ -1063,7 +1070,7 @@ __device__ __forceinline__ void pt_count_range(const R *__restrict__ src, const 
 // counts to the zeroed hist.
 constexpr int PT_COUNT_SPLIT = 4;
 template <class R>
-__global__ __launch_bounds__(PT_THREADS) void k_part_count(const R *__restrict__ src, PartArgs a) {
+__global__ __launch_bounds__(PT_THREADS) void k_part_count(PartArgs a) {
     __shared__ uint32_t s_h[PT_MAX_DIGITS];
     constexpr uint64_t TN = (uint64_t)PT_THREADS * PartRec<R>::ITEMS;
     for (uint32_t d = threadIdx.x; d < a.nc; d += PT_THREADS) s_h[d] = 0;
@@ -1075,7 +1082,7 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_count(const R *__restrict__
     const uint64_t gb = min(n, (uint64_t)g * per), ge = min(n, gb + per);
     const uint64_t sub = (per / TN + PT_COUNT_SPLIT - 1) / PT_COUNT_SPLIT * TN;
     const uint64_t b = min(ge, gb + part * sub), e = min(ge, b + sub);
-    pt_count_range<R>(src, a.stage_q, b, e, a.fbits, 0xffffffffu, s_h);
+    pt_count_range<R>(a.stage_q, b, e, a.fbits, 0xffffffffu, s_h);
     __syncthreads();
     for (uint32_t d = threadIdx.x; d < a.nc; d += PT_THREADS)
         if (s_h[d]) atomicAdd(&a.hist[(uint64_t)d * a.nblk + g], s_h[d]);
@@ -1167,7 +1174,7 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_fine(const R *__restrict__ 
     const uint64_t bs = a.cbase[bkt], be = a.cbase[bkt + 1];
     for (uint32_t d = threadIdx.x; d < a.nf; d += PT_THREADS) s_cnt[d] = 0;
     __syncthreads();
-    pt_count_range<R>(src, a.coarse_q, bs, be, 0, fmask, s_cnt);
+    pt_count_range<R>(a.coarse_q, bs, be, 0, fmask, s_cnt);
     __syncthreads();
     pt_digit_scan(s_cnt, s_start, a.nf, s_wsum);
     for (uint32_t f = threadIdx.x; f < a.nf; f += PT_THREADS) {
@@ -2590,8 +2597,7 @@ static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg,
     const PartArgs pa = part_args(p);
     if (wide) {
         fill_u32(pa.hist, (uint64_t)pa.nc * pa.nblk, 0u, s);
-        hipLaunchKernelGGL(k_part_count<Rec32>, dim3(pa.nblk * PT_COUNT_SPLIT), dim3(PT_THREADS), 0, s,
-                           p.stage_rec_w.as<Rec32>(), pa);
+        hipLaunchKernelGGL(k_part_count<Rec32>, dim3(pa.nblk * PT_COUNT_SPLIT), dim3(PT_THREADS), 0, s, pa);
         hipLaunchKernelGGL(k_part_colscan, dim3(pa.nc), dim3(256), 0, s, pa);
         hipLaunchKernelGGL(k_part_bucketscan, dim3(1), dim3(1024), 0, s, pa);
         hipLaunchKernelGGL(k_part_coarse<Rec32>, dim3(pa.nblk), dim3(PT_THREADS), 0, s, p.stage_rec_w.as<Rec32>(),
@@ -2600,8 +2606,7 @@ static void enqueue_stage_b(Batch &b, McPlan &p, const mczi_counter_config &cfg,
                            p.stage_rec_w.as<Rec32>(), pa);
     } else {
         fill_u32(pa.hist, (uint64_t)pa.nc * pa.nblk, 0u, s);
-        hipLaunchKernelGGL(k_part_count<ulonglong2>, dim3(pa.nblk * PT_COUNT_SPLIT), dim3(PT_THREADS), 0, s,
-                           p.stage_rec.as<ulonglong2>(), pa);
+        hipLaunchKernelGGL(k_part_count<ulonglong2>, dim3(pa.nblk * PT_COUNT_SPLIT), dim3(PT_THREADS), 0, s, pa);
         hipLaunchKernelGGL(k_part_colscan, dim3(pa.nc), dim3(256), 0, s, pa);
         hipLaunchKernelGGL(k_part_bucketscan, dim3(1), dim3(1024), 0, s, pa);
         hipLaunchKernelGGL(k_part_coarse<ulonglong2>, dim3(pa.nblk), dim3(PT_THREADS), 0, s,
