@@ -415,33 +415,39 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
     }
     uint32_t pword = 0;
     uint64_t poff = 0;
-    uint32_t pds = 255u;  // the dataset (within the staged group) of fragment f0 + tid of the next tile
+    // the dataset of fragment f0 + tid of the next tile (ds_lo + 255: none);
+    // it is mapped into the staged group of 32 when the tile starts
+    uint32_t pds = a.ds_lo + 255u;
+    // (the loads are issued before any of their values is used, so that all
+    // stay in flight while the current tile is scanned)
     auto load_first = [&](uint64_t ts_, int tl_, uint64_t f0_, uint64_t f1_) {
-        const int nw = (((int)(ts_ & 15u) + tl_ + 160) >> 4) + 3;
-        pword = tid < nw ? a.words[(ts_ >> 4) + tid] : 0u;
         const uint64_t j = f0_ + (uint64_t)tid;
         poff = (j <= f1_ + 1 && j <= a.n_frag) ? a.off[j] : (~0ull >> 1);
-        pds = 255u;
-        if (MODE == STAGE && a.dataset != nullptr && j <= f1_ && j < a.n_frag) {
-            const uint32_t d = a.dataset[j] - a.ds_lo;
-            pds = d < 32u ? d : 255u;
-        }
+        pds = a.ds_lo + 255u;
+        if (MODE == STAGE && a.dataset != nullptr && j <= f1_ && j < a.n_frag) pds = a.dataset[j];
+        const int nw = (((int)(ts_ & 15u) + tl_ + 160) >> 4) + 3;
+        pword = tid < nw ? a.words[(ts_ >> 4) + tid] : 0u;
     };
     if (t < a.tile_end) load_first(ts, tl, f0, f1);
     uint64_t ts_n = 0, f0_n = 0, f1_n = 0;  // the next tile's entry
     int tl_n = 0;
     for (; t < a.tile_end; t += gridDim.x, ts = ts_n, tl = tl_n, f0 = f0_n, f1 = f1_n) {
         const uint64_t tn = t + gridDim.x;
-        if (tn < a.tile_end) {
-            ts_n = a.tile_start[tn];
-            tl_n = (int)a.tile_len[tn];
-            f0_n = a.tile_frag[tn];
-            f1_n = next_frag(tn);
-        }
+        // The next tile's entry, loaded unconditionally (clamped to the last
+        // tile; unused past it), so that no wait on these loads opens the
+        // tile: they are first used once the fragment bounds are in LDS.
+        const uint64_t tc = min(tn, a.tile_end - 1);
+        ts_n = a.tile_start[tc];
+        tl_n = (int)a.tile_len[tc];
+        f0_n = a.tile_frag[tc];
+        const uint32_t f1_next = a.tile_frag[min(tn + 1, a.tile_end - 1)];
         const int rb0 = (int)(ts & 15u);
         __syncthreads();  // previous tile fully consumed
         if (tid < WORDS_SPAN) s_words[tid] = pword;
-        if (MODE == STAGE) s_ds[tid] = (uint8_t)pds;
+        if (MODE == STAGE) {
+            const uint32_t d = pds - a.ds_lo;
+            s_ds[tid] = (uint8_t)(d < 32u ? d : 255u);
+        }
         // Fragment boundaries relative to ts: s_fs[j] = off[f0 + j] - ts (clamped),
         // for j = 0 .. nf, nf = number of fragments with a base in [ts, ts + tl).
         int nf = 0;
@@ -458,7 +464,9 @@ __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(4)
             if (c < SCAN_THREADS) break;
         }
         __syncthreads();
-        if (tn < a.tile_end) load_first(ts_n, tl_n, f0_n, f1_n);
+        // (after the block's last tile this prefetches the clamped entry, unused)
+        f1_n = tn + 1 < a.tile_end ? (uint64_t)f1_next : a.n_frag;
+        load_first(ts_n, tl_n, f0_n, f1_n);
 
         // m-mer hashes for relative positions i in [0, tl + w - 1). ntHash is
         // XOR-linear, so with the exclusive prefix XORs over the tile's bases
