@@ -229,47 +229,72 @@ __global__ __launch_bounds__(ZI_THREADS) void k_zi_lookup(ZiTileArgs a) {
         Key2 key[ZI_WPT];
         uint64_t slot[ZI_WPT];
         uint32_t req[ZI_WPT];
+        int fj[ZI_WPT];
         uint32_t pending = 0, absent = 0;
+        // The request of every window's fragment, all loads issued before any
+        // is used (windows past the tile read the tile's first fragment).
 #pragma unroll
         for (int i = 0; i < ZI_WPT; i++) {
             const int r = i * ZI_THREADS + (int)threadIdx.x;
-            req[i] = NO_REQ;
+            fj[i] = r < tl ? frag_of(s_fs, nf, r) : 0;
+            req[i] = a.frag_req[f0 + (uint64_t)fj[i]];
+        }
+#pragma unroll
+        for (int i = 0; i < ZI_WPT; i++) {
+            const int r = i * ZI_THREADS + (int)threadIdx.x;
             key[i] = Key2{0, 0};
             slot[i] = 0;
-            if (r < tl) {
-                const int j = frag_of(s_fs, nf, r);
-                req[i] = a.frag_req[f0 + j];
-                if (r + a.k <= s_fs[j + 1] && req[i] != NO_REQ) {
-                    my_lookups++;
-                    if constexpr (!WIDE) {
-                        key[i].lo = canon64(s_words, rb + (uint64_t)r, a.k);
-                        slot[i] = slot_of(key[i].lo, 0) & a.mask;
-                    } else {
-                        key[i] = canon128(s_words, rb + (uint64_t)r, a.k);
-                        slot[i] = slot_of(key[i].lo, key[i].hi) & a.mask;
-                    }
-                    pending |= 1u << i;
+            if (r >= tl) req[i] = NO_REQ;
+            if (r < tl && r + a.k <= s_fs[fj[i] + 1] && req[i] != NO_REQ) {
+                my_lookups++;
+                if constexpr (!WIDE) {
+                    key[i].lo = canon64(s_words, rb + (uint64_t)r, a.k);
+                    slot[i] = slot_of(key[i].lo, 0) & a.mask;
+                } else {
+                    key[i] = canon128(s_words, rb + (uint64_t)r, a.k);
+                    slot[i] = slot_of(key[i].lo, key[i].hi) & a.mask;
                 }
+                pending |= 1u << i;
             }
         }
-        while (pending) {
+        // One probe round: the pending windows' table reads, then their
+        // checks. The first round is written out before the loop, so that its
+        // reads (nearly every lookup ends in it) are all in flight together.
+        // (k > 32: the low words of the slots whose high word matches are
+        // read in a second phase)
+        auto probe_round = [&]() {
             unsigned long long cur[ZI_WPT];
 #pragma unroll
             for (int i = 0; i < ZI_WPT; i++)
                 if (pending & (1u << i)) cur[i] = WIDE ? a.thi[slot[i]] : a.tlo[slot[i]];
+            uint32_t hit = 0;
+#pragma unroll
+            for (int i = 0; i < ZI_WPT; i++)
+                if ((pending & (1u << i)) && cur[i] == (WIDE ? key[i].hi : key[i].lo)) hit |= 1u << i;
+            if constexpr (WIDE) {
+                unsigned long long clo[ZI_WPT];
+#pragma unroll
+                for (int i = 0; i < ZI_WPT; i++)
+                    if (hit & (1u << i)) clo[i] = a.tlo[slot[i]];
+#pragma unroll
+                for (int i = 0; i < ZI_WPT; i++)
+                    if ((hit & (1u << i)) && clo[i] != key[i].lo) hit &= ~(1u << i);
+            }
 #pragma unroll
             for (int i = 0; i < ZI_WPT; i++) {
                 if (!(pending & (1u << i))) continue;
                 if (cur[i] == EMPTY_KEY) {
                     absent |= 1u << i;
                     pending &= ~(1u << i);
-                } else if (cur[i] == (WIDE ? key[i].hi : key[i].lo) && (!WIDE || a.tlo[slot[i]] == key[i].lo)) {
+                } else if (hit & (1u << i)) {
                     pending &= ~(1u << i);
                 } else {
                     slot[i] = (slot[i] + 1) & a.mask;
                 }
             }
-        }
+        };
+        if (pending) probe_round();
+        while (pending) probe_round();
         // Append the absent k-mers to their request's segment. When they all
         // belong to one request (every tile but those that straddle request
         // boundaries), the block takes one reservation for the tile and each
