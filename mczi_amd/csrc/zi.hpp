@@ -10,12 +10,14 @@
 namespace mczi {
 
 // Canonical k-mer set of an index, in HBM: open addressing, linear probing.
-// k <= 32: 64-bit keys in `lo` (empty = all ones); k <= 63: (hi, lo) pairs,
-// slot claimed through `hi` (a canonical k-mer of k <= 63 has hi < 2^62).
+// k <= 32: one 64-bit key per slot (empty = all ones); k <= 63: 16 bytes per
+// slot, the high word then the low word, the slot claimed through the high
+// word (a canonical k-mer of k <= 63 has hi < 2^62), so that one read brings
+// both words of a probed slot.
 struct ZiIndex {
     uint32_t k = 0, m = 0;
     bool wide = false;
-    DevBuf lo, hi;
+    DevBuf slots;
     uint64_t mask = 0;
     uint64_t n_kmers = 0;  // index windows inserted
 };
